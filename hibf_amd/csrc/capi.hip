@@ -1,0 +1,746 @@
+// C-ABI of the MI355X-native HIBF query path (declared in include/hibf_gpu.h).
+// Host-side validation, device memory management and kernel dispatch.  No
+// exception crosses this boundary; every entry point returns a status code.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "hibf_gpu.h"
+#include "kernels.hpp"
+
+using namespace hibf_amd;
+
+namespace
+{
+
+thread_local std::string g_last_error;
+
+int fail(int status, std::string msg)
+{
+    g_last_error = std::move(msg);
+    return status;
+}
+
+int hip_fail(hipError_t e, char const * what)
+{
+    (void)hipGetLastError(); // clear the sticky per-thread error
+    if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation)
+        return fail(HIBF_GPU_ERR_OUT_OF_MEMORY, std::string(what) + ": " + hipGetErrorString(e));
+    if (e == hipErrorNoDevice)
+        return fail(HIBF_GPU_ERR_NO_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+    return fail(HIBF_GPU_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIBF_TRY_HIP(call, what)                                                                                       \
+    do                                                                                                                 \
+    {                                                                                                                  \
+        hipError_t const e_ = (call);                                                                                  \
+        if (e_ != hipSuccess)                                                                                          \
+            return hip_fail(e_, what);                                                                                 \
+    } while (0)
+
+uint64_t countl_zero64(uint64_t x)
+{
+    return x == 0 ? 64u : static_cast<uint64_t>(__builtin_clzll(x));
+}
+
+// src/interleaved_bloom_filter.cpp:32-53: validation order and geometry.
+int make_geometry(uint64_t bins, uint64_t bin_size, uint64_t hash_funs, hibf_gpu_ibf_geometry & g)
+{
+    if (bins == 0)
+        return fail(HIBF_GPU_ERR_BIN_COUNT, "The number of bins must be > 0.");
+    if (hash_funs == 0 || hash_funs > 5)
+        return fail(HIBF_GPU_ERR_HASH_COUNT, "The number of hash functions must be > 0 and <= 5.");
+    if (bin_size == 0)
+        return fail(HIBF_GPU_ERR_BIN_SIZE, "The size of a bin must be > 0.");
+    if (bins >= (1ULL << 32))
+        return fail(HIBF_GPU_ERR_UNSUPPORTED, "bin_count must be < 2^32 on the device path");
+    g.bins = bins;
+    g.bin_size = bin_size;
+    g.hash_funs = hash_funs;
+    g.hash_shift = countl_zero64(bin_size);
+    g.bin_words = (bins + 63u) / 64u;
+    g.technical_bins = g.bin_words * 64u;
+    unsigned __int128 const bits = static_cast<unsigned __int128>(g.technical_bins) * bin_size;
+    if (bits >> 63)
+        return fail(HIBF_GPU_ERR_UNSUPPORTED, "bit matrix too large");
+    g.n_words = static_cast<uint64_t>(bits / 64u);
+    return HIBF_GPU_OK;
+}
+
+IbfView view_of(hibf_gpu_ibf_geometry const & g, uint64_t const * words)
+{
+    return IbfView{words,
+                   g.bin_size,
+                   static_cast<uint32_t>(g.bins),
+                   static_cast<uint32_t>(g.bin_words),
+                   static_cast<uint32_t>(g.hash_shift),
+                   static_cast<uint32_t>(g.hash_funs)};
+}
+
+int device_cus()
+{
+    static int cus = 0;
+    if (cus == 0)
+    {
+        int dev = 0;
+        hipDeviceProp_t prop{};
+        if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+            cus = prop.multiProcessorCount;
+        else
+            (void)hipGetLastError();
+        if (cus <= 0)
+            cus = 256;
+    }
+    return cus;
+}
+
+// Grows a device buffer to at least `bytes`.
+int ensure(void *& ptr, size_t & have, size_t bytes, char const * what)
+{
+    if (have >= bytes && ptr)
+        return HIBF_GPU_OK;
+    if (ptr)
+    {
+        HIBF_TRY_HIP(hipFree(ptr), what);
+        ptr = nullptr;
+        have = 0;
+    }
+    size_t const want = std::max<size_t>(bytes, 256);
+    HIBF_TRY_HIP(hipMalloc(&ptr, want), what);
+    have = want;
+    return HIBF_GPU_OK;
+}
+
+} // namespace
+
+struct hibf_gpu_ibf_s
+{
+    hibf_gpu_ibf_geometry g{};
+    uint64_t * d_words{nullptr};
+};
+
+struct hibf_gpu_agent_s
+{
+    uint64_t max_queries{65536};
+    void * scratch{nullptr}; // HIBF membership: per-block hit lists
+    size_t scratch_bytes{0};
+    void * ids_tmp{nullptr};
+    size_t ids_tmp_bytes{0};
+    void * hit_start{nullptr};
+    size_t hit_start_bytes{0};
+    void * hit_count{nullptr};
+    size_t hit_count_bytes{0};
+    void * incl{nullptr};
+    size_t incl_bytes{0};
+    void * cub_temp{nullptr};
+    size_t cub_temp_bytes{0};
+    void * counts{nullptr}; // IBF membership: uint16 counts of one chunk
+    size_t counts_bytes{0};
+    unsigned long long * d_total{nullptr};
+    unsigned long long * h_total{nullptr}; // pinned
+};
+
+struct hibf_gpu_hibf_s
+{
+    std::vector<hibf_gpu_ibf_geometry> geoms;
+    std::vector<uint64_t> word_offsets;
+    uint64_t n_words{0};
+    uint64_t n_user_bins{0};
+    uint32_t max_bins{0};
+    uint32_t depth{0};
+    uint64_t * d_arena{nullptr};
+    HibfIbfDesc * d_desc{nullptr};
+    uint64_t * d_tb_info{nullptr};
+};
+
+extern "C" {
+
+const char * hibf_gpu_status_string(int status)
+{
+    switch (status)
+    {
+    case HIBF_GPU_OK: return "ok";
+    case HIBF_GPU_ERR_INVALID_ARGUMENT: return "invalid argument";
+    case HIBF_GPU_ERR_BIN_COUNT: return "The number of bins must be > 0.";
+    case HIBF_GPU_ERR_HASH_COUNT: return "The number of hash functions must be > 0 and <= 5.";
+    case HIBF_GPU_ERR_BIN_SIZE: return "The size of a bin must be > 0.";
+    case HIBF_GPU_ERR_OUT_OF_MEMORY: return "device out of memory";
+    case HIBF_GPU_ERR_DEVICE: return "HIP runtime error";
+    case HIBF_GPU_ERR_CAPACITY: return "output buffer too small";
+    case HIBF_GPU_ERR_UNSUPPORTED: return "unsupported geometry";
+    case HIBF_GPU_ERR_CONFIG: return "invalid config";
+    case HIBF_GPU_ERR_BUDGET: return "over the counter-memory budget";
+    case HIBF_GPU_ERR_NO_DEVICE: return "no HIP device";
+    default: return "unknown status";
+    }
+}
+
+const char * hibf_gpu_last_error(void)
+{
+    return g_last_error.c_str();
+}
+
+/* ---------------------------------------------------------------- IBF --- */
+
+int hibf_gpu_ibf_create(uint64_t bins, uint64_t bin_size, uint32_t hash_funs, hibf_gpu_ibf * out)
+{
+    if (!out)
+        return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "out is null");
+    *out = nullptr;
+    hibf_gpu_ibf_geometry g{};
+    if (int const rc = make_geometry(bins, bin_size, hash_funs, g))
+        return rc;
+    auto * h = new (std::nothrow) hibf_gpu_ibf_s{};
+    if (!h)
+        return fail(HIBF_GPU_ERR_OUT_OF_MEMORY, "host allocation failed");
+    h->g = g;
+    hipError_t e = hipMalloc(reinterpret_cast<void **>(&h->d_words), g.n_words * sizeof(uint64_t));
+    if (e == hipSuccess)
+        e = hipMemset(h->d_words, 0, g.n_words * sizeof(uint64_t));
+    if (e != hipSuccess)
+    {
+        if (h->d_words)
+            (void)hipFree(h->d_words);
+        delete h;
+        return hip_fail(e, "hibf_gpu_ibf_create");
+    }
+    *out = h;
+    return HIBF_GPU_OK;
+}
+
+int hibf_gpu_ibf_destroy(hibf_gpu_ibf ibf)
+{
+    if (!ibf)
+        return HIBF_GPU_OK;
+    hipError_t const e = ibf->d_words ? hipFree(ibf->d_words) : hipSuccess;
+    delete ibf;
+    return e == hipSuccess ? HIBF_GPU_OK : hip_fail(e, "hibf_gpu_ibf_destroy");
+}
+
+int hibf_gpu_ibf_get_geometry(hibf_gpu_ibf ibf, hibf_gpu_ibf_geometry * out)
+{
+    if (!ibf || !out)
+        return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "null argument");
+    *out = ibf->g;
+    return HIBF_GPU_OK;
+}
+
+int hibf_gpu_ibf_device_words(hibf_gpu_ibf ibf, uint64_t ** d_words)
+{
+    if (!ibf || !d_words)
+        return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "null argument");
+    *d_words = ibf->d_words;
+    return HIBF_GPU_OK;
+}
+
+int hibf_gpu_ibf_upload(hibf_gpu_ibf ibf, uint64_t const * h_words, uint64_t n_words)
+{
+    if (!ibf || (!h_words && n_words))
+        return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "null argument");
+    if (n_words != ibf->g.n_words)
+        return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "word count does not match the geometry");
+    HIBF_TRY_HIP(hipMemcpy(ibf->d_words, h_words, n_words * sizeof(uint64_t), hipMemcpyHostToDevice),
+                 "hibf_gpu_ibf_upload");
+    return HIBF_GPU_OK;
+}
+
+int hibf_gpu_ibf_download(hibf_gpu_ibf ibf, uint64_t * h_words, uint64_t n_words)
+{
+    if (!ibf || (!h_words && n_words))
+        return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "null argument");
+    if (n_words != ibf->g.n_words)
+        return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "word count does not match the geometry");
+    HIBF_TRY_HIP(hipDeviceSynchronize(), "hibf_gpu_ibf_download");
+    HIBF_TRY_HIP(hipMemcpy(h_words, ibf->d_words, n_words * sizeof(uint64_t), hipMemcpyDeviceToHost),
+                 "hibf_gpu_ibf_download");
+    return HIBF_GPU_OK;
+}
+
+int hibf_gpu_ibf_emplace(hibf_gpu_ibf ibf,
+                         uint64_t const * d_values,
+                         uint64_t const * d_bins,
+                         uint64_t bin,
+                         uint64_t n,
+                         void * stream)
+{
+    if (!ibf || (!d_values && n))
+        return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "null argument");
+    if (!d_bins && bin >= ibf->g.bins)
+        return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "bin index >= bin_count");
+    HIBF_TRY_HIP(launch_emplace(ibf->d_words, ibf->g.bin_size, static_cast<uint32_t>(ibf->g.bin_words),
+                                static_cast<uint32_t>(ibf->g.hash_shift), static_cast<uint32_t>(ibf->g.hash_funs),
+                                d_values, d_bins, bin, n, ibf->g.bins, static_cast<hipStream_t>(stream)),
+                 "emplace launch");
+    return HIBF_GPU_OK;
+}
+
+int hibf_gpu_ibf_clear(hibf_gpu_ibf ibf, uint64_t const * h_bins, uint64_t n, void * stream)
+{
+    if (!ibf || (!h_bins && n))
+        return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "null argument");
+    if (n == 0)
+        return HIBF_GPU_OK;
+    for (uint64_t i = 0; i < n; ++i)
+        if (h_bins[i] >= ibf->g.bins)
+            return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "bin index >= bin_count");
+    if (n >= (1ULL << 31))
+        return fail(HIBF_GPU_ERR_UNSUPPORTED, "too many bins");
+    hipStream_t const s = static_cast<hipStream_t>(stream);
+    uint64_t * d_bins = nullptr;
+    HIBF_TRY_HIP(hipMalloc(reinterpret_cast<void **>(&d_bins), n * sizeof(uint64_t)), "clear alloc");
+    hipError_t e = hipMemcpyAsync(d_bins, h_bins, n * sizeof(uint64_t), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess)
+        e = launch_clear(ibf->d_words, ibf->g.bin_size, static_cast<uint32_t>(ibf->g.bin_words), d_bins,
+                         static_cast<uint32_t>(n), s);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(s);
+    (void)hipFree(d_bins);
+    return e == hipSuccess ? HIBF_GPU_OK : hip_fail(e, "hibf_gpu_ibf_clear");
+}
+
+int hibf_gpu_ibf_bulk_contains(hibf_gpu_ibf ibf, uint64_t const * d_values, uint64_t n, uint64_t * d_out, void * stream)
+{
+    if (!ibf || ((!d_values || !d_out) && n))
+        return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "null argument");
+    HIBF_TRY_HIP(launch_contains(view_of(ibf->g, ibf->d_words), d_values, n, d_out, static_cast<hipStream_t>(stream)),
+                 "bulk_contains launch");
+    return HIBF_GPU_OK;
+}
+
+int hibf_gpu_ibf_bulk_count(hibf_gpu_ibf ibf,
+                            uint64_t const * d_values,
+                            uint64_t const * d_offsets,
+                            uint64_t nq,
+                            uint32_t counter_bytes,
+                            void * d_counts,
+                            void * stream)
+{
+    if (!ibf || ((!d_offsets || !d_counts) && nq))
+        return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "null argument");
+    if (counter_bytes != 1 && counter_bytes != 2 && counter_bytes != 4 && counter_bytes != 8)
+        return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "counter_bytes must be 1, 2, 4 or 8");
+    HIBF_TRY_HIP(launch_count(view_of(ibf->g, ibf->d_words), d_values, d_offsets, nq, counter_bytes, d_counts,
+                              1u << 24, static_cast<hipStream_t>(stream)),
+                 "bulk_count launch");
+    return HIBF_GPU_OK;
+}
+
+/* -------------------------------------------------------------- agent --- */
+
+int hibf_gpu_agent_create(uint64_t max_queries_per_launch, hibf_gpu_agent * out)
+{
+    if (!out)
+        return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "out is null");
+    *out = nullptr;
+    auto * a = new (std::nothrow) hibf_gpu_agent_s{};
+    if (!a)
+        return fail(HIBF_GPU_ERR_OUT_OF_MEMORY, "host allocation failed");
+    a->max_queries = max_queries_per_launch ? max_queries_per_launch : 65536u;
+    hipError_t e = hipMalloc(reinterpret_cast<void **>(&a->d_total), sizeof(unsigned long long));
+    if (e == hipSuccess)
+        e = hipHostMalloc(reinterpret_cast<void **>(&a->h_total), sizeof(unsigned long long), hipHostMallocDefault);
+    if (e != hipSuccess)
+    {
+        if (a->d_total)
+            (void)hipFree(a->d_total);
+        delete a;
+        return hip_fail(e, "hibf_gpu_agent_create");
+    }
+    *out = a;
+    return HIBF_GPU_OK;
+}
+
+int hibf_gpu_agent_destroy(hibf_gpu_agent a)
+{
+    if (!a)
+        return HIBF_GPU_OK;
+    for (void * p : {a->scratch, a->ids_tmp, a->hit_start, a->hit_count, a->incl, a->cub_temp, a->counts,
+                     static_cast<void *>(a->d_total)})
+        if (p)
+            (void)hipFree(p);
+    if (a->h_total)
+        (void)hipHostFree(a->h_total);
+    delete a;
+    return HIBF_GPU_OK;
+}
+
+namespace
+{
+
+// d_hit_offsets[q0 .. q0+n] from hit_count[0..n) (write_first: also d_hit_offsets[q0] = base).
+int build_offsets(hibf_gpu_agent a,
+                  uint64_t const * d_count,
+                  uint64_t n,
+                  uint64_t base,
+                  bool write_first,
+                  uint64_t * d_offsets,
+                  hipStream_t s)
+{
+    if (int rc = ensure(a->incl, a->incl_bytes, n * sizeof(uint64_t), "offsets alloc"))
+        return rc;
+    size_t need = 0;
+    HIBF_TRY_HIP(inclusive_sum_u64(nullptr, need, d_count, static_cast<uint64_t *>(a->incl), n, s), "scan size");
+    if (int rc = ensure(a->cub_temp, a->cub_temp_bytes, need, "scan alloc"))
+        return rc;
+    size_t have = a->cub_temp_bytes;
+    HIBF_TRY_HIP(inclusive_sum_u64(a->cub_temp, have, d_count, static_cast<uint64_t *>(a->incl), n, s), "scan");
+    HIBF_TRY_HIP(launch_finalize_offsets(static_cast<uint64_t *>(a->incl), n, base, write_first, d_offsets, s),
+                 "offsets");
+    return HIBF_GPU_OK;
+}
+
+} // namespace
+
+int hibf_gpu_ibf_membership_for(hibf_gpu_ibf ibf,
+                                hibf_gpu_agent a,
+                                uint64_t const * d_values,
+                                uint64_t const * d_offsets,
+                                uint64_t nq,
+                                uint16_t const * d_thresholds,
+                                uint64_t * d_ids,
+                                uint64_t ids_cap,
+                                uint64_t * d_hit_offsets,
+                                uint64_t * h_total,
+                                void * stream)
+{
+    if (!ibf || !a || !h_total || ((!d_offsets || !d_thresholds || !d_hit_offsets) && nq))
+        return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "null argument");
+    hipStream_t const s = static_cast<hipStream_t>(stream);
+    *h_total = 0;
+    if (nq == 0)
+    {
+        return HIBF_GPU_OK;
+    }
+    // Chunk the queries so that the uint16 counts of one chunk stay within 256 MiB.
+    uint64_t const bins = ibf->g.bins;
+    uint64_t chunk = std::max<uint64_t>(1u, (256ULL << 20) / (bins * 2u));
+    chunk = std::min<uint64_t>(chunk, std::min<uint64_t>(nq, a->max_queries));
+    if (int rc = ensure(a->counts, a->counts_bytes, chunk * bins * 2u, "membership counts"))
+        return rc;
+    if (int rc = ensure(a->hit_count, a->hit_count_bytes, chunk * sizeof(uint64_t), "membership counts"))
+        return rc;
+    IbfView const g = view_of(ibf->g, ibf->d_words);
+    uint64_t base = 0;
+    for (uint64_t q0 = 0; q0 < nq; q0 += chunk)
+    {
+        uint64_t const n = std::min(chunk, nq - q0);
+        auto * counts = static_cast<uint16_t *>(a->counts);
+        auto * hc = static_cast<uint64_t *>(a->hit_count);
+        HIBF_TRY_HIP(launch_count(g, d_values, d_offsets + q0, n, 2u, counts, 1u << 24, s), "count");
+        HIBF_TRY_HIP(launch_threshold_compact(counts, bins, n, d_thresholds + q0, hc, nullptr, nullptr, 0, 0, s),
+                     "threshold pass 0");
+        if (int rc = build_offsets(a, hc, n, base, q0 == 0, d_hit_offsets + q0, s))
+            return rc;
+        HIBF_TRY_HIP(hipMemcpyAsync(a->h_total, d_hit_offsets + q0 + n, sizeof(uint64_t), hipMemcpyDeviceToHost, s),
+                     "total copy");
+        HIBF_TRY_HIP(launch_threshold_compact(counts, bins, n, d_thresholds + q0, hc, d_hit_offsets + q0, d_ids,
+                                              d_ids ? ids_cap : 0, 1, s),
+                     "threshold pass 1");
+        HIBF_TRY_HIP(hipStreamSynchronize(s), "membership sync");
+        base = *a->h_total;
+    }
+    *h_total = base;
+    if (base > ids_cap)
+        return fail(HIBF_GPU_ERR_CAPACITY, "ids_cap too small; *h_total holds the needed size");
+    return HIBF_GPU_OK;
+}
+
+/* --------------------------------------------------------------- HIBF --- */
+
+int hibf_gpu_hibf_create(uint64_t n_ibfs,
+                         hibf_gpu_ibf_geometry const * geoms,
+                         uint64_t const * h_arena,
+                         uint64_t const * h_word_offsets,
+                         uint64_t const * h_tb_offsets,
+                         uint64_t const * h_next_ibf,
+                         uint64_t const * h_user_bin,
+                         uint64_t number_of_user_bins,
+                         hibf_gpu_hibf * out)
+{
+    if (!out || !geoms || !h_word_offsets || !h_tb_offsets || !h_next_ibf || !h_user_bin || n_ibfs == 0)
+        return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "null argument or no IBF");
+    *out = nullptr;
+    if (n_ibfs >= (1ULL << 32) || number_of_user_bins >= 0xFFFFFFFEULL)
+        return fail(HIBF_GPU_ERR_UNSUPPORTED, "too many IBFs or user bins for 32-bit device ids");
+    auto * h = new (std::nothrow) hibf_gpu_hibf_s{};
+    if (!h)
+        return fail(HIBF_GPU_ERR_OUT_OF_MEMORY, "host allocation failed");
+    std::unique_ptr<hibf_gpu_hibf_s> guard{h};
+    h->n_user_bins = number_of_user_bins;
+    h->geoms.resize(n_ibfs);
+    h->word_offsets.assign(h_word_offsets, h_word_offsets + n_ibfs);
+    std::vector<HibfIbfDesc> desc(n_ibfs);
+    uint64_t const n_tbs = h_tb_offsets[n_ibfs];
+    std::vector<uint64_t> info(n_tbs);
+    uint64_t end_words = 0;
+    for (uint64_t i = 0; i < n_ibfs; ++i)
+    {
+        hibf_gpu_ibf_geometry g{};
+        if (int rc = make_geometry(geoms[i].bins, geoms[i].bin_size, geoms[i].hash_funs, g))
+            return rc;
+        h->geoms[i] = g;
+        if (h_tb_offsets[i + 1] - h_tb_offsets[i] != g.bins)
+            return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "table size of an IBF differs from its bin_count");
+        if (g.bins > kHibfMaxBins)
+            return fail(HIBF_GPU_ERR_UNSUPPORTED, "an IBF of the HIBF has more than 8192 technical bins");
+        if (h_word_offsets[i] % 4u)
+            return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "IBF word offsets must be multiples of 4 (32-byte aligned)");
+        end_words = std::max(end_words, h_word_offsets[i] + g.n_words);
+        uint32_t lpv_log, cw, tile;
+        count_shape(static_cast<uint32_t>(g.bin_words), lpv_log, cw, tile);
+        desc[i] = HibfIbfDesc{h_word_offsets[i],
+                              g.bin_size,
+                              h_tb_offsets[i],
+                              static_cast<uint32_t>(g.bins),
+                              static_cast<uint32_t>(g.bin_words),
+                              static_cast<uint32_t>(g.hash_shift),
+                              static_cast<uint32_t>(g.hash_funs),
+                              cw == 2u ? 7u : lpv_log,
+                              0u};
+        h->max_bins = std::max<uint32_t>(h->max_bins, static_cast<uint32_t>(g.bins));
+        uint64_t const * ub = h_user_bin + h_tb_offsets[i];
+        uint64_t const * nx = h_next_ibf + h_tb_offsets[i];
+        for (uint64_t b = 0; b < g.bins; ++b)
+        {
+            uint64_t kind, payload;
+            if (ub[b] == HIBF_GPU_BIN_MERGED)
+            {
+                if (nx[b] >= n_ibfs || nx[b] == 0)
+                    return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "merged bin points to an invalid IBF");
+                kind = kTbMerged;
+                payload = nx[b];
+            }
+            else if (ub[b] == HIBF_GPU_BIN_DELETED)
+            {
+                kind = kTbDeleted;
+                payload = 0;
+            }
+            else
+            {
+                if (ub[b] >= number_of_user_bins)
+                    return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "user bin id >= number_of_user_bins");
+                bool const end = (b + 1 == g.bins) || ub[b + 1] != ub[b];
+                kind = end ? kTbRunEnd : kTbRegular;
+                payload = ub[b];
+            }
+            info[h_tb_offsets[i] + b] = (kind << 32) | payload;
+        }
+    }
+    // Depth of the tree below IBF 0 (iterative, with a cycle guard).
+    {
+        std::vector<uint32_t> level(n_ibfs, 0u);
+        std::vector<uint64_t> stack{0};
+        level[0] = 1;
+        uint32_t depth = 1;
+        uint64_t visited = 0;
+        while (!stack.empty())
+        {
+            uint64_t const i = stack.back();
+            stack.pop_back();
+            if (++visited > n_ibfs)
+                return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "merged bins do not form a tree");
+            for (uint64_t b = h_tb_offsets[i]; b < h_tb_offsets[i + 1]; ++b)
+                if ((info[b] >> 32) == kTbMerged)
+                {
+                    uint64_t const c = info[b] & 0xFFFFFFFFu;
+                    if (level[c])
+                        return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "merged bins do not form a tree");
+                    level[c] = level[i] + 1;
+                    depth = std::max(depth, level[c]);
+                    stack.push_back(c);
+                }
+        }
+        if (depth > kHibfMaxDepth)
+            return fail(HIBF_GPU_ERR_UNSUPPORTED, "HIBF deeper than 16 levels");
+        h->depth = depth;
+    }
+    if (static_cast<size_t>(h->max_bins) * (1u + h->depth) * sizeof(uint32_t) > 64u * 1024u)
+        return fail(HIBF_GPU_ERR_UNSUPPORTED, "largest IBF x depth exceeds the LDS budget of the descent kernel");
+    h->n_words = end_words;
+    hipError_t e = hipMalloc(reinterpret_cast<void **>(&h->d_arena), std::max<uint64_t>(end_words, 1) * 8u);
+    if (e == hipSuccess)
+        e = h_arena ? hipMemcpy(h->d_arena, h_arena, end_words * 8u, hipMemcpyHostToDevice)
+                    : hipMemset(h->d_arena, 0, end_words * 8u);
+    if (e == hipSuccess)
+        e = hipMalloc(reinterpret_cast<void **>(&h->d_desc), n_ibfs * sizeof(HibfIbfDesc));
+    if (e == hipSuccess)
+        e = hipMemcpy(h->d_desc, desc.data(), n_ibfs * sizeof(HibfIbfDesc), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMalloc(reinterpret_cast<void **>(&h->d_tb_info), std::max<uint64_t>(n_tbs, 1) * 8u);
+    if (e == hipSuccess)
+        e = hipMemcpy(h->d_tb_info, info.data(), n_tbs * 8u, hipMemcpyHostToDevice);
+    if (e != hipSuccess)
+    {
+        for (void * p : {static_cast<void *>(h->d_arena), static_cast<void *>(h->d_desc),
+                         static_cast<void *>(h->d_tb_info)})
+            if (p)
+                (void)hipFree(p);
+        return hip_fail(e, "hibf_gpu_hibf_create");
+    }
+    *out = guard.release();
+    return HIBF_GPU_OK;
+}
+
+int hibf_gpu_hibf_destroy(hibf_gpu_hibf h)
+{
+    if (!h)
+        return HIBF_GPU_OK;
+    for (void * p : {static_cast<void *>(h->d_arena), static_cast<void *>(h->d_desc),
+                     static_cast<void *>(h->d_tb_info)})
+        if (p)
+            (void)hipFree(p);
+    delete h;
+    return HIBF_GPU_OK;
+}
+
+int hibf_gpu_hibf_emplace(hibf_gpu_hibf h,
+                          uint64_t ibf_idx,
+                          uint64_t const * d_values,
+                          uint64_t const * d_bins,
+                          uint64_t bin,
+                          uint64_t n,
+                          void * stream)
+{
+    if (!h || ibf_idx >= h->geoms.size() || (!d_values && n))
+        return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "null argument or IBF index out of range");
+    hibf_gpu_ibf_geometry const & g = h->geoms[ibf_idx];
+    if (!d_bins && bin >= g.bins)
+        return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "bin index >= bin_count");
+    HIBF_TRY_HIP(launch_emplace(h->d_arena + h->word_offsets[ibf_idx], g.bin_size,
+                                static_cast<uint32_t>(g.bin_words), static_cast<uint32_t>(g.hash_shift),
+                                static_cast<uint32_t>(g.hash_funs), d_values, d_bins, bin, n, g.bins,
+                                static_cast<hipStream_t>(stream)),
+                 "hibf emplace launch");
+    return HIBF_GPU_OK;
+}
+
+int hibf_gpu_hibf_arena_words(hibf_gpu_hibf h, uint64_t * n_words)
+{
+    if (!h || !n_words)
+        return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "null argument");
+    *n_words = h->n_words;
+    return HIBF_GPU_OK;
+}
+
+int hibf_gpu_hibf_download_arena(hibf_gpu_hibf h, uint64_t * h_arena, uint64_t n_words)
+{
+    if (!h || (!h_arena && n_words))
+        return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "null argument");
+    if (n_words != h->n_words)
+        return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "word count does not match the arena");
+    HIBF_TRY_HIP(hipDeviceSynchronize(), "download sync");
+    HIBF_TRY_HIP(hipMemcpy(h_arena, h->d_arena, n_words * 8u, hipMemcpyDeviceToHost), "download arena");
+    return HIBF_GPU_OK;
+}
+
+namespace
+{
+HibfLaunch launch_of(hibf_gpu_hibf h)
+{
+    return HibfLaunch{h->d_desc, h->d_arena, h->d_tb_info, h->max_bins, h->depth, h->n_user_bins};
+}
+
+unsigned hibf_blocks(uint64_t nq, uint64_t per_block_scratch_bytes)
+{
+    uint64_t blocks = static_cast<uint64_t>(device_cus()) * 8u;
+    if (per_block_scratch_bytes)
+        blocks = std::min<uint64_t>(blocks, std::max<uint64_t>(1u, (1ULL << 30) / per_block_scratch_bytes));
+    blocks = std::min<uint64_t>(blocks, nq);
+    return static_cast<unsigned>(std::max<uint64_t>(blocks, 1u));
+}
+} // namespace
+
+int hibf_gpu_hibf_membership_for(hibf_gpu_hibf h,
+                                 hibf_gpu_agent a,
+                                 uint64_t const * d_values,
+                                 uint64_t const * d_offsets,
+                                 uint64_t nq,
+                                 uint16_t const * d_thresholds,
+                                 uint64_t * d_ids,
+                                 uint64_t ids_cap,
+                                 uint64_t * d_hit_offsets,
+                                 uint64_t * h_total,
+                                 void * stream)
+{
+    if (!h || !a || !h_total || ((!d_offsets || !d_thresholds || !d_hit_offsets) && nq))
+        return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "null argument");
+    hipStream_t const s = static_cast<hipStream_t>(stream);
+    *h_total = 0;
+    if (nq == 0)
+        return HIBF_GPU_OK;
+    uint64_t const per_block = h->n_user_bins * sizeof(uint32_t);
+    unsigned const blocks = hibf_blocks(nq, per_block);
+    if (int rc = ensure(a->scratch, a->scratch_bytes, blocks * per_block, "membership scratch"))
+        return rc;
+    if (int rc = ensure(a->hit_start, a->hit_start_bytes, nq * 8u, "membership hits"))
+        return rc;
+    if (int rc = ensure(a->hit_count, a->hit_count_bytes, nq * 8u, "membership hits"))
+        return rc;
+    if (!a->ids_tmp)
+        if (int rc = ensure(a->ids_tmp, a->ids_tmp_bytes, std::max<uint64_t>(nq, 1024u) * 8u, "membership ids"))
+            return rc;
+    HibfLaunch const L = launch_of(h);
+    for (int attempt = 0; attempt < 2; ++attempt)
+    {
+        uint64_t const cap = a->ids_tmp_bytes / 8u;
+        HIBF_TRY_HIP(hipMemsetAsync(a->d_total, 0, sizeof(unsigned long long), s), "total reset");
+        HIBF_TRY_HIP(launch_hibf_membership(L, d_values, d_offsets, nq, d_thresholds,
+                                            static_cast<uint32_t *>(a->scratch), blocks,
+                                            static_cast<uint64_t *>(a->ids_tmp), cap,
+                                            static_cast<uint64_t *>(a->hit_start),
+                                            static_cast<uint64_t *>(a->hit_count), a->d_total, s),
+                     "membership launch");
+        HIBF_TRY_HIP(hipMemcpyAsync(a->h_total, a->d_total, sizeof(unsigned long long), hipMemcpyDeviceToHost, s),
+                     "total copy");
+        HIBF_TRY_HIP(hipStreamSynchronize(s), "membership sync");
+        uint64_t const total = *a->h_total;
+        if (total <= cap)
+            break;
+        if (int rc = ensure(a->ids_tmp, a->ids_tmp_bytes, total * 8u, "membership ids"))
+            return rc;
+    }
+    uint64_t const total = *a->h_total;
+    *h_total = total;
+    if (int rc = build_offsets(a, static_cast<uint64_t *>(a->hit_count), nq, 0, true, d_hit_offsets, s))
+        return rc;
+    if (total > ids_cap || (!d_ids && total))
+    {
+        HIBF_TRY_HIP(hipStreamSynchronize(s), "membership sync");
+        return fail(HIBF_GPU_ERR_CAPACITY, "ids_cap too small; *h_total holds the needed size");
+    }
+    HIBF_TRY_HIP(launch_gather_hits(static_cast<uint64_t *>(a->ids_tmp), static_cast<uint64_t *>(a->hit_start),
+                                    static_cast<uint64_t *>(a->hit_count), d_hit_offsets, nq, d_ids, s),
+                 "gather launch");
+    return HIBF_GPU_OK;
+}
+
+int hibf_gpu_hibf_bulk_count(hibf_gpu_hibf h,
+                             hibf_gpu_agent a,
+                             uint64_t const * d_values,
+                             uint64_t const * d_offsets,
+                             uint64_t nq,
+                             uint64_t const * d_thresholds,
+                             uint32_t counter_bytes,
+                             void * d_counts,
+                             void * stream)
+{
+    if (!h || !a || ((!d_offsets || !d_thresholds || !d_counts) && nq))
+        return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "null argument");
+    if (counter_bytes != 1 && counter_bytes != 2 && counter_bytes != 4 && counter_bytes != 8)
+        return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "counter_bytes must be 1, 2, 4 or 8");
+    if (nq == 0)
+        return HIBF_GPU_OK;
+    HIBF_TRY_HIP(launch_hibf_count(launch_of(h), d_values, d_offsets, nq, d_thresholds, counter_bytes, d_counts,
+                                   hibf_blocks(nq, 0), static_cast<hipStream_t>(stream)),
+                 "hibf count launch");
+    return HIBF_GPU_OK;
+}
+
+} // extern "C"

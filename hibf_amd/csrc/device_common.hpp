@@ -1,0 +1,198 @@
+// Device-side building blocks shared by the HIBF query kernels (gfx950).
+//
+// Bit-matrix layout (interleaved_bloom_filter.hpp:110-126, bit_vector.hpp:1045-1054):
+// bit (row r, technical bin b) lives at bit position r*technical_bins + b, i.e.
+// word r*bin_words + b/64, bit b%64, LSB first.  A "row" is the bin_words
+// consecutive words that one hash position selects across all technical bins.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace hibf_amd
+{
+
+// interleaved_bloom_filter.hpp:172-176
+__device__ __forceinline__ uint64_t hash_seed(int k)
+{
+    switch (k)
+    {
+    case 0: return 13572355802537770549ULL;
+    case 1: return 13043817825332782213ULL;
+    case 2: return 10650232656628343401ULL;
+    case 3: return 16499269484942379435ULL;
+    default: return 4893150838803335377ULL;
+    }
+}
+
+// First word of the row selected by value v under seed k.
+// interleaved_bloom_filter.hpp:185-195 returns the bit position
+// fastrange(...) * technical_bins; technical_bins = 64 * bin_words and the
+// product never exceeds the matrix size, so position / 64 == fastrange * bin_words.
+__device__ __forceinline__ uint64_t row_word(uint64_t v, uint64_t seed, uint32_t hash_shift, uint64_t bin_size,
+                                             uint64_t bin_words)
+{
+    uint64_t h = v * seed;
+    h ^= h >> hash_shift;
+    h *= 11400714819323198485ULL;
+    return __umul64hi(h, bin_size) * bin_words;
+}
+
+// Geometry of one IBF as the kernels need it.
+struct IbfView
+{
+    uint64_t const * words; // bit matrix (device)
+    uint64_t bin_size;      // rows
+    uint32_t bins;          // bin_count()
+    uint32_t bin_words;     // ceil(bins / 64)
+    uint32_t hash_shift;    // countl_zero(bin_size)
+    uint32_t hash_funs;     // 1..5
+};
+
+__device__ __forceinline__ uint64_t tail_mask(IbfView const & g, uint32_t w)
+{
+    uint32_t const r = g.bins & 63u;
+    return (w + 1u == g.bin_words && r) ? ((1ULL << r) - 1ULL) : ~0ULL;
+}
+
+__device__ __forceinline__ uint64_t shfl64(uint64_t x, int src)
+{
+    uint32_t lo = static_cast<uint32_t>(x);
+    uint32_t hi = static_cast<uint32_t>(x >> 32);
+    lo = __shfl(lo, src, 64);
+    hi = __shfl(hi, src, 64);
+    return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+
+__device__ __forceinline__ uint64_t readlane64(uint64_t x, int src)
+{
+    uint32_t lo = __builtin_amdgcn_readlane(static_cast<int>(static_cast<uint32_t>(x)), src);
+    uint32_t hi = __builtin_amdgcn_readlane(static_cast<int>(static_cast<uint32_t>(x >> 32)), src);
+    return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+
+// Adds, for every value of vals[0..L) and every technical bin b of the word
+// tile [w0, w0+tw), one to lds[(word - w0) * 64 + bit] when all h rows have the
+// bin's bit set (bulk_contains + counting_vector += in one pass,
+// src/interleaved_bloom_filter.cpp:204-291, counting_vector.hpp:279-311).
+//
+// A wave takes 64 values at a time; lane i hashes value i.  LPV lanes then
+// share one value (VPS = 64 / LPV values per load step); each lane owns CW
+// words of the tile: w0 + (lane % LPV) + c * LPV.  Counters are 32-bit LDS
+// words updated with ds_add_u32 (values of one step may hit the same bin).
+// Callers truncate them to the agent's counter width, which gives the
+// reference's modulo-2^w wrap exactly.
+template <int LPV, int CW, int U>
+__device__ __forceinline__ void count_into_lds(IbfView const & g,
+                                               uint64_t const * __restrict__ vals,
+                                               uint32_t L,
+                                               uint32_t w0,
+                                               uint32_t tw,
+                                               uint32_t * lds,
+                                               int wave,
+                                               int nwaves,
+                                               int lane)
+{
+    constexpr int VPS = 64 / LPV;
+    int const sub = lane / LPV;
+    uint32_t const wl = static_cast<uint32_t>(lane % LPV);
+    uint64_t const bw = g.bin_words;
+    uint32_t const hf = g.hash_funs;
+    uint64_t const * __restrict__ words = g.words;
+
+    for (uint32_t base = static_cast<uint32_t>(wave) * 64u; base < L; base += static_cast<uint32_t>(nwaves) * 64u)
+    {
+        uint32_t const nv = min(64u, L - base);
+        uint64_t const v = static_cast<uint32_t>(lane) < nv ? vals[base + lane] : 0ULL;
+        uint64_t rows[5];
+#pragma unroll
+        for (int k = 0; k < 5; ++k)
+            rows[k] = (static_cast<uint32_t>(k) < hf) ? row_word(v, hash_seed(k), g.hash_shift, g.bin_size, bw) : 0ULL;
+
+        for (uint32_t s = 0; s < nv; s += VPS * U)
+        {
+            uint64_t x[U][CW];
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+            {
+                uint32_t const idx = s + static_cast<uint32_t>(u * VPS + sub);
+                bool const act = idx < nv;
+                uint64_t r[5];
+#pragma unroll
+                for (int k = 0; k < 5; ++k)
+                {
+                    r[k] = 0ULL;
+                    if (static_cast<uint32_t>(k) < hf)
+                    {
+                        if constexpr (LPV == 64)
+                            r[k] = readlane64(rows[k], static_cast<int>(min(idx, 63u)));
+                        else
+                            r[k] = shfl64(rows[k], static_cast<int>(min(idx, 63u)));
+                    }
+                }
+#pragma unroll
+                for (int c = 0; c < CW; ++c)
+                {
+                    uint32_t const wt = wl + static_cast<uint32_t>(c * LPV); // word within tile
+                    uint64_t acc = 0ULL;
+                    if (act && wt < tw)
+                    {
+                        uint64_t const w = w0 + wt;
+                        acc = words[r[0] + w];
+#pragma unroll
+                        for (int k = 1; k < 5; ++k)
+                            if (static_cast<uint32_t>(k) < hf)
+                                acc &= words[r[k] + w];
+                        acc &= tail_mask(g, static_cast<uint32_t>(w));
+                    }
+                    x[u][c] = acc;
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+            {
+#pragma unroll
+                for (int c = 0; c < CW; ++c)
+                {
+                    uint64_t bits = x[u][c];
+                    uint32_t const cbase = (wl + static_cast<uint32_t>(c * LPV)) * 64u;
+                    while (bits)
+                    {
+                        uint32_t const b = static_cast<uint32_t>(__builtin_ctzll(bits));
+                        atomicAdd(&lds[cbase + b], 1u);
+                        bits &= bits - 1ULL;
+                    }
+                }
+            }
+        }
+    }
+}
+
+// Runtime dispatch of count_into_lds on log2(lanes per value): 0..6 give
+// LPV = 1..64 with one word per lane (bin_words <= 64); 7 gives 64 lanes with
+// two words each (bin_words <= 128).  Used by kernels that visit IBFs of
+// different widths (the HIBF descent).
+template <int U>
+__device__ __forceinline__ void count_into_lds_dispatch(uint32_t lpv_log,
+                                                        IbfView const & g,
+                                                        uint64_t const * __restrict__ vals,
+                                                        uint32_t L,
+                                                        uint32_t * lds,
+                                                        int wave,
+                                                        int nwaves,
+                                                        int lane)
+{
+    switch (lpv_log)
+    {
+    case 0: count_into_lds<1, 1, U>(g, vals, L, 0, g.bin_words, lds, wave, nwaves, lane); break;
+    case 1: count_into_lds<2, 1, U>(g, vals, L, 0, g.bin_words, lds, wave, nwaves, lane); break;
+    case 2: count_into_lds<4, 1, U>(g, vals, L, 0, g.bin_words, lds, wave, nwaves, lane); break;
+    case 3: count_into_lds<8, 1, U>(g, vals, L, 0, g.bin_words, lds, wave, nwaves, lane); break;
+    case 4: count_into_lds<16, 1, U>(g, vals, L, 0, g.bin_words, lds, wave, nwaves, lane); break;
+    case 5: count_into_lds<32, 1, U>(g, vals, L, 0, g.bin_words, lds, wave, nwaves, lane); break;
+    case 6: count_into_lds<64, 1, U>(g, vals, L, 0, g.bin_words, lds, wave, nwaves, lane); break;
+    default: count_into_lds<64, 2, U>(g, vals, L, 0, g.bin_words, lds, wave, nwaves, lane); break;
+    }
+}
+
+} // namespace hibf_amd

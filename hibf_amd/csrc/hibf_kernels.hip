@@ -1,0 +1,411 @@
+// HIBF descent kernels for gfx950 (membership_for and bulk_count), plus the
+// small helpers that turn per-query hit lists into CSR output.
+#include "device_common.hpp"
+#include "kernels.hpp"
+
+namespace hibf_amd
+{
+
+namespace
+{
+
+__device__ __forceinline__ uint64_t width_mask(uint32_t counter_bytes)
+{
+    return counter_bytes >= 8u ? ~0ULL : ((1ULL << (8u * counter_bytes)) - 1ULL);
+}
+
+__device__ __forceinline__ void store_counter(void * base, uint64_t i, uint32_t counter_bytes, uint64_t v)
+{
+    switch (counter_bytes)
+    {
+    case 1: static_cast<uint8_t *>(base)[i] = static_cast<uint8_t>(v); break;
+    case 2: static_cast<uint16_t *>(base)[i] = static_cast<uint16_t>(v); break;
+    case 4: static_cast<uint32_t *>(base)[i] = static_cast<uint32_t>(v); break;
+    default: static_cast<uint64_t *>(base)[i] = v; break;
+    }
+}
+
+// Shared state of one block's depth-first descent.
+struct DescentState
+{
+    uint32_t nev[kHibfMaxDepth];    // events recorded at each level
+    uint32_t pos[kHibfMaxDepth];    // next event to process at each level
+    uint32_t ibf_at[kHibfMaxDepth]; // IBF visited at each level
+    uint32_t hits;                  // ids found for the current query
+    uint64_t start;                 // output slot of the current query
+};
+
+// Scan the technical bins of the IBF just counted into `cnt` (wave 0 only),
+// exactly as the loop of hierarchical_interleaved_bloom_filter.hpp:329-354:
+// the running sum adds every bin's count (deleted bins included), resets at a
+// merged bin and at the end of a user bin's run, and a reset point is an event
+// iff the sum (modulo 2^(8*counter_bytes)) reaches the threshold.  Events are
+// appended to ev in technical-bin order.  In counting mode user-bin events are
+// written straight to the output row and only merged bins become events.
+template <bool COUNTING>
+__device__ void scan_ibf(HibfIbfDesc const & d,
+                         uint64_t const * __restrict__ tb_info,
+                         uint32_t const * cnt,
+                         uint64_t threshold,
+                         uint32_t counter_bytes,
+                         uint32_t * ev,
+                         uint32_t & nev_out,
+                         void * count_row,
+                         int lane)
+{
+    uint64_t const mask = width_mask(counter_bytes);
+    uint64_t carry = 0;
+    uint32_t nev = 0;
+    for (uint32_t c0 = 0; c0 < d.bins; c0 += 64u)
+    {
+        uint32_t const tb = c0 + static_cast<uint32_t>(lane);
+        bool const valid = tb < d.bins;
+        uint64_t const info = valid ? tb_info[d.tb_offset + tb] : 0ULL;
+        uint32_t const kind = static_cast<uint32_t>(info >> 32);
+        uint32_t const payload = static_cast<uint32_t>(info);
+        bool const reset = valid && (kind == kTbRunEnd || kind == kTbMerged);
+        uint64_t s = valid ? (static_cast<uint64_t>(cnt[tb]) & mask) : 0ULL;
+        // Segment heads: the bin after a reset point.
+        bool const prev_reset = __shfl_up(static_cast<int>(reset), 1, 64) != 0;
+        bool f = lane > 0 && prev_reset;
+#pragma unroll
+        for (int dlt = 1; dlt < 64; dlt <<= 1)
+        {
+            uint64_t const y = shfl64(s, lane >= dlt ? lane - dlt : lane);
+            bool const fy = __shfl(static_cast<int>(f), lane >= dlt ? lane - dlt : lane, 64) != 0;
+            if (lane >= dlt)
+            {
+                if (!f)
+                    s += y;
+                f = f || fy;
+            }
+        }
+        if (!f)
+            s += carry;
+        uint64_t const sum = s & mask;
+        bool const hit = reset && sum >= threshold;
+        bool is_event = hit;
+        if constexpr (COUNTING)
+        {
+            if (hit && kind == kTbRunEnd)
+                store_counter(count_row, payload, counter_bytes, sum);
+            is_event = hit && kind == kTbMerged;
+        }
+        uint64_t const ballot = __ballot(is_event);
+        uint32_t const before = static_cast<uint32_t>(__popcll(ballot & ((1ULL << lane) - 1ULL)));
+        if (is_event)
+            ev[nev + before] = tb;
+        nev += static_cast<uint32_t>(__popcll(ballot));
+        // Pending sum after this chunk: lane 63's segment sum unless it is a reset point.
+        uint64_t const last_s = shfl64(s, 63);
+        bool const last_reset = __shfl(static_cast<int>(reset), 63, 64) != 0;
+        carry = last_reset ? 0ULL : last_s;
+    }
+    nev_out = nev;
+}
+
+// One block per query (grid-stride).  Depth-first descent from IBF 0 with an
+// explicit stack of per-level event lists in LDS, which reproduces the
+// reference's recursion order (hierarchical_interleaved_bloom_filter.hpp:321-355
+// and :462-496) without recursion.
+template <bool COUNTING>
+__global__ void __launch_bounds__(256) hibf_descent_kernel(HibfLaunch h,
+                                                           uint64_t const * __restrict__ values,
+                                                           uint64_t const * __restrict__ offsets,
+                                                           uint64_t nq,
+                                                           uint16_t const * __restrict__ thr16,
+                                                           uint64_t const * __restrict__ thr64,
+                                                           uint32_t counter_bytes,
+                                                           void * __restrict__ counts,
+                                                           uint32_t * __restrict__ scratch,
+                                                           uint64_t * __restrict__ ids,
+                                                           uint64_t ids_cap,
+                                                           uint64_t * __restrict__ hit_start,
+                                                           uint64_t * __restrict__ hit_count,
+                                                           unsigned long long * __restrict__ total)
+{
+    extern __shared__ uint32_t smem[];
+    __shared__ DescentState st;
+    uint32_t * const cnt = smem;              // max_bins counters
+    uint32_t * const evs = smem + h.max_bins; // depth x max_bins event lists
+
+    int const tid = static_cast<int>(threadIdx.x);
+    int const nthreads = static_cast<int>(blockDim.x);
+    int const lane = tid & 63;
+    int const wave = tid >> 6;
+    int const nwaves = nthreads >> 6;
+    uint32_t * const my_scratch = COUNTING ? nullptr : scratch + static_cast<uint64_t>(blockIdx.x) * h.n_user_bins;
+
+    for (uint64_t q = blockIdx.x; q < nq; q += gridDim.x)
+    {
+        uint64_t const beg = offsets[q];
+        uint32_t const L = static_cast<uint32_t>(offsets[q + 1] - beg);
+        uint64_t const * const vals = values + beg;
+        uint64_t const threshold = COUNTING ? thr64[q] : static_cast<uint64_t>(thr16[q]);
+        uint32_t const cb = COUNTING ? counter_bytes : 2u;
+        void * const row = COUNTING ? static_cast<void *>(static_cast<char *>(counts) + q * h.n_user_bins * cb)
+                                    : nullptr;
+        if constexpr (COUNTING)
+        {
+            // counting_vector fill(0) (hierarchical_interleaved_bloom_filter.hpp:561)
+            for (uint64_t i = static_cast<uint64_t>(tid); i < h.n_user_bins; i += static_cast<uint64_t>(nthreads))
+                store_counter(row, i, cb, 0ULL);
+        }
+        if (tid == 0)
+            st.hits = 0;
+
+        uint32_t depth = 0;
+        uint32_t ibf = 0;
+        bool descend = true;
+        while (true)
+        {
+            if (descend)
+            {
+                HibfIbfDesc const d = h.ibfs[ibf];
+                for (uint32_t i = static_cast<uint32_t>(tid); i < d.bins; i += static_cast<uint32_t>(nthreads))
+                    cnt[i] = 0u;
+                __syncthreads();
+                IbfView const g{h.arena + d.word_offset, d.bin_size, d.bins, d.bin_words, d.hash_shift, d.hash_funs};
+                count_into_lds_dispatch<2>(d.lpv_log, g, vals, L, cnt, wave, nwaves, lane);
+                __syncthreads();
+                if (wave == 0)
+                {
+                    uint32_t n = 0;
+                    scan_ibf<COUNTING>(d, h.tb_info, cnt, threshold, cb, evs + depth * h.max_bins, n, row, lane);
+                    if (lane == 0)
+                    {
+                        st.nev[depth] = n;
+                        st.pos[depth] = 0;
+                        st.ibf_at[depth] = ibf;
+                    }
+                }
+                __syncthreads();
+                descend = false;
+            }
+            // Next event of the current level (uniform across the block).
+            uint32_t const p = st.pos[depth];
+            if (p < st.nev[depth])
+            {
+                uint32_t const tb = evs[depth * h.max_bins + p];
+                HibfIbfDesc const & d = h.ibfs[ibf];
+                uint64_t const info = h.tb_info[d.tb_offset + tb];
+                uint32_t const kind = static_cast<uint32_t>(info >> 32);
+                uint32_t const payload = static_cast<uint32_t>(info);
+                __syncthreads();
+                if (tid == 0)
+                {
+                    st.pos[depth] = p + 1u;
+                    if (!COUNTING && kind != kTbMerged)
+                    {
+                        my_scratch[st.hits] = payload;
+                        st.hits = st.hits + 1u;
+                    }
+                }
+                __syncthreads();
+                if (kind == kTbMerged)
+                {
+                    ++depth;
+                    ibf = payload;
+                    descend = true;
+                }
+                continue;
+            }
+            // Level exhausted: return to the parent.
+            if (depth == 0)
+                break;
+            --depth;
+            ibf = st.ibf_at[depth];
+        }
+
+        if constexpr (!COUNTING)
+        {
+            __syncthreads();
+            if (tid == 0)
+            {
+                uint32_t const n = st.hits;
+                uint64_t const s = atomicAdd(total, static_cast<unsigned long long>(n));
+                st.start = s;
+                hit_start[q] = s;
+                hit_count[q] = n;
+            }
+            __syncthreads();
+            uint32_t const n = st.hits;
+            uint64_t const s = st.start;
+            for (uint32_t i = static_cast<uint32_t>(tid); i < n; i += static_cast<uint32_t>(nthreads))
+                if (s + i < ids_cap)
+                    ids[s + i] = my_scratch[i];
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void __launch_bounds__(256) gather_hits_kernel(uint64_t const * __restrict__ src,
+                                                          uint64_t const * __restrict__ start,
+                                                          uint64_t const * __restrict__ count,
+                                                          uint64_t const * __restrict__ offsets,
+                                                          uint64_t nq,
+                                                          uint64_t * __restrict__ out)
+{
+    for (uint64_t q = blockIdx.x; q < nq; q += gridDim.x)
+    {
+        uint64_t const n = count[q];
+        uint64_t const s = start[q];
+        uint64_t const o = offsets[q];
+        for (uint64_t i = threadIdx.x; i < n; i += blockDim.x)
+            out[o + i] = src[s + i];
+    }
+}
+
+// IBF membership_for from uint16 counts (interleaved_bloom_filter.hpp:757-761).
+// pass 0: hit_count[q] = #bins with count >= t.  pass 1: write the bins in
+// ascending order at hit_offsets[q].  One block per query.
+__global__ void __launch_bounds__(256) threshold_compact_kernel(uint16_t const * __restrict__ counts,
+                                                                uint64_t bins,
+                                                                uint64_t nq,
+                                                                uint16_t const * __restrict__ thresholds,
+                                                                uint64_t * __restrict__ hit_count,
+                                                                uint64_t const * __restrict__ hit_offsets,
+                                                                uint64_t * __restrict__ ids,
+                                                                uint64_t ids_cap,
+                                                                int pass)
+{
+    __shared__ uint32_t part[256];
+    int const tid = static_cast<int>(threadIdx.x);
+    uint64_t const chunk = (bins + 255u) / 256u;
+    for (uint64_t q = blockIdx.x; q < nq; q += gridDim.x)
+    {
+        uint16_t const t = thresholds[q];
+        uint16_t const * const c = counts + q * bins;
+        uint64_t const b0 = static_cast<uint64_t>(tid) * chunk;
+        uint64_t const b1 = b0 + chunk < bins ? b0 + chunk : bins;
+        uint32_t mine = 0;
+        for (uint64_t b = b0; b < b1; ++b)
+            mine += c[b] >= t ? 1u : 0u;
+        part[tid] = mine;
+        __syncthreads();
+        if (pass == 0)
+        {
+            if (tid == 0)
+            {
+                uint64_t s = 0;
+                for (int i = 0; i < 256; ++i)
+                    s += part[i];
+                hit_count[q] = s;
+            }
+        }
+        else
+        {
+            uint64_t o = hit_offsets[q];
+            for (int i = 0; i < tid; ++i)
+                o += part[i];
+            for (uint64_t b = b0; b < b1; ++b)
+                if (c[b] >= t)
+                {
+                    if (o < ids_cap)
+                        ids[o] = b;
+                    ++o;
+                }
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void __launch_bounds__(256) finalize_offsets_kernel(uint64_t const * __restrict__ incl,
+                                                               uint64_t n,
+                                                               uint64_t base,
+                                                               bool write_first,
+                                                               uint64_t * __restrict__ out)
+{
+    uint64_t const i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i == 0 && write_first)
+        out[0] = base;
+    if (i < n)
+        out[1 + i] = base + incl[i];
+}
+
+} // namespace
+
+hipError_t launch_finalize_offsets(uint64_t const * incl, uint64_t n, uint64_t base, bool write_first,
+                                   uint64_t * out, hipStream_t stream)
+{
+    unsigned const blocks = static_cast<unsigned>((n + 255u) / 256u + 1u);
+    hipLaunchKernelGGL(finalize_offsets_kernel, dim3(blocks), dim3(256), 0, stream, incl, n, base, write_first, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_hibf_membership(HibfLaunch const & h,
+                                  uint64_t const * values,
+                                  uint64_t const * offsets,
+                                  uint64_t nq,
+                                  uint16_t const * thresholds,
+                                  uint32_t * scratch,
+                                  unsigned blocks,
+                                  uint64_t * ids_tmp,
+                                  uint64_t ids_cap,
+                                  uint64_t * hit_start,
+                                  uint64_t * hit_count,
+                                  unsigned long long * total,
+                                  hipStream_t stream)
+{
+    if (nq == 0)
+        return hipSuccess;
+    size_t const lds = static_cast<size_t>(h.max_bins) * (1u + h.depth) * sizeof(uint32_t);
+    hipLaunchKernelGGL(hibf_descent_kernel<false>, dim3(blocks), dim3(256), lds, stream, h, values, offsets, nq,
+                       thresholds, nullptr, 2u, nullptr, scratch, ids_tmp, ids_cap, hit_start, hit_count, total);
+    return hipGetLastError();
+}
+
+hipError_t launch_hibf_count(HibfLaunch const & h,
+                             uint64_t const * values,
+                             uint64_t const * offsets,
+                             uint64_t nq,
+                             uint64_t const * thresholds,
+                             uint32_t counter_bytes,
+                             void * counts,
+                             unsigned blocks,
+                             hipStream_t stream)
+{
+    if (nq == 0)
+        return hipSuccess;
+    size_t const lds = static_cast<size_t>(h.max_bins) * (1u + h.depth) * sizeof(uint32_t);
+    hipLaunchKernelGGL(hibf_descent_kernel<true>, dim3(blocks), dim3(256), lds, stream, h, values, offsets, nq,
+                       nullptr, thresholds, counter_bytes, counts, nullptr, nullptr, 0ULL, nullptr, nullptr,
+                       nullptr);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather_hits(uint64_t const * src,
+                              uint64_t const * start,
+                              uint64_t const * count,
+                              uint64_t const * offsets,
+                              uint64_t nq,
+                              uint64_t * out,
+                              hipStream_t stream)
+{
+    if (nq == 0)
+        return hipSuccess;
+    unsigned const blocks = static_cast<unsigned>(nq < 65536u ? nq : 65536u);
+    hipLaunchKernelGGL(gather_hits_kernel, dim3(blocks), dim3(256), 0, stream, src, start, count, offsets, nq, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_threshold_compact(uint16_t const * counts,
+                                    uint64_t bins,
+                                    uint64_t nq,
+                                    uint16_t const * thresholds,
+                                    uint64_t * hit_count,
+                                    uint64_t const * hit_offsets,
+                                    uint64_t * ids,
+                                    uint64_t ids_cap,
+                                    int pass,
+                                    hipStream_t stream)
+{
+    if (nq == 0)
+        return hipSuccess;
+    unsigned const blocks = static_cast<unsigned>(nq < 65536u ? nq : 65536u);
+    hipLaunchKernelGGL(threshold_compact_kernel, dim3(blocks), dim3(256), 0, stream, counts, bins, nq, thresholds,
+                       hit_count, hit_offsets, ids, ids_cap, pass);
+    return hipGetLastError();
+}
+
+} // namespace hibf_amd

@@ -1,0 +1,255 @@
+// IBF kernels for gfx950: bulk emplace, clear, bulk_contains and bulk_count.
+#include "device_common.hpp"
+#include "kernels.hpp"
+
+namespace hibf_amd
+{
+
+// One thread per (value, bin) pair: set bit hash_and_fit(v, seed_k) + bin for
+// every k < h (src/interleaved_bloom_filter.cpp:119-140).  atomicOr makes the
+// result independent of order, so the matrix equals the sequential one.
+__global__ void __launch_bounds__(256) emplace_kernel(uint64_t * __restrict__ words,
+                                                      uint64_t bin_size,
+                                                      uint32_t bin_words,
+                                                      uint32_t hash_shift,
+                                                      uint32_t hash_funs,
+                                                      uint64_t const * __restrict__ values,
+                                                      uint64_t const * __restrict__ bins,
+                                                      uint64_t bin,
+                                                      uint64_t n,
+                                                      uint64_t bin_count)
+{
+    uint64_t const stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
+    {
+        uint64_t const v = values[i];
+        uint64_t const b = bins ? bins[i] : bin;
+        if (b >= bin_count) // the reference asserts bin < bins; out-of-range pairs are skipped
+            continue;
+        uint64_t const word_in_row = b >> 6;
+        unsigned long long const bit = 1ULL << (b & 63u);
+#pragma unroll
+        for (int k = 0; k < 5; ++k)
+            if (static_cast<uint32_t>(k) < hash_funs)
+            {
+                uint64_t const w = row_word(v, hash_seed(k), hash_shift, bin_size, bin_words) + word_in_row;
+                atomicOr(reinterpret_cast<unsigned long long *>(words + w), bit);
+            }
+    }
+}
+
+// clear(bin): zero the bin's column in every row (src/interleaved_bloom_filter.cpp:142-148).
+__global__ void __launch_bounds__(256) clear_kernel(uint64_t * __restrict__ words,
+                                                    uint64_t bin_size,
+                                                    uint32_t bin_words,
+                                                    uint64_t const * __restrict__ bins,
+                                                    uint32_t nbins)
+{
+    uint64_t const stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    for (uint64_t r = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; r < bin_size; r += stride)
+    {
+        for (uint32_t j = 0; j < nbins; ++j)
+        {
+            uint64_t const b = bins[j];
+            unsigned long long const keep = ~(1ULL << (b & 63u));
+            atomicAnd(reinterpret_cast<unsigned long long *>(words + r * bin_words + (b >> 6)), keep);
+        }
+    }
+}
+
+// bulk_contains for n values: out[i*bin_words + w] = AND_k row_k[w]
+// (src/interleaved_bloom_filter.cpp:204-291).  One thread per output word.
+__global__ void __launch_bounds__(256) contains_kernel(IbfView g,
+                                                       uint64_t const * __restrict__ values,
+                                                       uint64_t n,
+                                                       uint64_t * __restrict__ out)
+{
+    uint64_t const total = n * g.bin_words;
+    uint64_t const stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    for (uint64_t t = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < total; t += stride)
+    {
+        uint64_t const i = t / g.bin_words;
+        uint64_t const w = t - i * g.bin_words;
+        uint64_t const v = values[i];
+        uint64_t acc = g.words[row_word(v, hash_seed(0), g.hash_shift, g.bin_size, g.bin_words) + w];
+#pragma unroll
+        for (int k = 1; k < 5; ++k)
+            if (static_cast<uint32_t>(k) < g.hash_funs)
+                acc &= g.words[row_word(v, hash_seed(k), g.hash_shift, g.bin_size, g.bin_words) + w];
+        out[t] = acc;
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void store_counts(T * __restrict__ out, uint32_t const * lds, uint32_t n, int tid, int nthreads)
+{
+    for (uint32_t i = static_cast<uint32_t>(tid); i < n; i += static_cast<uint32_t>(nthreads))
+        out[i] = static_cast<T>(lds[i]);
+}
+
+// Batched counting_agent_type<T>::bulk_count (interleaved_bloom_filter.hpp:652-665).
+// One block per query (grid-stride); the block's waves split the query's
+// values, accumulate per-bin counters of one word tile in LDS, then write the
+// tile's counters truncated to counter_bytes.
+template <int LPV, int CW>
+__global__ void __launch_bounds__(256) count_kernel(IbfView g,
+                                                    uint64_t const * __restrict__ values,
+                                                    uint64_t const * __restrict__ offsets,
+                                                    uint64_t nq,
+                                                    uint32_t tile_words,
+                                                    uint32_t counter_bytes,
+                                                    void * __restrict__ counts)
+{
+    extern __shared__ uint32_t lds[];
+    int const tid = static_cast<int>(threadIdx.x);
+    int const nthreads = static_cast<int>(blockDim.x);
+    int const lane = tid & 63;
+    int const wave = tid >> 6;
+    int const nwaves = nthreads >> 6;
+
+    for (uint64_t q = blockIdx.x; q < nq; q += gridDim.x)
+    {
+        uint64_t const beg = offsets[q];
+        uint32_t const L = static_cast<uint32_t>(offsets[q + 1] - beg);
+        for (uint32_t w0 = 0; w0 < g.bin_words; w0 += tile_words)
+        {
+            uint32_t const tw = min(tile_words, g.bin_words - w0);
+            for (uint32_t i = static_cast<uint32_t>(tid); i < tw * 64u; i += static_cast<uint32_t>(nthreads))
+                lds[i] = 0u;
+            __syncthreads();
+            count_into_lds<LPV, CW, 2>(g, values + beg, L, w0, tw, lds, wave, nwaves, lane);
+            __syncthreads();
+            uint32_t const first_bin = w0 * 64u;
+            uint32_t const nb = min(tw * 64u, g.bins - first_bin);
+            uint64_t const o = q * g.bins + first_bin;
+            switch (counter_bytes)
+            {
+            case 1: store_counts(static_cast<uint8_t *>(counts) + o, lds, nb, tid, nthreads); break;
+            case 2: store_counts(static_cast<uint16_t *>(counts) + o, lds, nb, tid, nthreads); break;
+            case 4: store_counts(static_cast<uint32_t *>(counts) + o, lds, nb, tid, nthreads); break;
+            default: store_counts(static_cast<uint64_t *>(counts) + o, lds, nb, tid, nthreads); break;
+            }
+            __syncthreads();
+        }
+    }
+}
+
+namespace
+{
+inline unsigned grid_for(uint64_t items, unsigned per_block, unsigned cap = 1u << 20)
+{
+    uint64_t b = (items + per_block - 1) / per_block;
+    if (b == 0)
+        b = 1;
+    return static_cast<unsigned>(b < cap ? b : cap);
+}
+} // namespace
+
+hipError_t launch_emplace(uint64_t * words,
+                          uint64_t bin_size,
+                          uint32_t bin_words,
+                          uint32_t hash_shift,
+                          uint32_t hash_funs,
+                          uint64_t const * values,
+                          uint64_t const * bins,
+                          uint64_t bin,
+                          uint64_t n,
+                          uint64_t bin_count,
+                          hipStream_t stream)
+{
+    if (n == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(emplace_kernel, dim3(grid_for(n, 256)), dim3(256), 0, stream, words, bin_size, bin_words,
+                       hash_shift, hash_funs, values, bins, bin, n, bin_count);
+    return hipGetLastError();
+}
+
+hipError_t launch_clear(uint64_t * words,
+                        uint64_t bin_size,
+                        uint32_t bin_words,
+                        uint64_t const * d_bins,
+                        uint32_t nbins,
+                        hipStream_t stream)
+{
+    if (nbins == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(clear_kernel, dim3(grid_for(bin_size, 256)), dim3(256), 0, stream, words, bin_size,
+                       bin_words, d_bins, nbins);
+    return hipGetLastError();
+}
+
+hipError_t launch_contains(IbfView const & g, uint64_t const * values, uint64_t n, uint64_t * out, hipStream_t stream)
+{
+    if (n == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(contains_kernel, dim3(grid_for(n * g.bin_words, 256)), dim3(256), 0, stream, g, values, n,
+                       out);
+    return hipGetLastError();
+}
+
+// Shape of the count kernel for an IBF: tiles of at most 128 words (8192
+// technical bins, 32 KiB of LDS counters).
+void count_shape(uint32_t bin_words, uint32_t & lpv_log, uint32_t & cw, uint32_t & tile_words)
+{
+    if (bin_words > 64u)
+    {
+        lpv_log = 6u;
+        cw = 2u;
+        tile_words = 128u;
+        return;
+    }
+    uint32_t l = 0;
+    while ((1u << l) < bin_words)
+        ++l;
+    lpv_log = l;
+    cw = 1u;
+    tile_words = bin_words;
+}
+
+hipError_t launch_count(IbfView const & g,
+                        uint64_t const * values,
+                        uint64_t const * offsets,
+                        uint64_t nq,
+                        uint32_t counter_bytes,
+                        void * counts,
+                        unsigned max_blocks,
+                        hipStream_t stream)
+{
+    if (nq == 0)
+        return hipSuccess;
+    uint32_t lpv_log, cw, tile_words;
+    count_shape(g.bin_words, lpv_log, cw, tile_words);
+    size_t const lds = static_cast<size_t>(tile_words) * 64u * sizeof(uint32_t);
+    unsigned const grid = static_cast<unsigned>(nq < max_blocks ? nq : max_blocks);
+    dim3 const blk(256);
+#define HIBF_COUNT_CASE(LL, LPVV, CWW)                                                                                 \
+    case LL:                                                                                                           \
+        hipLaunchKernelGGL((count_kernel<LPVV, CWW>), dim3(grid), blk, lds, stream, g, values, offsets, nq,           \
+                           tile_words, counter_bytes, counts);                                                        \
+        break;
+    if (cw == 2u)
+    {
+        hipLaunchKernelGGL((count_kernel<64, 2>), dim3(grid), blk, lds, stream, g, values, offsets, nq, tile_words,
+                           counter_bytes, counts);
+    }
+    else
+    {
+        switch (lpv_log)
+        {
+            HIBF_COUNT_CASE(0, 1, 1)
+            HIBF_COUNT_CASE(1, 2, 1)
+            HIBF_COUNT_CASE(2, 4, 1)
+            HIBF_COUNT_CASE(3, 8, 1)
+            HIBF_COUNT_CASE(4, 16, 1)
+            HIBF_COUNT_CASE(5, 32, 1)
+        default:
+            hipLaunchKernelGGL((count_kernel<64, 1>), dim3(grid), blk, lds, stream, g, values, offsets, nq,
+                               tile_words, counter_bytes, counts);
+            break;
+        }
+    }
+#undef HIBF_COUNT_CASE
+    return hipGetLastError();
+}
+
+} // namespace hibf_amd

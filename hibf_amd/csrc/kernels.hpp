@@ -1,0 +1,142 @@
+// Host-side launch wrappers of the HIBF query kernels (gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "device_common.hpp"
+
+namespace hibf_amd
+{
+
+// Largest IBF (technical bins) the HIBF descent kernel handles: per-IBF
+// counters live in LDS.
+constexpr uint32_t kHibfMaxBins = 8192;
+// Deepest HIBF (levels) the descent kernel handles.
+constexpr uint32_t kHibfMaxDepth = 16;
+
+// Kinds of a technical bin in the descent tables (hierarchical_interleaved_bloom_filter.hpp:329-354).
+enum TbKind : uint32_t
+{
+    kTbRegular = 0u, // user bin, not the last bin of its run
+    kTbRunEnd = 1u,  // user bin, last bin of its run (or last bin of the IBF)
+    kTbMerged = 2u,  // merged bin: payload = child IBF index
+    kTbDeleted = 3u  // deleted bin
+};
+
+// One IBF of an HIBF as the descent kernel sees it.
+struct HibfIbfDesc
+{
+    uint64_t word_offset; // first word in the arena
+    uint64_t bin_size;
+    uint64_t tb_offset; // first entry in tb_info
+    uint32_t bins;
+    uint32_t bin_words;
+    uint32_t hash_shift;
+    uint32_t hash_funs;
+    uint32_t lpv_log; // count_into_lds_dispatch shape
+    uint32_t pad;
+};
+
+void count_shape(uint32_t bin_words, uint32_t & lpv_log, uint32_t & cw, uint32_t & tile_words);
+
+hipError_t launch_emplace(uint64_t * words,
+                          uint64_t bin_size,
+                          uint32_t bin_words,
+                          uint32_t hash_shift,
+                          uint32_t hash_funs,
+                          uint64_t const * values,
+                          uint64_t const * bins,
+                          uint64_t bin,
+                          uint64_t n,
+                          uint64_t bin_count,
+                          hipStream_t stream);
+
+hipError_t launch_clear(uint64_t * words,
+                        uint64_t bin_size,
+                        uint32_t bin_words,
+                        uint64_t const * d_bins,
+                        uint32_t nbins,
+                        hipStream_t stream);
+
+hipError_t launch_contains(IbfView const & g, uint64_t const * values, uint64_t n, uint64_t * out, hipStream_t stream);
+
+hipError_t launch_count(IbfView const & g,
+                        uint64_t const * values,
+                        uint64_t const * offsets,
+                        uint64_t nq,
+                        uint32_t counter_bytes,
+                        void * counts,
+                        unsigned max_blocks,
+                        hipStream_t stream);
+
+// IBF membership_for from counts already in device memory: bins with
+// uint16 count >= threshold[q], ascending, into per-query slots.
+hipError_t launch_threshold_compact(uint16_t const * counts,
+                                    uint64_t bins,
+                                    uint64_t nq,
+                                    uint16_t const * thresholds,
+                                    uint64_t * hit_count,
+                                    uint64_t const * hit_offsets,
+                                    uint64_t * ids,
+                                    uint64_t ids_cap,
+                                    int pass,
+                                    hipStream_t stream);
+
+struct HibfLaunch
+{
+    HibfIbfDesc const * ibfs;
+    uint64_t const * arena;
+    uint64_t const * tb_info; // (kind << 32) | payload
+    uint32_t max_bins;        // largest IBF of this HIBF
+    uint32_t depth;           // levels
+    uint64_t n_user_bins;
+};
+
+// Membership descent.  Writes, per query, hit_start/hit_count into ids_tmp
+// (atomic order of queries); *total receives the number of ids.
+hipError_t launch_hibf_membership(HibfLaunch const & h,
+                                  uint64_t const * values,
+                                  uint64_t const * offsets,
+                                  uint64_t nq,
+                                  uint16_t const * thresholds,
+                                  uint32_t * scratch,
+                                  unsigned blocks,
+                                  uint64_t * ids_tmp,
+                                  uint64_t ids_cap,
+                                  uint64_t * hit_start,
+                                  uint64_t * hit_count,
+                                  unsigned long long * total,
+                                  hipStream_t stream);
+
+hipError_t launch_hibf_count(HibfLaunch const & h,
+                             uint64_t const * values,
+                             uint64_t const * offsets,
+                             uint64_t nq,
+                             uint64_t const * thresholds,
+                             uint32_t counter_bytes,
+                             void * counts,
+                             unsigned blocks,
+                             hipStream_t stream);
+
+// out[offsets[q] + i] = src[start[q] + i] for i < count[q].
+hipError_t launch_gather_hits(uint64_t const * src,
+                              uint64_t const * start,
+                              uint64_t const * count,
+                              uint64_t const * offsets,
+                              uint64_t nq,
+                              uint64_t * out,
+                              hipStream_t stream);
+
+// out[0] = base0 (only when write_first), out[1 + i] = base + incl[i].
+hipError_t launch_finalize_offsets(uint64_t const * incl, uint64_t n, uint64_t base, bool write_first,
+                                   uint64_t * out, hipStream_t stream);
+
+} // namespace hibf_amd
+
+namespace hibf_amd
+{
+// hipCUB inclusive sum; call with temp == nullptr to size temp_bytes.
+hipError_t inclusive_sum_u64(void * temp, size_t & temp_bytes, uint64_t const * in, uint64_t * out, uint64_t n,
+                             hipStream_t stream);
+} // namespace hibf_amd

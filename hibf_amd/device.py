@@ -1,0 +1,259 @@
+"""Device-resident IBF / HIBF handles over the C-ABI, with torch tensors as HBM buffers.
+
+Mirrors the reference's query interface (names and argument meaning of
+seqan::hibf::interleaved_bloom_filter and hierarchical_interleaved_bloom_filter,
+interleaved_bloom_filter.hpp:225-446 and hierarchical_interleaved_bloom_filter.hpp:197-247),
+batched: a batch of queries is a CSR pair (values, offsets) of uint64 tensors.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import native
+from .native import Geometry, check, lib
+
+_DTYPES = {1: torch.uint8, 2: torch.uint16, 4: torch.uint32, 8: torch.uint64}
+
+
+def _dev() -> torch.device:
+    if not torch.cuda.is_available():
+        raise RuntimeError("hibf_amd needs a HIP device; none is visible")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def _stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _ptr(t) -> ctypes.c_void_p:
+    return ctypes.c_void_p(0 if t is None else t.data_ptr())
+
+
+def as_device_u64(x) -> torch.Tensor:
+    """uint64 tensor on the device from a numpy array, list or tensor."""
+    if isinstance(x, torch.Tensor):
+        t = x
+        if t.dtype != torch.uint64:
+            t = t.to(torch.int64).view(torch.uint64) if t.dtype != torch.int64 else t.view(torch.uint64)
+        return t.to(_dev()).contiguous()
+    a = np.ascontiguousarray(np.asarray(x, dtype=np.uint64))
+    return torch.from_numpy(a.view(np.int64)).to(_dev()).view(torch.uint64)
+
+
+def to_numpy_u64(t: torch.Tensor) -> np.ndarray:
+    return t.view(torch.int64).cpu().numpy().view(np.uint64)
+
+
+def csr(queries) -> tuple[np.ndarray, np.ndarray]:
+    """(values, offsets) for a list of uint64 sequences."""
+    lens = [len(q) for q in queries]
+    offsets = np.zeros(len(queries) + 1, dtype=np.uint64)
+    offsets[1:] = np.cumsum(lens, dtype=np.uint64)
+    values = np.concatenate([np.asarray(q, dtype=np.uint64) for q in queries]) if queries and sum(lens) else \
+        np.zeros(0, dtype=np.uint64)
+    return values, offsets
+
+
+class Agent:
+    """Per-caller device workspace (the reference's one-agent-per-thread rule)."""
+
+    def __init__(self, max_queries_per_launch: int = 0):
+        h = ctypes.c_void_p()
+        check(lib().hibf_gpu_agent_create(max_queries_per_launch, ctypes.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().hibf_gpu_agent_destroy(self.h)
+            self.h = None
+
+
+class DeviceIbf:
+    """interleaved_bloom_filter(bin_count, bin_size, hash_function_count) in HBM."""
+
+    def __init__(self, bin_count: int, bin_size: int, hash_function_count: int = 2):
+        _dev()
+        h = ctypes.c_void_p()
+        check(lib().hibf_gpu_ibf_create(bin_count, bin_size, hash_function_count, ctypes.byref(h)))
+        self.h = h
+        g = Geometry()
+        check(lib().hibf_gpu_ibf_get_geometry(self.h, ctypes.byref(g)))
+        self.geometry = g
+        self._agent = None
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().hibf_gpu_ibf_destroy(self.h)
+            self.h = None
+
+    # interleaved_bloom_filter.hpp:400-427
+    def bin_count(self) -> int:
+        return int(self.geometry.bins)
+
+    def bin_size(self) -> int:
+        return int(self.geometry.bin_size)
+
+    def hash_function_count(self) -> int:
+        return int(self.geometry.hash_funs)
+
+    def bit_size(self) -> int:
+        return int(self.geometry.technical_bins * self.geometry.bin_size)
+
+    def emplace(self, values, bins) -> None:
+        v = as_device_u64(values)
+        if isinstance(bins, (int, np.integer)):
+            check(lib().hibf_gpu_ibf_emplace(self.h, _ptr(v), None, int(bins), v.numel(), _stream()))
+        else:
+            b = as_device_u64(bins)
+            assert b.numel() == v.numel()
+            check(lib().hibf_gpu_ibf_emplace(self.h, _ptr(v), _ptr(b), 0, v.numel(), _stream()))
+        self._keep = (v,)
+        torch.cuda.current_stream().synchronize()
+
+    def clear(self, bins) -> None:
+        b = np.ascontiguousarray(np.atleast_1d(np.asarray(bins, dtype=np.uint64)))
+        check(lib().hibf_gpu_ibf_clear(self.h, b.ctypes.data_as(ctypes.c_void_p), b.size, _stream()))
+
+    def download(self) -> np.ndarray:
+        out = np.zeros(self.geometry.n_words, dtype=np.uint64)
+        check(lib().hibf_gpu_ibf_download(self.h, out.ctypes.data_as(ctypes.c_void_p), out.size))
+        return out
+
+    def upload(self, words: np.ndarray) -> None:
+        w = np.ascontiguousarray(words, dtype=np.uint64)
+        check(lib().hibf_gpu_ibf_upload(self.h, w.ctypes.data_as(ctypes.c_void_p), w.size))
+
+    def bulk_contains(self, values) -> torch.Tensor:
+        v = as_device_u64(values)
+        out = torch.empty((v.numel(), int(self.geometry.bin_words)), dtype=torch.uint64, device=v.device)
+        check(lib().hibf_gpu_ibf_bulk_contains(self.h, _ptr(v), v.numel(), _ptr(out), _stream()))
+        return out
+
+    def bulk_count(self, values, offsets, counter_bytes: int = 2) -> torch.Tensor:
+        v = as_device_u64(values)
+        o = as_device_u64(offsets)
+        nq = o.numel() - 1
+        out = torch.empty((nq, self.bin_count()), dtype=_DTYPES[counter_bytes], device=o.device)
+        check(lib().hibf_gpu_ibf_bulk_count(self.h, _ptr(v), _ptr(o), nq, counter_bytes, _ptr(out), _stream()))
+        return out
+
+    def membership_for(self, values, offsets, thresholds, agent: Agent | None = None):
+        """Returns (ids, hit_offsets) as device tensors (CSR over queries)."""
+        v = as_device_u64(values)
+        o = as_device_u64(offsets)
+        nq = o.numel() - 1
+        t = _thresholds16(thresholds, nq, o.device)
+        agent = agent or self._default_agent()
+        return _membership(lambda ids, cap, hoff, total: lib().hibf_gpu_ibf_membership_for(
+            self.h, agent.h, _ptr(v), _ptr(o), nq, _ptr(t), _ptr(ids), cap, _ptr(hoff), ctypes.byref(total),
+            _stream()), nq, o.device)
+
+    def _default_agent(self) -> Agent:
+        if self._agent is None:
+            self._agent = Agent()
+        return self._agent
+
+
+def _thresholds16(thresholds, nq: int, device) -> torch.Tensor:
+    a = np.ascontiguousarray(np.broadcast_to(np.asarray(thresholds, dtype=np.uint16), (nq,)))
+    return torch.from_numpy(a.view(np.int16)).to(device).view(torch.uint16)
+
+
+def _membership(call, nq: int, device):
+    cap = max(1024, nq * 4)
+    for _ in range(2):
+        ids = torch.empty(cap, dtype=torch.uint64, device=device)
+        hoff = torch.empty(nq + 1, dtype=torch.uint64, device=device)
+        total = ctypes.c_uint64(0)
+        rc = call(ids, cap, hoff, total)
+        if rc == native.ERR_CAPACITY:
+            cap = int(total.value)
+            continue
+        check(rc)
+        torch.cuda.current_stream().synchronize()
+        return ids[: int(total.value)], hoff
+    raise native.HibfGpuError(native.ERR_CAPACITY, "membership output did not fit after resizing")
+
+
+class DeviceHibf:
+    """A built HIBF uploaded to HBM: IBF arena + routing tables
+    (hierarchical_interleaved_bloom_filter.hpp:145, 209-238)."""
+
+    def __init__(self, geoms, word_offsets, tb_offsets, next_ibf, user_bin, number_of_user_bins: int,
+                 arena: np.ndarray | None = None):
+        _dev()
+        n = len(geoms)
+        garr = (Geometry * n)()
+        for i, (bins, bin_size, h) in enumerate(geoms):
+            garr[i].bins, garr[i].bin_size, garr[i].hash_funs = bins, bin_size, h
+        arrs = [np.ascontiguousarray(np.asarray(x, dtype=np.uint64)) for x in
+                (word_offsets, tb_offsets, next_ibf, user_bin)]
+        ar = None if arena is None else np.ascontiguousarray(arena, dtype=np.uint64)
+        h = ctypes.c_void_p()
+        check(lib().hibf_gpu_hibf_create(n, garr, None if ar is None else ar.ctypes.data_as(ctypes.c_void_p),
+                                         *[a.ctypes.data_as(ctypes.c_void_p) for a in arrs],
+                                         number_of_user_bins, ctypes.byref(h)))
+        self.h = h
+        self.number_of_user_bins = int(number_of_user_bins)
+        self._agent = None
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().hibf_gpu_hibf_destroy(self.h)
+            self.h = None
+
+    def emplace(self, ibf_idx: int, values, bins) -> None:
+        v = as_device_u64(values)
+        if isinstance(bins, (int, np.integer)):
+            check(lib().hibf_gpu_hibf_emplace(self.h, ibf_idx, _ptr(v), None, int(bins), v.numel(), _stream()))
+        else:
+            b = as_device_u64(bins)
+            check(lib().hibf_gpu_hibf_emplace(self.h, ibf_idx, _ptr(v), _ptr(b), 0, v.numel(), _stream()))
+        torch.cuda.current_stream().synchronize()
+
+    def download_arena(self) -> np.ndarray:
+        n = ctypes.c_uint64()
+        check(lib().hibf_gpu_hibf_arena_words(self.h, ctypes.byref(n)))
+        out = np.zeros(n.value, dtype=np.uint64)
+        check(lib().hibf_gpu_hibf_download_arena(self.h, out.ctypes.data_as(ctypes.c_void_p), out.size))
+        return out
+
+    def membership_for(self, values, offsets, thresholds, agent: Agent | None = None):
+        v = as_device_u64(values)
+        o = as_device_u64(offsets)
+        nq = o.numel() - 1
+        t = _thresholds16(thresholds, nq, o.device)
+        agent = agent or self._default_agent()
+        return _membership(lambda ids, cap, hoff, total: lib().hibf_gpu_hibf_membership_for(
+            self.h, agent.h, _ptr(v), _ptr(o), nq, _ptr(t), _ptr(ids), cap, _ptr(hoff), ctypes.byref(total),
+            _stream()), nq, o.device)
+
+    def bulk_count(self, values, offsets, thresholds, counter_bytes: int = 2, agent: Agent | None = None):
+        v = as_device_u64(values)
+        o = as_device_u64(offsets)
+        nq = o.numel() - 1
+        t = as_device_u64(np.broadcast_to(np.asarray(thresholds, dtype=np.uint64), (nq,)))
+        agent = agent or self._default_agent()
+        out = torch.empty((nq, self.number_of_user_bins), dtype=_DTYPES[counter_bytes], device=o.device)
+        check(lib().hibf_gpu_hibf_bulk_count(self.h, agent.h, _ptr(v), _ptr(o), nq, _ptr(t), counter_bytes,
+                                             _ptr(out), _stream()))
+        return out
+
+    def _default_agent(self) -> Agent:
+        if self._agent is None:
+            self._agent = Agent()
+        return self._agent
+
+
+_NP_VIEW = {torch.uint8: (torch.uint8, np.uint8), torch.uint16: (torch.int16, np.uint16),
+            torch.uint32: (torch.int32, np.uint32), torch.uint64: (torch.int64, np.uint64)}
+
+
+def to_numpy(t: torch.Tensor) -> np.ndarray:
+    """Host copy of an unsigned device tensor as the matching numpy dtype."""
+    sview, ndt = _NP_VIEW[t.dtype]
+    src = t if sview == t.dtype else t.view(sview)
+    return src.cpu().numpy().view(ndt)

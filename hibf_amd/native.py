@@ -1,0 +1,110 @@
+"""ctypes binding of libhibf_gpu.so (the C-ABI declared in include/hibf_gpu.h).
+
+The library is the only compute path: importing works on a machine without a
+GPU (for symbol and argument checks), but every query needs the HIP device and
+fails loudly if the library or the device is missing.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+LIB_PATH = os.path.join(HERE, "libhibf_gpu.so")
+HEADER = os.path.join(ROOT, "include", "hibf_gpu.h")
+
+OK = 0
+ERR_INVALID_ARGUMENT = 1
+ERR_BIN_COUNT = 2
+ERR_HASH_COUNT = 3
+ERR_BIN_SIZE = 4
+ERR_OUT_OF_MEMORY = 5
+ERR_DEVICE = 6
+ERR_CAPACITY = 7
+ERR_UNSUPPORTED = 8
+ERR_CONFIG = 9
+ERR_BUDGET = 10
+ERR_NO_DEVICE = 11
+
+BIN_MERGED = 0xFFFFFFFFFFFFFFFF
+BIN_DELETED = 0xFFFFFFFFFFFFFFFE
+
+
+class HibfGpuError(RuntimeError):
+    def __init__(self, code: int, what: str):
+        super().__init__(f"[{code}] {what}")
+        self.code = code
+
+
+class Geometry(ctypes.Structure):
+    _fields_ = [
+        ("bins", ctypes.c_uint64),
+        ("technical_bins", ctypes.c_uint64),
+        ("bin_size", ctypes.c_uint64),
+        ("hash_shift", ctypes.c_uint64),
+        ("bin_words", ctypes.c_uint64),
+        ("hash_funs", ctypes.c_uint64),
+        ("n_words", ctypes.c_uint64),
+    ]
+
+
+_V = ctypes.c_void_p
+_U64 = ctypes.c_uint64
+_U32 = ctypes.c_uint32
+_INT = ctypes.c_int
+
+_SIGNATURES = {
+    "hibf_gpu_status_string": (ctypes.c_char_p, [_INT]),
+    "hibf_gpu_last_error": (ctypes.c_char_p, []),
+    "hibf_gpu_ibf_create": (_INT, [_U64, _U64, _U32, ctypes.POINTER(_V)]),
+    "hibf_gpu_ibf_destroy": (_INT, [_V]),
+    "hibf_gpu_ibf_get_geometry": (_INT, [_V, ctypes.POINTER(Geometry)]),
+    "hibf_gpu_ibf_device_words": (_INT, [_V, ctypes.POINTER(_V)]),
+    "hibf_gpu_ibf_upload": (_INT, [_V, _V, _U64]),
+    "hibf_gpu_ibf_download": (_INT, [_V, _V, _U64]),
+    "hibf_gpu_ibf_emplace": (_INT, [_V, _V, _V, _U64, _U64, _V]),
+    "hibf_gpu_ibf_clear": (_INT, [_V, _V, _U64, _V]),
+    "hibf_gpu_ibf_bulk_contains": (_INT, [_V, _V, _U64, _V, _V]),
+    "hibf_gpu_ibf_bulk_count": (_INT, [_V, _V, _V, _U64, _U32, _V, _V]),
+    "hibf_gpu_agent_create": (_INT, [_U64, ctypes.POINTER(_V)]),
+    "hibf_gpu_agent_destroy": (_INT, [_V]),
+    "hibf_gpu_ibf_membership_for": (_INT, [_V, _V, _V, _V, _U64, _V, _V, _U64, _V, ctypes.POINTER(_U64), _V]),
+    "hibf_gpu_hibf_create": (_INT, [_U64, ctypes.POINTER(Geometry), _V, _V, _V, _V, _V, _U64, ctypes.POINTER(_V)]),
+    "hibf_gpu_hibf_destroy": (_INT, [_V]),
+    "hibf_gpu_hibf_emplace": (_INT, [_V, _U64, _V, _V, _U64, _U64, _V]),
+    "hibf_gpu_hibf_download_arena": (_INT, [_V, _V, _U64]),
+    "hibf_gpu_hibf_arena_words": (_INT, [_V, ctypes.POINTER(_U64)]),
+    "hibf_gpu_hibf_membership_for": (_INT, [_V, _V, _V, _V, _U64, _V, _V, _U64, _V, ctypes.POINTER(_U64), _V]),
+    "hibf_gpu_hibf_bulk_count": (_INT, [_V, _V, _V, _V, _U64, _V, _U32, _V, _V]),
+}
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    """Load libhibf_gpu.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m hibf_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def declared_functions() -> list[str]:
+    """Names of the functions declared in include/hibf_gpu.h."""
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char \*)\s+(hibf_gpu_\w+)\s*\(", text, re.M)))
+
+
+def check(rc: int) -> None:
+    if rc != OK:
+        L = lib()
+        raise HibfGpuError(rc, f"{L.hibf_gpu_status_string(rc).decode()}: {L.hibf_gpu_last_error().decode()}")

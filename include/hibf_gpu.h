@@ -1,0 +1,223 @@
+/*
+ * hibf_gpu.h -- C-ABI of the MI355X-native HIBF query path.
+ *
+ * Every entry point is extern "C", takes plain pointers and sizes, never
+ * throws, and returns an int status (HIBF_GPU_OK == 0).  Pointers named d_*
+ * are device pointers (HBM); pointers named h_* are host pointers.  `stream`
+ * is a hipStream_t passed as void* (NULL = the default stream).  Query entry
+ * points are asynchronous on `stream` unless stated otherwise.
+ *
+ * Each declaration names the reference interface it replaces (paths relative
+ * to the seqan/hibf 1.0.0-rc.1 source tree).
+ */
+#ifndef HIBF_GPU_H
+#define HIBF_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+enum hibf_gpu_status
+{
+    HIBF_GPU_OK = 0,
+    HIBF_GPU_ERR_INVALID_ARGUMENT = 1, /* null handle/pointer, size mismatch          */
+    HIBF_GPU_ERR_BIN_COUNT = 2,        /* "The number of bins must be > 0."           */
+    HIBF_GPU_ERR_HASH_COUNT = 3,       /* "The number of hash functions must be > 0 and <= 5." */
+    HIBF_GPU_ERR_BIN_SIZE = 4,         /* "The size of a bin must be > 0."            */
+    HIBF_GPU_ERR_OUT_OF_MEMORY = 5,    /* device allocation failed                    */
+    HIBF_GPU_ERR_DEVICE = 6,           /* a HIP runtime call failed                   */
+    HIBF_GPU_ERR_CAPACITY = 7,         /* an output buffer is too small               */
+    HIBF_GPU_ERR_UNSUPPORTED = 8,      /* geometry outside what the kernels handle    */
+    HIBF_GPU_ERR_CONFIG = 9,           /* invalid config (config.cpp:64-130)          */
+    HIBF_GPU_ERR_BUDGET = 10,          /* admission over the counter-memory budget    */
+    HIBF_GPU_ERR_NO_DEVICE = 11        /* no HIP device visible                       */
+};
+
+/* Text of a status code (static storage). */
+const char * hibf_gpu_status_string(int status);
+/* Detail of the last error raised on the calling thread (static per thread). */
+const char * hibf_gpu_last_error(void);
+
+/* Sentinels of ibf_bin_to_user_bin_id, hierarchical_interleaved_bloom_filter.hpp:32-40. */
+#define HIBF_GPU_BIN_MERGED 0xFFFFFFFFFFFFFFFFULL
+#define HIBF_GPU_BIN_DELETED 0xFFFFFFFFFFFFFFFEULL
+
+/* ======================================================================== */
+/* Interleaved Bloom Filter (device resident)                               */
+/* ======================================================================== */
+typedef struct hibf_gpu_ibf_s * hibf_gpu_ibf;
+
+/* Geometry as interleaved_bloom_filter.hpp:160-170. */
+typedef struct hibf_gpu_ibf_geometry
+{
+    uint64_t bins;           /* bin_count()                              */
+    uint64_t technical_bins; /* 64 * bin_words                            */
+    uint64_t bin_size;       /* bin_size(): rows                          */
+    uint64_t hash_shift;     /* countl_zero(bin_size)                     */
+    uint64_t bin_words;      /* ceil(bins / 64)                           */
+    uint64_t hash_funs;      /* hash_function_count()                     */
+    uint64_t n_words;        /* technical_bins * bin_size / 64            */
+} hibf_gpu_ibf_geometry;
+
+/* Replaces interleaved_bloom_filter(bin_count, bin_size, hash_function_count)
+ * (interleaved_bloom_filter.hpp:225-228; src/interleaved_bloom_filter.cpp:32-53):
+ * allocates a zeroed bit matrix in HBM.  Errors 2/3/4 mirror the reference's
+ * three std::logic_error cases in the same order. */
+int hibf_gpu_ibf_create(uint64_t bins, uint64_t bin_size, uint32_t hash_funs, hibf_gpu_ibf * out);
+int hibf_gpu_ibf_destroy(hibf_gpu_ibf ibf);
+
+/* Replaces bin_count()/bin_size()/bit_size()/hash_function_count()
+ * (interleaved_bloom_filter.hpp:400-427). */
+int hibf_gpu_ibf_get_geometry(hibf_gpu_ibf ibf, hibf_gpu_ibf_geometry * out);
+
+/* Device address of the bit matrix (n_words uint64 words, row-major:
+ * row r = words [r*bin_words, (r+1)*bin_words)); replaces data()
+ * (interleaved_bloom_filter.hpp:440-446). */
+int hibf_gpu_ibf_device_words(hibf_gpu_ibf ibf, uint64_t ** d_words);
+
+/* Host <-> device copies of the whole bit matrix (synchronous).  The word
+ * layout is the reference's bit_vector storage (bit_vector.hpp:868-876). */
+int hibf_gpu_ibf_upload(hibf_gpu_ibf ibf, uint64_t const * h_words, uint64_t n_words);
+int hibf_gpu_ibf_download(hibf_gpu_ibf ibf, uint64_t * h_words, uint64_t n_words);
+
+/* Bulk emplace; replaces emplace(value, bin_index) (interleaved_bloom_filter.hpp:259;
+ * src/interleaved_bloom_filter.cpp:119-140) called for n (value, bin) pairs.
+ * d_bins may be NULL, in which case every value goes to `bin`.  Bits are set
+ * with atomicOr, so the result does not depend on order or concurrency.
+ * Bins >= bin_count are rejected only when d_bins == NULL (checked on the
+ * host); per-pair bins are the caller's precondition as in the reference. */
+int hibf_gpu_ibf_emplace(hibf_gpu_ibf ibf,
+                         uint64_t const * d_values,
+                         uint64_t const * d_bins,
+                         uint64_t bin,
+                         uint64_t n,
+                         void * stream);
+
+/* Replaces clear(bin_index) / clear(range) (interleaved_bloom_filter.hpp:270, 283-297;
+ * src/interleaved_bloom_filter.cpp:142-148) for n bins given on the host. */
+int hibf_gpu_ibf_clear(hibf_gpu_ibf ibf, uint64_t const * h_bins, uint64_t n, void * stream);
+
+/* Batched containment_agent_type::bulk_contains (src/interleaved_bloom_filter.cpp:204-291):
+ * d_out receives n * bin_words words, the AND of the h rows of each value. */
+int hibf_gpu_ibf_bulk_contains(hibf_gpu_ibf ibf,
+                               uint64_t const * d_values,
+                               uint64_t n,
+                               uint64_t * d_out,
+                               void * stream);
+
+/* Batched counting_agent_type<T>::bulk_count (interleaved_bloom_filter.hpp:652-665)
+ * over nq CSR queries: query q is d_values[d_offsets[q] .. d_offsets[q+1]).
+ * d_counts receives nq * bin_count counters of counter_bytes (1, 2, 4 or 8)
+ * bytes each, wrapping modulo 2^(8*counter_bytes) like counting_vector<T>. */
+int hibf_gpu_ibf_bulk_count(hibf_gpu_ibf ibf,
+                            uint64_t const * d_values,
+                            uint64_t const * d_offsets,
+                            uint64_t nq,
+                            uint32_t counter_bytes,
+                            void * d_counts,
+                            void * stream);
+
+/* ======================================================================== */
+/* Query agents: per-caller device workspace (one agent per host thread, as  */
+/* the reference asks for its agents, interleaved_bloom_filter.hpp:547-551). */
+/* ======================================================================== */
+typedef struct hibf_gpu_agent_s * hibf_gpu_agent;
+
+/* max_queries_per_launch bounds the per-call workspace (0 = 65536). */
+int hibf_gpu_agent_create(uint64_t max_queries_per_launch, hibf_gpu_agent * out);
+int hibf_gpu_agent_destroy(hibf_gpu_agent agent);
+
+/* Batched membership_agent_type::membership_for (interleaved_bloom_filter.hpp:744-764):
+ * bins with uint16 count >= d_thresholds[q], ascending.  Results of query q
+ * are d_ids[d_hit_offsets[q] .. d_hit_offsets[q+1]) (d_hit_offsets has nq+1
+ * entries).  Synchronous: returns HIBF_GPU_ERR_CAPACITY and writes the
+ * needed total to *h_total when ids_cap is too small. */
+int hibf_gpu_ibf_membership_for(hibf_gpu_ibf ibf,
+                                hibf_gpu_agent agent,
+                                uint64_t const * d_values,
+                                uint64_t const * d_offsets,
+                                uint64_t nq,
+                                uint16_t const * d_thresholds,
+                                uint64_t * d_ids,
+                                uint64_t ids_cap,
+                                uint64_t * d_hit_offsets,
+                                uint64_t * h_total,
+                                void * stream);
+
+/* ======================================================================== */
+/* Hierarchical Interleaved Bloom Filter (device resident)                  */
+/* ======================================================================== */
+typedef struct hibf_gpu_hibf_s * hibf_gpu_hibf;
+
+/* Upload of a built HIBF: replaces the public members ibf_vector, next_ibf_id
+ * and ibf_bin_to_user_bin_id (hierarchical_interleaved_bloom_filter.hpp:145, 209-238).
+ *   geoms[i]            bins, bin_size, hash_funs of IBF i (other fields ignored)
+ *   h_arena             concatenated bit matrices; IBF i starts at h_word_offsets[i]
+ *                       (NULL: allocate zeroed and fill later with hibf_gpu_hibf_emplace)
+ *   h_tb_offsets        n_ibfs+1 CSR offsets into the per-technical-bin tables
+ *   h_next_ibf          next_ibf_id[i][b]
+ *   h_user_bin          ibf_bin_to_user_bin_id[i][b] (HIBF_GPU_BIN_MERGED/DELETED sentinels) */
+int hibf_gpu_hibf_create(uint64_t n_ibfs,
+                         hibf_gpu_ibf_geometry const * geoms,
+                         uint64_t const * h_arena,
+                         uint64_t const * h_word_offsets,
+                         uint64_t const * h_tb_offsets,
+                         uint64_t const * h_next_ibf,
+                         uint64_t const * h_user_bin,
+                         uint64_t number_of_user_bins,
+                         hibf_gpu_hibf * out);
+int hibf_gpu_hibf_destroy(hibf_gpu_hibf hibf);
+
+/* Bulk emplace into IBF `ibf_idx` of the HIBF (same contract as hibf_gpu_ibf_emplace). */
+int hibf_gpu_hibf_emplace(hibf_gpu_hibf hibf,
+                          uint64_t ibf_idx,
+                          uint64_t const * d_values,
+                          uint64_t const * d_bins,
+                          uint64_t bin,
+                          uint64_t n,
+                          void * stream);
+
+/* Copies the whole device arena back (synchronous, for parity checks). */
+int hibf_gpu_hibf_download_arena(hibf_gpu_hibf hibf, uint64_t * h_arena, uint64_t n_words);
+int hibf_gpu_hibf_arena_words(hibf_gpu_hibf hibf, uint64_t * n_words);
+
+/* Batched HIBF membership_agent_type::membership_for
+ * (hierarchical_interleaved_bloom_filter.hpp:321-355, 424-439): user-bin ids in
+ * the reference's unsorted depth-first order.  Same output contract as
+ * hibf_gpu_ibf_membership_for (synchronous; HIBF_GPU_ERR_CAPACITY + *h_total). */
+int hibf_gpu_hibf_membership_for(hibf_gpu_hibf hibf,
+                                 hibf_gpu_agent agent,
+                                 uint64_t const * d_values,
+                                 uint64_t const * d_offsets,
+                                 uint64_t nq,
+                                 uint16_t const * d_thresholds,
+                                 uint64_t * d_ids,
+                                 uint64_t ids_cap,
+                                 uint64_t * d_hit_offsets,
+                                 uint64_t * h_total,
+                                 void * stream);
+
+/* Batched HIBF counting_agent_type<T>::bulk_count
+ * (hierarchical_interleaved_bloom_filter.hpp:462-496, 549-566): d_counts gets
+ * nq * number_of_user_bins counters of counter_bytes (1, 2, 4, 8); a user bin's
+ * counter is its run sum if that sum >= d_thresholds[q] (size_t, > 0) and 0
+ * otherwise.  Asynchronous on `stream`. */
+int hibf_gpu_hibf_bulk_count(hibf_gpu_hibf hibf,
+                             hibf_gpu_agent agent,
+                             uint64_t const * d_values,
+                             uint64_t const * d_offsets,
+                             uint64_t nq,
+                             uint64_t const * d_thresholds,
+                             uint32_t counter_bytes,
+                             void * d_counts,
+                             void * stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HIBF_GPU_H */

@@ -1,0 +1,275 @@
+"""Oracle for the HIBF query path (test infrastructure only).
+
+Python bindings to ``liboracle.so`` (``hibf_oracle.c``), a plain-C restatement
+of the reference query path, plus an independent pure-Python big-integer
+restatement of the hash used to cross-check it.  Only ``tests/``, the smoke
+check in ``__graft_entry__.py`` and the CPU-baseline leg of ``bench.py`` import
+this package, and only as the checker.  The product package ``hibf_amd`` never
+imports it.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+BIN_MERGED = 0xFFFFFFFFFFFFFFFF  # hierarchical_interleaved_bloom_filter.hpp:34
+BIN_DELETED = 0xFFFFFFFFFFFFFFFE  # hierarchical_interleaved_bloom_filter.hpp:35
+
+# interleaved_bloom_filter.hpp:172-176
+HASH_SEEDS = (
+    13572355802537770549,
+    13043817825332782213,
+    10650232656628343401,
+    16499269484942379435,
+    4893150838803335377,
+)
+
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+
+
+class OrcIbf(ctypes.Structure):
+    """Geometry of one IBF (interleaved_bloom_filter.hpp:160-170)."""
+
+    _fields_ = [
+        ("bins", ctypes.c_uint64),
+        ("technical_bins", ctypes.c_uint64),
+        ("bin_size", ctypes.c_uint64),
+        ("hash_shift", ctypes.c_uint64),
+        ("bin_words", ctypes.c_uint64),
+        ("hash_funs", ctypes.c_uint64),
+    ]
+
+    @property
+    def word_count(self) -> int:
+        return self.technical_bins * self.bin_size // 64
+
+
+class OrcHibf(ctypes.Structure):
+    _fields_ = [
+        ("n_ibfs", ctypes.c_uint64),
+        ("ibfs", ctypes.POINTER(OrcIbf)),
+        ("arena", _u64p),
+        ("word_offsets", _u64p),
+        ("tb_offsets", _u64p),
+        ("next_ibf", _u64p),
+        ("user_bin", _u64p),
+    ]
+
+
+def build() -> str:
+    """Compile liboracle.so with the committed Makefile."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _LIB_PATH
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        L.orc_hash_and_fit.restype = ctypes.c_uint64
+        L.orc_hash_and_fit.argtypes = [ctypes.c_uint64] * 5
+        L.orc_ibf_init.restype = ctypes.c_int
+        L.orc_ibf_init.argtypes = [ctypes.POINTER(OrcIbf), ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64]
+        L.orc_ibf_emplace.argtypes = [ctypes.c_void_p, ctypes.POINTER(OrcIbf), ctypes.c_void_p, ctypes.c_void_p,
+                                      ctypes.c_uint64]
+        L.orc_ibf_clear.argtypes = [ctypes.c_void_p, ctypes.POINTER(OrcIbf), ctypes.c_uint64]
+        L.orc_ibf_bulk_contains.argtypes = [ctypes.c_void_p, ctypes.POINTER(OrcIbf), ctypes.c_uint64,
+                                            ctypes.c_void_p]
+        L.orc_counting_vector_add.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_uint64]
+        L.orc_ibf_bulk_count.argtypes = [ctypes.c_void_p, ctypes.POINTER(OrcIbf), ctypes.c_void_p, ctypes.c_uint64,
+                                         ctypes.c_int, ctypes.c_void_p]
+        L.orc_ibf_bulk_count_batch.argtypes = [ctypes.c_void_p, ctypes.POINTER(OrcIbf), ctypes.c_void_p,
+                                               ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_void_p,
+                                               ctypes.c_int]
+        L.orc_ibf_membership_for.restype = ctypes.c_uint64
+        L.orc_ibf_membership_for.argtypes = [ctypes.c_void_p, ctypes.POINTER(OrcIbf), ctypes.c_void_p,
+                                             ctypes.c_uint64, ctypes.c_uint16, ctypes.c_void_p]
+        L.orc_hibf_membership_for.restype = ctypes.c_uint64
+        L.orc_hibf_membership_for.argtypes = [ctypes.POINTER(OrcHibf), ctypes.c_void_p, ctypes.c_uint64,
+                                              ctypes.c_uint16, ctypes.c_void_p, ctypes.c_uint64]
+        L.orc_hibf_membership_batch.restype = ctypes.c_uint64
+        L.orc_hibf_membership_batch.argtypes = [ctypes.POINTER(OrcHibf), ctypes.c_void_p, ctypes.c_void_p,
+                                                ctypes.c_uint64, ctypes.c_void_p, ctypes.c_void_p,
+                                                ctypes.c_uint64, ctypes.c_void_p]
+        L.orc_hibf_bulk_count.argtypes = [ctypes.POINTER(OrcHibf), ctypes.c_uint64, ctypes.c_void_p,
+                                          ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int, ctypes.c_void_p]
+        L.orc_hibf_bulk_count_batch.argtypes = [ctypes.POINTER(OrcHibf), ctypes.c_uint64, ctypes.c_void_p,
+                                                ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p, ctypes.c_int,
+                                                ctypes.c_void_p, ctypes.c_int]
+        L.orc_hibf_membership_count_batch.argtypes = [ctypes.POINTER(OrcHibf), ctypes.c_void_p, ctypes.c_void_p,
+                                                      ctypes.c_uint64, ctypes.c_void_p, ctypes.c_void_p,
+                                                      ctypes.c_int]
+        _lib = L
+    return _lib
+
+
+_DT = {1: np.uint8, 2: np.uint16, 4: np.uint32, 8: np.uint64}
+
+
+def _p(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _u64(a) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(a, dtype=np.uint64))
+
+
+def hash_and_fit(h: int, seed: int, hash_shift: int, bin_size: int, technical_bins: int) -> int:
+    return int(lib().orc_hash_and_fit(h, seed, hash_shift, bin_size, technical_bins))
+
+
+def hash_and_fit_py(h: int, seed: int, hash_shift: int, bin_size: int, technical_bins: int) -> int:
+    """Independent big-integer restatement of interleaved_bloom_filter.hpp:185-195."""
+    m = (1 << 64) - 1
+    h = (h * seed) & m
+    h ^= h >> hash_shift
+    h = (h * 11400714819323198485) & m
+    h = (h * bin_size) >> 64
+    return (h * technical_bins) & m
+
+
+class Ibf:
+    """An IBF held on the host: geometry plus its bit matrix as uint64 words."""
+
+    def __init__(self, bins: int, bin_size: int, hash_funs: int = 2):
+        self.g = OrcIbf()
+        rc = lib().orc_ibf_init(ctypes.byref(self.g), bins, bin_size, hash_funs)
+        if rc:
+            raise ValueError(
+                {1: "The number of bins must be > 0.",
+                 2: "The number of hash functions must be > 0 and <= 5.",
+                 3: "The size of a bin must be > 0."}[rc])
+        self.words = np.zeros(self.g.word_count, dtype=np.uint64)
+
+    @classmethod
+    def from_words(cls, bins: int, bin_size: int, hash_funs: int, words: np.ndarray) -> "Ibf":
+        ibf = cls(bins, bin_size, hash_funs)
+        w = _u64(words)
+        assert w.size == ibf.words.size
+        ibf.words = w.copy()
+        return ibf
+
+    def emplace(self, values, bins) -> None:
+        v = _u64(values)
+        b = _u64(np.broadcast_to(np.asarray(bins, dtype=np.uint64), v.shape))
+        lib().orc_ibf_emplace(_p(self.words), ctypes.byref(self.g), _p(v), _p(b), v.size)
+
+    def clear(self, bin: int) -> None:
+        lib().orc_ibf_clear(_p(self.words), ctypes.byref(self.g), bin)
+
+    def bulk_contains(self, value: int) -> np.ndarray:
+        out = np.zeros(self.g.bin_words, dtype=np.uint64)
+        lib().orc_ibf_bulk_contains(_p(self.words), ctypes.byref(self.g), int(value), _p(out))
+        return out
+
+    def contains_bits(self, value: int) -> list[int]:
+        w = self.bulk_contains(value)
+        return [int((int(w[b // 64]) >> (b % 64)) & 1) for b in range(self.g.bins)]
+
+    def bulk_count(self, values, width: int = 2) -> np.ndarray:
+        v = _u64(values)
+        out = np.zeros(self.g.bins, dtype=_DT[width])
+        lib().orc_ibf_bulk_count(_p(self.words), ctypes.byref(self.g), _p(v), v.size, width, _p(out))
+        return out
+
+    def bulk_count_batch(self, values, offsets, width: int = 2, threads: int = 1) -> np.ndarray:
+        v, o = _u64(values), _u64(offsets)
+        nq = o.size - 1
+        out = np.zeros((nq, self.g.bins), dtype=_DT[width])
+        lib().orc_ibf_bulk_count_batch(_p(self.words), ctypes.byref(self.g), _p(v), _p(o), nq, width, _p(out),
+                                       threads)
+        return out
+
+    def membership_for(self, values, threshold: int) -> list[int]:
+        v = _u64(values)
+        out = np.zeros(max(self.g.bins, 1), dtype=np.uint64)
+        n = lib().orc_ibf_membership_for(_p(self.words), ctypes.byref(self.g), _p(v), v.size, threshold, _p(out))
+        return [int(x) for x in out[:n]]
+
+
+def counting_vector_add(counts: np.ndarray, bits: np.ndarray, nbits: int) -> None:
+    b = _u64(bits)
+    lib().orc_counting_vector_add(_p(counts), counts.dtype.itemsize, _p(b), nbits)
+
+
+class Hibf:
+    """An HIBF as flat arrays: IBF geometries, one word arena and CSR tables."""
+
+    def __init__(self, geoms, arena, word_offsets, tb_offsets, next_ibf, user_bin, n_user_bins):
+        self.n_ibfs = len(geoms)
+        self._geoms = (OrcIbf * max(self.n_ibfs, 1))()
+        for i, g in enumerate(geoms):
+            self._geoms[i] = g
+        self.arena = _u64(arena)
+        self.word_offsets = _u64(word_offsets)
+        self.tb_offsets = _u64(tb_offsets)
+        self.next_ibf = _u64(next_ibf)
+        self.user_bin = _u64(user_bin)
+        self.n_user_bins = int(n_user_bins)
+        self.s = OrcHibf(self.n_ibfs, self._geoms,
+                         self.arena.ctypes.data_as(_u64p), self.word_offsets.ctypes.data_as(_u64p),
+                         self.tb_offsets.ctypes.data_as(_u64p), self.next_ibf.ctypes.data_as(_u64p),
+                         self.user_bin.ctypes.data_as(_u64p))
+
+    def membership_for(self, values, threshold: int) -> list[int]:
+        v = _u64(values)
+        cap = max(self.n_user_bins, 1)
+        out = np.zeros(cap, dtype=np.uint64)
+        n = lib().orc_hibf_membership_for(ctypes.byref(self.s), _p(v), v.size, threshold, _p(out), cap)
+        return [int(x) for x in out[:n]]
+
+    def membership_batch(self, values, offsets, thresholds):
+        v, o = _u64(values), _u64(offsets)
+        nq = o.size - 1
+        t = np.ascontiguousarray(np.broadcast_to(np.asarray(thresholds, dtype=np.uint16), (nq,)))
+        cap = 1 << 16
+        while True:
+            ids = np.zeros(cap, dtype=np.uint64)
+            offs = np.zeros(nq + 1, dtype=np.uint64)
+            total = lib().orc_hibf_membership_batch(ctypes.byref(self.s), _p(v), _p(o), nq, _p(t), _p(ids), cap,
+                                                    _p(offs))
+            if total <= cap:
+                return ids[:total], offs
+            cap = int(total)
+
+    def bulk_count(self, values, threshold: int, width: int = 2) -> np.ndarray:
+        v = _u64(values)
+        out = np.zeros(self.n_user_bins, dtype=_DT[width])
+        lib().orc_hibf_bulk_count(ctypes.byref(self.s), self.n_user_bins, _p(v), v.size, threshold, width, _p(out))
+        return out
+
+    def bulk_count_batch(self, values, offsets, thresholds, width: int = 2, threads: int = 1) -> np.ndarray:
+        v, o = _u64(values), _u64(offsets)
+        nq = o.size - 1
+        t = _u64(np.broadcast_to(np.asarray(thresholds, dtype=np.uint64), (nq,)))
+        out = np.zeros((nq, self.n_user_bins), dtype=_DT[width])
+        lib().orc_hibf_bulk_count_batch(ctypes.byref(self.s), self.n_user_bins, _p(v), _p(o), nq, _p(t), width,
+                                        _p(out), threads)
+        return out
+
+    def membership_hits_batch(self, values, offsets, thresholds, threads: int = 1) -> np.ndarray:
+        v, o = _u64(values), _u64(offsets)
+        nq = o.size - 1
+        t = np.ascontiguousarray(np.broadcast_to(np.asarray(thresholds, dtype=np.uint16), (nq,)))
+        hits = np.zeros(nq, dtype=np.uint64)
+        lib().orc_hibf_membership_count_batch(ctypes.byref(self.s), _p(v), _p(o), nq, _p(t), _p(hits), threads)
+        return hits
+
+
+def geom(bins: int, bin_size: int, hash_funs: int) -> OrcIbf:
+    g = OrcIbf()
+    rc = lib().orc_ibf_init(ctypes.byref(g), bins, bin_size, hash_funs)
+    if rc:
+        raise ValueError(f"invalid IBF geometry (code {rc})")
+    return g

@@ -1,0 +1,469 @@
+/*
+ * Oracle for the HIBF query path: a plain-C restatement of the reference
+ * algorithm.  TEST INFRASTRUCTURE ONLY.  It is linked by tests/, by the smoke
+ * check and by bench.py's CPU-baseline leg, always as the checker and never
+ * as the thing that is measured or shipped.  The product (hibf_amd/) never
+ * links it.
+ *
+ * Each function cites the reference lines it restates (paths relative to the
+ * seqan/hibf 1.0.0-rc.1 source tree).
+ *
+ * Pinning (what checks each function beyond retyping its formula):
+ *   orc_hash_and_fit       Python big-integer restatement + committed vectors
+ *                          + closed-form special cases (tests/test_oracle_hash.py).
+ *   orc_ibf_init/emplace   snippet goldens (counting_agent.out, ...), the
+ *                          insert_iterator unit test counts, brute force
+ *                          "h bits set in column b only".
+ *   orc_ibf_bulk_contains  containment_agent_bulk_contains.out, invariants of
+ *                          interleaved_bloom_filter_test.cpp (no false negatives).
+ *   orc_ibf_bulk_count     counting_agent.out (uint16 and uint8), counting_vector.out,
+ *                          counting_vector_test.cpp wrap-around cases.
+ *   orc_ibf_membership_for membership_agent_membership_for.out ([3,5,7]).
+ *   orc_hibf_membership_for / orc_hibf_bulk_count
+ *                          hierarchical_interleaved_bloom_filter.out, readme.out,
+ *                          the reference HIBF unit-test expectations, and a
+ *                          brute-force tree check (a user bin is reported iff its
+ *                          run sum and every ancestor merged-bin sum reach t).
+ */
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* interleaved_bloom_filter.hpp:172-176 */
+static const uint64_t ORC_HASH_SEEDS[5] = {13572355802537770549ULL, 13043817825332782213ULL,
+                                           10650232656628343401ULL, 16499269484942379435ULL,
+                                           4893150838803335377ULL};
+
+/* hierarchical_interleaved_bloom_filter.hpp:32-40 */
+#define ORC_BIN_MERGED 0xFFFFFFFFFFFFFFFFULL
+#define ORC_BIN_DELETED 0xFFFFFFFFFFFFFFFEULL
+
+/* Geometry of one IBF: interleaved_bloom_filter.hpp:160-170. */
+typedef struct orc_ibf
+{
+    uint64_t bins;
+    uint64_t technical_bins;
+    uint64_t bin_size;
+    uint64_t hash_shift;
+    uint64_t bin_words;
+    uint64_t hash_funs;
+} orc_ibf;
+
+uint64_t orc_hash_seed(uint32_t i) { return i < 5 ? ORC_HASH_SEEDS[i] : 0; }
+
+/* misc/divide_and_ceil.hpp:20-26 */
+static uint64_t orc_divide_and_ceil(uint64_t a, uint64_t b) { return (a + (b - 1u)) / b; }
+
+static uint64_t orc_countl_zero(uint64_t x)
+{
+    uint64_t n = 0;
+    if (x == 0)
+        return 64;
+    while (!(x & 0x8000000000000000ULL))
+    {
+        x <<= 1;
+        ++n;
+    }
+    return n;
+}
+
+/* src/interleaved_bloom_filter.cpp:32-53.  Returns 0, or 1/2/3 for the three
+ * logic_error cases in the order the reference checks them. */
+int orc_ibf_init(orc_ibf * g, uint64_t bins, uint64_t bin_size, uint64_t hash_funs)
+{
+    if (bins == 0)
+        return 1;
+    if (hash_funs == 0 || hash_funs > 5)
+        return 2;
+    if (bin_size == 0)
+        return 3;
+    g->bins = bins;
+    g->bin_size = bin_size;
+    g->hash_funs = hash_funs;
+    g->hash_shift = orc_countl_zero(bin_size);
+    g->bin_words = orc_divide_and_ceil(bins, 64u);
+    g->technical_bins = g->bin_words * 64u;
+    return 0;
+}
+
+uint64_t orc_ibf_word_count(orc_ibf const * g) { return g->technical_bins * g->bin_size / 64u; }
+
+/* interleaved_bloom_filter.hpp:185-195: seeded multiply, xor-shift, golden-ratio
+ * multiply, fastrange (high half of the 128-bit product), times technical_bins. */
+uint64_t orc_hash_and_fit(uint64_t h, uint64_t seed, uint64_t hash_shift, uint64_t bin_size, uint64_t technical_bins)
+{
+    h *= seed;
+    h ^= h >> hash_shift;
+    h *= 11400714819323198485ULL;
+    h = (uint64_t)(((unsigned __int128)h * (unsigned __int128)bin_size) >> 64);
+    h *= technical_bins;
+    return h;
+}
+
+/* src/interleaved_bloom_filter.cpp:119-140 with track_occupancy == false:
+ * set bit hash_and_fit(v, seed_i) + bin for every i < h.  Bits are LSB-first
+ * inside 64-bit words (bit_vector.hpp:1045-1054). */
+void orc_ibf_emplace(uint64_t * words, orc_ibf const * g, uint64_t const * values, uint64_t const * bins, uint64_t n)
+{
+    for (uint64_t k = 0; k < n; ++k)
+    {
+        for (uint64_t i = 0; i < g->hash_funs; ++i)
+        {
+            uint64_t idx = orc_hash_and_fit(values[k], ORC_HASH_SEEDS[i], g->hash_shift, g->bin_size, g->technical_bins)
+                         + bins[k];
+            words[idx >> 6] |= 1ULL << (idx & 63u);
+        }
+    }
+}
+
+/* src/interleaved_bloom_filter.cpp:142-148: clear(bin) zeroes the bin's column. */
+void orc_ibf_clear(uint64_t * words, orc_ibf const * g, uint64_t bin)
+{
+    for (uint64_t idx = bin, i = 0; i < g->bin_size; idx += g->technical_bins, ++i)
+        words[idx >> 6] &= ~(1ULL << (idx & 63u));
+}
+
+/* src/interleaved_bloom_filter.cpp:204-291: copy row 0, AND rows 1..h-1.
+ * out receives bin_words words. */
+void orc_ibf_bulk_contains(uint64_t const * words, orc_ibf const * g, uint64_t value, uint64_t * out)
+{
+    uint64_t idx[5];
+    for (uint64_t i = 0; i < g->hash_funs; ++i)
+        idx[i] = orc_hash_and_fit(value, ORC_HASH_SEEDS[i], g->hash_shift, g->bin_size, g->technical_bins) / 64u;
+    memcpy(out, words + idx[0], sizeof(uint64_t) * g->bin_words);
+    for (uint64_t i = 1; i < g->hash_funs; ++i)
+        for (uint64_t j = 0; j < g->bin_words; ++j)
+            out[j] &= words[idx[i] + j];
+}
+
+/* Store/load a counter of `width` bytes (1, 2, 4 or 8), wrapping modulo
+ * 2^(8*width) like counting_vector<uintN_t>. */
+static uint64_t orc_width_mask(int width) { return width >= 8 ? ~0ULL : ((1ULL << (8 * width)) - 1u); }
+
+static uint64_t orc_load(void const * p, uint64_t i, int width)
+{
+    switch (width)
+    {
+    case 1: return ((uint8_t const *)p)[i];
+    case 2: return ((uint16_t const *)p)[i];
+    case 4: return ((uint32_t const *)p)[i];
+    default: return ((uint64_t const *)p)[i];
+    }
+}
+
+static void orc_store(void * p, uint64_t i, int width, uint64_t v)
+{
+    switch (width)
+    {
+    case 1: ((uint8_t *)p)[i] = (uint8_t)v; break;
+    case 2: ((uint16_t *)p)[i] = (uint16_t)v; break;
+    case 4: ((uint32_t *)p)[i] = (uint32_t)v; break;
+    default: ((uint64_t *)p)[i] = v; break;
+    }
+}
+
+/* counting_vector.hpp:171-175 and the scalar impl :279-311: for each set bit
+ * b of the bit vector (b < nbits), counts[b] += 1 modulo 2^(8*width). */
+void orc_counting_vector_add(void * counts, int width, uint64_t const * bits, uint64_t nbits)
+{
+    uint64_t words = orc_divide_and_ceil(nbits, 64u);
+    for (uint64_t w = 0; w < words; ++w)
+    {
+        uint64_t x = bits[w];
+        while (x)
+        {
+            uint64_t b = w * 64u + (uint64_t)__builtin_ctzll(x);
+            x &= x - 1u;
+            if (b < nbits)
+                orc_store(counts, b, width, orc_load(counts, b, width) + 1u);
+        }
+    }
+}
+
+/* interleaved_bloom_filter.hpp:652-665: zero the counters, then add
+ * bulk_contains(v) for every value.  counts holds g->bins counters. */
+void orc_ibf_bulk_count(uint64_t const * words,
+                        orc_ibf const * g,
+                        uint64_t const * values,
+                        uint64_t n,
+                        int width,
+                        void * counts)
+{
+    uint64_t row[4096];
+    uint64_t * buf = g->bin_words <= 4096 ? row : (uint64_t *)malloc(8u * g->bin_words);
+    memset(counts, 0, (size_t)g->bins * (size_t)width);
+    for (uint64_t k = 0; k < n; ++k)
+    {
+        orc_ibf_bulk_contains(words, g, values[k], buf);
+        orc_counting_vector_add(counts, width, buf, g->bins);
+    }
+    if (buf != row)
+        free(buf);
+}
+
+/* Batched form over CSR queries (query q = values[offsets[q] .. offsets[q+1])),
+ * one agent per thread with static scheduling as the reference documents for
+ * multi-threaded callers (interleaved_bloom_filter.hpp:547-551).
+ * counts is nq x g->bins counters of `width` bytes. */
+void orc_ibf_bulk_count_batch(uint64_t const * words,
+                              orc_ibf const * g,
+                              uint64_t const * values,
+                              uint64_t const * offsets,
+                              uint64_t nq,
+                              int width,
+                              void * counts,
+                              int threads)
+{
+    (void)threads;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static) num_threads(threads > 0 ? threads : 1)
+#endif
+    for (int64_t q = 0; q < (int64_t)nq; ++q)
+        orc_ibf_bulk_count(words,
+                           g,
+                           values + offsets[q],
+                           offsets[q + 1] - offsets[q],
+                           width,
+                           (char *)counts + (size_t)q * (size_t)g->bins * (size_t)width);
+}
+
+/* interleaved_bloom_filter.hpp:744-764: uint16 counts, then every bin with
+ * count >= threshold in ascending order.  Returns the number of bins written. */
+uint64_t orc_ibf_membership_for(uint64_t const * words,
+                                orc_ibf const * g,
+                                uint64_t const * values,
+                                uint64_t n,
+                                uint16_t threshold,
+                                uint64_t * out)
+{
+    uint16_t * counts = (uint16_t *)malloc(sizeof(uint16_t) * (g->bins ? g->bins : 1));
+    uint64_t found = 0;
+    orc_ibf_bulk_count(words, g, values, n, 2, counts);
+    for (uint64_t b = 0; b < g->bins; ++b)
+        if (counts[b] >= threshold)
+            out[found++] = b;
+    free(counts);
+    return found;
+}
+
+/* An HIBF as flat arrays: per-IBF geometry and word offset into one arena;
+ * per-IBF technical-bin tables in CSR form (tb_offsets[i] .. tb_offsets[i+1]),
+ * holding next_ibf_id and ibf_bin_to_user_bin_id
+ * (hierarchical_interleaved_bloom_filter.hpp:209-238). */
+typedef struct orc_hibf
+{
+    uint64_t n_ibfs;
+    orc_ibf const * ibfs;
+    uint64_t const * arena;
+    uint64_t const * word_offsets;
+    uint64_t const * tb_offsets;
+    uint64_t const * next_ibf;
+    uint64_t const * user_bin;
+} orc_hibf;
+
+typedef struct orc_out
+{
+    uint64_t * ids;
+    uint64_t n;
+    uint64_t cap;
+} orc_out;
+
+/* hierarchical_interleaved_bloom_filter.hpp:321-355 (membership_for_impl):
+ * pre-order DFS, uint16 per-bin counts and a uint16 running sum.  Merged bin:
+ * recurse iff sum >= t, then reset.  Regular bin at the end of its run (last
+ * bin, or the next bin has another user-bin id): report iff sum >= t, then
+ * reset.  Deleted bin: neither report nor reset. */
+static void orc_hibf_membership_impl(orc_hibf const * hb,
+                                     uint64_t const * values,
+                                     uint64_t n,
+                                     uint64_t ibf_idx,
+                                     uint16_t threshold,
+                                     orc_out * out)
+{
+    orc_ibf const * g = &hb->ibfs[ibf_idx];
+    uint16_t * counts = (uint16_t *)malloc(sizeof(uint16_t) * (g->bins ? g->bins : 1));
+    uint64_t const * ub = hb->user_bin + hb->tb_offsets[ibf_idx];
+    uint64_t const * nx = hb->next_ibf + hb->tb_offsets[ibf_idx];
+    uint16_t sum = 0;
+    orc_ibf_bulk_count(hb->arena + hb->word_offsets[ibf_idx], g, values, n, 2, counts);
+    for (uint64_t bin = 0; bin < g->bins; ++bin)
+    {
+        sum = (uint16_t)(sum + counts[bin]);
+        uint64_t const id = ub[bin];
+        if (id == ORC_BIN_DELETED)
+            continue;
+        if (id == ORC_BIN_MERGED)
+        {
+            if (sum >= threshold)
+                orc_hibf_membership_impl(hb, values, n, nx[bin], threshold, out);
+            sum = 0;
+            continue;
+        }
+        if (bin + 1u == g->bins || id != ub[bin + 1u])
+        {
+            if (sum >= threshold)
+            {
+                if (out->n < out->cap)
+                    out->ids[out->n] = id;
+                ++out->n;
+            }
+            sum = 0;
+        }
+    }
+    free(counts);
+}
+
+/* hierarchical_interleaved_bloom_filter.hpp:424-439.  Writes at most cap ids
+ * in the reference's (unsorted, DFS) order and returns how many there are. */
+uint64_t orc_hibf_membership_for(orc_hibf const * hb,
+                                 uint64_t const * values,
+                                 uint64_t n,
+                                 uint16_t threshold,
+                                 uint64_t * out,
+                                 uint64_t cap)
+{
+    orc_out o = {out, 0, cap};
+    orc_hibf_membership_impl(hb, values, n, 0, threshold, &o);
+    return o.n;
+}
+
+/* Batched membership over CSR queries.  out_offsets[q] .. out_offsets[q+1]
+ * index the ids of query q; out_cap bounds the total.  Returns the total
+ * number of ids (which may exceed out_cap, in which case ids are truncated). */
+uint64_t orc_hibf_membership_batch(orc_hibf const * hb,
+                                   uint64_t const * values,
+                                   uint64_t const * offsets,
+                                   uint64_t nq,
+                                   uint16_t const * thresholds,
+                                   uint64_t * out_ids,
+                                   uint64_t out_cap,
+                                   uint64_t * out_offsets)
+{
+    uint64_t total = 0;
+    out_offsets[0] = 0;
+    for (uint64_t q = 0; q < nq; ++q)
+    {
+        uint64_t cap = total < out_cap ? out_cap - total : 0;
+        uint64_t got = orc_hibf_membership_for(hb,
+                                               values + offsets[q],
+                                               offsets[q + 1] - offsets[q],
+                                               thresholds[q],
+                                               out_ids + (total < out_cap ? total : out_cap),
+                                               cap);
+        total += got;
+        out_offsets[q + 1] = total;
+    }
+    return total;
+}
+
+/* hierarchical_interleaved_bloom_filter.hpp:462-496 (bulk_count_impl): the same
+ * descent with value_t (width bytes, unsigned) counts and sum and a size_t
+ * threshold; counts[ub] = sum iff sum >= threshold. */
+static void orc_hibf_count_impl(orc_hibf const * hb,
+                                uint64_t const * values,
+                                uint64_t n,
+                                uint64_t ibf_idx,
+                                uint64_t threshold,
+                                int width,
+                                void * result)
+{
+    orc_ibf const * g = &hb->ibfs[ibf_idx];
+    void * counts = malloc((size_t)width * (g->bins ? g->bins : 1));
+    uint64_t const * ub = hb->user_bin + hb->tb_offsets[ibf_idx];
+    uint64_t const * nx = hb->next_ibf + hb->tb_offsets[ibf_idx];
+    uint64_t const mask = orc_width_mask(width);
+    uint64_t sum = 0;
+    orc_ibf_bulk_count(hb->arena + hb->word_offsets[ibf_idx], g, values, n, width, counts);
+    for (uint64_t bin = 0; bin < g->bins; ++bin)
+    {
+        sum = (sum + orc_load(counts, bin, width)) & mask;
+        uint64_t const id = ub[bin];
+        if (id == ORC_BIN_DELETED)
+            continue;
+        if (id == ORC_BIN_MERGED)
+        {
+            if (sum >= threshold)
+                orc_hibf_count_impl(hb, values, n, nx[bin], threshold, width, result);
+            sum = 0;
+            continue;
+        }
+        if (bin + 1u == g->bins || id != ub[bin + 1u])
+        {
+            if (sum >= threshold)
+                orc_store(result, id, width, sum);
+            sum = 0;
+        }
+    }
+    free(counts);
+}
+
+/* hierarchical_interleaved_bloom_filter.hpp:549-566: zero n_user_bins counters,
+ * then descend from IBF 0. */
+void orc_hibf_bulk_count(orc_hibf const * hb,
+                         uint64_t n_user_bins,
+                         uint64_t const * values,
+                         uint64_t n,
+                         uint64_t threshold,
+                         int width,
+                         void * result)
+{
+    memset(result, 0, (size_t)n_user_bins * (size_t)width);
+    orc_hibf_count_impl(hb, values, n, 0, threshold, width, result);
+}
+
+void orc_hibf_bulk_count_batch(orc_hibf const * hb,
+                               uint64_t n_user_bins,
+                               uint64_t const * values,
+                               uint64_t const * offsets,
+                               uint64_t nq,
+                               uint64_t const * thresholds,
+                               int width,
+                               void * result,
+                               int threads)
+{
+    (void)threads;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static) num_threads(threads > 0 ? threads : 1)
+#endif
+    for (int64_t q = 0; q < (int64_t)nq; ++q)
+        orc_hibf_bulk_count(hb,
+                            n_user_bins,
+                            values + offsets[q],
+                            offsets[q + 1] - offsets[q],
+                            thresholds[q],
+                            width,
+                            (char *)result + (size_t)q * (size_t)n_user_bins * (size_t)width);
+}
+
+/* Multi-threaded membership for the CPU baseline: per-thread agents over a
+ * static partition of the queries; only the per-query hit counts are kept. */
+void orc_hibf_membership_count_batch(orc_hibf const * hb,
+                                     uint64_t const * values,
+                                     uint64_t const * offsets,
+                                     uint64_t nq,
+                                     uint16_t const * thresholds,
+                                     uint64_t * hits_per_query,
+                                     int threads)
+{
+    (void)threads;
+#ifdef _OPENMP
+#pragma omp parallel num_threads(threads > 0 ? threads : 1)
+#endif
+    {
+        uint64_t scratch[1];
+#ifdef _OPENMP
+#pragma omp for schedule(static)
+#endif
+        for (int64_t q = 0; q < (int64_t)nq; ++q)
+            hits_per_query[q] = orc_hibf_membership_for(hb,
+                                                        values + offsets[q],
+                                                        offsets[q + 1] - offsets[q],
+                                                        thresholds[q],
+                                                        scratch,
+                                                        0);
+    }
+}

@@ -1,0 +1,165 @@
+"""GPU parity of the flat IBF path (emplace, clear, bulk_contains, bulk_count,
+membership_for) against the oracle, bit-exact."""
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+GEOMS = [  # (bins, bin_size, hash_funs)
+    (1, 64, 1), (7, 1000, 2), (63, 513, 2), (64, 4096, 3), (65, 777, 5), (127, 1024, 2), (130, 2000, 2),
+    (1000, 300, 4), (4096, 3000, 2), (8192, 2048, 3), (9000, 500, 2), (20000, 129, 1),
+]
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def _rand_u64(rng, n):
+    return rng.integers(0, 2**63, size=n, dtype=np.int64).astype(np.uint64) * np.uint64(2) + \
+        rng.integers(0, 2, size=n).astype(np.uint64)
+
+
+def _pair(bins, bin_size, h, n, seed):
+    from hibf_amd.device import DeviceIbf
+    rng = np.random.default_rng(seed)
+    vals = _rand_u64(rng, n)
+    bidx = rng.integers(0, bins, size=n).astype(np.uint64)
+    ref = O.Ibf(bins, bin_size, h)
+    ref.emplace(vals, bidx)
+    dev = DeviceIbf(bins, bin_size, h)
+    dev.emplace(vals, bidx)
+    return ref, dev, vals, bidx, rng
+
+
+def _queries(rng, vals, lengths, frac_in=0.7):
+    qs = []
+    for L in lengths:
+        k = int(L * frac_in)
+        q = np.concatenate([vals[rng.integers(0, len(vals), size=k)], _rand_u64(rng, L - k)])
+        qs.append(q)
+    return qs
+
+
+@pytest.mark.parametrize("bins,bin_size,h", GEOMS)
+def test_emplace_matrix_identical(bins, bin_size, h):
+    _say(f"emplace bins={bins} bin_size={bin_size} h={h}")
+    ref, dev, *_ = _pair(bins, bin_size, h, 20000, bins)
+    assert np.array_equal(dev.download(), ref.words)
+
+
+def test_emplace_single_bin_and_geometry():
+    from hibf_amd.device import DeviceIbf
+    _say("emplace constant bin")
+    ref = O.Ibf(73, 1019, 3)
+    dev = DeviceIbf(73, 1019, 3)
+    assert (dev.bin_count(), dev.bin_size(), dev.hash_function_count(), dev.bit_size()) == (73, 1019, 3, 130_432)
+    vals = np.arange(5000, dtype=np.uint64)
+    ref.emplace(vals, 72)
+    dev.emplace(vals, 72)
+    assert np.array_equal(dev.download(), ref.words)
+
+
+@pytest.mark.parametrize("bins,bin_size,h", GEOMS[:9])
+def test_bulk_contains(bins, bin_size, h):
+    from hibf_amd.device import to_numpy
+    _say(f"contains bins={bins} bin_size={bin_size} h={h}")
+    ref, dev, vals, _, rng = _pair(bins, bin_size, h, 5000, bins + 1)
+    q = np.concatenate([vals[:500], _rand_u64(rng, 500)])
+    got = to_numpy(dev.bulk_contains(q))
+    want = np.stack([ref.bulk_contains(int(v)) for v in q])
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("bins,bin_size,h", GEOMS)
+@pytest.mark.parametrize("width", [1, 2, 4, 8])
+def test_bulk_count(bins, bin_size, h, width):
+    from hibf_amd.device import csr, to_numpy
+    _say(f"count bins={bins} bin_size={bin_size} h={h} width={width}")
+    ref, dev, vals, _, rng = _pair(bins, bin_size, h, 30000, bins + 2)
+    lengths = [0, 1, 2, 63, 64, 65, 127, 250, 300, 1, 0, 700]
+    qs = _queries(rng, vals, lengths)
+    values, offsets = csr(qs)
+    got = to_numpy(dev.bulk_count(values, offsets, width))
+    want = ref.bulk_count_batch(values, offsets, width)
+    assert got.shape == want.shape
+    assert np.array_equal(got, want)
+
+
+def test_bulk_count_wraps_like_counting_vector():
+    from hibf_amd.device import DeviceIbf, csr, to_numpy
+    _say("count wrap uint8/uint16")
+    dev = DeviceIbf(3, 4096, 2)
+    dev.emplace([42], 1)
+    values, offsets = csr([np.full(300, 42, dtype=np.uint64), np.full(70000, 42, dtype=np.uint64)])
+    assert to_numpy(dev.bulk_count(values, offsets, 1)).tolist() == [[0, 300 % 256, 0], [0, 70000 % 256, 0]]
+    assert to_numpy(dev.bulk_count(values, offsets, 2)).tolist() == [[0, 300, 0], [0, 70000 % 65536, 0]]
+    assert to_numpy(dev.bulk_count(values, offsets, 8)).tolist() == [[0, 300, 0], [0, 70000, 0]]
+
+
+@pytest.mark.parametrize("bins,bin_size,h", [(12, 8192, 2), (200, 3000, 3), (8192, 1024, 2), (9000, 300, 2)])
+def test_membership_for(bins, bin_size, h):
+    from hibf_amd.device import csr, to_numpy
+    _say(f"membership bins={bins}")
+    ref, dev, vals, _, rng = _pair(bins, bin_size, h, 40000, bins + 3)
+    qs = _queries(rng, vals, [0, 5, 40, 100, 250, 33])
+    values, offsets = csr(qs)
+    thresholds = [0, 1, 2, 3, 4, 1]
+    ids, hoff = dev.membership_for(values, offsets, thresholds)
+    ids, hoff = to_numpy(ids), to_numpy(hoff)
+    for i, (q, t) in enumerate(zip(qs, thresholds)):
+        assert ids[hoff[i]:hoff[i + 1]].tolist() == ref.membership_for(q, t)
+
+
+def test_clear_matches_oracle():
+    from hibf_amd.device import csr, to_numpy
+    _say("clear")
+    ref, dev, vals, _, rng = _pair(150, 2000, 2, 20000, 99)
+    for bins in ([17], [0, 63, 64, 149]):
+        ref_words = ref.words
+        for b in bins:
+            ref.clear(b)
+        dev.clear(bins)
+        assert np.array_equal(dev.download(), ref.words)
+    values, offsets = csr(_queries(rng, vals, [100, 200]))
+    assert np.array_equal(to_numpy(dev.bulk_count(values, offsets)), ref.bulk_count_batch(values, offsets))
+
+
+def test_snippet_goldens_on_gpu(goldens):
+    from hibf_amd.device import DeviceIbf, csr, to_numpy
+    _say("snippet goldens")
+
+    def vals(spec):
+        return np.arange(*spec["range"], dtype=np.uint64) if "range" in spec else np.asarray(spec["values"],
+                                                                                             dtype=np.uint64)
+
+    def build(case):
+        d = DeviceIbf(case["bins"], case["bin_size"], case["hash_funs"])
+        for e in case["emplace"]:
+            d.emplace(vals(e), e["bin"])
+        return d
+
+    case = goldens["ibf_counting_agent"]
+    d = build(case)
+    for q in case["queries"]:
+        v, o = csr([vals(q)])
+        assert to_numpy(d.bulk_count(v, o, q["width"]))[0].tolist() == q["expected"]
+    case = goldens["ibf_clear"]
+    d = build(case)
+    for step in case["steps"]:
+        if step["clear"]:
+            d.clear(step["clear"])
+        v, o = csr([vals(step)])
+        assert to_numpy(d.bulk_count(v, o))[0].tolist() == step["expected"]
+    case = goldens["ibf_containment"]
+    w = to_numpy(build(case).bulk_contains([case["value"]]))[0]
+    assert [int((int(w[b // 64]) >> (b % 64)) & 1) for b in range(case["bins"])] == case["expected"]
+    case = goldens["ibf_membership"]
+    v, o = csr([case["query"]])
+    ids, hoff = build(case).membership_for(v, o, [case["threshold"]])
+    assert to_numpy(ids).tolist() == case["expected"]
+    case = goldens["insert_iterator_ibf"]
+    v, o = csr([np.arange(*case["query_range"], dtype=np.uint64)])
+    assert to_numpy(build(case).bulk_count(v, o, 1))[0].tolist() == case["expected"]

@@ -1,0 +1,308 @@
+"""Pins the oracle (CPU restatement of the reference query path) against the
+reference's own known answers and against what the mathematics fixes."""
+import json
+import os
+import random
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _values(spec):
+    if "range" in spec:
+        return np.arange(spec["range"][0], spec["range"][1], dtype=np.uint64)
+    return np.asarray(spec["values"], dtype=np.uint64)
+
+
+def _build(case):
+    ibf = O.Ibf(case["bins"], case["bin_size"], case["hash_funs"])
+    for e in case["emplace"]:
+        ibf.emplace(_values(e), e["bin"])
+    return ibf
+
+
+# ---------------------------------------------------------------- hashing --
+
+def test_hash_known_answers_match_c_oracle():
+    with open(os.path.join(GOLDEN, "hash_vectors.json")) as f:
+        cases = json.load(f)["cases"]
+    assert len(cases) > 1000
+    for v, k, bs, tb, want in cases:
+        v, k, bs, tb, want = int(v), int(k), int(bs), int(tb), int(want)
+        shift = 64 - bs.bit_length()
+        assert O.hash_and_fit(v, O.HASH_SEEDS[k], shift, bs, tb) == want
+
+
+def test_hash_c_matches_python_bigint_random():
+    rng = random.Random(7)
+    for _ in range(3000):
+        bs = rng.choice([1, 2, 3, 1019, 8192, 1 << 23, rng.getrandbits(40) + 1, rng.getrandbits(63) | 1])
+        tb = 64 * rng.randint(1, 200)
+        v = rng.getrandbits(64)
+        k = rng.randrange(5)
+        shift = 64 - bs.bit_length()
+        assert O.hash_and_fit(v, O.HASH_SEEDS[k], shift, bs, tb) == O.hash_and_fit_py(v, O.HASH_SEEDS[k], shift, bs, tb)
+
+
+def test_hash_closed_form_properties():
+    rng = random.Random(11)
+    for k in range(5):
+        for e in range(0, 40):
+            bs = 1 << e
+            g = O.geom(64, bs, 5)
+            assert g.hash_shift == 63 - e  # countl_zero(2^e)
+            for _ in range(50):
+                v = rng.getrandbits(64)
+                pos = O.hash_and_fit(v, O.HASH_SEEDS[k], g.hash_shift, bs, g.technical_bins)
+                assert pos % g.technical_bins == 0          # a row start
+                assert pos // g.technical_bins < bs          # fastrange stays in range
+    # bin_size 1: every value maps to row 0
+    g = O.geom(10, 1, 2)
+    assert all(O.hash_and_fit(rng.getrandbits(64), s, g.hash_shift, 1, g.technical_bins) == 0 for s in O.HASH_SEEDS)
+
+
+def test_geometry_and_errors():
+    g = O.geom(73, 1019, 3)  # interleaved_bloom_filter_test.cpp:134-143
+    assert (g.bins, g.bin_size, g.hash_funs, g.technical_bins) == (73, 1019, 3, 128)
+    assert g.technical_bins * g.bin_size == 130_432
+    g = O.geom(64, 1024, 2)
+    assert g.technical_bins * g.bin_size == 65_536
+    with pytest.raises(ValueError, match="bins must be > 0"):
+        O.Ibf(0, 32, 2)
+    with pytest.raises(ValueError, match="hash functions"):
+        O.Ibf(64, 32, 0)
+    with pytest.raises(ValueError, match="hash functions"):
+        O.Ibf(64, 32, 6)
+    with pytest.raises(ValueError, match="size of a bin"):
+        O.Ibf(64, 0, 2)
+
+
+# ----------------------------------------------------------- IBF goldens --
+
+def test_snippet_counting_agent(goldens):
+    case = goldens["ibf_counting_agent"]
+    ibf = _build(case)
+    for q in case["queries"]:
+        assert ibf.bulk_count(_values(q), q["width"]).tolist() == q["expected"]
+
+
+def test_snippet_clear(goldens):
+    case = goldens["ibf_clear"]
+    ibf = _build(case)
+    for step in case["steps"]:
+        for b in step["clear"]:
+            ibf.clear(b)
+        assert ibf.bulk_count(_values(step)).tolist() == step["expected"]
+
+
+def test_snippet_containment(goldens):
+    case = goldens["ibf_containment"]
+    assert _build(case).contains_bits(case["value"]) == case["expected"]
+
+
+def test_snippet_counting_vector(goldens):
+    case = goldens["ibf_counting_vector"]
+    ibf = _build(case)
+    counts = np.zeros(12, dtype=np.uint16)
+    for v, want in zip(case["adds"], case["expected"]):
+        O.counting_vector_add(counts, ibf.bulk_contains(v), ibf.g.bins)
+        assert counts.tolist() == want
+    assert (counts + counts).tolist() == case["doubled"]
+
+
+def test_snippet_membership(goldens):
+    case = goldens["ibf_membership"]
+    ibf = _build(case)
+    assert ibf.membership_for(case["query"], case["threshold"]) == case["expected"]
+
+
+def test_insert_iterator_counts(goldens):
+    case = goldens["insert_iterator_ibf"]
+    ibf = _build(case)
+    assert ibf.bulk_count(np.arange(*case["query_range"], dtype=np.uint64), 1).tolist() == case["expected"]
+
+
+# ----------------------------------------------- IBF unit-test invariants --
+
+def test_ibf_unit_expectations():
+    # interleaved_bloom_filter_test.cpp:145-207: empty IBF -> no bits; full emplace -> all bits
+    ibf = O.Ibf(64, 1024, 2)
+    assert all(not any(ibf.contains_bits(v)) for v in range(64))
+    for b in range(64):
+        ibf.emplace(np.arange(64, dtype=np.uint64), b)
+    assert all(all(ibf.contains_bits(v)) for v in range(64))
+    ibf.clear(17)  # :233-259
+    assert all(ibf.contains_bits(v) == [1] * 17 + [0] + [1] * 46 for v in range(64))
+    # :296-331 counting: 128 bins x 128 values
+    ibf = O.Ibf(128, 1024, 2)
+    for b in range(128):
+        ibf.emplace(np.arange(128, dtype=np.uint64), b)
+    assert ibf.bulk_count(np.arange(128, dtype=np.uint64), 8).tolist() == [128] * 128
+    # :356-407: only bins 63 and 126 of 127, size not divisible by 64
+    ibf = O.Ibf(127, 1024, 2)
+    for b in (63, 126):
+        ibf.emplace(np.arange(128, dtype=np.uint64), b)
+    want = [0] * 127
+    want[63] = want[126] = 128
+    assert ibf.bulk_count(np.arange(128, dtype=np.uint64)).tolist() == want
+    assert ibf.bulk_count(np.arange(128, dtype=np.uint64), 8).tolist() == want
+
+
+@pytest.mark.parametrize("width", [1, 2, 4, 8])
+def test_counting_vector_wraps(width):
+    # counting_vector_test.cpp:90-107 / :185-209: all bits set on max-1 -> max, then wrap to 0
+    nbits = 2033
+    dt = {1: np.uint8, 2: np.uint16, 4: np.uint32, 8: np.uint64}[width]
+    mx = np.iinfo(dt).max
+    counts = np.full(nbits, mx - 1, dtype=dt)
+    bits = np.full((nbits + 63) // 64, np.uint64(0xFFFFFFFFFFFFFFFF), dtype=np.uint64)
+    O.counting_vector_add(counts, bits, nbits)
+    assert (counts == mx).all()
+    O.counting_vector_add(counts, bits, nbits)
+    assert (counts == 0).all()
+    # every other bit
+    counts[:] = 0
+    bits[:] = np.uint64(0xAAAAAAAAAAAAAAAA)
+    O.counting_vector_add(counts, bits, nbits)
+    assert counts.tolist() == [i % 2 for i in range(nbits)]
+
+
+def test_bulk_count_wraps_uint8():
+    ibf = O.Ibf(3, 4096, 2)
+    ibf.emplace([42], 1)
+    c = ibf.bulk_count(np.full(300, 42, dtype=np.uint64), 1)
+    assert c.tolist() == [0, 300 % 256, 0]
+    assert ibf.bulk_count(np.full(300, 42, dtype=np.uint64), 2).tolist() == [0, 300, 0]
+
+
+def _textbook_counts(bins, bin_size, h, inserts, query, width):
+    """Independent Bloom-filter restatement: per-bin sets of set positions."""
+    tbs = 64 * ((bins + 63) // 64)
+    shift = 64 - bin_size.bit_length()
+    sets = [set() for _ in range(bins)]
+    for v, b in inserts:
+        for k in range(h):
+            sets[b].add(O.hash_and_fit_py(v, O.HASH_SEEDS[k], shift, bin_size, tbs) // tbs)
+    out = []
+    for b in range(bins):
+        c = sum(all(O.hash_and_fit_py(v, O.HASH_SEEDS[k], shift, bin_size, tbs) // tbs in sets[b] for k in range(h))
+                for v in query)
+        out.append(c % (1 << (8 * width)))
+    return out
+
+
+@pytest.mark.parametrize("bins,bin_size,h", [(5, 97, 1), (64, 256, 2), (70, 33, 3), (130, 64, 5)])
+def test_counts_match_textbook_bloom_filter(bins, bin_size, h):
+    rng = random.Random(bins * 1000 + h)
+    inserts = [(rng.getrandbits(64), rng.randrange(bins)) for _ in range(300)]
+    query = [v for v, _ in inserts[:60]] + [rng.getrandbits(64) for _ in range(40)]
+    ibf = O.Ibf(bins, bin_size, h)
+    ibf.emplace([v for v, _ in inserts], [b for _, b in inserts])
+    assert ibf.bulk_count(query, 2).tolist() == _textbook_counts(bins, bin_size, h, inserts, query, 2)
+
+
+def test_emplace_sets_h_bits_in_its_column_only():
+    rng = np.random.default_rng(3)
+    for bins, bs, h in [(1, 64, 1), (65, 129, 3), (200, 50, 5)]:
+        for _ in range(20):
+            v = int(rng.integers(0, 2**63))
+            b = int(rng.integers(0, bins))
+            ibf = O.Ibf(bins, bs, h)
+            ibf.emplace([v], b)
+            words = ibf.words
+            nz = np.flatnonzero(words)
+            total = sum(bin(int(w)).count("1") for w in words[nz])
+            assert 1 <= total <= h
+            for wi in nz:
+                w = int(words[wi])
+                for bit in range(64):
+                    if (w >> bit) & 1:
+                        assert (wi % ibf.g.bin_words) * 64 + bit == b
+
+
+def test_no_false_negatives():
+    rng = np.random.default_rng(5)
+    ibf = O.Ibf(100, 4096, 3)
+    vals = rng.integers(0, 2**63, size=2000, dtype=np.int64).astype(np.uint64)
+    bins = rng.integers(0, 100, size=2000).astype(np.uint64)
+    ibf.emplace(vals, bins)
+    for v, b in zip(vals[:300], bins[:300]):
+        assert ibf.contains_bits(int(v))[int(b)] == 1
+
+
+# ------------------------------------------------------------ HIBF oracle --
+
+def _tiny_hibf():
+    """Root IBF: [UB0 split over TBs 0-1, merged->IBF1, UB3, deleted, UB4];
+    IBF1: [UB1, UB2 split over TBs 1-2].  The deleted TB still holds the
+    values of ub[5] (not a user bin), whose hits carry into UB4's run."""
+    rng = np.random.default_rng(9)
+    ub = [rng.integers(0, 2**63, size=n, dtype=np.int64).astype(np.uint64) for n in (40, 30, 30, 20, 10, 10)]
+    root = O.Ibf(6, 2048, 2)
+    root.emplace(ub[0][:20], 0)
+    root.emplace(ub[0][20:], 1)
+    root.emplace(np.concatenate([ub[1], ub[2]]), 2)
+    root.emplace(ub[3], 3)
+    root.emplace(ub[5], 4)
+    root.emplace(ub[4], 5)
+    child = O.Ibf(3, 2048, 2)
+    child.emplace(ub[1], 0)
+    child.emplace(ub[2][:15], 1)
+    child.emplace(ub[2][15:], 2)
+    M, D = O.BIN_MERGED, O.BIN_DELETED
+    user_bin = [0, 0, M, 3, D, 4, 1, 2, 2]
+    next_ibf = [0, 0, 1, 0, 0, 0, 1, 1, 1]
+    arena = np.concatenate([root.words, child.words])
+    hb = O.Hibf([root.g, child.g], arena, [0, root.words.size], [0, 6, 9], next_ibf, user_bin, 5)
+    return hb, ub
+
+
+def _brute_membership(ub, query, t):
+    """Reference order for the tiny tree: pre-order over TBs."""
+    q = set(int(x) for x in query)
+    cnt = [len(q & set(int(x) for x in u)) for u in ub]
+    out = []
+    if cnt[0] >= t:
+        out.append(0)
+    if cnt[1] + cnt[2] >= t:
+        if cnt[1] >= t:
+            out.append(1)
+        if cnt[2] >= t:
+            out.append(2)
+    # UB3 at TB 3; the deleted TB 4 is not reset, so its hits add to UB4's run (TB 5)
+    if cnt[3] >= t:
+        out.append(3)
+    if cnt[4] + cnt[5] >= t:
+        out.append(4)
+    return out
+
+
+def test_hibf_oracle_tiny_tree_bruteforce():
+    hb, ub = _tiny_hibf()
+    rng = np.random.default_rng(1)
+    for trial in range(60):
+        parts = [u[rng.integers(0, len(u), size=int(rng.integers(0, 12)))] for u in ub]
+        query = np.unique(np.concatenate(parts))
+        t = int(rng.integers(1, 10))
+        assert hb.membership_for(query, t) == _brute_membership(ub, query, t)
+        counts = hb.bulk_count(query, t, 2)
+        mem = set(hb.membership_for(query, t))
+        for u in range(5):
+            assert (counts[u] > 0) == (u in mem)
+
+
+def test_hibf_oracle_dfs_order_and_deleted_carry():
+    hb, ub = _tiny_hibf()
+    # values of UB1 only: merged bin (TB 2) fires, results come from the child
+    assert hb.membership_for(ub[1][:8], 5) == [1]
+    # UB3 then UB4 in TB order; UB0 first
+    q = np.concatenate([ub[4][:6], ub[0][:6], ub[3][:6]])
+    assert hb.membership_for(q, 6) == [0, 3, 4]
+    # hits in the deleted TB carry into the next run (hierarchical_interleaved_bloom_filter.hpp:339-340)
+    q = np.concatenate([ub[5][:4], ub[4][:2]])
+    assert hb.membership_for(q, 6) == [4]
+    assert hb.bulk_count(q, 6, 2).tolist() == [0, 0, 0, 0, 6]
