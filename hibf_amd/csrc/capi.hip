@@ -281,6 +281,32 @@ int hibf_gpu_ibf_emplace(hibf_gpu_ibf ibf,
     return HIBF_GPU_OK;
 }
 
+int hibf_gpu_ibf_emplace_synthetic(hibf_gpu_ibf ibf,
+                                   uint64_t seed,
+                                   uint64_t first_bin,
+                                   uint64_t n_bins,
+                                   uint64_t values_per_bin,
+                                   void * stream)
+{
+    if (!ibf)
+        return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "null argument");
+    if (first_bin > ibf->g.bins || n_bins > ibf->g.bins - first_bin)
+        return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "bin range exceeds bin_count");
+    if (values_per_bin >= (1ULL << 36))
+        return fail(HIBF_GPU_ERR_UNSUPPORTED, "values_per_bin must be < 2^36");
+    HIBF_TRY_HIP(launch_emplace_synth(ibf->d_words, ibf->g.bin_size, static_cast<uint32_t>(ibf->g.bin_words),
+                                      static_cast<uint32_t>(ibf->g.hash_shift),
+                                      static_cast<uint32_t>(ibf->g.hash_funs), seed, first_bin, n_bins,
+                                      values_per_bin, static_cast<hipStream_t>(stream)),
+                 "emplace_synthetic launch");
+    return HIBF_GPU_OK;
+}
+
+uint64_t hibf_gpu_synth_value(uint64_t seed, uint64_t bin, uint64_t i)
+{
+    return synth_value(seed, bin, i);
+}
+
 int hibf_gpu_ibf_clear(hibf_gpu_ibf ibf, uint64_t const * h_bins, uint64_t n, void * stream)
 {
     if (!ibf || (!h_bins && n))

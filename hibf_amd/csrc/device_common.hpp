@@ -38,6 +38,22 @@ __device__ __forceinline__ uint64_t row_word(uint64_t v, uint64_t seed, uint32_t
     return __umul64hi(h, bin_size) * bin_words;
 }
 
+// Synthetic value stream used by generator-driven bulk emplace: value i of
+// user bin (or technical bin) `bin` under `seed`.  fmix64 (the MurmurHash3
+// finaliser) is a bijection of 2^64, and (bin << 36 | i) + seed * golden is
+// injective for i < 2^36, bin < 2^28, so every (bin, i) pair of one seed gives
+// a distinct, uniformly spread value.
+__host__ __device__ __forceinline__ uint64_t synth_value(uint64_t seed, uint64_t bin, uint64_t i)
+{
+    uint64_t k = ((bin << 36) | i) + seed * 0x9E3779B97F4A7C15ULL;
+    k ^= k >> 33;
+    k *= 0xFF51AFD7ED558CCDULL;
+    k ^= k >> 33;
+    k *= 0xC4CEB9FE1A85EC53ULL;
+    k ^= k >> 33;
+    return k;
+}
+
 // Geometry of one IBF as the kernels need it.
 struct IbfView
 {

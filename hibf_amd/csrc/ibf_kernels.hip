@@ -38,6 +38,35 @@ __global__ void __launch_bounds__(256) emplace_kernel(uint64_t * __restrict__ wo
     }
 }
 
+// Generator-driven bulk emplace: values synth_value(seed, first_bin + b, j)
+// for j < per_bin go to technical bin first_bin + b, b < n_bins.
+__global__ void __launch_bounds__(256) emplace_synth_kernel(uint64_t * __restrict__ words,
+                                                            uint64_t bin_size,
+                                                            uint32_t bin_words,
+                                                            uint32_t hash_shift,
+                                                            uint32_t hash_funs,
+                                                            uint64_t seed,
+                                                            uint64_t first_bin,
+                                                            uint64_t per_bin,
+                                                            uint64_t n)
+{
+    uint64_t const stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
+    {
+        uint64_t const b = first_bin + i / per_bin;
+        uint64_t const v = synth_value(seed, b, i % per_bin);
+        uint64_t const word_in_row = b >> 6;
+        unsigned long long const bit = 1ULL << (b & 63u);
+#pragma unroll
+        for (int k = 0; k < 5; ++k)
+            if (static_cast<uint32_t>(k) < hash_funs)
+            {
+                uint64_t const w = row_word(v, hash_seed(k), hash_shift, bin_size, bin_words) + word_in_row;
+                atomicOr(reinterpret_cast<unsigned long long *>(words + w), bit);
+            }
+    }
+}
+
 // clear(bin): zero the bin's column in every row (src/interleaved_bloom_filter.cpp:142-148).
 __global__ void __launch_bounds__(256) clear_kernel(uint64_t * __restrict__ words,
                                                     uint64_t bin_size,
@@ -161,6 +190,25 @@ hipError_t launch_emplace(uint64_t * words,
         return hipSuccess;
     hipLaunchKernelGGL(emplace_kernel, dim3(grid_for(n, 256)), dim3(256), 0, stream, words, bin_size, bin_words,
                        hash_shift, hash_funs, values, bins, bin, n, bin_count);
+    return hipGetLastError();
+}
+
+hipError_t launch_emplace_synth(uint64_t * words,
+                                uint64_t bin_size,
+                                uint32_t bin_words,
+                                uint32_t hash_shift,
+                                uint32_t hash_funs,
+                                uint64_t seed,
+                                uint64_t first_bin,
+                                uint64_t n_bins,
+                                uint64_t per_bin,
+                                hipStream_t stream)
+{
+    uint64_t const n = n_bins * per_bin;
+    if (n == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(emplace_synth_kernel, dim3(grid_for(n, 256)), dim3(256), 0, stream, words, bin_size,
+                       bin_words, hash_shift, hash_funs, seed, first_bin, per_bin, n);
     return hipGetLastError();
 }
 

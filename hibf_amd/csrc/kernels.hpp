@@ -52,6 +52,17 @@ hipError_t launch_emplace(uint64_t * words,
                           uint64_t bin_count,
                           hipStream_t stream);
 
+hipError_t launch_emplace_synth(uint64_t * words,
+                                uint64_t bin_size,
+                                uint32_t bin_words,
+                                uint32_t hash_shift,
+                                uint32_t hash_funs,
+                                uint64_t seed,
+                                uint64_t first_bin,
+                                uint64_t n_bins,
+                                uint64_t per_bin,
+                                hipStream_t stream);
+
 hipError_t launch_clear(uint64_t * words,
                         uint64_t bin_size,
                         uint32_t bin_words,

@@ -113,6 +113,23 @@ class DeviceIbf:
         self._keep = (v,)
         torch.cuda.current_stream().synchronize()
 
+    def emplace_synthetic(self, seed: int, first_bin: int, n_bins: int, values_per_bin: int) -> None:
+        """Technical bin first_bin+b receives synth_values(seed, first_bin+b, 0..values_per_bin)."""
+        check(lib().hibf_gpu_ibf_emplace_synthetic(self.h, seed, first_bin, n_bins, values_per_bin, _stream()))
+        torch.cuda.current_stream().synchronize()
+
+    def device_words(self) -> int:
+        p = ctypes.c_void_p()
+        check(lib().hibf_gpu_ibf_device_words(self.h, ctypes.byref(p)))
+        return int(p.value)
+
+    def bulk_count_into(self, values: torch.Tensor, offsets: torch.Tensor, out: torch.Tensor,
+                        counter_bytes: int = 2, stream=None) -> None:
+        """bulk_count into a preallocated (nq, bin_count) tensor, asynchronously."""
+        nq = offsets.numel() - 1
+        s = _stream() if stream is None else ctypes.c_void_p(stream.cuda_stream)
+        check(lib().hibf_gpu_ibf_bulk_count(self.h, _ptr(values), _ptr(offsets), nq, counter_bytes, _ptr(out), s))
+
     def clear(self, bins) -> None:
         b = np.ascontiguousarray(np.atleast_1d(np.asarray(bins, dtype=np.uint64)))
         check(lib().hibf_gpu_ibf_clear(self.h, b.ctypes.data_as(ctypes.c_void_p), b.size, _stream()))

@@ -66,6 +66,8 @@ _SIGNATURES = {
     "hibf_gpu_ibf_download": (_INT, [_V, _V, _U64]),
     "hibf_gpu_ibf_emplace": (_INT, [_V, _V, _V, _U64, _U64, _V]),
     "hibf_gpu_ibf_clear": (_INT, [_V, _V, _U64, _V]),
+    "hibf_gpu_ibf_emplace_synthetic": (_INT, [_V, _U64, _U64, _U64, _U64, _V]),
+    "hibf_gpu_synth_value": (_U64, [_U64, _U64, _U64]),
     "hibf_gpu_ibf_bulk_contains": (_INT, [_V, _V, _U64, _V, _V]),
     "hibf_gpu_ibf_bulk_count": (_INT, [_V, _V, _V, _U64, _U32, _V, _V]),
     "hibf_gpu_agent_create": (_INT, [_U64, ctypes.POINTER(_V)]),
@@ -101,7 +103,7 @@ def lib() -> ctypes.CDLL:
 def declared_functions() -> list[str]:
     """Names of the functions declared in include/hibf_gpu.h."""
     text = open(HEADER).read()
-    return sorted(set(re.findall(r"^\s*(?:int|const char \*)\s+(hibf_gpu_\w+)\s*\(", text, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|uint64_t|const char \*)\s+(hibf_gpu_\w+)\s*\(", text, re.M)))
 
 
 def check(rc: int) -> None:

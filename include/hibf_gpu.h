@@ -97,6 +97,23 @@ int hibf_gpu_ibf_emplace(hibf_gpu_ibf ibf,
                          uint64_t n,
                          void * stream);
 
+/* Generator-driven bulk emplace (no counterpart in the reference API; it
+ * replaces the caller loop that emplaces a generated set per bin, as the
+ * reference's IBF config constructor does through input_fn,
+ * src/interleaved_bloom_filter.cpp:104-117): technical bin first_bin + b
+ * (b < n_bins) receives hibf_gpu_synth_value(seed, first_bin + b, j) for
+ * j < values_per_bin. */
+int hibf_gpu_ibf_emplace_synthetic(hibf_gpu_ibf ibf,
+                                   uint64_t seed,
+                                   uint64_t first_bin,
+                                   uint64_t n_bins,
+                                   uint64_t values_per_bin,
+                                   void * stream);
+
+/* The synthetic value stream: fmix64((bin << 36 | i) + seed * 0x9E3779B97F4A7C15),
+ * distinct for distinct (bin, i) with i < 2^36 and bin < 2^28. */
+uint64_t hibf_gpu_synth_value(uint64_t seed, uint64_t bin, uint64_t i);
+
 /* Replaces clear(bin_index) / clear(range) (interleaved_bloom_filter.hpp:270, 283-297;
  * src/interleaved_bloom_filter.cpp:142-148) for n bins given on the host. */
 int hibf_gpu_ibf_clear(hibf_gpu_ibf ibf, uint64_t const * h_bins, uint64_t n, void * stream);

@@ -154,10 +154,11 @@ class Ibf:
 
     @classmethod
     def from_words(cls, bins: int, bin_size: int, hash_funs: int, words: np.ndarray) -> "Ibf":
-        ibf = cls(bins, bin_size, hash_funs)
+        ibf = cls.__new__(cls)
+        ibf.g = geom(bins, bin_size, hash_funs)
         w = _u64(words)
-        assert w.size == ibf.words.size
-        ibf.words = w.copy()
+        assert w.size == ibf.g.word_count
+        ibf.words = w
         return ibf
 
     def emplace(self, values, bins) -> None:
