@@ -42,6 +42,8 @@ def parse():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--plain-queries", type=int, default=2000)
     ap.add_argument("--check-queries", type=int, default=256)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run a 1 GiB elementwise copy kernel (known bytes) for counter calibration")
     return ap.parse_args()
 
 
@@ -116,8 +118,8 @@ def main():
         ibf.bulk_count_into(values, offsets[q:q + 2], outs, 2)
     torch.cuda.synchronize()
     plain_s = time.perf_counter() - t0
-    plain_value = npl * L / plain_s
-    lat = []
+    plain_value = npl * L / plain_s if npl else float("nan")
+    lat = [0.0]
     for q in range(min(300, npl)):
         t1 = time.perf_counter()
         ibf.bulk_count_into(values, offsets[q:q + 2], outs, 2)
@@ -126,10 +128,17 @@ def main():
     log(f"plain: {plain_value / 1e9:.4f} G lookups/s; latency p50 {np.percentile(lat, 50):.3f} ms "
         f"p99 {np.percentile(lat, 99):.3f} ms")
 
+    if args.calibrate_copy:
+        a = torch.ones(1 << 27, dtype=torch.int64, device=dev)
+        b = torch.empty_like(a)
+        torch.add(a, 0, out=b)  # reads 1 GiB, writes 1 GiB
+        torch.cuda.synchronize()
+        del a, b
+
     # Parity on a bounded sample (oracle on the downloaded matrix) and the CPU baseline.
     cpu = None
     parity = None
-    if not args.skip_cpu or args.check_queries:
+    if not args.skip_cpu or args.check_queries > 0:
         import oracle as O
         words = ibf.download()
         ref = O.Ibf.from_words(bins, bin_size, h, words)
