@@ -42,6 +42,8 @@ def parse():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--plain-queries", type=int, default=2000)
     ap.add_argument("--check-queries", type=int, default=256)
+    ap.add_argument("--tune", action="append", default=[], metavar="KEY=VALUE",
+                    help="launch-shape knob passed to hibf_gpu_tune_set (repeatable)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a 1 GiB elementwise copy kernel (known bytes) for counter calibration")
     return ap.parse_args()
@@ -61,6 +63,11 @@ def main():
     import torch
     from hibf_amd.device import DeviceIbf, as_device_u64
     from hibf_amd.synth import flat_ibf_queries
+
+    from hibf_amd import native
+    for kv in args.tune:
+        k, v = kv.split("=")
+        native.tune_set(k, int(v))
 
     torch.cuda.set_device(0)
     dev = torch.device("cuda", 0)
@@ -188,6 +195,8 @@ def main():
                         f"({int(args.frac_in * 100)}% from one bin); uint16 bulk_count",
             "bins": bins, "bin_size": bin_size, "hash_functions": h, "queries": nq, "query_length": L,
             "counter": "uint16",
+            "tuning": {k: native.tune_get(k) for k in ("count.wide_kernel", "count.wide_unroll",
+                                                        "count.wide_waves")},
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

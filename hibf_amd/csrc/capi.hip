@@ -187,6 +187,38 @@ const char * hibf_gpu_last_error(void)
     return g_last_error.c_str();
 }
 
+int hibf_gpu_tune_set(char const * key, int64_t value)
+{
+    if (!key)
+        return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "null key");
+    std::string const k{key};
+    if (k == "count.wide_kernel" && (value == 0 || value == 1))
+        g_count_tuning.wide_kernel = static_cast<uint32_t>(value);
+    else if (k == "count.wide_unroll" && (value == 1 || value == 2 || value == 4 || value == 8))
+        g_count_tuning.wide_unroll = static_cast<uint32_t>(value);
+    else if (k == "count.wide_waves" && value >= 1 && value <= 4)
+        g_count_tuning.wide_waves = static_cast<uint32_t>(value);
+    else
+        return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "unknown tuning key or value out of range: " + k);
+    return HIBF_GPU_OK;
+}
+
+int hibf_gpu_tune_get(char const * key, int64_t * value)
+{
+    if (!key || !value)
+        return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "null argument");
+    std::string const k{key};
+    if (k == "count.wide_kernel")
+        *value = g_count_tuning.wide_kernel;
+    else if (k == "count.wide_unroll")
+        *value = g_count_tuning.wide_unroll;
+    else if (k == "count.wide_waves")
+        *value = g_count_tuning.wide_waves;
+    else
+        return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "unknown tuning key: " + k);
+    return HIBF_GPU_OK;
+}
+
 /* ---------------------------------------------------------------- IBF --- */
 
 int hibf_gpu_ibf_create(uint64_t bins, uint64_t bin_size, uint32_t hash_funs, hibf_gpu_ibf * out)

@@ -1,0 +1,225 @@
+// bulk_count for wide IBFs (bin_words even and >= 64, e.g. 8,192 technical
+// bins = 128 words = 1 KiB rows): the HBM-bound shape of the flat-IBF workload.
+//
+// One block (4 waves) per query.  Each lane owns two adjacent words of a
+// 128-word tile and fetches them with one 16-byte load per row, so one wave
+// instruction moves a whole 1 KiB row.  The row starts of a value are
+// wave-uniform (readlane of the lane that hashed it), U values are in flight
+// per wave before their hits are counted.  Counters are 16-bit pairs packed in
+// 32-bit LDS words (16 KiB per 8,192 bins); a query longer than 65,535 values
+// is processed in segments whose counters are flushed (added) to the output,
+// so no packed counter can carry into its neighbour.
+#include "device_common.hpp"
+#include "kernels.hpp"
+
+namespace hibf_amd
+{
+
+namespace
+{
+
+constexpr uint32_t kWideTileWords = 128;   // 8,192 bins per tile
+constexpr uint32_t kSegmentValues = 65535; // packed 16-bit counters never overflow
+
+template <typename T>
+__device__ __forceinline__ void put(void * out, uint64_t i, uint32_t v, bool accumulate)
+{
+    T * p = static_cast<T *>(out) + i;
+    *p = accumulate ? static_cast<T>(*p + static_cast<T>(v)) : static_cast<T>(v);
+}
+
+__device__ __forceinline__ void put_counter(void * out, uint64_t i, uint32_t cb, uint32_t v, bool accumulate)
+{
+    switch (cb)
+    {
+    case 1: put<uint8_t>(out, i, v, accumulate); break;
+    case 2: put<uint16_t>(out, i, v, accumulate); break;
+    case 4: put<uint32_t>(out, i, v, accumulate); break;
+    default: put<uint64_t>(out, i, v, accumulate); break;
+    }
+}
+
+// Writes nb counters of a tile (packed pairs in lds) to out[first ..).
+__device__ void store_tile(uint32_t const * lds,
+                           uint32_t nb,
+                           void * counts,
+                           uint64_t first,
+                           uint32_t cb,
+                           bool accumulate,
+                           int tid,
+                           int nthreads)
+{
+    bool const vec = !accumulate && cb == 2u && (first % 8u) == 0u;
+    if (vec)
+    {
+        // 8 counters (4 packed LDS words) per 16-byte store.
+        uint32_t const n8 = nb / 8u;
+        uint4 * const out = reinterpret_cast<uint4 *>(static_cast<uint16_t *>(counts) + first);
+        for (uint32_t i = static_cast<uint32_t>(tid); i < n8; i += static_cast<uint32_t>(nthreads))
+        {
+            uint4 const w = *reinterpret_cast<uint4 const *>(lds + 4u * i);
+            out[i] = w;
+        }
+        for (uint32_t i = n8 * 8u + static_cast<uint32_t>(tid); i < nb; i += static_cast<uint32_t>(nthreads))
+            static_cast<uint16_t *>(counts)[first + i] = static_cast<uint16_t>(lds[i >> 1] >> (16u * (i & 1u)));
+        return;
+    }
+    for (uint32_t i = static_cast<uint32_t>(tid); i < nb; i += static_cast<uint32_t>(nthreads))
+        put_counter(counts, first + i, cb, (lds[i >> 1] >> (16u * (i & 1u))) & 0xFFFFu, accumulate);
+}
+
+template <int U>
+__global__ void __launch_bounds__(256) count_wide_kernel(IbfView g,
+                                                         uint64_t const * __restrict__ values,
+                                                         uint64_t const * __restrict__ offsets,
+                                                         uint64_t nq,
+                                                         uint32_t counter_bytes,
+                                                         void * __restrict__ counts)
+{
+    extern __shared__ uint32_t lds[]; // kWideTileWords * 32 packed counter pairs
+    int const tid = static_cast<int>(threadIdx.x);
+    int const nthreads = static_cast<int>(blockDim.x);
+    int const lane = tid & 63;
+    int const wave = tid >> 6;
+    int const nwaves = nthreads >> 6;
+    uint64_t const bw = g.bin_words;
+    uint32_t const hf = g.hash_funs;
+    uint64_t const * __restrict__ words = g.words;
+
+    for (uint64_t q = blockIdx.x; q < nq; q += gridDim.x)
+    {
+        uint64_t const beg = offsets[q];
+        uint32_t const L = static_cast<uint32_t>(offsets[q + 1] - beg);
+        uint64_t const * __restrict__ vals = values + beg;
+        for (uint32_t w0 = 0; w0 < g.bin_words; w0 += kWideTileWords)
+        {
+            uint32_t const tw = min(kWideTileWords, g.bin_words - w0);
+            uint32_t const wl = 2u * static_cast<uint32_t>(lane); // first of my two words in the tile
+            bool const mine = wl < tw;                          // tw is even: both words or none
+            uint64_t const m0 = mine ? tail_mask(g, w0 + wl) : 0ULL;
+            uint64_t const m1 = mine ? tail_mask(g, w0 + wl + 1u) : 0ULL;
+            uint32_t seg0 = 0;
+            do
+            {
+                uint32_t const segL = min(kSegmentValues, L - seg0);
+                for (uint32_t i = static_cast<uint32_t>(tid); i < tw * 32u; i += static_cast<uint32_t>(nthreads))
+                    lds[i] = 0u;
+                __syncthreads();
+                for (uint32_t base = static_cast<uint32_t>(wave) * 64u; base < segL;
+                     base += static_cast<uint32_t>(nwaves) * 64u)
+                {
+                    uint32_t const nv = min(64u, segL - base);
+                    uint64_t const v = static_cast<uint32_t>(lane) < nv ? vals[seg0 + base + lane] : 0ULL;
+                    uint64_t rows[5];
+#pragma unroll
+                    for (int k = 0; k < 5; ++k)
+                        rows[k] = static_cast<uint32_t>(k) < hf
+                                      ? row_word(v, hash_seed(k), g.hash_shift, g.bin_size, bw) + w0
+                                      : 0ULL;
+                    for (uint32_t s = 0; s < nv; s += U)
+                    {
+                        uint64_t x0[U], x1[U];
+#pragma unroll
+                        for (int u = 0; u < U; ++u)
+                        {
+                            x0[u] = 0ULL;
+                            x1[u] = 0ULL;
+                            uint32_t const idx = s + static_cast<uint32_t>(u);
+                            if (idx < nv)
+                            {
+                                // Row start of value idx (uniform) + my word offset.
+                                uint64_t const r0 = readlane64(rows[0], static_cast<int>(idx)) + wl;
+                                if (mine)
+                                {
+                                    ulonglong2 a = *reinterpret_cast<ulonglong2 const *>(words + r0);
+#pragma unroll
+                                    for (int k = 1; k < 5; ++k)
+                                        if (static_cast<uint32_t>(k) < hf)
+                                        {
+                                            uint64_t const rk = readlane64(rows[k], static_cast<int>(idx)) + wl;
+                                            ulonglong2 const b = *reinterpret_cast<ulonglong2 const *>(words + rk);
+                                            a.x &= b.x;
+                                            a.y &= b.y;
+                                        }
+                                    x0[u] = a.x & m0;
+                                    x1[u] = a.y & m1;
+                                }
+                            }
+                        }
+#pragma unroll
+                        for (int u = 0; u < U; ++u)
+                        {
+                            uint64_t bits = x0[u];
+                            while (bits)
+                            {
+                                uint32_t const b = wl * 64u + static_cast<uint32_t>(__builtin_ctzll(bits));
+                                atomicAdd(&lds[b >> 1], 1u << (16u * (b & 1u)));
+                                bits &= bits - 1ULL;
+                            }
+                            bits = x1[u];
+                            while (bits)
+                            {
+                                uint32_t const b = (wl + 1u) * 64u + static_cast<uint32_t>(__builtin_ctzll(bits));
+                                atomicAdd(&lds[b >> 1], 1u << (16u * (b & 1u)));
+                                bits &= bits - 1ULL;
+                            }
+                        }
+                    }
+                }
+                __syncthreads();
+                uint32_t const first_bin = w0 * 64u;
+                uint32_t const nb = min(tw * 64u, g.bins - first_bin);
+                store_tile(lds, nb, counts, q * g.bins + first_bin, counter_bytes, seg0 != 0, tid, nthreads);
+                __syncthreads();
+                seg0 += segL;
+            } while (seg0 < L);
+        }
+    }
+}
+
+} // namespace
+
+CountTuning g_count_tuning{};
+
+bool count_wide_applies(IbfView const & g)
+{
+    return g.bin_words >= 64u && (g.bin_words % 2u) == 0u && g_count_tuning.wide_kernel != 0u;
+}
+
+hipError_t launch_count_wide(IbfView const & g,
+                             uint64_t const * values,
+                             uint64_t const * offsets,
+                             uint64_t nq,
+                             uint32_t counter_bytes,
+                             void * counts,
+                             unsigned max_blocks,
+                             hipStream_t stream)
+{
+    if (nq == 0)
+        return hipSuccess;
+    size_t const lds = static_cast<size_t>(kWideTileWords) * 32u * sizeof(uint32_t);
+    unsigned const grid = static_cast<unsigned>(nq < max_blocks ? nq : max_blocks);
+    dim3 const blk(64u * g_count_tuning.wide_waves);
+    switch (g_count_tuning.wide_unroll)
+    {
+    case 1:
+        hipLaunchKernelGGL((count_wide_kernel<1>), dim3(grid), blk, lds, stream, g, values, offsets, nq,
+                           counter_bytes, counts);
+        break;
+    case 2:
+        hipLaunchKernelGGL((count_wide_kernel<2>), dim3(grid), blk, lds, stream, g, values, offsets, nq,
+                           counter_bytes, counts);
+        break;
+    case 8:
+        hipLaunchKernelGGL((count_wide_kernel<8>), dim3(grid), blk, lds, stream, g, values, offsets, nq,
+                           counter_bytes, counts);
+        break;
+    default:
+        hipLaunchKernelGGL((count_wide_kernel<4>), dim3(grid), blk, lds, stream, g, values, offsets, nq,
+                           counter_bytes, counts);
+        break;
+    }
+    return hipGetLastError();
+}
+
+} // namespace hibf_amd

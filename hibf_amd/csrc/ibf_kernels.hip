@@ -265,6 +265,8 @@ hipError_t launch_count(IbfView const & g,
 {
     if (nq == 0)
         return hipSuccess;
+    if (count_wide_applies(g))
+        return launch_count_wide(g, values, offsets, nq, counter_bytes, counts, max_blocks, stream);
     uint32_t lpv_log, cw, tile_words;
     count_shape(g.bin_words, lpv_log, cw, tile_words);
     size_t const lds = static_cast<size_t>(tile_words) * 64u * sizeof(uint32_t);

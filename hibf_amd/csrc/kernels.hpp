@@ -40,6 +40,27 @@ struct HibfIbfDesc
 
 void count_shape(uint32_t bin_words, uint32_t & lpv_log, uint32_t & cw, uint32_t & tile_words);
 
+// Launch-shape knobs of the count kernels, chosen by the autotuner
+// (hibf_gpu_tune_set); defaults are the tuned values for MI355X.
+struct CountTuning
+{
+    uint32_t wide_kernel = 1; // use count_wide_kernel for even bin_words >= 64
+    uint32_t wide_unroll = 4; // values in flight per wave (1, 2, 4, 8)
+    uint32_t wide_waves = 4;  // waves per block (1..4)
+};
+extern CountTuning g_count_tuning;
+
+struct IbfView;
+bool count_wide_applies(IbfView const & g);
+hipError_t launch_count_wide(IbfView const & g,
+                             uint64_t const * values,
+                             uint64_t const * offsets,
+                             uint64_t nq,
+                             uint32_t counter_bytes,
+                             void * counts,
+                             unsigned max_blocks,
+                             hipStream_t stream);
+
 hipError_t launch_emplace(uint64_t * words,
                           uint64_t bin_size,
                           uint32_t bin_words,

@@ -58,6 +58,8 @@ _INT = ctypes.c_int
 _SIGNATURES = {
     "hibf_gpu_status_string": (ctypes.c_char_p, [_INT]),
     "hibf_gpu_last_error": (ctypes.c_char_p, []),
+    "hibf_gpu_tune_set": (_INT, [ctypes.c_char_p, ctypes.c_int64]),
+    "hibf_gpu_tune_get": (_INT, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int64)]),
     "hibf_gpu_ibf_create": (_INT, [_U64, _U64, _U32, ctypes.POINTER(_V)]),
     "hibf_gpu_ibf_destroy": (_INT, [_V]),
     "hibf_gpu_ibf_get_geometry": (_INT, [_V, ctypes.POINTER(Geometry)]),
@@ -104,6 +106,16 @@ def declared_functions() -> list[str]:
     """Names of the functions declared in include/hibf_gpu.h."""
     text = open(HEADER).read()
     return sorted(set(re.findall(r"^\s*(?:int|uint64_t|const char \*)\s+(hibf_gpu_\w+)\s*\(", text, re.M)))
+
+
+def tune_set(key: str, value: int) -> None:
+    check(lib().hibf_gpu_tune_set(key.encode(), int(value)))
+
+
+def tune_get(key: str) -> int:
+    v = ctypes.c_int64()
+    check(lib().hibf_gpu_tune_get(key.encode(), ctypes.byref(v)))
+    return int(v.value)
 
 
 def check(rc: int) -> None:

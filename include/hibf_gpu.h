@@ -42,6 +42,13 @@ const char * hibf_gpu_status_string(int status);
 /* Detail of the last error raised on the calling thread (static per thread). */
 const char * hibf_gpu_last_error(void);
 
+/* Launch-shape knobs of the query kernels (no counterpart in the reference:
+ * the reference has no device tiles).  Process-wide; set by the autotuner.
+ * Keys: "count.wide_kernel" (0/1), "count.wide_unroll" (1,2,4,8),
+ * "count.wide_waves" (1..4).  Results never depend on these values. */
+int hibf_gpu_tune_set(const char * key, int64_t value);
+int hibf_gpu_tune_get(const char * key, int64_t * value);
+
 /* Sentinels of ibf_bin_to_user_bin_id, hierarchical_interleaved_bloom_filter.hpp:32-40. */
 #define HIBF_GPU_BIN_MERGED 0xFFFFFFFFFFFFFFFFULL
 #define HIBF_GPU_BIN_DELETED 0xFFFFFFFFFFFFFFFEULL

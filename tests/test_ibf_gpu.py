@@ -9,7 +9,8 @@ pytestmark = pytest.mark.gpu
 
 GEOMS = [  # (bins, bin_size, hash_funs)
     (1, 64, 1), (7, 1000, 2), (63, 513, 2), (64, 4096, 3), (65, 777, 5), (127, 1024, 2), (130, 2000, 2),
-    (1000, 300, 4), (4096, 3000, 2), (8192, 2048, 3), (9000, 500, 2), (20000, 129, 1),
+    (1000, 300, 4), (4096, 3000, 2), (8192, 2048, 3), (8190, 700, 3), (9000, 500, 2), (16384, 300, 2),
+    (20000, 129, 1),
 ]
 
 
@@ -97,6 +98,27 @@ def test_bulk_count_wraps_like_counting_vector():
     assert to_numpy(dev.bulk_count(values, offsets, 1)).tolist() == [[0, 300 % 256, 0], [0, 70000 % 256, 0]]
     assert to_numpy(dev.bulk_count(values, offsets, 2)).tolist() == [[0, 300, 0], [0, 70000 % 65536, 0]]
     assert to_numpy(dev.bulk_count(values, offsets, 8)).tolist() == [[0, 300, 0], [0, 70000, 0]]
+
+
+@pytest.mark.parametrize("bins", [8192, 16384, 8190])
+def test_bulk_count_long_queries_wide_rows(bins):
+    """Queries longer than 65,535 values on wide rows: counters are flushed per
+    segment and must still wrap exactly like counting_vector<T>."""
+    from hibf_amd.device import DeviceIbf, csr, to_numpy
+    _say(f"count long queries bins={bins}")
+    ref = O.Ibf(bins, 2048, 2)
+    dev = DeviceIbf(bins, 2048, 2)
+    for b in (0, 5, bins - 1):
+        ref.emplace([42, 43], b)
+        dev.emplace([42, 43], b)
+    rng = np.random.default_rng(bins)
+    qs = [np.full(70000, 42, dtype=np.uint64),
+          np.concatenate([np.full(140000, 43, dtype=np.uint64), _rand_u64(rng, 1000)]),
+          np.full(65535, 42, dtype=np.uint64), np.full(65536, 43, dtype=np.uint64)]
+    values, offsets = csr(qs)
+    for width in (1, 2, 4, 8):
+        got = to_numpy(dev.bulk_count(values, offsets, width))
+        assert np.array_equal(got, ref.bulk_count_batch(values, offsets, width)), width
 
 
 @pytest.mark.parametrize("bins,bin_size,h", [(12, 8192, 2), (200, 3000, 3), (8192, 1024, 2), (9000, 300, 2)])
