@@ -16,7 +16,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libhibf_gpu.so")
 
 HIP_SOURCES = ["ibf_kernels.hip", "count_wide.hip", "hibf_kernels.hip", "scan.hip", "capi.hip"]
-CXX_SOURCES: list[str] = []
+CXX_SOURCES = ["layout.cpp", "build.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 COMMON = ["-O3", "-std=c++20", "-fPIC", "-Wall", "-Wno-unused-function",

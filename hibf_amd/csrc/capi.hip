@@ -12,7 +12,9 @@
 #include <vector>
 
 #include "hibf_gpu.h"
+#include "build.hpp"
 #include "kernels.hpp"
+#include "layout.hpp"
 
 using namespace hibf_amd;
 
@@ -151,6 +153,9 @@ struct hibf_gpu_hibf_s
 {
     std::vector<hibf_gpu_ibf_geometry> geoms;
     std::vector<uint64_t> word_offsets;
+    std::vector<uint64_t> tb_offsets;
+    std::vector<uint64_t> next_ibf;
+    std::vector<uint64_t> user_bin;
     uint64_t n_words{0};
     uint64_t n_user_bins{0};
     uint32_t max_bins{0};
@@ -534,6 +539,9 @@ int hibf_gpu_hibf_create(uint64_t n_ibfs,
     h->n_user_bins = number_of_user_bins;
     h->geoms.resize(n_ibfs);
     h->word_offsets.assign(h_word_offsets, h_word_offsets + n_ibfs);
+    h->tb_offsets.assign(h_tb_offsets, h_tb_offsets + n_ibfs + 1);
+    h->next_ibf.assign(h_next_ibf, h_next_ibf + h_tb_offsets[n_ibfs]);
+    h->user_bin.assign(h_user_bin, h_user_bin + h_tb_offsets[n_ibfs]);
     std::vector<HibfIbfDesc> desc(n_ibfs);
     uint64_t const n_tbs = h_tb_offsets[n_ibfs];
     std::vector<uint64_t> info(n_tbs);
@@ -643,6 +651,45 @@ int hibf_gpu_hibf_create(uint64_t n_ibfs,
         return hip_fail(e, "hibf_gpu_hibf_create");
     }
     *out = guard.release();
+    return HIBF_GPU_OK;
+}
+
+int hibf_gpu_hibf_info(hibf_gpu_hibf h,
+                       uint64_t * n_ibfs,
+                       uint64_t * n_tbs,
+                       uint64_t * n_words,
+                       uint64_t * n_user_bins,
+                       uint64_t * depth)
+{
+    if (!h)
+        return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "null handle");
+    if (n_ibfs)
+        *n_ibfs = h->geoms.size();
+    if (n_tbs)
+        *n_tbs = h->tb_offsets.back();
+    if (n_words)
+        *n_words = h->n_words;
+    if (n_user_bins)
+        *n_user_bins = h->n_user_bins;
+    if (depth)
+        *depth = h->depth;
+    return HIBF_GPU_OK;
+}
+
+int hibf_gpu_hibf_export_tables(hibf_gpu_hibf h,
+                                hibf_gpu_ibf_geometry * geoms,
+                                uint64_t * word_offsets,
+                                uint64_t * tb_offsets,
+                                uint64_t * next_ibf,
+                                uint64_t * user_bin)
+{
+    if (!h || !geoms || !word_offsets || !tb_offsets || !next_ibf || !user_bin)
+        return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "null argument");
+    std::copy(h->geoms.begin(), h->geoms.end(), geoms);
+    std::copy(h->word_offsets.begin(), h->word_offsets.end(), word_offsets);
+    std::copy(h->tb_offsets.begin(), h->tb_offsets.end(), tb_offsets);
+    std::copy(h->next_ibf.begin(), h->next_ibf.end(), next_ibf);
+    std::copy(h->user_bin.begin(), h->user_bin.end(), user_bin);
     return HIBF_GPU_OK;
 }
 
@@ -799,6 +846,284 @@ int hibf_gpu_hibf_bulk_count(hibf_gpu_hibf h,
                                    hibf_blocks(nq, 0), static_cast<hipStream_t>(stream)),
                  "hibf count launch");
     return HIBF_GPU_OK;
+}
+
+/* ------------------------------------------------------- HIBF build --- */
+
+int hibf_gpu_config_default(hibf_gpu_config * c)
+{
+    if (!c)
+        return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "null config");
+    host::Config const d{};
+    *c = hibf_gpu_config{d.number_of_user_bins, d.number_of_hash_functions, d.maximum_fpr, d.relaxed_fpr, d.threads,
+                         d.sketch_bits, d.tmax, d.empty_bin_fraction, d.alpha, d.max_rearrangement_ratio,
+                         d.disable_estimate_union, d.disable_rearrangement, d.track_occupancy, 0};
+    return HIBF_GPU_OK;
+}
+
+namespace
+{
+host::Config to_host(hibf_gpu_config const & c)
+{
+    host::Config h{};
+    h.number_of_user_bins = c.number_of_user_bins;
+    h.number_of_hash_functions = c.number_of_hash_functions;
+    h.maximum_fpr = c.maximum_fpr;
+    h.relaxed_fpr = c.relaxed_fpr;
+    h.threads = c.threads;
+    h.sketch_bits = c.sketch_bits;
+    h.tmax = c.tmax;
+    h.empty_bin_fraction = c.empty_bin_fraction;
+    h.alpha = c.alpha;
+    h.max_rearrangement_ratio = c.max_rearrangement_ratio;
+    h.disable_estimate_union = c.disable_estimate_union != 0;
+    h.disable_rearrangement = c.disable_rearrangement != 0;
+    h.track_occupancy = c.track_occupancy != 0;
+    return h;
+}
+
+void from_host(host::Config const & h, hibf_gpu_config & c)
+{
+    c.tmax = h.tmax;
+    c.track_occupancy = h.track_occupancy;
+    c.disable_rearrangement = h.disable_rearrangement;
+}
+
+int copy_text(std::string const & t, char * out, uint64_t cap, uint64_t * needed)
+{
+    if (needed)
+        *needed = t.size() + 1;
+    if (!out || cap < t.size() + 1)
+        return fail(HIBF_GPU_ERR_CAPACITY, "text buffer too small; *needed holds the size");
+    std::memcpy(out, t.c_str(), t.size() + 1);
+    return HIBF_GPU_OK;
+}
+
+// Executes a build plan on the device: allocates the HIBF (zeroed arena) and
+// runs the segment emplace over explicit values or the synthetic stream.
+int run_plan(host::Plan const & plan,
+             uint64_t n_user_bins,
+             uint64_t const * h_ub_offsets,
+             uint64_t const * h_values,
+             uint64_t seed,
+             hibf_gpu_hibf * out)
+{
+    uint64_t const n = plan.ibfs.size();
+    std::vector<hibf_gpu_ibf_geometry> geoms(n);
+    std::vector<uint64_t> word_offsets(n), tb_offsets(n + 1, 0), next_ibf, user_bin;
+    uint64_t words = 0;
+    for (uint64_t i = 0; i < n; ++i)
+    {
+        host::PlannedIbf const & p = plan.ibfs[i];
+        hibf_gpu_ibf_geometry g{};
+        if (int rc = make_geometry(p.bins, p.bin_size, p.hash_funs, g))
+            return rc;
+        geoms[i] = g;
+        word_offsets[i] = words;
+        words += (g.n_words + 31u) / 32u * 32u; // 256-byte aligned IBFs
+        tb_offsets[i + 1] = tb_offsets[i] + p.bins;
+        next_ibf.insert(next_ibf.end(), p.next_ibf.begin(), p.next_ibf.end());
+        user_bin.insert(user_bin.end(), p.user_bin.begin(), p.user_bin.end());
+    }
+    hibf_gpu_hibf h = nullptr;
+    if (int rc = hibf_gpu_hibf_create(n, geoms.data(), nullptr, word_offsets.data(), tb_offsets.data(),
+                                      next_ibf.data(), user_bin.data(), n_user_bins, &h))
+        return rc;
+    // Chunk the segments so that one block handles at most 4096 values.
+    static_assert(sizeof(EmplaceSegment) == sizeof(host::Segment), "segment layouts differ");
+    std::vector<EmplaceChunk> chunks;
+    for (uint64_t s = 0; s < plan.segments.size(); ++s)
+        for (uint64_t b = 0; b < plan.segments[s].count; b += 4096u)
+            chunks.push_back({static_cast<uint32_t>(s), static_cast<uint32_t>(std::min<uint64_t>(4096u, plan.segments[s].count - b)), b});
+    void * d_segs = nullptr;
+    void * d_chunks = nullptr;
+    void * d_vals = nullptr;
+    void * d_offs = nullptr;
+    auto cleanup = [&]()
+    {
+        for (void * p : {d_segs, d_chunks, d_vals, d_offs})
+            if (p)
+                (void)hipFree(p);
+    };
+    hipError_t e = hipMalloc(&d_segs, std::max<size_t>(plan.segments.size(), 1) * sizeof(EmplaceSegment));
+    if (e == hipSuccess)
+        e = hipMemcpy(d_segs, plan.segments.data(), plan.segments.size() * sizeof(EmplaceSegment), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMalloc(&d_chunks, std::max<size_t>(chunks.size(), 1) * sizeof(EmplaceChunk));
+    if (e == hipSuccess)
+        e = hipMemcpy(d_chunks, chunks.data(), chunks.size() * sizeof(EmplaceChunk), hipMemcpyHostToDevice);
+    if (e == hipSuccess && h_values)
+    {
+        uint64_t const nv = h_ub_offsets[n_user_bins];
+        e = hipMalloc(&d_vals, std::max<uint64_t>(nv, 1) * 8u);
+        if (e == hipSuccess)
+            e = hipMemcpy(d_vals, h_values, nv * 8u, hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+            e = hipMalloc(&d_offs, (n_user_bins + 1) * 8u);
+        if (e == hipSuccess)
+            e = hipMemcpy(d_offs, h_ub_offsets, (n_user_bins + 1) * 8u, hipMemcpyHostToDevice);
+    }
+    if (e == hipSuccess)
+        e = launch_segment_emplace(h->d_desc, h->d_arena, static_cast<EmplaceSegment *>(d_segs),
+                                   static_cast<EmplaceChunk *>(d_chunks), chunks.size(),
+                                   static_cast<uint64_t *>(d_vals), static_cast<uint64_t *>(d_offs), seed, nullptr);
+    if (e == hipSuccess)
+        e = hipDeviceSynchronize();
+    cleanup();
+    if (e != hipSuccess)
+    {
+        hibf_gpu_hibf_destroy(h);
+        return hip_fail(e, "hibf build");
+    }
+    *out = h;
+    return HIBF_GPU_OK;
+}
+
+int get_layout(host::Config const & hc, std::vector<uint64_t> const & counts, char const * layout_text,
+               host::Layout & layout)
+{
+    if (layout_text)
+    {
+        if (!host::Layout::from_text(layout_text, layout))
+            return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "layout text could not be parsed");
+        return HIBF_GPU_OK;
+    }
+    layout = host::compute_layout(hc, counts);
+    return HIBF_GPU_OK;
+}
+
+} // namespace
+
+int hibf_gpu_config_validate(hibf_gpu_config * c)
+{
+    if (!c)
+        return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "null config");
+    host::Config h = to_host(*c);
+    std::string const e = host::validate_and_set_defaults(h);
+    if (!e.empty())
+        return fail(HIBF_GPU_ERR_CONFIG, e);
+    from_host(h, *c);
+    return HIBF_GPU_OK;
+}
+
+int hibf_gpu_layout_compute(hibf_gpu_config const * c,
+                            uint64_t const * h_kmer_counts,
+                            char * text,
+                            uint64_t cap,
+                            uint64_t * needed)
+{
+    if (!c || !h_kmer_counts)
+        return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "null argument");
+    host::Config hc = to_host(*c);
+    std::string const e = host::validate_and_set_defaults(hc);
+    if (!e.empty())
+        return fail(HIBF_GPU_ERR_CONFIG, e);
+    std::vector<uint64_t> counts(h_kmer_counts, h_kmer_counts + hc.number_of_user_bins);
+    try
+    {
+        return copy_text(host::compute_layout(hc, counts).to_text(), text, cap, needed);
+    }
+    catch (std::exception const & ex)
+    {
+        return fail(HIBF_GPU_ERR_OUT_OF_MEMORY, ex.what());
+    }
+}
+
+int hibf_gpu_hibf_build(hibf_gpu_config const * c,
+                        uint64_t const * h_ub_offsets,
+                        uint64_t const * h_values,
+                        char const * layout_text,
+                        hibf_gpu_hibf * out)
+{
+    if (!c || !h_ub_offsets || !h_values || !out)
+        return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    host::Config hc = to_host(*c);
+    std::string const e = host::validate_and_set_defaults(hc);
+    if (!e.empty())
+        return fail(HIBF_GPU_ERR_CONFIG, e);
+    uint64_t const nub = hc.number_of_user_bins;
+    if (nub >= (1ULL << 32))
+        return fail(HIBF_GPU_ERR_UNSUPPORTED, "too many user bins");
+    try
+    {
+        // Each user bin is a set (the reference inserts into a hash set): sort + unique.
+        std::vector<uint64_t> offs(nub + 1, 0);
+        std::vector<uint64_t> vals;
+        vals.reserve(h_ub_offsets[nub]);
+        for (uint64_t u = 0; u < nub; ++u)
+        {
+            if (h_ub_offsets[u + 1] < h_ub_offsets[u])
+                return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "user-bin offsets must not decrease");
+            size_t const b = vals.size();
+            vals.insert(vals.end(), h_values + h_ub_offsets[u], h_values + h_ub_offsets[u + 1]);
+            std::sort(vals.begin() + b, vals.end());
+            vals.erase(std::unique(vals.begin() + b, vals.end()), vals.end());
+            offs[u + 1] = vals.size();
+        }
+        host::BuildInput in;
+        in.ub_size.resize(nub);
+        for (uint64_t u = 0; u < nub; ++u)
+            in.ub_size[u] = offs[u + 1] - offs[u];
+        in.union_size = [&](std::vector<uint32_t> const & ubs)
+        {
+            std::vector<uint64_t> all;
+            for (uint32_t u : ubs)
+                all.insert(all.end(), vals.begin() + offs[u], vals.begin() + offs[u + 1]);
+            std::sort(all.begin(), all.end());
+            return static_cast<uint64_t>(std::unique(all.begin(), all.end()) - all.begin());
+        };
+        host::Layout layout;
+        if (int rc = get_layout(hc, in.ub_size, layout_text, layout))
+            return rc;
+        host::Plan plan;
+        std::string const pe = host::build_plan(hc, layout, in, plan);
+        if (!pe.empty())
+            return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, pe);
+        return run_plan(plan, nub, offs.data(), vals.data(), 0, out);
+    }
+    catch (std::exception const & ex)
+    {
+        return fail(HIBF_GPU_ERR_OUT_OF_MEMORY, ex.what());
+    }
+}
+
+int hibf_gpu_hibf_build_synthetic(hibf_gpu_config const * c,
+                                  uint64_t const * h_ub_sizes,
+                                  uint64_t seed,
+                                  char const * layout_text,
+                                  hibf_gpu_hibf * out)
+{
+    if (!c || !h_ub_sizes || !out)
+        return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    host::Config hc = to_host(*c);
+    std::string const e = host::validate_and_set_defaults(hc);
+    if (!e.empty())
+        return fail(HIBF_GPU_ERR_CONFIG, e);
+    uint64_t const nub = hc.number_of_user_bins;
+    if (nub >= (1ULL << 28))
+        return fail(HIBF_GPU_ERR_UNSUPPORTED, "the synthetic stream supports < 2^28 user bins");
+    try
+    {
+        host::BuildInput in;
+        in.ub_size.assign(h_ub_sizes, h_ub_sizes + nub);
+        for (uint64_t s : in.ub_size)
+            if (s >= (1ULL << 36))
+                return fail(HIBF_GPU_ERR_UNSUPPORTED, "the synthetic stream supports < 2^36 values per user bin");
+        host::Layout layout;
+        if (int rc = get_layout(hc, in.ub_size, layout_text, layout))
+            return rc;
+        host::Plan plan;
+        std::string const pe = host::build_plan(hc, layout, in, plan);
+        if (!pe.empty())
+            return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, pe);
+        return run_plan(plan, nub, nullptr, nullptr, seed, out);
+    }
+    catch (std::exception const & ex)
+    {
+        return fail(HIBF_GPU_ERR_OUT_OF_MEMORY, ex.what());
+    }
 }
 
 } // extern "C"

@@ -67,6 +67,40 @@ __global__ void __launch_bounds__(256) emplace_synth_kernel(uint64_t * __restric
     }
 }
 
+// Build-plan execution: one block per chunk of one segment.
+__global__ void __launch_bounds__(256) segment_emplace_kernel(HibfIbfDesc const * __restrict__ desc,
+                                                              uint64_t * __restrict__ arena,
+                                                              EmplaceSegment const * __restrict__ segs,
+                                                              EmplaceChunk const * __restrict__ chunks,
+                                                              uint64_t n_chunks,
+                                                              uint64_t const * __restrict__ values,
+                                                              uint64_t const * __restrict__ ub_offsets,
+                                                              uint64_t seed)
+{
+    for (uint64_t c = blockIdx.x; c < n_chunks; c += gridDim.x)
+    {
+        EmplaceChunk const ch = chunks[c];
+        EmplaceSegment const sg = segs[ch.seg];
+        HibfIbfDesc const d = desc[sg.ibf];
+        uint64_t * const words = arena + d.word_offset;
+        uint64_t const word_in_row = sg.tb >> 6;
+        unsigned long long const bit = 1ULL << (sg.tb & 63u);
+        uint64_t const base = values ? ub_offsets[sg.ub] : 0ULL;
+        for (uint32_t t = threadIdx.x; t < ch.len; t += blockDim.x)
+        {
+            uint64_t const i = sg.begin + ch.start + t;
+            uint64_t const v = values ? values[base + i] : synth_value(seed, sg.ub, i);
+#pragma unroll
+            for (int k = 0; k < 5; ++k)
+                if (static_cast<uint32_t>(k) < d.hash_funs)
+                {
+                    uint64_t const w = row_word(v, hash_seed(k), d.hash_shift, d.bin_size, d.bin_words) + word_in_row;
+                    atomicOr(reinterpret_cast<unsigned long long *>(words + w), bit);
+                }
+        }
+    }
+}
+
 // clear(bin): zero the bin's column in every row (src/interleaved_bloom_filter.cpp:142-148).
 __global__ void __launch_bounds__(256) clear_kernel(uint64_t * __restrict__ words,
                                                     uint64_t bin_size,
@@ -209,6 +243,24 @@ hipError_t launch_emplace_synth(uint64_t * words,
         return hipSuccess;
     hipLaunchKernelGGL(emplace_synth_kernel, dim3(grid_for(n, 256)), dim3(256), 0, stream, words, bin_size,
                        bin_words, hash_shift, hash_funs, seed, first_bin, per_bin, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_segment_emplace(HibfIbfDesc const * desc,
+                                  uint64_t * arena,
+                                  EmplaceSegment const * segs,
+                                  EmplaceChunk const * chunks,
+                                  uint64_t n_chunks,
+                                  uint64_t const * values,
+                                  uint64_t const * ub_offsets,
+                                  uint64_t seed,
+                                  hipStream_t stream)
+{
+    if (n_chunks == 0)
+        return hipSuccess;
+    unsigned const grid = static_cast<unsigned>(n_chunks < (1u << 20) ? n_chunks : (1u << 20));
+    hipLaunchKernelGGL(segment_emplace_kernel, dim3(grid), dim3(256), 0, stream, desc, arena, segs, chunks, n_chunks,
+                       values, ub_offsets, seed);
     return hipGetLastError();
 }
 

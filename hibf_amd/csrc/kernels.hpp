@@ -45,7 +45,7 @@ void count_shape(uint32_t bin_words, uint32_t & lpv_log, uint32_t & cw, uint32_t
 struct CountTuning
 {
     uint32_t wide_kernel = 1; // use count_wide_kernel for even bin_words >= 64
-    uint32_t wide_unroll = 4; // values in flight per wave (1, 2, 4, 8)
+    uint32_t wide_unroll = 2; // values in flight per wave (1, 2, 4, 8)
     uint32_t wide_waves = 4;  // waves per block (1..4)
 };
 extern CountTuning g_count_tuning;
@@ -150,6 +150,39 @@ hipError_t launch_hibf_count(HibfLaunch const & h,
                              void * counts,
                              unsigned blocks,
                              hipStream_t stream);
+
+// Device form of a build-plan segment: values [begin, begin+count) of user
+// bin `ub` go to technical bin `tb` of IBF `ibf` (same layout as host::Segment).
+struct EmplaceSegment
+{
+    uint32_t ibf;
+    uint32_t tb;
+    uint32_t ub;
+    uint32_t pad;
+    uint64_t begin;
+    uint64_t count;
+};
+
+// A block's share of one segment: values [start, start+len) of segment seg.
+struct EmplaceChunk
+{
+    uint32_t seg;
+    uint32_t len;
+    uint64_t start;
+};
+
+// Executes a build plan: sets the bits of every (value, technical bin) pair of
+// every segment.  Values come from values[ub_offsets[ub] + i] or, when
+// values == nullptr, from synth_value(seed, ub, i).
+hipError_t launch_segment_emplace(HibfIbfDesc const * desc,
+                                  uint64_t * arena,
+                                  EmplaceSegment const * segs,
+                                  EmplaceChunk const * chunks,
+                                  uint64_t n_chunks,
+                                  uint64_t const * values,
+                                  uint64_t const * ub_offsets,
+                                  uint64_t seed,
+                                  hipStream_t stream);
 
 // out[offsets[q] + i] = src[start[q] + i] for i < count[q].
 hipError_t launch_gather_hits(uint64_t const * src,

@@ -217,6 +217,58 @@ class DeviceHibf:
         self.number_of_user_bins = int(number_of_user_bins)
         self._agent = None
 
+    @classmethod
+    def _from_handle(cls, h: ctypes.c_void_p) -> "DeviceHibf":
+        self = cls.__new__(cls)
+        self.h = h
+        n_ub = ctypes.c_uint64()
+        check(lib().hibf_gpu_hibf_info(self.h, None, None, None, ctypes.byref(n_ub), None))
+        self.number_of_user_bins = int(n_ub.value)
+        self._agent = None
+        return self
+
+    @classmethod
+    def build(cls, user_bins, layout_text: str | None = None, **config) -> "DeviceHibf":
+        """hierarchical_interleaved_bloom_filter(config) over explicit user bins
+        (a list of uint64 sequences); config keywords as seqan::hibf::config."""
+        _dev()
+        values, offsets = csr(user_bins)
+        c = native.make_config(number_of_user_bins=len(user_bins), **config)
+        h = ctypes.c_void_p()
+        check(lib().hibf_gpu_hibf_build(ctypes.byref(c), offsets.ctypes.data_as(ctypes.c_void_p),
+                                        values.ctypes.data_as(ctypes.c_void_p),
+                                        None if layout_text is None else layout_text.encode(), ctypes.byref(h)))
+        return cls._from_handle(h)
+
+    @classmethod
+    def build_synthetic(cls, sizes, seed: int, layout_text: str | None = None, **config) -> "DeviceHibf":
+        """HIBF over synthetic user bins: user bin u holds synth_values(seed, u, 0..sizes[u])."""
+        _dev()
+        sz = np.ascontiguousarray(np.asarray(sizes, dtype=np.uint64))
+        c = native.make_config(number_of_user_bins=sz.size, **config)
+        h = ctypes.c_void_p()
+        check(lib().hibf_gpu_hibf_build_synthetic(ctypes.byref(c), sz.ctypes.data_as(ctypes.c_void_p), seed,
+                                                  None if layout_text is None else layout_text.encode(),
+                                                  ctypes.byref(h)))
+        return cls._from_handle(h)
+
+    def tables(self) -> dict:
+        """Host copy of the routing tables and IBF geometries."""
+        n_ibfs, n_tbs, n_words, n_ub, depth = (ctypes.c_uint64() for _ in range(5))
+        check(lib().hibf_gpu_hibf_info(self.h, ctypes.byref(n_ibfs), ctypes.byref(n_tbs), ctypes.byref(n_words),
+                                       ctypes.byref(n_ub), ctypes.byref(depth)))
+        n, t = n_ibfs.value, n_tbs.value
+        geoms = (Geometry * n)()
+        wo = np.zeros(n, dtype=np.uint64)
+        to = np.zeros(n + 1, dtype=np.uint64)
+        nx = np.zeros(t, dtype=np.uint64)
+        ub = np.zeros(t, dtype=np.uint64)
+        check(lib().hibf_gpu_hibf_export_tables(self.h, geoms, *[a.ctypes.data_as(ctypes.c_void_p)
+                                                                 for a in (wo, to, nx, ub)]))
+        return {"geoms": [(g.bins, g.bin_size, g.hash_funs) for g in geoms], "word_offsets": wo, "tb_offsets": to,
+                "next_ibf": nx, "user_bin": ub, "n_words": n_words.value, "n_user_bins": n_ub.value,
+                "depth": depth.value}
+
     def __del__(self):
         if getattr(self, "h", None):
             lib().hibf_gpu_hibf_destroy(self.h)

@@ -38,6 +38,27 @@ class HibfGpuError(RuntimeError):
         self.code = code
 
 
+class Config(ctypes.Structure):
+    """hibf_gpu_config: numeric fields of seqan::hibf::config (config.hpp:111-324)."""
+
+    _fields_ = [
+        ("number_of_user_bins", ctypes.c_uint64),
+        ("number_of_hash_functions", ctypes.c_uint64),
+        ("maximum_fpr", ctypes.c_double),
+        ("relaxed_fpr", ctypes.c_double),
+        ("threads", ctypes.c_uint64),
+        ("sketch_bits", ctypes.c_uint64),
+        ("tmax", ctypes.c_uint64),
+        ("empty_bin_fraction", ctypes.c_double),
+        ("alpha", ctypes.c_double),
+        ("max_rearrangement_ratio", ctypes.c_double),
+        ("disable_estimate_union", ctypes.c_int32),
+        ("disable_rearrangement", ctypes.c_int32),
+        ("track_occupancy", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+    ]
+
+
 class Geometry(ctypes.Structure):
     _fields_ = [
         ("bins", ctypes.c_uint64),
@@ -82,6 +103,13 @@ _SIGNATURES = {
     "hibf_gpu_hibf_arena_words": (_INT, [_V, ctypes.POINTER(_U64)]),
     "hibf_gpu_hibf_membership_for": (_INT, [_V, _V, _V, _V, _U64, _V, _V, _U64, _V, ctypes.POINTER(_U64), _V]),
     "hibf_gpu_hibf_bulk_count": (_INT, [_V, _V, _V, _V, _U64, _V, _U32, _V, _V]),
+    "hibf_gpu_hibf_info": (_INT, [_V] + [ctypes.POINTER(_U64)] * 5),
+    "hibf_gpu_hibf_export_tables": (_INT, [_V, ctypes.POINTER(Geometry), _V, _V, _V, _V]),
+    "hibf_gpu_config_default": (_INT, [ctypes.POINTER(Config)]),
+    "hibf_gpu_config_validate": (_INT, [ctypes.POINTER(Config)]),
+    "hibf_gpu_layout_compute": (_INT, [ctypes.POINTER(Config), _V, ctypes.c_char_p, _U64, ctypes.POINTER(_U64)]),
+    "hibf_gpu_hibf_build": (_INT, [ctypes.POINTER(Config), _V, _V, ctypes.c_char_p, ctypes.POINTER(_V)]),
+    "hibf_gpu_hibf_build_synthetic": (_INT, [ctypes.POINTER(Config), _V, _U64, ctypes.c_char_p, ctypes.POINTER(_V)]),
 }
 
 _lib = None
@@ -116,6 +144,37 @@ def tune_get(key: str) -> int:
     v = ctypes.c_int64()
     check(lib().hibf_gpu_tune_get(key.encode(), ctypes.byref(v)))
     return int(v.value)
+
+
+def make_config(**kw) -> Config:
+    """A hibf_gpu_config with the reference defaults, overridden by keywords."""
+    c = Config()
+    check(lib().hibf_gpu_config_default(ctypes.byref(c)))
+    for k, v in kw.items():
+        if not hasattr(c, k):
+            raise KeyError(f"unknown config field {k}")
+        setattr(c, k, int(v) if isinstance(v, bool) else v)
+    return c
+
+
+def validate_config(c: Config) -> Config:
+    check(lib().hibf_gpu_config_validate(ctypes.byref(c)))
+    return c
+
+
+def compute_layout(c: Config, kmer_counts) -> str:
+    """Layout text (reference format) for the given user-bin cardinalities."""
+    import numpy as np
+    k = np.ascontiguousarray(np.asarray(kmer_counts, dtype=np.uint64))
+    need = ctypes.c_uint64(0)
+    rc = lib().hibf_gpu_layout_compute(ctypes.byref(c), k.ctypes.data_as(ctypes.c_void_p), None, 0,
+                                       ctypes.byref(need))
+    if rc != ERR_CAPACITY:
+        check(rc)
+    buf = ctypes.create_string_buffer(need.value)
+    check(lib().hibf_gpu_layout_compute(ctypes.byref(c), k.ctypes.data_as(ctypes.c_void_p), buf, need.value,
+                                        ctypes.byref(need)))
+    return buf.value.decode()
 
 
 def check(rc: int) -> None:

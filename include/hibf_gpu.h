@@ -240,6 +240,83 @@ int hibf_gpu_hibf_bulk_count(hibf_gpu_hibf hibf,
                              void * d_counts,
                              void * stream);
 
+/* Host-side table export (for checks and persistence): sizes first, then the
+ * arrays hibf_gpu_hibf_create takes.  geoms has n_ibfs entries, word_offsets
+ * n_ibfs, tb_offsets n_ibfs+1, next_ibf and user_bin n_tbs. */
+int hibf_gpu_hibf_info(hibf_gpu_hibf hibf,
+                       uint64_t * n_ibfs,
+                       uint64_t * n_tbs,
+                       uint64_t * n_words,
+                       uint64_t * n_user_bins,
+                       uint64_t * depth);
+int hibf_gpu_hibf_export_tables(hibf_gpu_hibf hibf,
+                                hibf_gpu_ibf_geometry * geoms,
+                                uint64_t * word_offsets,
+                                uint64_t * tb_offsets,
+                                uint64_t * next_ibf,
+                                uint64_t * user_bin);
+
+/* ======================================================================== */
+/* HIBF construction (host layout + device bulk emplace)                    */
+/* ======================================================================== */
+
+/* Numeric fields of seqan::hibf::config (config.hpp:111-324); the reference's
+ * input_fn is replaced by the user-bin contents passed to the build calls. */
+typedef struct hibf_gpu_config
+{
+    uint64_t number_of_user_bins;
+    uint64_t number_of_hash_functions; /* default 2    */
+    double maximum_fpr;                /* default 0.05 */
+    double relaxed_fpr;                /* default 0.3  */
+    uint64_t threads;                  /* default 1    */
+    uint64_t sketch_bits;              /* default 12   */
+    uint64_t tmax;                     /* 0 = next_multiple_of_64(ceil(sqrt(#UB))) */
+    double empty_bin_fraction;         /* default 0    */
+    double alpha;                      /* default 1.2  */
+    double max_rearrangement_ratio;    /* default 0.5  */
+    int32_t disable_estimate_union;
+    int32_t disable_rearrangement;
+    int32_t track_occupancy;
+    int32_t reserved;
+} hibf_gpu_config;
+
+/* The reference's default member initialisers (config.hpp:136-324). */
+int hibf_gpu_config_default(hibf_gpu_config * c);
+/* Replaces config::validate_and_set_defaults (src/config.cpp:64-130):
+ * HIBF_GPU_ERR_CONFIG with the reference's message (hibf_gpu_last_error). */
+int hibf_gpu_config_validate(hibf_gpu_config * c);
+
+/* Replaces layout::compute_layout (src/layout/compute_layout.cpp:26-70) for
+ * the given user-bin cardinalities, with merged-bin cardinalities taken as
+ * sums and no similarity rearrangement.  Writes the reference's layout text
+ * format (src/layout/layout.cpp:125-136); HIBF_GPU_ERR_CAPACITY + *needed
+ * when cap is too small. */
+int hibf_gpu_layout_compute(const hibf_gpu_config * c,
+                            uint64_t const * h_kmer_counts,
+                            char * text,
+                            uint64_t cap,
+                            uint64_t * needed);
+
+/* Replaces hierarchical_interleaved_bloom_filter(config&) and (config&, layout)
+ * (src/hierarchical_interleaved_bloom_filter.cpp:211-253): user bin u holds
+ * h_values[h_ub_offsets[u] .. h_ub_offsets[u+1]) (duplicates ignored).  The
+ * layout is computed unless layout_text (reference layout format) is given.
+ * IBF geometries and tables follow the reference's hierarchical build; the
+ * bit matrices are filled by the device bulk emplace. */
+int hibf_gpu_hibf_build(const hibf_gpu_config * c,
+                        uint64_t const * h_ub_offsets,
+                        uint64_t const * h_values,
+                        const char * layout_text,
+                        hibf_gpu_hibf * out);
+
+/* As hibf_gpu_hibf_build for synthetic user bins: user bin u holds
+ * hibf_gpu_synth_value(seed, u, i) for i < h_ub_sizes[u] (pairwise distinct). */
+int hibf_gpu_hibf_build_synthetic(const hibf_gpu_config * c,
+                                  uint64_t const * h_ub_sizes,
+                                  uint64_t seed,
+                                  const char * layout_text,
+                                  hibf_gpu_hibf * out);
+
 #ifdef __cplusplus
 }
 #endif

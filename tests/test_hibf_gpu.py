@@ -1,0 +1,196 @@
+"""GPU parity of the HIBF path: build (host layout + device bulk emplace),
+membership_for in the reference's depth-first order, and bulk_count, against
+the oracle and the reference's own known answers."""
+import numpy as np
+import pytest
+
+import oracle as O
+from hibf_helpers import expected_arena, oracle_of
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def _csr(qs):
+    from hibf_amd.device import csr
+    return csr(qs)
+
+
+def _membership_lists(dev, qs, thresholds):
+    from hibf_amd.device import to_numpy
+    values, offsets = _csr(qs)
+    ids, hoff = dev.membership_for(values, offsets, thresholds)
+    ids, hoff = to_numpy(ids), to_numpy(hoff)
+    return [ids[hoff[i]:hoff[i + 1]].tolist() for i in range(len(qs))]
+
+
+# ----------------------------------------------------------- goldens ------
+
+def test_snippet_hibf_two_user_bins(goldens):
+    from hibf_amd.device import DeviceHibf
+    _say("hibf snippet")
+    case = goldens["hibf_two_user_bins"]
+    dev = DeviceHibf.build(case["user_bins"])
+    for q in case["queries"]:
+        got = _membership_lists(dev, [q["values"]], [q["threshold"]])[0]
+        assert got == q["expected"]
+        if "expected_sorted" in q:
+            assert sorted(got) == q["expected_sorted"]
+
+
+def test_readme_golden(goldens):
+    from hibf_amd.device import DeviceHibf
+    _say("readme")
+    case = goldens["readme"]
+    dev = DeviceHibf.build(case["user_bins"], **case["config"])
+    q1, q2 = case["queries"]
+    assert _membership_lists(dev, [q1["values"]], [q1["threshold"]])[0] == q1["expected"]
+    r2 = _membership_lists(dev, [list(range(*q2["range"]))], [q2["threshold"]])[0]
+    assert sorted(r2) == q2["expected_sorted"]
+
+
+def test_build_from_layout_text(goldens):
+    from hibf_amd.device import DeviceHibf
+    _say("build from layout")
+    case = goldens["layout_two_user_bins"]
+    lines = [f"#TOP_LEVEL_IBF fullest_technical_bin_idx:{case['top_level_max_bin_id']}",
+             "#USER_BIN_IDX\tTECHNICAL_BIN_INDICES\tNUMBER_OF_TECHNICAL_BINS"]
+    for r in case["user_bin_records"]:
+        lines.append(f"{r['idx']}\t{r['storage_TB_id']}\t{r['number_of_technical_bins']}")
+    dev = DeviceHibf.build(case["user_bins"], layout_text="\n".join(lines) + "\n", **case["config"])
+    t = dev.tables()
+    assert t["geoms"][0][0] == 64
+    assert t["user_bin"].tolist() == [1] * 34 + [0] * 30
+    got = _membership_lists(dev, [case["query"]], [case["threshold"]])[0]
+    assert sorted(got) == case["expected_sorted"]
+
+
+# ---------------------------------- reference unit-test expectations ------
+
+def test_three_level_hibf():
+    """hierarchical_interleaved_bloom_filter_test.cpp:112-154."""
+    from hibf_amd.device import DeviceHibf
+    _say("three level")
+    ubs = [np.arange(u * 16, u * 16 + 20, dtype=np.uint64) for u in range(4097)]
+    dev = DeviceHibf.build(ubs, maximum_fpr=0.001, threads=4, tmax=64, disable_estimate_union=True,
+                           disable_rearrangement=True)
+    assert dev.tables()["depth"] >= 3
+    qs, ts, want = [], [], []
+    for ub in range(0, 4096, 100):
+        s = ub * 16
+        qs.append(np.arange(s + 16, s + 20, dtype=np.uint64))
+        want.append([ub, ub + 1])
+        qs.append(np.arange(s + 5, s + 9, dtype=np.uint64))
+        want.append([ub])
+    got = _membership_lists(dev, qs, 4)
+    assert [sorted(g) for g in got] == want
+    ref = oracle_of(dev)
+    for q, g in zip(qs, got):
+        assert g == ref.membership_for(q, 4)
+
+
+def test_unevenly_sized_and_similar_user_bins():
+    """hierarchical_interleaved_bloom_filter_test.cpp:164-231."""
+    from hibf_amd.device import DeviceHibf
+    _say("uneven + similar")
+    opts = dict(maximum_fpr=0.001, threads=4, tmax=64, disable_estimate_union=True, disable_rearrangement=True)
+    ubs = [np.arange((u + 1) ** 2, (u + 2) ** 2 - 1, dtype=np.uint64) for u in range(500)]
+    dev = DeviceHibf.build(ubs, **opts)
+    qs = [np.arange((u + 1) ** 2 + 5, (u + 1) ** 2 + 9, dtype=np.uint64) for u in range(5, 499, 20)]
+    assert _membership_lists(dev, qs, 4) == [[u] for u in range(5, 499, 20)]
+    ubs = [np.arange(u * 5, u * 5 + 35, dtype=np.uint64) for u in range(1000)]
+    dev = DeviceHibf.build(ubs, **opts)
+    qs = [np.arange(u * 5, u * 5 + 5, dtype=np.uint64) for u in range(6, 994, 20)]
+    got = _membership_lists(dev, qs, 5)
+    assert [sorted(g) for g in got] == [list(range(u - 6, u + 1)) for u in range(6, 994, 20)]
+
+
+def test_counting_agent_reference_bounds():
+    """hierarchical_interleaved_bloom_filter_test.cpp:233-295."""
+    from hibf_amd.device import DeviceHibf, to_numpy
+    _say("counting bounds")
+    opts = dict(maximum_fpr=0.001, threads=4, tmax=64, disable_estimate_union=True, disable_rearrangement=True)
+    dev = DeviceHibf.build([np.arange(500, dtype=np.uint64)] * 500, **opts)
+    values, offsets = _csr([np.arange(500, dtype=np.uint64)])
+    c = to_numpy(dev.bulk_count(values, offsets, 1, counter_bytes=4))[0]
+    assert (c >= 500).all() and (c <= 502).all()
+    ref = oracle_of(dev)
+    assert np.array_equal(c, ref.bulk_count(np.arange(500, dtype=np.uint64), 1, 4))
+    ubs = [np.arange((u + 1) ** 2, (u + 2) ** 2 - 1, dtype=np.uint64) for u in range(500)]
+    dev = DeviceHibf.build(ubs, **opts)
+    q = np.arange(1, 251_000, dtype=np.uint64)
+    values, offsets = _csr([q])
+    c = to_numpy(dev.bulk_count(values, offsets, 1, counter_bytes=4))[0]
+    assert all(c[u] >= 2 * u + 2 for u in range(500))
+    assert np.array_equal(c, oracle_of(dev).bulk_count(q, 1, 4))
+
+
+# ------------------------------------------------- parity vs oracle -------
+
+def _random_user_bins(rng, n, lo, hi):
+    sizes = rng.integers(lo, hi, size=n)
+    return [rng.integers(0, 2**63, size=int(s), dtype=np.int64).astype(np.uint64) for s in sizes]
+
+
+def _mixed_queries(rng, ubs, n, length):
+    qs = []
+    for _ in range(n):
+        k = int(rng.integers(1, 4))
+        src = rng.integers(0, len(ubs), size=k)
+        parts = [ubs[s][rng.integers(0, len(ubs[s]), size=int(length * 0.8) // k)] for s in src]
+        noise = rng.integers(0, 2**63, size=length - sum(len(p) for p in parts), dtype=np.int64).astype(np.uint64)
+        qs.append(np.concatenate(parts + [noise]))
+    return qs
+
+
+CASES = [  # (n_ub, size range, config)
+    (2, (10, 40), dict()),
+    (64, (50, 400), dict(number_of_hash_functions=2)),
+    (70, (50, 400), dict(tmax=64, disable_estimate_union=True)),
+    (300, (20, 300), dict(tmax=64, number_of_hash_functions=3)),
+    (900, (5, 200), dict(tmax=64, maximum_fpr=0.01, number_of_hash_functions=1)),
+    (5000, (2, 60), dict(tmax=64, relaxed_fpr=0.5)),
+]
+
+
+@pytest.mark.parametrize("n_ub,sizes,config", CASES)
+def test_hibf_parity_random(n_ub, sizes, config):
+    from hibf_amd.device import DeviceHibf, to_numpy
+    _say(f"hibf parity n_ub={n_ub} {config}")
+    rng = np.random.default_rng(n_ub)
+    ubs = _random_user_bins(rng, n_ub, *sizes)
+    dev = DeviceHibf.build(ubs, **config)
+    assert np.array_equal(dev.download_arena(), expected_arena(dev, ubs)), "bit matrices differ from the plan"
+    ref = oracle_of(dev)
+    qs = _mixed_queries(rng, ubs, 60, 120) + [np.zeros(0, dtype=np.uint64), ubs[0][:1]]
+    thresholds = [int(t) for t in rng.integers(0, 60, size=len(qs))]
+    got = _membership_lists(dev, qs, thresholds)
+    for q, t, g in zip(qs, thresholds, got):
+        assert g == ref.membership_for(q, t)
+    values, offsets = _csr(qs)
+    cthr = [max(1, t) for t in thresholds]
+    for width in (1, 2, 4, 8):
+        c = to_numpy(dev.bulk_count(values, offsets, cthr, counter_bytes=width))
+        assert np.array_equal(c, ref.bulk_count_batch(values, offsets, cthr, width)), width
+
+
+def test_hibf_synthetic_build_matches_oracle():
+    from hibf_amd.device import DeviceHibf, to_numpy
+    from hibf_amd.synth import synth_values
+    _say("synthetic build")
+    rng = np.random.default_rng(3)
+    sizes = np.clip(rng.lognormal(np.log(300), 1.0, size=700).astype(np.int64), 20, 5000)
+    seed = 77
+    dev = DeviceHibf.build_synthetic(sizes, seed, tmax=64)
+    ubs = [synth_values(seed, u, np.arange(s, dtype=np.uint64)) for u, s in enumerate(sizes)]
+    # split bins are chunked in generation order (the stream is distinct)
+    assert np.array_equal(dev.download_arena(), expected_arena(dev, ubs, keep_order=True))
+    ref = oracle_of(dev)
+    qs = _mixed_queries(rng, ubs, 80, 200)
+    ts = [int(0.7 * len(q)) // 2 for q in qs]
+    assert _membership_lists(dev, qs, ts) == [ref.membership_for(q, t) for q, t in zip(qs, ts)]
+    values, offsets = _csr(qs)
+    assert np.array_equal(to_numpy(dev.bulk_count(values, offsets, ts)), ref.bulk_count_batch(values, offsets, ts))
