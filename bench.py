@@ -28,6 +28,13 @@ HBM_PEAK_GBS = 8000.0
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--workload", choices=["flat", "hibf"], default="flat",
+                    help="flat: 8,192-bin flat IBF bulk_count (default); hibf: 32,768-user-bin HIBF membership "
+                         "(fixed-length batch, log-normal batched hipGraph form, plain per-query form)")
+    ap.add_argument("--hibf-queries", type=int, default=500_000)
+    ap.add_argument("--hibf-batched-queries", type=int, default=2_000_000)
+    ap.add_argument("--graph-batch", type=int, default=65536)
+    ap.add_argument("--hibf-user-bins", type=int, default=32768)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--queries", type=int, default=1_000_000)
@@ -59,7 +66,13 @@ def main():
         print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported (the query path runs on a "
               "single MI355X; sharding across GPUs is not part of this build).", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "flat":
+        run_flat(args)
+    else:
+        run_hibf(args)
 
+
+def run_flat(args):
     import torch
     from hibf_amd.device import DeviceIbf, as_device_u64
     from hibf_amd.synth import flat_ibf_queries
@@ -210,6 +223,213 @@ def main():
         "batched_vs_plain": value / plain_value,
         "parity_sample_ok": parity,
         "build_emplace_s": build_s,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def run_hibf(args):
+    """32,768-user-bin HIBF: (a) 500k x 250 queries, t = 0.7 L, one persistent
+    launch; (b) 2M queries of log-normal length in hipGraph-captured batches of
+    64k; (c) the plain one-launch-per-query form on a sample of (b)."""
+    import torch
+    from hibf_amd import native
+    from hibf_amd import workloads as W
+    from hibf_amd.device import Agent, DeviceHibf, as_device_u64
+
+    for kv in args.tune:
+        k, v = kv.split("=")
+        native.tune_set(k, int(v))
+    torch.cuda.set_device(0)
+    dev = torch.device("cuda", 0)
+    spec = W.HIBF_AT_SCALE
+    n_ub = args.hibf_user_bins
+    sizes = W.hibf_user_bin_sizes(spec["size_seed"], n_ub)
+    t0 = time.perf_counter()
+    layout = W.cached_layout(sizes, spec["config"])
+    layout_s = time.perf_counter() - t0
+    log(f"layout ({n_ub} user bins, {sizes.sum():.3e} values): {layout_s:.1f} s")
+    t0 = time.perf_counter()
+    hibf = DeviceHibf.build_synthetic(sizes, spec["build_seed"], layout_text=layout, **spec["config"])
+    build_s = time.perf_counter() - t0
+    tab = hibf.tables()
+    log(f"build: {len(tab['geoms'])} IBFs, {tab['n_words'] * 8 / 1e9:.2f} GB, depth {tab['depth']}, {build_s:.1f} s")
+    agent = Agent()
+    stream = torch.cuda.current_stream()
+
+    def thresholds_of(offsets_h):
+        lens = np.diff(offsets_h.astype(np.int64))
+        return np.minimum((0.7 * lens).astype(np.int64), 65535).astype(np.uint16)
+
+    def upload_u16(a):
+        return torch.from_numpy(np.ascontiguousarray(a).view(np.int16)).to(dev).view(torch.uint16)
+
+    # (a) fixed-length batch, one persistent launch per step.
+    nq = args.hibf_queries
+    lengths = W.query_lengths(args.seed + 3, nq, fixed=250)
+    va_h, oa_h, _ = W.hibf_queries(args.seed + 4, sizes, spec["build_seed"], lengths, 0.8)
+    ta_h = thresholds_of(oa_h)
+    va, oa, ta = as_device_u64(va_h), as_device_u64(oa_h), upload_u16(ta_h)
+    cap = 8 * nq
+    ids = torch.empty(cap, dtype=torch.uint64, device=dev)
+    hs = torch.empty(nq, dtype=torch.uint64, device=dev)
+    hc = torch.empty(nq, dtype=torch.uint64, device=dev)
+    hibf.membership_prepare(agent, nq)
+    hibf.membership_launch(agent, va, oa, ta, ids, hs, hc)
+    torch.cuda.synchronize()
+    total, dropped = agent.status()
+    assert not dropped, "output capacity too small"
+    agent.stats(reset=True)
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    for a, b in evs:
+        a.record(stream)
+        hibf.membership_launch(agent, va, oa, ta, ids, hs, hc)
+        b.record(stream)
+    torch.cuda.synchronize()
+    wall_a = (time.perf_counter() - t0) / args.steps
+    kern_a = float(np.mean([a.elapsed_time(b) for a, b in evs])) / 1e3
+    st = agent.stats(reset=True)
+    lookups_a = st["lookups"] / args.steps
+    bytes_a = st["row_bytes"] / args.steps
+    log(f"(a) {nq} x 250: {wall_a * 1e3:.2f} ms/step, {lookups_a / wall_a / 1e9:.3f} G lookups/s, "
+        f"{bytes_a / kern_a / 1e9:.0f} GB/s rows, {st['visits'] / args.steps / nq:.2f} IBFs/query, "
+        f"{total / nq:.3f} hits/query")
+
+    # (b) log-normal lengths, hipGraph batches.
+    nb = args.hibf_batched_queries
+    lengths_b = W.query_lengths(args.seed + 5, nb)
+    vb_h, ob_h, _ = W.hibf_queries(args.seed + 6, sizes, spec["build_seed"], lengths_b, 0.6)
+    tb_h = thresholds_of(ob_h)
+    vb, ob, tb = as_device_u64(vb_h), as_device_u64(ob_h), upload_u16(tb_h)
+    G = args.graph_batch
+    nbatches = (nb + G - 1) // G
+    cap_b = 8 * G
+    ids_b = torch.empty(nbatches * cap_b, dtype=torch.uint64, device=dev)
+    hs_b = torch.empty(nb, dtype=torch.uint64, device=dev)
+    hc_b = torch.empty(nb, dtype=torch.uint64, device=dev)
+    hibf.membership_prepare(agent, G)
+    graphs = []
+    for b in range(nbatches):
+        q0, q1 = b * G, min(nb, (b + 1) * G)
+        args_b = (agent, vb, ob[q0:q1 + 1], tb[q0:q1], ids_b[b * cap_b:(b + 1) * cap_b], hs_b[q0:q1], hc_b[q0:q1])
+        hibf.membership_launch(*args_b)  # warm the capture inputs once outside the graph
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            hibf.membership_launch(*args_b)
+        graphs.append(g)
+    torch.cuda.synchronize()
+    agent.stats(reset=True)
+    for g in graphs:  # warm replay
+        g.replay()
+    torch.cuda.synchronize()
+    _, dropped = agent.status()
+    assert not dropped, "batch output capacity too small"
+    agent.stats(reset=True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        for g in graphs:
+            g.replay()
+    torch.cuda.synchronize()
+    wall_b = (time.perf_counter() - t0) / args.steps
+    st = agent.stats(reset=True)
+    lookups_b = st["lookups"] / args.steps
+    bytes_b = st["row_bytes"] / args.steps
+    log(f"(b) {nb} log-normal queries ({int(ob_h[-1]) / nb:.0f} values/query), {nbatches} graphs: "
+        f"{wall_b * 1e3:.1f} ms/step, {lookups_b / wall_b / 1e9:.3f} G lookups/s, {bytes_b / wall_b / 1e9:.0f} GB/s rows")
+
+    # (c) plain form: one launch per query of (b), back to back.
+    npl = min(args.plain_queries, nb)
+    hibf.membership_prepare(agent, 1)
+    ids1 = torch.empty(1 << 16, dtype=torch.uint64, device=dev)
+    torch.cuda.synchronize()
+    agent.stats(reset=True)
+    t0 = time.perf_counter()
+    for q in range(npl):
+        hibf.membership_launch(agent, vb, ob[q:q + 2], tb[q:q + 1], ids1, hs_b[q:q + 1], hc_b[q:q + 1])
+    torch.cuda.synchronize()
+    wall_c = time.perf_counter() - t0
+    st = agent.stats(reset=True)
+    plain = st["lookups"] / wall_c
+    lat = []
+    for q in range(min(300, npl)):
+        t1 = time.perf_counter()
+        hibf.membership_launch(agent, vb, ob[q:q + 2], tb[q:q + 1], ids1, hs_b[q:q + 1], hc_b[q:q + 1])
+        torch.cuda.synchronize()
+        lat.append((time.perf_counter() - t1) * 1e3)
+    log(f"(c) plain per-query: {plain / 1e9:.4f} G lookups/s; latency p50 {np.percentile(lat, 50):.3f} ms "
+        f"p99 {np.percentile(lat, 99):.3f} ms; batched/plain = {lookups_b / wall_b / plain:.1f}")
+
+    # Parity on samples of (a) and (b) against the oracle on the downloaded arena, and the CPU baseline.
+    parity = None
+    cpu = None
+    if args.check_queries > 0 or not args.skip_cpu:
+        import oracle as O
+        from hibf_amd.device import to_numpy
+        arena = hibf.download_arena()
+        geoms = [O.geom(b_, s_, h_) for (b_, s_, h_) in tab["geoms"]]
+        ref = O.Hibf(geoms, arena, tab["word_offsets"], tab["tb_offsets"], tab["next_ibf"], tab["user_bin"], n_ub)
+        rng = np.random.default_rng(args.seed + 9)
+        ok = True
+        # (a) outputs are from the last timed step; (b) from the last replay.
+        for v_h, o_h, t_h, ids_t, hs_t, hc_t, n, per_batch in (
+                (va_h, oa_h, ta_h, ids, hs, hc, nq, None), (vb_h, ob_h, tb_h, ids_b, hs_b, hc_b, nb, G)):
+            ids_n, hs_n, hc_n = to_numpy(ids_t), to_numpy(hs_t), to_numpy(hc_t)
+            for q in rng.choice(n, size=min(args.check_queries, n), replace=False):
+                base = 0 if per_batch is None else (q // per_batch) * cap_b
+                got = ids_n[base + hs_n[q]: base + hs_n[q] + hc_n[q]].tolist()
+                want = ref.membership_for(v_h[o_h[q]:o_h[q + 1]], int(t_h[q]))
+                if got != want:
+                    ok = False
+                    log(f"parity mismatch at query {q}: {got} vs {want}")
+                    break
+        parity = ok
+        log(f"parity vs oracle on {2 * args.check_queries} sampled queries: {parity}")
+        if not args.skip_cpu:
+            threads = max(1, min(16, os.cpu_count() or 1))
+            probe = 200
+            t1 = time.perf_counter()
+            ref.membership_hits_batch(va_h[: probe * 250], oa_h[: probe + 1], ta_h[:probe], threads)
+            rate = probe / max(time.perf_counter() - t1, 1e-6)
+            n_cpu = int(min(nq, max(probe, rate * args.cpu_seconds)))
+            t1 = time.perf_counter()
+            ref.membership_hits_batch(va_h[: n_cpu * 250], oa_h[: n_cpu + 1], ta_h[:n_cpu], threads)
+            cpu_s = time.perf_counter() - t1
+            cpu = {"value": n_cpu * 250 / cpu_s, "unit": "query values/s", "cores": threads, "kind": "port",
+                   "sample": f"first {n_cpu} queries of (a), oracle membership_for with {threads} threads, "
+                             f"{cpu_s:.1f} s (values/s, each value checked in every visited IBF)"}
+            log(f"cpu baseline: {cpu['value'] / 1e6:.2f} M query values/s on {threads} threads")
+        del arena, ref
+
+    out = {
+        "metric": "k-mer lookups/s (HIBF membership_for, batched)",
+        "value": lookups_b / wall_b,
+        "unit": "lookups/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": 1,
+        "ms_per_step": wall_b * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64 hash / u16 counters",
+        "data": "synthetic (seeded generator values, uniform uint64 noise)",
+        "config": {"workload": f"HIBF {n_ub} user bins (log-normal sizes, median 1e5, sigma 1.5), tmax 256, h=2, "
+                               f"FPR 0.05/0.3; {nb} queries, log-normal length median 300 [50, 5000], 60% from one "
+                               f"user bin, t = 0.7 L; hipGraph batches of {G}",
+                   "ibfs": len(tab["geoms"]), "arena_gb": tab["n_words"] * 8 / 1e9, "depth": tab["depth"]},
+        "roofline": {"bound": "hbm", "achieved": bytes_b / wall_b / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": bytes_b / wall_b / 1e9 / HBM_PEAK_GBS, "traffic": None},
+        "fixed_batch": {"queries": nq, "length": 250, "ms": wall_a * 1e3, "lookups_per_s": lookups_a / wall_a,
+                        "row_gbs": bytes_a / kern_a / 1e9, "kernel_ms": kern_a * 1e3,
+                        "ibfs_per_query": st and None},
+        "plain_per_query": {"value": plain, "unit": "lookups/s", "queries": npl,
+                            "latency_ms_p50": float(np.percentile(lat, 50)),
+                            "latency_ms_p99": float(np.percentile(lat, 99))},
+        "batched_vs_plain": lookups_b / wall_b / plain,
+        "cpu_baseline": cpu,
+        "parity_sample_ok": parity,
+        "build_s": build_s,
+        "layout_s": layout_s,
     }
     print(json.dumps(out), flush=True)
 

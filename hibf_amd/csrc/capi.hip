@@ -145,8 +145,8 @@ struct hibf_gpu_agent_s
     size_t cub_temp_bytes{0};
     void * counts{nullptr}; // IBF membership: uint16 counts of one chunk
     size_t counts_bytes{0};
-    unsigned long long * d_total{nullptr};
-    unsigned long long * h_total{nullptr}; // pinned
+    unsigned long long * d_total{nullptr}; // 8 counters, see HibfLaunch::counters
+    unsigned long long * h_total{nullptr}; // pinned, 8 entries
 };
 
 struct hibf_gpu_hibf_s
@@ -406,9 +406,11 @@ int hibf_gpu_agent_create(uint64_t max_queries_per_launch, hibf_gpu_agent * out)
     if (!a)
         return fail(HIBF_GPU_ERR_OUT_OF_MEMORY, "host allocation failed");
     a->max_queries = max_queries_per_launch ? max_queries_per_launch : 65536u;
-    hipError_t e = hipMalloc(reinterpret_cast<void **>(&a->d_total), sizeof(unsigned long long));
+    hipError_t e = hipMalloc(reinterpret_cast<void **>(&a->d_total), 8 * sizeof(unsigned long long));
     if (e == hipSuccess)
-        e = hipHostMalloc(reinterpret_cast<void **>(&a->h_total), sizeof(unsigned long long), hipHostMallocDefault);
+        e = hipMemset(a->d_total, 0, 8 * sizeof(unsigned long long));
+    if (e == hipSuccess)
+        e = hipHostMalloc(reinterpret_cast<void **>(&a->h_total), 8 * sizeof(unsigned long long), hipHostMallocDefault);
     if (e != hipSuccess)
     {
         if (a->d_total)
@@ -431,6 +433,36 @@ int hibf_gpu_agent_destroy(hibf_gpu_agent a)
     if (a->h_total)
         (void)hipHostFree(a->h_total);
     delete a;
+    return HIBF_GPU_OK;
+}
+
+int hibf_gpu_agent_stats(hibf_gpu_agent a, uint64_t * visits, uint64_t * lookups, uint64_t * row_bytes, int reset)
+{
+    if (!a)
+        return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "null agent");
+    HIBF_TRY_HIP(hipDeviceSynchronize(), "stats sync");
+    HIBF_TRY_HIP(hipMemcpy(a->h_total, a->d_total, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost), "stats copy");
+    if (visits)
+        *visits = a->h_total[2];
+    if (row_bytes)
+        *row_bytes = a->h_total[3];
+    if (lookups)
+        *lookups = a->h_total[4];
+    if (reset)
+        HIBF_TRY_HIP(hipMemset(a->d_total + 2, 0, 4 * sizeof(unsigned long long)), "stats reset");
+    return HIBF_GPU_OK;
+}
+
+int hibf_gpu_agent_status(hibf_gpu_agent a, uint64_t * last_total, uint64_t * dropped)
+{
+    if (!a)
+        return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "null agent");
+    HIBF_TRY_HIP(hipDeviceSynchronize(), "status sync");
+    HIBF_TRY_HIP(hipMemcpy(a->h_total, a->d_total, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost), "status copy");
+    if (last_total)
+        *last_total = a->h_total[0];
+    if (dropped)
+        *dropped = a->h_total[5];
     return HIBF_GPU_OK;
 }
 
@@ -747,9 +779,9 @@ int hibf_gpu_hibf_download_arena(hibf_gpu_hibf h, uint64_t * h_arena, uint64_t n
 
 namespace
 {
-HibfLaunch launch_of(hibf_gpu_hibf h)
+HibfLaunch launch_of(hibf_gpu_hibf h, hibf_gpu_agent a)
 {
-    return HibfLaunch{h->d_desc, h->d_arena, h->d_tb_info, h->max_bins, h->depth, h->n_user_bins};
+    return HibfLaunch{h->d_desc, h->d_arena, h->d_tb_info, h->max_bins, h->depth, h->n_user_bins, a->d_total};
 }
 
 unsigned hibf_blocks(uint64_t nq, uint64_t per_block_scratch_bytes)
@@ -761,6 +793,47 @@ unsigned hibf_blocks(uint64_t nq, uint64_t per_block_scratch_bytes)
     return static_cast<unsigned>(std::max<uint64_t>(blocks, 1u));
 }
 } // namespace
+
+int hibf_gpu_hibf_membership_prepare(hibf_gpu_hibf h, hibf_gpu_agent a, uint64_t max_queries)
+{
+    if (!h || !a)
+        return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "null argument");
+    uint64_t const per_block = h->n_user_bins * sizeof(uint32_t);
+    unsigned const blocks = hibf_blocks(std::max<uint64_t>(max_queries, 1), per_block);
+    if (int rc = ensure(a->scratch, a->scratch_bytes, blocks * per_block, "membership scratch"))
+        return rc;
+    HIBF_TRY_HIP(hipMemset(a->d_total + 5, 0, sizeof(unsigned long long)), "flag reset");
+    return HIBF_GPU_OK;
+}
+
+int hibf_gpu_hibf_membership_launch(hibf_gpu_hibf h,
+                                    hibf_gpu_agent a,
+                                    uint64_t const * d_values,
+                                    uint64_t const * d_offsets,
+                                    uint64_t nq,
+                                    uint16_t const * d_thresholds,
+                                    uint64_t * d_ids,
+                                    uint64_t ids_cap,
+                                    uint64_t * d_hit_start,
+                                    uint64_t * d_hit_count,
+                                    void * stream)
+{
+    if (!h || !a || ((!d_offsets || !d_thresholds || !d_hit_start || !d_hit_count) && nq))
+        return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "null argument");
+    if (nq == 0)
+        return HIBF_GPU_OK;
+    uint64_t const per_block = h->n_user_bins * sizeof(uint32_t);
+    unsigned const blocks = hibf_blocks(nq, per_block);
+    if (a->scratch_bytes < blocks * per_block)
+        return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "agent not prepared for this batch size (hibf_gpu_hibf_membership_prepare)");
+    hipStream_t const s = static_cast<hipStream_t>(stream);
+    HIBF_TRY_HIP(launch_reset_counters(a->d_total, 2u, s), "counter reset");
+    HIBF_TRY_HIP(launch_hibf_membership(launch_of(h, a), d_values, d_offsets, nq, d_thresholds,
+                                        static_cast<uint32_t *>(a->scratch), blocks, d_ids, d_ids ? ids_cap : 0,
+                                        d_hit_start, d_hit_count, s),
+                 "membership launch");
+    return HIBF_GPU_OK;
+}
 
 int hibf_gpu_hibf_membership_for(hibf_gpu_hibf h,
                                  hibf_gpu_agent a,
@@ -791,16 +864,16 @@ int hibf_gpu_hibf_membership_for(hibf_gpu_hibf h,
     if (!a->ids_tmp)
         if (int rc = ensure(a->ids_tmp, a->ids_tmp_bytes, std::max<uint64_t>(nq, 1024u) * 8u, "membership ids"))
             return rc;
-    HibfLaunch const L = launch_of(h);
+    HibfLaunch const L = launch_of(h, a);
     for (int attempt = 0; attempt < 2; ++attempt)
     {
         uint64_t const cap = a->ids_tmp_bytes / 8u;
-        HIBF_TRY_HIP(hipMemsetAsync(a->d_total, 0, sizeof(unsigned long long), s), "total reset");
+        HIBF_TRY_HIP(launch_reset_counters(a->d_total, 2u, s), "counter reset");
         HIBF_TRY_HIP(launch_hibf_membership(L, d_values, d_offsets, nq, d_thresholds,
                                             static_cast<uint32_t *>(a->scratch), blocks,
                                             static_cast<uint64_t *>(a->ids_tmp), cap,
                                             static_cast<uint64_t *>(a->hit_start),
-                                            static_cast<uint64_t *>(a->hit_count), a->d_total, s),
+                                            static_cast<uint64_t *>(a->hit_count), s),
                      "membership launch");
         HIBF_TRY_HIP(hipMemcpyAsync(a->h_total, a->d_total, sizeof(unsigned long long), hipMemcpyDeviceToHost, s),
                      "total copy");
@@ -842,8 +915,10 @@ int hibf_gpu_hibf_bulk_count(hibf_gpu_hibf h,
         return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "counter_bytes must be 1, 2, 4 or 8");
     if (nq == 0)
         return HIBF_GPU_OK;
-    HIBF_TRY_HIP(launch_hibf_count(launch_of(h), d_values, d_offsets, nq, d_thresholds, counter_bytes, d_counts,
-                                   hibf_blocks(nq, 0), static_cast<hipStream_t>(stream)),
+    hipStream_t const s = static_cast<hipStream_t>(stream);
+    HIBF_TRY_HIP(launch_reset_counters(a->d_total + 1, 1u, s), "counter reset");
+    HIBF_TRY_HIP(launch_hibf_count(launch_of(h, a), d_values, d_offsets, nq, d_thresholds, counter_bytes, d_counts,
+                                   hibf_blocks(nq, 0), s),
                  "hibf count launch");
     return HIBF_GPU_OK;
 }
@@ -1081,6 +1156,57 @@ int hibf_gpu_hibf_build(hibf_gpu_config const * c,
         if (!pe.empty())
             return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, pe);
         return run_plan(plan, nub, offs.data(), vals.data(), 0, out);
+    }
+    catch (std::exception const & ex)
+    {
+        return fail(HIBF_GPU_ERR_OUT_OF_MEMORY, ex.what());
+    }
+}
+
+int hibf_gpu_hibf_plan_stats(hibf_gpu_config const * c,
+                             uint64_t const * h_ub_sizes,
+                             char const * layout_text,
+                             uint64_t * n_ibfs,
+                             uint64_t * n_words,
+                             uint64_t * n_segments,
+                             uint64_t * n_emplaces)
+{
+    if (!c || !h_ub_sizes)
+        return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "null argument");
+    host::Config hc = to_host(*c);
+    std::string const e = host::validate_and_set_defaults(hc);
+    if (!e.empty())
+        return fail(HIBF_GPU_ERR_CONFIG, e);
+    try
+    {
+        host::BuildInput in;
+        in.ub_size.assign(h_ub_sizes, h_ub_sizes + hc.number_of_user_bins);
+        host::Layout layout;
+        if (int rc = get_layout(hc, in.ub_size, layout_text, layout))
+            return rc;
+        host::Plan plan;
+        std::string const pe = host::build_plan(hc, layout, in, plan);
+        if (!pe.empty())
+            return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, pe);
+        uint64_t words = 0, emplaces = 0;
+        for (auto const & p : plan.ibfs)
+        {
+            hibf_gpu_ibf_geometry g{};
+            if (int rc = make_geometry(p.bins, p.bin_size, p.hash_funs, g))
+                return rc;
+            words += (g.n_words + 31u) / 32u * 32u;
+        }
+        for (auto const & s : plan.segments)
+            emplaces += s.count;
+        if (n_ibfs)
+            *n_ibfs = plan.ibfs.size();
+        if (n_words)
+            *n_words = words;
+        if (n_segments)
+            *n_segments = plan.segments.size();
+        if (n_emplaces)
+            *n_emplaces = emplaces;
+        return HIBF_GPU_OK;
     }
     catch (std::exception const & ex)
     {

@@ -121,11 +121,12 @@ __global__ void __launch_bounds__(256) hibf_descent_kernel(HibfLaunch h,
                                                            uint64_t * __restrict__ ids,
                                                            uint64_t ids_cap,
                                                            uint64_t * __restrict__ hit_start,
-                                                           uint64_t * __restrict__ hit_count,
-                                                           unsigned long long * __restrict__ total)
+                                                           uint64_t * __restrict__ hit_count)
 {
     extern __shared__ uint32_t smem[];
     __shared__ DescentState st;
+    __shared__ uint64_t next_q;
+    unsigned long long * const total = h.counters;
     uint32_t * const cnt = smem;              // max_bins counters
     uint32_t * const evs = smem + h.max_bins; // depth x max_bins event lists
 
@@ -136,8 +137,19 @@ __global__ void __launch_bounds__(256) hibf_descent_kernel(HibfLaunch h,
     int const nwaves = nthreads >> 6;
     uint32_t * const my_scratch = COUNTING ? nullptr : scratch + static_cast<uint64_t>(blockIdx.x) * h.n_user_bins;
 
-    for (uint64_t q = blockIdx.x; q < nq; q += gridDim.x)
+    // Work stealing: each block takes the next query from the agent's cursor.
+    uint64_t visits = 0;
+    uint64_t row_bytes = 0;
+    uint64_t lookups = 0;
+    while (true)
     {
+        if (tid == 0)
+            next_q = atomicAdd(&h.counters[1], 1ULL);
+        __syncthreads();
+        uint64_t const q = next_q;
+        __syncthreads();
+        if (q >= nq)
+            break;
         uint64_t const beg = offsets[q];
         uint32_t const L = static_cast<uint32_t>(offsets[q + 1] - beg);
         uint64_t const * const vals = values + beg;
@@ -167,6 +179,9 @@ __global__ void __launch_bounds__(256) hibf_descent_kernel(HibfLaunch h,
                 __syncthreads();
                 IbfView const g{h.arena + d.word_offset, d.bin_size, d.bins, d.bin_words, d.hash_shift, d.hash_funs};
                 count_into_lds_dispatch<2>(d.lpv_log, g, vals, L, cnt, wave, nwaves, lane);
+                ++visits;
+                lookups += L;
+                row_bytes += static_cast<uint64_t>(L) * d.hash_funs * d.bin_words * 8u;
                 __syncthreads();
                 if (wave == 0)
                 {
@@ -227,6 +242,8 @@ __global__ void __launch_bounds__(256) hibf_descent_kernel(HibfLaunch h,
                 st.start = s;
                 hit_start[q] = s;
                 hit_count[q] = n;
+                if (s + n > ids_cap)
+                    h.counters[5] = 1ULL;
             }
             __syncthreads();
             uint32_t const n = st.hits;
@@ -237,6 +254,18 @@ __global__ void __launch_bounds__(256) hibf_descent_kernel(HibfLaunch h,
         }
         __syncthreads();
     }
+    if (tid == 0 && visits)
+    {
+        atomicAdd(&h.counters[2], static_cast<unsigned long long>(visits));
+        atomicAdd(&h.counters[3], static_cast<unsigned long long>(row_bytes));
+        atomicAdd(&h.counters[4], static_cast<unsigned long long>(lookups));
+    }
+}
+
+__global__ void reset_counters_kernel(unsigned long long * counters, uint32_t n)
+{
+    if (threadIdx.x < n)
+        counters[threadIdx.x] = 0ULL;
 }
 
 __global__ void __launch_bounds__(256) gather_hits_kernel(uint64_t const * __restrict__ src,
@@ -333,6 +362,12 @@ hipError_t launch_finalize_offsets(uint64_t const * incl, uint64_t n, uint64_t b
     return hipGetLastError();
 }
 
+hipError_t launch_reset_counters(unsigned long long * counters, uint32_t n, hipStream_t stream)
+{
+    hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(64), 0, stream, counters, n);
+    return hipGetLastError();
+}
+
 hipError_t launch_hibf_membership(HibfLaunch const & h,
                                   uint64_t const * values,
                                   uint64_t const * offsets,
@@ -344,14 +379,13 @@ hipError_t launch_hibf_membership(HibfLaunch const & h,
                                   uint64_t ids_cap,
                                   uint64_t * hit_start,
                                   uint64_t * hit_count,
-                                  unsigned long long * total,
                                   hipStream_t stream)
 {
     if (nq == 0)
         return hipSuccess;
     size_t const lds = static_cast<size_t>(h.max_bins) * (1u + h.depth) * sizeof(uint32_t);
     hipLaunchKernelGGL(hibf_descent_kernel<false>, dim3(blocks), dim3(256), lds, stream, h, values, offsets, nq,
-                       thresholds, nullptr, 2u, nullptr, scratch, ids_tmp, ids_cap, hit_start, hit_count, total);
+                       thresholds, nullptr, 2u, nullptr, scratch, ids_tmp, ids_cap, hit_start, hit_count);
     return hipGetLastError();
 }
 
@@ -369,8 +403,7 @@ hipError_t launch_hibf_count(HibfLaunch const & h,
         return hipSuccess;
     size_t const lds = static_cast<size_t>(h.max_bins) * (1u + h.depth) * sizeof(uint32_t);
     hipLaunchKernelGGL(hibf_descent_kernel<true>, dim3(blocks), dim3(256), lds, stream, h, values, offsets, nq,
-                       nullptr, thresholds, counter_bytes, counts, nullptr, nullptr, 0ULL, nullptr, nullptr,
-                       nullptr);
+                       nullptr, thresholds, counter_bytes, counts, nullptr, nullptr, 0ULL, nullptr, nullptr);
     return hipGetLastError();
 }
 

@@ -123,7 +123,15 @@ struct HibfLaunch
     uint32_t max_bins;        // largest IBF of this HIBF
     uint32_t depth;           // levels
     uint64_t n_user_bins;
+    // Agent counters (device): [0] ids found, [1] next query (work stealing),
+    // [2] IBF visits, [3] row bytes fetched (h * bin_words * 8 per value and
+    // visit), [4] lookups (query values checked against one IBF), [5] set when
+    // ids were dropped because the output was full.
+    unsigned long long * counters;
 };
+
+// Zeroes the first n agent counters (a kernel, so it is safe inside captured graphs).
+hipError_t launch_reset_counters(unsigned long long * counters, uint32_t n, hipStream_t stream);
 
 // Membership descent.  Writes, per query, hit_start/hit_count into ids_tmp
 // (atomic order of queries); *total receives the number of ids.
@@ -138,7 +146,6 @@ hipError_t launch_hibf_membership(HibfLaunch const & h,
                                   uint64_t ids_cap,
                                   uint64_t * hit_start,
                                   uint64_t * hit_count,
-                                  unsigned long long * total,
                                   hipStream_t stream);
 
 hipError_t launch_hibf_count(HibfLaunch const & h,

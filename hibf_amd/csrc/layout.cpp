@@ -5,7 +5,9 @@
 #include "layout.hpp"
 
 #include <algorithm>
+#include <barrier>
 #include <cmath>
+#include <thread>
 #include <limits>
 #include <sstream>
 #include <utility>
@@ -241,50 +243,110 @@ uint64_t hierarchical_binning(Store & d, Config const & c)
         trace[0][j] = {0u, j - 1};
     }
 
-    // recursion (:136-232)
-    for (uint64_t j = 1; j < num_user_bins; ++j)
+    // recursion (:136-232).  Cells of one column only read earlier columns, so
+    // the rows of a column are computed in parallel (interleaved over worker
+    // threads, a barrier per column); column j-1 is mirrored into contiguous
+    // arrays for the i' scan.  The arithmetic per cell is the reference's.
+    std::vector<uint64_t> mml(num_user_bins + 1, 0);
+    for (uint64_t n = 1; n <= num_user_bins; ++n)
+        mml[n] = max_merge_levels(c, n);
+    std::vector<uint64_t> sorted_counts(num_user_bins);
+    for (uint64_t j = 0; j < num_user_bins; ++j)
+        sorted_counts[j] = counts[d.positions[j]];
+    std::vector<uint64_t> colM(num_technical_bins), colL(num_technical_bins);
+    for (uint64_t i = 0; i < num_technical_bins; ++i)
     {
-        uint64_t const current_weight = counts[d.positions[j]];
+        colM[i] = matrix[i][0];
+        colL[i] = ll[i][0];
+    }
+    auto cell = [&](uint64_t i, uint64_t j)
+    {
+        uint64_t const current_weight = sorted_counts[j];
         double const card = static_cast<double>(current_weight);
-        for (uint64_t i = 1; i < num_technical_bins; ++i)
+        uint64_t minimum = maxv;
+        uint64_t full_minimum = maxv;
+        std::pair<uint64_t, uint64_t> best{maxv, maxv};
+        uint64_t best_ll = 0;
+        for (uint64_t ip = 0; ip < i; ++ip)
         {
-            uint64_t minimum = maxv;
-            uint64_t full_minimum = maxv;
-            for (uint64_t ip = 0; ip < i; ++ip)
+            uint64_t const corrected = static_cast<uint64_t>(card * corr[i - ip]);
+            uint64_t const score = std::max<uint64_t>(divide_and_ceil(corrected, i - ip), colM[ip]);
+            uint64_t const full_score =
+                static_cast<uint64_t>(static_cast<double>(score * (i + 1)) + c.alpha * static_cast<double>(colL[ip]));
+            if (full_score < full_minimum)
             {
-                uint64_t const corrected = static_cast<uint64_t>(card * corr[i - ip]);
-                uint64_t const score = std::max<uint64_t>(divide_and_ceil(corrected, i - ip), matrix[ip][j - 1]);
-                uint64_t const full_score =
-                    static_cast<uint64_t>(static_cast<double>(score * (i + 1)) + c.alpha * static_cast<double>(ll[ip][j - 1]));
-                if (full_score < full_minimum)
-                {
-                    minimum = score;
-                    full_minimum = full_score;
-                    trace[i][j] = {ip, j - 1};
-                    ll[i][j] = ll[ip][j - 1];
-                }
+                minimum = score;
+                full_minimum = full_score;
+                best = {ip, j - 1};
+                best_ll = colL[ip];
             }
-            uint64_t jp = j - 1;
-            uint64_t weight = current_weight;
-            auto get_weight = [&]() -> double { return rcorr * static_cast<double>(weight); };
-            while (jp != 0 && ((i - trace[i][jp].first) < 2) && get_weight() < static_cast<double>(minimum))
-            {
-                weight += counts[d.positions[jp]];
-                --jp;
-                uint64_t const score = std::max<uint64_t>(matrix[i - 1][jp], static_cast<uint64_t>(get_weight()));
-                uint64_t const ll_kmers = ll[i - 1][jp] + max_merge_levels(c, j - jp) * weight;
-                uint64_t const full_score =
-                    static_cast<uint64_t>(static_cast<double>(score * (i + 1)) + c.alpha * static_cast<double>(ll_kmers));
-                if (full_score < full_minimum)
-                {
-                    minimum = score;
-                    full_minimum = full_score;
-                    trace[i][j] = {i - 1, jp};
-                    ll[i][j] = ll_kmers;
-                }
-            }
-            matrix[i][j] = minimum;
         }
+        auto const & trow = trace[i];
+        auto const & mprev = matrix[i - 1];
+        auto const & lprev = ll[i - 1];
+        uint64_t jp = j - 1;
+        uint64_t weight = current_weight;
+        auto get_weight = [&]() -> double { return rcorr * static_cast<double>(weight); };
+        while (jp != 0 && ((i - trow[jp].first) < 2) && get_weight() < static_cast<double>(minimum))
+        {
+            weight += sorted_counts[jp];
+            --jp;
+            uint64_t const score = std::max<uint64_t>(mprev[jp], static_cast<uint64_t>(get_weight()));
+            uint64_t const ll_kmers = lprev[jp] + mml[j - jp] * weight;
+            uint64_t const full_score =
+                static_cast<uint64_t>(static_cast<double>(score * (i + 1)) + c.alpha * static_cast<double>(ll_kmers));
+            if (full_score < full_minimum)
+            {
+                minimum = score;
+                full_minimum = full_score;
+                best = {i - 1, jp};
+                best_ll = ll_kmers;
+            }
+        }
+        trace[i][j] = best;
+        ll[i][j] = best_ll;
+        matrix[i][j] = minimum;
+    };
+    uint64_t const work = num_technical_bins * num_user_bins;
+    unsigned const hw = std::max(1u, std::thread::hardware_concurrency());
+    unsigned const nthreads = work < (1u << 16) ? 1u : std::min<unsigned>(16u, hw);
+    if (nthreads == 1)
+    {
+        for (uint64_t j = 1; j < num_user_bins; ++j)
+        {
+            for (uint64_t i = 1; i < num_technical_bins; ++i)
+                cell(i, j);
+            for (uint64_t i = 0; i < num_technical_bins; ++i)
+            {
+                colM[i] = matrix[i][j];
+                colL[i] = ll[i][j];
+            }
+        }
+    }
+    else
+    {
+        std::barrier sync(static_cast<std::ptrdiff_t>(nthreads));
+        auto worker = [&](unsigned t)
+        {
+            for (uint64_t j = 1; j < num_user_bins; ++j)
+            {
+                for (uint64_t i = 1 + t; i < num_technical_bins; i += nthreads)
+                    cell(i, j);
+                sync.arrive_and_wait();
+                for (uint64_t i = t; i < num_technical_bins; i += nthreads)
+                {
+                    colM[i] = matrix[i][j];
+                    colL[i] = ll[i][j];
+                }
+                sync.arrive_and_wait();
+            }
+        };
+        std::vector<std::thread> pool;
+        for (unsigned t = 1; t < nthreads; ++t)
+            pool.emplace_back(worker, t);
+        worker(0);
+        for (auto & th : pool)
+            th.join();
     }
 
     // backtracking (:302-331) with backtrack_merged_bin / backtrack_split_bin

@@ -65,6 +65,18 @@ class Agent:
         check(lib().hibf_gpu_agent_create(max_queries_per_launch, ctypes.byref(h)))
         self.h = h
 
+    def stats(self, reset: bool = False) -> dict:
+        """IBF visits, lookups and row bytes of HIBF descents since the last reset."""
+        v, lk, b = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+        check(lib().hibf_gpu_agent_stats(self.h, ctypes.byref(v), ctypes.byref(lk), ctypes.byref(b), int(reset)))
+        return {"visits": int(v.value), "lookups": int(lk.value), "row_bytes": int(b.value)}
+
+    def status(self) -> tuple[int, bool]:
+        """(ids found by the last async membership launch, whether any launch dropped ids)."""
+        t, d = ctypes.c_uint64(), ctypes.c_uint64()
+        check(lib().hibf_gpu_agent_status(self.h, ctypes.byref(t), ctypes.byref(d)))
+        return int(t.value), bool(d.value)
+
     def __del__(self):
         if getattr(self, "h", None):
             lib().hibf_gpu_agent_destroy(self.h)
@@ -299,6 +311,20 @@ class DeviceHibf:
         return _membership(lambda ids, cap, hoff, total: lib().hibf_gpu_hibf_membership_for(
             self.h, agent.h, _ptr(v), _ptr(o), nq, _ptr(t), _ptr(ids), cap, _ptr(hoff), ctypes.byref(total),
             _stream()), nq, o.device)
+
+    def membership_prepare(self, agent: Agent, max_queries: int) -> None:
+        check(lib().hibf_gpu_hibf_membership_prepare(self.h, agent.h, max_queries))
+
+    def membership_launch(self, agent: Agent, values: torch.Tensor, offsets: torch.Tensor,
+                          thresholds: torch.Tensor, ids: torch.Tensor, hit_start: torch.Tensor,
+                          hit_count: torch.Tensor, stream=None) -> None:
+        """Asynchronous membership over the CSR batch (offsets has nq+1 entries);
+        query q's ids are ids[hit_start[q] : hit_start[q] + hit_count[q]]."""
+        nq = offsets.numel() - 1
+        s = _stream() if stream is None else ctypes.c_void_p(stream.cuda_stream)
+        check(lib().hibf_gpu_hibf_membership_launch(self.h, agent.h, _ptr(values), _ptr(offsets), nq,
+                                                    _ptr(thresholds), _ptr(ids), ids.numel(), _ptr(hit_start),
+                                                    _ptr(hit_count), s))
 
     def bulk_count(self, values, offsets, thresholds, counter_bytes: int = 2, agent: Agent | None = None):
         v = as_device_u64(values)

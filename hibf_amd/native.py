@@ -95,6 +95,10 @@ _SIGNATURES = {
     "hibf_gpu_ibf_bulk_count": (_INT, [_V, _V, _V, _U64, _U32, _V, _V]),
     "hibf_gpu_agent_create": (_INT, [_U64, ctypes.POINTER(_V)]),
     "hibf_gpu_agent_destroy": (_INT, [_V]),
+    "hibf_gpu_agent_stats": (_INT, [_V, ctypes.POINTER(_U64), ctypes.POINTER(_U64), ctypes.POINTER(_U64), _INT]),
+    "hibf_gpu_agent_status": (_INT, [_V, ctypes.POINTER(_U64), ctypes.POINTER(_U64)]),
+    "hibf_gpu_hibf_membership_prepare": (_INT, [_V, _V, _U64]),
+    "hibf_gpu_hibf_membership_launch": (_INT, [_V, _V, _V, _V, _U64, _V, _V, _U64, _V, _V, _V]),
     "hibf_gpu_ibf_membership_for": (_INT, [_V, _V, _V, _V, _U64, _V, _V, _U64, _V, ctypes.POINTER(_U64), _V]),
     "hibf_gpu_hibf_create": (_INT, [_U64, ctypes.POINTER(Geometry), _V, _V, _V, _V, _V, _U64, ctypes.POINTER(_V)]),
     "hibf_gpu_hibf_destroy": (_INT, [_V]),
@@ -110,6 +114,7 @@ _SIGNATURES = {
     "hibf_gpu_layout_compute": (_INT, [ctypes.POINTER(Config), _V, ctypes.c_char_p, _U64, ctypes.POINTER(_U64)]),
     "hibf_gpu_hibf_build": (_INT, [ctypes.POINTER(Config), _V, _V, ctypes.c_char_p, ctypes.POINTER(_V)]),
     "hibf_gpu_hibf_build_synthetic": (_INT, [ctypes.POINTER(Config), _V, _U64, ctypes.c_char_p, ctypes.POINTER(_V)]),
+    "hibf_gpu_hibf_plan_stats": (_INT, [ctypes.POINTER(Config), _V, ctypes.c_char_p] + [ctypes.POINTER(_U64)] * 4),
 }
 
 _lib = None
@@ -175,6 +180,17 @@ def compute_layout(c: Config, kmer_counts) -> str:
     check(lib().hibf_gpu_layout_compute(ctypes.byref(c), k.ctypes.data_as(ctypes.c_void_p), buf, need.value,
                                         ctypes.byref(need)))
     return buf.value.decode()
+
+
+def plan_stats(c: Config, sizes, layout_text: str | None = None) -> dict:
+    """Host-only dry run of a build: IBFs, arena words, segments, insertions."""
+    import numpy as np
+    sz = np.ascontiguousarray(np.asarray(sizes, dtype=np.uint64))
+    out = [ctypes.c_uint64() for _ in range(4)]
+    check(lib().hibf_gpu_hibf_plan_stats(ctypes.byref(c), sz.ctypes.data_as(ctypes.c_void_p),
+                                         None if layout_text is None else layout_text.encode(),
+                                         *[ctypes.byref(o) for o in out]))
+    return dict(zip(("n_ibfs", "n_words", "n_segments", "n_emplaces"), (o.value for o in out)))
 
 
 def check(rc: int) -> None:

@@ -155,6 +155,18 @@ typedef struct hibf_gpu_agent_s * hibf_gpu_agent;
 int hibf_gpu_agent_create(uint64_t max_queries_per_launch, hibf_gpu_agent * out);
 int hibf_gpu_agent_destroy(hibf_gpu_agent agent);
 
+/* Work counters of the HIBF descent launched through this agent since the
+ * last reset: IBF visits (one per query and visited IBF) and row bytes
+ * fetched (query length x h x bin_words x 8 per visit).  Synchronous.  No
+ * counterpart in the reference (it has no device traffic to report). */
+int hibf_gpu_agent_stats(hibf_gpu_agent agent, uint64_t * visits, uint64_t * lookups, uint64_t * row_bytes,
+                         int reset);
+
+/* After hibf_gpu_hibf_membership_launch calls: ids found by the last launch
+ * and whether any launch since the last prepare dropped ids because its
+ * output was full.  Synchronous. */
+int hibf_gpu_agent_status(hibf_gpu_agent agent, uint64_t * last_total, uint64_t * dropped);
+
 /* Batched membership_agent_type::membership_for (interleaved_bloom_filter.hpp:744-764):
  * bins with uint16 count >= d_thresholds[q], ascending.  Results of query q
  * are d_ids[d_hit_offsets[q] .. d_hit_offsets[q+1]) (d_hit_offsets has nq+1
@@ -224,6 +236,25 @@ int hibf_gpu_hibf_membership_for(hibf_gpu_hibf hibf,
                                  uint64_t * d_hit_offsets,
                                  uint64_t * h_total,
                                  void * stream);
+
+/* Asynchronous, graph-capturable form of hibf_gpu_hibf_membership_for: one
+ * persistent work-stealing launch over nq queries.  Query q's ids (DFS order)
+ * are d_ids[d_hit_start[q] .. d_hit_start[q] + d_hit_count[q]); ids beyond
+ * ids_cap are dropped (hibf_gpu_agent_status reports it).  The agent must
+ * first be prepared (allocates its workspace; not capturable) for batches of
+ * up to max_queries queries.  Calls on one agent must be serialised. */
+int hibf_gpu_hibf_membership_prepare(hibf_gpu_hibf hibf, hibf_gpu_agent agent, uint64_t max_queries);
+int hibf_gpu_hibf_membership_launch(hibf_gpu_hibf hibf,
+                                    hibf_gpu_agent agent,
+                                    uint64_t const * d_values,
+                                    uint64_t const * d_offsets,
+                                    uint64_t nq,
+                                    uint16_t const * d_thresholds,
+                                    uint64_t * d_ids,
+                                    uint64_t ids_cap,
+                                    uint64_t * d_hit_start,
+                                    uint64_t * d_hit_count,
+                                    void * stream);
 
 /* Batched HIBF counting_agent_type<T>::bulk_count
  * (hierarchical_interleaved_bloom_filter.hpp:462-496, 549-566): d_counts gets
@@ -316,6 +347,17 @@ int hibf_gpu_hibf_build_synthetic(const hibf_gpu_config * c,
                                   uint64_t seed,
                                   const char * layout_text,
                                   hibf_gpu_hibf * out);
+
+/* Dry run of a build (host only, no device memory): number of IBFs, arena
+ * words, emplace segments and (value, technical bin) insertions the build of
+ * these user-bin sizes would make.  For capacity planning. */
+int hibf_gpu_hibf_plan_stats(const hibf_gpu_config * c,
+                             uint64_t const * h_ub_sizes,
+                             const char * layout_text,
+                             uint64_t * n_ibfs,
+                             uint64_t * n_words,
+                             uint64_t * n_segments,
+                             uint64_t * n_emplaces);
 
 #ifdef __cplusplus
 }
