@@ -375,9 +375,11 @@ def run_hibf(args):
                 (va_h, oa_h, ta_h, ids, hs, hc, nq, None), (vb_h, ob_h, tb_h, ids_b, hs_b, hc_b, nb, G)):
             ids_n, hs_n, hc_n = to_numpy(ids_t), to_numpy(hs_t), to_numpy(hc_t)
             for q in rng.choice(n, size=min(args.check_queries, n), replace=False):
+                q = int(q)
                 base = 0 if per_batch is None else (q // per_batch) * cap_b
-                got = ids_n[base + hs_n[q]: base + hs_n[q] + hc_n[q]].tolist()
-                want = ref.membership_for(v_h[o_h[q]:o_h[q + 1]], int(t_h[q]))
+                s0 = base + int(hs_n[q])
+                got = ids_n[s0: s0 + int(hc_n[q])].tolist()
+                want = ref.membership_for(v_h[int(o_h[q]):int(o_h[q + 1])], int(t_h[q]))
                 if got != want:
                     ok = False
                     log(f"parity mismatch at query {q}: {got} vs {want}")
@@ -420,8 +422,7 @@ def run_hibf(args):
         "roofline": {"bound": "hbm", "achieved": bytes_b / wall_b / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": bytes_b / wall_b / 1e9 / HBM_PEAK_GBS, "traffic": None},
         "fixed_batch": {"queries": nq, "length": 250, "ms": wall_a * 1e3, "lookups_per_s": lookups_a / wall_a,
-                        "row_gbs": bytes_a / kern_a / 1e9, "kernel_ms": kern_a * 1e3,
-                        "ibfs_per_query": st and None},
+                        "row_gbs": bytes_a / kern_a / 1e9, "kernel_ms": kern_a * 1e3},
         "plain_per_query": {"value": plain, "unit": "lookups/s", "queries": npl,
                             "latency_ms_p50": float(np.percentile(lat, 50)),
                             "latency_ms_p99": float(np.percentile(lat, 99))},
