@@ -145,6 +145,8 @@ struct hibf_gpu_agent_s
     size_t cub_temp_bytes{0};
     void * counts{nullptr}; // IBF membership: uint16 counts of one chunk
     size_t counts_bytes{0};
+    void * overflow{nullptr}; // HIBF membership: queries answered by the block kernel
+    size_t overflow_bytes{0};
     unsigned long long * d_total{nullptr}; // 8 counters, see HibfLaunch::counters
     unsigned long long * h_total{nullptr}; // pinned, 8 entries
 };
@@ -203,6 +205,8 @@ int hibf_gpu_tune_set(char const * key, int64_t value)
         g_count_tuning.wide_unroll = static_cast<uint32_t>(value);
     else if (k == "count.wide_waves" && value >= 1 && value <= 4)
         g_count_tuning.wide_waves = static_cast<uint32_t>(value);
+    else if (k == "hibf.wave_kernel" && (value == 0 || value == 1))
+        g_count_tuning.hibf_wave = static_cast<uint32_t>(value);
     else
         return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "unknown tuning key or value out of range: " + k);
     return HIBF_GPU_OK;
@@ -219,6 +223,8 @@ int hibf_gpu_tune_get(char const * key, int64_t * value)
         *value = g_count_tuning.wide_unroll;
     else if (k == "count.wide_waves")
         *value = g_count_tuning.wide_waves;
+    else if (k == "hibf.wave_kernel")
+        *value = g_count_tuning.hibf_wave;
     else
         return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "unknown tuning key: " + k);
     return HIBF_GPU_OK;
@@ -426,7 +432,7 @@ int hibf_gpu_agent_destroy(hibf_gpu_agent a)
 {
     if (!a)
         return HIBF_GPU_OK;
-    for (void * p : {a->scratch, a->ids_tmp, a->hit_start, a->hit_count, a->incl, a->cub_temp, a->counts,
+    for (void * p : {a->scratch, a->ids_tmp, a->hit_start, a->hit_count, a->incl, a->cub_temp, a->counts, a->overflow,
                      static_cast<void *>(a->d_total)})
         if (p)
             (void)hipFree(p);
@@ -781,7 +787,8 @@ namespace
 {
 HibfLaunch launch_of(hibf_gpu_hibf h, hibf_gpu_agent a)
 {
-    return HibfLaunch{h->d_desc, h->d_arena, h->d_tb_info, h->max_bins, h->depth, h->n_user_bins, a->d_total};
+    return HibfLaunch{h->d_desc,      h->d_arena, h->d_tb_info, h->max_bins, h->depth, h->n_user_bins,
+                      (h->max_bins + 63u) / 64u, 0u, a->d_total};
 }
 
 unsigned hibf_blocks(uint64_t nq, uint64_t per_block_scratch_bytes)
@@ -794,13 +801,87 @@ unsigned hibf_blocks(uint64_t nq, uint64_t per_block_scratch_bytes)
 }
 } // namespace
 
+namespace
+{
+
+constexpr uint32_t kMembershipCounters = (1u << 0) | (1u << 1) | (1u << 6) | (1u << 7);
+
+bool wave_kernel_fits(HibfLaunch const & L)
+{
+    return g_count_tuning.hibf_wave != 0u && hibf_wave_lds_bytes(L, 4) <= 64u * 1024u;
+}
+
+unsigned overflow_blocks(hibf_gpu_hibf h, uint64_t nq)
+{
+    return std::min<unsigned>(256u, hibf_blocks(nq, h->n_user_bins * sizeof(uint32_t)));
+}
+
+// Workspace of the membership path for batches of up to nq queries.
+int prepare_membership(hibf_gpu_hibf h, hibf_gpu_agent a, uint64_t nq)
+{
+    uint64_t const per_block = h->n_user_bins * sizeof(uint32_t);
+    HibfLaunch const L = launch_of(h, a);
+    unsigned const blocks = wave_kernel_fits(L) ? overflow_blocks(h, nq) : hibf_blocks(nq, per_block);
+    if (int rc = ensure(a->scratch, a->scratch_bytes, static_cast<size_t>(blocks) * per_block, "membership scratch"))
+        return rc;
+    if (int rc = ensure(a->overflow, a->overflow_bytes, std::max<uint64_t>(nq, 1) * 8u, "membership overflow"))
+        return rc;
+    return HIBF_GPU_OK;
+}
+
+// Enqueues one membership pass: wave-per-query kernel plus the block-per-query
+// pass over the queries whose ids did not fit the wave's buffer (or the block
+// kernel alone when the HIBF's IBFs are too wide for per-wave LDS state).
+int enqueue_membership(hibf_gpu_hibf h,
+                       hibf_gpu_agent a,
+                       uint64_t const * d_values,
+                       uint64_t const * d_offsets,
+                       uint64_t nq,
+                       uint16_t const * d_thresholds,
+                       uint64_t * d_ids,
+                       uint64_t ids_cap,
+                       uint64_t * d_hit_start,
+                       uint64_t * d_hit_count,
+                       hipStream_t s)
+{
+    HibfLaunch const L = launch_of(h, a);
+    uint64_t const per_block = h->n_user_bins * sizeof(uint32_t);
+    HIBF_TRY_HIP(launch_reset_counters(a->d_total, kMembershipCounters, s), "counter reset");
+    if (wave_kernel_fits(L))
+    {
+        unsigned const ob = overflow_blocks(h, nq);
+        if (a->scratch_bytes < static_cast<size_t>(ob) * per_block || a->overflow_bytes < nq * 8u)
+            return fail(HIBF_GPU_ERR_INVALID_ARGUMENT,
+                        "agent not prepared for this batch size (hibf_gpu_hibf_membership_prepare)");
+        unsigned const blocks =
+            static_cast<unsigned>(std::min<uint64_t>(static_cast<uint64_t>(device_cus()) * 8u, (nq + 3u) / 4u));
+        HIBF_TRY_HIP(launch_hibf_wave_membership(L, d_values, d_offsets, nq, d_thresholds, std::max(blocks, 1u), d_ids,
+                                                 d_ids ? ids_cap : 0, d_hit_start, d_hit_count,
+                                                 static_cast<uint64_t *>(a->overflow), s),
+                     "membership launch");
+        HIBF_TRY_HIP(launch_hibf_membership(L, d_values, d_offsets, nq, d_thresholds,
+                                            static_cast<uint32_t *>(a->scratch), ob, d_ids, d_ids ? ids_cap : 0,
+                                            d_hit_start, d_hit_count, static_cast<uint64_t *>(a->overflow), s),
+                     "membership overflow launch");
+        return HIBF_GPU_OK;
+    }
+    unsigned const blocks = hibf_blocks(nq, per_block);
+    if (a->scratch_bytes < static_cast<size_t>(blocks) * per_block)
+        return fail(HIBF_GPU_ERR_INVALID_ARGUMENT,
+                    "agent not prepared for this batch size (hibf_gpu_hibf_membership_prepare)");
+    HIBF_TRY_HIP(launch_hibf_membership(L, d_values, d_offsets, nq, d_thresholds, static_cast<uint32_t *>(a->scratch),
+                                        blocks, d_ids, d_ids ? ids_cap : 0, d_hit_start, d_hit_count, nullptr, s),
+                 "membership launch");
+    return HIBF_GPU_OK;
+}
+
+} // namespace
+
 int hibf_gpu_hibf_membership_prepare(hibf_gpu_hibf h, hibf_gpu_agent a, uint64_t max_queries)
 {
     if (!h || !a)
         return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "null argument");
-    uint64_t const per_block = h->n_user_bins * sizeof(uint32_t);
-    unsigned const blocks = hibf_blocks(std::max<uint64_t>(max_queries, 1), per_block);
-    if (int rc = ensure(a->scratch, a->scratch_bytes, blocks * per_block, "membership scratch"))
+    if (int rc = prepare_membership(h, a, std::max<uint64_t>(max_queries, 1)))
         return rc;
     HIBF_TRY_HIP(hipMemset(a->d_total + 5, 0, sizeof(unsigned long long)), "flag reset");
     return HIBF_GPU_OK;
@@ -822,17 +903,8 @@ int hibf_gpu_hibf_membership_launch(hibf_gpu_hibf h,
         return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "null argument");
     if (nq == 0)
         return HIBF_GPU_OK;
-    uint64_t const per_block = h->n_user_bins * sizeof(uint32_t);
-    unsigned const blocks = hibf_blocks(nq, per_block);
-    if (a->scratch_bytes < blocks * per_block)
-        return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "agent not prepared for this batch size (hibf_gpu_hibf_membership_prepare)");
-    hipStream_t const s = static_cast<hipStream_t>(stream);
-    HIBF_TRY_HIP(launch_reset_counters(a->d_total, 2u, s), "counter reset");
-    HIBF_TRY_HIP(launch_hibf_membership(launch_of(h, a), d_values, d_offsets, nq, d_thresholds,
-                                        static_cast<uint32_t *>(a->scratch), blocks, d_ids, d_ids ? ids_cap : 0,
-                                        d_hit_start, d_hit_count, s),
-                 "membership launch");
-    return HIBF_GPU_OK;
+    return enqueue_membership(h, a, d_values, d_offsets, nq, d_thresholds, d_ids, ids_cap, d_hit_start, d_hit_count,
+                              static_cast<hipStream_t>(stream));
 }
 
 int hibf_gpu_hibf_membership_for(hibf_gpu_hibf h,
@@ -853,9 +925,7 @@ int hibf_gpu_hibf_membership_for(hibf_gpu_hibf h,
     *h_total = 0;
     if (nq == 0)
         return HIBF_GPU_OK;
-    uint64_t const per_block = h->n_user_bins * sizeof(uint32_t);
-    unsigned const blocks = hibf_blocks(nq, per_block);
-    if (int rc = ensure(a->scratch, a->scratch_bytes, blocks * per_block, "membership scratch"))
+    if (int rc = prepare_membership(h, a, nq))
         return rc;
     if (int rc = ensure(a->hit_start, a->hit_start_bytes, nq * 8u, "membership hits"))
         return rc;
@@ -864,17 +934,14 @@ int hibf_gpu_hibf_membership_for(hibf_gpu_hibf h,
     if (!a->ids_tmp)
         if (int rc = ensure(a->ids_tmp, a->ids_tmp_bytes, std::max<uint64_t>(nq, 1024u) * 8u, "membership ids"))
             return rc;
-    HibfLaunch const L = launch_of(h, a);
     for (int attempt = 0; attempt < 2; ++attempt)
     {
         uint64_t const cap = a->ids_tmp_bytes / 8u;
-        HIBF_TRY_HIP(launch_reset_counters(a->d_total, 2u, s), "counter reset");
-        HIBF_TRY_HIP(launch_hibf_membership(L, d_values, d_offsets, nq, d_thresholds,
-                                            static_cast<uint32_t *>(a->scratch), blocks,
-                                            static_cast<uint64_t *>(a->ids_tmp), cap,
-                                            static_cast<uint64_t *>(a->hit_start),
-                                            static_cast<uint64_t *>(a->hit_count), s),
-                     "membership launch");
+        if (int rc = enqueue_membership(h, a, d_values, d_offsets, nq, d_thresholds,
+                                        static_cast<uint64_t *>(a->ids_tmp), cap,
+                                        static_cast<uint64_t *>(a->hit_start), static_cast<uint64_t *>(a->hit_count),
+                                        s))
+            return rc;
         HIBF_TRY_HIP(hipMemcpyAsync(a->h_total, a->d_total, sizeof(unsigned long long), hipMemcpyDeviceToHost, s),
                      "total copy");
         HIBF_TRY_HIP(hipStreamSynchronize(s), "membership sync");
@@ -916,8 +983,18 @@ int hibf_gpu_hibf_bulk_count(hibf_gpu_hibf h,
     if (nq == 0)
         return HIBF_GPU_OK;
     hipStream_t const s = static_cast<hipStream_t>(stream);
-    HIBF_TRY_HIP(launch_reset_counters(a->d_total + 1, 1u, s), "counter reset");
-    HIBF_TRY_HIP(launch_hibf_count(launch_of(h, a), d_values, d_offsets, nq, d_thresholds, counter_bytes, d_counts,
+    HibfLaunch const L = launch_of(h, a);
+    HIBF_TRY_HIP(launch_reset_counters(a->d_total, 1u << 1, s), "counter reset");
+    if (wave_kernel_fits(L))
+    {
+        unsigned const blocks =
+            static_cast<unsigned>(std::min<uint64_t>(static_cast<uint64_t>(device_cus()) * 8u, (nq + 3u) / 4u));
+        HIBF_TRY_HIP(launch_hibf_wave_count(L, d_values, d_offsets, nq, d_thresholds, counter_bytes, d_counts,
+                                            std::max(blocks, 1u), s),
+                     "hibf count launch");
+        return HIBF_GPU_OK;
+    }
+    HIBF_TRY_HIP(launch_hibf_count(L, d_values, d_offsets, nq, d_thresholds, counter_bytes, d_counts,
                                    hibf_blocks(nq, 0), s),
                  "hibf count launch");
     return HIBF_GPU_OK;

@@ -121,7 +121,8 @@ __global__ void __launch_bounds__(256) hibf_descent_kernel(HibfLaunch h,
                                                            uint64_t * __restrict__ ids,
                                                            uint64_t ids_cap,
                                                            uint64_t * __restrict__ hit_start,
-                                                           uint64_t * __restrict__ hit_count)
+                                                           uint64_t * __restrict__ hit_count,
+                                                           uint64_t const * __restrict__ qlist)
 {
     extern __shared__ uint32_t smem[];
     __shared__ DescentState st;
@@ -143,8 +144,20 @@ __global__ void __launch_bounds__(256) hibf_descent_kernel(HibfLaunch h,
     uint64_t lookups = 0;
     while (true)
     {
+        // With a query list (the overflow pass), entry k of qlist is the query
+        // and counters[6] holds the list length.
         if (tid == 0)
-            next_q = atomicAdd(&h.counters[1], 1ULL);
+        {
+            if (qlist)
+            {
+                uint64_t const k = atomicAdd(&h.counters[7], 1ULL);
+                next_q = k < h.counters[6] ? qlist[k] : ~0ULL;
+            }
+            else
+            {
+                next_q = atomicAdd(&h.counters[1], 1ULL);
+            }
+        }
         __syncthreads();
         uint64_t const q = next_q;
         __syncthreads();
@@ -262,9 +275,9 @@ __global__ void __launch_bounds__(256) hibf_descent_kernel(HibfLaunch h,
     }
 }
 
-__global__ void reset_counters_kernel(unsigned long long * counters, uint32_t n)
+__global__ void reset_counters_kernel(unsigned long long * counters, uint32_t mask)
 {
-    if (threadIdx.x < n)
+    if (threadIdx.x < 32u && ((mask >> threadIdx.x) & 1u))
         counters[threadIdx.x] = 0ULL;
 }
 
@@ -362,9 +375,9 @@ hipError_t launch_finalize_offsets(uint64_t const * incl, uint64_t n, uint64_t b
     return hipGetLastError();
 }
 
-hipError_t launch_reset_counters(unsigned long long * counters, uint32_t n, hipStream_t stream)
+hipError_t launch_reset_counters(unsigned long long * counters, uint32_t mask, hipStream_t stream)
 {
-    hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(64), 0, stream, counters, n);
+    hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(64), 0, stream, counters, mask);
     return hipGetLastError();
 }
 
@@ -379,13 +392,14 @@ hipError_t launch_hibf_membership(HibfLaunch const & h,
                                   uint64_t ids_cap,
                                   uint64_t * hit_start,
                                   uint64_t * hit_count,
+                                  uint64_t const * qlist,
                                   hipStream_t stream)
 {
     if (nq == 0)
         return hipSuccess;
     size_t const lds = static_cast<size_t>(h.max_bins) * (1u + h.depth) * sizeof(uint32_t);
     hipLaunchKernelGGL(hibf_descent_kernel<false>, dim3(blocks), dim3(256), lds, stream, h, values, offsets, nq,
-                       thresholds, nullptr, 2u, nullptr, scratch, ids_tmp, ids_cap, hit_start, hit_count);
+                       thresholds, nullptr, 2u, nullptr, scratch, ids_tmp, ids_cap, hit_start, hit_count, qlist);
     return hipGetLastError();
 }
 
@@ -403,7 +417,8 @@ hipError_t launch_hibf_count(HibfLaunch const & h,
         return hipSuccess;
     size_t const lds = static_cast<size_t>(h.max_bins) * (1u + h.depth) * sizeof(uint32_t);
     hipLaunchKernelGGL(hibf_descent_kernel<true>, dim3(blocks), dim3(256), lds, stream, h, values, offsets, nq,
-                       nullptr, thresholds, counter_bytes, counts, nullptr, nullptr, 0ULL, nullptr, nullptr);
+                       nullptr, thresholds, counter_bytes, counts, nullptr, nullptr, 0ULL, nullptr, nullptr,
+                       nullptr);
     return hipGetLastError();
 }
 

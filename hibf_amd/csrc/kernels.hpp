@@ -47,6 +47,7 @@ struct CountTuning
     uint32_t wide_kernel = 1; // use count_wide_kernel for even bin_words >= 64
     uint32_t wide_unroll = 2; // values in flight per wave (1, 2, 4, 8)
     uint32_t wide_waves = 4;  // waves per block (1..4)
+    uint32_t hibf_wave = 1;   // HIBF descent: one wave per query (1) or one block per query (0)
 };
 extern CountTuning g_count_tuning;
 
@@ -123,18 +124,23 @@ struct HibfLaunch
     uint32_t max_bins;        // largest IBF of this HIBF
     uint32_t depth;           // levels
     uint64_t n_user_bins;
+    uint32_t max_bin_words;   // widest row of this HIBF (words)
+    uint32_t pad;
     // Agent counters (device): [0] ids found, [1] next query (work stealing),
     // [2] IBF visits, [3] row bytes fetched (h * bin_words * 8 per value and
     // visit), [4] lookups (query values checked against one IBF), [5] set when
-    // ids were dropped because the output was full.
+    // ids were dropped because the output was full, [6] length of the overflow
+    // query list, [7] next entry of that list.
     unsigned long long * counters;
 };
 
-// Zeroes the first n agent counters (a kernel, so it is safe inside captured graphs).
-hipError_t launch_reset_counters(unsigned long long * counters, uint32_t n, hipStream_t stream);
+// Zeroes the agent counters selected by `mask` (bit i = counter i).  A kernel,
+// so it is safe inside captured graphs.
+hipError_t launch_reset_counters(unsigned long long * counters, uint32_t mask, hipStream_t stream);
 
-// Membership descent.  Writes, per query, hit_start/hit_count into ids_tmp
-// (atomic order of queries); *total receives the number of ids.
+// Membership descent, one block per query.  Writes, per query,
+// hit_start/hit_count into ids_tmp (atomic order of queries).  With a query
+// list it answers only the queries listed there (counters[6] entries).
 hipError_t launch_hibf_membership(HibfLaunch const & h,
                                   uint64_t const * values,
                                   uint64_t const * offsets,
@@ -146,6 +152,32 @@ hipError_t launch_hibf_membership(HibfLaunch const & h,
                                   uint64_t ids_cap,
                                   uint64_t * hit_start,
                                   uint64_t * hit_count,
+                                  uint64_t const * qlist,
+                                  hipStream_t stream);
+
+// One wave per query (hibf_wave.hip); queries with more ids than the wave's
+// LDS buffer are appended to `overflow` for launch_hibf_membership.
+size_t hibf_wave_lds_bytes(HibfLaunch const & h, unsigned waves_per_block);
+hipError_t launch_hibf_wave_membership(HibfLaunch const & h,
+                                       uint64_t const * values,
+                                       uint64_t const * offsets,
+                                       uint64_t nq,
+                                       uint16_t const * thresholds,
+                                       unsigned blocks,
+                                       uint64_t * ids,
+                                       uint64_t ids_cap,
+                                       uint64_t * hit_start,
+                                       uint64_t * hit_count,
+                                       uint64_t * overflow,
+                                       hipStream_t stream);
+hipError_t launch_hibf_wave_count(HibfLaunch const & h,
+                                  uint64_t const * values,
+                                  uint64_t const * offsets,
+                                  uint64_t nq,
+                                  uint64_t const * thresholds,
+                                  uint32_t counter_bytes,
+                                  void * counts,
+                                  unsigned blocks,
                                   hipStream_t stream);
 
 hipError_t launch_hibf_count(HibfLaunch const & h,

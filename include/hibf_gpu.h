@@ -45,7 +45,8 @@ const char * hibf_gpu_last_error(void);
 /* Launch-shape knobs of the query kernels (no counterpart in the reference:
  * the reference has no device tiles).  Process-wide; set by the autotuner.
  * Keys: "count.wide_kernel" (0/1), "count.wide_unroll" (1,2,4,8),
- * "count.wide_waves" (1..4).  Results never depend on these values. */
+ * "count.wide_waves" (1..4), "hibf.wave_kernel" (0/1).  Results never depend
+ * on these values. */
 int hibf_gpu_tune_set(const char * key, int64_t value);
 int hibf_gpu_tune_get(const char * key, int64_t * value);
 

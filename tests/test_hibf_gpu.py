@@ -153,13 +153,23 @@ CASES = [  # (n_ub, size range, config)
     (300, (20, 300), dict(tmax=64, number_of_hash_functions=3)),
     (900, (5, 200), dict(tmax=64, maximum_fpr=0.01, number_of_hash_functions=1)),
     (5000, (2, 60), dict(tmax=64, relaxed_fpr=0.5)),
+    (700, (20, 200), dict(tmax=512)),       # 8-word rows: lanes-per-value counting in the wave kernel
+    (1500, (20, 200), dict(tmax=128, number_of_hash_functions=4)),
 ]
 
 
+@pytest.fixture(params=[1, 0], ids=["wave", "block"])
+def descent_kernel(request):
+    from hibf_amd import native
+    native.tune_set("hibf.wave_kernel", request.param)
+    yield request.param
+    native.tune_set("hibf.wave_kernel", 1)
+
+
 @pytest.mark.parametrize("n_ub,sizes,config", CASES)
-def test_hibf_parity_random(n_ub, sizes, config):
+def test_hibf_parity_random(n_ub, sizes, config, descent_kernel):
     from hibf_amd.device import DeviceHibf, to_numpy
-    _say(f"hibf parity n_ub={n_ub} {config}")
+    _say(f"hibf parity n_ub={n_ub} {config} kernel={'wave' if descent_kernel else 'block'}")
     rng = np.random.default_rng(n_ub)
     ubs = _random_user_bins(rng, n_ub, *sizes)
     dev = DeviceHibf.build(ubs, **config)
@@ -167,6 +177,7 @@ def test_hibf_parity_random(n_ub, sizes, config):
     ref = oracle_of(dev)
     qs = _mixed_queries(rng, ubs, 60, 120) + [np.zeros(0, dtype=np.uint64), ubs[0][:1]]
     thresholds = [int(t) for t in rng.integers(0, 60, size=len(qs))]
+    thresholds[0], thresholds[1], thresholds[-1] = 0, 1, 0   # t = 0 reports every user bin (overflow path)
     got = _membership_lists(dev, qs, thresholds)
     for q, t, g in zip(qs, thresholds, got):
         assert g == ref.membership_for(q, t)
