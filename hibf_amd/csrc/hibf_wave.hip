@@ -153,7 +153,10 @@ __device__ __forceinline__ uint32_t scan_wave(HibfIbfDesc const & d,
         uint32_t const payload = static_cast<uint32_t>(info);
         bool const reset = valid && (kind == kTbRunEnd || kind == kTbMerged);
         uint64_t s = valid ? (static_cast<uint64_t>(cnt[tb]) & mask) : 0ULL;
-        bool f = lane > 0 && (__shfl_up(static_cast<int>(reset), 1, 64) != 0);
+        // The shuffle runs on every lane (a short-circuit would leave lane 0
+        // inactive while lane 1 reads it).
+        bool const prev_reset = __shfl_up(static_cast<int>(reset), 1, 64) != 0;
+        bool f = lane > 0 && prev_reset;
 #pragma unroll
         for (int dlt = 1; dlt < 64; dlt <<= 1)
         {
