@@ -49,5 +49,27 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
+CPP_TESTS = [os.path.join(ROOT, "tests", "cpp", "snippets.cpp")]
+
+
+def build_cpp_tests(verbose: bool = False) -> list[str]:
+    """Compiles the C++ programs that use the seqan::hibf headers (host g++,
+    linked against libhibf_gpu.so)."""
+    out = []
+    for src in CPP_TESTS:
+        exe = os.path.splitext(src)[0]
+        cmd = ["g++", "-std=c++20", "-O1", "-Wall", f"-I{os.path.join(ROOT, 'include')}", src, f"-L{HERE}",
+               "-lhibf_gpu", "-Wl,-rpath,$ORIGIN/../../hibf_amd", "-o", exe]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            sys.stderr.write(r.stdout + r.stderr)
+            raise RuntimeError(f"g++ failed on {src}")
+        out.append(exe)
+    return out
+
+
 if __name__ == "__main__":
     print(build(verbose=True))
+    print(build_cpp_tests(verbose=True))

@@ -230,6 +230,39 @@ int hibf_gpu_tune_get(char const * key, int64_t * value)
     return HIBF_GPU_OK;
 }
 
+int hibf_gpu_malloc(void ** d_ptr, uint64_t bytes)
+{
+    if (!d_ptr)
+        return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "null argument");
+    *d_ptr = nullptr;
+    HIBF_TRY_HIP(hipMalloc(d_ptr, std::max<uint64_t>(bytes, 1)), "hibf_gpu_malloc");
+    return HIBF_GPU_OK;
+}
+
+int hibf_gpu_free(void * d_ptr)
+{
+    if (d_ptr)
+        HIBF_TRY_HIP(hipFree(d_ptr), "hibf_gpu_free");
+    return HIBF_GPU_OK;
+}
+
+int hibf_gpu_memcpy(void * dst, void const * src, uint64_t bytes, int kind)
+{
+    if ((!dst || !src) && bytes)
+        return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "null argument");
+    if (bytes == 0)
+        return HIBF_GPU_OK;
+    hipMemcpyKind const k = kind == 0 ? hipMemcpyHostToDevice : kind == 1 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+    HIBF_TRY_HIP(hipMemcpy(dst, src, bytes, k), "hibf_gpu_memcpy");
+    return HIBF_GPU_OK;
+}
+
+int hibf_gpu_synchronize(void)
+{
+    HIBF_TRY_HIP(hipDeviceSynchronize(), "hibf_gpu_synchronize");
+    return HIBF_GPU_OK;
+}
+
 /* ---------------------------------------------------------------- IBF --- */
 
 int hibf_gpu_ibf_create(uint64_t bins, uint64_t bin_size, uint32_t hash_funs, hibf_gpu_ibf * out)
@@ -1145,6 +1178,11 @@ int get_layout(host::Config const & hc, std::vector<uint64_t> const & counts, ch
 }
 
 } // namespace
+
+uint64_t hibf_gpu_bin_size_in_bits(double fpr, uint64_t hash_count, uint64_t elements)
+{
+    return host::bin_size_in_bits(fpr, hash_count, elements);
+}
 
 int hibf_gpu_config_validate(hibf_gpu_config * c)
 {

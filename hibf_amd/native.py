@@ -79,6 +79,11 @@ _INT = ctypes.c_int
 _SIGNATURES = {
     "hibf_gpu_status_string": (ctypes.c_char_p, [_INT]),
     "hibf_gpu_last_error": (ctypes.c_char_p, []),
+"hibf_gpu_bin_size_in_bits": (_U64, [ctypes.c_double, _U64, _U64]),
+    "hibf_gpu_malloc": (_INT, [ctypes.POINTER(_V), _U64]),
+    "hibf_gpu_free": (_INT, [_V]),
+    "hibf_gpu_memcpy": (_INT, [_V, _V, _U64, _INT]),
+    "hibf_gpu_synchronize": (_INT, []),
     "hibf_gpu_tune_set": (_INT, [ctypes.c_char_p, ctypes.c_int64]),
     "hibf_gpu_tune_get": (_INT, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int64)]),
     "hibf_gpu_ibf_create": (_INT, [_U64, _U64, _U32, ctypes.POINTER(_V)]),

@@ -50,6 +50,13 @@ const char * hibf_gpu_last_error(void);
 int hibf_gpu_tune_set(const char * key, int64_t value);
 int hibf_gpu_tune_get(const char * key, int64_t * value);
 
+/* Device memory helpers so that host code needs no HIP headers.
+ * kind: 0 host->device, 1 device->host, 2 device->device (synchronous). */
+int hibf_gpu_malloc(void ** d_ptr, uint64_t bytes);
+int hibf_gpu_free(void * d_ptr);
+int hibf_gpu_memcpy(void * dst, void const * src, uint64_t bytes, int kind);
+int hibf_gpu_synchronize(void);
+
 /* Sentinels of ibf_bin_to_user_bin_id, hierarchical_interleaved_bloom_filter.hpp:32-40. */
 #define HIBF_GPU_BIN_MERGED 0xFFFFFFFFFFFFFFFFULL
 #define HIBF_GPU_BIN_DELETED 0xFFFFFFFFFFFFFFFEULL
@@ -317,6 +324,10 @@ int hibf_gpu_config_default(hibf_gpu_config * c);
 /* Replaces config::validate_and_set_defaults (src/config.cpp:64-130):
  * HIBF_GPU_ERR_CONFIG with the reference's message (hibf_gpu_last_error). */
 int hibf_gpu_config_validate(hibf_gpu_config * c);
+
+/* build::bin_size_in_bits (include/hibf/build/bin_size_in_bits.hpp:35-45):
+ * bits per technical bin for `elements` values at false-positive rate fpr. */
+uint64_t hibf_gpu_bin_size_in_bits(double fpr, uint64_t hash_count, uint64_t elements);
 
 /* Replaces layout::compute_layout (src/layout/compute_layout.cpp:26-70) for
  * the given user-bin cardinalities, with merged-bin cardinalities taken as
