@@ -161,9 +161,10 @@ CASES = [  # (n_ub, size range, config)
 @pytest.fixture(params=[1, 0], ids=["wave", "block"])
 def descent_kernel(request):
     from hibf_amd import native
+    default = native.tune_get("hibf.wave_kernel")
     native.tune_set("hibf.wave_kernel", request.param)
     yield request.param
-    native.tune_set("hibf.wave_kernel", 1)
+    native.tune_set("hibf.wave_kernel", default)
 
 
 @pytest.mark.parametrize("n_ub,sizes,config", CASES)
