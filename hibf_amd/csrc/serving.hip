@@ -1,0 +1,277 @@
+// Serving loop for HIBF membership queries: the filter stays resident, requests
+// arrive on the host at their own times, are admitted while their device
+// memory fits a budget, gathered for a batching window, copied to the device
+// and answered by the asynchronous membership launch on one of several
+// streams, so batches overlap on the device.  Latency of a request is the time
+// from its arrival to the moment its ids are back on the host.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <deque>
+#include <thread>
+#include <vector>
+
+#include "hibf_gpu.h"
+
+namespace
+{
+
+struct Slot
+{
+    hipStream_t stream{};
+    hipEvent_t done{};
+    hibf_gpu_agent agent{};
+    uint64_t * h_vals{}; // pinned staging
+    uint64_t * h_offs{};
+    uint16_t * h_thr{};
+    uint64_t * h_ids{};
+    uint64_t * h_hs{};
+    uint64_t * h_hc{};
+    uint64_t * d_vals{};
+    uint64_t * d_offs{};
+    uint16_t * d_thr{};
+    uint64_t * d_ids{};
+    uint64_t * d_hs{};
+    uint64_t * d_hc{};
+    uint64_t q0{}, q1{};
+    uint64_t bytes{};
+    bool busy{false};
+};
+
+double now_s(std::chrono::steady_clock::time_point t0)
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+} // namespace
+
+extern "C" int hibf_gpu_hibf_serve(hibf_gpu_hibf hibf,
+                                   hibf_gpu_serve_params const * params,
+                                   uint64_t const * h_values,
+                                   uint64_t const * h_offsets,
+                                   uint16_t const * h_thresholds,
+                                   double const * h_arrival_s,
+                                   uint64_t nq,
+                                   double * h_latency_s,
+                                   uint64_t * h_ids,
+                                   uint32_t * h_counts,
+                                   hibf_gpu_serve_stats * stats)
+{
+    if (!hibf || !params || !h_values || !h_offsets || !h_thresholds || !h_arrival_s || !h_latency_s || !h_ids
+        || !h_counts)
+        return HIBF_GPU_ERR_INVALID_ARGUMENT;
+    hibf_gpu_serve_params const p = *params;
+    if (p.streams == 0 || p.max_batch == 0 || p.results_per_query == 0 || p.max_batch_values == 0)
+        return HIBF_GPU_ERR_INVALID_ARGUMENT;
+    uint32_t const R = p.results_per_query;
+    auto bytes_of = [&](uint64_t q) { return (h_offsets[q + 1] - h_offsets[q]) * 8u + 34u + 8u * R; };
+    for (uint64_t q = 0; q < nq; ++q)
+    {
+        if (h_offsets[q + 1] - h_offsets[q] > p.max_batch_values)
+            return HIBF_GPU_ERR_INVALID_ARGUMENT; // a single request larger than a batch
+        if (bytes_of(q) > p.budget_bytes)
+            return HIBF_GPU_ERR_BUDGET; // could never be admitted
+        if (q && h_arrival_s[q] < h_arrival_s[q - 1])
+            return HIBF_GPU_ERR_INVALID_ARGUMENT;
+    }
+
+    std::vector<Slot> slots(p.streams);
+    int rc = HIBF_GPU_OK;
+    auto cleanup = [&]()
+    {
+        for (Slot & s : slots)
+        {
+            if (s.stream)
+                (void)hipStreamSynchronize(s.stream);
+            for (void * d : {static_cast<void *>(s.d_vals), static_cast<void *>(s.d_offs), static_cast<void *>(s.d_thr),
+                             static_cast<void *>(s.d_ids), static_cast<void *>(s.d_hs), static_cast<void *>(s.d_hc)})
+                if (d)
+                    (void)hipFree(d);
+            for (void * hp : {static_cast<void *>(s.h_vals), static_cast<void *>(s.h_offs),
+                              static_cast<void *>(s.h_thr), static_cast<void *>(s.h_ids), static_cast<void *>(s.h_hs),
+                              static_cast<void *>(s.h_hc)})
+                if (hp)
+                    (void)hipHostFree(hp);
+            if (s.done)
+                (void)hipEventDestroy(s.done);
+            if (s.stream)
+                (void)hipStreamDestroy(s.stream);
+            if (s.agent)
+                hibf_gpu_agent_destroy(s.agent);
+        }
+    };
+#define SERVE_TRY(expr)                                                                                                \
+    do                                                                                                                 \
+    {                                                                                                                  \
+        if ((expr) != hipSuccess)                                                                                      \
+        {                                                                                                              \
+            (void)hipGetLastError();                                                                                   \
+            cleanup();                                                                                                 \
+            return HIBF_GPU_ERR_DEVICE;                                                                                \
+        }                                                                                                              \
+    } while (0)
+    uint64_t const B = p.max_batch;
+    uint64_t const V = p.max_batch_values;
+    for (Slot & s : slots)
+    {
+        SERVE_TRY(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
+        SERVE_TRY(hipEventCreateWithFlags(&s.done, hipEventDisableTiming));
+        SERVE_TRY(hipHostMalloc(reinterpret_cast<void **>(&s.h_vals), V * 8u, hipHostMallocDefault));
+        SERVE_TRY(hipHostMalloc(reinterpret_cast<void **>(&s.h_offs), (B + 1) * 8u, hipHostMallocDefault));
+        SERVE_TRY(hipHostMalloc(reinterpret_cast<void **>(&s.h_thr), B * 2u, hipHostMallocDefault));
+        SERVE_TRY(hipHostMalloc(reinterpret_cast<void **>(&s.h_ids), B * R * 8u, hipHostMallocDefault));
+        SERVE_TRY(hipHostMalloc(reinterpret_cast<void **>(&s.h_hs), B * 8u, hipHostMallocDefault));
+        SERVE_TRY(hipHostMalloc(reinterpret_cast<void **>(&s.h_hc), B * 8u, hipHostMallocDefault));
+        SERVE_TRY(hipMalloc(reinterpret_cast<void **>(&s.d_vals), V * 8u));
+        SERVE_TRY(hipMalloc(reinterpret_cast<void **>(&s.d_offs), (B + 1) * 8u));
+        SERVE_TRY(hipMalloc(reinterpret_cast<void **>(&s.d_thr), B * 2u));
+        SERVE_TRY(hipMalloc(reinterpret_cast<void **>(&s.d_ids), B * R * 8u));
+        SERVE_TRY(hipMalloc(reinterpret_cast<void **>(&s.d_hs), B * 8u));
+        SERVE_TRY(hipMalloc(reinterpret_cast<void **>(&s.d_hc), B * 8u));
+        if ((rc = hibf_gpu_agent_create(B, &s.agent)) != HIBF_GPU_OK ||
+            (rc = hibf_gpu_hibf_membership_prepare(hibf, s.agent, B)) != HIBF_GPU_OK)
+        {
+            cleanup();
+            return rc;
+        }
+        uint64_t v = 0, lk = 0, rb = 0;
+        (void)hibf_gpu_agent_stats(s.agent, &v, &lk, &rb, 1);
+    }
+
+    uint64_t next = 0;          // next request to arrive
+    uint64_t pending = 0;       // first admitted request not yet launched
+    uint64_t completed = 0;
+    uint64_t in_flight_bytes = 0;
+    uint64_t max_in_flight = 0;
+    uint64_t batches = 0;
+    uint64_t overflowed = 0;
+    double const window = p.window_us * 1e-6;
+    auto const t0 = std::chrono::steady_clock::now();
+    double last_done = 0.0;
+    while (completed < nq)
+    {
+        double const now = now_s(t0);
+        // Completed batches: results back on the host.
+        for (Slot & s : slots)
+        {
+            if (!s.busy)
+                continue;
+            hipError_t const e = hipEventQuery(s.done);
+            if (e == hipErrorNotReady)
+                continue;
+            if (e != hipSuccess)
+            {
+                (void)hipGetLastError();
+                cleanup();
+                return HIBF_GPU_ERR_DEVICE;
+            }
+            double const t = now_s(t0);
+            for (uint64_t q = s.q0; q < s.q1; ++q)
+            {
+                uint64_t const i = q - s.q0;
+                uint64_t const n = s.h_hc[i];
+                uint64_t const cap = (s.q1 - s.q0) * R;
+                uint64_t keep = std::min<uint64_t>(n, R);
+                if (s.h_hs[i] > cap || keep > cap - s.h_hs[i])
+                    keep = 0; // the batch's id buffer was full: counted as overflowed
+                overflowed += (n > R || keep < std::min<uint64_t>(n, R));
+                h_counts[q] = static_cast<uint32_t>(n);
+                if (keep)
+                    std::memcpy(h_ids + q * R, s.h_ids + s.h_hs[i], keep * 8u);
+                h_latency_s[q] = t - h_arrival_s[q];
+            }
+            completed += s.q1 - s.q0;
+            in_flight_bytes -= s.bytes;
+            s.busy = false;
+            last_done = t;
+        }
+        // Admission: arrived requests while their memory fits the budget.
+        while (next < nq && h_arrival_s[next] <= now && in_flight_bytes + bytes_of(next) <= p.budget_bytes)
+        {
+            in_flight_bytes += bytes_of(next);
+            ++next;
+        }
+        max_in_flight = std::max(max_in_flight, in_flight_bytes);
+        // Launch: the oldest admitted request has waited a window, or a batch is full.
+        if (pending < next)
+        {
+            uint64_t q1 = pending;
+            uint64_t nv = 0;
+            while (q1 < next && q1 - pending < B && nv + (h_offsets[q1 + 1] - h_offsets[q1]) <= V)
+            {
+                nv += h_offsets[q1 + 1] - h_offsets[q1];
+                ++q1;
+            }
+            bool const full = q1 - pending >= B || q1 < next;
+            bool const waited = now - h_arrival_s[pending] >= window;
+            bool const drained = next == nq;
+            if (full || waited || drained)
+            {
+                auto free_slot = std::find_if(slots.begin(), slots.end(), [](Slot const & s) { return !s.busy; });
+                if (free_slot != slots.end())
+                {
+                    Slot & s = *free_slot;
+                    uint64_t const nb = q1 - pending;
+                    uint64_t const base = h_offsets[pending];
+                    std::memcpy(s.h_vals, h_values + base, nv * 8u);
+                    for (uint64_t i = 0; i <= nb; ++i)
+                        s.h_offs[i] = h_offsets[pending + i] - base;
+                    std::memcpy(s.h_thr, h_thresholds + pending, nb * 2u);
+                    SERVE_TRY(hipMemcpyAsync(s.d_vals, s.h_vals, std::max<uint64_t>(nv, 1) * 8u, hipMemcpyHostToDevice,
+                                             s.stream));
+                    SERVE_TRY(hipMemcpyAsync(s.d_offs, s.h_offs, (nb + 1) * 8u, hipMemcpyHostToDevice, s.stream));
+                    SERVE_TRY(hipMemcpyAsync(s.d_thr, s.h_thr, nb * 2u, hipMemcpyHostToDevice, s.stream));
+                    rc = hibf_gpu_hibf_membership_launch(hibf, s.agent, s.d_vals, s.d_offs, nb, s.d_thr, s.d_ids,
+                                                         nb * R, s.d_hs, s.d_hc, s.stream);
+                    if (rc != HIBF_GPU_OK)
+                    {
+                        cleanup();
+                        return rc;
+                    }
+                    SERVE_TRY(hipMemcpyAsync(s.h_hs, s.d_hs, nb * 8u, hipMemcpyDeviceToHost, s.stream));
+                    SERVE_TRY(hipMemcpyAsync(s.h_hc, s.d_hc, nb * 8u, hipMemcpyDeviceToHost, s.stream));
+                    SERVE_TRY(hipMemcpyAsync(s.h_ids, s.d_ids, nb * R * 8u, hipMemcpyDeviceToHost, s.stream));
+                    SERVE_TRY(hipEventRecord(s.done, s.stream));
+                    s.q0 = pending;
+                    s.q1 = q1;
+                    s.bytes = 0;
+                    for (uint64_t q = pending; q < q1; ++q)
+                        s.bytes += bytes_of(q);
+                    s.busy = true;
+                    pending = q1;
+                    ++batches;
+                }
+            }
+        }
+        if (next < nq && pending == next && std::all_of(slots.begin(), slots.end(), [](Slot const & s) { return !s.busy; }))
+        {
+            // Idle until the next arrival.
+            double const wait = h_arrival_s[next] - now_s(t0);
+            if (wait > 50e-6)
+                std::this_thread::sleep_for(std::chrono::duration<double>(wait - 20e-6));
+        }
+    }
+    if (stats)
+    {
+        uint64_t lookups = 0, visits = 0, row_bytes = 0;
+        for (Slot & s : slots)
+        {
+            uint64_t v = 0, lk = 0, rb = 0;
+            (void)hibf_gpu_agent_stats(s.agent, &v, &lk, &rb, 0);
+            lookups += lk;
+            visits += v;
+            row_bytes += rb;
+        }
+        stats->seconds = last_done - h_arrival_s[0];
+        stats->batches = batches;
+        stats->lookups = lookups;
+        stats->row_bytes = row_bytes;
+        stats->max_in_flight_bytes = max_in_flight;
+        stats->overflowed = overflowed;
+    }
+    cleanup();
+    return HIBF_GPU_OK;
+#undef SERVE_TRY
+}

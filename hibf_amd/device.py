@@ -326,6 +326,27 @@ class DeviceHibf:
                                                     _ptr(thresholds), _ptr(ids), ids.numel(), _ptr(hit_start),
                                                     _ptr(hit_count), s))
 
+    def serve(self, values: np.ndarray, offsets: np.ndarray, thresholds: np.ndarray, arrival_s: np.ndarray,
+              window_us: float = 500.0, max_batch: int = 65536, max_batch_values: int = 1 << 24,
+              budget_bytes: int = 4 << 30, streams: int = 4, results_per_query: int = 16):
+        """Runs the serving loop over requests arriving at arrival_s (host arrays).
+        Returns (latency_s, ids[nq, R], counts, stats dict)."""
+        v = np.ascontiguousarray(values, dtype=np.uint64)
+        o = np.ascontiguousarray(offsets, dtype=np.uint64)
+        t = np.ascontiguousarray(thresholds, dtype=np.uint16)
+        a = np.ascontiguousarray(arrival_s, dtype=np.float64)
+        nq = o.size - 1
+        lat = np.zeros(nq, dtype=np.float64)
+        ids = np.zeros((nq, results_per_query), dtype=np.uint64)
+        cnt = np.zeros(nq, dtype=np.uint32)
+        p = native.ServeParams(window_us, max_batch, max_batch_values, budget_bytes, streams, results_per_query)
+        st = native.ServeStats()
+        P = lambda x: x.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
+        check(lib().hibf_gpu_hibf_serve(self.h, ctypes.byref(p), P(v), P(o), P(t), P(a), nq, P(lat), P(ids), P(cnt),
+                                        ctypes.byref(st)))
+        stats = {k: getattr(st, k) for k, _ in native.ServeStats._fields_}
+        return lat, ids, cnt, stats
+
     def bulk_count(self, values, offsets, thresholds, counter_bytes: int = 2, agent: Agent | None = None):
         v = as_device_u64(values)
         o = as_device_u64(offsets)

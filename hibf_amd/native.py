@@ -59,6 +59,17 @@ class Config(ctypes.Structure):
     ]
 
 
+class ServeParams(ctypes.Structure):
+    _fields_ = [("window_us", ctypes.c_double), ("max_batch", ctypes.c_uint64), ("max_batch_values", ctypes.c_uint64),
+                ("budget_bytes", ctypes.c_uint64), ("streams", ctypes.c_uint32), ("results_per_query", ctypes.c_uint32)]
+
+
+class ServeStats(ctypes.Structure):
+    _fields_ = [("seconds", ctypes.c_double), ("batches", ctypes.c_uint64), ("lookups", ctypes.c_uint64),
+                ("row_bytes", ctypes.c_uint64), ("max_in_flight_bytes", ctypes.c_uint64),
+                ("overflowed", ctypes.c_uint64)]
+
+
 class Geometry(ctypes.Structure):
     _fields_ = [
         ("bins", ctypes.c_uint64),
@@ -112,6 +123,8 @@ _SIGNATURES = {
     "hibf_gpu_hibf_arena_words": (_INT, [_V, ctypes.POINTER(_U64)]),
     "hibf_gpu_hibf_membership_for": (_INT, [_V, _V, _V, _V, _U64, _V, _V, _U64, _V, ctypes.POINTER(_U64), _V]),
     "hibf_gpu_hibf_bulk_count": (_INT, [_V, _V, _V, _V, _U64, _V, _U32, _V, _V]),
+    "hibf_gpu_hibf_serve": (_INT, [_V, ctypes.POINTER(ServeParams), _V, _V, _V, _V, _U64, _V, _V, _V,
+                                   ctypes.POINTER(ServeStats)]),
     "hibf_gpu_hibf_info": (_INT, [_V] + [ctypes.POINTER(_U64)] * 5),
     "hibf_gpu_hibf_export_tables": (_INT, [_V, ctypes.POINTER(Geometry), _V, _V, _V, _V]),
     "hibf_gpu_config_default": (_INT, [ctypes.POINTER(Config)]),

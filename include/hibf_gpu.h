@@ -264,6 +264,49 @@ int hibf_gpu_hibf_membership_launch(hibf_gpu_hibf hibf,
                                     uint64_t * d_hit_count,
                                     void * stream);
 
+/* Serving loop (no counterpart in the reference, whose agents answer one
+ * query per call).  Requests q = 0..nq-1 arrive at h_arrival_s[q] seconds
+ * (non-decreasing, relative to the call's start) with values
+ * h_values[h_offsets[q] .. h_offsets[q+1]) and threshold h_thresholds[q].
+ * A request is admitted once it has arrived and its device memory fits
+ * budget_bytes; admitted requests are batched until the oldest has waited
+ * window_us (or a batch holds max_batch requests / max_batch_values values),
+ * copied to the device and answered on one of `streams` streams, so batches
+ * overlap.  Outputs: h_latency_s[q] (arrival to ids on the host), up to
+ * results_per_query ids per request in h_ids[q*R ..] with the full count in
+ * h_counts[q] (ids in the reference's depth-first order). */
+typedef struct hibf_gpu_serve_params
+{
+    double window_us;
+    uint64_t max_batch;
+    uint64_t max_batch_values;
+    uint64_t budget_bytes;
+    uint32_t streams;
+    uint32_t results_per_query;
+} hibf_gpu_serve_params;
+
+typedef struct hibf_gpu_serve_stats
+{
+    double seconds;               /* first arrival to last completion */
+    uint64_t batches;
+    uint64_t lookups;             /* query values checked against one IBF */
+    uint64_t row_bytes;
+    uint64_t max_in_flight_bytes; /* peak admitted device memory */
+    uint64_t overflowed;          /* requests with more than results_per_query ids */
+} hibf_gpu_serve_stats;
+
+int hibf_gpu_hibf_serve(hibf_gpu_hibf hibf,
+                        const hibf_gpu_serve_params * params,
+                        uint64_t const * h_values,
+                        uint64_t const * h_offsets,
+                        uint16_t const * h_thresholds,
+                        double const * h_arrival_s,
+                        uint64_t nq,
+                        double * h_latency_s,
+                        uint64_t * h_ids,
+                        uint32_t * h_counts,
+                        hibf_gpu_serve_stats * stats);
+
 /* Batched HIBF counting_agent_type<T>::bulk_count
  * (hierarchical_interleaved_bloom_filter.hpp:462-496, 549-566): d_counts gets
  * nq * number_of_user_bins counters of counter_bytes (1, 2, 4, 8); a user bin's

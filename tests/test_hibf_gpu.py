@@ -206,3 +206,29 @@ def test_hibf_synthetic_build_matches_oracle():
     assert _membership_lists(dev, qs, ts) == [ref.membership_for(q, t) for q, t in zip(qs, ts)]
     values, offsets = _csr(qs)
     assert np.array_equal(to_numpy(dev.bulk_count(values, offsets, ts)), ref.bulk_count_batch(values, offsets, ts))
+
+
+def test_serving_results_equal_alone_and_in_the_mix():
+    """Each request's ids from the serving loop (batched with others, several
+    streams in flight) equal its ids when it is answered alone."""
+    from hibf_amd.device import DeviceHibf
+    _say("serving mixed vs alone")
+    rng = np.random.default_rng(21)
+    ubs = _random_user_bins(rng, 400, 20, 300)
+    dev = DeviceHibf.build(ubs, tmax=64)
+    qs = _mixed_queries(rng, ubs, 3000, 100)
+    ts = np.array([int(0.3 * len(q)) for q in qs], dtype=np.uint16)
+    ts[::97] = 1   # some requests with many hits
+    values, offsets = _csr(qs)
+    arrival = np.cumsum(rng.exponential(1 / 200000, size=len(qs)))
+    arrival -= arrival[0]
+    lat, ids, cnt, st = dev.serve(values, offsets, ts, arrival, window_us=300, streams=3, max_batch=256,
+                                  results_per_query=64)
+    assert (lat > 0).all() and st["batches"] > 1
+    alone = [ _membership_lists(dev, [qs[i]], [int(ts[i])])[0] for i in range(0, len(qs), 37)]
+    for k, i in enumerate(range(0, len(qs), 37)):
+        assert int(cnt[i]) == len(alone[k])
+        assert ids[i, :min(64, int(cnt[i]))].tolist() == alone[k][:64]
+    batch = _membership_lists(dev, qs, ts)
+    for i in range(len(qs)):
+        assert int(cnt[i]) == len(batch[i]) and ids[i, :min(64, int(cnt[i]))].tolist() == batch[i][:64]
