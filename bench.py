@@ -269,6 +269,13 @@ def run_hibf(args):
     def upload_u16(a):
         return torch.from_numpy(np.ascontiguousarray(a).view(np.int16)).to(dev).view(torch.uint16)
 
+    if args.calibrate_copy:
+        a = torch.ones(1 << 27, dtype=torch.int64, device=dev)
+        b = torch.empty_like(a)
+        torch.add(a, 0, out=b)  # reads 1 GiB, writes 1 GiB
+        torch.cuda.synchronize()
+        del a, b
+
     # (a) fixed-length batch, one persistent launch per step.
     nq = args.hibf_queries
     lengths = W.query_lengths(args.seed + 3, nq, fixed=250)
