@@ -71,6 +71,17 @@ __device__ __forceinline__ uint64_t tail_mask(IbfView const & g, uint32_t w)
     return (w + 1u == g.bin_words && r) ? ((1ULL << r) - 1ULL) : ~0ULL;
 }
 
+// Row-word load; NT marks it non-temporal (streaming) so the line is not kept
+// in the caches for reuse that random rows never get.
+template <bool NT>
+__device__ __forceinline__ uint64_t load_row_word(uint64_t const * p)
+{
+    if constexpr (NT)
+        return __builtin_nontemporal_load(p);
+    else
+        return *p;
+}
+
 __device__ __forceinline__ uint64_t shfl64(uint64_t x, int src)
 {
     uint32_t lo = static_cast<uint32_t>(x);
@@ -98,7 +109,7 @@ __device__ __forceinline__ uint64_t readlane64(uint64_t x, int src)
 // words updated with ds_add_u32 (values of one step may hit the same bin).
 // Callers truncate them to the agent's counter width, which gives the
 // reference's modulo-2^w wrap exactly.
-template <int LPV, int CW, int U>
+template <int LPV, int CW, int U, bool NT = false>
 __device__ __forceinline__ void count_into_lds(IbfView const & g,
                                                uint64_t const * __restrict__ vals,
                                                uint32_t L,
@@ -154,11 +165,11 @@ __device__ __forceinline__ void count_into_lds(IbfView const & g,
                     if (act && wt < tw)
                     {
                         uint64_t const w = w0 + wt;
-                        acc = words[r[0] + w];
+                        acc = load_row_word<NT>(words + r[0] + w);
 #pragma unroll
                         for (int k = 1; k < 5; ++k)
                             if (static_cast<uint32_t>(k) < hf)
-                                acc &= words[r[k] + w];
+                                acc &= load_row_word<NT>(words + r[k] + w);
                         acc &= tail_mask(g, static_cast<uint32_t>(w));
                     }
                     x[u][c] = acc;
@@ -188,7 +199,7 @@ __device__ __forceinline__ void count_into_lds(IbfView const & g,
 // LPV = 1..64 with one word per lane (bin_words <= 64); 7 gives 64 lanes with
 // two words each (bin_words <= 128).  Used by kernels that visit IBFs of
 // different widths (the HIBF descent).
-template <int U>
+template <int U, bool NT = false>
 __device__ __forceinline__ void count_into_lds_dispatch(uint32_t lpv_log,
                                                         IbfView const & g,
                                                         uint64_t const * __restrict__ vals,
@@ -200,14 +211,14 @@ __device__ __forceinline__ void count_into_lds_dispatch(uint32_t lpv_log,
 {
     switch (lpv_log)
     {
-    case 0: count_into_lds<1, 1, U>(g, vals, L, 0, g.bin_words, lds, wave, nwaves, lane); break;
-    case 1: count_into_lds<2, 1, U>(g, vals, L, 0, g.bin_words, lds, wave, nwaves, lane); break;
-    case 2: count_into_lds<4, 1, U>(g, vals, L, 0, g.bin_words, lds, wave, nwaves, lane); break;
-    case 3: count_into_lds<8, 1, U>(g, vals, L, 0, g.bin_words, lds, wave, nwaves, lane); break;
-    case 4: count_into_lds<16, 1, U>(g, vals, L, 0, g.bin_words, lds, wave, nwaves, lane); break;
-    case 5: count_into_lds<32, 1, U>(g, vals, L, 0, g.bin_words, lds, wave, nwaves, lane); break;
-    case 6: count_into_lds<64, 1, U>(g, vals, L, 0, g.bin_words, lds, wave, nwaves, lane); break;
-    default: count_into_lds<64, 2, U>(g, vals, L, 0, g.bin_words, lds, wave, nwaves, lane); break;
+    case 0: count_into_lds<1, 1, U, NT>(g, vals, L, 0, g.bin_words, lds, wave, nwaves, lane); break;
+    case 1: count_into_lds<2, 1, U, NT>(g, vals, L, 0, g.bin_words, lds, wave, nwaves, lane); break;
+    case 2: count_into_lds<4, 1, U, NT>(g, vals, L, 0, g.bin_words, lds, wave, nwaves, lane); break;
+    case 3: count_into_lds<8, 1, U, NT>(g, vals, L, 0, g.bin_words, lds, wave, nwaves, lane); break;
+    case 4: count_into_lds<16, 1, U, NT>(g, vals, L, 0, g.bin_words, lds, wave, nwaves, lane); break;
+    case 5: count_into_lds<32, 1, U, NT>(g, vals, L, 0, g.bin_words, lds, wave, nwaves, lane); break;
+    case 6: count_into_lds<64, 1, U, NT>(g, vals, L, 0, g.bin_words, lds, wave, nwaves, lane); break;
+    default: count_into_lds<64, 2, U, NT>(g, vals, L, 0, g.bin_words, lds, wave, nwaves, lane); break;
     }
 }
 

@@ -108,7 +108,7 @@ __device__ void scan_ibf(HibfIbfDesc const & d,
 // explicit stack of per-level event lists in LDS, which reproduces the
 // reference's recursion order (hierarchical_interleaved_bloom_filter.hpp:321-355
 // and :462-496) without recursion.
-template <bool COUNTING>
+template <bool COUNTING, bool NT>
 __global__ void __launch_bounds__(256) hibf_descent_kernel(HibfLaunch h,
                                                            uint64_t const * __restrict__ values,
                                                            uint64_t const * __restrict__ offsets,
@@ -191,7 +191,7 @@ __global__ void __launch_bounds__(256) hibf_descent_kernel(HibfLaunch h,
                     cnt[i] = 0u;
                 __syncthreads();
                 IbfView const g{h.arena + d.word_offset, d.bin_size, d.bins, d.bin_words, d.hash_shift, d.hash_funs};
-                count_into_lds_dispatch<2>(d.lpv_log, g, vals, L, cnt, wave, nwaves, lane);
+                count_into_lds_dispatch<2, NT>(d.lpv_log, g, vals, L, cnt, wave, nwaves, lane);
                 ++visits;
                 lookups += L;
                 row_bytes += static_cast<uint64_t>(L) * d.hash_funs * d.bin_words * 8u;
@@ -398,8 +398,12 @@ hipError_t launch_hibf_membership(HibfLaunch const & h,
     if (nq == 0)
         return hipSuccess;
     size_t const lds = static_cast<size_t>(h.max_bins) * (1u + h.depth) * sizeof(uint32_t);
-    hipLaunchKernelGGL(hibf_descent_kernel<false>, dim3(blocks), dim3(256), lds, stream, h, values, offsets, nq,
-                       thresholds, nullptr, 2u, nullptr, scratch, ids_tmp, ids_cap, hit_start, hit_count, qlist);
+    if (g_count_tuning.nt_loads)
+        hipLaunchKernelGGL((hibf_descent_kernel<false, true>), dim3(blocks), dim3(256), lds, stream, h, values, offsets,
+                           nq, thresholds, nullptr, 2u, nullptr, scratch, ids_tmp, ids_cap, hit_start, hit_count, qlist);
+    else
+        hipLaunchKernelGGL((hibf_descent_kernel<false, false>), dim3(blocks), dim3(256), lds, stream, h, values, offsets,
+                           nq, thresholds, nullptr, 2u, nullptr, scratch, ids_tmp, ids_cap, hit_start, hit_count, qlist);
     return hipGetLastError();
 }
 
@@ -416,7 +420,7 @@ hipError_t launch_hibf_count(HibfLaunch const & h,
     if (nq == 0)
         return hipSuccess;
     size_t const lds = static_cast<size_t>(h.max_bins) * (1u + h.depth) * sizeof(uint32_t);
-    hipLaunchKernelGGL(hibf_descent_kernel<true>, dim3(blocks), dim3(256), lds, stream, h, values, offsets, nq,
+    hipLaunchKernelGGL((hibf_descent_kernel<true, false>), dim3(blocks), dim3(256), lds, stream, h, values, offsets, nq,
                        nullptr, thresholds, counter_bytes, counts, nullptr, nullptr, 0ULL, nullptr, nullptr,
                        nullptr);
     return hipGetLastError();
