@@ -53,11 +53,27 @@ def parse():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--plain-queries", type=int, default=2000)
     ap.add_argument("--check-queries", type=int, default=256)
+    ap.add_argument("--contains-values", type=int, default=1 << 20,
+                    help="values of the bulk_contains leg of the flat workload (0 skips it)")
     ap.add_argument("--tune", action="append", default=[], metavar="KEY=VALUE",
                     help="launch-shape knob passed to hibf_gpu_tune_set (repeatable)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a 1 GiB elementwise copy kernel (known bytes) for counter calibration")
     return ap.parse_args()
+
+
+def recorded_traffic(key: str, args) -> float | None:
+    """HBM bytes per launch measured with rocprofv3 counters on this exact
+    workload (profiles/traffic.json), or None when the workload differs."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            rec = json.load(f)[key]
+    except (OSError, KeyError, ValueError):
+        return None
+    if any(getattr(args, k) != v for k, v in rec["workload"].items()):
+        return None
+    return (rec["fetch_kib"] * rec["fetch_factor"] + rec["write_kib"] * rec["write_factor"]) * 1024.0
 
 
 def log(msg):
@@ -154,6 +170,27 @@ def run_flat(args):
     log(f"plain: {plain_value / 1e9:.4f} G lookups/s; latency p50 {np.percentile(lat, 50):.3f} ms "
         f"p99 {np.percentile(lat, 99):.3f} ms")
 
+    # bulk_contains: one result row (bin_words words) per value; reads h rows, writes one.
+    contains = None
+    nc = min(args.contains_values, values.numel())
+    if nc > 0:
+        cvals = values[:nc]
+        cout = torch.empty((nc, (bins + 63) // 64), dtype=torch.uint64, device=dev)
+        ibf.bulk_contains_into(cvals, cout)
+        torch.cuda.synchronize()
+        cev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(3)]
+        for a, b in cev:
+            a.record(stream)
+            ibf.bulk_contains_into(cvals, cout)
+            b.record(stream)
+        torch.cuda.synchronize()
+        c_ms = float(np.mean([a.elapsed_time(b) for a, b in cev]))
+        c_bytes = (h + 1) * ((bins + 63) // 64) * 8
+        contains = {"value": nc / (c_ms / 1e3), "unit": "values/s", "values": nc, "kernel_ms": c_ms,
+                    "bytes_per_value": c_bytes, "achieved_GBs": c_bytes * nc / (c_ms / 1e3) / 1e9}
+        log(f"bulk_contains: {nc} values in {c_ms:.3f} ms, {contains['value'] / 1e6:.1f} M values/s, "
+            f"{contains['achieved_GBs']:.0f} GB/s")
+
     if args.calibrate_copy:
         a = torch.ones(1 << 27, dtype=torch.int64, device=dev)
         b = torch.empty_like(a)
@@ -179,6 +216,12 @@ def run_flat(args):
         so = np.arange(len(sample) + 1, dtype=np.uint64) * np.uint64(L)
         want = ref.bulk_count_batch(sv, so, 2, threads)
         parity = bool(np.array_equal(got, want))
+        if contains is not None:
+            pick = rng.choice(nc, size=min(256, nc), replace=False)
+            got_c = cout.index_select(0, torch.as_tensor(pick.astype(np.int64), device=dev)).cpu().numpy()
+            want_c = np.stack([ref.bulk_contains(int(values_h[i])) for i in pick])
+            contains["parity_sample_ok"] = bool(np.array_equal(got_c, want_c))
+            parity = parity and contains["parity_sample_ok"]
         log(f"parity vs oracle on {len(sample)} queries: {parity}")
         if not args.skip_cpu:
             probe = 64
@@ -219,7 +262,7 @@ def run_flat(args):
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "frac": achieved / HBM_PEAK_GBS, "traffic": recorded_traffic("flat", args),
             "kernel_ms": float(np.mean(kernel_ms)), "bytes_per_lookup": row_bytes,
         },
         "cpu_baseline": cpu,
@@ -227,6 +270,7 @@ def run_flat(args):
                             "latency_ms_p50": float(np.percentile(lat, 50)),
                             "latency_ms_p99": float(np.percentile(lat, 99))},
         "batched_vs_plain": value / plain_value,
+        "bulk_contains": contains,
         "parity_sample_ok": parity,
         "build_emplace_s": build_s,
     }
@@ -435,7 +479,8 @@ def run_hibf(args):
         "roofline": {"bound": "hbm", "achieved": bytes_b / wall_b / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": bytes_b / wall_b / 1e9 / HBM_PEAK_GBS, "traffic": None},
         "fixed_batch": {"queries": nq, "length": 250, "ms": wall_a * 1e3, "lookups_per_s": lookups_a / wall_a,
-                        "row_gbs": bytes_a / kern_a / 1e9, "kernel_ms": kern_a * 1e3},
+                        "row_gbs": bytes_a / kern_a / 1e9, "kernel_ms": kern_a * 1e3,
+                        "traffic": recorded_traffic("hibf_fixed_batch", args)},
         "plain_per_query": {"value": plain, "unit": "lookups/s", "queries": npl,
                             "latency_ms_p50": float(np.percentile(lat, 50)),
                             "latency_ms_p99": float(np.percentile(lat, 99))},

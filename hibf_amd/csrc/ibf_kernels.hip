@@ -121,25 +121,45 @@ __global__ void __launch_bounds__(256) clear_kernel(uint64_t * __restrict__ word
 }
 
 // bulk_contains for n values: out[i*bin_words + w] = AND_k row_k[w]
-// (src/interleaved_bloom_filter.cpp:204-291).  One thread per output word.
+// (src/interleaved_bloom_filter.cpp:204-291).  One wave per 64 values: lane j
+// hashes value j once, then the wave walks the 64 values, broadcasting each
+// value's row starts (readlane) and moving its bin_words-word rows with
+// coalesced loads and stores.
 __global__ void __launch_bounds__(256) contains_kernel(IbfView g,
                                                        uint64_t const * __restrict__ values,
                                                        uint64_t n,
                                                        uint64_t * __restrict__ out)
 {
-    uint64_t const total = n * g.bin_words;
-    uint64_t const stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
-    for (uint64_t t = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < total; t += stride)
+    int const lane = static_cast<int>(threadIdx.x & 63u);
+    uint64_t const waves = static_cast<uint64_t>(gridDim.x) * (blockDim.x >> 6);
+    uint64_t const bw = g.bin_words;
+    for (uint64_t base = (static_cast<uint64_t>(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 64u;
+         base < n; base += waves * 64u)
     {
-        uint64_t const i = t / g.bin_words;
-        uint64_t const w = t - i * g.bin_words;
-        uint64_t const v = values[i];
-        uint64_t acc = g.words[row_word(v, hash_seed(0), g.hash_shift, g.bin_size, g.bin_words) + w];
+        uint32_t const nv = static_cast<uint32_t>(min<uint64_t>(64u, n - base));
+        uint64_t const v = static_cast<uint32_t>(lane) < nv ? values[base + lane] : 0ULL;
+        uint64_t rows[5];
 #pragma unroll
-        for (int k = 1; k < 5; ++k)
-            if (static_cast<uint32_t>(k) < g.hash_funs)
-                acc &= g.words[row_word(v, hash_seed(k), g.hash_shift, g.bin_size, g.bin_words) + w];
-        out[t] = acc;
+        for (int k = 0; k < 5; ++k)
+            rows[k] = static_cast<uint32_t>(k) < g.hash_funs ? row_word(v, hash_seed(k), g.hash_shift, g.bin_size, bw)
+                                                             : 0ULL;
+        for (uint32_t i = 0; i < nv; ++i)
+        {
+            uint64_t r[5];
+#pragma unroll
+            for (int k = 0; k < 5; ++k)
+                r[k] = static_cast<uint32_t>(k) < g.hash_funs ? readlane64(rows[k], static_cast<int>(i)) : 0ULL;
+            uint64_t * const o = out + (base + i) * bw;
+            for (uint64_t w = static_cast<uint64_t>(lane); w < bw; w += 64u)
+            {
+                uint64_t acc = g.words[r[0] + w];
+#pragma unroll
+                for (int k = 1; k < 5; ++k)
+                    if (static_cast<uint32_t>(k) < g.hash_funs)
+                        acc &= g.words[r[k] + w];
+                o[w] = acc;
+            }
+        }
     }
 }
 
@@ -282,8 +302,7 @@ hipError_t launch_contains(IbfView const & g, uint64_t const * values, uint64_t 
 {
     if (n == 0)
         return hipSuccess;
-    hipLaunchKernelGGL(contains_kernel, dim3(grid_for(n * g.bin_words, 256)), dim3(256), 0, stream, g, values, n,
-                       out);
+    hipLaunchKernelGGL(contains_kernel, dim3(grid_for(n, 256)), dim3(256), 0, stream, g, values, n, out);
     return hipGetLastError();
 }
 

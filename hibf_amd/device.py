@@ -161,6 +161,16 @@ class DeviceIbf:
         check(lib().hibf_gpu_ibf_bulk_contains(self.h, _ptr(v), v.numel(), _ptr(out), _stream()))
         return out
 
+    def bulk_contains_into(self, values: torch.Tensor, out: torch.Tensor) -> None:
+        """bulk_contains into a preallocated (n, bin_words) uint64 device tensor."""
+        n = values.numel()
+        if values.dtype not in (torch.uint64, torch.int64) or not values.is_contiguous():
+            raise ValueError("values must be a contiguous 64-bit device tensor")
+        if out.dtype not in (torch.uint64, torch.int64) or out.numel() < n * int(self.geometry.bin_words) \
+                or not out.is_contiguous():
+            raise ValueError("out must be a contiguous 64-bit tensor of at least n * bin_words words")
+        check(lib().hibf_gpu_ibf_bulk_contains(self.h, _ptr(values), n, _ptr(out), _stream()))
+
     def bulk_count(self, values, offsets, counter_bytes: int = 2) -> torch.Tensor:
         v = as_device_u64(values)
         o = as_device_u64(offsets)
