@@ -162,6 +162,7 @@ struct hibf_gpu_hibf_s
     uint64_t n_user_bins{0};
     uint32_t max_bins{0};
     uint32_t depth{0};
+    uint32_t uniform_hash_funs{0}; // hash-function count shared by every IBF, 0 if they differ
     uint64_t * d_arena{nullptr};
     HibfIbfDesc * d_desc{nullptr};
     uint64_t * d_tb_info{nullptr};
@@ -646,6 +647,10 @@ int hibf_gpu_hibf_create(uint64_t n_ibfs,
                               cw == 2u ? 7u : lpv_log,
                               0u};
         h->max_bins = std::max<uint32_t>(h->max_bins, static_cast<uint32_t>(g.bins));
+        if (i == 0)
+            h->uniform_hash_funs = static_cast<uint32_t>(g.hash_funs);
+        else if (h->uniform_hash_funs != g.hash_funs)
+            h->uniform_hash_funs = 0u;
         uint64_t const * ub = h_user_bin + h_tb_offsets[i];
         uint64_t const * nx = h_next_ibf + h_tb_offsets[i];
         for (uint64_t b = 0; b < g.bins; ++b)
@@ -825,7 +830,7 @@ namespace
 HibfLaunch launch_of(hibf_gpu_hibf h, hibf_gpu_agent a)
 {
     return HibfLaunch{h->d_desc,      h->d_arena, h->d_tb_info, h->max_bins, h->depth, h->n_user_bins,
-                      (h->max_bins + 63u) / 64u, 0u, a->d_total};
+                      (h->max_bins + 63u) / 64u, h->uniform_hash_funs, a->d_total};
 }
 
 unsigned hibf_blocks(uint64_t nq, uint64_t per_block_scratch_bytes)

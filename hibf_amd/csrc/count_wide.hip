@@ -68,8 +68,10 @@ __device__ void store_tile(uint32_t const * lds,
         put_counter(counts, first + i, cb, (lds[i >> 1] >> (16u * (i & 1u))) & 0xFFFFu, accumulate);
 }
 
-template <int U>
-__global__ void __launch_bounds__(256) count_wide_kernel(IbfView g,
+// HF > 0 fixes the number of hash functions (== g.hash_funs) at compile time,
+// so only HF row starts per value occupy registers; HF == 0 reads it at run time.
+template <int U, int HF>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((HF > 0 && U <= 4) ? 8 : 1, 8))) count_wide_kernel(IbfView g,
                                                          uint64_t const * __restrict__ values,
                                                          uint64_t const * __restrict__ offsets,
                                                          uint64_t nq,
@@ -85,6 +87,7 @@ __global__ void __launch_bounds__(256) count_wide_kernel(IbfView g,
     uint64_t const bw = g.bin_words;
     uint32_t const hf = g.hash_funs;
     uint64_t const * __restrict__ words = g.words;
+    constexpr int KMAX = HF > 0 ? HF : 5;
 
     for (uint64_t q = blockIdx.x; q < nq; q += gridDim.x)
     {
@@ -110,10 +113,10 @@ __global__ void __launch_bounds__(256) count_wide_kernel(IbfView g,
                 {
                     uint32_t const nv = min(64u, segL - base);
                     uint64_t const v = static_cast<uint32_t>(lane) < nv ? vals[seg0 + base + lane] : 0ULL;
-                    uint64_t rows[5];
+                    uint64_t rows[KMAX];
 #pragma unroll
-                    for (int k = 0; k < 5; ++k)
-                        rows[k] = static_cast<uint32_t>(k) < hf
+                    for (int k = 0; k < KMAX; ++k)
+                        rows[k] = (HF > 0 || static_cast<uint32_t>(k) < hf)
                                       ? row_word(v, hash_seed(k), g.hash_shift, g.bin_size, bw) + w0
                                       : 0ULL;
                     for (uint32_t s = 0; s < nv; s += U)
@@ -133,8 +136,8 @@ __global__ void __launch_bounds__(256) count_wide_kernel(IbfView g,
                                 {
                                     ulonglong2 a = *reinterpret_cast<ulonglong2 const *>(words + r0);
 #pragma unroll
-                                    for (int k = 1; k < 5; ++k)
-                                        if (static_cast<uint32_t>(k) < hf)
+                                    for (int k = 1; k < KMAX; ++k)
+                                        if (HF > 0 || static_cast<uint32_t>(k) < hf)
                                         {
                                             uint64_t const rk = readlane64(rows[k], static_cast<int>(idx)) + wl;
                                             ulonglong2 const b = *reinterpret_cast<ulonglong2 const *>(words + rk);
@@ -200,25 +203,18 @@ hipError_t launch_count_wide(IbfView const & g,
     size_t const lds = static_cast<size_t>(kWideTileWords) * 32u * sizeof(uint32_t);
     unsigned const grid = static_cast<unsigned>(nq < max_blocks ? nq : max_blocks);
     dim3 const blk(64u * g_count_tuning.wide_waves);
-    switch (g_count_tuning.wide_unroll)
-    {
-    case 1:
-        hipLaunchKernelGGL((count_wide_kernel<1>), dim3(grid), blk, lds, stream, g, values, offsets, nq,
-                           counter_bytes, counts);
-        break;
-    case 2:
-        hipLaunchKernelGGL((count_wide_kernel<2>), dim3(grid), blk, lds, stream, g, values, offsets, nq,
-                           counter_bytes, counts);
-        break;
-    case 8:
-        hipLaunchKernelGGL((count_wide_kernel<8>), dim3(grid), blk, lds, stream, g, values, offsets, nq,
-                           counter_bytes, counts);
-        break;
-    default:
-        hipLaunchKernelGGL((count_wide_kernel<4>), dim3(grid), blk, lds, stream, g, values, offsets, nq,
-                           counter_bytes, counts);
-        break;
-    }
+    using kernel_t = void (*)(IbfView, uint64_t const *, uint64_t const *, uint64_t, uint32_t, void *);
+    // [unroll 1, 2, 4, 8][hash functions: run time, 2, 3]
+    static kernel_t const table[4][3] = {
+        {count_wide_kernel<1, 0>, count_wide_kernel<1, 2>, count_wide_kernel<1, 3>},
+        {count_wide_kernel<2, 0>, count_wide_kernel<2, 2>, count_wide_kernel<2, 3>},
+        {count_wide_kernel<4, 0>, count_wide_kernel<4, 2>, count_wide_kernel<4, 3>},
+        {count_wide_kernel<8, 0>, count_wide_kernel<8, 2>, count_wide_kernel<8, 3>},
+    };
+    uint32_t const u = g_count_tuning.wide_unroll;
+    int const ui = u == 1u ? 0 : u == 2u ? 1 : u == 8u ? 3 : 2;
+    int const hi = g.hash_funs == 2u ? 1 : g.hash_funs == 3u ? 2 : 0;
+    hipLaunchKernelGGL(table[ui][hi], dim3(grid), blk, lds, stream, g, values, offsets, nq, counter_bytes, counts);
     return hipGetLastError();
 }
 

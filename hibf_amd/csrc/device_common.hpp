@@ -109,7 +109,11 @@ __device__ __forceinline__ uint64_t readlane64(uint64_t x, int src)
 // words updated with ds_add_u32 (values of one step may hit the same bin).
 // Callers truncate them to the agent's counter width, which gives the
 // reference's modulo-2^w wrap exactly.
-template <int LPV, int CW, int U, bool NT = false>
+//
+// HF > 0 fixes the number of hash functions at compile time (it must equal
+// g.hash_funs), which keeps only HF row starts per value in registers; HF == 0
+// handles any count up to 5 at run time.
+template <int LPV, int CW, int U, bool NT = false, int HF = 0>
 __device__ __forceinline__ void count_into_lds(IbfView const & g,
                                                uint64_t const * __restrict__ vals,
                                                uint32_t L,
@@ -121,6 +125,7 @@ __device__ __forceinline__ void count_into_lds(IbfView const & g,
                                                int lane)
 {
     constexpr int VPS = 64 / LPV;
+    constexpr int KMAX = HF > 0 ? HF : 5;
     int const sub = lane / LPV;
     uint32_t const wl = static_cast<uint32_t>(lane % LPV);
     uint64_t const bw = g.bin_words;
@@ -131,10 +136,11 @@ __device__ __forceinline__ void count_into_lds(IbfView const & g,
     {
         uint32_t const nv = min(64u, L - base);
         uint64_t const v = static_cast<uint32_t>(lane) < nv ? vals[base + lane] : 0ULL;
-        uint64_t rows[5];
+        uint64_t rows[KMAX];
 #pragma unroll
-        for (int k = 0; k < 5; ++k)
-            rows[k] = (static_cast<uint32_t>(k) < hf) ? row_word(v, hash_seed(k), g.hash_shift, g.bin_size, bw) : 0ULL;
+        for (int k = 0; k < KMAX; ++k)
+            rows[k] = (HF > 0 || static_cast<uint32_t>(k) < hf) ? row_word(v, hash_seed(k), g.hash_shift, g.bin_size, bw)
+                                                                 : 0ULL;
 
         for (uint32_t s = 0; s < nv; s += VPS * U)
         {
@@ -144,12 +150,12 @@ __device__ __forceinline__ void count_into_lds(IbfView const & g,
             {
                 uint32_t const idx = s + static_cast<uint32_t>(u * VPS + sub);
                 bool const act = idx < nv;
-                uint64_t r[5];
+                uint64_t r[KMAX];
 #pragma unroll
-                for (int k = 0; k < 5; ++k)
+                for (int k = 0; k < KMAX; ++k)
                 {
                     r[k] = 0ULL;
-                    if (static_cast<uint32_t>(k) < hf)
+                    if (HF > 0 || static_cast<uint32_t>(k) < hf)
                     {
                         if constexpr (LPV == 64)
                             r[k] = readlane64(rows[k], static_cast<int>(min(idx, 63u)));
@@ -167,8 +173,8 @@ __device__ __forceinline__ void count_into_lds(IbfView const & g,
                         uint64_t const w = w0 + wt;
                         acc = load_row_word<NT>(words + r[0] + w);
 #pragma unroll
-                        for (int k = 1; k < 5; ++k)
-                            if (static_cast<uint32_t>(k) < hf)
+                        for (int k = 1; k < KMAX; ++k)
+                            if (HF > 0 || static_cast<uint32_t>(k) < hf)
                                 acc &= load_row_word<NT>(words + r[k] + w);
                         acc &= tail_mask(g, static_cast<uint32_t>(w));
                     }
@@ -199,7 +205,7 @@ __device__ __forceinline__ void count_into_lds(IbfView const & g,
 // LPV = 1..64 with one word per lane (bin_words <= 64); 7 gives 64 lanes with
 // two words each (bin_words <= 128).  Used by kernels that visit IBFs of
 // different widths (the HIBF descent).
-template <int U, bool NT = false>
+template <int U, bool NT = false, int HF = 0>
 __device__ __forceinline__ void count_into_lds_dispatch(uint32_t lpv_log,
                                                         IbfView const & g,
                                                         uint64_t const * __restrict__ vals,
@@ -211,14 +217,14 @@ __device__ __forceinline__ void count_into_lds_dispatch(uint32_t lpv_log,
 {
     switch (lpv_log)
     {
-    case 0: count_into_lds<1, 1, U, NT>(g, vals, L, 0, g.bin_words, lds, wave, nwaves, lane); break;
-    case 1: count_into_lds<2, 1, U, NT>(g, vals, L, 0, g.bin_words, lds, wave, nwaves, lane); break;
-    case 2: count_into_lds<4, 1, U, NT>(g, vals, L, 0, g.bin_words, lds, wave, nwaves, lane); break;
-    case 3: count_into_lds<8, 1, U, NT>(g, vals, L, 0, g.bin_words, lds, wave, nwaves, lane); break;
-    case 4: count_into_lds<16, 1, U, NT>(g, vals, L, 0, g.bin_words, lds, wave, nwaves, lane); break;
-    case 5: count_into_lds<32, 1, U, NT>(g, vals, L, 0, g.bin_words, lds, wave, nwaves, lane); break;
-    case 6: count_into_lds<64, 1, U, NT>(g, vals, L, 0, g.bin_words, lds, wave, nwaves, lane); break;
-    default: count_into_lds<64, 2, U, NT>(g, vals, L, 0, g.bin_words, lds, wave, nwaves, lane); break;
+    case 0: count_into_lds<1, 1, U, NT, HF>(g, vals, L, 0, g.bin_words, lds, wave, nwaves, lane); break;
+    case 1: count_into_lds<2, 1, U, NT, HF>(g, vals, L, 0, g.bin_words, lds, wave, nwaves, lane); break;
+    case 2: count_into_lds<4, 1, U, NT, HF>(g, vals, L, 0, g.bin_words, lds, wave, nwaves, lane); break;
+    case 3: count_into_lds<8, 1, U, NT, HF>(g, vals, L, 0, g.bin_words, lds, wave, nwaves, lane); break;
+    case 4: count_into_lds<16, 1, U, NT, HF>(g, vals, L, 0, g.bin_words, lds, wave, nwaves, lane); break;
+    case 5: count_into_lds<32, 1, U, NT, HF>(g, vals, L, 0, g.bin_words, lds, wave, nwaves, lane); break;
+    case 6: count_into_lds<64, 1, U, NT, HF>(g, vals, L, 0, g.bin_words, lds, wave, nwaves, lane); break;
+    default: count_into_lds<64, 2, U, NT, HF>(g, vals, L, 0, g.bin_words, lds, wave, nwaves, lane); break;
     }
 }
 

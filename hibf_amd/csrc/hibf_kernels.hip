@@ -107,22 +107,22 @@ __device__ void scan_ibf(HibfIbfDesc const & d,
 // One block per query (grid-stride).  Depth-first descent from IBF 0 with an
 // explicit stack of per-level event lists in LDS, which reproduces the
 // reference's recursion order (hierarchical_interleaved_bloom_filter.hpp:321-355
-// and :462-496) without recursion.
-template <bool COUNTING, bool NT>
-__global__ void __launch_bounds__(256) hibf_descent_kernel(HibfLaunch h,
-                                                           uint64_t const * __restrict__ values,
-                                                           uint64_t const * __restrict__ offsets,
-                                                           uint64_t nq,
-                                                           uint16_t const * __restrict__ thr16,
-                                                           uint64_t const * __restrict__ thr64,
-                                                           uint32_t counter_bytes,
-                                                           void * __restrict__ counts,
-                                                           uint32_t * __restrict__ scratch,
-                                                           uint64_t * __restrict__ ids,
-                                                           uint64_t ids_cap,
-                                                           uint64_t * __restrict__ hit_start,
-                                                           uint64_t * __restrict__ hit_count,
-                                                           uint64_t const * __restrict__ qlist)
+// and :462-496) without recursion.  HF > 0: every IBF has HF hash functions.
+template <bool COUNTING, bool NT, int HF, int U = 2>
+__device__ __forceinline__ void descent_body(HibfLaunch const & h,
+                                             uint64_t const * __restrict__ values,
+                                             uint64_t const * __restrict__ offsets,
+                                             uint64_t nq,
+                                             uint16_t const * __restrict__ thr16,
+                                             uint64_t const * __restrict__ thr64,
+                                             uint32_t counter_bytes,
+                                             void * __restrict__ counts,
+                                             uint32_t * __restrict__ scratch,
+                                             uint64_t * __restrict__ ids,
+                                             uint64_t ids_cap,
+                                             uint64_t * __restrict__ hit_start,
+                                             uint64_t * __restrict__ hit_count,
+                                             uint64_t const * __restrict__ qlist)
 {
     extern __shared__ uint32_t smem[];
     __shared__ DescentState st;
@@ -191,7 +191,7 @@ __global__ void __launch_bounds__(256) hibf_descent_kernel(HibfLaunch h,
                     cnt[i] = 0u;
                 __syncthreads();
                 IbfView const g{h.arena + d.word_offset, d.bin_size, d.bins, d.bin_words, d.hash_shift, d.hash_funs};
-                count_into_lds_dispatch<2, NT>(d.lpv_log, g, vals, L, cnt, wave, nwaves, lane);
+                count_into_lds_dispatch<U, NT, HF>(d.lpv_log, g, vals, L, cnt, wave, nwaves, lane);
                 ++visits;
                 lookups += L;
                 row_bytes += static_cast<uint64_t>(L) * d.hash_funs * d.bin_words * 8u;
@@ -274,6 +274,31 @@ __global__ void __launch_bounds__(256) hibf_descent_kernel(HibfLaunch h,
         atomicAdd(&h.counters[4], static_cast<unsigned long long>(lookups));
     }
 }
+
+#define HIBF_DESCENT_PARAMS                                                                                          \
+    HibfLaunch h, uint64_t const * __restrict__ values, uint64_t const * __restrict__ offsets, uint64_t nq,            \
+        uint16_t const * __restrict__ thr16, uint64_t const * __restrict__ thr64, uint32_t counter_bytes,               \
+        void * __restrict__ counts, uint32_t * __restrict__ scratch, uint64_t * __restrict__ ids, uint64_t ids_cap,     \
+        uint64_t * __restrict__ hit_start, uint64_t * __restrict__ hit_count, uint64_t const * __restrict__ qlist
+#define HIBF_DESCENT_ARGS                                                                                            \
+    h, values, offsets, nq, thr16, thr64, counter_bytes, counts, scratch, ids, ids_cap, hit_start, hit_count, qlist
+
+template <bool COUNTING, bool NT, int HF>
+__global__ void __launch_bounds__(256) hibf_descent_kernel(HIBF_DESCENT_PARAMS)
+{
+    descent_body<COUNTING, NT, HF>(HIBF_DESCENT_ARGS);
+}
+
+// Membership with two hash functions (the HIBF default) fits 64 VGPRs without
+// spilling, so it runs at 8 waves per SIMD: more row fetches in flight.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8)))
+hibf_descent_membership_h2_kernel(HIBF_DESCENT_PARAMS)
+{
+    descent_body<false, false, 2>(HIBF_DESCENT_ARGS);
+}
+
+#undef HIBF_DESCENT_PARAMS
+#undef HIBF_DESCENT_ARGS
 
 __global__ void reset_counters_kernel(unsigned long long * counters, uint32_t mask)
 {
@@ -398,12 +423,11 @@ hipError_t launch_hibf_membership(HibfLaunch const & h,
     if (nq == 0)
         return hipSuccess;
     size_t const lds = static_cast<size_t>(h.max_bins) * (1u + h.depth) * sizeof(uint32_t);
-    if (g_count_tuning.nt_loads)
-        hipLaunchKernelGGL((hibf_descent_kernel<false, true>), dim3(blocks), dim3(256), lds, stream, h, values, offsets,
-                           nq, thresholds, nullptr, 2u, nullptr, scratch, ids_tmp, ids_cap, hit_start, hit_count, qlist);
-    else
-        hipLaunchKernelGGL((hibf_descent_kernel<false, false>), dim3(blocks), dim3(256), lds, stream, h, values, offsets,
-                           nq, thresholds, nullptr, 2u, nullptr, scratch, ids_tmp, ids_cap, hit_start, hit_count, qlist);
+    auto const kernel = g_count_tuning.nt_loads        ? hibf_descent_kernel<false, true, 0>
+                        : h.uniform_hash_funs == 2u   ? hibf_descent_membership_h2_kernel
+                                                      : hibf_descent_kernel<false, false, 0>;
+    hipLaunchKernelGGL(kernel, dim3(blocks), dim3(256), lds, stream, h, values, offsets, nq, thresholds, nullptr, 2u,
+                       nullptr, scratch, ids_tmp, ids_cap, hit_start, hit_count, qlist);
     return hipGetLastError();
 }
 
@@ -420,9 +444,10 @@ hipError_t launch_hibf_count(HibfLaunch const & h,
     if (nq == 0)
         return hipSuccess;
     size_t const lds = static_cast<size_t>(h.max_bins) * (1u + h.depth) * sizeof(uint32_t);
-    hipLaunchKernelGGL((hibf_descent_kernel<true, false>), dim3(blocks), dim3(256), lds, stream, h, values, offsets, nq,
-                       nullptr, thresholds, counter_bytes, counts, nullptr, nullptr, 0ULL, nullptr, nullptr,
-                       nullptr);
+    auto const kernel = h.uniform_hash_funs == 2u ? hibf_descent_kernel<true, false, 2>
+                                                  : hibf_descent_kernel<true, false, 0>;
+    hipLaunchKernelGGL(kernel, dim3(blocks), dim3(256), lds, stream, h, values, offsets, nq, nullptr, thresholds,
+                       counter_bytes, counts, nullptr, nullptr, 0ULL, nullptr, nullptr, nullptr);
     return hipGetLastError();
 }
 

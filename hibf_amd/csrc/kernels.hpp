@@ -126,7 +126,7 @@ struct HibfLaunch
     uint32_t depth;           // levels
     uint64_t n_user_bins;
     uint32_t max_bin_words;   // widest row of this HIBF (words)
-    uint32_t pad;
+    uint32_t uniform_hash_funs; // hash functions of every IBF, 0 if they differ
     // Agent counters (device): [0] ids found, [1] next query (work stealing),
     // [2] IBF visits, [3] row bytes fetched (h * bin_words * 8 per value and
     // visit), [4] lookups (query values checked against one IBF), [5] set when
