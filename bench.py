@@ -57,6 +57,11 @@ def parse():
                     help="values of the bulk_contains leg of the flat workload (0 skips it)")
     ap.add_argument("--tune", action="append", default=[], metavar="KEY=VALUE",
                     help="launch-shape knob passed to hibf_gpu_tune_set (repeatable)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the library's default launch shapes instead of autotuning them on a sub-batch")
+    ap.add_argument("--autotune-queries", type=int, default=100_000,
+                    help="queries of the workload the autotuner times each candidate shape on")
+    ap.add_argument("--autotune-save", default=None, metavar="PATH", help="write the autotuner record as JSON")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a 1 GiB elementwise copy kernel (known bytes) for counter calibration")
     return ap.parse_args()
@@ -123,6 +128,19 @@ def run_flat(args):
     offsets = as_device_u64(offsets_h)
     counts = torch.empty((nq, bins), dtype=torch.uint16, device=dev)
     stream = torch.cuda.current_stream()
+
+    tuned = None
+    if not args.no_autotune and not args.tune:
+        from hibf_amd import autotune
+        nt = min(args.autotune_queries, nq)
+        rec = autotune.tune_flat_count(ibf, values[: int(offsets_h[nt])], offsets[: nt + 1], 2, reps=2)
+        tuned = {"best": rec["best"], "best_ms": rec["best_ms"], "default_ms": rec["default_ms"],
+                 "queries": nt, "candidates": len(rec["candidates"])}
+        log(f"autotune ({len(rec['candidates'])} shapes on {nt} queries): {rec['best']} "
+            f"{rec['best_ms']:.3f} ms (default {rec['default_ms']:.3f} ms)")
+        if args.autotune_save:
+            rec["command"] = "python bench.py " + " ".join(sys.argv[1:])
+            autotune.save_record(rec, args.autotune_save)
 
     def step():
         ibf.bulk_count_into(values, offsets, counts, 2)
@@ -258,7 +276,8 @@ def run_flat(args):
             "bins": bins, "bin_size": bin_size, "hash_functions": h, "queries": nq, "query_length": L,
             "counter": "uint16",
             "tuning": {k: native.tune_get(k) for k in ("count.wide_kernel", "count.wide_unroll",
-                                                        "count.wide_waves")},
+                                                        "count.wide_waves", "count.counter_bits")},
+            "autotune": tuned,
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -326,6 +345,20 @@ def run_hibf(args):
     va_h, oa_h, _ = W.hibf_queries(args.seed + 4, sizes, spec["build_seed"], lengths, 0.8)
     ta_h = thresholds_of(oa_h)
     va, oa, ta = as_device_u64(va_h), as_device_u64(oa_h), upload_u16(ta_h)
+
+    tuned = None
+    if not args.no_autotune and not args.tune:
+        from hibf_amd import autotune
+        nt = min(args.autotune_queries, nq)
+        rec = autotune.tune_hibf_membership(hibf, va[: int(oa_h[nt])], oa[: nt + 1], ta[:nt], reps=2)
+        tuned = {"best": rec["best"], "best_ms": rec["best_ms"], "default_ms": rec["default_ms"],
+                 "queries": nt, "candidates": len(rec["candidates"])}
+        log(f"autotune ({len(rec['candidates'])} shapes on {nt} queries): {rec['best']} "
+            f"{rec['best_ms']:.3f} ms (default {rec['default_ms']:.3f} ms)")
+        if args.autotune_save:
+            rec["command"] = "python bench.py " + " ".join(sys.argv[1:])
+            autotune.save_record(rec, args.autotune_save)
+
     cap = 8 * nq
     ids = torch.empty(cap, dtype=torch.uint64, device=dev)
     hs = torch.empty(nq, dtype=torch.uint64, device=dev)
@@ -475,7 +508,9 @@ def run_hibf(args):
         "config": {"workload": f"HIBF {n_ub} user bins (log-normal sizes, median 1e5, sigma 1.5), tmax 256, h=2, "
                                f"FPR 0.05/0.3; {nb} queries, log-normal length median 300 [50, 5000], 60% from one "
                                f"user bin, t = 0.7 L; hipGraph batches of {G}",
-                   "ibfs": len(tab["geoms"]), "arena_gb": tab["n_words"] * 8 / 1e9, "depth": tab["depth"]},
+                   "ibfs": len(tab["geoms"]), "arena_gb": tab["n_words"] * 8 / 1e9, "depth": tab["depth"],
+                   "tuning": {k: native.tune_get(k) for k in ("hibf.wave_kernel", "hibf.nt_loads")},
+                   "autotune": tuned},
         "roofline": {"bound": "hbm", "achieved": bytes_b / wall_b / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": bytes_b / wall_b / 1e9 / HBM_PEAK_GBS, "traffic": None},
         "fixed_batch": {"queries": nq, "length": 250, "ms": wall_a * 1e3, "lookups_per_s": lookups_a / wall_a,

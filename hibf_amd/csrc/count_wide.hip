@@ -5,10 +5,12 @@
 // 128-word tile and fetches them with one 16-byte load per row, so one wave
 // instruction moves a whole 1 KiB row.  The row starts of a value are
 // wave-uniform (readlane of the lane that hashed it), U values are in flight
-// per wave before their hits are counted.  Counters are 16-bit pairs packed in
-// 32-bit LDS words (16 KiB per 8,192 bins); a query longer than 65,535 values
-// is processed in segments whose counters are flushed (added) to the output,
-// so no packed counter can carry into its neighbour.
+// per wave before their hits are counted.  Counters are CB-bit fields packed in
+// 32-bit LDS words: 16-bit pairs (16 KiB per 8,192 bins) or 8-bit quads
+// (8 KiB, which lets more small blocks share a CU).  A query longer than
+// 2^CB - 1 values is processed in segments whose counters are flushed (added)
+// to the output, so no packed counter can carry into its neighbour; the sum of
+// the segments modulo 2^(8 * counter_bytes) is the reference's wrapped count.
 #include "device_common.hpp"
 #include "kernels.hpp"
 
@@ -18,8 +20,7 @@ namespace hibf_amd
 namespace
 {
 
-constexpr uint32_t kWideTileWords = 128;   // 8,192 bins per tile
-constexpr uint32_t kSegmentValues = 65535; // packed 16-bit counters never overflow
+constexpr uint32_t kWideTileWords = 128; // 8,192 bins per tile
 
 template <typename T>
 __device__ __forceinline__ void put(void * out, uint64_t i, uint32_t v, bool accumulate)
@@ -39,7 +40,8 @@ __device__ __forceinline__ void put_counter(void * out, uint64_t i, uint32_t cb,
     }
 }
 
-// Writes nb counters of a tile (packed pairs in lds) to out[first ..).
+// Writes nb counters of a tile (CB-bit fields packed in lds) to out[first ..).
+template <int CB>
 __device__ void store_tile(uint32_t const * lds,
                            uint32_t nb,
                            void * counts,
@@ -49,28 +51,43 @@ __device__ void store_tile(uint32_t const * lds,
                            int tid,
                            int nthreads)
 {
-    bool const vec = !accumulate && cb == 2u && (first % 8u) == 0u;
-    if (vec)
+    constexpr uint32_t per_word = 32u / CB;
+    constexpr uint32_t field = (1u << CB) - 1u;
+    if (!accumulate && cb == 2u && (first % 8u) == 0u)
     {
-        // 8 counters (4 packed LDS words) per 16-byte store.
+        // 8 uint16 counters per 16-byte store.
         uint32_t const n8 = nb / 8u;
         uint4 * const out = reinterpret_cast<uint4 *>(static_cast<uint16_t *>(counts) + first);
         for (uint32_t i = static_cast<uint32_t>(tid); i < n8; i += static_cast<uint32_t>(nthreads))
         {
-            uint4 const w = *reinterpret_cast<uint4 const *>(lds + 4u * i);
+            uint4 w;
+            if constexpr (CB == 16)
+            {
+                w = *reinterpret_cast<uint4 const *>(lds + 4u * i);
+            }
+            else
+            {
+                uint32_t const a = lds[2u * i], b = lds[2u * i + 1u];
+                w.x = (a & 0xFFu) | ((a & 0xFF00u) << 8);
+                w.y = ((a >> 16) & 0xFFu) | ((a >> 8) & 0xFF0000u);
+                w.z = (b & 0xFFu) | ((b & 0xFF00u) << 8);
+                w.w = ((b >> 16) & 0xFFu) | ((b >> 8) & 0xFF0000u);
+            }
             out[i] = w;
         }
         for (uint32_t i = n8 * 8u + static_cast<uint32_t>(tid); i < nb; i += static_cast<uint32_t>(nthreads))
-            static_cast<uint16_t *>(counts)[first + i] = static_cast<uint16_t>(lds[i >> 1] >> (16u * (i & 1u)));
+            static_cast<uint16_t *>(counts)[first + i] =
+                static_cast<uint16_t>((lds[i / per_word] >> (CB * (i % per_word))) & field);
         return;
     }
     for (uint32_t i = static_cast<uint32_t>(tid); i < nb; i += static_cast<uint32_t>(nthreads))
-        put_counter(counts, first + i, cb, (lds[i >> 1] >> (16u * (i & 1u))) & 0xFFFFu, accumulate);
+        put_counter(counts, first + i, cb, (lds[i / per_word] >> (CB * (i % per_word))) & field, accumulate);
 }
 
 // HF > 0 fixes the number of hash functions (== g.hash_funs) at compile time,
 // so only HF row starts per value occupy registers; HF == 0 reads it at run time.
-template <int U, int HF>
+// CB is the LDS counter width in bits (8 or 16).
+template <int U, int HF, int CB>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((HF > 0 && U <= 4) ? 8 : 1, 8))) count_wide_kernel(IbfView g,
                                                          uint64_t const * __restrict__ values,
                                                          uint64_t const * __restrict__ offsets,
@@ -78,7 +95,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((HF > 
                                                          uint32_t counter_bytes,
                                                          void * __restrict__ counts)
 {
-    extern __shared__ uint32_t lds[]; // kWideTileWords * 32 packed counter pairs
+    extern __shared__ uint32_t lds[]; // kWideTileWords * 64 counters of CB bits
+    constexpr uint32_t per_word = 32u / CB;
+    constexpr uint32_t segment_values = (1u << CB) - 1u; // a packed counter never overflows
     int const tid = static_cast<int>(threadIdx.x);
     int const nthreads = static_cast<int>(blockDim.x);
     int const lane = tid & 63;
@@ -104,8 +123,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((HF > 
             uint32_t seg0 = 0;
             do
             {
-                uint32_t const segL = min(kSegmentValues, L - seg0);
-                for (uint32_t i = static_cast<uint32_t>(tid); i < tw * 32u; i += static_cast<uint32_t>(nthreads))
+                uint32_t const segL = min(segment_values, L - seg0);
+                for (uint32_t i = static_cast<uint32_t>(tid); i < tw * 64u / per_word;
+                     i += static_cast<uint32_t>(nthreads))
                     lds[i] = 0u;
                 __syncthreads();
                 for (uint32_t base = static_cast<uint32_t>(wave) * 64u; base < segL;
@@ -156,14 +176,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((HF > 
                             while (bits)
                             {
                                 uint32_t const b = wl * 64u + static_cast<uint32_t>(__builtin_ctzll(bits));
-                                atomicAdd(&lds[b >> 1], 1u << (16u * (b & 1u)));
+                                atomicAdd(&lds[b / per_word], 1u << (CB * (b % per_word)));
                                 bits &= bits - 1ULL;
                             }
                             bits = x1[u];
                             while (bits)
                             {
                                 uint32_t const b = (wl + 1u) * 64u + static_cast<uint32_t>(__builtin_ctzll(bits));
-                                atomicAdd(&lds[b >> 1], 1u << (16u * (b & 1u)));
+                                atomicAdd(&lds[b / per_word], 1u << (CB * (b % per_word)));
                                 bits &= bits - 1ULL;
                             }
                         }
@@ -172,7 +192,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((HF > 
                 __syncthreads();
                 uint32_t const first_bin = w0 * 64u;
                 uint32_t const nb = min(tw * 64u, g.bins - first_bin);
-                store_tile(lds, nb, counts, q * g.bins + first_bin, counter_bytes, seg0 != 0, tid, nthreads);
+                store_tile<CB>(lds, nb, counts, q * g.bins + first_bin, counter_bytes, seg0 != 0, tid, nthreads);
                 __syncthreads();
                 seg0 += segL;
             } while (seg0 < L);
@@ -200,21 +220,27 @@ hipError_t launch_count_wide(IbfView const & g,
 {
     if (nq == 0)
         return hipSuccess;
-    size_t const lds = static_cast<size_t>(kWideTileWords) * 32u * sizeof(uint32_t);
+    bool const narrow = g_count_tuning.counter_bits == 8u;
+    size_t const lds = static_cast<size_t>(kWideTileWords) * (narrow ? 16u : 32u) * sizeof(uint32_t);
     unsigned const grid = static_cast<unsigned>(nq < max_blocks ? nq : max_blocks);
     dim3 const blk(64u * g_count_tuning.wide_waves);
     using kernel_t = void (*)(IbfView, uint64_t const *, uint64_t const *, uint64_t, uint32_t, void *);
-    // [unroll 1, 2, 4, 8][hash functions: run time, 2, 3]
-    static kernel_t const table[4][3] = {
-        {count_wide_kernel<1, 0>, count_wide_kernel<1, 2>, count_wide_kernel<1, 3>},
-        {count_wide_kernel<2, 0>, count_wide_kernel<2, 2>, count_wide_kernel<2, 3>},
-        {count_wide_kernel<4, 0>, count_wide_kernel<4, 2>, count_wide_kernel<4, 3>},
-        {count_wide_kernel<8, 0>, count_wide_kernel<8, 2>, count_wide_kernel<8, 3>},
+    // [counter bits 16, 8][unroll 1, 2, 4, 8][hash functions: run time, 2, 3]
+    static kernel_t const table[2][4][3] = {
+        {{count_wide_kernel<1, 0, 16>, count_wide_kernel<1, 2, 16>, count_wide_kernel<1, 3, 16>},
+         {count_wide_kernel<2, 0, 16>, count_wide_kernel<2, 2, 16>, count_wide_kernel<2, 3, 16>},
+         {count_wide_kernel<4, 0, 16>, count_wide_kernel<4, 2, 16>, count_wide_kernel<4, 3, 16>},
+         {count_wide_kernel<8, 0, 16>, count_wide_kernel<8, 2, 16>, count_wide_kernel<8, 3, 16>}},
+        {{count_wide_kernel<1, 0, 8>, count_wide_kernel<1, 2, 8>, count_wide_kernel<1, 3, 8>},
+         {count_wide_kernel<2, 0, 8>, count_wide_kernel<2, 2, 8>, count_wide_kernel<2, 3, 8>},
+         {count_wide_kernel<4, 0, 8>, count_wide_kernel<4, 2, 8>, count_wide_kernel<4, 3, 8>},
+         {count_wide_kernel<8, 0, 8>, count_wide_kernel<8, 2, 8>, count_wide_kernel<8, 3, 8>}},
     };
     uint32_t const u = g_count_tuning.wide_unroll;
     int const ui = u == 1u ? 0 : u == 2u ? 1 : u == 8u ? 3 : 2;
     int const hi = g.hash_funs == 2u ? 1 : g.hash_funs == 3u ? 2 : 0;
-    hipLaunchKernelGGL(table[ui][hi], dim3(grid), blk, lds, stream, g, values, offsets, nq, counter_bytes, counts);
+    hipLaunchKernelGGL(table[narrow ? 1 : 0][ui][hi], dim3(grid), blk, lds, stream, g, values, offsets, nq,
+                       counter_bytes, counts);
     return hipGetLastError();
 }
 

@@ -197,6 +197,10 @@ class DeviceIbf:
 
 
 def _thresholds16(thresholds, nq: int, device) -> torch.Tensor:
+    if isinstance(thresholds, torch.Tensor) and thresholds.is_cuda:
+        if thresholds.dtype != torch.uint16 or thresholds.numel() != nq or not thresholds.is_contiguous():
+            raise ValueError("device thresholds must be a contiguous uint16 tensor of one entry per query")
+        return thresholds
     a = np.ascontiguousarray(np.broadcast_to(np.asarray(thresholds, dtype=np.uint16), (nq,)))
     return torch.from_numpy(a.view(np.int16)).to(device).view(torch.uint16)
 

@@ -43,10 +43,12 @@ const char * hibf_gpu_status_string(int status);
 const char * hibf_gpu_last_error(void);
 
 /* Launch-shape knobs of the query kernels (no counterpart in the reference:
- * the reference has no device tiles).  Process-wide; set by the autotuner.
- * Keys: "count.wide_kernel" (0/1), "count.wide_unroll" (1,2,4,8),
- * "count.wide_waves" (1..4), "hibf.wave_kernel" (0/1).  Results never depend
- * on these values. */
+ * the reference has no device tiles).  Process-wide; set by the autotuner
+ * (hibf_amd/autotune.py) or by hand.  Keys: "count.wide_kernel" (0/1),
+ * "count.wide_unroll" (values in flight per wave: 1,2,4,8),
+ * "count.wide_waves" (waves per query block: 1..4), "count.counter_bits"
+ * (LDS counter width: 8 or 16), "hibf.wave_kernel" (query per wave 1, per
+ * block 0), "hibf.nt_loads" (0/1).  Results never depend on these values. */
 int hibf_gpu_tune_set(const char * key, int64_t value);
 int hibf_gpu_tune_get(const char * key, int64_t * value);
 

@@ -1,0 +1,78 @@
+"""GPU tests of the launch-shape autotuner: every candidate must reproduce the
+default shape's output (the tuner raises otherwise), the result equals the
+oracle, and a saved record applies again."""
+import json
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+@pytest.fixture
+def restore_knobs():
+    from hibf_amd import autotune, native
+    keys = list(autotune.FLAT_SPACE) + list(autotune.HIBF_SPACE)
+    saved = {k: native.tune_get(k) for k in keys}
+    yield
+    for k, v in saved.items():
+        native.tune_set(k, v)
+
+
+def test_tune_flat_count(tmp_path, restore_knobs):
+    import torch
+    from hibf_amd import autotune, native
+    from hibf_amd.device import DeviceIbf, as_device_u64, csr, to_numpy
+    _say("autotune flat bulk_count 8192 bins")
+    rng = np.random.default_rng(5)
+    vals = rng.integers(0, 2**63, size=40000, dtype=np.int64).astype(np.uint64)
+    bidx = rng.integers(0, 8192, size=vals.size).astype(np.uint64)
+    ref = O.Ibf(8192, 2048, 3)
+    ref.emplace(vals, bidx)
+    dev = DeviceIbf(8192, 2048, 3)
+    dev.emplace(vals, bidx)
+    qs = [vals[rng.integers(0, vals.size, size=L)] for L in rng.integers(0, 400, size=300)]
+    v_h, o_h = csr(qs)
+    rec = autotune.tune_flat_count(dev, as_device_u64(v_h), as_device_u64(o_h), 2, reps=1)
+    assert len(rec["candidates"]) == 24
+    assert {k: native.tune_get(k) for k in autotune.FLAT_SPACE} == rec["best"]
+    got = to_numpy(dev.bulk_count(v_h, o_h, 2))
+    assert np.array_equal(got, ref.bulk_count_batch(v_h, o_h, 2))
+    path = tmp_path / "flat.json"
+    autotune.save_record(rec, str(path))
+    for k in autotune.FLAT_SPACE:
+        native.tune_set(k, autotune.FLAT_SPACE[k][0])
+    assert autotune.apply_record(autotune.load_record(str(path))) == rec["best"]
+    assert json.loads(path.read_text())["workload"]["bins"] == 8192
+    torch.cuda.synchronize()
+
+
+def test_tune_hibf_membership(restore_knobs):
+    from hibf_amd import autotune
+    from hibf_amd.device import DeviceHibf, csr, to_numpy
+    from hibf_helpers import oracle_of
+    _say("autotune hibf membership 600 user bins")
+    rng = np.random.default_rng(9)
+    ubs = [rng.integers(0, 2**63, size=int(n), dtype=np.int64).astype(np.uint64)
+           for n in rng.integers(20, 300, size=600)]
+    dev = DeviceHibf.build(ubs, tmax=64)
+    ref = oracle_of(dev)
+    qs = [np.concatenate([ubs[int(rng.integers(0, 600))][:40], rng.integers(0, 2**63, size=20, dtype=np.int64)
+                          .astype(np.uint64)]) for _ in range(200)]
+    v_h, o_h = csr(qs)
+    thr = np.full(len(qs), 30, dtype=np.uint16)
+    import torch
+    from hibf_amd.device import as_device_u64
+    t = torch.from_numpy(thr.view(np.int16)).cuda().view(torch.uint16)
+    rec = autotune.tune_hibf_membership(dev, as_device_u64(v_h), as_device_u64(o_h), t, reps=1)
+    assert len(rec["candidates"]) == 4
+    ids, hoff = dev.membership_for(v_h, o_h, thr)
+    ids, hoff = to_numpy(ids), to_numpy(hoff)
+    for q in range(len(qs)):
+        assert ids[hoff[q]:hoff[q + 1]].tolist() == ref.membership_for(qs[q], 30)

@@ -121,6 +121,40 @@ def test_bulk_count_long_queries_wide_rows(bins):
         assert np.array_equal(got, ref.bulk_count_batch(values, offsets, width)), width
 
 
+@pytest.fixture
+def knobs():
+    """Sets launch-shape knobs for one test and restores the defaults after it."""
+    from hibf_amd import native
+    keys = ("count.wide_unroll", "count.wide_waves", "count.counter_bits")
+    saved = {k: native.tune_get(k) for k in keys}
+
+    def set_(**kv):
+        for k, v in kv.items():
+            native.tune_set(k.replace("__", "."), v)
+    yield set_
+    for k, v in saved.items():
+        native.tune_set(k, v)
+
+
+@pytest.mark.parametrize("h", [2, 3, 4])
+@pytest.mark.parametrize("counter_bits", [8, 16])
+@pytest.mark.parametrize("unroll,waves", [(1, 1), (2, 4), (4, 2), (8, 4), (2, 1), (4, 4)])
+def test_bulk_count_tile_shapes(h, counter_bits, unroll, waves, knobs):
+    """Every tile shape of the wide count kernel (values in flight, waves per
+    block, 8/16-bit LDS counters, compile-time and run-time hash counts) gives
+    the oracle's counts; lengths straddle the 255- and 65,535-value segments."""
+    from hibf_amd.device import csr, to_numpy
+    _say(f"count tile h={h} bits={counter_bits} unroll={unroll} waves={waves}")
+    knobs(count__counter_bits=counter_bits, count__wide_unroll=unroll, count__wide_waves=waves)
+    ref, dev, vals, _, rng = _pair(8192, 1024, h, 20000, 77 + h)
+    qs = _queries(rng, vals, [0, 1, 63, 254, 255, 256, 300, 511, 1200])
+    qs += [np.full(300, vals[3], dtype=np.uint64), np.full(66000, vals[5], dtype=np.uint64)]
+    values, offsets = csr(qs)
+    for width in (1, 2, 8):
+        got = to_numpy(dev.bulk_count(values, offsets, width))
+        assert np.array_equal(got, ref.bulk_count_batch(values, offsets, width)), width
+
+
 @pytest.mark.parametrize("bins,bin_size,h", [(12, 8192, 2), (200, 3000, 3), (8192, 1024, 2), (9000, 300, 2)])
 def test_membership_for(bins, bin_size, h):
     from hibf_amd.device import csr, to_numpy
