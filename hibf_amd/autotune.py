@@ -17,6 +17,8 @@ Search spaces:
     count.wide_waves    waves per query block (1, 2, 4)
   HIBF membership descent:
     hibf.wave_kernel    one query per wave (1) or per block (0)
+    hibf.row_per_lane   rows of <= 4 words: one value and its whole row per
+                        lane (1), or lanes sharing a value's row (0)
     hibf.nt_loads       non-temporal row loads (0, 1)
 """
 from __future__ import annotations
@@ -37,6 +39,7 @@ FLAT_SPACE = {
 }
 HIBF_SPACE = {
     "hibf.wave_kernel": (0, 1),
+    "hibf.row_per_lane": (0, 1),
     "hibf.nt_loads": (0, 1),
 }
 

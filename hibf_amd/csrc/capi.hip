@@ -212,6 +212,8 @@ int hibf_gpu_tune_set(char const * key, int64_t value)
         g_count_tuning.nt_loads = static_cast<uint32_t>(value);
     else if (k == "count.counter_bits" && (value == 8 || value == 16))
         g_count_tuning.counter_bits = static_cast<uint32_t>(value);
+    else if (k == "hibf.row_per_lane" && (value == 0 || value == 1))
+        g_count_tuning.hibf_narrow = static_cast<uint32_t>(value);
     else
         return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "unknown tuning key or value out of range: " + k);
     return HIBF_GPU_OK;
@@ -234,6 +236,8 @@ int hibf_gpu_tune_get(char const * key, int64_t * value)
         *value = g_count_tuning.nt_loads;
     else if (k == "count.counter_bits")
         *value = g_count_tuning.counter_bits;
+    else if (k == "hibf.row_per_lane")
+        *value = g_count_tuning.hibf_narrow;
     else
         return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "unknown tuning key: " + k);
     return HIBF_GPU_OK;

@@ -157,7 +157,9 @@ __device__ __forceinline__ void count_into_lds(IbfView const & g,
                     r[k] = 0ULL;
                     if (HF > 0 || static_cast<uint32_t>(k) < hf)
                     {
-                        if constexpr (LPV == 64)
+                        if constexpr (LPV == 1 && U == 1)
+                            r[k] = rows[k]; // each lane's own value (idx == lane)
+                        else if constexpr (LPV == 64)
                             r[k] = readlane64(rows[k], static_cast<int>(min(idx, 63u)));
                         else
                             r[k] = shfl64(rows[k], static_cast<int>(min(idx, 63u)));
@@ -226,6 +228,22 @@ __device__ __forceinline__ void count_into_lds_dispatch(uint32_t lpv_log,
     case 6: count_into_lds<64, 1, U, NT, HF>(g, vals, L, 0, g.bin_words, lds, wave, nwaves, lane); break;
     default: count_into_lds<64, 2, U, NT, HF>(g, vals, L, 0, g.bin_words, lds, wave, nwaves, lane); break;
     }
+}
+
+// Counting for HIBFs whose IBFs all have rows of at most 4 words (256
+// technical bins, the tmax = 256 layout): each lane takes one value and reads
+// its whole row (4 words; fewer are masked), so a wave has 64 values x h rows
+// in flight at once, one memory round trip per 64 values.
+template <bool NT = false, int HF = 0>
+__device__ __forceinline__ void count_into_lds_row_per_lane(IbfView const & g,
+                                                            uint64_t const * __restrict__ vals,
+                                                            uint32_t L,
+                                                            uint32_t * lds,
+                                                            int wave,
+                                                            int nwaves,
+                                                            int lane)
+{
+    count_into_lds<1, 4, 1, NT, HF>(g, vals, L, 0, g.bin_words, lds, wave, nwaves, lane);
 }
 
 } // namespace hibf_amd

@@ -40,7 +40,7 @@ def test_tune_flat_count(tmp_path, restore_knobs):
     qs = [vals[rng.integers(0, vals.size, size=L)] for L in rng.integers(0, 400, size=300)]
     v_h, o_h = csr(qs)
     rec = autotune.tune_flat_count(dev, as_device_u64(v_h), as_device_u64(o_h), 2, reps=1)
-    assert len(rec["candidates"]) == 24
+    assert len(rec["candidates"]) == len(autotune._candidates(autotune.FLAT_SPACE))
     assert {k: native.tune_get(k) for k in autotune.FLAT_SPACE} == rec["best"]
     got = to_numpy(dev.bulk_count(v_h, o_h, 2))
     assert np.array_equal(got, ref.bulk_count_batch(v_h, o_h, 2))
@@ -71,7 +71,7 @@ def test_tune_hibf_membership(restore_knobs):
     from hibf_amd.device import as_device_u64
     t = torch.from_numpy(thr.view(np.int16)).cuda().view(torch.uint16)
     rec = autotune.tune_hibf_membership(dev, as_device_u64(v_h), as_device_u64(o_h), t, reps=1)
-    assert len(rec["candidates"]) == 4
+    assert len(rec["candidates"]) == len(autotune._candidates(autotune.HIBF_SPACE))
     ids, hoff = dev.membership_for(v_h, o_h, thr)
     ids, hoff = to_numpy(ids), to_numpy(hoff)
     for q in range(len(qs)):

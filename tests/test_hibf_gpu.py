@@ -158,19 +158,25 @@ CASES = [  # (n_ub, size range, config)
 ]
 
 
-@pytest.fixture(params=[1, 0], ids=["wave", "block"])
+@pytest.fixture(params=[(1, 1), (0, 1), (0, 0)], ids=["wave", "block-row-per-lane", "block-lanes-per-value"])
 def descent_kernel(request):
+    """Runs a test with each descent kernel: query per wave, query per block
+    with one value and its row per lane, query per block with lanes sharing a
+    value's row.  Yields the variant's name."""
     from hibf_amd import native
-    default = native.tune_get("hibf.wave_kernel")
-    native.tune_set("hibf.wave_kernel", request.param)
-    yield request.param
-    native.tune_set("hibf.wave_kernel", default)
+    keys = ("hibf.wave_kernel", "hibf.row_per_lane")
+    saved = {k: native.tune_get(k) for k in keys}
+    for k, v in zip(keys, request.param):
+        native.tune_set(k, v)
+    yield {(1, 1): "wave", (0, 1): "block-row-per-lane", (0, 0): "block-lanes-per-value"}[request.param]
+    for k, v in saved.items():
+        native.tune_set(k, v)
 
 
 @pytest.mark.parametrize("n_ub,sizes,config", CASES)
 def test_hibf_parity_random(n_ub, sizes, config, descent_kernel):
     from hibf_amd.device import DeviceHibf, to_numpy
-    _say(f"hibf parity n_ub={n_ub} {config} kernel={'wave' if descent_kernel else 'block'}")
+    _say(f"hibf parity n_ub={n_ub} {config} kernel={descent_kernel}")
     rng = np.random.default_rng(n_ub)
     ubs = _random_user_bins(rng, n_ub, *sizes)
     dev = DeviceHibf.build(ubs, **config)
