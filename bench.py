@@ -32,6 +32,8 @@ def parse():
                     help="Poisson arrival rates (queries/s) of the serving workload")
     ap.add_argument("--serve-seconds", type=float, default=1.0)
     ap.add_argument("--serve-windows", default="200,1000,2000", help="batching windows (us)")
+    ap.add_argument("--serve-pageable", action="store_true",
+                    help="hand the serving loop pageable query values (staged) instead of a pinned receive buffer")
     ap.add_argument("--workload", choices=["flat", "hibf", "serve"], default="flat",
                     help="flat: 8,192-bin flat IBF bulk_count (default); hibf: 32,768-user-bin HIBF membership "
                          "(fixed-length batch, log-normal batched hipGraph form, plain per-query form)")
@@ -79,6 +81,33 @@ def recorded_traffic(key: str, args) -> float | None:
     if any(getattr(args, k) != v for k, v in rec["workload"].items()):
         return None
     return (rec["fetch_kib"] * rec["fetch_factor"] + rec["write_kib"] * rec["write_factor"]) * 1024.0
+
+
+def host_cpu() -> dict:
+    """CPU model and core count of the host the CPU baseline runs on."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count(),
+            "counting_path": "scalar (countr_zero over the AND of the rows; no AVX-512 path in the oracle)"}
+
+
+def timed_sample(run, n_probe: int, n_max: int, seconds: float) -> tuple[int, float]:
+    """Runs run(n) on a probe of n_probe units, then on as many units as fit
+    `seconds`; returns (units, wall seconds) of the second run."""
+    t1 = time.perf_counter()
+    run(n_probe)
+    rate = n_probe / max(time.perf_counter() - t1, 1e-6)
+    n = int(min(n_max, max(n_probe, rate * seconds)))
+    t1 = time.perf_counter()
+    run(n)
+    return n, time.perf_counter() - t1
 
 
 def log(msg):
@@ -242,18 +271,16 @@ def run_flat(args):
             parity = parity and contains["parity_sample_ok"]
         log(f"parity vs oracle on {len(sample)} queries: {parity}")
         if not args.skip_cpu:
-            probe = 64
-            t1 = time.perf_counter()
-            ref.bulk_count_batch(values_h[: probe * L], offsets_h[: probe + 1], 2, threads)
-            rate = probe / max(time.perf_counter() - t1, 1e-6)
-            n_cpu = int(min(nq, max(probe, rate * args.cpu_seconds)))
-            t1 = time.perf_counter()
-            ref.bulk_count_batch(values_h[: n_cpu * L], offsets_h[: n_cpu + 1], 2, threads)
-            cpu_s = time.perf_counter() - t1
+            def run_cpu(n, t):
+                ref.bulk_count_batch(values_h[: n * L], offsets_h[: n + 1], 2, t)
+            n_cpu, cpu_s = timed_sample(lambda n: run_cpu(n, threads), 64, nq, args.cpu_seconds)
+            n_one, one_s = timed_sample(lambda n: run_cpu(n, 1), 16, nq, args.cpu_seconds / 4)
             cpu = {"value": n_cpu * L / cpu_s, "unit": "lookups/s", "cores": threads, "kind": "port",
                    "sample": f"first {n_cpu} queries x {L} values of the same workload, oracle bulk_count "
-                             f"(uint16) with {threads} OpenMP threads, one agent per thread, {cpu_s:.1f} s"}
-            log(f"cpu baseline: {cpu['value'] / 1e6:.2f} M lookups/s on {threads} threads")
+                             f"(uint16) with {threads} OpenMP threads, one agent per thread, {cpu_s:.1f} s",
+                   "one_thread": {"value": n_one * L / one_s, "queries": n_one, "seconds": one_s}, **host_cpu()}
+            log(f"cpu baseline: {cpu['value'] / 1e6:.2f} M lookups/s on {threads} threads, "
+                f"{cpu['one_thread']['value'] / 1e6:.2f} M lookups/s on 1 thread")
         del words, ref
 
     out = {
@@ -478,18 +505,28 @@ def run_hibf(args):
         log(f"parity vs oracle on {2 * args.check_queries} sampled queries: {parity}")
         if not args.skip_cpu:
             threads = max(1, min(16, os.cpu_count() or 1))
-            probe = 200
-            t1 = time.perf_counter()
-            ref.membership_hits_batch(va_h[: probe * 250], oa_h[: probe + 1], ta_h[:probe], threads)
-            rate = probe / max(time.perf_counter() - t1, 1e-6)
-            n_cpu = int(min(nq, max(probe, rate * args.cpu_seconds)))
-            t1 = time.perf_counter()
-            ref.membership_hits_batch(va_h[: n_cpu * 250], oa_h[: n_cpu + 1], ta_h[:n_cpu], threads)
-            cpu_s = time.perf_counter() - t1
-            cpu = {"value": n_cpu * 250 / cpu_s, "unit": "query values/s", "cores": threads, "kind": "port",
-                   "sample": f"first {n_cpu} queries of (a), oracle membership_for with {threads} threads, "
-                             f"{cpu_s:.1f} s (values/s, each value checked in every visited IBF)"}
-            log(f"cpu baseline: {cpu['value'] / 1e6:.2f} M query values/s on {threads} threads")
+
+            def run_cpu(n, t):
+                ref.membership_hits_batch(va_h[: n * 250], oa_h[: n + 1], ta_h[:n], t)
+            n_cpu, cpu_s = timed_sample(lambda n: run_cpu(n, threads), 200, nq, args.cpu_seconds)
+            n_one, one_s = timed_sample(lambda n: run_cpu(n, 1), 50, nq, args.cpu_seconds / 4)
+            # Lookups (a value checked against one visited IBF) of the same prefix, counted by the device.
+            hibf.membership_prepare(agent, n_cpu)
+            agent.stats(reset=True)
+            hibf.membership_launch(agent, va, oa[: n_cpu + 1], ta[:n_cpu], ids, hs[:n_cpu], hc[:n_cpu])
+            torch.cuda.synchronize()
+            lk_cpu = agent.stats(reset=True)["lookups"]
+            hibf.membership_prepare(agent, n_one)
+            hibf.membership_launch(agent, va, oa[: n_one + 1], ta[:n_one], ids, hs[:n_one], hc[:n_one])
+            torch.cuda.synchronize()
+            lk_one = agent.stats(reset=True)["lookups"]
+            cpu = {"value": lk_cpu / cpu_s, "unit": "lookups/s", "cores": threads, "kind": "port",
+                   "sample": f"first {n_cpu} queries of (a) ({lk_cpu} lookups), oracle membership_for with "
+                             f"{threads} OpenMP threads, one agent per thread, {cpu_s:.1f} s",
+                   "query_values_per_s": n_cpu * 250 / cpu_s,
+                   "one_thread": {"value": lk_one / one_s, "queries": n_one, "seconds": one_s}, **host_cpu()}
+            log(f"cpu baseline: {cpu['value'] / 1e6:.2f} M lookups/s on {threads} threads, "
+                f"{cpu['one_thread']['value'] / 1e6:.2f} M lookups/s on 1 thread")
         del arena, ref
 
     out = {
@@ -555,8 +592,13 @@ def run_serve(args):
         rng = np.random.default_rng(args.seed + 13)
         arrival = np.cumsum(rng.exponential(1.0 / rate, size=nq))
         arrival -= arrival[0]
+        v_in = v_h
+        if not args.serve_pageable:
+            # Requests land in a pinned receive buffer, as a server's network path would place them.
+            v_in = torch.empty(v_h.size, dtype=torch.int64, pin_memory=True)
+            v_in.numpy()[:] = v_h.view(np.int64)
         for win in windows:
-            lat, ids, cnt, st = hibf.serve(v_h, o_h, t_h, arrival, window_us=win, streams=4,
+            lat, ids, cnt, st = hibf.serve(v_in, o_h, t_h, arrival, window_us=win, streams=4,
                                            budget_bytes=4 << 30, results_per_query=16)
             row = {"rate_qps": rate, "window_us": win, "queries": nq,
                    "p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
@@ -584,7 +626,8 @@ def run_serve(args):
     out = {"metric": "p99 query latency (HIBF serving loop)", "value": best["p99_ms"], "unit": "ms",
            "n_gpus": 1, "higher_is_better": False, "data": "synthetic", "dtype": "u64 hash / u16 counters",
            "config": {"workload": f"serving on the {n_ub}-user-bin HIBF, Poisson arrivals, log-normal lengths "
-                                  f"(median 300, max 5000), 60% from one user bin, t = 0.7 L, 4 GiB budget, 4 streams"},
+                                  f"(median 300, max 5000), 60% from one user bin, t = 0.7 L, 4 GiB budget, 4 streams",
+                      "query_values": "pageable, staged" if args.serve_pageable else "pinned receive buffer"},
            "sweep": rows}
     print(json.dumps(out), flush=True)
 

@@ -4,6 +4,10 @@
 // and answered by the asynchronous membership launch on one of several
 // streams, so batches overlap on the device.  Latency of a request is the time
 // from its arrival to the moment its ids are back on the host.
+//
+// Query values in pinned host memory (hipHostMalloc, e.g. a receive ring) are
+// copied to the device directly; pageable values go through pinned staging
+// buffers, large batches with several host threads.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -45,6 +49,43 @@ double now_s(std::chrono::steady_clock::time_point t0)
     return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
+// True when both ends of [p, p + bytes) are pinned host memory known to HIP.
+bool host_pinned(void const * p, uint64_t bytes)
+{
+    if (!p || !bytes)
+        return false;
+    for (char const * q : {static_cast<char const *>(p), static_cast<char const *>(p) + bytes - 1})
+    {
+        hipPointerAttribute_t a{};
+        if (hipPointerGetAttributes(&a, q) != hipSuccess)
+        {
+            (void)hipGetLastError();
+            return false;
+        }
+        if (a.type != hipMemoryTypeHost)
+            return false;
+    }
+    return true;
+}
+
+// memcpy split over up to `threads` threads for copies of 8 MiB and more.
+void copy_host(void * dst, void const * src, uint64_t bytes, unsigned threads)
+{
+    if (bytes < (8ull << 20) || threads <= 1)
+    {
+        std::memcpy(dst, src, bytes);
+        return;
+    }
+    uint64_t const chunk = ((bytes + threads - 1) / threads + 4095u) / 4096u * 4096u;
+    std::vector<std::thread> pool;
+    for (uint64_t off = chunk; off < bytes; off += chunk)
+        pool.emplace_back([=] { std::memcpy(static_cast<char *>(dst) + off, static_cast<char const *>(src) + off,
+                                            std::min(chunk, bytes - off)); });
+    std::memcpy(dst, src, std::min(chunk, bytes));
+    for (std::thread & t : pool)
+        t.join();
+}
+
 } // namespace
 
 extern "C" int hibf_gpu_hibf_serve(hibf_gpu_hibf hibf,
@@ -77,6 +118,8 @@ extern "C" int hibf_gpu_hibf_serve(hibf_gpu_hibf hibf,
             return HIBF_GPU_ERR_INVALID_ARGUMENT;
     }
 
+    bool const pinned = host_pinned(h_values, h_offsets[nq] * 8u);
+    unsigned const copy_threads = std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
     std::vector<Slot> slots(p.streams);
     int rc = HIBF_GPU_OK;
     auto cleanup = [&]()
@@ -118,7 +161,8 @@ extern "C" int hibf_gpu_hibf_serve(hibf_gpu_hibf hibf,
     {
         SERVE_TRY(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
         SERVE_TRY(hipEventCreateWithFlags(&s.done, hipEventDisableTiming));
-        SERVE_TRY(hipHostMalloc(reinterpret_cast<void **>(&s.h_vals), V * 8u, hipHostMallocDefault));
+        if (!pinned)
+            SERVE_TRY(hipHostMalloc(reinterpret_cast<void **>(&s.h_vals), V * 8u, hipHostMallocDefault));
         SERVE_TRY(hipHostMalloc(reinterpret_cast<void **>(&s.h_offs), (B + 1) * 8u, hipHostMallocDefault));
         SERVE_TRY(hipHostMalloc(reinterpret_cast<void **>(&s.h_thr), B * 2u, hipHostMallocDefault));
         SERVE_TRY(hipHostMalloc(reinterpret_cast<void **>(&s.h_ids), B * R * 8u, hipHostMallocDefault));
@@ -139,6 +183,12 @@ extern "C" int hibf_gpu_hibf_serve(hibf_gpu_hibf hibf,
         uint64_t v = 0, lk = 0, rb = 0;
         (void)hibf_gpu_agent_stats(s.agent, &v, &lk, &rb, 1);
     }
+
+    // Touch the result arrays before the clock starts: first-touch page faults
+    // inside the loop would be charged to the requests that hit them.
+    std::memset(h_latency_s, 0, nq * sizeof(double));
+    std::memset(h_counts, 0, nq * sizeof(uint32_t));
+    std::memset(h_ids, 0, nq * R * sizeof(uint64_t));
 
     uint64_t next = 0;          // next request to arrive
     uint64_t pending = 0;       // first admitted request not yet launched
@@ -215,12 +265,18 @@ extern "C" int hibf_gpu_hibf_serve(hibf_gpu_hibf hibf,
                     Slot & s = *free_slot;
                     uint64_t const nb = q1 - pending;
                     uint64_t const base = h_offsets[pending];
-                    std::memcpy(s.h_vals, h_values + base, nv * 8u);
                     for (uint64_t i = 0; i <= nb; ++i)
                         s.h_offs[i] = h_offsets[pending + i] - base;
                     std::memcpy(s.h_thr, h_thresholds + pending, nb * 2u);
-                    SERVE_TRY(hipMemcpyAsync(s.d_vals, s.h_vals, std::max<uint64_t>(nv, 1) * 8u, hipMemcpyHostToDevice,
-                                             s.stream));
+                    if (nv && pinned)
+                    {
+                        SERVE_TRY(hipMemcpyAsync(s.d_vals, h_values + base, nv * 8u, hipMemcpyHostToDevice, s.stream));
+                    }
+                    else if (nv)
+                    {
+                        copy_host(s.h_vals, h_values + base, nv * 8u, copy_threads);
+                        SERVE_TRY(hipMemcpyAsync(s.d_vals, s.h_vals, nv * 8u, hipMemcpyHostToDevice, s.stream));
+                    }
                     SERVE_TRY(hipMemcpyAsync(s.d_offs, s.h_offs, (nb + 1) * 8u, hipMemcpyHostToDevice, s.stream));
                     SERVE_TRY(hipMemcpyAsync(s.d_thr, s.h_thr, nb * 2u, hipMemcpyHostToDevice, s.stream));
                     rc = hibf_gpu_hibf_membership_launch(hibf, s.agent, s.d_vals, s.d_offs, nb, s.d_thr, s.d_ids,

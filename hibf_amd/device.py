@@ -344,8 +344,15 @@ class DeviceHibf:
               window_us: float = 500.0, max_batch: int = 65536, max_batch_values: int = 1 << 24,
               budget_bytes: int = 4 << 30, streams: int = 4, results_per_query: int = 16):
         """Runs the serving loop over requests arriving at arrival_s (host arrays).
+        values may be a numpy array or a pinned CPU tensor (torch pin_memory),
+        which the loop copies to the device without staging.
         Returns (latency_s, ids[nq, R], counts, stats dict)."""
-        v = np.ascontiguousarray(values, dtype=np.uint64)
+        if isinstance(values, torch.Tensor):
+            if values.is_cuda or values.dtype not in (torch.int64, torch.uint64) or not values.is_contiguous():
+                raise ValueError("values must be a contiguous 64-bit host tensor")
+            v = values
+        else:
+            v = np.ascontiguousarray(values, dtype=np.uint64)
         o = np.ascontiguousarray(offsets, dtype=np.uint64)
         t = np.ascontiguousarray(thresholds, dtype=np.uint16)
         a = np.ascontiguousarray(arrival_s, dtype=np.float64)
@@ -355,7 +362,8 @@ class DeviceHibf:
         cnt = np.zeros(nq, dtype=np.uint32)
         p = native.ServeParams(window_us, max_batch, max_batch_values, budget_bytes, streams, results_per_query)
         st = native.ServeStats()
-        P = lambda x: x.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
+        P = lambda x: ctypes.c_void_p(x.data_ptr()) if isinstance(x, torch.Tensor) \
+            else x.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
         check(lib().hibf_gpu_hibf_serve(self.h, ctypes.byref(p), P(v), P(o), P(t), P(a), nq, P(lat), P(ids), P(cnt),
                                         ctypes.byref(st)))
         stats = {k: getattr(st, k) for k, _ in native.ServeStats._fields_}

@@ -278,7 +278,9 @@ int hibf_gpu_hibf_membership_launch(hibf_gpu_hibf hibf,
  * copied to the device and answered on one of `streams` streams, so batches
  * overlap.  Outputs: h_latency_s[q] (arrival to ids on the host), up to
  * results_per_query ids per request in h_ids[q*R ..] with the full count in
- * h_counts[q] (ids in the reference's depth-first order). */
+ * h_counts[q] (ids in the reference's depth-first order; unused id slots are
+ * 0).  When h_values is pinned host memory (hipHostMalloc), batches are copied
+ * from it directly; otherwise through pinned staging buffers. */
 typedef struct hibf_gpu_serve_params
 {
     double window_us;

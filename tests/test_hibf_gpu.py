@@ -214,11 +214,14 @@ def test_hibf_synthetic_build_matches_oracle():
     assert np.array_equal(to_numpy(dev.bulk_count(values, offsets, ts)), ref.bulk_count_batch(values, offsets, ts))
 
 
-def test_serving_results_equal_alone_and_in_the_mix():
+@pytest.mark.parametrize("pinned", [False, True], ids=["pageable", "pinned"])
+def test_serving_results_equal_alone_and_in_the_mix(pinned):
     """Each request's ids from the serving loop (batched with others, several
-    streams in flight) equal its ids when it is answered alone."""
+    streams in flight) equal its ids when it is answered alone; query values
+    handed over in pageable memory (staged) or in a pinned buffer (direct)."""
+    import torch
     from hibf_amd.device import DeviceHibf
-    _say("serving mixed vs alone")
+    _say(f"serving mixed vs alone, {'pinned' if pinned else 'pageable'} values")
     rng = np.random.default_rng(21)
     ubs = _random_user_bins(rng, 400, 20, 300)
     dev = DeviceHibf.build(ubs, tmax=64)
@@ -228,7 +231,11 @@ def test_serving_results_equal_alone_and_in_the_mix():
     values, offsets = _csr(qs)
     arrival = np.cumsum(rng.exponential(1 / 200000, size=len(qs)))
     arrival -= arrival[0]
-    lat, ids, cnt, st = dev.serve(values, offsets, ts, arrival, window_us=300, streams=3, max_batch=256,
+    v_in = values
+    if pinned:
+        v_in = torch.empty(values.size, dtype=torch.int64, pin_memory=True)
+        v_in.numpy()[:] = values.view(np.int64)
+    lat, ids, cnt, st = dev.serve(v_in, offsets, ts, arrival, window_us=300, streams=3, max_batch=256,
                                   results_per_query=64)
     assert (lat > 0).all() and st["batches"] > 1
     alone = [ _membership_lists(dev, [qs[i]], [int(ts[i])])[0] for i in range(0, len(qs), 37)]
@@ -238,3 +245,24 @@ def test_serving_results_equal_alone_and_in_the_mix():
     batch = _membership_lists(dev, qs, ts)
     for i in range(len(qs)):
         assert int(cnt[i]) == len(batch[i]) and ids[i, :min(64, int(cnt[i]))].tolist() == batch[i][:64]
+
+
+def test_serving_large_staged_batches():
+    """Batches above 8 MiB of pageable query values are staged by several host
+    threads; every request's ids still equal the batched membership result."""
+    from hibf_amd.device import DeviceHibf
+    _say("serving large staged batches")
+    rng = np.random.default_rng(22)
+    ubs = _random_user_bins(rng, 300, 50, 400)
+    dev = DeviceHibf.build(ubs, tmax=64)
+    qs = _mixed_queries(rng, ubs, 12000, 200)
+    ts = np.array([int(0.5 * len(q)) for q in qs], dtype=np.uint16)
+    values, offsets = _csr(qs)
+    arrival = np.cumsum(rng.exponential(1 / 4e6, size=len(qs)))
+    arrival -= arrival[0]
+    lat, ids, cnt, st = dev.serve(values, offsets, ts, arrival, window_us=5000, streams=2, max_batch=65536,
+                                  max_batch_values=1 << 23, results_per_query=8)
+    assert int(offsets[-1]) * 8 / st["batches"] > (8 << 20) / 4  # batches of megabytes
+    batch = _membership_lists(dev, qs, ts)
+    for i in range(len(qs)):
+        assert int(cnt[i]) == len(batch[i]) and ids[i, :min(8, int(cnt[i]))].tolist() == batch[i][:8]
