@@ -16,7 +16,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libhibf_gpu.so")
 
 HIP_SOURCES = ["ibf_kernels.hip", "count_wide.hip", "hibf_kernels.hip", "hibf_wave.hip", "serving.hip", "scan.hip", "capi.hip"]
-CXX_SOURCES = ["layout.cpp", "build.cpp"]
+CXX_SOURCES = ["layout.cpp", "build.cpp", "archive.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 COMMON = ["-O3", "-std=c++20", "-fPIC", "-Wall", "-Wno-unused-function",
@@ -68,7 +68,7 @@ def build_tools(verbose: bool = False) -> list[str]:
     return out
 
 
-CPP_TESTS = [os.path.join(ROOT, "tests", "cpp", "snippets.cpp")]
+CPP_TESTS = [os.path.join(ROOT, "tests", "cpp", "snippets.cpp"), os.path.join(ROOT, "tests", "cpp", "archive_roundtrip.cpp")]
 
 
 def build_cpp_tests(verbose: bool = False) -> list[str]:

@@ -122,10 +122,22 @@ int ensure(void *& ptr, size_t & have, size_t bytes, char const * what)
 
 } // namespace
 
+namespace hibf_amd
+{
+// Records the detail message of a failing call (hibf_gpu_last_error); used by
+// the host files of the library that are not part of this one.
+int set_error(int status, std::string msg)
+{
+    return fail(status, std::move(msg));
+}
+} // namespace hibf_amd
+
 struct hibf_gpu_ibf_s
 {
     hibf_gpu_ibf_geometry g{};
     uint64_t * d_words{nullptr};
+    std::vector<uint64_t> occupancy; // host copy, technical_bins entries (interleaved_bloom_filter.hpp:451-458)
+    bool track_occupancy{false};
 };
 
 struct hibf_gpu_agent_s
@@ -186,6 +198,7 @@ const char * hibf_gpu_status_string(int status)
     case HIBF_GPU_ERR_CONFIG: return "invalid config";
     case HIBF_GPU_ERR_BUDGET: return "over the counter-memory budget";
     case HIBF_GPU_ERR_NO_DEVICE: return "no HIP device";
+    case HIBF_GPU_ERR_IO: return "file or archive error";
     default: return "unknown status";
     }
 }
@@ -300,7 +313,40 @@ int hibf_gpu_ibf_create(uint64_t bins, uint64_t bin_size, uint32_t hash_funs, hi
         delete h;
         return hip_fail(e, "hibf_gpu_ibf_create");
     }
+    try
+    {
+        h->occupancy.assign(g.technical_bins, 0u);
+    }
+    catch (...)
+    {
+        (void)hipFree(h->d_words);
+        delete h;
+        return fail(HIBF_GPU_ERR_OUT_OF_MEMORY, "host allocation failed");
+    }
     *out = h;
+    return HIBF_GPU_OK;
+}
+
+int hibf_gpu_ibf_occupancy(hibf_gpu_ibf ibf, uint64_t * h_occupancy, int * track_occupancy)
+{
+    if (!ibf)
+        return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "null handle");
+    if (h_occupancy)
+        std::copy(ibf->occupancy.begin(), ibf->occupancy.end(), h_occupancy);
+    if (track_occupancy)
+        *track_occupancy = ibf->track_occupancy ? 1 : 0;
+    return HIBF_GPU_OK;
+}
+
+int hibf_gpu_ibf_set_occupancy(hibf_gpu_ibf ibf, uint64_t const * h_occupancy, int track_occupancy)
+{
+    if (!ibf)
+        return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "null handle");
+    if (h_occupancy)
+        std::copy(h_occupancy, h_occupancy + ibf->occupancy.size(), ibf->occupancy.begin());
+    else
+        std::fill(ibf->occupancy.begin(), ibf->occupancy.end(), 0u);
+    ibf->track_occupancy = track_occupancy != 0;
     return HIBF_GPU_OK;
 }
 
@@ -417,7 +463,11 @@ int hibf_gpu_ibf_clear(hibf_gpu_ibf ibf, uint64_t const * h_bins, uint64_t n, vo
     if (e == hipSuccess)
         e = hipStreamSynchronize(s);
     (void)hipFree(d_bins);
-    return e == hipSuccess ? HIBF_GPU_OK : hip_fail(e, "hibf_gpu_ibf_clear");
+    if (e != hipSuccess)
+        return hip_fail(e, "hibf_gpu_ibf_clear");
+    for (uint64_t i = 0; i < n; ++i) // src/interleaved_bloom_filter.cpp:145
+        ibf->occupancy[h_bins[i]] = 0u;
+    return HIBF_GPU_OK;
 }
 
 int hibf_gpu_ibf_bulk_contains(hibf_gpu_ibf ibf, uint64_t const * d_values, uint64_t n, uint64_t * d_out, void * stream)
@@ -811,6 +861,14 @@ int hibf_gpu_hibf_emplace(hibf_gpu_hibf h,
                                 static_cast<uint32_t>(g.hash_funs), d_values, d_bins, bin, n, g.bins,
                                 static_cast<hipStream_t>(stream)),
                  "hibf emplace launch");
+    return HIBF_GPU_OK;
+}
+
+int hibf_gpu_hibf_device_arena(hibf_gpu_hibf h, uint64_t ** d_arena)
+{
+    if (!h || !d_arena)
+        return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "null argument");
+    *d_arena = h->d_arena;
     return HIBF_GPU_OK;
 }
 

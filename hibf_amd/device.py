@@ -101,6 +101,37 @@ class DeviceIbf:
             lib().hibf_gpu_ibf_destroy(self.h)
             self.h = None
 
+    @classmethod
+    def load(cls, path: str, offset: int = 0) -> tuple["DeviceIbf", int]:
+        """Loads an interleaved_bloom_filter archive (cereal binary layout)
+        starting at byte `offset`; returns the IBF and the offset after it."""
+        _dev()
+        h, end = ctypes.c_void_p(), ctypes.c_uint64()
+        check(lib().hibf_gpu_ibf_load(str(path).encode(), offset, ctypes.byref(h), ctypes.byref(end)))
+        self = cls.__new__(cls)
+        self.h = h
+        g = Geometry()
+        check(lib().hibf_gpu_ibf_get_geometry(self.h, ctypes.byref(g)))
+        self.geometry = g
+        self._agent = None
+        return self, int(end.value)
+
+    def save(self, path: str, append: bool = False) -> None:
+        """Writes the interleaved_bloom_filter archive (cereal binary layout)."""
+        check(lib().hibf_gpu_ibf_save(self.h, str(path).encode(), int(append)))
+
+    def occupancy(self) -> tuple[np.ndarray, bool]:
+        occ = np.zeros(int(self.geometry.technical_bins), dtype=np.uint64)
+        track = ctypes.c_int()
+        check(lib().hibf_gpu_ibf_occupancy(self.h, occ.ctypes.data_as(ctypes.c_void_p), ctypes.byref(track)))
+        return occ, bool(track.value)
+
+    def set_occupancy(self, occupancy, track: bool) -> None:
+        occ = np.ascontiguousarray(occupancy, dtype=np.uint64)
+        if occ.size != int(self.geometry.technical_bins):
+            raise ValueError("occupancy needs technical_bins entries")
+        check(lib().hibf_gpu_ibf_set_occupancy(self.h, occ.ctypes.data_as(ctypes.c_void_p), int(track)))
+
     # interleaved_bloom_filter.hpp:400-427
     def bin_count(self) -> int:
         return int(self.geometry.bins)
@@ -252,6 +283,19 @@ class DeviceHibf:
         self.number_of_user_bins = int(n_ub.value)
         self._agent = None
         return self
+
+    @classmethod
+    def load(cls, path: str, offset: int = 0) -> tuple["DeviceHibf", int]:
+        """Loads a hierarchical_interleaved_bloom_filter archive (cereal binary
+        layout) starting at byte `offset`; returns the HIBF and the offset after it."""
+        _dev()
+        h, end = ctypes.c_void_p(), ctypes.c_uint64()
+        check(lib().hibf_gpu_hibf_load(str(path).encode(), offset, ctypes.byref(h), ctypes.byref(end)))
+        return cls._from_handle(h), int(end.value)
+
+    def save(self, path: str, append: bool = False) -> None:
+        """Writes the hierarchical_interleaved_bloom_filter archive (cereal binary layout)."""
+        check(lib().hibf_gpu_hibf_save(self.h, str(path).encode(), int(append)))
 
     @classmethod
     def build(cls, user_bins, layout_text: str | None = None, **config) -> "DeviceHibf":

@@ -90,7 +90,7 @@ _INT = ctypes.c_int
 _SIGNATURES = {
     "hibf_gpu_status_string": (ctypes.c_char_p, [_INT]),
     "hibf_gpu_last_error": (ctypes.c_char_p, []),
-"hibf_gpu_bin_size_in_bits": (_U64, [ctypes.c_double, _U64, _U64]),
+    "hibf_gpu_bin_size_in_bits": (_U64, [ctypes.c_double, _U64, _U64]),
     "hibf_gpu_malloc": (_INT, [ctypes.POINTER(_V), _U64]),
     "hibf_gpu_free": (_INT, [_V]),
     "hibf_gpu_memcpy": (_INT, [_V, _V, _U64, _INT]),
@@ -121,6 +121,13 @@ _SIGNATURES = {
     "hibf_gpu_hibf_emplace": (_INT, [_V, _U64, _V, _V, _U64, _U64, _V]),
     "hibf_gpu_hibf_download_arena": (_INT, [_V, _V, _U64]),
     "hibf_gpu_hibf_arena_words": (_INT, [_V, ctypes.POINTER(_U64)]),
+    "hibf_gpu_hibf_device_arena": (_INT, [_V, ctypes.POINTER(_V)]),
+    "hibf_gpu_ibf_occupancy": (_INT, [_V, _V, ctypes.POINTER(_INT)]),
+    "hibf_gpu_ibf_set_occupancy": (_INT, [_V, _V, _INT]),
+    "hibf_gpu_ibf_save": (_INT, [_V, ctypes.c_char_p, _INT]),
+    "hibf_gpu_ibf_load": (_INT, [ctypes.c_char_p, _U64, ctypes.POINTER(_V), ctypes.POINTER(_U64)]),
+    "hibf_gpu_hibf_save": (_INT, [_V, ctypes.c_char_p, _INT]),
+    "hibf_gpu_hibf_load": (_INT, [ctypes.c_char_p, _U64, ctypes.POINTER(_V), ctypes.POINTER(_U64)]),
     "hibf_gpu_hibf_membership_for": (_INT, [_V, _V, _V, _V, _U64, _V, _V, _U64, _V, ctypes.POINTER(_U64), _V]),
     "hibf_gpu_hibf_bulk_count": (_INT, [_V, _V, _V, _V, _U64, _V, _U32, _V, _V]),
     "hibf_gpu_hibf_serve": (_INT, [_V, ctypes.POINTER(ServeParams), _V, _V, _V, _V, _U64, _V, _V, _V,

@@ -12,6 +12,7 @@
 #include <concepts>
 #include <cstddef>
 #include <cstdint>
+#include <filesystem>
 #include <istream>
 #include <iterator>
 #include <memory>
@@ -133,6 +134,24 @@ public:
 
     hibf_gpu_hibf handle() const noexcept { return handle_ ? handle_->h : nullptr; }
 
+    // Archive in the byte layout a cereal::BinaryOutputArchive writes for the
+    // reference's object (serialize, hierarchical_interleaved_bloom_filter.hpp:267-277);
+    // `append` adds it to the end of an existing file.
+    void store(std::filesystem::path const & path, bool const append = false) const
+    {
+        detail::check(hibf_gpu_hibf_save(handle(), path.c_str(), append ? 1 : 0));
+    }
+
+    // Reads such an archive starting at byte `offset`; returns the offset after it.
+    uint64_t load(std::filesystem::path const & path, uint64_t const offset = 0u)
+    {
+        hibf_gpu_hibf h = nullptr;
+        uint64_t end = 0;
+        detail::check(hibf_gpu_hibf_load(path.c_str(), offset, &h, &end));
+        adopt(h);
+        return end;
+    }
+
 private:
     void build(config & configuration, std::string const * layout_text)
     {
@@ -150,10 +169,16 @@ private:
         hibf_gpu_hibf h = nullptr;
         detail::check(hibf_gpu_hibf_build(&c, offsets.data(), values.empty() ? offsets.data() : values.data(),
                                           layout_text ? layout_text->c_str() : nullptr, &h));
+        adopt(h);
+    }
+
+    // Takes ownership of a device HIBF and mirrors its routing tables.
+    void adopt(hibf_gpu_hibf h)
+    {
         handle_ = std::make_shared<detail::hibf_handle>(h);
-        number_of_user_bins = configuration.number_of_user_bins;
-        uint64_t n_ibfs = 0, n_tbs = 0;
-        detail::check(hibf_gpu_hibf_info(h, &n_ibfs, &n_tbs, nullptr, nullptr, nullptr));
+        uint64_t n_ibfs = 0, n_tbs = 0, n_ub = 0;
+        detail::check(hibf_gpu_hibf_info(h, &n_ibfs, &n_tbs, nullptr, &n_ub, nullptr));
+        number_of_user_bins = n_ub;
         std::vector<hibf_gpu_ibf_geometry> geoms(n_ibfs);
         std::vector<uint64_t> wo(n_ibfs), to(n_ibfs + 1), nx(n_tbs), ub(n_tbs);
         detail::check(hibf_gpu_hibf_export_tables(h, geoms.data(), wo.data(), to.data(), nx.data(), ub.data()));

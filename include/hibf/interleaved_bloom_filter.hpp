@@ -13,6 +13,7 @@
 #include <concepts>
 #include <cstddef>
 #include <cstdint>
+#include <filesystem>
 #include <iterator>
 #include <memory>
 #include <ranges>
@@ -244,6 +245,32 @@ public:
 
     hibf_gpu_ibf handle() const noexcept { return handle_ ? handle_->h : nullptr; }
     hibf_gpu_ibf_geometry const & geometry() const noexcept { return geometry_; }
+
+    // Archive in the byte layout a cereal::BinaryOutputArchive writes for the
+    // reference's object (serialize, interleaved_bloom_filter.hpp:471-485);
+    // `append` adds it to the end of an existing file.
+    void store(std::filesystem::path const & path, bool const append = false) const
+    {
+        std::vector<uint64_t> const occ(occupancy.begin(), occupancy.end());
+        detail::check(hibf_gpu_ibf_set_occupancy(handle_->h, occ.data(), track_occupancy ? 1 : 0));
+        detail::check(hibf_gpu_ibf_save(handle_->h, path.c_str(), append ? 1 : 0));
+    }
+
+    // Reads such an archive starting at byte `offset`; returns the offset after it.
+    uint64_t load(std::filesystem::path const & path, uint64_t const offset = 0u)
+    {
+        hibf_gpu_ibf h = nullptr;
+        uint64_t end = 0;
+        detail::check(hibf_gpu_ibf_load(path.c_str(), offset, &h, &end));
+        handle_ = std::make_shared<detail::ibf_handle>(h);
+        detail::check(hibf_gpu_ibf_get_geometry(h, &geometry_));
+        std::vector<uint64_t> occ(geometry_.technical_bins);
+        int track = 0;
+        detail::check(hibf_gpu_ibf_occupancy(h, occ.data(), &track));
+        occupancy.assign(occ.begin(), occ.end());
+        track_occupancy = track != 0;
+        return end;
+    }
 
     bool track_occupancy{false};
     std::vector<size_t> occupancy{};

@@ -34,7 +34,8 @@ enum hibf_gpu_status
     HIBF_GPU_ERR_UNSUPPORTED = 8,      /* geometry outside what the kernels handle    */
     HIBF_GPU_ERR_CONFIG = 9,           /* invalid config (config.cpp:64-130)          */
     HIBF_GPU_ERR_BUDGET = 10,          /* admission over the counter-memory budget    */
-    HIBF_GPU_ERR_NO_DEVICE = 11        /* no HIP device visible                       */
+    HIBF_GPU_ERR_NO_DEVICE = 11,       /* no HIP device visible                       */
+    HIBF_GPU_ERR_IO = 12               /* file cannot be read/written, or is not an archive of the type */
 };
 
 /* Text of a status code (static storage). */
@@ -102,6 +103,14 @@ int hibf_gpu_ibf_device_words(hibf_gpu_ibf ibf, uint64_t ** d_words);
  * layout is the reference's bit_vector storage (bit_vector.hpp:868-876). */
 int hibf_gpu_ibf_upload(hibf_gpu_ibf ibf, uint64_t const * h_words, uint64_t n_words);
 int hibf_gpu_ibf_download(hibf_gpu_ibf ibf, uint64_t * h_words, uint64_t n_words);
+
+/* Occupancy kept with the handle (interleaved_bloom_filter.hpp:451-458):
+ * technical_bins counters and the track_occupancy flag.  The device emplace
+ * does not update them (track_occupancy is order-dependent); clear() resets
+ * the cleared bins' counters as src/interleaved_bloom_filter.cpp:145 does.
+ * h_occupancy may be NULL (get: skipped; set: all zero). */
+int hibf_gpu_ibf_occupancy(hibf_gpu_ibf ibf, uint64_t * h_occupancy, int * track_occupancy);
+int hibf_gpu_ibf_set_occupancy(hibf_gpu_ibf ibf, uint64_t const * h_occupancy, int track_occupancy);
 
 /* Bulk emplace; replaces emplace(value, bin_index) (interleaved_bloom_filter.hpp:259;
  * src/interleaved_bloom_filter.cpp:119-140) called for n (value, bin) pairs.
@@ -232,6 +241,8 @@ int hibf_gpu_hibf_emplace(hibf_gpu_hibf hibf,
 /* Copies the whole device arena back (synchronous, for parity checks). */
 int hibf_gpu_hibf_download_arena(hibf_gpu_hibf hibf, uint64_t * h_arena, uint64_t n_words);
 int hibf_gpu_hibf_arena_words(hibf_gpu_hibf hibf, uint64_t * n_words);
+/* Device address of the arena (IBF i's words start at word_offsets[i]). */
+int hibf_gpu_hibf_device_arena(hibf_gpu_hibf hibf, uint64_t ** d_arena);
 
 /* Batched HIBF membership_agent_type::membership_for
  * (hierarchical_interleaved_bloom_filter.hpp:321-355, 424-439): user-bin ids in
@@ -343,6 +354,36 @@ int hibf_gpu_hibf_export_tables(hibf_gpu_hibf hibf,
                                 uint64_t * tb_offsets,
                                 uint64_t * next_ibf,
                                 uint64_t * user_bin);
+
+/* ======================================================================== */
+/* Archives: cereal BinaryArchive layout                                    */
+/* ======================================================================== */
+
+/* interleaved_bloom_filter::serialize (interleaved_bloom_filter.hpp:471-485)
+ * with bit_vector::save/load (bit_vector.hpp:885-924), in the byte layout a
+ * cereal::BinaryOutputArchive writes for it: uint32 version (1); uint64 bins,
+ * technical_bins, bin_size, hash_shift, bin_words, hash_funs; uint64 bit count
+ * followed by the bit_vector words; uint64 occupancy length followed by the
+ * occupancy counters; one byte track_occupancy.  Files written by util/inspect-
+ * style tools (a bare archive of the object) load here and vice versa.
+ * save: append != 0 appends to an existing file (for wrapper formats).
+ * load: reads from byte `offset` of the file and reports the offset after the
+ * object in *end_offset (may be NULL).  Errors: HIBF_GPU_ERR_IO. */
+int hibf_gpu_ibf_save(hibf_gpu_ibf ibf, const char * path, int append);
+int hibf_gpu_ibf_load(const char * path, uint64_t offset, hibf_gpu_ibf * out, uint64_t * end_offset);
+
+/* hierarchical_interleaved_bloom_filter::serialize
+ * (hierarchical_interleaved_bloom_filter.hpp:267-277): uint32 version (1);
+ * uint64 number_of_user_bins; ibf_vector (uint64 count, then each IBF as
+ * above); next_ibf_id (uint64 count, then per IBF a uint64 length and the
+ * ids); prev_ibf_id (uint64 count, then per IBF uint64 ibf_idx, bin_idx);
+ * ibf_bin_to_user_bin_id (as next_ibf_id).  The device HIBF keeps neither
+ * occupancy nor prev_ibf_id: save writes zero occupancy with track_occupancy
+ * false (the reference's HIBF build at empty_bin_fraction 0) and prev_ibf_id
+ * derived from the merged bins ({0, 0} for the root); load checks and drops
+ * them. */
+int hibf_gpu_hibf_save(hibf_gpu_hibf hibf, const char * path, int append);
+int hibf_gpu_hibf_load(const char * path, uint64_t offset, hibf_gpu_hibf * out, uint64_t * end_offset);
 
 /* ======================================================================== */
 /* HIBF construction (host layout + device bulk emplace)                    */

@@ -31,3 +31,14 @@ def test_cpp_snippets_print_reference_outputs():
     got = [line.rstrip() for line in r.stdout.splitlines()]
     want = [line.rstrip() for line in open(EXPECTED).read().splitlines()]
     assert got == want
+
+
+@pytest.mark.gpu
+def test_cpp_store_and_load(tmp_path):
+    """store()/load() of the C++ classes: loaded objects answer as the originals."""
+    exe = os.path.join(ROOT, "tests", "cpp", "archive_roundtrip")
+    assert os.path.exists(exe), "build tests/cpp/archive_roundtrip first (python -m hibf_amd.build)"
+    r = subprocess.run([exe, str(tmp_path)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    lines = r.stdout.splitlines()
+    assert len(lines) == 6 and all(line.endswith(" same") for line in lines), r.stdout
