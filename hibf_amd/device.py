@@ -232,7 +232,7 @@ def _thresholds16(thresholds, nq: int, device) -> torch.Tensor:
         if thresholds.dtype != torch.uint16 or thresholds.numel() != nq or not thresholds.is_contiguous():
             raise ValueError("device thresholds must be a contiguous uint16 tensor of one entry per query")
         return thresholds
-    a = np.ascontiguousarray(np.broadcast_to(np.asarray(thresholds, dtype=np.uint16), (nq,)))
+    a = np.array(np.broadcast_to(np.asarray(thresholds, dtype=np.uint16), (nq,)), dtype=np.uint16)  # own copy
     return torch.from_numpy(a.view(np.int16)).to(device).view(torch.uint16)
 
 
