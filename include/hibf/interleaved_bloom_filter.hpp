@@ -229,7 +229,17 @@ public:
     size_t bin_size() const noexcept { return geometry_.bin_size; }
     size_t bit_size() const noexcept { return geometry_.technical_bins * geometry_.bin_size; }
 
-    // Host copy of the bit matrix (the reference's data(), bit_vector words).
+    // The reference's data() is a host pointer to the bit_vector words; here
+    // the words live in HBM, so there is no data(): device_data() is their
+    // device address and words() a host copy.
+    uint64_t * device_data() const
+    {
+        uint64_t * d = nullptr;
+        detail::check(hibf_gpu_ibf_device_words(handle_->h, &d));
+        return d;
+    }
+
+    // Host copy of the bit matrix (bit_vector words, row-major).
     std::vector<uint64_t> words() const
     {
         std::vector<uint64_t> w(geometry_.n_words);
