@@ -227,6 +227,8 @@ int hibf_gpu_tune_set(char const * key, int64_t value)
         g_count_tuning.counter_bits = static_cast<uint32_t>(value);
     else if (k == "hibf.row_per_lane" && (value == 0 || value == 1))
         g_count_tuning.hibf_narrow = static_cast<uint32_t>(value);
+    else if (k == "hibf.sliced_counts" && (value == 0 || value == 1))
+        g_count_tuning.sliced_counts = static_cast<uint32_t>(value);
     else
         return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "unknown tuning key or value out of range: " + k);
     return HIBF_GPU_OK;
@@ -251,6 +253,8 @@ int hibf_gpu_tune_get(char const * key, int64_t * value)
         *value = g_count_tuning.counter_bits;
     else if (k == "hibf.row_per_lane")
         *value = g_count_tuning.hibf_narrow;
+    else if (k == "hibf.sliced_counts")
+        *value = g_count_tuning.sliced_counts;
     else
         return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "unknown tuning key: " + k);
     return HIBF_GPU_OK;

@@ -113,7 +113,13 @@ __device__ __forceinline__ uint64_t readlane64(uint64_t x, int src)
 // HF > 0 fixes the number of hash functions at compile time (it must equal
 // g.hash_funs), which keeps only HF row starts per value in registers; HF == 0
 // handles any count up to 5 at run time.
-template <int LPV, int CW, int U, bool NT = false, int HF = 0>
+//
+// With 2 or 4 lanes per value, a lane sees LPV values of each 64-value chunk,
+// all for the same tile words.  Their masks are summed in bit-sliced registers
+// (plane p holds bit p of each bin's count) and added to LDS once per chunk:
+// one atomic per distinct bin instead of one per set bit and value, which
+// matters because most values of a query set the same bin.
+template <int LPV, int CW, int U, bool NT = false, int HF = 0, bool SLICE = true>
 __device__ __forceinline__ void count_into_lds(IbfView const & g,
                                                uint64_t const * __restrict__ vals,
                                                uint32_t L,
@@ -126,6 +132,8 @@ __device__ __forceinline__ void count_into_lds(IbfView const & g,
 {
     constexpr int VPS = 64 / LPV;
     constexpr int KMAX = HF > 0 ? HF : 5;
+    constexpr bool SLICED = SLICE && LPV >= 2 && LPV <= 4;
+    constexpr int PLANES = LPV >= 4 ? 3 : 2; // counts up to LPV per chunk
     int const sub = lane / LPV;
     uint32_t const wl = static_cast<uint32_t>(lane % LPV);
     uint64_t const bw = g.bin_words;
@@ -141,6 +149,13 @@ __device__ __forceinline__ void count_into_lds(IbfView const & g,
         for (int k = 0; k < KMAX; ++k)
             rows[k] = (HF > 0 || static_cast<uint32_t>(k) < hf) ? row_word(v, hash_seed(k), g.hash_shift, g.bin_size, bw)
                                                                  : 0ULL;
+
+        uint64_t plane[PLANES][CW];
+#pragma unroll
+        for (int p = 0; p < PLANES; ++p)
+#pragma unroll
+            for (int c = 0; c < CW; ++c)
+                plane[p][c] = 0ULL;
 
         for (uint32_t s = 0; s < nv; s += VPS * U)
         {
@@ -189,14 +204,50 @@ __device__ __forceinline__ void count_into_lds(IbfView const & g,
 #pragma unroll
                 for (int c = 0; c < CW; ++c)
                 {
-                    uint64_t bits = x[u][c];
-                    uint32_t const cbase = (wl + static_cast<uint32_t>(c * LPV)) * 64u;
-                    while (bits)
+                    if constexpr (SLICED)
                     {
-                        uint32_t const b = static_cast<uint32_t>(__builtin_ctzll(bits));
-                        atomicAdd(&lds[cbase + b], 1u);
-                        bits &= bits - 1ULL;
+                        uint64_t carry = x[u][c]; // ripple-carry add of a 0/1 per bin
+#pragma unroll
+                        for (int p = 0; p < PLANES; ++p)
+                        {
+                            uint64_t const t = plane[p][c] & carry;
+                            plane[p][c] ^= carry;
+                            carry = t;
+                        }
                     }
+                    else
+                    {
+                        uint64_t bits = x[u][c];
+                        uint32_t const cbase = (wl + static_cast<uint32_t>(c * LPV)) * 64u;
+                        while (bits)
+                        {
+                            uint32_t const b = static_cast<uint32_t>(__builtin_ctzll(bits));
+                            atomicAdd(&lds[cbase + b], 1u);
+                            bits &= bits - 1ULL;
+                        }
+                    }
+                }
+            }
+        }
+        if constexpr (SLICED)
+        {
+#pragma unroll
+            for (int c = 0; c < CW; ++c)
+            {
+                uint64_t bits = 0ULL;
+#pragma unroll
+                for (int p = 0; p < PLANES; ++p)
+                    bits |= plane[p][c];
+                uint32_t const cbase = (wl + static_cast<uint32_t>(c * LPV)) * 64u;
+                while (bits)
+                {
+                    uint32_t const b = static_cast<uint32_t>(__builtin_ctzll(bits));
+                    uint32_t n = 0;
+#pragma unroll
+                    for (int p = 0; p < PLANES; ++p)
+                        n |= static_cast<uint32_t>((plane[p][c] >> b) & 1ULL) << p;
+                    atomicAdd(&lds[cbase + b], n);
+                    bits &= bits - 1ULL;
                 }
             }
         }
@@ -207,7 +258,7 @@ __device__ __forceinline__ void count_into_lds(IbfView const & g,
 // LPV = 1..64 with one word per lane (bin_words <= 64); 7 gives 64 lanes with
 // two words each (bin_words <= 128).  Used by kernels that visit IBFs of
 // different widths (the HIBF descent).
-template <int U, bool NT = false, int HF = 0>
+template <int U, bool NT = false, int HF = 0, bool SLICE = true>
 __device__ __forceinline__ void count_into_lds_dispatch(uint32_t lpv_log,
                                                         IbfView const & g,
                                                         uint64_t const * __restrict__ vals,
@@ -219,14 +270,35 @@ __device__ __forceinline__ void count_into_lds_dispatch(uint32_t lpv_log,
 {
     switch (lpv_log)
     {
+    case 0: count_into_lds<1, 1, U, NT, HF, SLICE>(g, vals, L, 0, g.bin_words, lds, wave, nwaves, lane); break;
+    case 1: count_into_lds<2, 1, U, NT, HF, SLICE>(g, vals, L, 0, g.bin_words, lds, wave, nwaves, lane); break;
+    case 2: count_into_lds<4, 1, U, NT, HF, SLICE>(g, vals, L, 0, g.bin_words, lds, wave, nwaves, lane); break;
+    case 3: count_into_lds<8, 1, U, NT, HF, SLICE>(g, vals, L, 0, g.bin_words, lds, wave, nwaves, lane); break;
+    case 4: count_into_lds<16, 1, U, NT, HF, SLICE>(g, vals, L, 0, g.bin_words, lds, wave, nwaves, lane); break;
+    case 5: count_into_lds<32, 1, U, NT, HF, SLICE>(g, vals, L, 0, g.bin_words, lds, wave, nwaves, lane); break;
+    case 6: count_into_lds<64, 1, U, NT, HF, SLICE>(g, vals, L, 0, g.bin_words, lds, wave, nwaves, lane); break;
+    default: count_into_lds<64, 2, U, NT, HF, SLICE>(g, vals, L, 0, g.bin_words, lds, wave, nwaves, lane); break;
+    }
+}
+
+// Dispatch for HIBFs whose IBFs all have rows of at most 4 words (256
+// technical bins, the tmax = 256 layout): only the 1-, 2- and 4-word shapes are
+// compiled, which keeps the kernel within 64 VGPRs.
+template <int U, bool NT = false, int HF = 0>
+__device__ __forceinline__ void count_into_lds_dispatch_narrow(uint32_t lpv_log,
+                                                               IbfView const & g,
+                                                               uint64_t const * __restrict__ vals,
+                                                               uint32_t L,
+                                                               uint32_t * lds,
+                                                               int wave,
+                                                               int nwaves,
+                                                               int lane)
+{
+    switch (lpv_log)
+    {
     case 0: count_into_lds<1, 1, U, NT, HF>(g, vals, L, 0, g.bin_words, lds, wave, nwaves, lane); break;
     case 1: count_into_lds<2, 1, U, NT, HF>(g, vals, L, 0, g.bin_words, lds, wave, nwaves, lane); break;
-    case 2: count_into_lds<4, 1, U, NT, HF>(g, vals, L, 0, g.bin_words, lds, wave, nwaves, lane); break;
-    case 3: count_into_lds<8, 1, U, NT, HF>(g, vals, L, 0, g.bin_words, lds, wave, nwaves, lane); break;
-    case 4: count_into_lds<16, 1, U, NT, HF>(g, vals, L, 0, g.bin_words, lds, wave, nwaves, lane); break;
-    case 5: count_into_lds<32, 1, U, NT, HF>(g, vals, L, 0, g.bin_words, lds, wave, nwaves, lane); break;
-    case 6: count_into_lds<64, 1, U, NT, HF>(g, vals, L, 0, g.bin_words, lds, wave, nwaves, lane); break;
-    default: count_into_lds<64, 2, U, NT, HF>(g, vals, L, 0, g.bin_words, lds, wave, nwaves, lane); break;
+    default: count_into_lds<4, 1, U, NT, HF>(g, vals, L, 0, g.bin_words, lds, wave, nwaves, lane); break;
     }
 }
 

@@ -108,7 +108,7 @@ __device__ void scan_ibf(HibfIbfDesc const & d,
 // explicit stack of per-level event lists in LDS, which reproduces the
 // reference's recursion order (hierarchical_interleaved_bloom_filter.hpp:321-355
 // and :462-496) without recursion.  HF > 0: every IBF has HF hash functions.
-template <bool COUNTING, bool NT, int HF, int U = 2, bool NARROW = false>
+template <bool COUNTING, bool NT, int HF, int U = 2, int NARROW = 0, bool SLICE = true>
 __device__ __forceinline__ void descent_body(HibfLaunch const & h,
                                              uint64_t const * __restrict__ values,
                                              uint64_t const * __restrict__ offsets,
@@ -191,10 +191,12 @@ __device__ __forceinline__ void descent_body(HibfLaunch const & h,
                     cnt[i] = 0u;
                 __syncthreads();
                 IbfView const g{h.arena + d.word_offset, d.bin_size, d.bins, d.bin_words, d.hash_shift, d.hash_funs};
-                if constexpr (NARROW)
+                if constexpr (NARROW == 2)
                     count_into_lds_row_per_lane<NT, HF>(g, vals, L, cnt, wave, nwaves, lane);
+                else if constexpr (NARROW == 1)
+                    count_into_lds_dispatch_narrow<U, NT, HF>(d.lpv_log, g, vals, L, cnt, wave, nwaves, lane);
                 else
-                    count_into_lds_dispatch<U, NT, HF>(d.lpv_log, g, vals, L, cnt, wave, nwaves, lane);
+                    count_into_lds_dispatch<U, NT, HF, SLICE>(d.lpv_log, g, vals, L, cnt, wave, nwaves, lane);
                 ++visits;
                 lookups += L;
                 row_bytes += static_cast<uint64_t>(L) * d.hash_funs * d.bin_words * 8u;
@@ -297,14 +299,24 @@ __global__ void __launch_bounds__(256) hibf_descent_kernel(HIBF_DESCENT_PARAMS)
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8)))
 hibf_descent_membership_h2_kernel(HIBF_DESCENT_PARAMS)
 {
-    descent_body<false, false, 2>(HIBF_DESCENT_ARGS);
+    descent_body<false, false, 2, 2, 0, false>(HIBF_DESCENT_ARGS);
 }
 
 // Same, for HIBFs whose rows are at most 4 words: one value and its whole row per lane.
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8)))
 hibf_descent_membership_h2_narrow_kernel(HIBF_DESCENT_PARAMS)
 {
-    descent_body<false, false, 2, 1, true>(HIBF_DESCENT_ARGS);
+    descent_body<false, false, 2, 1, 2>(HIBF_DESCENT_ARGS);
+}
+
+// Membership with two hash functions over rows of at most 4 words (the
+// tmax = 256 layout): only the 1-, 2- and 4-word counting shapes are compiled.
+// SLICE: bit-sliced per-lane counts with one LDS atomic per bin and chunk.
+template <bool SLICE>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8)))
+hibf_descent_membership_h2_w4_kernel(HIBF_DESCENT_PARAMS)
+{
+    descent_body<false, false, 2, 2, 1, SLICE>(HIBF_DESCENT_ARGS);
 }
 
 #undef HIBF_DESCENT_PARAMS
@@ -433,11 +445,14 @@ hipError_t launch_hibf_membership(HibfLaunch const & h,
     if (nq == 0)
         return hipSuccess;
     size_t const lds = static_cast<size_t>(h.max_bins) * (1u + h.depth) * sizeof(uint32_t);
-    auto const kernel = g_count_tuning.nt_loads        ? hibf_descent_kernel<false, true, 0>
-                        : h.uniform_hash_funs == 2u && h.max_bin_words <= 4u && g_count_tuning.hibf_narrow
-                            ? hibf_descent_membership_h2_narrow_kernel
-                        : h.uniform_hash_funs == 2u   ? hibf_descent_membership_h2_kernel
-                                                      : hibf_descent_kernel<false, false, 0>;
+    bool const h2 = h.uniform_hash_funs == 2u;
+    bool const w4 = h2 && h.max_bin_words <= 4u;
+    auto const kernel = g_count_tuning.nt_loads                ? hibf_descent_kernel<false, true, 0>
+                        : w4 && g_count_tuning.hibf_narrow     ? hibf_descent_membership_h2_narrow_kernel
+                        : w4 && g_count_tuning.sliced_counts   ? hibf_descent_membership_h2_w4_kernel<true>
+                        : w4                                   ? hibf_descent_membership_h2_w4_kernel<false>
+                        : h2                                   ? hibf_descent_membership_h2_kernel
+                                                               : hibf_descent_kernel<false, false, 0>;
     hipLaunchKernelGGL(kernel, dim3(blocks), dim3(256), lds, stream, h, values, offsets, nq, thresholds, nullptr, 2u,
                        nullptr, scratch, ids_tmp, ids_cap, hit_start, hit_count, qlist);
     return hipGetLastError();

@@ -158,17 +158,26 @@ CASES = [  # (n_ub, size range, config)
 ]
 
 
-@pytest.fixture(params=[(1, 1), (0, 1), (0, 0)], ids=["wave", "block-row-per-lane", "block-lanes-per-value"])
+_DESCENT_VARIANTS = {  # (hibf.wave_kernel, hibf.row_per_lane, hibf.sliced_counts)
+    "wave": (1, 0, 1),
+    "block-row-per-lane": (0, 1, 1),
+    "block-lanes-per-value-sliced": (0, 0, 1),
+    "block-lanes-per-value-atomics": (0, 0, 0),
+}
+
+
+@pytest.fixture(params=list(_DESCENT_VARIANTS))
 def descent_kernel(request):
-    """Runs a test with each descent kernel: query per wave, query per block
-    with one value and its row per lane, query per block with lanes sharing a
-    value's row.  Yields the variant's name."""
+    """Runs a test with each descent kernel: query per wave; query per block
+    with one value and its row per lane; query per block with lanes sharing a
+    value's row, counting through bit-sliced registers or with one LDS atomic
+    per set bit.  Yields the variant's name."""
     from hibf_amd import native
-    keys = ("hibf.wave_kernel", "hibf.row_per_lane")
+    keys = ("hibf.wave_kernel", "hibf.row_per_lane", "hibf.sliced_counts")
     saved = {k: native.tune_get(k) for k in keys}
-    for k, v in zip(keys, request.param):
+    for k, v in zip(keys, _DESCENT_VARIANTS[request.param]):
         native.tune_set(k, v)
-    yield {(1, 1): "wave", (0, 1): "block-row-per-lane", (0, 0): "block-lanes-per-value"}[request.param]
+    yield request.param
     for k, v in saved.items():
         native.tune_set(k, v)
 
