@@ -20,6 +20,8 @@ Search spaces:
     hibf.wave_kernel    one query per wave (1) or per block (0)
     hibf.row_per_lane   rows of <= 4 words: one value and its whole row per
                         lane (1), or lanes sharing a value's row (0)
+    hibf.sliced_counts  rows of <= 4 words: bit-sliced per-lane counts (1) or
+                        one LDS atomic per set bit (0)
     hibf.nt_loads       non-temporal row loads (0, 1)
 """
 from __future__ import annotations
@@ -46,6 +48,7 @@ MARGIN = 0.02
 HIBF_SPACE = {
     "hibf.wave_kernel": (0, 1),
     "hibf.row_per_lane": (0, 1),
+    "hibf.sliced_counts": (1, 0),
     "hibf.nt_loads": (0, 1),
 }
 

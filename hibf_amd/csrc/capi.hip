@@ -474,6 +474,62 @@ int hibf_gpu_ibf_clear(hibf_gpu_ibf ibf, uint64_t const * h_bins, uint64_t n, vo
     return HIBF_GPU_OK;
 }
 
+int hibf_gpu_ibf_try_increase_bin_number_to(hibf_gpu_ibf ibf, uint64_t new_bins, int * increased)
+{
+    if (!ibf || !increased)
+        return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "null argument");
+    uint64_t const new_bw = (new_bins + 63u) / 64u;
+    *increased = 0;
+    if (new_bins < ibf->g.bins || new_bw > ibf->g.bin_words) // src/interleaved_bloom_filter.cpp:150-160
+        return HIBF_GPU_OK;
+    if (new_bins >= (1ULL << 32))
+        return fail(HIBF_GPU_ERR_UNSUPPORTED, "bin_count must be < 2^32 on the device path");
+    ibf->g.bins = new_bins;
+    *increased = 1;
+    return HIBF_GPU_OK;
+}
+
+int hibf_gpu_ibf_increase_bin_number_to(hibf_gpu_ibf ibf, uint64_t new_bins)
+{
+    if (!ibf)
+        return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "null handle");
+    if (new_bins < ibf->g.bins) // src/interleaved_bloom_filter.cpp:165-166
+        return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "The number of new bins must be >= the current number of bins.");
+    int same_words = 0;
+    if (int rc = hibf_gpu_ibf_try_increase_bin_number_to(ibf, new_bins, &same_words))
+        return rc;
+    if (same_words)
+        return HIBF_GPU_OK;
+    hibf_gpu_ibf_geometry g{};
+    if (int rc = make_geometry(new_bins, ibf->g.bin_size, ibf->g.hash_funs, g))
+        return rc;
+    std::vector<uint64_t> occ;
+    try
+    {
+        occ.assign(g.technical_bins, 0u);
+    }
+    catch (...)
+    {
+        return fail(HIBF_GPU_ERR_OUT_OF_MEMORY, "host allocation failed");
+    }
+    std::copy(ibf->occupancy.begin(), ibf->occupancy.end(), occ.begin());
+    uint64_t * d_new = nullptr;
+    HIBF_TRY_HIP(hipMalloc(reinterpret_cast<void **>(&d_new), g.n_words * sizeof(uint64_t)), "increase_bin_number_to");
+    hipError_t e = launch_restride(ibf->d_words, ibf->g.bin_words, d_new, g.bin_words, g.bin_size, nullptr);
+    if (e == hipSuccess)
+        e = hipDeviceSynchronize();
+    if (e != hipSuccess)
+    {
+        (void)hipFree(d_new);
+        return hip_fail(e, "increase_bin_number_to");
+    }
+    (void)hipFree(ibf->d_words);
+    ibf->d_words = d_new;
+    ibf->g = g;
+    ibf->occupancy.swap(occ);
+    return HIBF_GPU_OK;
+}
+
 int hibf_gpu_ibf_bulk_contains(hibf_gpu_ibf ibf, uint64_t const * d_values, uint64_t n, uint64_t * d_out, void * stream)
 {
     if (!ibf || ((!d_values || !d_out) && n))

@@ -217,6 +217,24 @@ __global__ void __launch_bounds__(256) count_kernel(IbfView g,
     }
 }
 
+// increase_bin_number_to (src/interleaved_bloom_filter.cpp:163-201): row r of
+// the old matrix (old_bw words) becomes row r of the new one (new_bw words);
+// the added words of every row are zero.  One thread per new word.
+__global__ void __launch_bounds__(256) restride_kernel(uint64_t const * __restrict__ old_words,
+                                                       uint64_t old_bw,
+                                                       uint64_t * __restrict__ new_words,
+                                                       uint64_t new_bw,
+                                                       uint64_t n_new)
+{
+    uint64_t const stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n_new; i += stride)
+    {
+        uint64_t const r = i / new_bw;
+        uint64_t const w = i - r * new_bw;
+        new_words[i] = w < old_bw ? old_words[r * old_bw + w] : 0ULL;
+    }
+}
+
 namespace
 {
 inline unsigned grid_for(uint64_t items, unsigned per_block, unsigned cap = 1u << 20)
@@ -281,6 +299,21 @@ hipError_t launch_segment_emplace(HibfIbfDesc const * desc,
     unsigned const grid = static_cast<unsigned>(n_chunks < (1u << 20) ? n_chunks : (1u << 20));
     hipLaunchKernelGGL(segment_emplace_kernel, dim3(grid), dim3(256), 0, stream, desc, arena, segs, chunks, n_chunks,
                        values, ub_offsets, seed);
+    return hipGetLastError();
+}
+
+hipError_t launch_restride(uint64_t const * old_words,
+                           uint64_t old_bw,
+                           uint64_t * new_words,
+                           uint64_t new_bw,
+                           uint64_t bin_size,
+                           hipStream_t stream)
+{
+    uint64_t const n = new_bw * bin_size;
+    if (n == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(restride_kernel, dim3(grid_for(n, 256)), dim3(256), 0, stream, old_words, old_bw, new_words,
+                       new_bw, n);
     return hipGetLastError();
 }
 

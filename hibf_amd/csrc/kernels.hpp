@@ -89,6 +89,12 @@ hipError_t launch_emplace_synth(uint64_t * words,
                                 uint64_t per_bin,
                                 hipStream_t stream);
 
+hipError_t launch_restride(uint64_t const * old_words,
+                           uint64_t old_bw,
+                           uint64_t * new_words,
+                           uint64_t new_bw,
+                           uint64_t bin_size,
+                           hipStream_t stream);
 hipError_t launch_clear(uint64_t * words,
                         uint64_t bin_size,
                         uint32_t bin_words,

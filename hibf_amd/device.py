@@ -120,6 +120,17 @@ class DeviceIbf:
         """Writes the interleaved_bloom_filter archive (cereal binary layout)."""
         check(lib().hibf_gpu_ibf_save(self.h, str(path).encode(), int(append)))
 
+    def try_increase_bin_number_to(self, new_bins: int) -> bool:
+        ok = ctypes.c_int()
+        check(lib().hibf_gpu_ibf_try_increase_bin_number_to(self.h, new_bins, ctypes.byref(ok)))
+        check(lib().hibf_gpu_ibf_get_geometry(self.h, ctypes.byref(self.geometry)))
+        return bool(ok.value)
+
+    def increase_bin_number_to(self, new_bins: int) -> None:
+        """interleaved_bloom_filter::increase_bin_number_to, re-strided on the device."""
+        check(lib().hibf_gpu_ibf_increase_bin_number_to(self.h, new_bins))
+        check(lib().hibf_gpu_ibf_get_geometry(self.h, ctypes.byref(self.geometry)))
+
     def occupancy(self) -> tuple[np.ndarray, bool]:
         occ = np.zeros(int(self.geometry.technical_bins), dtype=np.uint64)
         track = ctypes.c_int()

@@ -199,24 +199,26 @@ public:
     }
 
     // src/interleaved_bloom_filter.cpp:150-160
+    // src/interleaved_bloom_filter.cpp:150-160: only bin_count changes.
     bool try_increase_bin_number_to(seqan::hibf::bin_count const new_bin_count) noexcept
     {
-        size_t const new_words = (new_bin_count.value + 63u) / 64u;
-        if (new_bin_count.value < geometry_.bins || new_words > geometry_.bin_words)
+        int increased = 0;
+        if (!handle_ || hibf_gpu_ibf_try_increase_bin_number_to(handle_->h, new_bin_count.value, &increased) != HIBF_GPU_OK)
             return false;
-        rebuild_with(new_bin_count.value);
-        return true;
+        if (increased)
+            geometry_.bins = new_bin_count.value;
+        return increased != 0;
     }
 
-    // src/interleaved_bloom_filter.cpp:162-202: re-stride the rows to the new
-    // technical-bin count (done on the host copy of the matrix).
+    // src/interleaved_bloom_filter.cpp:163-201: the rows are re-strided to the
+    // new technical-bin count on the device.
     void increase_bin_number_to(seqan::hibf::bin_count const new_bin_count)
     {
         if (new_bin_count.value < geometry_.bins)
             throw std::invalid_argument{"The number of new bins must be >= the current number of bins."};
-        if (try_increase_bin_number_to(new_bin_count))
-            return;
-        rebuild_with(new_bin_count.value);
+        detail::check(hibf_gpu_ibf_increase_bin_number_to(handle_->h, new_bin_count.value));
+        detail::check(hibf_gpu_ibf_get_geometry(handle_->h, &geometry_));
+        occupancy.resize(geometry_.technical_bins, 0u);
     }
 
     containment_agent_type containment_agent() const;
@@ -286,22 +288,6 @@ public:
     std::vector<size_t> occupancy{};
 
 private:
-    void rebuild_with(size_t const new_bins)
-    {
-        std::vector<uint64_t> const old = words();
-        size_t const old_words = geometry_.bin_words;
-        interleaved_bloom_filter next{seqan::hibf::bin_count{new_bins}, seqan::hibf::bin_size{geometry_.bin_size},
-                                      seqan::hibf::hash_function_count{geometry_.hash_funs}, track_occupancy};
-        size_t const new_words = next.geometry_.bin_words;
-        std::vector<uint64_t> w(next.geometry_.n_words, 0u);
-        for (size_t r = 0; r < geometry_.bin_size; ++r)
-            std::copy_n(old.begin() + r * old_words, old_words, w.begin() + r * new_words);
-        detail::check(hibf_gpu_ibf_upload(next.handle_->h, w.data(), w.size()));
-        next.occupancy = occupancy;
-        next.occupancy.resize(next.geometry_.technical_bins, 0u);
-        *this = std::move(next);
-    }
-
     std::shared_ptr<detail::ibf_handle> handle_{};
     hibf_gpu_ibf_geometry geometry_{};
 };

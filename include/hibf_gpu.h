@@ -106,6 +106,18 @@ int hibf_gpu_ibf_device_words(hibf_gpu_ibf ibf, uint64_t ** d_words);
 int hibf_gpu_ibf_upload(hibf_gpu_ibf ibf, uint64_t const * h_words, uint64_t n_words);
 int hibf_gpu_ibf_download(hibf_gpu_ibf ibf, uint64_t * h_words, uint64_t n_words);
 
+/* Replace try_increase_bin_number_to / increase_bin_number_to
+ * (interleaved_bloom_filter.hpp:320-344; src/interleaved_bloom_filter.cpp:150-201).
+ * try: when the words per row stay the same only bin_count changes and
+ * *increased is 1; otherwise nothing changes and *increased is 0.
+ * increase: additionally re-strides the bit matrix on the device to
+ * ceil(new_bins/64) words per row (new columns zero), growing the occupancy
+ * with zeros; new_bins < bin_count returns HIBF_GPU_ERR_INVALID_ARGUMENT with
+ * the reference's message.  Device addresses from hibf_gpu_ibf_device_words
+ * are invalid afterwards; synchronous. */
+int hibf_gpu_ibf_try_increase_bin_number_to(hibf_gpu_ibf ibf, uint64_t new_bins, int * increased);
+int hibf_gpu_ibf_increase_bin_number_to(hibf_gpu_ibf ibf, uint64_t new_bins);
+
 /* Occupancy kept with the handle (interleaved_bloom_filter.hpp:451-458):
  * technical_bins counters and the track_occupancy flag.  The device emplace
  * does not update them (track_occupancy is order-dependent); clear() resets

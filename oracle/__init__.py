@@ -84,6 +84,8 @@ def lib():
         L.orc_ibf_emplace.argtypes = [ctypes.c_void_p, ctypes.POINTER(OrcIbf), ctypes.c_void_p, ctypes.c_void_p,
                                       ctypes.c_uint64]
         L.orc_ibf_clear.argtypes = [ctypes.c_void_p, ctypes.POINTER(OrcIbf), ctypes.c_uint64]
+        L.orc_ibf_increase_bin_number_to.argtypes = [ctypes.c_void_p, ctypes.POINTER(OrcIbf), ctypes.c_uint64]
+        L.orc_ibf_increase_bin_number_to.restype = ctypes.c_int
         L.orc_ibf_bulk_contains.argtypes = [ctypes.c_void_p, ctypes.POINTER(OrcIbf), ctypes.c_uint64,
                                             ctypes.c_void_p]
         L.orc_counting_vector_add.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_uint64]
@@ -168,6 +170,14 @@ class Ibf:
 
     def clear(self, bin: int) -> None:
         lib().orc_ibf_clear(_p(self.words), ctypes.byref(self.g), bin)
+
+    def increase_bin_number_to(self, new_bins: int) -> None:
+        """interleaved_bloom_filter::increase_bin_number_to (re-strides in place)."""
+        new_words = (new_bins + 63) // 64 * self.g.bin_size
+        if new_words > self.words.size:
+            self.words = np.concatenate([self.words, np.zeros(new_words - self.words.size, dtype=np.uint64)])
+        if lib().orc_ibf_increase_bin_number_to(_p(self.words), ctypes.byref(self.g), new_bins):
+            raise ValueError("The number of new bins must be >= the current number of bins.")
 
     def bulk_contains(self, value: int) -> np.ndarray:
         out = np.zeros(self.g.bin_words, dtype=np.uint64)

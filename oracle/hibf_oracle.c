@@ -127,6 +127,34 @@ void orc_ibf_clear(uint64_t * words, orc_ibf const * g, uint64_t bin)
         words[idx >> 6] &= ~(1ULL << (idx & 63u));
 }
 
+/* src/interleaved_bloom_filter.cpp:150-201: increase_bin_number_to.  When the
+ * words per row stay the same only `bins` changes (try_increase_bin_number_to).
+ * Otherwise every row moves, last row first and last word first, from its
+ * old stride to the new one in place; each moved word's old place is zeroed,
+ * so the new columns end up zero.  `words` must hold the new matrix size with
+ * the old matrix in front and zeros behind it.  Returns 1 when new_bins is
+ * below bins (the reference throws std::invalid_argument). */
+int orc_ibf_increase_bin_number_to(uint64_t * words, orc_ibf * g, uint64_t new_bins)
+{
+    if (new_bins < g->bins)
+        return 1;
+    uint64_t const old_bw = g->bin_words;
+    uint64_t const new_bw = (new_bins + 63u) / 64u;
+    if (new_bw > old_bw)
+    {
+        for (uint64_t r = g->bin_size; r-- > 1u;)
+            for (uint64_t w = old_bw; w-- > 0u;)
+            {
+                words[r * new_bw + w] = words[r * old_bw + w];
+                words[r * old_bw + w] = 0u;
+            }
+        g->bin_words = new_bw;
+        g->technical_bins = new_bw * 64u;
+    }
+    g->bins = new_bins;
+    return 0;
+}
+
 /* src/interleaved_bloom_filter.cpp:204-291: copy row 0, AND rows 1..h-1.
  * out receives bin_words words. */
 void orc_ibf_bulk_contains(uint64_t const * words, orc_ibf const * g, uint64_t value, uint64_t * out)

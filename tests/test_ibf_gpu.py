@@ -219,3 +219,27 @@ def test_snippet_goldens_on_gpu(goldens):
     case = goldens["insert_iterator_ibf"]
     v, o = csr([np.arange(*case["query_range"], dtype=np.uint64)])
     assert to_numpy(build(case).bulk_count(v, o, 1))[0].tolist() == case["expected"]
+
+
+@pytest.mark.parametrize("bins,bin_size,h,new_bins", [(70, 1000, 2, 130), (64, 333, 3, 65), (3, 500, 2, 64),
+                                                      (130, 777, 2, 1000), (8192, 64, 3, 8193), (4000, 2000, 2, 9000)])
+def test_increase_bin_number_on_device(bins, bin_size, h, new_bins):
+    """Device re-stride equals the oracle's in-place re-stride and a fresh
+    build with the new bin count; counts afterwards equal the oracle's."""
+    from hibf_amd.device import DeviceIbf, csr, to_numpy
+    _say(f"increase bins {bins} -> {new_bins}")
+    ref, dev, vals, bidx, rng = _pair(bins, bin_size, h, 5000, bins + new_bins)
+    same_words = (new_bins + 63) // 64 == (bins + 63) // 64
+    assert dev.try_increase_bin_number_to(new_bins) == same_words
+    dev.increase_bin_number_to(new_bins)
+    ref.increase_bin_number_to(new_bins)
+    fresh = DeviceIbf(new_bins, bin_size, h)
+    fresh.emplace(vals, bidx)
+    assert dev.bin_count() == new_bins and dev.geometry.bin_words == (new_bins + 63) // 64
+    assert np.array_equal(dev.download(), ref.words)
+    assert np.array_equal(dev.download(), fresh.download())
+    qs = _queries(rng, vals, [0, 1, 100, 250])
+    v, o = csr(qs)
+    assert np.array_equal(to_numpy(dev.bulk_count(v, o, 2)), ref.bulk_count_batch(v, o, 2))
+    with pytest.raises(Exception, match="must be >= the current number of bins"):
+        dev.increase_bin_number_to(new_bins - 1)

@@ -306,3 +306,23 @@ def test_hibf_oracle_dfs_order_and_deleted_carry():
     q = np.concatenate([ub[5][:4], ub[4][:2]])
     assert hb.membership_for(q, 6) == [4]
     assert hb.bulk_count(q, 6, 2).tolist() == [0, 0, 0, 0, 6]
+
+
+@pytest.mark.parametrize("bins,bin_size,h,new_bins", [(70, 1000, 2, 130), (64, 333, 3, 65), (3, 500, 2, 64),
+                                                      (130, 777, 2, 1000), (8192, 64, 3, 8193), (5, 100, 1, 5)])
+def test_oracle_increase_bin_number_closed_form(bins, bin_size, h, new_bins):
+    """increase_bin_number_to(n) after emplacing equals an IBF built with n
+    bins and the same emplaces: rows move, a bin's bits do not."""
+    rng = np.random.default_rng(bins + new_bins)
+    vals = rng.integers(0, 2**63, size=3000, dtype=np.int64).astype(np.uint64)
+    bidx = rng.integers(0, bins, size=vals.size).astype(np.uint64)
+    grown = O.Ibf(bins, bin_size, h)
+    grown.emplace(vals, bidx)
+    grown.increase_bin_number_to(new_bins)
+    fresh = O.Ibf(new_bins, bin_size, h)
+    fresh.emplace(vals, bidx)
+    assert (grown.g.bins, grown.g.bin_words, grown.g.technical_bins) == (new_bins, fresh.g.bin_words,
+                                                                         fresh.g.technical_bins)
+    assert np.array_equal(grown.words, fresh.words)
+    with pytest.raises(ValueError):
+        grown.increase_bin_number_to(new_bins - 1 if new_bins > 1 else 0)
