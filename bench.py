@@ -39,6 +39,9 @@ def parse():
                          "(fixed-length batch, log-normal batched hipGraph form, plain per-query form)")
     ap.add_argument("--hibf-queries", type=int, default=500_000)
     ap.add_argument("--hibf-batched-queries", type=int, default=2_000_000)
+    ap.add_argument("--hibf-count-queries", type=int, default=50_000,
+                    help="queries of (a) for the HIBF bulk_count leg (uint16 counts of every user bin, t = 0.7 L; "
+                         "0 skips it)")
     ap.add_argument("--graph-batch", type=int, default=65536)
     ap.add_argument("--hibf-user-bins", type=int, default=32768)
     ap.add_argument("--steps", type=int, default=5)
@@ -454,6 +457,31 @@ def run_hibf(args):
     log(f"(b) {nb} log-normal queries ({int(ob_h[-1]) / nb:.0f} values/query), {nbatches} graphs: "
         f"{wall_b * 1e3:.1f} ms/step, {lookups_b / wall_b / 1e9:.3f} G lookups/s, {bytes_b / wall_b / 1e9:.0f} GB/s rows")
 
+    # (d) counting agent: uint16 bulk_count over every user bin for a prefix of (a), t = 0.7 L as in (a).
+    count_leg = None
+    ncq = min(args.hibf_count_queries, nq)
+    if ncq > 0:
+        thr_c = torch.from_numpy(ta_h[:ncq].astype(np.int64)).to(dev).view(torch.uint64)
+        counts_c = torch.empty((ncq, n_ub), dtype=torch.uint16, device=dev)
+        hibf.bulk_count_into(agent, va, oa[: ncq + 1], thr_c, counts_c, 2)
+        torch.cuda.synchronize()
+        agent.stats(reset=True)
+        cev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(3)]
+        for a, b in cev:
+            a.record(stream)
+            hibf.bulk_count_into(agent, va, oa[: ncq + 1], thr_c, counts_c, 2)
+            b.record(stream)
+        torch.cuda.synchronize()
+        st = agent.stats(reset=True)
+        c_ms = float(np.mean([a.elapsed_time(b) for a, b in cev]))
+        out_bytes = ncq * n_ub * 2
+        count_leg = {"queries": ncq, "threshold": "0.7 L", "kernel_ms": c_ms,
+                     "lookups_per_s": st["lookups"] / 3 / (c_ms / 1e3),
+                     "row_GBs": st["row_bytes"] / 3 / (c_ms / 1e3) / 1e9,
+                     "counter_GBs": out_bytes / (c_ms / 1e3) / 1e9}
+        log(f"(d) bulk_count {ncq} x 250 over {n_ub} user bins: {c_ms:.2f} ms, "
+            f"{count_leg['lookups_per_s'] / 1e9:.3f} G lookups/s, {count_leg['counter_GBs']:.0f} GB/s of counters")
+
     # (c) plain form: one launch per query of (b), back to back.
     npl = min(args.plain_queries, nb)
     hibf.membership_prepare(agent, 1)
@@ -501,6 +529,17 @@ def run_hibf(args):
                     ok = False
                     log(f"parity mismatch at query {q}: {got} vs {want}")
                     break
+        if count_leg is not None and ok:
+            got_c = counts_c.view(torch.int16).cpu().numpy().view(np.uint16)
+            pick = rng.choice(ncq, size=min(args.check_queries, ncq), replace=False)
+            for q in pick:
+                q = int(q)
+                want = ref.bulk_count(va_h[int(oa_h[q]):int(oa_h[q + 1])], int(ta_h[q]), 2)
+                if not np.array_equal(got_c[q], want):
+                    ok = False
+                    log(f"bulk_count mismatch at query {q}")
+                    break
+            count_leg["parity_sample_ok"] = ok
         parity = ok
         log(f"parity vs oracle on {2 * args.check_queries} sampled queries: {parity}")
         if not args.skip_cpu:
@@ -553,6 +592,7 @@ def run_hibf(args):
         "fixed_batch": {"queries": nq, "length": 250, "ms": wall_a * 1e3, "lookups_per_s": lookups_a / wall_a,
                         "row_gbs": bytes_a / kern_a / 1e9, "kernel_ms": kern_a * 1e3,
                         "traffic": recorded_traffic("hibf_fixed_batch", args)},
+        "bulk_count": count_leg,
         "plain_per_query": {"value": plain, "unit": "lookups/s", "queries": npl,
                             "latency_ms_p50": float(np.percentile(lat, 50)),
                             "latency_ms_p99": float(np.percentile(lat, 99))},

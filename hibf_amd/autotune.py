@@ -93,6 +93,11 @@ def _tune(space: dict, run: Callable[[], None], output: Callable[[], object], sa
     saved = {k: native.tune_get(k) for k in space}
     results = []
     reference = None
+    # Warm the device (TLB, caches, clocks) on the current shape before the first
+    # candidate is timed, so that the first candidate is not charged for it.
+    for _ in range(3):
+        run()
+    torch.cuda.synchronize()
     try:
         for knobs in _candidates(space):
             _apply(knobs)

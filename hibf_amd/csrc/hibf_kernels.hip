@@ -172,9 +172,22 @@ __device__ __forceinline__ void descent_body(HibfLaunch const & h,
                                     : nullptr;
         if constexpr (COUNTING)
         {
-            // counting_vector fill(0) (hierarchical_interleaved_bloom_filter.hpp:561)
+            // counting_vector fill(0) (hierarchical_interleaved_bloom_filter.hpp:561): 16-byte
+            // stores over the 16-byte-aligned middle of the row, counter stores at its ends.
+            char * const bytes = static_cast<char *>(row);
+            uint64_t const len = h.n_user_bins * cb;
+            uint64_t const head = (16u - (reinterpret_cast<uintptr_t>(bytes) & 15u)) & 15u;
+            uint64_t const n16 = len > head ? (len - head) / 16u : 0u;
+            uint64_t const tail = head + n16 * 16u;
+            uint4 * const mid = reinterpret_cast<uint4 *>(bytes + head);
+            for (uint64_t i = static_cast<uint64_t>(tid); i < n16; i += static_cast<uint64_t>(nthreads))
+                mid[i] = make_uint4(0u, 0u, 0u, 0u);
             for (uint64_t i = static_cast<uint64_t>(tid); i < h.n_user_bins; i += static_cast<uint64_t>(nthreads))
-                store_counter(row, i, cb, 0ULL);
+            {
+                uint64_t const at = i * cb;
+                if (at < head || at >= tail)
+                    store_counter(row, i, cb, 0ULL);
+            }
         }
         if (tid == 0)
             st.hits = 0;

@@ -424,6 +424,18 @@ class DeviceHibf:
         stats = {k: getattr(st, k) for k, _ in native.ServeStats._fields_}
         return lat, ids, cnt, stats
 
+    def bulk_count_into(self, agent: Agent, values: torch.Tensor, offsets: torch.Tensor, thresholds: torch.Tensor,
+                        out: torch.Tensor, counter_bytes: int = 2) -> None:
+        """HIBF bulk_count into a preallocated (nq, number_of_user_bins) device
+        tensor; thresholds is a uint64 device tensor of one entry per query."""
+        nq = offsets.numel() - 1
+        if out.numel() < nq * self.number_of_user_bins or out.element_size() != counter_bytes:
+            raise ValueError("out must hold nq x number_of_user_bins counters of counter_bytes bytes")
+        if thresholds.numel() < nq or thresholds.element_size() != 8:
+            raise ValueError("thresholds must be a 64-bit tensor of one entry per query")
+        check(lib().hibf_gpu_hibf_bulk_count(self.h, agent.h, _ptr(values), _ptr(offsets), nq, _ptr(thresholds),
+                                             counter_bytes, _ptr(out), _stream()))
+
     def bulk_count(self, values, offsets, thresholds, counter_bytes: int = 2, agent: Agent | None = None):
         v = as_device_u64(values)
         o = as_device_u64(offsets)
