@@ -40,6 +40,8 @@ def as_device_u64(x) -> torch.Tensor:
             t = t.to(torch.int64).view(torch.uint64) if t.dtype != torch.int64 else t.view(torch.uint64)
         return t.to(_dev()).contiguous()
     a = np.ascontiguousarray(np.asarray(x, dtype=np.uint64))
+    if not a.flags.writeable:
+        a = a.copy()
     return torch.from_numpy(a.view(np.int64)).to(_dev()).view(torch.uint64)
 
 

@@ -27,6 +27,7 @@ ERR_UNSUPPORTED = 8
 ERR_CONFIG = 9
 ERR_BUDGET = 10
 ERR_NO_DEVICE = 11
+ERR_IO = 12
 
 BIN_MERGED = 0xFFFFFFFFFFFFFFFF
 BIN_DELETED = 0xFFFFFFFFFFFFFFFE
@@ -153,6 +154,12 @@ def lib() -> ctypes.CDLL:
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m hibf_amd.build`")
+        # torch's wheel carries its own HIP runtime (libamdhip64.so, soname
+        # libamdhip64.so.7).  Loading torch first makes this library's
+        # libamdhip64.so.7 dependency resolve to that same runtime; loaded the
+        # other way round, the process would hold two HIP/HSA runtimes and the
+        # second to initialise finds no device.
+        import torch  # noqa: F401
         L = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in _SIGNATURES.items():
             fn = getattr(L, name)

@@ -275,3 +275,61 @@ def test_serving_large_staged_batches():
     batch = _membership_lists(dev, qs, ts)
     for i in range(len(qs)):
         assert int(cnt[i]) == len(batch[i]) and ids[i, :min(8, int(cnt[i]))].tolist() == batch[i][:8]
+
+
+def _flat_layout(n_ub, split0=1):
+    """Layout text with every user bin in the top-level IBF, user bin 0 split
+    over `split0` technical bins (src/layout/layout.cpp:125-136 format)."""
+    lines = ["#TOP_LEVEL_IBF fullest_technical_bin_idx:0",
+             "#USER_BIN_IDX\tTECHNICAL_BIN_INDICES\tNUMBER_OF_TECHNICAL_BINS",
+             f"0\t0\t{split0}"]
+    lines += [f"{u}\t{u + split0 - 1}\t1" for u in range(1, n_ub)]
+    return "\n".join(lines) + "\n"
+
+
+@pytest.mark.parametrize("n_ub", [3000, 5000])
+def test_wide_single_ibf_hibf(n_ub, descent_kernel):
+    """One IBF of 3,002 / 5,002 technical bins (47 / 79 words per row): the
+    widest counting shapes of the descent, with a split user bin."""
+    from hibf_amd.device import DeviceHibf, to_numpy
+    _say(f"wide single-IBF HIBF n_ub={n_ub} kernel={descent_kernel}")
+    rng = np.random.default_rng(n_ub)
+    ubs = _random_user_bins(rng, n_ub, 5, 60)
+    dev = DeviceHibf.build(ubs, layout_text=_flat_layout(n_ub, 3), tmax=8192)
+    t = dev.tables()
+    assert len(t["geoms"]) == 1 and t["geoms"][0][0] == n_ub + 2
+    ref = oracle_of(dev)
+    qs = _mixed_queries(rng, ubs, 80, 120) + [np.zeros(0, dtype=np.uint64)]
+    thresholds = [int(x) for x in rng.integers(1, 60, size=len(qs))]
+    got = _membership_lists(dev, qs, thresholds)
+    for q, th, g in zip(qs, thresholds, got):
+        assert g == ref.membership_for(q, th)
+    values, offsets = _csr(qs)
+    counts = to_numpy(dev.bulk_count(values, offsets, thresholds, 2))
+    assert np.array_equal(counts, ref.bulk_count_batch(values, offsets, thresholds, 2))
+
+
+def test_hibf_long_queries_wrap(descent_kernel):
+    """Queries longer than 65,535 values: the membership sums are uint16 and
+    the counting sums value_t, and both wrap as the reference's do
+    (hierarchical_interleaved_bloom_filter.hpp:329-354, :462-496)."""
+    from hibf_amd.device import DeviceHibf, to_numpy
+    _say(f"hibf long queries kernel={descent_kernel}")
+    rng = np.random.default_rng(77)
+    ubs = _random_user_bins(rng, 300, 20, 200)
+    dev = DeviceHibf.build(ubs, tmax=64)
+    ref = oracle_of(dev)
+    qs = [np.full(70000, ubs[5][0], dtype=np.uint64),
+          np.full(65536, ubs[7][1], dtype=np.uint64),
+          np.full(65535, ubs[9][2], dtype=np.uint64),
+          np.concatenate([ubs[3][rng.integers(0, ubs[3].size, size=66000)],
+                          rng.integers(0, 2**63, size=1000, dtype=np.int64).astype(np.uint64)])]
+    for th in (1, 100, 4465, 5000, 65535):
+        got = _membership_lists(dev, qs, [th] * len(qs))
+        for q, g in zip(qs, got):
+            assert g == ref.membership_for(q, th), th
+    values, offsets = _csr(qs)
+    for width in (1, 2, 4, 8):
+        for th in (1, 3000):
+            got = to_numpy(dev.bulk_count(values, offsets, [th] * len(qs), width))
+            assert np.array_equal(got, ref.bulk_count_batch(values, offsets, [th] * len(qs), width)), (width, th)
