@@ -15,6 +15,7 @@ oracle restatement on a bounded sample, on this host's cores).
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -39,6 +40,8 @@ def parse():
                          "(fixed-length batch, log-normal batched hipGraph form, plain per-query form)")
     ap.add_argument("--hibf-queries", type=int, default=500_000)
     ap.add_argument("--hibf-batched-queries", type=int, default=2_000_000)
+    ap.add_argument("--reads", type=int, default=200_000,
+                    help="250-base reads for the k-mer hashing leg of the HIBF workload (0 skips it)")
     ap.add_argument("--hibf-count-queries", type=int, default=50_000,
                     help="queries of (a) for the HIBF bulk_count leg (uint16 counts of every user bin, t = 0.7 L; "
                          "0 skips it)")
@@ -482,6 +485,34 @@ def run_hibf(args):
         log(f"(d) bulk_count {ncq} x 250 over {n_ub} user bins: {c_ms:.2f} ms, "
             f"{count_leg['lookups_per_s'] / 1e9:.3f} G lookups/s, {count_leg['counter_GBs']:.0f} GB/s of counters")
 
+    # (e) reads in: k-mer hashes (k = 19) of random 250-base reads, as the tutorial search feeds membership_for.
+    kmer_leg = None
+    if args.reads > 0:
+        from hibf_amd import native as N
+        rng_r = np.random.default_rng(args.seed + 21)
+        nr, rl, k = args.reads, 250, 19
+        seq = torch.from_numpy(np.frombuffer(b"ACGT", dtype=np.uint8)[rng_r.integers(0, 4, size=nr * rl)].copy()).to(dev)
+        soff = torch.arange(0, (nr + 1) * rl, rl, dtype=torch.int64, device=dev).view(torch.uint64)
+        hoff = torch.empty(nr + 1, dtype=torch.uint64, device=dev)
+        hashes_r = torch.empty(nr * (rl - k + 1), dtype=torch.uint64, device=dev)
+        total = ctypes.c_uint64()
+
+        def hash_reads():
+            N.check(N.lib().hibf_gpu_kmer_hash(ctypes.c_void_p(seq.data_ptr()), ctypes.c_void_p(soff.data_ptr()), nr, k,
+                                               ctypes.c_void_p(hoff.data_ptr()), ctypes.c_void_p(hashes_r.data_ptr()),
+                                               hashes_r.numel(), ctypes.byref(total),
+                                               ctypes.c_void_p(stream.cuda_stream)))
+        hash_reads()
+        t1 = time.perf_counter()
+        for _ in range(3):
+            hash_reads()
+        r_s = (time.perf_counter() - t1) / 3
+        kmer_leg = {"reads": nr, "read_length": rl, "k": k, "hashes": int(total.value), "seconds": r_s,
+                    "hashes_per_s": total.value / r_s, "GBs": (nr * rl + total.value * 8) / r_s / 1e9}
+        log(f"(e) k-mer hashes of {nr} x {rl} bases (k = {k}): {r_s * 1e3:.2f} ms per call incl. offsets and sync, "
+            f"{kmer_leg['hashes_per_s'] / 1e9:.2f} G hashes/s")
+        del seq, soff, hoff, hashes_r
+
     # (c) plain form: one launch per query of (b), back to back.
     npl = min(args.plain_queries, nb)
     hibf.membership_prepare(agent, 1)
@@ -593,6 +624,7 @@ def run_hibf(args):
                         "row_gbs": bytes_a / kern_a / 1e9, "kernel_ms": kern_a * 1e3,
                         "traffic": recorded_traffic("hibf_fixed_batch", args)},
         "bulk_count": count_leg,
+        "kmer_hash": kmer_leg,
         "plain_per_query": {"value": plain, "unit": "lookups/s", "queries": npl,
                             "latency_ms_p50": float(np.percentile(lat, 50)),
                             "latency_ms_p99": float(np.percentile(lat, 99))},

@@ -49,6 +49,33 @@ def to_numpy_u64(t: torch.Tensor) -> np.ndarray:
     return t.view(torch.int64).cpu().numpy().view(np.uint64)
 
 
+def reads_csr(reads) -> tuple[np.ndarray, np.ndarray]:
+    """(bytes, offsets) of a list of DNA reads given as str or bytes."""
+    parts = [r.encode() if isinstance(r, str) else bytes(r) for r in reads]
+    offsets = np.zeros(len(parts) + 1, dtype=np.uint64)
+    offsets[1:] = np.cumsum([len(p) for p in parts], dtype=np.uint64)
+    data = np.frombuffer(b"".join(parts), dtype=np.uint8) if parts and offsets[-1] else np.zeros(0, dtype=np.uint8)
+    return data, offsets
+
+
+def kmer_hash(reads, k: int) -> tuple[torch.Tensor, torch.Tensor]:
+    """Device k-mer hashes of DNA reads (hibf_gpu_kmer_hash): (hashes, offsets)
+    as uint64 device tensors in CSR form, one row of hashes per read."""
+    data, offsets = reads_csr(reads)
+    dev = _dev()
+    d_seq = torch.from_numpy(np.ascontiguousarray(data).copy()).to(dev) if data.size else \
+        torch.zeros(1, dtype=torch.uint8, device=dev)
+    d_off = as_device_u64(offsets)
+    n = offsets.size - 1
+    out_off = torch.empty(n + 1, dtype=torch.uint64, device=dev)
+    total = ctypes.c_uint64()
+    cap = max(int(offsets[-1]), 1)
+    hashes = torch.empty(cap, dtype=torch.uint64, device=dev)
+    check(lib().hibf_gpu_kmer_hash(_ptr(d_seq), _ptr(d_off), n, k, _ptr(out_off), _ptr(hashes), cap,
+                                   ctypes.byref(total), _stream()))
+    return hashes[: total.value], out_off
+
+
 def csr(queries) -> tuple[np.ndarray, np.ndarray]:
     """(values, offsets) for a list of uint64 sequences."""
     lens = [len(q) for q in queries]
@@ -382,6 +409,21 @@ class DeviceHibf:
         return _membership(lambda ids, cap, hoff, total: lib().hibf_gpu_hibf_membership_for(
             self.h, agent.h, _ptr(v), _ptr(o), nq, _ptr(t), _ptr(ids), cap, _ptr(hoff), ctypes.byref(total),
             _stream()), nq, o.device)
+
+    def membership_for_reads(self, reads, k: int, fraction: float = 0.7, agent: Agent | None = None):
+        """The tutorial search (test/tutorial/search.cpp:37-45) on the device:
+        each read's k-mer hashes are queried with threshold
+        uint16_t(fraction * number of k-mers).  Returns one id list per read,
+        in the reference's depth-first order."""
+        hashes, offsets = kmer_hash(reads, k)
+        n = offsets.numel() - 1
+        thr = torch.empty(max(n, 1), dtype=torch.uint16, device=offsets.device)
+        check(lib().hibf_gpu_kmer_thresholds(_ptr(offsets), n, float(fraction), _ptr(thr), _stream()))
+        if hashes.numel() == 0:
+            hashes = torch.zeros(1, dtype=torch.uint64, device=offsets.device)
+        ids, hoff = self.membership_for(hashes, offsets, thr[:n], agent)
+        ids, hoff = to_numpy(ids), to_numpy(hoff)
+        return [ids[int(hoff[i]):int(hoff[i + 1])].tolist() for i in range(n)]
 
     def membership_prepare(self, agent: Agent, max_queries: int) -> None:
         check(lib().hibf_gpu_hibf_membership_prepare(self.h, agent.h, max_queries))

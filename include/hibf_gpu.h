@@ -370,6 +370,38 @@ int hibf_gpu_hibf_export_tables(hibf_gpu_hibf hibf,
                                 uint64_t * user_bin);
 
 /* ======================================================================== */
+/* Reads in: k-mer hashes                                                   */
+/* ======================================================================== */
+
+/* k-mer hashes of DNA reads, the values the reference's tutorial search passes
+ * to membership_for (test/tutorial/search.cpp:37-41, there computed by
+ * seqan3::views::kmer_hash(seqan3::ungapped{k}) over dna4; no counterpart in
+ * the library itself).  Read i is d_seq[d_seq_offsets[i] .. d_seq_offsets[i+1])
+ * (ASCII).  The hash of a window is the base-4 number of its letters' dna4
+ * ranks, first letter most significant (A=0, C=1, G=2, T=U=3, any other
+ * character = A); a read shorter than k has none.  Writes n_seqs+1 CSR offsets
+ * to d_hash_offsets and the hashes to d_hashes; *h_total is the hash count,
+ * HIBF_GPU_ERR_CAPACITY when it exceeds hashes_cap (offsets are written).
+ * k in [1, 32].  Synchronous. */
+int hibf_gpu_kmer_hash(const uint8_t * d_seq,
+                       const uint64_t * d_seq_offsets,
+                       uint64_t n_seqs,
+                       uint32_t k,
+                       uint64_t * d_hash_offsets,
+                       uint64_t * d_hashes,
+                       uint64_t hashes_cap,
+                       uint64_t * h_total,
+                       void * stream);
+
+/* Per-read thresholds uint16_t(fraction * number of k-mers)
+ * (test/tutorial/search.cpp:42), truncated and capped at 65535.  Asynchronous. */
+int hibf_gpu_kmer_thresholds(const uint64_t * d_hash_offsets,
+                             uint64_t n_seqs,
+                             double fraction,
+                             uint16_t * d_thresholds,
+                             void * stream);
+
+/* ======================================================================== */
 /* Archives: cereal BinaryArchive layout                                    */
 /* ======================================================================== */
 

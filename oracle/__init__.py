@@ -284,3 +284,32 @@ def geom(bins: int, bin_size: int, hash_funs: int) -> OrcIbf:
     if rc:
         raise ValueError(f"invalid IBF geometry (code {rc})")
     return g
+
+
+_DNA4_RANK = np.zeros(256, dtype=np.uint64)
+for _c, _r in (("C", 1), ("G", 2), ("T", 3), ("U", 3)):
+    _DNA4_RANK[ord(_c)] = _r
+    _DNA4_RANK[ord(_c.lower())] = _r
+
+
+def kmer_hashes(seq, k: int) -> np.ndarray:
+    """k-mer hashes of one DNA read as seqan3::views::kmer_hash(ungapped{k})
+    over dna4 gives them to the reference's tutorial search
+    (test/tutorial/search.cpp:37-41): for each window, the base-4 number of
+    the dna4 ranks (A=0, C=1, G=2, T=U=3, anything else A), first letter most
+    significant.  Restated from that definition (seqan3 is not available)."""
+    b = np.frombuffer(seq.encode() if isinstance(seq, str) else bytes(seq), dtype=np.uint8)
+    n = b.size - k + 1
+    if n <= 0:
+        return np.zeros(0, dtype=np.uint64)
+    r = _DNA4_RANK[b]
+    h = np.zeros(n, dtype=np.uint64)
+    for j in range(k):
+        h = (h << np.uint64(2)) | r[j:j + n]
+    return h
+
+
+def kmer_threshold(n_kmers: int, fraction: float) -> int:
+    """uint16_t(fraction * number of k-mers) (test/tutorial/search.cpp:42)."""
+    t = fraction * n_kmers
+    return 65535 if t >= 65535 else max(0, int(t))

@@ -1,0 +1,71 @@
+"""GPU parity of the reads-in path: device k-mer hashes equal the oracle's
+restatement, and the tutorial search (k-mers of each read, threshold
+uint16_t(0.7 * #k-mers)) returns the oracle's ids."""
+import random
+
+import numpy as np
+import pytest
+
+import oracle as O
+from hibf_helpers import oracle_of
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def _reads(rng, n, lo, hi, alphabet="ACGT"):
+    return ["".join(rng.choice(alphabet) for _ in range(rng.randint(lo, hi))) for _ in range(n)]
+
+
+@pytest.mark.parametrize("k", [1, 12, 19, 31, 32])
+def test_kmer_hash_matches_oracle(k):
+    from hibf_amd.device import kmer_hash, to_numpy
+    _say(f"kmer hash k={k}")
+    rng = random.Random(k)
+    reads = _reads(rng, 300, 0, 400, "ACGTACGTACGTacgtNU") + ["", "A" * (k - 1), "C" * k]
+    hashes, offsets = kmer_hash(reads, k)
+    h, o = to_numpy(hashes), to_numpy(offsets)
+    assert o[0] == 0 and o.size == len(reads) + 1
+    for i, r in enumerate(reads):
+        assert h[int(o[i]):int(o[i + 1])].tolist() == O.kmer_hashes(r, k).tolist(), i
+
+
+def test_kmer_hash_rejects_bad_k():
+    from hibf_amd.device import kmer_hash
+    from hibf_amd.native import HibfGpuError
+    for k in (0, 33):
+        with pytest.raises(HibfGpuError):
+            kmer_hash(["ACGT"], k)
+
+
+def test_search_reads_like_the_tutorial():
+    """User bins are k-mer sets of random genomes; reads are sampled from them
+    (some with errors, some random); ids equal the oracle's membership_for on
+    the oracle's hashes with threshold uint16_t(0.7 * #k-mers)."""
+    from hibf_amd.device import DeviceHibf
+    _say("tutorial search on reads")
+    k = 19
+    rng = random.Random(7)
+    genomes = ["".join(rng.choice("ACGT") for _ in range(rng.randint(2000, 6000))) for _ in range(120)]
+    ubs = [np.unique(O.kmer_hashes(g, k)) for g in genomes]
+    dev = DeviceHibf.build(ubs, tmax=64)
+    ref = oracle_of(dev)
+    reads = []
+    for _ in range(400):
+        g = genomes[rng.randrange(len(genomes))]
+        start = rng.randrange(0, len(g) - 250)
+        read = list(g[start:start + rng.randint(100, 250)])
+        for _ in range(rng.randint(0, 3)):
+            read[rng.randrange(len(read))] = rng.choice("ACGTN")
+        reads.append("".join(read))
+    reads += _reads(rng, 50, 0, 200) + ["", "ACGT"]
+    got = dev.membership_for_reads(reads, k, 0.7)
+    found = 0
+    for r, g in zip(reads, got):
+        hashes = O.kmer_hashes(r, k)
+        assert g == ref.membership_for(hashes, O.kmer_threshold(hashes.size, 0.7))
+        found += bool(g)
+    assert found >= 380

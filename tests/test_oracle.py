@@ -326,3 +326,35 @@ def test_oracle_increase_bin_number_closed_form(bins, bin_size, h, new_bins):
     assert np.array_equal(grown.words, fresh.words)
     with pytest.raises(ValueError):
         grown.increase_bin_number_to(new_bins - 1 if new_bins > 1 else 0)
+
+
+def _kmer_brute(seq: str, k: int) -> list[int]:
+    rank = {"A": 0, "C": 1, "G": 2, "T": 3, "U": 3}
+    out = []
+    for p in range(len(seq) - k + 1):
+        h = 0
+        for ch in seq[p:p + k]:
+            h = h * 4 + rank.get(ch.upper(), 0)
+        out.append(h)
+    return out
+
+
+def test_kmer_hashes_hand_examples():
+    # base-4 numbers of the dna4 ranks, first letter most significant
+    assert O.kmer_hashes("ACGT", 2).tolist() == [1, 6, 11]
+    assert O.kmer_hashes("ACGTN", 3).tolist() == [6, 27, 44]       # N reads as A
+    assert O.kmer_hashes("acgu", 4).tolist() == [0b00011011]        # lower case, U = T
+    assert O.kmer_hashes("TTTT", 4).tolist() == [255]
+    assert O.kmer_hashes("AC", 3).tolist() == []                   # shorter than k
+    assert O.kmer_hashes("T" * 32, 32).tolist() == [2**64 - 1]     # k = 32 fills 64 bits
+    assert O.kmer_threshold(30, 0.7) == 21 and O.kmer_threshold(0, 0.7) == 0
+    assert O.kmer_threshold(10**6, 0.7) == 65535
+
+
+@pytest.mark.parametrize("k", [1, 5, 19, 31, 32])
+def test_kmer_hashes_brute_force(k):
+    rng = random.Random(k)
+    for _ in range(20):
+        n = rng.randint(0, 80)
+        seq = "".join(rng.choice("ACGTacgtNnUx") for _ in range(n))
+        assert O.kmer_hashes(seq, k).tolist() == _kmer_brute(seq, k)
