@@ -105,7 +105,10 @@ __device__ __forceinline__ void count_narrow(IbfView const & g,
     }
 }
 
-template <bool NARROW>
+// HF == 2 (every IBF with two hash functions, rows of at most 4 words):
+// lanes share a value's row (count_into_lds_dispatch_narrow), so a row is one
+// load instruction's worth of lanes and no line is fetched twice.
+template <bool NARROW, int HF = 0>
 __device__ __forceinline__ void count_visit(HibfIbfDesc const & d,
                                             uint64_t const * arena,
                                             uint64_t const * __restrict__ vals,
@@ -114,6 +117,11 @@ __device__ __forceinline__ void count_visit(HibfIbfDesc const & d,
                                             int lane)
 {
     IbfView const g{arena + d.word_offset, d.bin_size, d.bins, d.bin_words, d.hash_shift, d.hash_funs};
+    if constexpr (NARROW && HF == 2)
+    {
+        count_into_lds_dispatch_narrow<2, false, 2>(d.lpv_log, g, vals, L, cnt, 0, 1, lane);
+        return;
+    }
     switch (d.bin_words)
     {
     case 1: count_narrow<1, 2>(g, vals, L, cnt, lane); break;
@@ -193,8 +201,8 @@ __device__ __forceinline__ uint32_t scan_wave(HibfIbfDesc const & d,
 }
 
 // NARROW: every IBF of the HIBF has at most 4 words per row (checked on the host).
-template <bool COUNTING, bool NARROW>
-__global__ void __launch_bounds__(256) hibf_wave_kernel(HibfLaunch h,
+template <bool COUNTING, bool NARROW, int HF = 0>
+__device__ __forceinline__ void wave_body(HibfLaunch const & h,
                                                         uint64_t const * __restrict__ values,
                                                         uint64_t const * __restrict__ offsets,
                                                         uint64_t nq,
@@ -254,7 +262,7 @@ __global__ void __launch_bounds__(256) hibf_wave_kernel(HibfLaunch h,
                 for (uint32_t i = static_cast<uint32_t>(lane); i < d.bins; i += 64u)
                     cnt[i] = 0u;
                 wave_sync();
-                count_visit<NARROW>(d, h.arena, vals, L, cnt, lane);
+                count_visit<NARROW, HF>(d, h.arena, vals, L, cnt, lane);
                 ++visits;
                 lookups += L;
                 row_bytes += static_cast<uint64_t>(L) * d.hash_funs * d.bin_words * 8u;
@@ -344,6 +352,29 @@ __global__ void __launch_bounds__(256) hibf_wave_kernel(HibfLaunch h,
     }
 }
 
+#define HIBF_WAVE_PARAMS                                                                                             \
+    HibfLaunch h, uint64_t const * __restrict__ values, uint64_t const * __restrict__ offsets, uint64_t nq,            \
+        uint16_t const * __restrict__ thr16, uint64_t const * __restrict__ thr64, uint32_t counter_bytes,               \
+        void * __restrict__ counts, uint64_t * __restrict__ ids, uint64_t ids_cap, uint64_t * __restrict__ hit_start,  \
+        uint64_t * __restrict__ hit_count, uint64_t * __restrict__ overflow
+#define HIBF_WAVE_ARGS                                                                                               \
+    h, values, offsets, nq, thr16, thr64, counter_bytes, counts, ids, ids_cap, hit_start, hit_count, overflow
+
+template <bool COUNTING, bool NARROW>
+__global__ void __launch_bounds__(256) hibf_wave_kernel(HIBF_WAVE_PARAMS)
+{
+    wave_body<COUNTING, NARROW>(HIBF_WAVE_ARGS);
+}
+
+// Membership with two hash functions over rows of at most 4 words.
+__global__ void __launch_bounds__(256) hibf_wave_membership_h2_w4_kernel(HIBF_WAVE_PARAMS)
+{
+    wave_body<false, true, 2>(HIBF_WAVE_ARGS);
+}
+
+#undef HIBF_WAVE_PARAMS
+#undef HIBF_WAVE_ARGS
+
 } // namespace
 
 size_t hibf_wave_lds_bytes(HibfLaunch const & h, unsigned waves_per_block)
@@ -367,7 +398,11 @@ hipError_t launch_hibf_wave_membership(HibfLaunch const & h,
 {
     if (nq == 0)
         return hipSuccess;
-    if (h.max_bin_words <= 4u)
+    if (h.max_bin_words <= 4u && h.uniform_hash_funs == 2u)
+        hipLaunchKernelGGL(hibf_wave_membership_h2_w4_kernel, dim3(blocks), dim3(256), hibf_wave_lds_bytes(h, 4),
+                           stream, h, values, offsets, nq, thresholds, nullptr, 2u, nullptr, ids, ids_cap, hit_start,
+                           hit_count, overflow);
+    else if (h.max_bin_words <= 4u)
         hipLaunchKernelGGL((hibf_wave_kernel<false, true>), dim3(blocks), dim3(256), hibf_wave_lds_bytes(h, 4), stream,
                            h, values, offsets, nq, thresholds, nullptr, 2u, nullptr, ids, ids_cap, hit_start,
                            hit_count, overflow);
