@@ -18,6 +18,19 @@ from .native import Geometry, check, lib
 _DTYPES = {1: torch.uint8, 2: torch.uint16, 4: torch.uint32, 8: torch.uint64}
 
 
+def _release(obj, destroy: str) -> None:
+    """Frees a handle from __del__; at interpreter exit the binding may already
+    be torn down, and the process's end releases the device memory anyway."""
+    h = getattr(obj, "h", None)
+    if not h:
+        return
+    obj.h = None
+    try:
+        getattr(lib(), destroy)(h)
+    except Exception:
+        pass
+
+
 def _dev() -> torch.device:
     if not torch.cuda.is_available():
         raise RuntimeError("hibf_amd needs a HIP device; none is visible")
@@ -107,9 +120,7 @@ class Agent:
         return int(t.value), bool(d.value)
 
     def __del__(self):
-        if getattr(self, "h", None):
-            lib().hibf_gpu_agent_destroy(self.h)
-            self.h = None
+        _release(self, "hibf_gpu_agent_destroy")
 
 
 class DeviceIbf:
@@ -126,9 +137,7 @@ class DeviceIbf:
         self._agent = None
 
     def __del__(self):
-        if getattr(self, "h", None):
-            lib().hibf_gpu_ibf_destroy(self.h)
-            self.h = None
+        _release(self, "hibf_gpu_ibf_destroy")
 
     @classmethod
     def load(cls, path: str, offset: int = 0) -> tuple["DeviceIbf", int]:
@@ -380,9 +389,7 @@ class DeviceHibf:
                 "depth": depth.value}
 
     def __del__(self):
-        if getattr(self, "h", None):
-            lib().hibf_gpu_hibf_destroy(self.h)
-            self.h = None
+        _release(self, "hibf_gpu_hibf_destroy")
 
     def emplace(self, ibf_idx: int, values, bins) -> None:
         v = as_device_u64(values)
