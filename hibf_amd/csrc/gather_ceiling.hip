@@ -9,11 +9,14 @@
 // IBFs); 1 KiB rows bound the flat count kernel (8,192 bins).
 //
 // Usage: gather_ceiling [GiB]      one JSON line per row size
+//        gather_ceiling --window      32-byte rows over buffers of 64 MiB to
+//                                     16 GiB: the rate inside the caches
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
+#include <string>
 
 namespace
 {
@@ -102,6 +105,25 @@ void measure(uint4 const * buf, uint64_t bytes, uint64_t * sink, unsigned blocks
 
 int main(int argc, char ** argv)
 {
+    if (argc > 1 && std::string(argv[1]) == "--window")
+    {
+        uint64_t const max_bytes = 16ull << 30;
+        uint4 * buf = nullptr;
+        uint64_t * sink = nullptr;
+        unsigned const blocks = 4096;
+        CHECK(hipMalloc(&buf, max_bytes));
+        CHECK(hipMalloc(&sink, static_cast<uint64_t>(blocks) * 256u * sizeof(uint64_t)));
+        CHECK(hipMemset(buf, 0x5a, max_bytes));
+        CHECK(hipDeviceSynchronize());
+        for (uint64_t mib : {64ull, 128ull, 256ull, 512ull, 1024ull, 4096ull, 16384ull})
+        {
+            std::printf("{\"window_MiB\": %llu}\n", static_cast<unsigned long long>(mib));
+            measure<32, 8>(buf, mib << 20, sink, blocks, 64);
+        }
+        CHECK(hipFree(buf));
+        CHECK(hipFree(sink));
+        return 0;
+    }
     uint64_t const gib = argc > 1 ? std::strtoull(argv[1], nullptr, 10) : 16;
     if (gib < 1 || gib > 128)
     {
