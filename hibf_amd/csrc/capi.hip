@@ -226,7 +226,7 @@ int hibf_gpu_tune_set(char const * key, int64_t value)
     else if (k == "count.counter_bits" && (value == 8 || value == 16))
         g_count_tuning.counter_bits = static_cast<uint32_t>(value);
     else if (k == "hibf.row_per_lane" && (value == 0 || value == 1))
-        g_count_tuning.hibf_narrow = static_cast<uint32_t>(value);
+        g_count_tuning.row_per_lane = static_cast<uint32_t>(value);
     else if (k == "hibf.sliced_counts" && (value == 0 || value == 1))
         g_count_tuning.sliced_counts = static_cast<uint32_t>(value);
     else
@@ -252,7 +252,7 @@ int hibf_gpu_tune_get(char const * key, int64_t * value)
     else if (k == "count.counter_bits")
         *value = g_count_tuning.counter_bits;
     else if (k == "hibf.row_per_lane")
-        *value = g_count_tuning.hibf_narrow;
+        *value = g_count_tuning.row_per_lane;
     else if (k == "hibf.sliced_counts")
         *value = g_count_tuning.sliced_counts;
     else

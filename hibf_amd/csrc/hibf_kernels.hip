@@ -317,7 +317,7 @@ hibf_descent_membership_h2_kernel(HIBF_DESCENT_PARAMS)
 
 // Same, for HIBFs whose rows are at most 4 words: one value and its whole row per lane.
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8)))
-hibf_descent_membership_h2_narrow_kernel(HIBF_DESCENT_PARAMS)
+hibf_descent_membership_h2_row_per_lane_kernel(HIBF_DESCENT_PARAMS)
 {
     descent_body<false, false, 2, 1, 2>(HIBF_DESCENT_ARGS);
 }
@@ -461,7 +461,7 @@ hipError_t launch_hibf_membership(HibfLaunch const & h,
     bool const h2 = h.uniform_hash_funs == 2u;
     bool const w4 = h2 && h.max_bin_words <= 4u;
     auto const kernel = g_count_tuning.nt_loads                ? hibf_descent_kernel<false, true, 0>
-                        : w4 && g_count_tuning.hibf_narrow     ? hibf_descent_membership_h2_narrow_kernel
+                        : w4 && g_count_tuning.row_per_lane     ? hibf_descent_membership_h2_row_per_lane_kernel
                         : w4 && g_count_tuning.sliced_counts   ? hibf_descent_membership_h2_w4_kernel<true>
                         : w4                                   ? hibf_descent_membership_h2_w4_kernel<false>
                         : h2                                   ? hibf_descent_membership_h2_kernel

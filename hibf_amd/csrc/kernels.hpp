@@ -50,7 +50,7 @@ struct CountTuning
     uint32_t hibf_wave = 0;   // HIBF descent: one wave per query (1) or one block per query (0)
     uint32_t nt_loads = 0;    // HIBF descent: non-temporal row loads
     uint32_t counter_bits = 16; // count_wide_kernel LDS counter width (8 or 16)
-    uint32_t hibf_narrow = 0;   // HIBF descent: one value and its whole row per lane when every row is <= 4 words
+    uint32_t row_per_lane = 0;  // HIBF descent: one value and its whole row per lane when every row is <= 4 words
     uint32_t sliced_counts = 1; // HIBF descent over rows <= 4 words: bit-sliced per-lane counts
 };
 extern CountTuning g_count_tuning;
