@@ -63,6 +63,8 @@ def parse():
     ap.add_argument("--check-queries", type=int, default=256)
     ap.add_argument("--contains-values", type=int, default=1 << 20,
                     help="values of the bulk_contains leg of the flat workload (0 skips it)")
+    ap.add_argument("--membership-queries", type=int, default=1_000_000,
+                    help="queries of the membership_for leg of the flat workload, t = 0.7 L (0 skips it)")
     ap.add_argument("--tune", action="append", default=[], metavar="KEY=VALUE",
                     help="launch-shape knob passed to hibf_gpu_tune_set (repeatable)")
     ap.add_argument("--no-autotune", action="store_true",
@@ -136,7 +138,7 @@ def main():
 
 def run_flat(args):
     import torch
-    from hibf_amd.device import DeviceIbf, as_device_u64
+    from hibf_amd.device import Agent, DeviceIbf, as_device_u64, to_numpy
     from hibf_amd.synth import flat_ibf_queries
 
     from hibf_amd import native
@@ -244,6 +246,28 @@ def run_flat(args):
         log(f"bulk_contains: {nc} values in {c_ms:.3f} ms, {contains['value'] / 1e6:.1f} M values/s, "
             f"{contains['achieved_GBs']:.0f} GB/s")
 
+    # membership_for over the whole batch at t = 0.7 L: count, threshold pass, scan, compaction (per chunk of
+    # queries whose uint16 counts fit the agent's 256 MiB buffer).  Wall clock: the call returns the hit total.
+    member = None
+    nm = min(args.membership_queries, nq)
+    if nm > 0:
+        m_thr = int(0.7 * L)
+        m_off = offsets[: nm + 1]
+        agent = Agent()
+        ibf.membership_for(values, m_off, m_thr, agent)
+        m_s = []
+        for _ in range(2):
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            m_ids, m_hoff = ibf.membership_for(values, m_off, m_thr, agent)
+            torch.cuda.synchronize()
+            m_s.append(time.perf_counter() - t1)
+        m_best = min(m_s)
+        member = {"value": nm * L / m_best, "unit": "lookups/s", "queries": nm, "threshold": m_thr,
+                  "seconds": m_best, "hits": int(m_ids.numel()), "vs_bulk_count": (nm * L / m_best) / value}
+        log(f"membership_for: {nm} queries in {m_best * 1e3:.1f} ms, {member['value'] / 1e9:.3f} G lookups/s, "
+            f"{member['hits']} hits")
+
     if args.calibrate_copy:
         a = torch.ones(1 << 27, dtype=torch.int64, device=dev)
         b = torch.empty_like(a)
@@ -275,6 +299,19 @@ def run_flat(args):
             want_c = np.stack([ref.bulk_contains(int(values_h[i])) for i in pick])
             contains["parity_sample_ok"] = bool(np.array_equal(got_c, want_c))
             parity = parity and contains["parity_sample_ok"]
+        if member is not None:
+            hoff = to_numpy(m_hoff)
+            ids = to_numpy(m_ids)
+            ok = int(hoff[nm]) == ids.size
+            in_m = sample[sample < nm]
+            for j, q in enumerate(in_m):
+                got_ids = ids[int(hoff[q]):int(hoff[q + 1])].tolist()
+                want_ids = np.nonzero(want[int(np.searchsorted(sample, q))] >= m_thr)[0].tolist()
+                if j < 16:  # the oracle's own membership_for on a few of them
+                    ok = ok and ref.membership_for(values_h[offsets_h[q]:offsets_h[q + 1]], m_thr) == want_ids
+                ok = ok and got_ids == want_ids
+            member["parity_sample_ok"] = bool(ok)
+            parity = parity and member["parity_sample_ok"]
         log(f"parity vs oracle on {len(sample)} queries: {parity}")
         if not args.skip_cpu:
             def run_cpu(n, t):
@@ -323,6 +360,7 @@ def run_flat(args):
                             "latency_ms_p99": float(np.percentile(lat, 99))},
         "batched_vs_plain": value / plain_value,
         "bulk_contains": contains,
+        "membership_for": member,
         "parity_sample_ok": parity,
         "build_emplace_s": build_s,
     }

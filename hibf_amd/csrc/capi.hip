@@ -157,6 +157,8 @@ struct hibf_gpu_agent_s
     size_t cub_temp_bytes{0};
     void * counts{nullptr}; // IBF membership: uint16 counts of one chunk
     size_t counts_bytes{0};
+    void * mask{nullptr}; // IBF membership: count >= threshold bits of one chunk
+    size_t mask_bytes{0};
     void * overflow{nullptr}; // HIBF membership: queries answered by the block kernel
     size_t overflow_bytes{0};
     unsigned long long * d_total{nullptr}; // 8 counters, see HibfLaunch::counters
@@ -588,7 +590,8 @@ int hibf_gpu_agent_destroy(hibf_gpu_agent a)
 {
     if (!a)
         return HIBF_GPU_OK;
-    for (void * p : {a->scratch, a->ids_tmp, a->hit_start, a->hit_count, a->incl, a->cub_temp, a->counts, a->overflow,
+    for (void * p : {a->scratch, a->ids_tmp, a->hit_start, a->hit_count, a->incl, a->cub_temp, a->counts, a->mask,
+                     a->overflow,
                      static_cast<void *>(a->d_total)})
         if (p)
             (void)hipFree(p);
@@ -632,11 +635,12 @@ namespace
 {
 
 // d_hit_offsets[q0 .. q0+n] from hit_count[0..n) (write_first: also d_hit_offsets[q0] = base).
+// d_offsets[1 + i] = *d_base (0 when d_base is null) + inclusive sum of
+// d_count[0 .. i]; d_offsets[0] is written (0) only when d_base is null.
 int build_offsets(hibf_gpu_agent a,
                   uint64_t const * d_count,
                   uint64_t n,
-                  uint64_t base,
-                  bool write_first,
+                  uint64_t const * d_base,
                   uint64_t * d_offsets,
                   hipStream_t s)
 {
@@ -648,7 +652,8 @@ int build_offsets(hibf_gpu_agent a,
         return rc;
     size_t have = a->cub_temp_bytes;
     HIBF_TRY_HIP(inclusive_sum_u64(a->cub_temp, have, d_count, static_cast<uint64_t *>(a->incl), n, s), "scan");
-    HIBF_TRY_HIP(launch_finalize_offsets(static_cast<uint64_t *>(a->incl), n, base, write_first, d_offsets, s),
+    HIBF_TRY_HIP(launch_finalize_offsets(static_cast<uint64_t *>(a->incl), n, 0, d_base, d_base == nullptr, d_offsets,
+                                         s),
                  "offsets");
     return HIBF_GPU_OK;
 }
@@ -675,36 +680,40 @@ int hibf_gpu_ibf_membership_for(hibf_gpu_ibf ibf,
     {
         return HIBF_GPU_OK;
     }
-    // Chunk the queries so that the uint16 counts of one chunk stay within 256 MiB.
+    // Chunk the queries so that the uint16 counts of one chunk stay within 256 MiB.  The chunks are queued
+    // back to back: each one's offsets continue from the previous chunk's last offset in device memory, so
+    // the host waits once, for the total.
     uint64_t const bins = ibf->g.bins;
+    uint64_t const mask_words = ibf->g.bin_words;
     uint64_t chunk = std::max<uint64_t>(1u, (256ULL << 20) / (bins * 2u));
     chunk = std::min<uint64_t>(chunk, std::min<uint64_t>(nq, a->max_queries));
     if (int rc = ensure(a->counts, a->counts_bytes, chunk * bins * 2u, "membership counts"))
         return rc;
     if (int rc = ensure(a->hit_count, a->hit_count_bytes, chunk * sizeof(uint64_t), "membership counts"))
         return rc;
+    if (int rc = ensure(a->mask, a->mask_bytes, chunk * mask_words * sizeof(uint64_t), "membership masks"))
+        return rc;
     IbfView const g = view_of(ibf->g, ibf->d_words);
-    uint64_t base = 0;
+    auto * counts = static_cast<uint16_t *>(a->counts);
+    auto * hc = static_cast<uint64_t *>(a->hit_count);
+    auto * mask = static_cast<uint64_t *>(a->mask);
     for (uint64_t q0 = 0; q0 < nq; q0 += chunk)
     {
         uint64_t const n = std::min(chunk, nq - q0);
-        auto * counts = static_cast<uint16_t *>(a->counts);
-        auto * hc = static_cast<uint64_t *>(a->hit_count);
         HIBF_TRY_HIP(launch_count(g, d_values, d_offsets + q0, n, 2u, counts, 1u << 24, s), "count");
-        HIBF_TRY_HIP(launch_threshold_compact(counts, bins, n, d_thresholds + q0, hc, nullptr, nullptr, 0, 0, s),
-                     "threshold pass 0");
-        if (int rc = build_offsets(a, hc, n, base, q0 == 0, d_hit_offsets + q0, s))
+        HIBF_TRY_HIP(launch_threshold_mask(counts, bins, n, d_thresholds + q0, hc, mask, mask_words, s),
+                     "threshold mask");
+        if (int rc = build_offsets(a, hc, n, q0 == 0 ? nullptr : d_hit_offsets + q0, d_hit_offsets + q0, s))
             return rc;
-        HIBF_TRY_HIP(hipMemcpyAsync(a->h_total, d_hit_offsets + q0 + n, sizeof(uint64_t), hipMemcpyDeviceToHost, s),
-                     "total copy");
-        HIBF_TRY_HIP(launch_threshold_compact(counts, bins, n, d_thresholds + q0, hc, d_hit_offsets + q0, d_ids,
-                                              d_ids ? ids_cap : 0, 1, s),
-                     "threshold pass 1");
-        HIBF_TRY_HIP(hipStreamSynchronize(s), "membership sync");
-        base = *a->h_total;
+        HIBF_TRY_HIP(launch_mask_compact(mask, mask_words, n, d_hit_offsets + q0, d_ids, d_ids ? ids_cap : 0, s),
+                     "mask compaction");
     }
-    *h_total = base;
-    if (base > ids_cap)
+    HIBF_TRY_HIP(hipMemcpyAsync(a->h_total, d_hit_offsets + nq, sizeof(uint64_t), hipMemcpyDeviceToHost, s),
+                 "total copy");
+    HIBF_TRY_HIP(hipStreamSynchronize(s), "membership sync");
+    uint64_t const total = *a->h_total;
+    *h_total = total;
+    if (total > ids_cap || (!d_ids && total))
         return fail(HIBF_GPU_ERR_CAPACITY, "ids_cap too small; *h_total holds the needed size");
     return HIBF_GPU_OK;
 }
@@ -1121,7 +1130,7 @@ int hibf_gpu_hibf_membership_for(hibf_gpu_hibf h,
     }
     uint64_t const total = *a->h_total;
     *h_total = total;
-    if (int rc = build_offsets(a, static_cast<uint64_t *>(a->hit_count), nq, 0, true, d_hit_offsets, s))
+    if (int rc = build_offsets(a, static_cast<uint64_t *>(a->hit_count), nq, nullptr, d_hit_offsets, s))
         return rc;
     if (total > ids_cap || (!d_ids && total))
     {

@@ -358,80 +358,126 @@ __global__ void __launch_bounds__(256) gather_hits_kernel(uint64_t const * __res
     }
 }
 
-// IBF membership_for from uint16 counts (interleaved_bloom_filter.hpp:757-761).
-// pass 0: hit_count[q] = #bins with count >= t.  pass 1: write the bins in
-// ascending order at hit_offsets[q].  One block per query.
-__global__ void __launch_bounds__(256) threshold_compact_kernel(uint16_t const * __restrict__ counts,
-                                                                uint64_t bins,
-                                                                uint64_t nq,
-                                                                uint16_t const * __restrict__ thresholds,
-                                                                uint64_t * __restrict__ hit_count,
-                                                                uint64_t const * __restrict__ hit_offsets,
-                                                                uint64_t * __restrict__ ids,
-                                                                uint64_t ids_cap,
-                                                                int pass)
+// IBF membership_for from uint16 counts (interleaved_bloom_filter.hpp:757-761),
+// one wave per query.  threshold_mask_kernel reads a query's counts once, 8
+// counters per lane and 16-byte load, and writes one bit per bin (count >= t)
+// into a mask of mask_words 64-bit words (bit b of the mask = bin b) plus the
+// query's hit count.  mask_compact_kernel, after the scan of the hit counts,
+// writes the set bins of each mask in ascending order from hit_offsets[q]:
+// lane j owns mask word j of a 64-word stretch, and a wave prefix sum of the
+// words' popcounts places each lane's bins.
+__global__ void __launch_bounds__(256) threshold_mask_kernel(uint16_t const * __restrict__ counts,
+                                                             uint64_t bins,
+                                                             uint64_t nq,
+                                                             uint16_t const * __restrict__ thresholds,
+                                                             uint64_t * __restrict__ hit_count,
+                                                             uint8_t * __restrict__ mask,
+                                                             uint64_t mask_words)
 {
-    __shared__ uint32_t part[256];
-    int const tid = static_cast<int>(threadIdx.x);
-    uint64_t const chunk = (bins + 255u) / 256u;
-    for (uint64_t q = blockIdx.x; q < nq; q += gridDim.x)
+    int const lane = static_cast<int>(threadIdx.x) & 63;
+    uint64_t const waves_per_block = blockDim.x / 64u;
+    uint64_t const stride = static_cast<uint64_t>(gridDim.x) * waves_per_block;
+    uint64_t const mask_bytes = mask_words * 8u;
+    bool const vec = (bins % 8u) == 0u; // then every query's counts start 16-byte aligned
+    for (uint64_t q = blockIdx.x * waves_per_block + (threadIdx.x >> 6); q < nq; q += stride)
     {
-        uint16_t const t = thresholds[q];
+        uint32_t const t = thresholds[q];
         uint16_t const * const c = counts + q * bins;
-        uint64_t const b0 = static_cast<uint64_t>(tid) * chunk;
-        uint64_t const b1 = b0 + chunk < bins ? b0 + chunk : bins;
-        uint32_t mine = 0;
-        for (uint64_t b = b0; b < b1; ++b)
-            mine += c[b] >= t ? 1u : 0u;
-        part[tid] = mine;
-        __syncthreads();
-        if (pass == 0)
+        uint8_t * const m = mask + q * mask_bytes;
+        uint32_t hits = 0;
+        for (uint64_t g = static_cast<uint64_t>(lane); g < mask_bytes; g += 64u) // g: bins 8g .. 8g + 7
         {
-            if (tid == 0)
+            uint64_t const b0 = g * 8u;
+            uint32_t byte = 0;
+            if (vec && b0 < bins)
             {
-                uint64_t s = 0;
-                for (int i = 0; i < 256; ++i)
-                    s += part[i];
-                hit_count[q] = s;
-            }
-        }
-        else
-        {
-            uint64_t o = hit_offsets[q];
-            for (int i = 0; i < tid; ++i)
-                o += part[i];
-            for (uint64_t b = b0; b < b1; ++b)
-                if (c[b] >= t)
+                uint4 const w = *reinterpret_cast<uint4 const *>(c + b0);
+                uint32_t const x[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
                 {
-                    if (o < ids_cap)
-                        ids[o] = b;
-                    ++o;
+                    byte |= ((x[j] & 0xFFFFu) >= t ? 1u : 0u) << (2 * j);
+                    byte |= ((x[j] >> 16) >= t ? 1u : 0u) << (2 * j + 1);
                 }
+            }
+            else
+            {
+                for (int j = 0; j < 8; ++j)
+                    if (b0 + j < bins && c[b0 + j] >= t)
+                        byte |= 1u << j;
+            }
+            m[g] = static_cast<uint8_t>(byte);
+            hits += static_cast<uint32_t>(__popc(byte));
         }
-        __syncthreads();
+        for (int o = 32; o > 0; o >>= 1)
+            hits += __shfl_xor(hits, o);
+        if (lane == 0)
+            hit_count[q] = hits;
+    }
+}
+
+__global__ void __launch_bounds__(256) mask_compact_kernel(uint64_t const * __restrict__ mask,
+                                                           uint64_t mask_words,
+                                                           uint64_t nq,
+                                                           uint64_t const * __restrict__ hit_offsets,
+                                                           uint64_t * __restrict__ ids,
+                                                           uint64_t ids_cap)
+{
+    int const lane = static_cast<int>(threadIdx.x) & 63;
+    uint64_t const waves_per_block = blockDim.x / 64u;
+    uint64_t const stride = static_cast<uint64_t>(gridDim.x) * waves_per_block;
+    for (uint64_t q = blockIdx.x * waves_per_block + (threadIdx.x >> 6); q < nq; q += stride)
+    {
+        uint64_t o = hit_offsets[q];
+        uint64_t const * const m = mask + q * mask_words;
+        for (uint64_t w0 = 0; w0 < mask_words; w0 += 64u)
+        {
+            uint64_t const wi = w0 + static_cast<uint64_t>(lane);
+            uint64_t bits = wi < mask_words ? m[wi] : 0ULL;
+            uint32_t const n = static_cast<uint32_t>(__popcll(bits));
+            uint32_t incl = n;
+            for (int d = 1; d < 64; d <<= 1)
+            {
+                uint32_t const y = __shfl_up(incl, d);
+                if (lane >= d)
+                    incl += y;
+            }
+            uint64_t p = o + (incl - n);
+            while (bits)
+            {
+                if (p < ids_cap)
+                    ids[p] = wi * 64u + static_cast<uint64_t>(__builtin_ctzll(bits));
+                ++p;
+                bits &= bits - 1ULL;
+            }
+            o += __shfl(incl, 63);
+        }
     }
 }
 
 __global__ void __launch_bounds__(256) finalize_offsets_kernel(uint64_t const * __restrict__ incl,
                                                                uint64_t n,
                                                                uint64_t base,
+                                                               uint64_t const * d_base,
                                                                bool write_first,
-                                                               uint64_t * __restrict__ out)
+                                                               uint64_t * out)
 {
     uint64_t const i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    uint64_t const b = base + (d_base ? *d_base : 0u);
     if (i == 0 && write_first)
-        out[0] = base;
+        out[0] = b;
     if (i < n)
-        out[1 + i] = base + incl[i];
+        out[1 + i] = b + incl[i];
 }
 
 } // namespace
 
-hipError_t launch_finalize_offsets(uint64_t const * incl, uint64_t n, uint64_t base, bool write_first,
-                                   uint64_t * out, hipStream_t stream)
+hipError_t launch_finalize_offsets(uint64_t const * incl, uint64_t n, uint64_t base, uint64_t const * d_base,
+                                   bool write_first, uint64_t * out, hipStream_t stream)
 {
     unsigned const blocks = static_cast<unsigned>((n + 255u) / 256u + 1u);
-    hipLaunchKernelGGL(finalize_offsets_kernel, dim3(blocks), dim3(256), 0, stream, incl, n, base, write_first, out);
+    hipLaunchKernelGGL(finalize_offsets_kernel, dim3(blocks), dim3(256), 0, stream, incl, n, base, d_base, write_first,
+                       out);
     return hipGetLastError();
 }
 
@@ -506,22 +552,36 @@ hipError_t launch_gather_hits(uint64_t const * src,
     return hipGetLastError();
 }
 
-hipError_t launch_threshold_compact(uint16_t const * counts,
-                                    uint64_t bins,
-                                    uint64_t nq,
-                                    uint16_t const * thresholds,
-                                    uint64_t * hit_count,
-                                    uint64_t const * hit_offsets,
-                                    uint64_t * ids,
-                                    uint64_t ids_cap,
-                                    int pass,
-                                    hipStream_t stream)
+hipError_t launch_threshold_mask(uint16_t const * counts,
+                                 uint64_t bins,
+                                 uint64_t nq,
+                                 uint16_t const * thresholds,
+                                 uint64_t * hit_count,
+                                 uint64_t * mask,
+                                 uint64_t mask_words,
+                                 hipStream_t stream)
 {
     if (nq == 0)
         return hipSuccess;
-    unsigned const blocks = static_cast<unsigned>(nq < 65536u ? nq : 65536u);
-    hipLaunchKernelGGL(threshold_compact_kernel, dim3(blocks), dim3(256), 0, stream, counts, bins, nq, thresholds,
-                       hit_count, hit_offsets, ids, ids_cap, pass);
+    unsigned const blocks = static_cast<unsigned>(std::min<uint64_t>((nq + 3u) / 4u, 1u << 16));
+    hipLaunchKernelGGL(threshold_mask_kernel, dim3(blocks), dim3(256), 0, stream, counts, bins, nq, thresholds,
+                       hit_count, reinterpret_cast<uint8_t *>(mask), mask_words);
+    return hipGetLastError();
+}
+
+hipError_t launch_mask_compact(uint64_t const * mask,
+                               uint64_t mask_words,
+                               uint64_t nq,
+                               uint64_t const * hit_offsets,
+                               uint64_t * ids,
+                               uint64_t ids_cap,
+                               hipStream_t stream)
+{
+    if (nq == 0)
+        return hipSuccess;
+    unsigned const blocks = static_cast<unsigned>(std::min<uint64_t>((nq + 3u) / 4u, 1u << 16));
+    hipLaunchKernelGGL(mask_compact_kernel, dim3(blocks), dim3(256), 0, stream, mask, mask_words, nq, hit_offsets, ids,
+                       ids_cap);
     return hipGetLastError();
 }
 

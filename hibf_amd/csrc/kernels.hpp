@@ -113,18 +113,25 @@ hipError_t launch_count(IbfView const & g,
                         unsigned max_blocks,
                         hipStream_t stream);
 
-// IBF membership_for from counts already in device memory: bins with
-// uint16 count >= threshold[q], ascending, into per-query slots.
-hipError_t launch_threshold_compact(uint16_t const * counts,
-                                    uint64_t bins,
-                                    uint64_t nq,
-                                    uint16_t const * thresholds,
-                                    uint64_t * hit_count,
-                                    uint64_t const * hit_offsets,
-                                    uint64_t * ids,
-                                    uint64_t ids_cap,
-                                    int pass,
-                                    hipStream_t stream);
+// IBF membership_for from counts already in device memory: mask bit b of query
+// q (mask words q * mask_words ..) = counts[q * bins + b] >= thresholds[q], and
+// hit_count[q] = the number of set bits; then the set bins of each mask, in
+// ascending order, into ids[hit_offsets[q] ..) (positions >= ids_cap dropped).
+hipError_t launch_threshold_mask(uint16_t const * counts,
+                                 uint64_t bins,
+                                 uint64_t nq,
+                                 uint16_t const * thresholds,
+                                 uint64_t * hit_count,
+                                 uint64_t * mask,
+                                 uint64_t mask_words,
+                                 hipStream_t stream);
+hipError_t launch_mask_compact(uint64_t const * mask,
+                               uint64_t mask_words,
+                               uint64_t nq,
+                               uint64_t const * hit_offsets,
+                               uint64_t * ids,
+                               uint64_t ids_cap,
+                               hipStream_t stream);
 
 struct HibfLaunch
 {
@@ -242,9 +249,10 @@ hipError_t launch_gather_hits(uint64_t const * src,
                               uint64_t * out,
                               hipStream_t stream);
 
-// out[0] = base0 (only when write_first), out[1 + i] = base + incl[i].
-hipError_t launch_finalize_offsets(uint64_t const * incl, uint64_t n, uint64_t base, bool write_first,
-                                   uint64_t * out, hipStream_t stream);
+// With b = base + (*d_base, or 0 when d_base is null): out[0] = b (only when
+// write_first), out[1 + i] = b + incl[i].  d_base may point at out[0].
+hipError_t launch_finalize_offsets(uint64_t const * incl, uint64_t n, uint64_t base, uint64_t const * d_base,
+                                   bool write_first, uint64_t * out, hipStream_t stream);
 
 } // namespace hibf_amd
 

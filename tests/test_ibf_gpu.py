@@ -169,6 +169,31 @@ def test_membership_for(bins, bin_size, h):
         assert ids[hoff[i]:hoff[i + 1]].tolist() == ref.membership_for(q, t)
 
 
+@pytest.mark.parametrize("bins,bin_size,h", [(8192, 1024, 3), (1003, 700, 2), (64, 2000, 2)])
+def test_membership_for_chunked(bins, bin_size, h):
+    """An agent limited to 7 queries per launch splits 60 queries into 9 chunks
+    whose offsets continue on the device; hits (threshold 0 = every bin,
+    empty queries, thresholds above the length) equal the oracle's and the
+    one-chunk agent's."""
+    from hibf_amd.device import Agent, csr, to_numpy
+    _say(f"membership chunked bins={bins}")
+    ref, dev, vals, _, rng = _pair(bins, bin_size, h, 30000, bins + 11)
+    qs = _queries(rng, vals, [int(n) for n in rng.integers(0, 120, size=60)])
+    qs[4] = np.zeros(0, dtype=np.uint64)
+    values, offsets = csr(qs)
+    thresholds = rng.integers(0, 60, size=len(qs)).astype(np.uint16)
+    thresholds[[0, 9, 30]] = 0
+    thresholds[17] = 500
+    ids, hoff = dev.membership_for(values, offsets, thresholds, Agent(7))
+    ids1, hoff1 = dev.membership_for(values, offsets, thresholds, Agent())
+    ids, hoff = to_numpy(ids), to_numpy(hoff)
+    assert np.array_equal(ids, to_numpy(ids1)) and np.array_equal(hoff, to_numpy(hoff1))
+    assert hoff[0] == 0 and hoff[-1] == ids.size
+    for i, (q, t) in enumerate(zip(qs, thresholds)):
+        assert ids[hoff[i]:hoff[i + 1]].tolist() == ref.membership_for(q, int(t)), i
+    assert hoff[1] - hoff[0] == bins  # threshold 0 on query 0
+
+
 def test_clear_matches_oracle():
     from hibf_amd.device import csr, to_numpy
     _say("clear")
