@@ -177,6 +177,14 @@ struct hibf_gpu_hibf_s
     uint32_t max_bins{0};
     uint32_t depth{0};
     uint32_t uniform_hash_funs{0}; // hash-function count shared by every IBF, 0 if they differ
+    // Run-end TBs of the IBFs reachable from IBF 0: the most ids one query can
+    // produce (each IBF is visited at most once; a user bin whose run is cut by
+    // a deleted TB, or that has two runs, is reported once per run).
+    uint64_t max_hits{0};
+    // Some user-bin id occurs in two IBFs.  The reference never builds such
+    // tables; bulk_count refuses them (its device scan stores an IBF's counts
+    // before descending into its merged bins, the reference in bin order).
+    bool user_bin_in_two_ibfs{false};
     uint64_t * d_arena{nullptr};
     HibfIbfDesc * d_desc{nullptr};
     uint64_t * d_tb_info{nullptr};
@@ -820,7 +828,9 @@ int hibf_gpu_hibf_create(uint64_t n_ibfs,
             if (++visited > n_ibfs)
                 return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "merged bins do not form a tree");
             for (uint64_t b = h_tb_offsets[i]; b < h_tb_offsets[i + 1]; ++b)
-                if ((info[b] >> 32) == kTbMerged)
+                if ((info[b] >> 32) == kTbRunEnd)
+                    ++h->max_hits;
+                else if ((info[b] >> 32) == kTbMerged)
                 {
                     uint64_t const c = info[b] & 0xFFFFFFFFu;
                     if (level[c])
@@ -833,6 +843,18 @@ int hibf_gpu_hibf_create(uint64_t n_ibfs,
         if (depth > kHibfMaxDepth)
             return fail(HIBF_GPU_ERR_UNSUPPORTED, "HIBF deeper than 16 levels");
         h->depth = depth;
+    }
+    {
+        std::vector<std::pair<uint64_t, uint64_t>> owner; // (user bin, IBF) of every user-bin TB
+        owner.reserve(n_tbs);
+        for (uint64_t i = 0; i < n_ibfs; ++i)
+            for (uint64_t b = h_tb_offsets[i]; b < h_tb_offsets[i + 1]; ++b)
+                if ((info[b] >> 32) == kTbRegular || (info[b] >> 32) == kTbRunEnd)
+                    owner.emplace_back(info[b] & 0xFFFFFFFFu, i);
+        std::sort(owner.begin(), owner.end());
+        for (size_t k = 1; k < owner.size(); ++k)
+            if (owner[k].first == owner[k - 1].first && owner[k].second != owner[k - 1].second)
+                h->user_bin_in_two_ibfs = true;
     }
     if (static_cast<size_t>(h->max_bins) * (1u + h->depth) * sizeof(uint32_t) > 64u * 1024u)
         return fail(HIBF_GPU_ERR_UNSUPPORTED, "largest IBF x depth exceeds the LDS budget of the descent kernel");
@@ -964,8 +986,8 @@ namespace
 {
 HibfLaunch launch_of(hibf_gpu_hibf h, hibf_gpu_agent a)
 {
-    return HibfLaunch{h->d_desc,      h->d_arena, h->d_tb_info, h->max_bins, h->depth, h->n_user_bins,
-                      (h->max_bins + 63u) / 64u, h->uniform_hash_funs, a->d_total};
+    return HibfLaunch{h->d_desc,      h->d_arena,   h->d_tb_info, h->max_bins, h->depth, h->n_user_bins,
+                      h->max_hits,    (h->max_bins + 63u) / 64u, h->uniform_hash_funs, a->d_total};
 }
 
 unsigned hibf_blocks(uint64_t nq, uint64_t per_block_scratch_bytes)
@@ -988,15 +1010,21 @@ bool wave_kernel_fits(HibfLaunch const & L)
     return g_count_tuning.hibf_wave != 0u && hibf_wave_lds_bytes(L, 4) <= 64u * 1024u;
 }
 
+// Per-block hit list of the block-per-query kernel: room for every id one query can produce.
+uint64_t block_hit_list_bytes(hibf_gpu_hibf h)
+{
+    return std::max<uint64_t>(h->max_hits, 1u) * sizeof(uint32_t);
+}
+
 unsigned overflow_blocks(hibf_gpu_hibf h, uint64_t nq)
 {
-    return std::min<unsigned>(256u, hibf_blocks(nq, h->n_user_bins * sizeof(uint32_t)));
+    return std::min<unsigned>(256u, hibf_blocks(nq, block_hit_list_bytes(h)));
 }
 
 // Workspace of the membership path for batches of up to nq queries.
 int prepare_membership(hibf_gpu_hibf h, hibf_gpu_agent a, uint64_t nq)
 {
-    uint64_t const per_block = h->n_user_bins * sizeof(uint32_t);
+    uint64_t const per_block = block_hit_list_bytes(h);
     HibfLaunch const L = launch_of(h, a);
     unsigned const blocks = wave_kernel_fits(L) ? overflow_blocks(h, nq) : hibf_blocks(nq, per_block);
     if (int rc = ensure(a->scratch, a->scratch_bytes, static_cast<size_t>(blocks) * per_block, "membership scratch"))
@@ -1022,7 +1050,7 @@ int enqueue_membership(hibf_gpu_hibf h,
                        hipStream_t s)
 {
     HibfLaunch const L = launch_of(h, a);
-    uint64_t const per_block = h->n_user_bins * sizeof(uint32_t);
+    uint64_t const per_block = block_hit_list_bytes(h);
     HIBF_TRY_HIP(launch_reset_counters(a->d_total, kMembershipCounters, s), "counter reset");
     if (wave_kernel_fits(L))
     {
@@ -1157,6 +1185,8 @@ int hibf_gpu_hibf_bulk_count(hibf_gpu_hibf h,
         return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "null argument");
     if (counter_bytes != 1 && counter_bytes != 2 && counter_bytes != 4 && counter_bytes != 8)
         return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "counter_bytes must be 1, 2, 4 or 8");
+    if (h->user_bin_in_two_ibfs)
+        return fail(HIBF_GPU_ERR_UNSUPPORTED, "bulk_count on an HIBF whose tables list one user bin in two IBFs");
     if (nq == 0)
         return HIBF_GPU_OK;
     hipStream_t const s = static_cast<hipStream_t>(stream);

@@ -91,6 +91,26 @@ __device__ __forceinline__ uint64_t shfl64(uint64_t x, int src)
     return (static_cast<uint64_t>(hi) << 32) | lo;
 }
 
+// Of the lanes with `store` set, keeps the highest lane of each id and clears
+// the others (a wave-uniform loop over the storing lanes; nothing to do for
+// fewer than two).  Used where one instruction would store several lanes'
+// values to the same address and the last in bin order must win.
+__device__ __forceinline__ bool highest_lane_per_id(bool store, uint32_t id, int lane)
+{
+    uint64_t rest = __ballot(store);
+    if (__popcll(rest) < 2)
+        return store;
+    while (rest)
+    {
+        int const j = __builtin_ctzll(rest);
+        rest &= rest - 1ULL;
+        uint32_t const idj = static_cast<uint32_t>(__shfl(static_cast<int>(id), j, 64));
+        if (store && lane < j && id == idj)
+            store = false;
+    }
+    return store;
+}
+
 __device__ __forceinline__ uint64_t readlane64(uint64_t x, int src)
 {
     uint32_t lo = __builtin_amdgcn_readlane(static_cast<int>(static_cast<uint32_t>(x)), src);

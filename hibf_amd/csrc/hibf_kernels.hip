@@ -87,7 +87,10 @@ __device__ void scan_ibf(HibfIbfDesc const & d,
         bool is_event = hit;
         if constexpr (COUNTING)
         {
-            if (hit && kind == kTbRunEnd)
+            // A user bin whose run is cut (by a deleted bin) keeps the sum of its last run that reached the
+            // threshold (hierarchical_interleaved_bloom_filter.hpp:488-491): of two such run ends in this chunk,
+            // the higher bin stores.
+            if (highest_lane_per_id(hit && kind == kTbRunEnd, payload, lane))
                 store_counter(count_row, payload, counter_bytes, sum);
             is_event = hit && kind == kTbMerged;
         }
@@ -136,7 +139,7 @@ __device__ __forceinline__ void descent_body(HibfLaunch const & h,
     int const lane = tid & 63;
     int const wave = tid >> 6;
     int const nwaves = nthreads >> 6;
-    uint32_t * const my_scratch = COUNTING ? nullptr : scratch + static_cast<uint64_t>(blockIdx.x) * h.n_user_bins;
+    uint32_t * const my_scratch = COUNTING ? nullptr : scratch + static_cast<uint64_t>(blockIdx.x) * h.max_hits;
 
     // Work stealing: each block takes the next query from the agent's cursor.
     uint64_t visits = 0;
@@ -243,8 +246,14 @@ __device__ __forceinline__ void descent_body(HibfLaunch const & h,
                     st.pos[depth] = p + 1u;
                     if (!COUNTING && kind != kTbMerged)
                     {
-                        my_scratch[st.hits] = payload;
-                        st.hits = st.hits + 1u;
+                        // max_hits bounds the run ends of one descent; the check only guards the list.
+                        if (st.hits < h.max_hits)
+                        {
+                            my_scratch[st.hits] = payload;
+                            st.hits = st.hits + 1u;
+                        }
+                        else
+                            h.counters[5] = 1ULL;
                     }
                 }
                 __syncthreads();

@@ -184,7 +184,10 @@ __device__ __forceinline__ uint32_t scan_wave(HibfIbfDesc const & d,
         bool is_event = hit;
         if constexpr (COUNTING)
         {
-            if (hit && kind == kTbRunEnd)
+            // A user bin whose run is cut (by a deleted bin) keeps the sum of its last run that reached the
+            // threshold (hierarchical_interleaved_bloom_filter.hpp:488-491): of two such run ends in this chunk,
+            // the higher bin stores.
+            if (highest_lane_per_id(hit && kind == kTbRunEnd, payload, lane))
                 store_counter(count_row, payload, counter_bytes, sum);
             is_event = hit && kind == kTbMerged;
         }

@@ -141,6 +141,7 @@ struct HibfLaunch
     uint32_t max_bins;        // largest IBF of this HIBF
     uint32_t depth;           // levels
     uint64_t n_user_bins;
+    uint64_t max_hits;        // most ids one query can produce (run-end TBs reachable from IBF 0)
     uint32_t max_bin_words;   // widest row of this HIBF (words)
     uint32_t uniform_hash_funs; // hash functions of every IBF, 0 if they differ
     // Agent counters (device): [0] ids found, [1] next query (work stealing),

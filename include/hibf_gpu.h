@@ -342,7 +342,10 @@ int hibf_gpu_hibf_serve(hibf_gpu_hibf hibf,
  * (hierarchical_interleaved_bloom_filter.hpp:462-496, 549-566): d_counts gets
  * nq * number_of_user_bins counters of counter_bytes (1, 2, 4, 8); a user bin's
  * counter is its run sum if that sum >= d_thresholds[q] (size_t, > 0) and 0
- * otherwise.  Asynchronous on `stream`. */
+ * otherwise; a user bin with several runs in its IBF (cut by deleted bins)
+ * keeps the last run sum that reached the threshold, as the reference does.
+ * Asynchronous on `stream`.  HIBF_GPU_ERR_UNSUPPORTED for tables that list one
+ * user bin in two IBFs (the reference never builds them). */
 int hibf_gpu_hibf_bulk_count(hibf_gpu_hibf hibf,
                              hibf_gpu_agent agent,
                              uint64_t const * d_values,

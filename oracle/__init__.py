@@ -236,9 +236,14 @@ class Hibf:
     def membership_for(self, values, threshold: int) -> list[int]:
         v = _u64(values)
         cap = max(self.n_user_bins, 1)
-        out = np.zeros(cap, dtype=np.uint64)
-        n = lib().orc_hibf_membership_for(ctypes.byref(self.s), _p(v), v.size, threshold, _p(out), cap)
-        return [int(x) for x in out[:n]]
+        while True:
+            # A user bin whose run is cut by deleted bins is reported once per run, so the list can be
+            # longer than number_of_user_bins: the C side returns the full length and the call is repeated.
+            out = np.zeros(cap, dtype=np.uint64)
+            n = lib().orc_hibf_membership_for(ctypes.byref(self.s), _p(v), v.size, threshold, _p(out), cap)
+            if n <= cap:
+                return [int(x) for x in out[:n]]
+            cap = int(n)
 
     def membership_batch(self, values, offsets, thresholds):
         v, o = _u64(values), _u64(offsets)

@@ -204,6 +204,156 @@ def test_hibf_parity_random(n_ub, sizes, config, descent_kernel):
         assert np.array_equal(c, ref.bulk_count_batch(values, offsets, cthr, width)), width
 
 
+def _tiny_tree_with_deleted_bin():
+    """Root IBF: [UB0 split over TBs 0-1, merged->IBF1, UB3, deleted, UB4];
+    IBF1: [UB1, UB2 split over TBs 1-2].  The deleted TB 4 holds the values of
+    ub[5], which is not a user bin."""
+    rng = np.random.default_rng(9)
+    ub = [rng.integers(0, 2**63, size=n, dtype=np.int64).astype(np.uint64) for n in (40, 30, 30, 20, 10, 10)]
+    root = O.Ibf(6, 2048, 2)
+    for vals, b in ((ub[0][:20], 0), (ub[0][20:], 1), (np.concatenate([ub[1], ub[2]]), 2), (ub[3], 3), (ub[5], 4),
+                    (ub[4], 5)):
+        root.emplace(vals, b)
+    child = O.Ibf(3, 2048, 2)
+    for vals, b in ((ub[1], 0), (ub[2][:15], 1), (ub[2][15:], 2)):
+        child.emplace(vals, b)
+    M, D = O.BIN_MERGED, O.BIN_DELETED
+    tables = dict(geoms=[(6, 2048, 2), (3, 2048, 2)], word_offsets=[0, root.words.size], tb_offsets=[0, 6, 9],
+                  next_ibf=[0, 0, 1, 0, 0, 0, 1, 1, 1], user_bin=[0, 0, M, 3, D, 4, 1, 2, 2], number_of_user_bins=5,
+                  arena=np.concatenate([root.words, child.words]))
+    return tables, ub
+
+
+def _device_hibf(t):
+    from hibf_amd.device import DeviceHibf
+    return DeviceHibf(t["geoms"], t["word_offsets"], t["tb_offsets"], t["next_ibf"], t["user_bin"],
+                      t["number_of_user_bins"], arena=t["arena"])
+
+
+def test_deleted_bin_hits_carry_into_the_next_run(descent_kernel):
+    """A deleted TB adds its hits to the running sum without ending a run
+    (hierarchical_interleaved_bloom_filter.hpp:331-340, 475-484): on the tiny
+    tree, hits in deleted TB 4 count towards UB4 (TB 5)."""
+    from hibf_amd.device import to_numpy
+    _say(f"deleted carry kernel={descent_kernel}")
+    t, ub = _tiny_tree_with_deleted_bin()
+    dev = _device_hibf(t)
+    q = np.concatenate([ub[5][:4], ub[4][:2]])
+    assert _membership_lists(dev, [q, ub[1][:8], np.concatenate([ub[4][:6], ub[0][:6], ub[3][:6]])], [6, 5, 6]) == \
+        [[4], [1], [0, 3, 4]]
+    values, offsets = _csr([q])
+    assert to_numpy(dev.bulk_count(values, offsets, [6])).tolist() == [[0, 0, 0, 0, 6]]
+    # TB 1 deleted as well: UB0's run ends at TB 0, and TB 1's hits go to the merged bin's sum.
+    t["user_bin"] = list(t["user_bin"])
+    t["user_bin"][1] = O.BIN_DELETED
+    dev = _device_hibf(t)
+    ref = oracle_of(dev)
+    qs = [np.concatenate([ub[0][20:26], ub[1][:3]]), ub[0][:10], np.concatenate([ub[0][20:40], ub[2][:2]]),
+          np.concatenate([ub[5][:4], ub[4][:2], ub[0][25:30]])]
+    assert _membership_lists(dev, qs, [8, 5, 6, 6]) == [ref.membership_for(q, th) for q, th in zip(qs, [8, 5, 6, 6])]
+    # The carry is seen in the visits: 6 hits in TB 1 + 3 in TB 2 reach t = 8 and the child IBF is visited
+    # (UB1's 3 hits stay below t); with one UB1 value the sum is 7 and only the root is visited.
+    from hibf_amd.device import Agent
+    for extra, visits in ((3, 2), (1, 1)):
+        agent = Agent()
+        agent.stats(reset=True)
+        v1, o1 = _csr([np.concatenate([ub[0][20:26], ub[1][:extra]])])
+        ids, _ = dev.membership_for(v1, o1, [8], agent)
+        assert ids.numel() == 0
+        assert agent.stats()["visits"] == visits, extra
+    values, offsets = _csr(qs)
+    assert np.array_equal(to_numpy(dev.bulk_count(values, offsets, [3, 3, 3, 3])),
+                          ref.bulk_count_batch(values, offsets, [3, 3, 3, 3], 2))
+
+
+def test_runs_cut_by_deleted_bins(descent_kernel):
+    """One IBF [UB0, deleted, UB0, deleted, UB0, UB1]: threshold 0 reports UB0
+    once per run (more ids than user bins), and bulk_count keeps the last run
+    sum that reached the threshold (hierarchical_interleaved_bloom_filter.hpp:488-491),
+    as the oracle's brute-forced answers say."""
+    from hibf_amd.device import to_numpy
+    _say(f"runs cut by deleted bins kernel={descent_kernel}")
+    rng = np.random.default_rng(5)
+    parts = [rng.integers(0, 2**63, size=20, dtype=np.int64).astype(np.uint64) for _ in range(6)]
+    ibf = O.Ibf(6, 4096, 2)
+    for b, p in enumerate(parts):
+        ibf.emplace(p, b)
+    D = O.BIN_DELETED
+    dev = _device_hibf(dict(geoms=[(6, 4096, 2)], word_offsets=[0], tb_offsets=[0, 6], next_ibf=[0] * 6,
+                            user_bin=[0, D, 0, D, 0, 1], number_of_user_bins=2, arena=ibf.words))
+    q = np.concatenate([parts[0][:7], parts[1][:2], parts[2][:3], parts[4][:4], parts[5][:1]])
+    assert _membership_lists(dev, [parts[5][:1], q], [0, 5]) == [[0, 0, 0, 1], [0, 0]]
+    values, offsets = _csr([q, q, q])
+    assert to_numpy(dev.bulk_count(values, offsets, [5, 4, 8])).tolist() == [[5, 0], [4, 0], [0, 0]]
+
+
+def test_bulk_count_refuses_a_user_bin_in_two_ibfs():
+    """Tables that list one user bin in two IBFs (never built by the
+    reference) answer membership_for, and bulk_count fails loudly."""
+    from hibf_amd import native
+    _say("user bin in two IBFs")
+    t, ub = _tiny_tree_with_deleted_bin()
+    t["user_bin"] = list(t["user_bin"])
+    t["user_bin"][5] = 1   # UB1 also lives in the child IBF
+    dev = _device_hibf(t)
+    ref = oracle_of(dev)
+    q = np.concatenate([ub[4][:6], ub[1][:6]])
+    assert _membership_lists(dev, [q], [5]) == [ref.membership_for(q, 5)]
+    values, offsets = _csr([q])
+    with pytest.raises(native.HibfGpuError) as e:
+        dev.bulk_count(values, offsets, [5])
+    assert e.value.code == native.ERR_UNSUPPORTED
+
+
+def test_hibf_with_deleted_bins(descent_kernel):
+    """A built HIBF with about 15 % of its user-bin TBs marked deleted (their
+    bits kept), including ends and middles of split runs and TBs just before
+    merged bins: membership and counts equal the oracle's on the same tables."""
+    from hibf_amd.device import to_numpy
+    _say(f"deleted bins kernel={descent_kernel}")
+    rng = np.random.default_rng(41)
+    ubs = _random_user_bins(rng, 400, 20, 400)
+    built = _build_cached_deleted(ubs)
+    t = dict(built)
+    user_bin = t["user_bin"].copy()
+    regular = np.nonzero(user_bin < np.uint64(O.BIN_DELETED))[0]
+    user_bin[rng.choice(regular, size=len(regular) * 15 // 100, replace=False)] = np.uint64(O.BIN_DELETED)
+    starts = set(int(x) for x in t["tb_offsets"][:-1])
+    before_merged = [b - 1 for b in range(1, len(user_bin)) if user_bin[b] == np.uint64(O.BIN_MERGED)
+                     and b not in starts and user_bin[b - 1] < np.uint64(O.BIN_DELETED)]
+    user_bin[before_merged[:3]] = np.uint64(O.BIN_DELETED)
+    t["user_bin"] = user_bin
+    dev = _device_hibf(t)
+    ref = oracle_of(dev)
+    qs = _mixed_queries(rng, ubs, 80, 150)
+    thresholds = [int(x) for x in rng.integers(1, 70, size=len(qs))]
+    thresholds[0] = 0
+    got = _membership_lists(dev, qs, thresholds)
+    for q, th, g in zip(qs, thresholds, got):
+        assert g == ref.membership_for(q, th)
+    values, offsets = _csr(qs)
+    cthr = [max(1, th) for th in thresholds]
+    for width in (1, 2):
+        c = to_numpy(dev.bulk_count(values, offsets, cthr, counter_bytes=width))
+        assert np.array_equal(c, ref.bulk_count_batch(values, offsets, cthr, width)), width
+
+
+_DELETED_CACHE = {}
+
+
+def _build_cached_deleted(ubs):
+    """Tables and arena of the 400-UB HIBF (built once for all descent kernels)."""
+    if "t" not in _DELETED_CACHE:
+        from hibf_amd.device import DeviceHibf
+        dev = DeviceHibf.build(ubs, tmax=64)
+        tab = dev.tables()
+        _DELETED_CACHE["t"] = dict(geoms=tab["geoms"], word_offsets=tab["word_offsets"],
+                                   tb_offsets=tab["tb_offsets"], next_ibf=tab["next_ibf"],
+                                   user_bin=tab["user_bin"], number_of_user_bins=tab["n_user_bins"],
+                                   arena=dev.download_arena())
+    return _DELETED_CACHE["t"]
+
+
 def test_hibf_synthetic_build_matches_oracle():
     from hibf_amd.device import DeviceHibf, to_numpy
     from hibf_amd.synth import synth_values

@@ -308,6 +308,28 @@ def test_hibf_oracle_dfs_order_and_deleted_carry():
     assert hb.bulk_count(q, 6, 2).tolist() == [0, 0, 0, 0, 6]
 
 
+def test_hibf_oracle_runs_cut_by_deleted_bins():
+    """One IBF [UB0, deleted, UB0, deleted, UB0, UB1] with two user bins: UB0
+    has three runs, so threshold 0 reports it three times (a list longer than
+    number_of_user_bins), and bulk_count keeps the last run sum that reached
+    the threshold (hierarchical_interleaved_bloom_filter.hpp:488-491)."""
+    rng = np.random.default_rng(5)
+    parts = [rng.integers(0, 2**63, size=20, dtype=np.int64).astype(np.uint64) for _ in range(6)]
+    ibf = O.Ibf(6, 4096, 2)
+    for b, p in enumerate(parts):
+        ibf.emplace(p, b)
+    D = O.BIN_DELETED
+    hb = O.Hibf([ibf.g], ibf.words, [0], [0, 6], [0] * 6, [0, D, 0, D, 0, 1], 2)
+    assert hb.membership_for(parts[5][:1], 0) == [0, 0, 0, 1]
+    # Runs: TB 0 alone; TB 1 (deleted) carries into TB 2; TB 3 carries into TB 4.
+    q = np.concatenate([parts[0][:7], parts[1][:2], parts[2][:3], parts[4][:4], parts[5][:1]])
+    assert hb.membership_for(q, 5) == [0, 0]           # runs with 7 and 5 hits; 4 < 5
+    assert hb.bulk_count(q, 5, 2).tolist() == [5, 0]   # the last run that reached 5
+    assert hb.bulk_count(q, 4, 2).tolist() == [4, 0]   # TB 4's run (4 hits) comes last
+    assert hb.bulk_count(q, 8, 2).tolist() == [0, 0]
+    assert hb.membership_batch(q, np.array([0, q.size], dtype=np.uint64), [0])[0].tolist() == [0, 0, 0, 1]
+
+
 @pytest.mark.parametrize("bins,bin_size,h,new_bins", [(70, 1000, 2, 130), (64, 333, 3, 65), (3, 500, 2, 64),
                                                       (130, 777, 2, 1000), (8192, 64, 3, 8193), (5, 100, 1, 5)])
 def test_oracle_increase_bin_number_closed_form(bins, bin_size, h, new_bins):
