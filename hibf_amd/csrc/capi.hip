@@ -233,6 +233,8 @@ int hibf_gpu_tune_set(char const * key, int64_t value)
         g_count_tuning.hibf_wave = static_cast<uint32_t>(value);
     else if (k == "hibf.nt_loads" && (value == 0 || value == 1))
         g_count_tuning.nt_loads = static_cast<uint32_t>(value);
+    else if (k == "count.nt_stores" && (value == 0 || value == 1))
+        g_count_tuning.nt_stores = static_cast<uint32_t>(value);
     else if (k == "count.counter_bits" && (value == 8 || value == 16))
         g_count_tuning.counter_bits = static_cast<uint32_t>(value);
     else if (k == "hibf.row_per_lane" && (value == 0 || value == 1))
@@ -259,6 +261,8 @@ int hibf_gpu_tune_get(char const * key, int64_t * value)
         *value = g_count_tuning.hibf_wave;
     else if (k == "hibf.nt_loads")
         *value = g_count_tuning.nt_loads;
+    else if (k == "count.nt_stores")
+        *value = g_count_tuning.nt_stores;
     else if (k == "count.counter_bits")
         *value = g_count_tuning.counter_bits;
     else if (k == "hibf.row_per_lane")
@@ -987,7 +991,8 @@ namespace
 HibfLaunch launch_of(hibf_gpu_hibf h, hibf_gpu_agent a)
 {
     return HibfLaunch{h->d_desc,      h->d_arena,   h->d_tb_info, h->max_bins, h->depth, h->n_user_bins,
-                      h->max_hits,    (h->max_bins + 63u) / 64u, h->uniform_hash_funs, a->d_total};
+                      h->max_hits,    (h->max_bins + 63u) / 64u, h->uniform_hash_funs, g_count_tuning.nt_stores,
+                      0u,             a->d_total};
 }
 
 unsigned hibf_blocks(uint64_t nq, uint64_t per_block_scratch_bytes)

@@ -41,6 +41,8 @@ __device__ __forceinline__ void put_counter(void * out, uint64_t i, uint32_t cb,
 }
 
 // Writes nb counters of a tile (CB-bit fields packed in lds) to out[first ..).
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
 template <int CB>
 __device__ void store_tile(uint32_t const * lds,
                            uint32_t nb,
@@ -48,6 +50,7 @@ __device__ void store_tile(uint32_t const * lds,
                            uint64_t first,
                            uint32_t cb,
                            bool accumulate,
+                           bool nt,
                            int tid,
                            int nthreads)
 {
@@ -73,7 +76,10 @@ __device__ void store_tile(uint32_t const * lds,
                 w.z = (b & 0xFFu) | ((b & 0xFF00u) << 8);
                 w.w = ((b >> 16) & 0xFFu) | ((b >> 8) & 0xFF0000u);
             }
-            out[i] = w;
+            if (nt) // streaming store: the counters are not read again by this kernel
+                __builtin_nontemporal_store(u32x4{w.x, w.y, w.z, w.w}, reinterpret_cast<u32x4 *>(out) + i);
+            else
+                out[i] = w;
         }
         for (uint32_t i = n8 * 8u + static_cast<uint32_t>(tid); i < nb; i += static_cast<uint32_t>(nthreads))
             static_cast<uint16_t *>(counts)[first + i] =
@@ -93,6 +99,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((HF > 
                                                          uint64_t const * __restrict__ offsets,
                                                          uint64_t nq,
                                                          uint32_t counter_bytes,
+                                                         uint32_t nt_stores,
                                                          void * __restrict__ counts)
 {
     extern __shared__ uint32_t lds[]; // kWideTileWords * 64 counters of CB bits
@@ -192,7 +199,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((HF > 
                 __syncthreads();
                 uint32_t const first_bin = w0 * 64u;
                 uint32_t const nb = min(tw * 64u, g.bins - first_bin);
-                store_tile<CB>(lds, nb, counts, q * g.bins + first_bin, counter_bytes, seg0 != 0, tid, nthreads);
+                store_tile<CB>(lds, nb, counts, q * g.bins + first_bin, counter_bytes, seg0 != 0, nt_stores != 0u, tid,
+                               nthreads);
                 __syncthreads();
                 seg0 += segL;
             } while (seg0 < L);
@@ -224,7 +232,7 @@ hipError_t launch_count_wide(IbfView const & g,
     size_t const lds = static_cast<size_t>(kWideTileWords) * (narrow ? 16u : 32u) * sizeof(uint32_t);
     unsigned const grid = static_cast<unsigned>(nq < max_blocks ? nq : max_blocks);
     dim3 const blk(64u * g_count_tuning.wide_waves);
-    using kernel_t = void (*)(IbfView, uint64_t const *, uint64_t const *, uint64_t, uint32_t, void *);
+    using kernel_t = void (*)(IbfView, uint64_t const *, uint64_t const *, uint64_t, uint32_t, uint32_t, void *);
     // [counter bits 16, 8][unroll 1, 2, 4, 8][hash functions: run time, 2, 3]
     static kernel_t const table[2][4][3] = {
         {{count_wide_kernel<1, 0, 16>, count_wide_kernel<1, 2, 16>, count_wide_kernel<1, 3, 16>},
@@ -240,7 +248,7 @@ hipError_t launch_count_wide(IbfView const & g,
     int const ui = u == 1u ? 0 : u == 2u ? 1 : u == 8u ? 3 : 2;
     int const hi = g.hash_funs == 2u ? 1 : g.hash_funs == 3u ? 2 : 0;
     hipLaunchKernelGGL(table[narrow ? 1 : 0][ui][hi], dim3(grid), blk, lds, stream, g, values, offsets, nq,
-                       counter_bytes, counts);
+                       counter_bytes, g_count_tuning.nt_stores, counts);
     return hipGetLastError();
 }
 

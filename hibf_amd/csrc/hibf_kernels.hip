@@ -182,15 +182,26 @@ __device__ __forceinline__ void descent_body(HibfLaunch const & h,
             uint64_t const head = (16u - (reinterpret_cast<uintptr_t>(bytes) & 15u)) & 15u;
             uint64_t const n16 = len > head ? (len - head) / 16u : 0u;
             uint64_t const tail = head + n16 * 16u;
-            uint4 * const mid = reinterpret_cast<uint4 *>(bytes + head);
-            for (uint64_t i = static_cast<uint64_t>(tid); i < n16; i += static_cast<uint64_t>(nthreads))
-                mid[i] = make_uint4(0u, 0u, 0u, 0u);
-            for (uint64_t i = static_cast<uint64_t>(tid); i < h.n_user_bins; i += static_cast<uint64_t>(nthreads))
+            // head and tail are multiples of cb: the row starts at a multiple of cb, and cb divides 16.
+            if (h.nt_stores)
             {
-                uint64_t const at = i * cb;
-                if (at < head || at >= tail)
-                    store_counter(row, i, cb, 0ULL);
+                typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+                u32x4 * const mid = reinterpret_cast<u32x4 *>(bytes + head);
+                for (uint64_t i = static_cast<uint64_t>(tid); i < n16; i += static_cast<uint64_t>(nthreads))
+                    __builtin_nontemporal_store(u32x4{0u, 0u, 0u, 0u}, mid + i);
             }
+            else
+            {
+                uint4 * const mid = reinterpret_cast<uint4 *>(bytes + head);
+                for (uint64_t i = static_cast<uint64_t>(tid); i < n16; i += static_cast<uint64_t>(nthreads))
+                    mid[i] = make_uint4(0u, 0u, 0u, 0u);
+            }
+            uint64_t const n_head = std::min<uint64_t>(head, len) / cb;
+            for (uint64_t i = static_cast<uint64_t>(tid); i < n_head; i += static_cast<uint64_t>(nthreads))
+                store_counter(row, i, cb, 0ULL);
+            for (uint64_t i = std::max<uint64_t>(tail / cb, n_head) + static_cast<uint64_t>(tid); i < h.n_user_bins;
+                 i += static_cast<uint64_t>(nthreads))
+                store_counter(row, i, cb, 0ULL);
         }
         if (tid == 0)
             st.hits = 0;

@@ -52,6 +52,7 @@ struct CountTuning
     uint32_t counter_bits = 16; // count_wide_kernel LDS counter width (8 or 16)
     uint32_t row_per_lane = 0;  // HIBF descent: one value and its whole row per lane when every row is <= 4 words
     uint32_t sliced_counts = 1; // HIBF descent over rows <= 4 words: bit-sliced per-lane counts
+    uint32_t nt_stores = 1;     // count_wide_kernel and the HIBF counting zero fill: non-temporal (streaming) stores
 };
 extern CountTuning g_count_tuning;
 
@@ -144,6 +145,8 @@ struct HibfLaunch
     uint64_t max_hits;        // most ids one query can produce (run-end TBs reachable from IBF 0)
     uint32_t max_bin_words;   // widest row of this HIBF (words)
     uint32_t uniform_hash_funs; // hash functions of every IBF, 0 if they differ
+    uint32_t nt_stores;       // counting: non-temporal stores of the zero fill
+    uint32_t pad;
     // Agent counters (device): [0] ids found, [1] next query (work stealing),
     // [2] IBF visits, [3] row bytes fetched (h * bin_words * 8 per value and
     // visit), [4] lookups (query values checked against one IBF), [5] set when

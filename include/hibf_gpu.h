@@ -52,7 +52,9 @@ const char * hibf_gpu_last_error(void);
  * block 0), "hibf.row_per_lane" (descent over rows of <= 4 words: one value
  * and its whole row per lane 1, lanes per value 0), "hibf.sliced_counts"
  * (descent over rows of <= 4 words: bit-sliced per-lane counts 1, one LDS
- * atomic per set bit 0), "hibf.nt_loads" (0/1).  Results never depend on
+ * atomic per set bit 0), "hibf.nt_loads" (0/1), "count.nt_stores"
+ * (non-temporal stores of the IBF count kernel's counters and of the HIBF
+ * count's zero fill: 1, default; plain stores 0).  Results never depend on
  * these values. */
 int hibf_gpu_tune_set(const char * key, int64_t value);
 int hibf_gpu_tune_get(const char * key, int64_t * value);

@@ -125,7 +125,7 @@ def test_bulk_count_long_queries_wide_rows(bins):
 def knobs():
     """Sets launch-shape knobs for one test and restores the defaults after it."""
     from hibf_amd import native
-    keys = ("count.wide_unroll", "count.wide_waves", "count.counter_bits")
+    keys = ("count.wide_unroll", "count.wide_waves", "count.counter_bits", "count.nt_stores")
     saved = {k: native.tune_get(k) for k in keys}
 
     def set_(**kv):
@@ -134,6 +134,30 @@ def knobs():
     yield set_
     for k, v in saved.items():
         native.tune_set(k, v)
+
+
+@pytest.mark.parametrize("nt_stores", [0, 1])
+def test_counter_store_forms(nt_stores, knobs):
+    """Plain and non-temporal counter stores give the oracle's counts (flat
+    wide kernel, segmented long query included) and the same HIBF counts."""
+    from hibf_amd.device import DeviceHibf, csr, to_numpy
+    from hibf_helpers import oracle_of
+    _say(f"counter stores nt={nt_stores}")
+    knobs(count__nt_stores=nt_stores)
+    ref, dev, vals, _, rng = _pair(8192, 1024, 3, 20000, 5)
+    qs = _queries(rng, vals, [0, 1, 250, 300, 1000]) + [np.full(70000, vals[7], dtype=np.uint64)]
+    values, offsets = csr(qs)
+    for width in (1, 2, 4):
+        assert np.array_equal(to_numpy(dev.bulk_count(values, offsets, width)),
+                              ref.bulk_count_batch(values, offsets, width)), width
+    ubs = [_rand_u64(rng, int(n)) for n in rng.integers(20, 200, size=301)]   # odd row length in bytes
+    hb = DeviceHibf.build(ubs, tmax=64)
+    href = oracle_of(hb)
+    hq = [np.concatenate([ubs[int(i)][:40], _rand_u64(rng, 10)]) for i in rng.integers(0, 301, size=50)]
+    hv, ho = csr(hq)
+    for width in (1, 2, 8):
+        assert np.array_equal(to_numpy(hb.bulk_count(hv, ho, [20] * 50, counter_bytes=width)),
+                              href.bulk_count_batch(hv, ho, [20] * 50, width)), width
 
 
 @pytest.mark.parametrize("h", [2, 3, 4])
