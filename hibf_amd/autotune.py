@@ -23,6 +23,7 @@ Search spaces:
     hibf.sliced_counts  rows of <= 4 words: bit-sliced per-lane counts (1) or
                         one LDS atomic per set bit (0)
     hibf.nt_loads       non-temporal row loads (0, 1)
+    hibf.block_waves    query per block: waves per block (4, 2, 8)
 """
 from __future__ import annotations
 
@@ -50,6 +51,7 @@ HIBF_SPACE = {
     "hibf.row_per_lane": (0, 1),
     "hibf.sliced_counts": (1, 0),
     "hibf.nt_loads": (0, 1),
+    "hibf.block_waves": (4, 2, 8),
 }
 
 

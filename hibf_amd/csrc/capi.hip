@@ -235,6 +235,8 @@ int hibf_gpu_tune_set(char const * key, int64_t value)
         g_count_tuning.nt_loads = static_cast<uint32_t>(value);
     else if (k == "count.nt_stores" && (value == 0 || value == 1))
         g_count_tuning.nt_stores = static_cast<uint32_t>(value);
+    else if (k == "hibf.block_waves" && (value == 1 || value == 2 || value == 4 || value == 8))
+        g_count_tuning.hibf_block_waves = static_cast<uint32_t>(value);
     else if (k == "count.counter_bits" && (value == 8 || value == 16))
         g_count_tuning.counter_bits = static_cast<uint32_t>(value);
     else if (k == "hibf.row_per_lane" && (value == 0 || value == 1))
@@ -263,6 +265,8 @@ int hibf_gpu_tune_get(char const * key, int64_t * value)
         *value = g_count_tuning.nt_loads;
     else if (k == "count.nt_stores")
         *value = g_count_tuning.nt_stores;
+    else if (k == "hibf.block_waves")
+        *value = g_count_tuning.hibf_block_waves;
     else if (k == "count.counter_bits")
         *value = g_count_tuning.counter_bits;
     else if (k == "hibf.row_per_lane")
@@ -995,9 +999,12 @@ HibfLaunch launch_of(hibf_gpu_hibf h, hibf_gpu_agent a)
                       0u,             a->d_total};
 }
 
-unsigned hibf_blocks(uint64_t nq, uint64_t per_block_scratch_bytes)
+// Blocks of `waves` waves for a query-per-block launch: 32 waves per CU (8 per
+// SIMD, what the 64-VGPR descent kernels can hold), fewer when the per-block
+// scratch would pass 1 GiB, never more than the queries.
+unsigned hibf_blocks(uint64_t nq, uint64_t per_block_scratch_bytes, uint32_t waves = 4u)
 {
-    uint64_t blocks = static_cast<uint64_t>(device_cus()) * 8u;
+    uint64_t blocks = static_cast<uint64_t>(device_cus()) * (32u / std::max<uint32_t>(waves, 1u));
     if (per_block_scratch_bytes)
         blocks = std::min<uint64_t>(blocks, std::max<uint64_t>(1u, (1ULL << 30) / per_block_scratch_bytes));
     blocks = std::min<uint64_t>(blocks, nq);
@@ -1023,7 +1030,7 @@ uint64_t block_hit_list_bytes(hibf_gpu_hibf h)
 
 unsigned overflow_blocks(hibf_gpu_hibf h, uint64_t nq)
 {
-    return std::min<unsigned>(256u, hibf_blocks(nq, block_hit_list_bytes(h)));
+    return std::min<unsigned>(256u, hibf_blocks(nq, block_hit_list_bytes(h), g_count_tuning.hibf_block_waves));
 }
 
 // Workspace of the membership path for batches of up to nq queries.
@@ -1031,7 +1038,8 @@ int prepare_membership(hibf_gpu_hibf h, hibf_gpu_agent a, uint64_t nq)
 {
     uint64_t const per_block = block_hit_list_bytes(h);
     HibfLaunch const L = launch_of(h, a);
-    unsigned const blocks = wave_kernel_fits(L) ? overflow_blocks(h, nq) : hibf_blocks(nq, per_block);
+    unsigned const blocks = wave_kernel_fits(L) ? overflow_blocks(h, nq)
+                                                : hibf_blocks(nq, per_block, g_count_tuning.hibf_block_waves);
     if (int rc = ensure(a->scratch, a->scratch_bytes, static_cast<size_t>(blocks) * per_block, "membership scratch"))
         return rc;
     if (int rc = ensure(a->overflow, a->overflow_bytes, std::max<uint64_t>(nq, 1) * 8u, "membership overflow"))
@@ -1075,7 +1083,7 @@ int enqueue_membership(hibf_gpu_hibf h,
                      "membership overflow launch");
         return HIBF_GPU_OK;
     }
-    unsigned const blocks = hibf_blocks(nq, per_block);
+    unsigned const blocks = hibf_blocks(nq, per_block, g_count_tuning.hibf_block_waves);
     if (a->scratch_bytes < static_cast<size_t>(blocks) * per_block)
         return fail(HIBF_GPU_ERR_INVALID_ARGUMENT,
                     "agent not prepared for this batch size (hibf_gpu_hibf_membership_prepare)");

@@ -322,21 +322,21 @@ __device__ __forceinline__ void descent_body(HibfLaunch const & h,
     h, values, offsets, nq, thr16, thr64, counter_bytes, counts, scratch, ids, ids_cap, hit_start, hit_count, qlist
 
 template <bool COUNTING, bool NT, int HF>
-__global__ void __launch_bounds__(256) hibf_descent_kernel(HIBF_DESCENT_PARAMS)
+__global__ void __launch_bounds__(512) hibf_descent_kernel(HIBF_DESCENT_PARAMS)
 {
     descent_body<COUNTING, NT, HF>(HIBF_DESCENT_ARGS);
 }
 
 // Membership with two hash functions (the HIBF default) fits 64 VGPRs without
 // spilling, so it runs at 8 waves per SIMD: more row fetches in flight.
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8)))
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8, 8)))
 hibf_descent_membership_h2_kernel(HIBF_DESCENT_PARAMS)
 {
     descent_body<false, false, 2, 2, 0, false>(HIBF_DESCENT_ARGS);
 }
 
 // Same, for HIBFs whose rows are at most 4 words: one value and its whole row per lane.
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8)))
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8, 8)))
 hibf_descent_membership_h2_row_per_lane_kernel(HIBF_DESCENT_PARAMS)
 {
     descent_body<false, false, 2, 1, 2>(HIBF_DESCENT_ARGS);
@@ -346,7 +346,7 @@ hibf_descent_membership_h2_row_per_lane_kernel(HIBF_DESCENT_PARAMS)
 // tmax = 256 layout): only the 1-, 2- and 4-word counting shapes are compiled.
 // SLICE: bit-sliced per-lane counts with one LDS atomic per bin and chunk.
 template <bool SLICE>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8)))
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8, 8)))
 hibf_descent_membership_h2_w4_kernel(HIBF_DESCENT_PARAMS)
 {
     descent_body<false, false, 2, 2, 1, SLICE>(HIBF_DESCENT_ARGS);
@@ -532,8 +532,9 @@ hipError_t launch_hibf_membership(HibfLaunch const & h,
                         : w4                                   ? hibf_descent_membership_h2_w4_kernel<false>
                         : h2                                   ? hibf_descent_membership_h2_kernel
                                                                : hibf_descent_kernel<false, false, 0>;
-    hipLaunchKernelGGL(kernel, dim3(blocks), dim3(256), lds, stream, h, values, offsets, nq, thresholds, nullptr, 2u,
-                       nullptr, scratch, ids_tmp, ids_cap, hit_start, hit_count, qlist);
+    unsigned const threads = 64u * g_count_tuning.hibf_block_waves;
+    hipLaunchKernelGGL(kernel, dim3(blocks), dim3(threads), lds, stream, h, values, offsets, nq, thresholds, nullptr,
+                       2u, nullptr, scratch, ids_tmp, ids_cap, hit_start, hit_count, qlist);
     return hipGetLastError();
 }
 

@@ -53,6 +53,7 @@ struct CountTuning
     uint32_t row_per_lane = 0;  // HIBF descent: one value and its whole row per lane when every row is <= 4 words
     uint32_t sliced_counts = 1; // HIBF descent over rows <= 4 words: bit-sliced per-lane counts
     uint32_t nt_stores = 1;     // count_wide_kernel and the HIBF counting zero fill: non-temporal (streaming) stores
+    uint32_t hibf_block_waves = 4; // HIBF membership, query per block: waves per block (1, 2, 4)
 };
 extern CountTuning g_count_tuning;
 

@@ -52,7 +52,8 @@ const char * hibf_gpu_last_error(void);
  * block 0), "hibf.row_per_lane" (descent over rows of <= 4 words: one value
  * and its whole row per lane 1, lanes per value 0), "hibf.sliced_counts"
  * (descent over rows of <= 4 words: bit-sliced per-lane counts 1, one LDS
- * atomic per set bit 0), "hibf.nt_loads" (0/1), "count.nt_stores"
+ * atomic per set bit 0), "hibf.nt_loads" (0/1), "hibf.block_waves" (waves
+ * per query block of the membership descent: 1, 2, 4 default, 8), "count.nt_stores"
  * (non-temporal stores of the IBF count kernel's counters and of the HIBF
  * count's zero fill: 1, default; plain stores 0).  Results never depend on
  * these values. */
@@ -281,7 +282,10 @@ int hibf_gpu_hibf_membership_for(hibf_gpu_hibf hibf,
  * are d_ids[d_hit_start[q] .. d_hit_start[q] + d_hit_count[q]); ids beyond
  * ids_cap are dropped (hibf_gpu_agent_status reports it).  The agent must
  * first be prepared (allocates its workspace; not capturable) for batches of
- * up to max_queries queries.  Calls on one agent must be serialised. */
+ * up to max_queries queries, and prepared again after "hibf.block_waves" or
+ * "hibf.wave_kernel" change (a launch on a workspace sized for another shape
+ * returns HIBF_GPU_ERR_INVALID_ARGUMENT).  Calls on one agent must be
+ * serialised. */
 int hibf_gpu_hibf_membership_prepare(hibf_gpu_hibf hibf, hibf_gpu_agent agent, uint64_t max_queries);
 int hibf_gpu_hibf_membership_launch(hibf_gpu_hibf hibf,
                                     hibf_gpu_agent agent,

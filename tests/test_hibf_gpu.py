@@ -158,11 +158,13 @@ CASES = [  # (n_ub, size range, config)
 ]
 
 
-_DESCENT_VARIANTS = {  # (hibf.wave_kernel, hibf.row_per_lane, hibf.sliced_counts)
-    "wave": (1, 0, 1),
-    "block-row-per-lane": (0, 1, 1),
-    "block-lanes-per-value-sliced": (0, 0, 1),
-    "block-lanes-per-value-atomics": (0, 0, 0),
+_DESCENT_VARIANTS = {  # (hibf.wave_kernel, hibf.row_per_lane, hibf.sliced_counts, hibf.block_waves)
+    "wave": (1, 0, 1, 4),
+    "block-row-per-lane": (0, 1, 1, 4),
+    "block-lanes-per-value-sliced": (0, 0, 1, 4),
+    "block-lanes-per-value-atomics": (0, 0, 0, 4),
+    "block-2-waves": (0, 0, 1, 2),
+    "block-8-waves": (0, 0, 1, 8),
 }
 
 
@@ -171,9 +173,9 @@ def descent_kernel(request):
     """Runs a test with each descent kernel: query per wave; query per block
     with one value and its row per lane; query per block with lanes sharing a
     value's row, counting through bit-sliced registers or with one LDS atomic
-    per set bit.  Yields the variant's name."""
+    per set bit; and blocks of 2 and 8 waves.  Yields the variant's name."""
     from hibf_amd import native
-    keys = ("hibf.wave_kernel", "hibf.row_per_lane", "hibf.sliced_counts")
+    keys = ("hibf.wave_kernel", "hibf.row_per_lane", "hibf.sliced_counts", "hibf.block_waves")
     saved = {k: native.tune_get(k) for k in keys}
     for k, v in zip(keys, _DESCENT_VARIANTS[request.param]):
         native.tune_set(k, v)
