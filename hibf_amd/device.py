@@ -89,6 +89,42 @@ def kmer_hash(reads, k: int) -> tuple[torch.Tensor, torch.Tensor]:
     return hashes[: total.value], out_off
 
 
+def hll_sketches(sets, bits: int = 12) -> torch.Tensor:
+    """HyperLogLog sketches of value sets on the device (hibf_gpu_hll_sketch):
+    a (len(sets), 2^bits) uint8 device tensor, one row of registers per set
+    (seqan::hibf::sketch::hyperloglog, one per user bin as compute_sketches
+    builds them).  `sets` is a list of uint64 sequences or a (values, offsets)
+    CSR pair."""
+    values, offsets = sets if isinstance(sets, tuple) else csr(sets)
+    n = int(np.asarray(offsets).size - 1) if not torch.is_tensor(offsets) else offsets.numel() - 1
+    v, o = as_device_u64(values), as_device_u64(offsets)
+    regs = torch.empty((max(n, 1), 1 << bits), dtype=torch.uint8, device=o.device)
+    check(lib().hibf_gpu_hll_sketch(_ptr(v) if v.numel() else None, _ptr(o), n, bits, _ptr(regs), _stream()))
+    return regs[:n]
+
+
+def hll_merge(registers: torch.Tensor, groups) -> torch.Tensor:
+    """Register-wise max of each group of sketches (hibf_gpu_hll_merge);
+    `groups` is a list of lists of sketch rows."""
+    bits = int(registers.shape[1]).bit_length() - 1
+    members, offsets = csr([np.asarray(g, dtype=np.uint64) for g in groups])
+    out = torch.empty((max(len(groups), 1), registers.shape[1]), dtype=torch.uint8, device=registers.device)
+    m, o = as_device_u64(members if members.size else np.zeros(1, dtype=np.uint64)), as_device_u64(offsets)
+    check(lib().hibf_gpu_hll_merge(_ptr(registers), bits, _ptr(m), _ptr(o), len(groups), _ptr(out), _stream()))
+    return out[: len(groups)]
+
+
+def hll_estimates(registers: torch.Tensor) -> np.ndarray:
+    """hyperloglog::estimate of every row of a register tensor (float64)."""
+    bits = int(registers.shape[1]).bit_length() - 1
+    n = registers.shape[0]
+    out = np.zeros(max(n, 1), dtype=np.float64)
+    regs = registers.contiguous()
+    check(lib().hibf_gpu_hll_estimate(_ptr(regs) if n else None, n, bits, out.ctypes.data_as(ctypes.c_void_p),
+                                      _stream()))
+    return out[:n]
+
+
 def csr(queries) -> tuple[np.ndarray, np.ndarray]:
     """(values, offsets) for a list of uint64 sequences."""
     lens = [len(q) for q in queries]

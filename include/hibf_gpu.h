@@ -411,6 +411,42 @@ int hibf_gpu_kmer_thresholds(const uint64_t * d_hash_offsets,
                              void * stream);
 
 /* ======================================================================== */
+/* Build side: HyperLogLog sketches                                         */
+/* ======================================================================== */
+
+/* seqan::hibf::sketch::hyperloglog(bits) + add() over n value sets on the
+ * device: sketch::compute_sketches (src/sketch/compute_sketches.cpp:27-43),
+ * one sketch per user bin, with hyperloglog::add (src/sketch/hyperloglog.cpp:
+ * 48-64).  Set i is d_values[d_offsets[i] .. d_offsets[i + 1]); its 2^bits
+ * uint8 registers are written to d_registers + (i << bits).  bits in [5, 32]
+ * (the constructor's range and message, hyperloglog.cpp:28-29, else
+ * HIBF_GPU_ERR_INVALID_ARGUMENT).  Synchronous. */
+int hibf_gpu_hll_sketch(const uint64_t * d_values,
+                        const uint64_t * d_offsets,
+                        uint64_t n_sketches,
+                        uint32_t bits,
+                        uint8_t * d_registers,
+                        void * stream);
+
+/* hyperloglog::merge (hyperloglog.cpp:136-149) over groups: group g's
+ * registers (d_out + (g << bits)) are the register-wise max of the sketches
+ * d_members[d_group_offsets[g] .. d_group_offsets[g + 1]).  Asynchronous. */
+int hibf_gpu_hll_merge(const uint8_t * d_registers,
+                       uint32_t bits,
+                       const uint64_t * d_members,
+                       const uint64_t * d_group_offsets,
+                       uint64_t n_groups,
+                       uint8_t * d_out,
+                       void * stream);
+
+/* hyperloglog::estimate (hyperloglog.cpp:66-129) of n sketches into
+ * h_estimates (host).  The float sum runs in the reference's eight lanes and
+ * lane order on the device; the division and the small-value correction run
+ * on the host in double.  Synchronous. */
+int hibf_gpu_hll_estimate(const uint8_t * d_registers, uint64_t n_sketches, uint32_t bits, double * h_estimates,
+                          void * stream);
+
+/* ======================================================================== */
 /* Archives: cereal BinaryArchive layout                                    */
 /* ======================================================================== */
 

@@ -112,6 +112,13 @@ def lib():
         L.orc_hibf_membership_count_batch.argtypes = [ctypes.POINTER(OrcHibf), ctypes.c_void_p, ctypes.c_void_p,
                                                       ctypes.c_uint64, ctypes.c_void_p, ctypes.c_void_p,
                                                       ctypes.c_int]
+        L.orc_hll_wyhash.restype = ctypes.c_uint64
+        L.orc_hll_wyhash.argtypes = [ctypes.c_uint64]
+        L.orc_hll_add.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint64]
+        L.orc_hll_merge.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32]
+        L.orc_hll_estimate.restype = ctypes.c_double
+        L.orc_hll_estimate.argtypes = [ctypes.c_void_p, ctypes.c_uint32]
+        L.orc_mt19937_64.argtypes = [ctypes.c_uint64, ctypes.c_void_p, ctypes.c_uint64]
         _lib = L
     return _lib
 
@@ -318,3 +325,37 @@ def kmer_threshold(n_kmers: int, fraction: float) -> int:
     """uint16_t(fraction * number of k-mers) (test/tutorial/search.cpp:42)."""
     t = fraction * n_kmers
     return 65535 if t >= 65535 else max(0, int(t))
+
+
+# ------------------------------------------------- HyperLogLog (build side) --
+# src/sketch/hyperloglog.cpp restated in hibf_oracle.c (orc_hll_*).
+
+def hll_wyhash(value: int) -> int:
+    return int(lib().orc_hll_wyhash(value))
+
+
+def hll_sketch(values, bits: int = 5) -> np.ndarray:
+    """Registers of hyperloglog(bits) after add() of every value."""
+    regs = np.zeros(1 << bits, dtype=np.uint8)
+    v = _u64(values)
+    lib().orc_hll_add(_p(regs), bits, _p(v), v.size)
+    return regs
+
+
+def hll_merge(a: np.ndarray, b: np.ndarray) -> np.ndarray:
+    out = np.ascontiguousarray(a, dtype=np.uint8).copy()
+    other = np.ascontiguousarray(b, dtype=np.uint8)
+    lib().orc_hll_merge(_p(out), _p(other), int(out.size).bit_length() - 1)
+    return out
+
+
+def hll_estimate(regs: np.ndarray) -> float:
+    r = np.ascontiguousarray(regs, dtype=np.uint8)
+    return float(lib().orc_hll_estimate(_p(r), int(r.size).bit_length() - 1))
+
+
+def mt19937_64(seed: int, n: int) -> np.ndarray:
+    """The first n outputs of std::mt19937_64(seed)."""
+    out = np.zeros(n, dtype=np.uint64)
+    lib().orc_mt19937_64(seed, _p(out), n)
+    return out

@@ -24,7 +24,11 @@
  *                          the reference HIBF unit-test expectations, and a
  *                          brute-force tree check (a user bin is reported iff its
  *                          run sum and every ancestor merged-bin sum reach t).
+ *   orc_hll_*              hyperloglog_test.cpp expectations and exact counts
+ *                          (see the HyperLogLog section).
+ *   orc_mt19937_64         the C++ standard's 10,000th output for seed 5489.
  */
+#include <math.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -493,5 +497,122 @@ void orc_hibf_membership_count_batch(orc_hibf const * hb,
                                                         thresholds[q],
                                                         scratch,
                                                         0);
+    }
+}
+
+/* ------------------------------------------------------------------------
+ * HyperLogLog sketches (src/sketch/hyperloglog.cpp), as the reference's build
+ * uses them for user-bin cardinalities (compute_sketches.cpp:27-43,
+ * estimate_kmer_counts.hpp).  Pinned by the reference's hyperloglog_test.cpp
+ * expectations (the 8-value estimate 7.899522493 at 5 bits with the stated
+ * index/rank of each value, the empty estimate 0, merge == add, the 1,500
+ * mt19937_64 values within the error bound) and, for the 2^bits >= 128 branch
+ * of alpha, by the mean relative error of many sketches against exact counts.
+ * ------------------------------------------------------------------------ */
+
+/* hyperloglog.cpp:48-54: 128-bit product with the golden-ratio constant, low
+ * half xor high half. */
+uint64_t orc_hll_wyhash(uint64_t value)
+{
+    unsigned __int128 const r = (unsigned __int128)value * 0x9E3779B97F4A7C15ULL;
+    return (uint64_t)r ^ (uint64_t)(r >> 64);
+}
+
+/* hyperloglog.cpp:56-64: register index = top `bits` bits of the hash; rank =
+ * leading zeros of the remaining bits (capped at 64 - bits) + 1; keep the max. */
+void orc_hll_add(uint8_t * regs, uint32_t bits, uint64_t const * values, uint64_t n)
+{
+    uint64_t const rank_mask = (bits >= 64u) ? ~0ULL : ((1ULL << bits) - 1u);
+    for (uint64_t i = 0; i < n; ++i)
+    {
+        uint64_t const h = orc_hll_wyhash(values[i]);
+        uint64_t const idx = h >> (64u - bits);
+        uint64_t const rest = (h << bits) | rank_mask;
+        uint8_t const rank = (uint8_t)(__builtin_clzll(rest) + 1);
+        if (rank > regs[idx])
+            regs[idx] = rank;
+    }
+}
+
+/* hyperloglog.cpp:136-149: register-wise max. */
+void orc_hll_merge(uint8_t * regs, uint8_t const * other, uint32_t bits)
+{
+    uint64_t const size = 1ULL << bits;
+    for (uint64_t i = 0; i < size; ++i)
+        if (other[i] > regs[i])
+            regs[i] = other[i];
+}
+
+/* hyperloglog.cpp:66-129.  The float sum runs in 8 lanes: for each 32-register
+ * block and j in {0, 8, 16, 24}, lane L adds 2^-reg[j + 7 - L] (the argument
+ * order of _mm256_set_ps puts register j + 7 in lane 0); the lanes are then
+ * added in order 0..7.  alpha: 0.697 (32 registers), 0.709 (64), otherwise
+ * 0.7213 / (1 + 1.079 / m) (hyperloglog.cpp:31-46).  Small-value correction:
+ * m * ln(m / zeros) when the raw estimate is <= 2.5 m and a register is 0. */
+double orc_hll_estimate(uint8_t const * regs, uint32_t bits)
+{
+    uint64_t const size = 1ULL << bits;
+    double alpha;
+    if (size == 32u)
+        alpha = 0.697;
+    else if (size == 64u)
+        alpha = 0.709;
+    else
+        alpha = 0.7213 / (1.0 + 1.079 / (double)size);
+    double const norm = alpha * (double)size * (double)size;
+    float lane[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (uint64_t b = 0; b < size / 32u; ++b)
+    {
+        uint8_t const * const cur = regs + 32u * b;
+        for (uint32_t j = 0; j <= 24u; j += 8u)
+            for (uint32_t L = 0; L < 8u; ++L)
+                lane[L] = lane[L] + 1.0f / (float)(1ULL << cur[j + 7u - L]);
+    }
+    float sum = 0.0f;
+    for (uint32_t L = 0; L < 8u; ++L)
+        sum = sum + lane[L];
+    double estimate = norm / (double)sum;
+    if (estimate <= 2.5 * (double)size)
+    {
+        uint32_t zeros = 0;
+        for (uint64_t i = 0; i < size; ++i)
+            zeros += regs[i] == 0u;
+        if (zeros != 0u)
+            estimate = (double)size * log((double)size / (double)zeros);
+    }
+    return estimate;
+}
+
+/* std::mt19937_64 (the C++ standard's parameters, [rand.predef]) seeded with
+ * `seed`; the reference's HLL tests draw their inputs from it through
+ * uniform_int_distribution<uint64_t> over the full range, which returns the
+ * engine's outputs unchanged. */
+void orc_mt19937_64(uint64_t seed, uint64_t * out, uint64_t n)
+{
+    enum { NN = 312, MM = 156 };
+    uint64_t mt[NN];
+    uint64_t const matrix_a = 0xB5026F5AA96619E9ULL;
+    uint64_t const upper = 0xFFFFFFFF80000000ULL, lower = 0x7FFFFFFFULL;
+    mt[0] = seed;
+    for (int i = 1; i < NN; ++i)
+        mt[i] = 6364136223846793005ULL * (mt[i - 1] ^ (mt[i - 1] >> 62)) + (uint64_t)i;
+    int idx = NN;
+    for (uint64_t k = 0; k < n; ++k)
+    {
+        if (idx >= NN)
+        {
+            for (int i = 0; i < NN; ++i)
+            {
+                uint64_t const x = (mt[i] & upper) | (mt[(i + 1) % NN] & lower);
+                mt[i] = mt[(i + MM) % NN] ^ (x >> 1) ^ ((x & 1u) ? matrix_a : 0u);
+            }
+            idx = 0;
+        }
+        uint64_t y = mt[idx++];
+        y ^= (y >> 29) & 0x5555555555555555ULL;
+        y ^= (y << 17) & 0x71D67FFFEDA60000ULL;
+        y ^= (y << 37) & 0xFFF7EEE000000000ULL;
+        y ^= y >> 43;
+        out[k] = y;
     }
 }
