@@ -367,6 +367,58 @@ def run_flat(args):
     print(json.dumps(out), flush=True)
 
 
+def hibf_sketch_leg(sizes, spec, args):
+    """HyperLogLog sketches (12 bits, the config default) of the synthetic user
+    bins on the device, as the reference's build computes them before the
+    layout (compute_sketches + estimate_kmer_counts).  Returns the leg's record
+    (None without args) and the truncated estimates."""
+    import torch
+    from hibf_amd.device import hll_estimates, hll_sketches_synthetic, to_numpy
+    bits = 12
+    hll_sketches_synthetic(sizes[:64], spec["build_seed"], bits)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    regs = hll_sketches_synthetic(sizes, spec["build_seed"], bits)
+    sketch_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    est = hll_estimates(regs)
+    est_s = time.perf_counter() - t0
+    counts = est.astype(np.uint64)
+    total = float(sizes.sum())
+    log(f"(f) sketches: {sizes.size} user bins, {total:.3e} values in {sketch_s * 1e3:.1f} ms "
+        f"({total / sketch_s / 1e9:.1f} G values/s), estimates {est_s * 1e3:.1f} ms; "
+        f"median relative error {np.median(np.abs(counts / sizes - 1)):.4f}")
+    if args is None:
+        return None, counts
+    leg = {"user_bins": int(sizes.size), "values": total, "bits": bits, "seconds": sketch_s,
+           "values_per_s": total / sketch_s, "estimate_seconds": est_s,
+           "median_relative_error": float(np.median(np.abs(counts / sizes - 1)))}
+    if args.check_queries > 0 or not args.skip_cpu:
+        import oracle as O
+        from hibf_amd.synth import synth_values
+        rng = np.random.default_rng(args.seed + 31)
+        small = np.nonzero(sizes <= 1_000_000)[0]
+        pick = rng.choice(small, size=min(12, small.size), replace=False)
+        host = to_numpy(regs[torch.as_tensor(pick.astype(np.int64), device=regs.device)])
+        ok = True
+        for row, u in zip(host, pick):
+            want = O.hll_sketch(synth_values(spec["build_seed"], int(u), np.arange(int(sizes[u]), dtype=np.uint64)),
+                                bits)
+            ok = ok and np.array_equal(row, want) and float(est[u]) == O.hll_estimate(want)
+        leg["parity_sample_ok"] = bool(ok)
+        if not args.skip_cpu:
+            u = int(pick[0])
+            vals = synth_values(spec["build_seed"], u, np.arange(int(sizes[u]), dtype=np.uint64))
+            reps = max(1, int(2e7 // max(vals.size, 1)))
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                O.hll_sketch(vals, bits)
+            cpu_s = time.perf_counter() - t0
+            leg["cpu_baseline"] = {"value": reps * vals.size / cpu_s, "unit": "values/s", "cores": 1, "kind": "port",
+                                   "sample": f"oracle hyperloglog add of user bin {u} ({vals.size} values) x {reps}"}
+    return leg, counts
+
+
 def run_hibf(args):
     """32,768-user-bin HIBF: (a) 500k x 250 queries, t = 0.7 L, one persistent
     launch; (b) 2M queries of log-normal length in hipGraph-captured batches of
@@ -384,10 +436,12 @@ def run_hibf(args):
     spec = W.HIBF_AT_SCALE
     n_ub = args.hibf_user_bins
     sizes = W.hibf_user_bin_sizes(spec["size_seed"], n_ub)
+    # (f) build side: one HyperLogLog sketch per user bin on the device; the layout sees their estimates.
+    sketch_leg, counts = hibf_sketch_leg(sizes, spec, args)
     t0 = time.perf_counter()
-    layout = W.cached_layout(sizes, spec["config"])
+    layout = W.cached_layout(counts, spec["config"])
     layout_s = time.perf_counter() - t0
-    log(f"layout ({n_ub} user bins, {sizes.sum():.3e} values): {layout_s:.1f} s")
+    log(f"layout ({n_ub} user bins, {sizes.sum():.3e} values, HyperLogLog estimates): {layout_s:.1f} s")
     t0 = time.perf_counter()
     hibf = DeviceHibf.build_synthetic(sizes, spec["build_seed"], layout_text=layout, **spec["config"])
     build_s = time.perf_counter() - t0
@@ -671,6 +725,7 @@ def run_hibf(args):
         "parity_sample_ok": parity,
         "build_s": build_s,
         "layout_s": layout_s,
+        "sketches": sketch_leg,
     }
     print(json.dumps(out), flush=True)
 
@@ -687,7 +742,7 @@ def run_serve(args):
     spec = W.HIBF_AT_SCALE
     n_ub = args.hibf_user_bins
     sizes = W.hibf_user_bin_sizes(spec["size_seed"], n_ub)
-    layout = W.cached_layout(sizes, spec["config"])
+    layout = W.cached_layout(hibf_sketch_leg(sizes, spec, None)[1], spec["config"])
     hibf = DeviceHibf.build_synthetic(sizes, spec["build_seed"], layout_text=layout, **spec["config"])
     tab = hibf.tables()
     rates = [float(r) for r in args.serve_rates.split(",")]

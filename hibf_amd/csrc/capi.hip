@@ -1271,12 +1271,63 @@ int copy_text(std::string const & t, char * out, uint64_t cap, uint64_t * needed
     return HIBF_GPU_OK;
 }
 
+// Device allocation freed when the owner goes out of scope.
+struct DeviceBuffer
+{
+    void * p{nullptr};
+    DeviceBuffer() = default;
+    DeviceBuffer(DeviceBuffer const &) = delete;
+    DeviceBuffer & operator=(DeviceBuffer const &) = delete;
+    ~DeviceBuffer()
+    {
+        if (p)
+            (void)hipFree(p);
+    }
+    hipError_t alloc(size_t bytes) { return hipMalloc(&p, std::max<size_t>(bytes, 16u)); }
+    uint8_t * u8() const { return static_cast<uint8_t *>(p); }
+    uint64_t * u64() const { return static_cast<uint64_t *>(p); }
+};
+
+// User-bin cardinalities as the reference's build computes them for the
+// layout: one HyperLogLog sketch of sketch_bits per user bin, each estimate
+// truncated to an integer (src/hierarchical_interleaved_bloom_filter.cpp:
+// 215-220; compute_sketches.cpp:27-43; estimate_kmer_counts.hpp).  Values come
+// from the CSR on the device or, when d_values is null, from the synthetic
+// stream of `seed` with h_sizes[u] values per user bin.
+int estimate_kmer_counts(host::Config const & hc,
+                         uint64_t const * d_values,
+                         uint64_t const * d_offsets,
+                         uint64_t const * h_sizes,
+                         uint64_t seed,
+                         std::vector<uint64_t> & counts)
+{
+    uint64_t const n = hc.number_of_user_bins;
+    uint32_t const bits = static_cast<uint32_t>(hc.sketch_bits);
+    counts.assign(n, 0u);
+    DeviceBuffer regs;
+    HIBF_TRY_HIP(regs.alloc(static_cast<size_t>(n) << bits), "sketch registers");
+    if (d_values)
+    {
+        if (int rc = hibf_gpu_hll_sketch(d_values, d_offsets, n, bits, regs.u8(), nullptr))
+            return rc;
+    }
+    else if (int rc = hll_sketch_synthetic(seed, h_sizes, n, bits, regs.u8(), nullptr))
+        return rc;
+    std::vector<double> est(n);
+    if (int rc = hll_estimates(regs.u8(), n, bits, est.data(), nullptr))
+        return rc;
+    for (uint64_t u = 0; u < n; ++u)
+        counts[u] = static_cast<uint64_t>(est[u]);
+    return HIBF_GPU_OK;
+}
+
 // Executes a build plan on the device: allocates the HIBF (zeroed arena) and
-// runs the segment emplace over explicit values or the synthetic stream.
+// runs the segment emplace over explicit values (device CSR) or the synthetic
+// stream.
 int run_plan(host::Plan const & plan,
              uint64_t n_user_bins,
-             uint64_t const * h_ub_offsets,
-             uint64_t const * h_values,
+             uint64_t const * d_ub_offsets,
+             uint64_t const * d_values,
              uint64_t seed,
              hibf_gpu_hibf * out)
 {
@@ -1309,11 +1360,9 @@ int run_plan(host::Plan const & plan,
             chunks.push_back({static_cast<uint32_t>(s), static_cast<uint32_t>(std::min<uint64_t>(4096u, plan.segments[s].count - b)), b});
     void * d_segs = nullptr;
     void * d_chunks = nullptr;
-    void * d_vals = nullptr;
-    void * d_offs = nullptr;
     auto cleanup = [&]()
     {
-        for (void * p : {d_segs, d_chunks, d_vals, d_offs})
+        for (void * p : {d_segs, d_chunks})
             if (p)
                 (void)hipFree(p);
     };
@@ -1324,21 +1373,10 @@ int run_plan(host::Plan const & plan,
         e = hipMalloc(&d_chunks, std::max<size_t>(chunks.size(), 1) * sizeof(EmplaceChunk));
     if (e == hipSuccess)
         e = hipMemcpy(d_chunks, chunks.data(), chunks.size() * sizeof(EmplaceChunk), hipMemcpyHostToDevice);
-    if (e == hipSuccess && h_values)
-    {
-        uint64_t const nv = h_ub_offsets[n_user_bins];
-        e = hipMalloc(&d_vals, std::max<uint64_t>(nv, 1) * 8u);
-        if (e == hipSuccess)
-            e = hipMemcpy(d_vals, h_values, nv * 8u, hipMemcpyHostToDevice);
-        if (e == hipSuccess)
-            e = hipMalloc(&d_offs, (n_user_bins + 1) * 8u);
-        if (e == hipSuccess)
-            e = hipMemcpy(d_offs, h_ub_offsets, (n_user_bins + 1) * 8u, hipMemcpyHostToDevice);
-    }
     if (e == hipSuccess)
         e = launch_segment_emplace(h->d_desc, h->d_arena, static_cast<EmplaceSegment *>(d_segs),
-                                   static_cast<EmplaceChunk *>(d_chunks), chunks.size(),
-                                   static_cast<uint64_t *>(d_vals), static_cast<uint64_t *>(d_offs), seed, nullptr);
+                                   static_cast<EmplaceChunk *>(d_chunks), chunks.size(), d_values, d_ub_offsets, seed,
+                                   nullptr);
     if (e == hipSuccess)
         e = hipDeviceSynchronize();
     cleanup();
@@ -1450,14 +1488,32 @@ int hibf_gpu_hibf_build(hibf_gpu_config const * c,
             std::sort(all.begin(), all.end());
             return static_cast<uint64_t>(std::unique(all.begin(), all.end()) - all.begin());
         };
+        DeviceBuffer d_vals, d_offs;
+        HIBF_TRY_HIP(d_vals.alloc(vals.size() * 8u), "build values");
+        HIBF_TRY_HIP(hipMemcpy(d_vals.p, vals.data(), vals.size() * 8u, hipMemcpyHostToDevice), "build values");
+        HIBF_TRY_HIP(d_offs.alloc(offs.size() * 8u), "build offsets");
+        HIBF_TRY_HIP(hipMemcpy(d_offs.p, offs.data(), offs.size() * 8u, hipMemcpyHostToDevice), "build offsets");
+        // The layout sees HyperLogLog estimates, as in the reference; IBF sizes use the exact sets
+        // (the reference sizes each IBF from the k-mers it inserts).
         host::Layout layout;
-        if (int rc = get_layout(hc, in.ub_size, layout_text, layout))
-            return rc;
+        if (layout_text)
+        {
+            if (int rc = get_layout(hc, in.ub_size, layout_text, layout))
+                return rc;
+        }
+        else
+        {
+            std::vector<uint64_t> kmer_counts;
+            if (int rc = estimate_kmer_counts(hc, d_vals.u64(), d_offs.u64(), nullptr, 0, kmer_counts))
+                return rc;
+            if (int rc = get_layout(hc, kmer_counts, nullptr, layout))
+                return rc;
+        }
         host::Plan plan;
         std::string const pe = host::build_plan(hc, layout, in, plan);
         if (!pe.empty())
             return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, pe);
-        return run_plan(plan, nub, offs.data(), vals.data(), 0, out);
+        return run_plan(plan, nub, d_offs.u64(), d_vals.u64(), 0, out);
     }
     catch (std::exception const & ex)
     {
@@ -1540,8 +1596,19 @@ int hibf_gpu_hibf_build_synthetic(hibf_gpu_config const * c,
             if (s >= (1ULL << 36))
                 return fail(HIBF_GPU_ERR_UNSUPPORTED, "the synthetic stream supports < 2^36 values per user bin");
         host::Layout layout;
-        if (int rc = get_layout(hc, in.ub_size, layout_text, layout))
-            return rc;
+        if (layout_text)
+        {
+            if (int rc = get_layout(hc, in.ub_size, layout_text, layout))
+                return rc;
+        }
+        else
+        {
+            std::vector<uint64_t> kmer_counts;
+            if (int rc = estimate_kmer_counts(hc, nullptr, nullptr, in.ub_size.data(), seed, kmer_counts))
+                return rc;
+            if (int rc = get_layout(hc, kmer_counts, nullptr, layout))
+                return rc;
+        }
         host::Plan plan;
         std::string const pe = host::build_plan(hc, layout, in, plan);
         if (!pe.empty())

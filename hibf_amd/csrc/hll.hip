@@ -419,6 +419,23 @@ int hibf_gpu_hll_sketch(uint64_t const * d_values,
     return run_sketches(chunks, n_sketches, d_values, 0, false, bits, d_registers, s);
 }
 
+int hibf_gpu_hll_sketch_synthetic(uint64_t seed,
+                                  uint64_t const * h_sizes,
+                                  uint64_t n_sketches,
+                                  uint32_t bits,
+                                  uint8_t * d_registers,
+                                  void * stream)
+{
+    if (int rc = check_bits(bits))
+        return rc;
+    if ((!h_sizes || !d_registers) && n_sketches)
+        return hibf_amd::set_error(HIBF_GPU_ERR_INVALID_ARGUMENT, "null argument");
+    if (n_sketches == 0)
+        return HIBF_GPU_OK;
+    return hibf_amd::hll_sketch_synthetic(seed, h_sizes, n_sketches, bits, d_registers,
+                                          static_cast<hipStream_t>(stream));
+}
+
 int hibf_gpu_hll_merge(uint8_t const * d_registers,
                        uint32_t bits,
                        uint64_t const * d_members,

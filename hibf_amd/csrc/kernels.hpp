@@ -266,4 +266,11 @@ namespace hibf_amd
 // hipCUB inclusive sum; call with temp == nullptr to size temp_bytes.
 hipError_t inclusive_sum_u64(void * temp, size_t & temp_bytes, uint64_t const * in, uint64_t * out, uint64_t n,
                              hipStream_t stream);
+// HyperLogLog sketches (hll.hip): registers of the synthetic user bins
+// (synth_value(seed, u, i), i < h_sizes[u]) and the estimates of n sketches on
+// the device (synchronous).
+int hll_sketch_synthetic(uint64_t seed, uint64_t const * h_sizes, uint64_t n, uint32_t bits, uint8_t * d_registers,
+                         hipStream_t s);
+int hll_estimates(uint8_t const * d_registers, uint64_t n, uint32_t bits, double * h_estimates, hipStream_t s);
+
 } // namespace hibf_amd

@@ -103,6 +103,15 @@ def hll_sketches(sets, bits: int = 12) -> torch.Tensor:
     return regs[:n]
 
 
+def hll_sketches_synthetic(sizes, seed: int, bits: int = 12) -> torch.Tensor:
+    """Sketches of the synthetic user bins of DeviceHibf.build_synthetic(sizes, seed)."""
+    s = np.ascontiguousarray(np.asarray(sizes, dtype=np.uint64))
+    regs = torch.empty((max(s.size, 1), 1 << bits), dtype=torch.uint8, device=_dev())
+    check(lib().hibf_gpu_hll_sketch_synthetic(int(seed), s.ctypes.data_as(ctypes.c_void_p), s.size, bits, _ptr(regs),
+                                              _stream()))
+    return regs[: s.size]
+
+
 def hll_merge(registers: torch.Tensor, groups) -> torch.Tensor:
     """Register-wise max of each group of sketches (hibf_gpu_hll_merge);
     `groups` is a list of lists of sketch rows."""

@@ -130,6 +130,7 @@ _SIGNATURES = {
     "hibf_gpu_kmer_hash": (_INT, [_V, _V, _U64, _U32, _V, _V, _U64, ctypes.POINTER(_U64), _V]),
     "hibf_gpu_kmer_thresholds": (_INT, [_V, _U64, ctypes.c_double, _V, _V]),
     "hibf_gpu_hll_sketch": (_INT, [_V, _V, _U64, ctypes.c_uint32, _V, _V]),
+    "hibf_gpu_hll_sketch_synthetic": (_INT, [_U64, _V, _U64, ctypes.c_uint32, _V, _V]),
     "hibf_gpu_hll_merge": (_INT, [_V, ctypes.c_uint32, _V, _V, _U64, _V, _V]),
     "hibf_gpu_hll_estimate": (_INT, [_V, _U64, ctypes.c_uint32, _V, _V]),
     "hibf_gpu_ibf_save": (_INT, [_V, ctypes.c_char_p, _INT]),

@@ -428,6 +428,16 @@ int hibf_gpu_hll_sketch(const uint64_t * d_values,
                         uint8_t * d_registers,
                         void * stream);
 
+/* As hibf_gpu_hll_sketch for the synthetic user bins of
+ * hibf_gpu_hibf_build_synthetic: set u is the h_sizes[u] values of the
+ * library's seeded generator (no counterpart in the reference).  Synchronous. */
+int hibf_gpu_hll_sketch_synthetic(uint64_t seed,
+                                  const uint64_t * h_sizes,
+                                  uint64_t n_sketches,
+                                  uint32_t bits,
+                                  uint8_t * d_registers,
+                                  void * stream);
+
 /* hyperloglog::merge (hyperloglog.cpp:136-149) over groups: group g's
  * registers (d_out + (g << bits)) are the register-wise max of the sketches
  * d_members[d_group_offsets[g] .. d_group_offsets[g + 1]).  Asynchronous. */

@@ -356,6 +356,39 @@ def _build_cached_deleted(ubs):
     return _DELETED_CACHE["t"]
 
 
+def _same_hibf(a, b):
+    ta, tb = a.tables(), b.tables()
+    assert ta["geoms"] == tb["geoms"]
+    for k in ("word_offsets", "tb_offsets", "next_ibf", "user_bin"):
+        assert np.array_equal(ta[k], tb[k]), k
+    assert np.array_equal(a.download_arena(), b.download_arena())
+
+
+def test_build_layout_sees_hyperloglog_estimates():
+    """The builders hand the layout the user bins' HyperLogLog estimates, as
+    the reference's build does (src/hierarchical_interleaved_bloom_filter.cpp:
+    215-220): each build equals the build from the layout of the oracle's
+    estimates (12-bit sketches), for explicit and for synthetic user bins."""
+    from hibf_amd import native
+    from hibf_amd.device import DeviceHibf
+    from hibf_amd.synth import synth_values
+    _say("layout from HLL estimates")
+    rng = np.random.default_rng(17)
+    cfg = dict(tmax=64)
+    ubs = _random_user_bins(rng, 300, 200, 20000)
+    counts = [int(O.hll_estimate(O.hll_sketch(np.unique(u), 12))) for u in ubs]
+    assert any(c != np.unique(u).size for c, u in zip(counts, ubs))   # the estimates are not the exact counts
+    text = native.compute_layout(native.make_config(number_of_user_bins=len(ubs), **cfg), counts)
+    _same_hibf(DeviceHibf.build(ubs, **cfg), DeviceHibf.build(ubs, layout_text=text, **cfg))
+    sizes = rng.integers(100, 30000, size=500)
+    seed = 5
+    counts = [int(O.hll_estimate(O.hll_sketch(synth_values(seed, u, np.arange(n, dtype=np.uint64)), 12)))
+              for u, n in enumerate(sizes)]
+    text = native.compute_layout(native.make_config(number_of_user_bins=len(sizes), **cfg), counts)
+    _same_hibf(DeviceHibf.build_synthetic(sizes, seed, **cfg),
+               DeviceHibf.build_synthetic(sizes, seed, layout_text=text, **cfg))
+
+
 def test_hibf_synthetic_build_matches_oracle():
     from hibf_amd.device import DeviceHibf, to_numpy
     from hibf_amd.synth import synth_values
