@@ -1304,6 +1304,8 @@ int estimate_kmer_counts(host::Config const & hc,
     uint64_t const n = hc.number_of_user_bins;
     uint32_t const bits = static_cast<uint32_t>(hc.sketch_bits);
     counts.assign(n, 0u);
+    if (n > (~0ULL >> bits))
+        return fail(HIBF_GPU_ERR_UNSUPPORTED, "number_of_user_bins x 2^sketch_bits sketch registers exceed 2^64 bytes");
     DeviceBuffer regs;
     HIBF_TRY_HIP(regs.alloc(static_cast<size_t>(n) << bits), "sketch registers");
     if (d_values)

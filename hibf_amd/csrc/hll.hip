@@ -256,6 +256,14 @@ int check_bits(uint32_t bits)
     return HIBF_GPU_OK;
 }
 
+// n sketches of 2^bits registers must be addressable in 64 bits.
+int check_size(uint64_t n, uint32_t bits)
+{
+    if (n > (~0ULL >> bits))
+        return hibf_amd::set_error(HIBF_GPU_ERR_UNSUPPORTED, "number of sketches x 2^bits registers exceeds 2^64 bytes");
+    return HIBF_GPU_OK;
+}
+
 unsigned grid_for(uint64_t n, uint64_t cap)
 {
     return static_cast<unsigned>(std::max<uint64_t>(1u, std::min<uint64_t>(n, cap)));
@@ -396,6 +404,8 @@ int hibf_gpu_hll_sketch(uint64_t const * d_values,
 {
     if (int rc = check_bits(bits))
         return rc;
+    if (int rc = check_size(n_sketches, bits))
+        return rc;
     if ((!d_offsets || !d_registers) && n_sketches)
         return hibf_amd::set_error(HIBF_GPU_ERR_INVALID_ARGUMENT, "null argument");
     if (n_sketches == 0)
@@ -428,6 +438,8 @@ int hibf_gpu_hll_sketch_synthetic(uint64_t seed,
 {
     if (int rc = check_bits(bits))
         return rc;
+    if (int rc = check_size(n_sketches, bits))
+        return rc;
     if ((!h_sizes || !d_registers) && n_sketches)
         return hibf_amd::set_error(HIBF_GPU_ERR_INVALID_ARGUMENT, "null argument");
     if (n_sketches == 0)
@@ -446,6 +458,8 @@ int hibf_gpu_hll_merge(uint8_t const * d_registers,
 {
     if (int rc = check_bits(bits))
         return rc;
+    if (int rc = check_size(n_groups, bits))
+        return rc;
     if ((!d_registers || !d_members || !d_group_offsets || !d_out) && n_groups)
         return hibf_amd::set_error(HIBF_GPU_ERR_INVALID_ARGUMENT, "null argument");
     if (n_groups == 0)
@@ -460,6 +474,8 @@ int hibf_gpu_hll_estimate(uint8_t const * d_registers, uint64_t n_sketches, uint
                           void * stream)
 {
     if (int rc = check_bits(bits))
+        return rc;
+    if (int rc = check_size(n_sketches, bits))
         return rc;
     if ((!d_registers || !h_estimates) && n_sketches)
         return hibf_amd::set_error(HIBF_GPU_ERR_INVALID_ARGUMENT, "null argument");
