@@ -1,5 +1,6 @@
 // HIBF descent kernels for gfx950 (membership_for and bulk_count), plus the
 // small helpers that turn per-query hit lists into CSR output.
+#include "descent_common.hpp"
 #include "device_common.hpp"
 #include "kernels.hpp"
 
@@ -8,22 +9,6 @@ namespace hibf_amd
 
 namespace
 {
-
-__device__ __forceinline__ uint64_t width_mask(uint32_t counter_bytes)
-{
-    return counter_bytes >= 8u ? ~0ULL : ((1ULL << (8u * counter_bytes)) - 1ULL);
-}
-
-__device__ __forceinline__ void store_counter(void * base, uint64_t i, uint32_t counter_bytes, uint64_t v)
-{
-    switch (counter_bytes)
-    {
-    case 1: static_cast<uint8_t *>(base)[i] = static_cast<uint8_t>(v); break;
-    case 2: static_cast<uint16_t *>(base)[i] = static_cast<uint16_t>(v); break;
-    case 4: static_cast<uint32_t *>(base)[i] = static_cast<uint32_t>(v); break;
-    default: static_cast<uint64_t *>(base)[i] = v; break;
-    }
-}
 
 // Shared state of one block's depth-first descent.
 struct DescentState
@@ -34,78 +19,6 @@ struct DescentState
     uint32_t hits;                  // ids found for the current query
     uint64_t start;                 // output slot of the current query
 };
-
-// Scan the technical bins of the IBF just counted into `cnt` (wave 0 only),
-// exactly as the loop of hierarchical_interleaved_bloom_filter.hpp:329-354:
-// the running sum adds every bin's count (deleted bins included), resets at a
-// merged bin and at the end of a user bin's run, and a reset point is an event
-// iff the sum (modulo 2^(8*counter_bytes)) reaches the threshold.  Events are
-// appended to ev in technical-bin order.  In counting mode user-bin events are
-// written straight to the output row and only merged bins become events.
-template <bool COUNTING>
-__device__ void scan_ibf(HibfIbfDesc const & d,
-                         uint64_t const * __restrict__ tb_info,
-                         uint32_t const * cnt,
-                         uint64_t threshold,
-                         uint32_t counter_bytes,
-                         uint32_t * ev,
-                         uint32_t & nev_out,
-                         void * count_row,
-                         int lane)
-{
-    uint64_t const mask = width_mask(counter_bytes);
-    uint64_t carry = 0;
-    uint32_t nev = 0;
-    for (uint32_t c0 = 0; c0 < d.bins; c0 += 64u)
-    {
-        uint32_t const tb = c0 + static_cast<uint32_t>(lane);
-        bool const valid = tb < d.bins;
-        uint64_t const info = valid ? tb_info[d.tb_offset + tb] : 0ULL;
-        uint32_t const kind = static_cast<uint32_t>(info >> 32);
-        uint32_t const payload = static_cast<uint32_t>(info);
-        bool const reset = valid && (kind == kTbRunEnd || kind == kTbMerged);
-        uint64_t s = valid ? (static_cast<uint64_t>(cnt[tb]) & mask) : 0ULL;
-        // Segment heads: the bin after a reset point.
-        bool const prev_reset = __shfl_up(static_cast<int>(reset), 1, 64) != 0;
-        bool f = lane > 0 && prev_reset;
-#pragma unroll
-        for (int dlt = 1; dlt < 64; dlt <<= 1)
-        {
-            uint64_t const y = shfl64(s, lane >= dlt ? lane - dlt : lane);
-            bool const fy = __shfl(static_cast<int>(f), lane >= dlt ? lane - dlt : lane, 64) != 0;
-            if (lane >= dlt)
-            {
-                if (!f)
-                    s += y;
-                f = f || fy;
-            }
-        }
-        if (!f)
-            s += carry;
-        uint64_t const sum = s & mask;
-        bool const hit = reset && sum >= threshold;
-        bool is_event = hit;
-        if constexpr (COUNTING)
-        {
-            // A user bin whose run is cut (by a deleted bin) keeps the sum of its last run that reached the
-            // threshold (hierarchical_interleaved_bloom_filter.hpp:488-491): of two such run ends in this chunk,
-            // the higher bin stores.
-            if (highest_lane_per_id(hit && kind == kTbRunEnd, payload, lane))
-                store_counter(count_row, payload, counter_bytes, sum);
-            is_event = hit && kind == kTbMerged;
-        }
-        uint64_t const ballot = __ballot(is_event);
-        uint32_t const before = static_cast<uint32_t>(__popcll(ballot & ((1ULL << lane) - 1ULL)));
-        if (is_event)
-            ev[nev + before] = tb;
-        nev += static_cast<uint32_t>(__popcll(ballot));
-        // Pending sum after this chunk: lane 63's segment sum unless it is a reset point.
-        uint64_t const last_s = shfl64(s, 63);
-        bool const last_reset = __shfl(static_cast<int>(reset), 63, 64) != 0;
-        carry = last_reset ? 0ULL : last_s;
-    }
-    nev_out = nev;
-}
 
 // One block per query (grid-stride).  Depth-first descent from IBF 0 with an
 // explicit stack of per-level event lists in LDS, which reproduces the
@@ -175,33 +88,7 @@ __device__ __forceinline__ void descent_body(HibfLaunch const & h,
                                     : nullptr;
         if constexpr (COUNTING)
         {
-            // counting_vector fill(0) (hierarchical_interleaved_bloom_filter.hpp:561): 16-byte
-            // stores over the 16-byte-aligned middle of the row, counter stores at its ends.
-            char * const bytes = static_cast<char *>(row);
-            uint64_t const len = h.n_user_bins * cb;
-            uint64_t const head = (16u - (reinterpret_cast<uintptr_t>(bytes) & 15u)) & 15u;
-            uint64_t const n16 = len > head ? (len - head) / 16u : 0u;
-            uint64_t const tail = head + n16 * 16u;
-            // head and tail are multiples of cb: the row starts at a multiple of cb, and cb divides 16.
-            if (h.nt_stores)
-            {
-                typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
-                u32x4 * const mid = reinterpret_cast<u32x4 *>(bytes + head);
-                for (uint64_t i = static_cast<uint64_t>(tid); i < n16; i += static_cast<uint64_t>(nthreads))
-                    __builtin_nontemporal_store(u32x4{0u, 0u, 0u, 0u}, mid + i);
-            }
-            else
-            {
-                uint4 * const mid = reinterpret_cast<uint4 *>(bytes + head);
-                for (uint64_t i = static_cast<uint64_t>(tid); i < n16; i += static_cast<uint64_t>(nthreads))
-                    mid[i] = make_uint4(0u, 0u, 0u, 0u);
-            }
-            uint64_t const n_head = std::min<uint64_t>(head, len) / cb;
-            for (uint64_t i = static_cast<uint64_t>(tid); i < n_head; i += static_cast<uint64_t>(nthreads))
-                store_counter(row, i, cb, 0ULL);
-            for (uint64_t i = std::max<uint64_t>(tail / cb, n_head) + static_cast<uint64_t>(tid); i < h.n_user_bins;
-                 i += static_cast<uint64_t>(nthreads))
-                store_counter(row, i, cb, 0ULL);
+            zero_counter_row(row, h.n_user_bins, cb, h.nt_stores != 0u, tid, nthreads);
         }
         if (tid == 0)
             st.hits = 0;
@@ -230,8 +117,8 @@ __device__ __forceinline__ void descent_body(HibfLaunch const & h,
                 __syncthreads();
                 if (wave == 0)
                 {
-                    uint32_t n = 0;
-                    scan_ibf<COUNTING>(d, h.tb_info, cnt, threshold, cb, evs + depth * h.max_bins, n, row, lane);
+                    uint32_t const n =
+                        scan_ibf<COUNTING>(d, h.tb_info, cnt, threshold, cb, evs + depth * h.max_bins, row, lane);
                     if (lane == 0)
                     {
                         st.nev[depth] = n;

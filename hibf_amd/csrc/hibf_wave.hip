@@ -14,6 +14,7 @@
 // Membership ids of a query are buffered in LDS (kWaveResults entries); a
 // query with more hits is recorded in an overflow list and answered again by
 // the block-per-query kernel, which keeps a full-size list per block.
+#include "descent_common.hpp"
 #include "device_common.hpp"
 #include "kernels.hpp"
 
@@ -29,22 +30,6 @@ __device__ __forceinline__ void wave_sync()
 {
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
-}
-
-__device__ __forceinline__ uint64_t width_mask(uint32_t counter_bytes)
-{
-    return counter_bytes >= 8u ? ~0ULL : ((1ULL << (8u * counter_bytes)) - 1ULL);
-}
-
-__device__ __forceinline__ void store_counter(void * base, uint64_t i, uint32_t counter_bytes, uint64_t v)
-{
-    switch (counter_bytes)
-    {
-    case 1: static_cast<uint8_t *>(base)[i] = static_cast<uint8_t>(v); break;
-    case 2: static_cast<uint16_t *>(base)[i] = static_cast<uint16_t>(v); break;
-    case 4: static_cast<uint32_t *>(base)[i] = static_cast<uint32_t>(v); break;
-    default: static_cast<uint64_t *>(base)[i] = v; break;
-    }
 }
 
 // Lane-per-value counting for IBFs of BW <= 4 words per row: B batches of
@@ -135,73 +120,6 @@ __device__ __forceinline__ void count_visit(HibfIbfDesc const & d,
     }
 }
 
-// Technical-bin scan of one counted IBF by one wave (same rules as the block
-// kernel's scan: sums modulo 2^(8*counter_bytes), resets at merged bins and
-// run ends, deleted bins carry).  Events are appended to ev in bin order; in
-// counting mode user-bin hits are written to count_row directly.
-template <bool COUNTING>
-__device__ __forceinline__ uint32_t scan_wave(HibfIbfDesc const & d,
-                                              uint64_t const * __restrict__ tb_info,
-                                              uint32_t const * cnt,
-                                              uint64_t threshold,
-                                              uint32_t counter_bytes,
-                                              uint32_t * ev,
-                                              void * count_row,
-                                              int lane)
-{
-    uint64_t const mask = width_mask(counter_bytes);
-    uint64_t carry = 0;
-    uint32_t nev = 0;
-    for (uint32_t c0 = 0; c0 < d.bins; c0 += 64u)
-    {
-        uint32_t const tb = c0 + static_cast<uint32_t>(lane);
-        bool const valid = tb < d.bins;
-        uint64_t const info = valid ? tb_info[d.tb_offset + tb] : 0ULL;
-        uint32_t const kind = static_cast<uint32_t>(info >> 32);
-        uint32_t const payload = static_cast<uint32_t>(info);
-        bool const reset = valid && (kind == kTbRunEnd || kind == kTbMerged);
-        uint64_t s = valid ? (static_cast<uint64_t>(cnt[tb]) & mask) : 0ULL;
-        // The shuffle runs on every lane (a short-circuit would leave lane 0
-        // inactive while lane 1 reads it).
-        bool const prev_reset = __shfl_up(static_cast<int>(reset), 1, 64) != 0;
-        bool f = lane > 0 && prev_reset;
-#pragma unroll
-        for (int dlt = 1; dlt < 64; dlt <<= 1)
-        {
-            uint64_t const y = shfl64(s, lane >= dlt ? lane - dlt : lane);
-            bool const fy = __shfl(static_cast<int>(f), lane >= dlt ? lane - dlt : lane, 64) != 0;
-            if (lane >= dlt)
-            {
-                if (!f)
-                    s += y;
-                f = f || fy;
-            }
-        }
-        if (!f)
-            s += carry;
-        uint64_t const sum = s & mask;
-        bool const hit = reset && sum >= threshold;
-        bool is_event = hit;
-        if constexpr (COUNTING)
-        {
-            // A user bin whose run is cut (by a deleted bin) keeps the sum of its last run that reached the
-            // threshold (hierarchical_interleaved_bloom_filter.hpp:488-491): of two such run ends in this chunk,
-            // the higher bin stores.
-            if (highest_lane_per_id(hit && kind == kTbRunEnd, payload, lane))
-                store_counter(count_row, payload, counter_bytes, sum);
-            is_event = hit && kind == kTbMerged;
-        }
-        uint64_t const ballot = __ballot(is_event);
-        uint32_t const before = static_cast<uint32_t>(__popcll(ballot & ((1ULL << lane) - 1ULL)));
-        if (is_event)
-            ev[nev + before] = tb;
-        nev += static_cast<uint32_t>(__popcll(ballot));
-        uint64_t const last_s = shfl64(s, 63);
-        bool const last_reset = __shfl(static_cast<int>(reset), 63, 64) != 0;
-        carry = last_reset ? 0ULL : last_s;
-    }
-    return nev;
-}
 
 // NARROW: every IBF of the HIBF has at most 4 words per row (checked on the host).
 template <bool COUNTING, bool NARROW, int HF = 0>
@@ -249,8 +167,7 @@ __device__ __forceinline__ void wave_body(HibfLaunch const & h,
                                     : nullptr;
         if constexpr (COUNTING)
         {
-            for (uint64_t i = static_cast<uint64_t>(lane); i < h.n_user_bins; i += 64u)
-                store_counter(row, i, cb, 0ULL);
+            zero_counter_row(row, h.n_user_bins, cb, h.nt_stores != 0u, lane, 64);
         }
         uint32_t hits = 0;
         bool spilled = false;
@@ -271,7 +188,7 @@ __device__ __forceinline__ void wave_body(HibfLaunch const & h,
                 row_bytes += static_cast<uint64_t>(L) * d.hash_funs * d.bin_words * 8u;
                 wave_sync();
                 uint32_t const n =
-                    scan_wave<COUNTING>(d, h.tb_info, cnt, threshold, cb, evs + depth * h.max_bins, row, lane);
+                    scan_ibf<COUNTING>(d, h.tb_info, cnt, threshold, cb, evs + depth * h.max_bins, row, lane);
                 if (lane == 0)
                 {
                     nev[depth] = n;
