@@ -69,7 +69,8 @@ def build_tools(verbose: bool = False) -> list[str]:
     return out
 
 
-CPP_TESTS = [os.path.join(ROOT, "tests", "cpp", "snippets.cpp"), os.path.join(ROOT, "tests", "cpp", "archive_roundtrip.cpp")]
+CPP_TESTS = [os.path.join(ROOT, "tests", "cpp", "snippets.cpp"), os.path.join(ROOT, "tests", "cpp", "archive_roundtrip.cpp"),
+             os.path.join(ROOT, "tests", "cpp", "hibf_members.cpp")]
 
 
 def build_cpp_tests(verbose: bool = False) -> list[str]:

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstdio>
 #include <cstring>
 #include <memory>
@@ -138,6 +139,7 @@ struct hibf_gpu_ibf_s
     uint64_t * d_words{nullptr};
     std::vector<uint64_t> occupancy; // host copy, technical_bins entries (interleaved_bloom_filter.hpp:451-458)
     bool track_occupancy{false};
+    bool owns_words{true}; // false: a view of one IBF in an HIBF's arena
 };
 
 struct hibf_gpu_agent_s
@@ -185,6 +187,9 @@ struct hibf_gpu_hibf_s
     // tables; bulk_count refuses them (its device scan stores an IBF's counts
     // before descending into its merged bins, the reference in bin order).
     bool user_bin_in_two_ibfs{false};
+    // Phase times of the build that made this HIBF (seconds): sketches and
+    // their estimates, layout, allocation (arena and tables), device fill.
+    double build_times[4]{};
     uint64_t * d_arena{nullptr};
     HibfIbfDesc * d_desc{nullptr};
     uint64_t * d_tb_info{nullptr};
@@ -376,7 +381,7 @@ int hibf_gpu_ibf_destroy(hibf_gpu_ibf ibf)
 {
     if (!ibf)
         return HIBF_GPU_OK;
-    hipError_t const e = ibf->d_words ? hipFree(ibf->d_words) : hipSuccess;
+    hipError_t const e = (ibf->d_words && ibf->owns_words) ? hipFree(ibf->d_words) : hipSuccess;
     delete ibf;
     return e == hipSuccess ? HIBF_GPU_OK : hip_fail(e, "hibf_gpu_ibf_destroy");
 }
@@ -496,6 +501,8 @@ int hibf_gpu_ibf_try_increase_bin_number_to(hibf_gpu_ibf ibf, uint64_t new_bins,
 {
     if (!ibf || !increased)
         return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "null argument");
+    if (!ibf->owns_words)
+        return fail(HIBF_GPU_ERR_UNSUPPORTED, "the IBFs of an HIBF keep the bin counts of its routing tables");
     uint64_t const new_bw = (new_bins + 63u) / 64u;
     *increased = 0;
     if (new_bins < ibf->g.bins || new_bw > ibf->g.bin_words) // src/interleaved_bloom_filter.cpp:150-160
@@ -963,6 +970,33 @@ int hibf_gpu_hibf_emplace(hibf_gpu_hibf h,
     return HIBF_GPU_OK;
 }
 
+int hibf_gpu_hibf_build_times(hibf_gpu_hibf h, double * h_seconds)
+{
+    if (!h || !h_seconds)
+        return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "null argument");
+    for (int i = 0; i < 4; ++i)
+        h_seconds[i] = h->build_times[i];
+    return HIBF_GPU_OK;
+}
+
+int hibf_gpu_hibf_ibf_view(hibf_gpu_hibf h, uint64_t i, hibf_gpu_ibf * out)
+{
+    if (!h || !out)
+        return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    if (i >= h->geoms.size())
+        return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "IBF index out of range");
+    auto * v = new (std::nothrow) hibf_gpu_ibf_s{};
+    if (!v)
+        return fail(HIBF_GPU_ERR_OUT_OF_MEMORY, "host allocation failed");
+    v->g = h->geoms[i];
+    v->d_words = h->d_arena + h->word_offsets[i];
+    v->occupancy.assign(v->g.technical_bins, 0u);
+    v->owns_words = false;
+    *out = v;
+    return HIBF_GPU_OK;
+}
+
 int hibf_gpu_hibf_device_arena(hibf_gpu_hibf h, uint64_t ** d_arena)
 {
     if (!h || !d_arena)
@@ -1331,8 +1365,10 @@ int run_plan(host::Plan const & plan,
              uint64_t const * d_ub_offsets,
              uint64_t const * d_values,
              uint64_t seed,
+             double const * phase_s,
              hibf_gpu_hibf * out)
 {
+    auto const t_alloc = std::chrono::steady_clock::now();
     uint64_t const n = plan.ibfs.size();
     std::vector<hibf_gpu_ibf_geometry> geoms(n);
     std::vector<uint64_t> word_offsets(n), tb_offsets(n + 1, 0), next_ibf, user_bin;
@@ -1354,6 +1390,7 @@ int run_plan(host::Plan const & plan,
     if (int rc = hibf_gpu_hibf_create(n, geoms.data(), nullptr, word_offsets.data(), tb_offsets.data(),
                                       next_ibf.data(), user_bin.data(), n_user_bins, &h))
         return rc;
+    auto const t_fill = std::chrono::steady_clock::now();
     // Chunk the segments so that one block handles at most 4096 values.
     static_assert(sizeof(EmplaceSegment) == sizeof(host::Segment), "segment layouts differ");
     std::vector<EmplaceChunk> chunks;
@@ -1387,6 +1424,11 @@ int run_plan(host::Plan const & plan,
         hibf_gpu_hibf_destroy(h);
         return hip_fail(e, "hibf build");
     }
+    auto const t_end = std::chrono::steady_clock::now();
+    h->build_times[0] = phase_s[0];
+    h->build_times[1] = phase_s[1];
+    h->build_times[2] = std::chrono::duration<double>(t_fill - t_alloc).count();
+    h->build_times[3] = std::chrono::duration<double>(t_end - t_fill).count();
     *out = h;
     return HIBF_GPU_OK;
 }
@@ -1402,6 +1444,38 @@ int get_layout(host::Config const & hc, std::vector<uint64_t> const & counts, ch
     }
     layout = host::compute_layout(hc, counts);
     return HIBF_GPU_OK;
+}
+
+// The layout of a build: the given text, or the DP over the user bins'
+// HyperLogLog estimates (values from the device CSR, or the synthetic stream
+// of `seed` when d_values is null).  phase_s[0] and [1] get the sketch and
+// layout times.
+int layout_for(host::Config const & hc,
+               std::vector<uint64_t> const & sizes,
+               char const * layout_text,
+               uint64_t const * d_values,
+               uint64_t const * d_offsets,
+               uint64_t seed,
+               host::Layout & layout,
+               double * phase_s)
+{
+    using clock = std::chrono::steady_clock;
+    if (layout_text)
+    {
+        auto const t0 = clock::now();
+        int const rc = get_layout(hc, sizes, layout_text, layout);
+        phase_s[1] = std::chrono::duration<double>(clock::now() - t0).count();
+        return rc;
+    }
+    auto const t0 = clock::now();
+    std::vector<uint64_t> kmer_counts;
+    if (int rc = estimate_kmer_counts(hc, d_values, d_offsets, d_values ? nullptr : sizes.data(), seed, kmer_counts))
+        return rc;
+    auto const t1 = clock::now();
+    int const rc = get_layout(hc, kmer_counts, nullptr, layout);
+    phase_s[0] = std::chrono::duration<double>(t1 - t0).count();
+    phase_s[1] = std::chrono::duration<double>(clock::now() - t1).count();
+    return rc;
 }
 
 } // namespace
@@ -1497,25 +1571,15 @@ int hibf_gpu_hibf_build(hibf_gpu_config const * c,
         HIBF_TRY_HIP(hipMemcpy(d_offs.p, offs.data(), offs.size() * 8u, hipMemcpyHostToDevice), "build offsets");
         // The layout sees HyperLogLog estimates, as in the reference; IBF sizes use the exact sets
         // (the reference sizes each IBF from the k-mers it inserts).
+        double phase_s[2]{};
         host::Layout layout;
-        if (layout_text)
-        {
-            if (int rc = get_layout(hc, in.ub_size, layout_text, layout))
-                return rc;
-        }
-        else
-        {
-            std::vector<uint64_t> kmer_counts;
-            if (int rc = estimate_kmer_counts(hc, d_vals.u64(), d_offs.u64(), nullptr, 0, kmer_counts))
-                return rc;
-            if (int rc = get_layout(hc, kmer_counts, nullptr, layout))
-                return rc;
-        }
+        if (int rc = layout_for(hc, in.ub_size, layout_text, d_vals.u64(), d_offs.u64(), 0, layout, phase_s))
+            return rc;
         host::Plan plan;
         std::string const pe = host::build_plan(hc, layout, in, plan);
         if (!pe.empty())
             return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, pe);
-        return run_plan(plan, nub, d_offs.u64(), d_vals.u64(), 0, out);
+        return run_plan(plan, nub, d_offs.u64(), d_vals.u64(), 0, phase_s, out);
     }
     catch (std::exception const & ex)
     {
@@ -1597,25 +1661,15 @@ int hibf_gpu_hibf_build_synthetic(hibf_gpu_config const * c,
         for (uint64_t s : in.ub_size)
             if (s >= (1ULL << 36))
                 return fail(HIBF_GPU_ERR_UNSUPPORTED, "the synthetic stream supports < 2^36 values per user bin");
+        double phase_s[2]{};
         host::Layout layout;
-        if (layout_text)
-        {
-            if (int rc = get_layout(hc, in.ub_size, layout_text, layout))
-                return rc;
-        }
-        else
-        {
-            std::vector<uint64_t> kmer_counts;
-            if (int rc = estimate_kmer_counts(hc, nullptr, nullptr, in.ub_size.data(), seed, kmer_counts))
-                return rc;
-            if (int rc = get_layout(hc, kmer_counts, nullptr, layout))
-                return rc;
-        }
+        if (int rc = layout_for(hc, in.ub_size, layout_text, nullptr, nullptr, seed, layout, phase_s))
+            return rc;
         host::Plan plan;
         std::string const pe = host::build_plan(hc, layout, in, plan);
         if (!pe.empty())
             return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, pe);
-        return run_plan(plan, nub, nullptr, nullptr, seed, out);
+        return run_plan(plan, nub, nullptr, nullptr, seed, phase_s, out);
     }
     catch (std::exception const & ex)
     {

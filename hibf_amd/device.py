@@ -445,6 +445,20 @@ class DeviceHibf:
             check(lib().hibf_gpu_hibf_emplace(self.h, ibf_idx, _ptr(v), _ptr(b), 0, v.numel(), _stream()))
         torch.cuda.current_stream().synchronize()
 
+    def ibf(self, i: int) -> DeviceIbf:
+        """ibf_vector[i]: a DeviceIbf over IBF i's words in the arena (no copy;
+        it keeps this HIBF alive)."""
+        h = ctypes.c_void_p()
+        check(lib().hibf_gpu_hibf_ibf_view(self.h, int(i), ctypes.byref(h)))
+        view = DeviceIbf.__new__(DeviceIbf)
+        view.h = h
+        g = Geometry()
+        check(lib().hibf_gpu_ibf_get_geometry(view.h, ctypes.byref(g)))
+        view.geometry = g
+        view._agent = None
+        view._owner = self
+        return view
+
     def download_arena(self) -> np.ndarray:
         n = ctypes.c_uint64()
         check(lib().hibf_gpu_hibf_arena_words(self.h, ctypes.byref(n)))

@@ -24,6 +24,7 @@
 #include <hibf/config.hpp>
 #include <hibf/detail/device_buffer.hpp>
 #include <hibf/interleaved_bloom_filter.hpp>
+#include <hibf/misc/timer.hpp>
 #include <hibf/misc/counting_vector.hpp>
 #include <hibf_gpu.h>
 
@@ -113,11 +114,63 @@ public:
     template <std::integral value_t>
     class counting_agent_type;
 
+    // hierarchical_interleaved_bloom_filter.hpp:292-302
+    struct previous_ibf_id_pair
+    {
+        size_t ibf_idx{};
+        size_t bin_idx{};
+
+        friend constexpr auto operator<=>(previous_ibf_id_pair const &, previous_ibf_id_pair const &) = default;
+    };
+
     size_t number_of_user_bins{};
+    // ibf_vector[i] is a view of IBF i in the device arena (no copy).
+    std::vector<interleaved_bloom_filter> ibf_vector;
     std::vector<std::vector<uint64_t>> next_ibf_id;
+    // Parent (IBF, bin) of each IBF, {0, 0} for the root (hierarchical_interleaved_bloom_filter.hpp:220-229).
+    std::vector<previous_ibf_id_pair> prev_ibf_id;
     std::vector<std::vector<uint64_t>> ibf_bin_to_user_bin_id;
 
+    static constexpr uint32_t version{1};
+
+    // Build phases (hierarchical_interleaved_bloom_filter.hpp:283-290).  The device build reports sketches,
+    // layout, allocation and fill; reading the user bins is timed here.  Union estimation and
+    // rearrangement are not performed, merging k-mers is part of the device fill: those stay zero.
+    concurrent_timer layout_compute_sketches_timer{};
+    concurrent_timer layout_union_estimation_timer{};
+    concurrent_timer layout_rearrangement_timer{};
+    concurrent_timer layout_dp_algorithm_timer{};
+    concurrent_timer index_allocation_timer{};
+    concurrent_timer user_bin_io_timer{};
+    concurrent_timer merge_kmers_timer{};
+    concurrent_timer fill_ibf_timer{};
+
     hierarchical_interleaved_bloom_filter() = default;
+
+    // Copies share the device HIBF (its arena and tables stay on the device once); the copy's
+    // ibf_vector views the same arena.
+    hierarchical_interleaved_bloom_filter(hierarchical_interleaved_bloom_filter const & o) { *this = o; }
+    hierarchical_interleaved_bloom_filter & operator=(hierarchical_interleaved_bloom_filter const & o)
+    {
+        if (this == &o)
+            return *this;
+        if (o.handle_)
+            share(o.handle_);
+        else
+            *this = hierarchical_interleaved_bloom_filter{};
+        layout_compute_sketches_timer = o.layout_compute_sketches_timer;
+        layout_union_estimation_timer = o.layout_union_estimation_timer;
+        layout_rearrangement_timer = o.layout_rearrangement_timer;
+        layout_dp_algorithm_timer = o.layout_dp_algorithm_timer;
+        index_allocation_timer = o.index_allocation_timer;
+        user_bin_io_timer = o.user_bin_io_timer;
+        merge_kmers_timer = o.merge_kmers_timer;
+        fill_ibf_timer = o.fill_ibf_timer;
+        return *this;
+    }
+    hierarchical_interleaved_bloom_filter(hierarchical_interleaved_bloom_filter &&) noexcept = default;
+    hierarchical_interleaved_bloom_filter & operator=(hierarchical_interleaved_bloom_filter &&) noexcept = default;
+    ~hierarchical_interleaved_bloom_filter() = default;
 
     // src/hierarchical_interleaved_bloom_filter.cpp:211-244
     hierarchical_interleaved_bloom_filter(config & configuration) { build(configuration, nullptr); }
@@ -158,6 +211,7 @@ private:
         configuration.validate_and_set_defaults();
         std::vector<uint64_t> values, offsets{0};
         std::vector<uint64_t> scratch;
+        user_bin_io_timer.start();
         for (size_t i = 0; i < configuration.number_of_user_bins; ++i)
         {
             scratch.clear();
@@ -165,17 +219,29 @@ private:
             values.insert(values.end(), scratch.begin(), scratch.end());
             offsets.push_back(values.size());
         }
+        user_bin_io_timer.stop();
         hibf_gpu_config const c = configuration.to_c();
         hibf_gpu_hibf h = nullptr;
         detail::check(hibf_gpu_hibf_build(&c, offsets.data(), values.empty() ? offsets.data() : values.data(),
                                           layout_text ? layout_text->c_str() : nullptr, &h));
         adopt(h);
+        double phase[4]{};
+        detail::check(hibf_gpu_hibf_build_times(h, phase));
+        if (!layout_text)
+            layout_compute_sketches_timer.add_seconds(phase[0]);
+        layout_dp_algorithm_timer.add_seconds(phase[1]);
+        index_allocation_timer.add_seconds(phase[2]);
+        fill_ibf_timer.add_seconds(phase[3]);
     }
 
     // Takes ownership of a device HIBF and mirrors its routing tables.
-    void adopt(hibf_gpu_hibf h)
+    void adopt(hibf_gpu_hibf h) { share(std::make_shared<detail::hibf_handle>(h)); }
+
+    // Mirrors the routing tables of a (shared) device HIBF and views its IBFs.
+    void share(std::shared_ptr<detail::hibf_handle> handle)
     {
-        handle_ = std::make_shared<detail::hibf_handle>(h);
+        handle_ = std::move(handle);
+        hibf_gpu_hibf const h = handle_->h;
         uint64_t n_ibfs = 0, n_tbs = 0, n_ub = 0;
         detail::check(hibf_gpu_hibf_info(h, &n_ibfs, &n_tbs, nullptr, &n_ub, nullptr));
         number_of_user_bins = n_ub;
@@ -184,10 +250,19 @@ private:
         detail::check(hibf_gpu_hibf_export_tables(h, geoms.data(), wo.data(), to.data(), nx.data(), ub.data()));
         next_ibf_id.assign(n_ibfs, {});
         ibf_bin_to_user_bin_id.assign(n_ibfs, {});
+        prev_ibf_id.assign(n_ibfs, previous_ibf_id_pair{});
+        ibf_vector.clear();
+        ibf_vector.reserve(n_ibfs);
         for (uint64_t i = 0; i < n_ibfs; ++i)
         {
             next_ibf_id[i].assign(nx.begin() + to[i], nx.begin() + to[i + 1]);
             ibf_bin_to_user_bin_id[i].assign(ub.begin() + to[i], ub.begin() + to[i + 1]);
+            for (uint64_t b = 0; b < to[i + 1] - to[i]; ++b)
+                if (ub[to[i] + b] == HIBF_GPU_BIN_MERGED)
+                    prev_ibf_id[nx[to[i] + b]] = previous_ibf_id_pair{i, b};
+            hibf_gpu_ibf view = nullptr;
+            detail::check(hibf_gpu_hibf_ibf_view(h, i, &view));
+            ibf_vector.push_back(interleaved_bloom_filter::view_of(view, handle_));
         }
     }
 

@@ -65,7 +65,8 @@ std::vector<uint64_t> to_u64(range_t && values)
 struct ibf_handle
 {
     hibf_gpu_ibf h{nullptr};
-    explicit ibf_handle(hibf_gpu_ibf x) : h{x} {}
+    std::shared_ptr<void const> owner{}; // what a view's words belong to (an HIBF), kept alive
+    explicit ibf_handle(hibf_gpu_ibf x, std::shared_ptr<void const> o = {}) : h{x}, owner{std::move(o)} {}
     ~ibf_handle() { hibf_gpu_ibf_destroy(h); }
 };
 } // namespace detail
@@ -257,6 +258,18 @@ public:
 
     hibf_gpu_ibf handle() const noexcept { return handle_ ? handle_->h : nullptr; }
     hibf_gpu_ibf_geometry const & geometry() const noexcept { return geometry_; }
+
+    // An IBF over words it does not own: hierarchical_interleaved_bloom_filter::ibf_vector[i]
+    // (hibf_gpu_hibf_ibf_view).  `owner` is kept alive as long as the view; copying the view
+    // duplicates the matrix, as for any IBF.  Its bin count cannot change.
+    static interleaved_bloom_filter view_of(hibf_gpu_ibf view, std::shared_ptr<void const> owner)
+    {
+        interleaved_bloom_filter ibf;
+        ibf.handle_ = std::make_shared<detail::ibf_handle>(view, std::move(owner));
+        detail::check(hibf_gpu_ibf_get_geometry(view, &ibf.geometry_));
+        ibf.occupancy.assign(ibf.geometry_.technical_bins, 0u);
+        return ibf;
+    }
 
     // Archive in the byte layout a cereal::BinaryOutputArchive writes for the
     // reference's object (serialize, interleaved_bloom_filter.hpp:471-485);

@@ -261,6 +261,20 @@ int hibf_gpu_hibf_arena_words(hibf_gpu_hibf hibf, uint64_t * n_words);
 /* Device address of the arena (IBF i's words start at word_offsets[i]). */
 int hibf_gpu_hibf_device_arena(hibf_gpu_hibf hibf, uint64_t ** d_arena);
 
+/* Phase times of the build that made this HIBF, in seconds (all 0 for an HIBF
+ * created from tables or loaded): [0] user-bin sketches and their estimates,
+ * [1] layout, [2] allocation of the arena and tables, [3] filling the IBFs on
+ * the device.  They feed the reference's timers
+ * (hierarchical_interleaved_bloom_filter.hpp:283-290). */
+int hibf_gpu_hibf_build_times(hibf_gpu_hibf hibf, double * h_seconds);
+
+/* ibf_vector[i] (hierarchical_interleaved_bloom_filter.hpp:209): an IBF handle
+ * whose bit matrix is IBF i's words in the HIBF's arena (no copy).  Queries,
+ * emplace, clear, upload and download act on the arena; the bin count cannot
+ * change (try_/increase_bin_number_to return HIBF_GPU_ERR_UNSUPPORTED).
+ * Destroying the view frees only the handle; it must not outlive the HIBF. */
+int hibf_gpu_hibf_ibf_view(hibf_gpu_hibf hibf, uint64_t i, hibf_gpu_ibf * out);
+
 /* Batched HIBF membership_agent_type::membership_for
  * (hierarchical_interleaved_bloom_filter.hpp:321-355, 424-439): user-bin ids in
  * the reference's unsorted depth-first order.  Same output contract as

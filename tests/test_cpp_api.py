@@ -15,7 +15,7 @@ def test_cpp_headers_compile(tmp_path):
     """Headers compile standalone with the host compiler (no HIP headers needed)."""
     for hdr in ("hibf/interleaved_bloom_filter.hpp", "hibf/hierarchical_interleaved_bloom_filter.hpp",
                 "hibf/config.hpp", "hibf/misc/counting_vector.hpp", "hibf/misc/bit_vector.hpp",
-                "hibf/misc/print.hpp", "hibf_gpu.h"):
+                "hibf/misc/print.hpp", "hibf/misc/timer.hpp", "hibf_gpu.h"):
         src = tmp_path / "t.cpp"
         src.write_text(f"#include <{hdr}>\nint main() {{}}\n")
         r = subprocess.run(["g++", "-std=c++20", "-fsyntax-only", f"-I{ROOT}/include", str(src)],
@@ -42,3 +42,15 @@ def test_cpp_store_and_load(tmp_path):
     assert r.returncode == 0, r.stderr
     lines = r.stdout.splitlines()
     assert len(lines) == 6 and all(line.endswith(" same") for line in lines), r.stdout
+
+
+@pytest.mark.gpu
+def test_cpp_hibf_public_members():
+    """ibf_vector (views of the arena), prev_ibf_id, version, the build
+    timers and copy semantics of the C++ HIBF class."""
+    exe = os.path.join(ROOT, "tests", "cpp", "hibf_members")
+    assert os.path.exists(exe), "build tests/cpp/hibf_members first (python -m hibf_amd.build)"
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    lines = r.stdout.splitlines()
+    assert len(lines) == 7 and all(line.endswith(" ok") for line in lines), r.stdout
