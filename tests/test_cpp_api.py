@@ -54,3 +54,64 @@ def test_cpp_hibf_public_members():
     assert r.returncode == 0, r.stderr
     lines = r.stdout.splitlines()
     assert len(lines) == 7 and all(line.endswith(" ok") for line in lines), r.stdout
+
+
+def _config_io(tmp_path):
+    exe = tmp_path / "config_io"
+    r = subprocess.run(["g++", "-std=c++20", "-O1", f"-I{ROOT}/include", os.path.join(ROOT, "tests", "cpp", "config_io.cpp"),
+                        "-o", str(exe)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    return str(exe)
+
+
+def _read(exe, text):
+    r = subprocess.run([exe, "read"], input=text, capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    fields, rest = {}, []
+    for line in r.stdout.splitlines():
+        k, _, v = line.partition(" ")
+        if k == "rest":
+            rest.append(v)
+        else:
+            fields[k] = v
+    return fields, rest
+
+
+def _want(d):
+    return {k: float(v) for k, v in d.items()}
+
+
+def _num(fields):
+    return {k: float(v) for k, v in fields.items()}
+
+
+def test_config_write_to_and_read_from(tmp_path):
+    """config::write_to prints the reference's expected block byte for byte
+    and read_from parses versions 1-3 as the reference's config_test.cpp
+    expects (version 1: no empty_bin_fraction; version 2: track_occupancy
+    follows empty_bin_fraction != 0), leaving the stream after the block."""
+    import json
+    g = json.load(open(os.path.join(ROOT, "tests", "golden", "config_header.json")))
+    exe = _config_io(tmp_path)
+    args = [f"{k}={str(v).lower() if isinstance(v, bool) else v}" for k, v in g["write_to_config"].items()]
+    r = subprocess.run([exe, "write", *args], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0 and r.stdout == g["write_to_expected"]
+
+    want = _want(g["read_from_expected"])
+    got, rest = _read(exe, "@blah some chopper stuff\n" + g["read_from_input"] + "#TOP_LEVEL_IBF fullest_technical_bin_idx:0\n")
+    assert _num(got) == want and rest == ["#TOP_LEVEL_IBF fullest_technical_bin_idx:0"]
+
+    v2 = g["read_from_input"].replace('"version": 3', '"version": 2').replace('@        "track_occupancy": true,\n', "")
+    got, _ = _read(exe, v2)
+    assert _num(got) == want   # track_occupancy from empty_bin_fraction 0.5
+
+    v1 = v2.replace('"version": 2', '"version": 1').replace('@        "empty_bin_fraction": 0.5,\n', "")
+    got, _ = _read(exe, v1)
+    assert _num(got) == dict(want, empty_bin_fraction=0.0, track_occupancy=0.0)
+
+    # write_to -> read_from round trip keeps every field
+    r = subprocess.run([exe, "write", *args, "empty_bin_fraction=0.25", "track_occupancy=true"], capture_output=True,
+                       text=True, timeout=60)
+    got, _ = _read(exe, r.stdout)
+    assert _num(got) == dict(_want(g["write_to_config"]), relaxed_fpr=0.3, empty_bin_fraction=0.25,
+                             track_occupancy=1.0)
