@@ -24,6 +24,7 @@
 #include <hibf/config.hpp>
 #include <hibf/detail/device_buffer.hpp>
 #include <hibf/interleaved_bloom_filter.hpp>
+#include <hibf/layout/layout.hpp>
 #include <hibf/misc/timer.hpp>
 #include <hibf/misc/counting_vector.hpp>
 #include <hibf_gpu.h>
@@ -37,21 +38,6 @@ static constexpr uint64_t merged{HIBF_GPU_BIN_MERGED};
 static constexpr uint64_t deleted{HIBF_GPU_BIN_DELETED};
 } // namespace bin_kind
 
-namespace layout
-{
-// The reference's layout text (src/layout/layout.cpp:60-145), kept as text.
-struct layout
-{
-    std::string text;
-    void read_from(std::istream & stream)
-    {
-        std::ostringstream os;
-        os << stream.rdbuf();
-        text = os.str();
-    }
-    void write_to(std::ostream & stream) const { stream << text; }
-};
-} // namespace layout
 
 namespace detail
 {
@@ -178,7 +164,10 @@ public:
     // src/hierarchical_interleaved_bloom_filter.cpp:246-253
     hierarchical_interleaved_bloom_filter(config & configuration, layout::layout const & l)
     {
-        build(configuration, &l.text);
+        std::ostringstream text;
+        l.write_to(text);
+        std::string const s = text.str();
+        build(configuration, &s);
     }
 
     membership_agent_type membership_agent() const;

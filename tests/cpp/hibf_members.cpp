@@ -5,6 +5,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <iostream>
+#include <sstream>
 #include <string>
 #include <vector>
 
@@ -84,6 +85,37 @@ int main()
 
     report("bin_count_fixed", !hibf.ibf_vector[0].try_increase_bin_number_to(sh::bin_count{hibf.ibf_vector[0].bin_count() + 1u}));
     report("version", hibf_t::version == 1u);
+
+    // hierarchical_interleaved_bloom_filter_test.cpp:59-106: config and layout read from one stream.
+    {
+        std::vector<std::vector<size_t>> hashes{{1u, 2u, 3u, 4u, 5u, 6u, 7u, 8u, 9u, 10u}, {1u, 2u, 3u, 4u, 5u}};
+        std::stringstream stream{"@HIBF_CONFIG\n@{\n@    \"hibf_config\": {\n@        \"version\": 1,\n"
+                                 "@        \"number_of_user_bins\": 2,\n@        \"number_of_hash_functions\": 2,\n"
+                                 "@        \"maximum_fpr\": 0.05,\n@        \"relaxed_fpr\": 0.3,\n"
+                                 "@        \"threads\": 1,\n@        \"sketch_bits\": 12,\n@        \"tmax\": 64,\n"
+                                 "@        \"alpha\": 1.2,\n@        \"max_rearrangement_ratio\": 0.5,\n"
+                                 "@        \"disable_estimate_union\": false,\n"
+                                 "@        \"disable_rearrangement\": true\n@    }\n@}\n@HIBF_CONFIG_END\n"
+                                 "#TOP_LEVEL_IBF fullest_technical_bin_idx:0\n"
+                                 "#USER_BIN_IDX\tTECHNICAL_BIN_INDICES\tNUMBER_OF_TECHNICAL_BINS\n"
+                                 "1\t0\t34\n0\t34\t30\n"};
+        sh::config cfg{};
+        cfg.read_from(stream);
+        cfg.input_fn = [&](size_t const num, sh::insert_iterator it)
+        {
+            for (auto const h : hashes[num])
+                it = h;
+        };
+        sh::layout::layout l{};
+        l.read_from(stream);
+        hibf_t h{cfg, l};
+        auto agent = h.membership_agent();
+        std::vector<size_t> query{1u, 2u, 3u, 4u, 5u};
+        auto & result = agent.membership_for(query, 2u);
+        agent.sort_results();
+        report("build_from_layout", result == std::vector<uint64_t>{0u, 1u} && h.number_of_user_bins == 2u
+                                        && l.user_bins.size() == 2u && l.user_bins[0].number_of_technical_bins == 34u);
+    }
     report("timers", hibf.fill_ibf_timer.in_seconds() > 0.0 && hibf.index_allocation_timer.in_seconds() > 0.0
                          && hibf.layout_compute_sketches_timer.in_seconds() > 0.0
                          && hibf.layout_dp_algorithm_timer.in_seconds() > 0.0

@@ -15,7 +15,7 @@ def test_cpp_headers_compile(tmp_path):
     """Headers compile standalone with the host compiler (no HIP headers needed)."""
     for hdr in ("hibf/interleaved_bloom_filter.hpp", "hibf/hierarchical_interleaved_bloom_filter.hpp",
                 "hibf/config.hpp", "hibf/misc/counting_vector.hpp", "hibf/misc/bit_vector.hpp",
-                "hibf/misc/print.hpp", "hibf/misc/timer.hpp", "hibf_gpu.h"):
+                "hibf/misc/print.hpp", "hibf/misc/timer.hpp", "hibf/layout/layout.hpp", "hibf_gpu.h"):
         src = tmp_path / "t.cpp"
         src.write_text(f"#include <{hdr}>\nint main() {{}}\n")
         r = subprocess.run(["g++", "-std=c++20", "-fsyntax-only", f"-I{ROOT}/include", str(src)],
@@ -47,13 +47,14 @@ def test_cpp_store_and_load(tmp_path):
 @pytest.mark.gpu
 def test_cpp_hibf_public_members():
     """ibf_vector (views of the arena), prev_ibf_id, version, the build
-    timers and copy semantics of the C++ HIBF class."""
+    timers and copy semantics of the C++ HIBF class, and the reference's
+    build_from_layout test (config and layout read from one stream)."""
     exe = os.path.join(ROOT, "tests", "cpp", "hibf_members")
     assert os.path.exists(exe), "build tests/cpp/hibf_members first (python -m hibf_amd.build)"
     r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     lines = r.stdout.splitlines()
-    assert len(lines) == 7 and all(line.endswith(" ok") for line in lines), r.stdout
+    assert len(lines) == 8 and all(line.endswith(" ok") for line in lines), r.stdout
 
 
 def _config_io(tmp_path):
@@ -115,3 +116,36 @@ def test_config_write_to_and_read_from(tmp_path):
     got, _ = _read(exe, r.stdout)
     assert _num(got) == dict(_want(g["write_to_config"]), relaxed_fpr=0.3, empty_bin_fraction=0.25,
                              track_occupancy=1.0)
+
+
+LAYOUT_FILE = ("#TOP_LEVEL_IBF fullest_technical_bin_idx:111\n"
+               "#LOWER_LEVEL_IBF_0 fullest_technical_bin_idx:0\n"
+               "#LOWER_LEVEL_IBF_2 fullest_technical_bin_idx:2\n"
+               "#LOWER_LEVEL_IBF_1;2;3;4 fullest_technical_bin_idx:22\n"
+               "#USER_BIN_IDX\tTECHNICAL_BIN_INDICES\tNUMBER_OF_TECHNICAL_BINS\n"
+               "7\t0\t1\n"
+               "4\t1;0\t1;22\n"
+               "5\t1;2;3;4;22\t1;1;1;1;21\n")   # test/unit/hibf/layout/layout_test.cpp:59-68
+
+
+def test_layout_records_read_and_write(tmp_path):
+    """layout::layout read_from/write_to/clear against the reference's
+    layout_test.cpp expectations: the records of its layout file, the same text
+    written back, and a partitioned file (three layouts in a row, the second
+    and third with another fullest bin of LOWER_LEVEL_IBF_0) read one by one."""
+    exe = tmp_path / "layout_io"
+    r = subprocess.run(["g++", "-std=c++20", "-O1", f"-I{ROOT}/include", os.path.join(ROOT, "tests", "cpp", "layout_io.cpp"),
+                        "-o", str(exe)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    parts = [LAYOUT_FILE.replace("IBF_0 fullest_technical_bin_idx:0", f"IBF_0 fullest_technical_bin_idx:{i}")
+             for i in range(3)]
+    r = subprocess.run([str(exe)], input="".join(parts), capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    out = r.stdout
+    for i, part in enumerate(parts):
+        records = (f"top 111\nmax 0 {i}\nmax 2 2\nmax 1;2;3;4 22\n"
+                   "ub 7 - 0 1\nub 4 1 0 22\nub 5 1;2;3;4 22 21\n")
+        block = records + part + "cleared 1 0\n"
+        assert out.startswith(block), (i, out[:400])
+        out = out[len(block):]
+    assert out == ""
