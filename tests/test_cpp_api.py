@@ -15,7 +15,8 @@ def test_cpp_headers_compile(tmp_path):
     """Headers compile standalone with the host compiler (no HIP headers needed)."""
     for hdr in ("hibf/interleaved_bloom_filter.hpp", "hibf/hierarchical_interleaved_bloom_filter.hpp",
                 "hibf/config.hpp", "hibf/misc/counting_vector.hpp", "hibf/misc/bit_vector.hpp",
-                "hibf/misc/print.hpp", "hibf/misc/timer.hpp", "hibf/layout/layout.hpp", "hibf_gpu.h"):
+                "hibf/misc/print.hpp", "hibf/misc/timer.hpp", "hibf/layout/layout.hpp", "hibf/sketch/hyperloglog.hpp",
+                "hibf/sketch/compute_sketches.hpp", "hibf_gpu.h"):
         src = tmp_path / "t.cpp"
         src.write_text(f"#include <{hdr}>\nint main() {{}}\n")
         r = subprocess.run(["g++", "-std=c++20", "-fsyntax-only", f"-I{ROOT}/include", str(src)],
@@ -149,3 +150,48 @@ def test_layout_records_read_and_write(tmp_path):
         assert out.startswith(block), (i, out[:400])
         out = out[len(block):]
     assert out == ""
+
+
+def test_cpp_hyperloglog_matches_oracle(tmp_path):
+    """The host seqan::hibf::sketch::hyperloglog class gives the oracle's
+    registers and, bit for bit, its estimates (which the reference's HLL test
+    expectations pin), including the 7.899522493 known answer."""
+    import numpy as np
+    import oracle as O
+    exe = tmp_path / "hll_host"
+    r = subprocess.run(["g++", "-std=c++20", "-O2", f"-I{ROOT}/include", os.path.join(ROOT, "tests", "cpp", "hll_host.cpp"),
+                        "-o", str(exe)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    rng = np.random.default_rng(2)
+    small = [255881241332063154, 13132817195163223578, 5120631300412165844, 16862690419523416066,
+             148518882728022940, 15892358469365346306, 10885195586503739779, 9563173945158404745]
+    cases = [(5, small), (5, []), (6, O.mt19937_64(0, 1500).tolist())]
+    for bits in (7, 12, 14):
+        for n in (1, 100, 5000, 60000):
+            cases.append((bits, rng.integers(0, 2**63, size=n, dtype=np.int64).astype(np.uint64).tolist()))
+    text = "".join(f"{b} {len(v)} {' '.join(str(int(x)) for x in v)}\n" for b, v in cases)
+    r = subprocess.run([str(exe)], input=text, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    lines = r.stdout.splitlines()
+    assert len(lines) == len(cases)
+    for (bits, v), line in zip(cases, lines):
+        regs_hex, est = line.split()
+        want = O.hll_sketch(np.asarray(v, dtype=np.uint64), bits)
+        assert regs_hex == want.tobytes().hex()
+        assert float(est) == O.hll_estimate(want)
+    assert abs(float(lines[0].split()[1]) - 7.899522493) < 1e-7
+
+
+@pytest.mark.gpu
+def test_cpp_compute_sketches_on_device():
+    """seqan::hibf::sketch::compute_sketches (device) equals the host class's add()."""
+    import numpy as np
+    exe = os.path.join(ROOT, "tests", "cpp", "hll_device")
+    assert os.path.exists(exe), "build tests/cpp/hll_device first (python -m hibf_amd.build)"
+    rng = np.random.default_rng(4)
+    sizes = [0, 1, 50, 70000, 1000, 3]
+    text = "".join(f"12 {n} {' '.join(str(int(x)) for x in rng.integers(0, 2**63, size=n, dtype=np.int64))}\n"
+                   for n in sizes)
+    r = subprocess.run([exe, "device"], input=text, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert r.stdout.splitlines()[-1] == f"device same {len(sizes)}"
