@@ -769,10 +769,12 @@ def run_serve(args):
                    "p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
                    "sustained_qps": nq / st["seconds"], "sustained_lookups_per_s": st["lookups"] / st["seconds"],
                    "batches": st["batches"], "mean_batch": nq / max(st["batches"], 1),
-                   "max_in_flight_mb": st["max_in_flight_bytes"] / 2**20, "overflowed": st["overflowed"]}
+                   "max_in_flight_mb": st["max_in_flight_bytes"] / 2**20, "overflowed": st["overflowed"],
+                   "longest_pass_ms": st["longest_pass_s"] * 1e3, "longest_launch_ms": st["longest_launch_s"] * 1e3}
             log(f"serve rate {rate:.0f} q/s window {win:.0f} us: p50 {row['p50_ms']:.3f} ms p99 {row['p99_ms']:.3f} ms, "
                 f"{row['sustained_qps']:.0f} q/s, {row['sustained_lookups_per_s'] / 1e9:.2f} G lookups/s, "
-                f"{st['batches']} batches (mean {row['mean_batch']:.1f})")
+                f"{st['batches']} batches (mean {row['mean_batch']:.1f}); longest loop pass "
+                f"{row['longest_pass_ms']:.2f} ms, longest batch launch {row['longest_launch_ms']:.2f} ms")
             rows.append(row)
         if args.check_queries > 0:
             import oracle as O

@@ -184,6 +184,90 @@ extern "C" int hibf_gpu_hibf_serve(hibf_gpu_hibf hibf,
         (void)hibf_gpu_agent_stats(s.agent, &v, &lk, &rb, 1);
     }
 
+    // Warm the copy paths on every stream before the clock starts: the HIP runtime sets up a path the
+    // first time it copies a size class, and that took about 10 ms inside the loop (a batch's latency).
+    // Each slot copies every size from 64 bytes up to its buffers' capacity, in both directions.
+    {
+        uint64_t const * const src_vals = pinned ? h_values : nullptr;
+        for (Slot & s : slots)
+        {
+            uint64_t * const h_src = src_vals ? const_cast<uint64_t *>(src_vals) : s.h_vals;
+            uint64_t const in_bytes = std::min<uint64_t>(V * 8u, pinned ? h_offsets[nq] * 8u : V * 8u);
+            uint64_t const out_bytes = B * R * 8u;
+            for (uint64_t b = 64u;; b *= 4u)
+            {
+                uint64_t const bi = std::min(b, in_bytes), bo = std::min(b, out_bytes);
+                if (bi)
+                    SERVE_TRY(hipMemcpyAsync(s.d_vals, h_src, bi, hipMemcpyHostToDevice, s.stream));
+                if (bo)
+                    SERVE_TRY(hipMemcpyAsync(s.h_ids, s.d_ids, bo, hipMemcpyDeviceToHost, s.stream));
+                if (b >= in_bytes && b >= out_bytes)
+                    break;
+            }
+            SERVE_TRY(hipMemcpyAsync(s.d_offs, s.h_offs, (B + 1) * 8u, hipMemcpyHostToDevice, s.stream));
+            SERVE_TRY(hipMemcpyAsync(s.d_thr, s.h_thr, B * 2u, hipMemcpyHostToDevice, s.stream));
+            SERVE_TRY(hipMemcpyAsync(s.h_hs, s.d_hs, B * 8u, hipMemcpyDeviceToHost, s.stream));
+            SERVE_TRY(hipStreamSynchronize(s.stream));
+        }
+        // A pinned input is read once by the device, chunk by chunk: the first device read of a region of
+        // a freshly pinned buffer took about 10 ms in the loop (the region's first batch paid it).
+        if (pinned && h_offsets[nq] > 0)
+        {
+            uint64_t const total = h_offsets[nq] * 8u;
+            uint64_t const chunk = V * 8u;
+            size_t k = 0;
+            for (uint64_t off = 0; off < total; off += chunk, ++k)
+            {
+                Slot & s = slots[k % slots.size()];
+                SERVE_TRY(hipMemcpyAsync(s.d_vals, reinterpret_cast<char const *>(h_values) + off,
+                                         std::min(chunk, total - off), hipMemcpyHostToDevice, s.stream));
+            }
+            for (Slot & s : slots)
+                SERVE_TRY(hipStreamSynchronize(s.stream));
+        }
+        // Then many one-query batches (the first request, results discarded) queued on all streams at
+        // once, as the loop queues them, so that the runtime's per-operation resources have grown to
+        // the loop's depth before any request is timed.
+        if (nq > 0 && h_offsets[1] - h_offsets[0] <= V)
+        {
+            uint64_t const n0 = h_offsets[1] - h_offsets[0];
+            for (int round = 0; round < 64; ++round)
+                for (Slot & s : slots)
+                {
+                    s.h_offs[0] = 0;
+                    s.h_offs[1] = n0;
+                    s.h_thr[0] = h_thresholds[0];
+                    if (n0 && pinned)
+                        SERVE_TRY(hipMemcpyAsync(s.d_vals, h_values + h_offsets[0], n0 * 8u, hipMemcpyHostToDevice,
+                                                 s.stream));
+                    else if (n0)
+                    {
+                        std::memcpy(s.h_vals, h_values + h_offsets[0], n0 * 8u);
+                        SERVE_TRY(hipMemcpyAsync(s.d_vals, s.h_vals, n0 * 8u, hipMemcpyHostToDevice, s.stream));
+                    }
+                    SERVE_TRY(hipMemcpyAsync(s.d_offs, s.h_offs, 2u * 8u, hipMemcpyHostToDevice, s.stream));
+                    SERVE_TRY(hipMemcpyAsync(s.d_thr, s.h_thr, 2u, hipMemcpyHostToDevice, s.stream));
+                    if ((rc = hibf_gpu_hibf_membership_launch(hibf, s.agent, s.d_vals, s.d_offs, 1u, s.d_thr, s.d_ids,
+                                                              R, s.d_hs, s.d_hc, s.stream)) != HIBF_GPU_OK)
+                    {
+                        cleanup();
+                        return rc;
+                    }
+                    SERVE_TRY(hipMemcpyAsync(s.h_hs, s.d_hs, 8u, hipMemcpyDeviceToHost, s.stream));
+                    SERVE_TRY(hipMemcpyAsync(s.h_hc, s.d_hc, 8u, hipMemcpyDeviceToHost, s.stream));
+                    SERVE_TRY(hipMemcpyAsync(s.h_ids, s.d_ids, R * 8u, hipMemcpyDeviceToHost, s.stream));
+                    // Every round writes the same staging contents, so rounds need not wait for each other.
+                    SERVE_TRY(hipEventRecord(s.done, s.stream));
+                }
+        }
+        for (Slot & s : slots)
+        {
+            SERVE_TRY(hipStreamSynchronize(s.stream));
+            uint64_t v = 0, lk = 0, rb = 0;
+            (void)hibf_gpu_agent_stats(s.agent, &v, &lk, &rb, 1);
+        }
+    }
+
     // Touch the result arrays before the clock starts: first-touch page faults
     // inside the loop would be charged to the requests that hit them.
     std::memset(h_latency_s, 0, nq * sizeof(double));
@@ -200,9 +284,14 @@ extern "C" int hibf_gpu_hibf_serve(hibf_gpu_hibf hibf,
     double const window = p.window_us * 1e-6;
     auto const t0 = std::chrono::steady_clock::now();
     double last_done = 0.0;
+    double longest_pass = 0.0;
+    double longest_launch = 0.0;
+    double pass_start = 0.0;
     while (completed < nq)
     {
         double const now = now_s(t0);
+        longest_pass = std::max(longest_pass, now - pass_start);
+        pass_start = now;
         // Completed batches: results back on the host.
         for (Slot & s : slots)
         {
@@ -262,6 +351,7 @@ extern "C" int hibf_gpu_hibf_serve(hibf_gpu_hibf hibf,
                 auto free_slot = std::find_if(slots.begin(), slots.end(), [](Slot const & s) { return !s.busy; });
                 if (free_slot != slots.end())
                 {
+                    double const launch_start = now_s(t0);
                     Slot & s = *free_slot;
                     uint64_t const nb = q1 - pending;
                     uint64_t const base = h_offsets[pending];
@@ -298,6 +388,7 @@ extern "C" int hibf_gpu_hibf_serve(hibf_gpu_hibf hibf,
                     s.busy = true;
                     pending = q1;
                     ++batches;
+                    longest_launch = std::max(longest_launch, now_s(t0) - launch_start);
                 }
             }
         }
@@ -306,7 +397,10 @@ extern "C" int hibf_gpu_hibf_serve(hibf_gpu_hibf hibf,
             // Idle until the next arrival.
             double const wait = h_arrival_s[next] - now_s(t0);
             if (wait > 50e-6)
+            {
                 std::this_thread::sleep_for(std::chrono::duration<double>(wait - 20e-6));
+                pass_start = now_s(t0); // an idle wait is not a stall
+            }
         }
     }
     if (stats)
@@ -326,6 +420,8 @@ extern "C" int hibf_gpu_hibf_serve(hibf_gpu_hibf hibf,
         stats->row_bytes = row_bytes;
         stats->max_in_flight_bytes = max_in_flight;
         stats->overflowed = overflowed;
+        stats->longest_pass_s = longest_pass;
+        stats->longest_launch_s = longest_launch;
     }
     cleanup();
     return HIBF_GPU_OK;

@@ -68,7 +68,8 @@ class ServeParams(ctypes.Structure):
 class ServeStats(ctypes.Structure):
     _fields_ = [("seconds", ctypes.c_double), ("batches", ctypes.c_uint64), ("lookups", ctypes.c_uint64),
                 ("row_bytes", ctypes.c_uint64), ("max_in_flight_bytes", ctypes.c_uint64),
-                ("overflowed", ctypes.c_uint64)]
+                ("overflowed", ctypes.c_uint64), ("longest_pass_s", ctypes.c_double),
+                ("longest_launch_s", ctypes.c_double)]
 
 
 class Geometry(ctypes.Structure):

@@ -344,6 +344,8 @@ typedef struct hibf_gpu_serve_stats
     uint64_t row_bytes;
     uint64_t max_in_flight_bytes; /* peak admitted device memory */
     uint64_t overflowed;          /* requests with more than results_per_query ids */
+    double longest_pass_s;        /* longest pass of the host loop (collect, admit, launch) */
+    double longest_launch_s;      /* longest enqueueing of one batch (copies and launches) */
 } hibf_gpu_serve_stats;
 
 int hibf_gpu_hibf_serve(hibf_gpu_hibf hibf,
