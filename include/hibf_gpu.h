@@ -325,7 +325,10 @@ int hibf_gpu_hibf_membership_launch(hibf_gpu_hibf hibf,
  * results_per_query ids per request in h_ids[q*R ..] with the full count in
  * h_counts[q] (ids in the reference's depth-first order; unused id slots are
  * 0).  When h_values is pinned host memory (hipHostMalloc), batches are copied
- * from it directly; otherwise through pinned staging buffers. */
+ * from it directly; otherwise through pinned staging buffers.  Before the
+ * first arrival's clock starts, the loop reads a pinned h_values once on the
+ * device and runs 64 one-query batches (request 0) per stream, so that no
+ * timed batch pays a first-use cost of the copy paths. */
 typedef struct hibf_gpu_serve_params
 {
     double window_us;
