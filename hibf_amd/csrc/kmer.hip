@@ -104,6 +104,54 @@ int device_error(hipError_t e, char const * what)
 
 } // namespace
 
+namespace hibf_amd
+{
+
+size_t kmer_hash_temp_bytes(uint64_t max_seqs)
+{
+    size_t bytes = 0;
+    (void)inclusive_sum_u64(nullptr, bytes, nullptr, nullptr, std::max<uint64_t>(max_seqs, 1u), nullptr);
+    return std::max<size_t>(bytes, 1u);
+}
+
+// Queues the hashes of n reads (CSR offsets on the device) without a host wait: offsets by a device
+// scan into d_hash_offsets (n + 1 entries), then the hashes.  d_hashes must hold one hash per base
+// (a read of length L has L - k + 1 <= L k-mers); temp holds kmer_hash_temp_bytes(n) bytes.
+hipError_t kmer_hash_async(uint8_t const * d_seq,
+                           uint64_t const * d_seq_offsets,
+                           uint64_t n,
+                           uint32_t k,
+                           uint64_t * d_hash_offsets,
+                           uint64_t * d_hashes,
+                           void * temp,
+                           size_t temp_bytes,
+                           hipStream_t s)
+{
+    hipLaunchKernelGGL(zero_first_kernel, dim3(1), dim3(64), 0, s, d_hash_offsets);
+    if (n == 0)
+        return hipGetLastError();
+    hipLaunchKernelGGL(kmer_count_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, d_seq_offsets, n, k,
+                       d_hash_offsets + 1);
+    hipError_t e = inclusive_sum_u64(temp, temp_bytes, d_hash_offsets + 1, d_hash_offsets + 1, n, s);
+    if (e != hipSuccess)
+        return e;
+    hipLaunchKernelGGL(kmer_hash_kernel, dim3(blocks_for(n, 1)), dim3(128), 0, s, d_seq, d_seq_offsets, n, k,
+                       d_hash_offsets, d_hashes);
+    return hipGetLastError();
+}
+
+hipError_t kmer_thresholds_async(uint64_t const * d_hash_offsets, uint64_t n, double fraction, uint16_t * d_thresholds,
+                                 hipStream_t s)
+{
+    if (n == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(kmer_threshold_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, d_hash_offsets, n, fraction,
+                       d_thresholds);
+    return hipGetLastError();
+}
+
+} // namespace hibf_amd
+
 extern "C" {
 
 int hibf_gpu_kmer_hash(uint8_t const * d_seq,

@@ -8,6 +8,10 @@
 // Query values in pinned host memory (hipHostMalloc, e.g. a receive ring) are
 // copied to the device directly; pageable values go through pinned staging
 // buffers, large batches with several host threads.
+//
+// The same loop serves reads: a request is then a DNA read whose bases are
+// copied (one byte per base instead of eight per k-mer hash), hashed on the
+// device and given the threshold uint16_t(fraction * number of k-mers) there.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -18,6 +22,7 @@
 #include <vector>
 
 #include "hibf_gpu.h"
+#include "kernels.hpp"
 
 namespace
 {
@@ -39,6 +44,11 @@ struct Slot
     uint64_t * d_ids{};
     uint64_t * d_hs{};
     uint64_t * d_hc{};
+    // reads in: the batch's k-mer hashes, their offsets and the scan's scratch
+    uint64_t * d_hashes{};
+    uint64_t * d_hash_offs{};
+    void * d_temp{};
+    size_t temp_bytes{};
     uint64_t q0{}, q1{};
     uint64_t bytes{};
     bool busy{false};
@@ -88,26 +98,39 @@ void copy_host(void * dst, void const * src, uint64_t bytes, unsigned threads)
 
 } // namespace
 
-extern "C" int hibf_gpu_hibf_serve(hibf_gpu_hibf hibf,
-                                   hibf_gpu_serve_params const * params,
-                                   uint64_t const * h_values,
-                                   uint64_t const * h_offsets,
-                                   uint16_t const * h_thresholds,
-                                   double const * h_arrival_s,
-                                   uint64_t nq,
-                                   double * h_latency_s,
-                                   uint64_t * h_ids,
-                                   uint32_t * h_counts,
-                                   hibf_gpu_serve_stats * stats)
+namespace
 {
-    if (!hibf || !params || !h_values || !h_offsets || !h_thresholds || !h_arrival_s || !h_latency_s || !h_ids
+
+// The loop.  Requests are elements of `eb` bytes (8: k-mer hashes with host thresholds; 1: read
+// bases, hashed with k on the device, thresholds from `fraction`).
+int serve_impl(hibf_gpu_hibf hibf,
+               hibf_gpu_serve_params const * params,
+               char const * h_in,
+               uint64_t const eb,
+               uint64_t const * h_offsets,
+               uint16_t const * h_thresholds,
+               uint32_t const k,
+               double const fraction,
+               double const * h_arrival_s,
+               uint64_t nq,
+               double * h_latency_s,
+               uint64_t * h_ids,
+               uint32_t * h_counts,
+               hibf_gpu_serve_stats * stats)
+{
+    bool const reads = eb == 1u;
+    if (!hibf || !params || !h_in || !h_offsets || (!reads && !h_thresholds) || !h_arrival_s || !h_latency_s || !h_ids
         || !h_counts)
+        return HIBF_GPU_ERR_INVALID_ARGUMENT;
+    if (reads && (k == 0 || k > 32 || !(fraction >= 0.0)))
         return HIBF_GPU_ERR_INVALID_ARGUMENT;
     hibf_gpu_serve_params const p = *params;
     if (p.streams == 0 || p.max_batch == 0 || p.results_per_query == 0 || p.max_batch_values == 0)
         return HIBF_GPU_ERR_INVALID_ARGUMENT;
     uint32_t const R = p.results_per_query;
-    auto bytes_of = [&](uint64_t q) { return (h_offsets[q + 1] - h_offsets[q]) * 8u + 34u + 8u * R; };
+    // device bytes of a request: its elements (+ its hashes for a read), offset, threshold, ids
+    auto bytes_of = [&](uint64_t q)
+    { return (h_offsets[q + 1] - h_offsets[q]) * (eb + (reads ? 8u : 0u)) + 34u + 8u * R; };
     for (uint64_t q = 0; q < nq; ++q)
     {
         if (h_offsets[q + 1] - h_offsets[q] > p.max_batch_values)
@@ -118,7 +141,7 @@ extern "C" int hibf_gpu_hibf_serve(hibf_gpu_hibf hibf,
             return HIBF_GPU_ERR_INVALID_ARGUMENT;
     }
 
-    bool const pinned = host_pinned(h_values, h_offsets[nq] * 8u);
+    bool const pinned = host_pinned(h_in, h_offsets[nq] * eb);
     unsigned const copy_threads = std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
     std::vector<Slot> slots(p.streams);
     int rc = HIBF_GPU_OK;
@@ -129,7 +152,8 @@ extern "C" int hibf_gpu_hibf_serve(hibf_gpu_hibf hibf,
             if (s.stream)
                 (void)hipStreamSynchronize(s.stream);
             for (void * d : {static_cast<void *>(s.d_vals), static_cast<void *>(s.d_offs), static_cast<void *>(s.d_thr),
-                             static_cast<void *>(s.d_ids), static_cast<void *>(s.d_hs), static_cast<void *>(s.d_hc)})
+                             static_cast<void *>(s.d_ids), static_cast<void *>(s.d_hs), static_cast<void *>(s.d_hc),
+                             static_cast<void *>(s.d_hashes), static_cast<void *>(s.d_hash_offs), s.d_temp})
                 if (d)
                     (void)hipFree(d);
             for (void * hp : {static_cast<void *>(s.h_vals), static_cast<void *>(s.h_offs),
@@ -157,18 +181,81 @@ extern "C" int hibf_gpu_hibf_serve(hibf_gpu_hibf hibf,
     } while (0)
     uint64_t const B = p.max_batch;
     uint64_t const V = p.max_batch_values;
+    // Queues one batch (requests qb .. qb + nb - 1, nv elements) on a slot: input copy, reads hashed
+    // on the device, membership launch, results back, completion event.
+    auto enqueue = [&](Slot & s, uint64_t const qb, uint64_t const nb, uint64_t const nv) -> int
+    {
+        uint64_t const base = h_offsets[qb];
+        for (uint64_t i = 0; i <= nb; ++i)
+            s.h_offs[i] = h_offsets[qb + i] - base;
+        hipError_t e = hipSuccess;
+        auto step = [&](hipError_t r)
+        {
+            if (e == hipSuccess)
+                e = r;
+        };
+        if (nv && pinned)
+            step(hipMemcpyAsync(s.d_vals, h_in + base * eb, nv * eb, hipMemcpyHostToDevice, s.stream));
+        else if (nv)
+        {
+            copy_host(s.h_vals, h_in + base * eb, nv * eb, copy_threads);
+            step(hipMemcpyAsync(s.d_vals, s.h_vals, nv * eb, hipMemcpyHostToDevice, s.stream));
+        }
+        step(hipMemcpyAsync(s.d_offs, s.h_offs, (nb + 1) * 8u, hipMemcpyHostToDevice, s.stream));
+        uint64_t const * d_v = s.d_vals;
+        uint64_t const * d_o = s.d_offs;
+        if (reads)
+        {
+            step(hibf_amd::kmer_hash_async(reinterpret_cast<uint8_t const *>(s.d_vals), s.d_offs, nb, k,
+                                           s.d_hash_offs, s.d_hashes, s.d_temp, s.temp_bytes, s.stream));
+            step(hibf_amd::kmer_thresholds_async(s.d_hash_offs, nb, fraction, s.d_thr, s.stream));
+            d_v = s.d_hashes;
+            d_o = s.d_hash_offs;
+        }
+        else
+        {
+            std::memcpy(s.h_thr, h_thresholds + qb, nb * 2u);
+            step(hipMemcpyAsync(s.d_thr, s.h_thr, nb * 2u, hipMemcpyHostToDevice, s.stream));
+        }
+        if (e != hipSuccess)
+        {
+            (void)hipGetLastError();
+            return HIBF_GPU_ERR_DEVICE;
+        }
+        if (int const r = hibf_gpu_hibf_membership_launch(hibf, s.agent, d_v, d_o, nb, s.d_thr, s.d_ids, nb * R,
+                                                          s.d_hs, s.d_hc, s.stream))
+            return r;
+        step(hipMemcpyAsync(s.h_hs, s.d_hs, nb * 8u, hipMemcpyDeviceToHost, s.stream));
+        step(hipMemcpyAsync(s.h_hc, s.d_hc, nb * 8u, hipMemcpyDeviceToHost, s.stream));
+        step(hipMemcpyAsync(s.h_ids, s.d_ids, nb * R * 8u, hipMemcpyDeviceToHost, s.stream));
+        step(hipEventRecord(s.done, s.stream));
+        if (e != hipSuccess)
+        {
+            (void)hipGetLastError();
+            return HIBF_GPU_ERR_DEVICE;
+        }
+        return HIBF_GPU_OK;
+    };
     for (Slot & s : slots)
     {
         SERVE_TRY(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
         SERVE_TRY(hipEventCreateWithFlags(&s.done, hipEventDisableTiming));
         if (!pinned)
-            SERVE_TRY(hipHostMalloc(reinterpret_cast<void **>(&s.h_vals), V * 8u, hipHostMallocDefault));
+            SERVE_TRY(hipHostMalloc(reinterpret_cast<void **>(&s.h_vals), std::max<uint64_t>(V * eb, 8u),
+                                    hipHostMallocDefault));
         SERVE_TRY(hipHostMalloc(reinterpret_cast<void **>(&s.h_offs), (B + 1) * 8u, hipHostMallocDefault));
         SERVE_TRY(hipHostMalloc(reinterpret_cast<void **>(&s.h_thr), B * 2u, hipHostMallocDefault));
         SERVE_TRY(hipHostMalloc(reinterpret_cast<void **>(&s.h_ids), B * R * 8u, hipHostMallocDefault));
         SERVE_TRY(hipHostMalloc(reinterpret_cast<void **>(&s.h_hs), B * 8u, hipHostMallocDefault));
         SERVE_TRY(hipHostMalloc(reinterpret_cast<void **>(&s.h_hc), B * 8u, hipHostMallocDefault));
-        SERVE_TRY(hipMalloc(reinterpret_cast<void **>(&s.d_vals), V * 8u));
+        SERVE_TRY(hipMalloc(reinterpret_cast<void **>(&s.d_vals), std::max<uint64_t>(V * eb, 8u)));
+        if (reads)
+        {
+            SERVE_TRY(hipMalloc(reinterpret_cast<void **>(&s.d_hashes), V * 8u));
+            SERVE_TRY(hipMalloc(reinterpret_cast<void **>(&s.d_hash_offs), (B + 1) * 8u));
+            s.temp_bytes = hibf_amd::kmer_hash_temp_bytes(B);
+            SERVE_TRY(hipMalloc(&s.d_temp, s.temp_bytes));
+        }
         SERVE_TRY(hipMalloc(reinterpret_cast<void **>(&s.d_offs), (B + 1) * 8u));
         SERVE_TRY(hipMalloc(reinterpret_cast<void **>(&s.d_thr), B * 2u));
         SERVE_TRY(hipMalloc(reinterpret_cast<void **>(&s.d_ids), B * R * 8u));
@@ -188,11 +275,10 @@ extern "C" int hibf_gpu_hibf_serve(hibf_gpu_hibf hibf,
     // first time it copies a size class, and that took about 10 ms inside the loop (a batch's latency).
     // Each slot copies every size from 64 bytes up to its buffers' capacity, in both directions.
     {
-        uint64_t const * const src_vals = pinned ? h_values : nullptr;
         for (Slot & s : slots)
         {
-            uint64_t * const h_src = src_vals ? const_cast<uint64_t *>(src_vals) : s.h_vals;
-            uint64_t const in_bytes = std::min<uint64_t>(V * 8u, pinned ? h_offsets[nq] * 8u : V * 8u);
+            void const * const h_src = pinned ? static_cast<void const *>(h_in) : static_cast<void const *>(s.h_vals);
+            uint64_t const in_bytes = std::min<uint64_t>(V * eb, pinned ? h_offsets[nq] * eb : V * eb);
             uint64_t const out_bytes = B * R * 8u;
             for (uint64_t b = 64u;; b *= 4u)
             {
@@ -213,14 +299,14 @@ extern "C" int hibf_gpu_hibf_serve(hibf_gpu_hibf hibf,
         // a freshly pinned buffer took about 10 ms in the loop (the region's first batch paid it).
         if (pinned && h_offsets[nq] > 0)
         {
-            uint64_t const total = h_offsets[nq] * 8u;
-            uint64_t const chunk = V * 8u;
-            size_t k = 0;
-            for (uint64_t off = 0; off < total; off += chunk, ++k)
+            uint64_t const total = h_offsets[nq] * eb;
+            uint64_t const chunk = V * eb;
+            size_t c = 0;
+            for (uint64_t off = 0; off < total; off += chunk, ++c)
             {
-                Slot & s = slots[k % slots.size()];
-                SERVE_TRY(hipMemcpyAsync(s.d_vals, reinterpret_cast<char const *>(h_values) + off,
-                                         std::min(chunk, total - off), hipMemcpyHostToDevice, s.stream));
+                Slot & s = slots[c % slots.size()];
+                SERVE_TRY(hipMemcpyAsync(s.d_vals, h_in + off, std::min(chunk, total - off), hipMemcpyHostToDevice,
+                                         s.stream));
             }
             for (Slot & s : slots)
                 SERVE_TRY(hipStreamSynchronize(s.stream));
@@ -230,35 +316,14 @@ extern "C" int hibf_gpu_hibf_serve(hibf_gpu_hibf hibf,
         // the loop's depth before any request is timed.
         if (nq > 0 && h_offsets[1] - h_offsets[0] <= V)
         {
-            uint64_t const n0 = h_offsets[1] - h_offsets[0];
+            // Every round writes the same staging contents, so rounds need not wait for each other.
             for (int round = 0; round < 64; ++round)
                 for (Slot & s : slots)
-                {
-                    s.h_offs[0] = 0;
-                    s.h_offs[1] = n0;
-                    s.h_thr[0] = h_thresholds[0];
-                    if (n0 && pinned)
-                        SERVE_TRY(hipMemcpyAsync(s.d_vals, h_values + h_offsets[0], n0 * 8u, hipMemcpyHostToDevice,
-                                                 s.stream));
-                    else if (n0)
-                    {
-                        std::memcpy(s.h_vals, h_values + h_offsets[0], n0 * 8u);
-                        SERVE_TRY(hipMemcpyAsync(s.d_vals, s.h_vals, n0 * 8u, hipMemcpyHostToDevice, s.stream));
-                    }
-                    SERVE_TRY(hipMemcpyAsync(s.d_offs, s.h_offs, 2u * 8u, hipMemcpyHostToDevice, s.stream));
-                    SERVE_TRY(hipMemcpyAsync(s.d_thr, s.h_thr, 2u, hipMemcpyHostToDevice, s.stream));
-                    if ((rc = hibf_gpu_hibf_membership_launch(hibf, s.agent, s.d_vals, s.d_offs, 1u, s.d_thr, s.d_ids,
-                                                              R, s.d_hs, s.d_hc, s.stream)) != HIBF_GPU_OK)
+                    if ((rc = enqueue(s, 0u, 1u, h_offsets[1] - h_offsets[0])) != HIBF_GPU_OK)
                     {
                         cleanup();
                         return rc;
                     }
-                    SERVE_TRY(hipMemcpyAsync(s.h_hs, s.d_hs, 8u, hipMemcpyDeviceToHost, s.stream));
-                    SERVE_TRY(hipMemcpyAsync(s.h_hc, s.d_hc, 8u, hipMemcpyDeviceToHost, s.stream));
-                    SERVE_TRY(hipMemcpyAsync(s.h_ids, s.d_ids, R * 8u, hipMemcpyDeviceToHost, s.stream));
-                    // Every round writes the same staging contents, so rounds need not wait for each other.
-                    SERVE_TRY(hipEventRecord(s.done, s.stream));
-                }
         }
         for (Slot & s : slots)
         {
@@ -354,32 +419,11 @@ extern "C" int hibf_gpu_hibf_serve(hibf_gpu_hibf hibf,
                     double const launch_start = now_s(t0);
                     Slot & s = *free_slot;
                     uint64_t const nb = q1 - pending;
-                    uint64_t const base = h_offsets[pending];
-                    for (uint64_t i = 0; i <= nb; ++i)
-                        s.h_offs[i] = h_offsets[pending + i] - base;
-                    std::memcpy(s.h_thr, h_thresholds + pending, nb * 2u);
-                    if (nv && pinned)
-                    {
-                        SERVE_TRY(hipMemcpyAsync(s.d_vals, h_values + base, nv * 8u, hipMemcpyHostToDevice, s.stream));
-                    }
-                    else if (nv)
-                    {
-                        copy_host(s.h_vals, h_values + base, nv * 8u, copy_threads);
-                        SERVE_TRY(hipMemcpyAsync(s.d_vals, s.h_vals, nv * 8u, hipMemcpyHostToDevice, s.stream));
-                    }
-                    SERVE_TRY(hipMemcpyAsync(s.d_offs, s.h_offs, (nb + 1) * 8u, hipMemcpyHostToDevice, s.stream));
-                    SERVE_TRY(hipMemcpyAsync(s.d_thr, s.h_thr, nb * 2u, hipMemcpyHostToDevice, s.stream));
-                    rc = hibf_gpu_hibf_membership_launch(hibf, s.agent, s.d_vals, s.d_offs, nb, s.d_thr, s.d_ids,
-                                                         nb * R, s.d_hs, s.d_hc, s.stream);
-                    if (rc != HIBF_GPU_OK)
+                    if ((rc = enqueue(s, pending, nb, nv)) != HIBF_GPU_OK)
                     {
                         cleanup();
                         return rc;
                     }
-                    SERVE_TRY(hipMemcpyAsync(s.h_hs, s.d_hs, nb * 8u, hipMemcpyDeviceToHost, s.stream));
-                    SERVE_TRY(hipMemcpyAsync(s.h_hc, s.d_hc, nb * 8u, hipMemcpyDeviceToHost, s.stream));
-                    SERVE_TRY(hipMemcpyAsync(s.h_ids, s.d_ids, nb * R * 8u, hipMemcpyDeviceToHost, s.stream));
-                    SERVE_TRY(hipEventRecord(s.done, s.stream));
                     s.q0 = pending;
                     s.q1 = q1;
                     s.bytes = 0;
@@ -426,4 +470,39 @@ extern "C" int hibf_gpu_hibf_serve(hibf_gpu_hibf hibf,
     cleanup();
     return HIBF_GPU_OK;
 #undef SERVE_TRY
+}
+
+} // namespace
+
+extern "C" int hibf_gpu_hibf_serve(hibf_gpu_hibf hibf,
+                                   hibf_gpu_serve_params const * params,
+                                   uint64_t const * h_values,
+                                   uint64_t const * h_offsets,
+                                   uint16_t const * h_thresholds,
+                                   double const * h_arrival_s,
+                                   uint64_t nq,
+                                   double * h_latency_s,
+                                   uint64_t * h_ids,
+                                   uint32_t * h_counts,
+                                   hibf_gpu_serve_stats * stats)
+{
+    return serve_impl(hibf, params, reinterpret_cast<char const *>(h_values), 8u, h_offsets, h_thresholds, 0u, 0.0,
+                      h_arrival_s, nq, h_latency_s, h_ids, h_counts, stats);
+}
+
+extern "C" int hibf_gpu_hibf_serve_reads(hibf_gpu_hibf hibf,
+                                         hibf_gpu_serve_params const * params,
+                                         uint8_t const * h_bases,
+                                         uint64_t const * h_read_offsets,
+                                         uint32_t k,
+                                         double fraction,
+                                         double const * h_arrival_s,
+                                         uint64_t nq,
+                                         double * h_latency_s,
+                                         uint64_t * h_ids,
+                                         uint32_t * h_counts,
+                                         hibf_gpu_serve_stats * stats)
+{
+    return serve_impl(hibf, params, reinterpret_cast<char const *>(h_bases), 1u, h_read_offsets, nullptr, k, fraction,
+                      h_arrival_s, nq, h_latency_s, h_ids, h_counts, stats);
 }

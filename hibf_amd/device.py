@@ -534,6 +534,36 @@ class DeviceHibf:
         stats = {k: getattr(st, k) for k, _ in native.ServeStats._fields_}
         return lat, ids, cnt, stats
 
+    def serve_reads(self, reads, arrival_s: np.ndarray, k: int = 19, fraction: float = 0.7, window_us: float = 500.0,
+                    max_batch: int = 65536, max_batch_bases: int = 1 << 26, budget_bytes: int = 4 << 30,
+                    streams: int = 4, results_per_query: int = 16, pinned: bool = True):
+        """The serving loop for reads (hibf_gpu_hibf_serve_reads): reads (str or
+        bytes, or a (bases, offsets) pair) arrive at arrival_s; each is hashed
+        on the device and answered with threshold uint16_t(fraction * #k-mers).
+        With pinned, the bases are placed in pinned host memory first, as a
+        server's receive buffer would hold them.  Returns (latency_s,
+        ids[nq, R], counts, stats dict)."""
+        data, o = reads if isinstance(reads, tuple) else reads_csr(reads)
+        o = np.ascontiguousarray(o, dtype=np.uint64)
+        if pinned:
+            b = torch.empty(max(int(data.size), 1), dtype=torch.uint8, pin_memory=True)
+            b.numpy()[: data.size] = data
+        else:
+            b = np.ascontiguousarray(data if data.size else np.zeros(1, dtype=np.uint8), dtype=np.uint8)
+        a = np.ascontiguousarray(arrival_s, dtype=np.float64)
+        nq = o.size - 1
+        lat = np.zeros(nq, dtype=np.float64)
+        ids = np.zeros((nq, results_per_query), dtype=np.uint64)
+        cnt = np.zeros(nq, dtype=np.uint32)
+        p = native.ServeParams(window_us, max_batch, max_batch_bases, budget_bytes, streams, results_per_query)
+        st = native.ServeStats()
+        P = lambda x: ctypes.c_void_p(x.data_ptr()) if isinstance(x, torch.Tensor) \
+            else x.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
+        check(lib().hibf_gpu_hibf_serve_reads(self.h, ctypes.byref(p), P(b), P(o), k, float(fraction), P(a), nq,
+                                              P(lat), P(ids), P(cnt), ctypes.byref(st)))
+        stats = {k_: getattr(st, k_) for k_, _ in native.ServeStats._fields_}
+        return lat, ids, cnt, stats
+
     def bulk_count_into(self, agent: Agent, values: torch.Tensor, offsets: torch.Tensor, thresholds: torch.Tensor,
                         out: torch.Tensor, counter_bytes: int = 2) -> None:
         """HIBF bulk_count into a preallocated (nq, number_of_user_bins) device

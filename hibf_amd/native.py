@@ -144,6 +144,8 @@ _SIGNATURES = {
     "hibf_gpu_hibf_bulk_count": (_INT, [_V, _V, _V, _V, _U64, _V, _U32, _V, _V]),
     "hibf_gpu_hibf_serve": (_INT, [_V, ctypes.POINTER(ServeParams), _V, _V, _V, _V, _U64, _V, _V, _V,
                                    ctypes.POINTER(ServeStats)]),
+    "hibf_gpu_hibf_serve_reads": (_INT, [_V, ctypes.POINTER(ServeParams), _V, _V, ctypes.c_uint32, ctypes.c_double,
+                                         _V, _U64, _V, _V, _V, ctypes.POINTER(ServeStats)]),
     "hibf_gpu_hibf_info": (_INT, [_V] + [ctypes.POINTER(_U64)] * 5),
     "hibf_gpu_hibf_export_tables": (_INT, [_V, ctypes.POINTER(Geometry), _V, _V, _V, _V]),
     "hibf_gpu_config_default": (_INT, [ctypes.POINTER(Config)]),

@@ -363,6 +363,27 @@ int hibf_gpu_hibf_serve(hibf_gpu_hibf hibf,
                         uint32_t * h_counts,
                         hibf_gpu_serve_stats * stats);
 
+/* The serving loop for reads (the tutorial search, test/tutorial/search.cpp:
+ * 37-45, served online): request q is the read h_bases[h_read_offsets[q] ..
+ * h_read_offsets[q+1]) (characters; dna4 reads anything but C/G/T/U as A);
+ * its bases are copied (one byte per base), its k-mer hashes (k in 1..32)
+ * and threshold uint16_t(fraction * number of k-mers) are computed on the
+ * device (as hibf_gpu_kmer_hash / hibf_gpu_kmer_thresholds), and it is
+ * answered like a request of hibf_gpu_hibf_serve.  max_batch_values counts
+ * bases; the budget counts bases, hashes and outputs. */
+int hibf_gpu_hibf_serve_reads(hibf_gpu_hibf hibf,
+                              const hibf_gpu_serve_params * params,
+                              const uint8_t * h_bases,
+                              const uint64_t * h_read_offsets,
+                              uint32_t k,
+                              double fraction,
+                              const double * h_arrival_s,
+                              uint64_t nq,
+                              double * h_latency_s,
+                              uint64_t * h_ids,
+                              uint32_t * h_counts,
+                              hibf_gpu_serve_stats * stats);
+
 /* Batched HIBF counting_agent_type<T>::bulk_count
  * (hierarchical_interleaved_bloom_filter.hpp:462-496, 549-566): d_counts gets
  * nq * number_of_user_bins counters of counter_bytes (1, 2, 4, 8); a user bin's

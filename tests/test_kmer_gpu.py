@@ -69,3 +69,37 @@ def test_search_reads_like_the_tutorial():
         assert g == ref.membership_for(hashes, O.kmer_threshold(hashes.size, 0.7))
         found += bool(g)
     assert found >= 380
+
+
+@pytest.mark.parametrize("pinned", [True, False], ids=["pinned", "pageable"])
+def test_serving_reads_equals_the_tutorial_search(pinned):
+    """hibf_gpu_hibf_serve_reads (bases copied, hashed and thresholded on the
+    device inside the serving loop) answers every read as the oracle's
+    membership_for on its k-mer hashes with threshold uint16_t(0.7 * #k-mers),
+    for Poisson arrivals and small batching windows; empty and short reads
+    included."""
+    from hibf_amd.device import DeviceHibf
+    _say(f"serving reads pinned={pinned}")
+    k = 19
+    rng = random.Random(11)
+    genomes = ["".join(rng.choice("ACGT") for _ in range(rng.randint(2000, 5000))) for _ in range(80)]
+    ubs = [np.unique(O.kmer_hashes(g, k)) for g in genomes]
+    dev = DeviceHibf.build(ubs, tmax=64)
+    ref = oracle_of(dev)
+    reads = []
+    for _ in range(3000):
+        g = genomes[rng.randrange(len(genomes))]
+        start = rng.randrange(0, len(g) - 250)
+        reads.append(g[start:start + rng.randint(50, 250)])
+    reads[5], reads[17], reads[100] = "", "ACG", "N" * 30
+    nrng = np.random.default_rng(3)
+    arrival = np.cumsum(nrng.exponential(1.0 / 200000.0, size=len(reads)))
+    for window in (50.0, 500.0):
+        lat, ids, cnt, st = dev.serve_reads(reads, arrival, k=k, fraction=0.7, window_us=window, max_batch=512,
+                                            max_batch_bases=1 << 18, results_per_query=16, pinned=pinned)
+        assert st["batches"] >= 2 and (lat > 0).all()
+        for i, r in enumerate(reads):
+            hashes = O.kmer_hashes(r, k)
+            want = ref.membership_for(hashes, O.kmer_threshold(hashes.size, 0.7))
+            assert int(cnt[i]) == len(want), i
+            assert ids[i, :min(len(want), 16)].tolist() == want[:16], i
