@@ -35,7 +35,10 @@ def parse():
     ap.add_argument("--serve-windows", default="200,1000,2000", help="batching windows (us)")
     ap.add_argument("--serve-pageable", action="store_true",
                     help="hand the serving loop pageable query values (staged) instead of a pinned receive buffer")
-    ap.add_argument("--workload", choices=["flat", "hibf", "serve"], default="flat",
+    ap.add_argument("--genomes", type=int, default=4096,
+                    help="serve_reads: random genomes (user bins) of --genome-length bases")
+    ap.add_argument("--genome-length", type=int, default=20000)
+    ap.add_argument("--workload", choices=["flat", "hibf", "serve", "serve_reads"], default="flat",
                     help="flat: 8,192-bin flat IBF bulk_count (default); hibf: 32,768-user-bin HIBF membership "
                          "(fixed-length batch, log-normal batched hipGraph form, plain per-query form)")
     ap.add_argument("--hibf-queries", type=int, default=500_000)
@@ -132,6 +135,8 @@ def main():
         run_flat(args)
     elif args.workload == "hibf":
         run_hibf(args)
+    elif args.workload == "serve_reads":
+        run_serve_reads(args)
     else:
         run_serve(args)
 
@@ -795,6 +800,82 @@ def run_serve(args):
            "config": {"workload": f"serving on the {n_ub}-user-bin HIBF, Poisson arrivals, log-normal lengths "
                                   f"(median 300, max 5000), 60% from one user bin, t = 0.7 L, 4 GiB budget, 4 streams",
                       "query_values": "pageable, staged" if args.serve_pageable else "pinned receive buffer"},
+           "sweep": rows}
+    print(json.dumps(out), flush=True)
+
+
+def run_serve_reads(args):
+    """Serving loop for reads (the tutorial search online): user bins are the
+    19-mer sets of random genomes; requests are 250-base reads sampled from
+    them (1 % base errors), Poisson arrivals; bases go to the device, which
+    hashes them and uses threshold uint16_t(0.7 x #k-mers)."""
+    import torch
+    from hibf_amd.device import DeviceHibf, kmer_hash, to_numpy
+
+    torch.cuda.set_device(0)
+    k, read_len = 19, 250
+    rng = np.random.default_rng(args.seed + 41)
+    n_g, g_len = args.genomes, args.genome_length
+    genome = rng.integers(0, 4, size=n_g * g_len, dtype=np.uint8)
+    acgt = np.frombuffer(b"ACGT", dtype=np.uint8)
+    genome_chars = acgt[genome]
+    t0 = time.perf_counter()
+    offs = np.arange(n_g + 1, dtype=np.uint64) * np.uint64(g_len)
+    hashes, hoff = kmer_hash((genome_chars, offs), k)
+    h, ho = to_numpy(hashes), to_numpy(hoff)
+    ubs = [h[int(ho[i]):int(ho[i + 1])] for i in range(n_g)]
+    hibf = DeviceHibf.build(ubs, tmax=64)
+    tab = hibf.tables()
+    log(f"serve_reads: {n_g} genomes x {g_len} bases, {int(ho[-1])} k-mers, {len(tab['geoms'])} IBFs, "
+        f"built in {time.perf_counter() - t0:.1f} s")
+    rates = [float(r) for r in args.serve_rates.split(",")]
+    windows = [float(w) for w in args.serve_windows.split(",")]
+    rows = []
+    ref = None
+    for rate in rates:
+        nq = int(rate * args.serve_seconds)
+        g = rng.integers(0, n_g, size=nq)
+        start = rng.integers(0, g_len - read_len, size=nq)
+        idx = (g * g_len + start)[:, None] + np.arange(read_len)[None, :]
+        bases = genome_chars[idx]
+        err = rng.random(bases.shape) < 0.01
+        bases[err] = acgt[rng.integers(0, 4, size=int(err.sum()))]
+        data = np.ascontiguousarray(bases).reshape(-1)
+        r_off = np.arange(nq + 1, dtype=np.uint64) * np.uint64(read_len)
+        arrival = np.cumsum(rng.exponential(1.0 / rate, size=nq))
+        arrival -= arrival[0]
+        for win in windows:
+            lat, ids, cnt, st = hibf.serve_reads((data, r_off), arrival, k=k, fraction=0.7, window_us=win,
+                                                 max_batch=65536, max_batch_bases=1 << 26, budget_bytes=4 << 30,
+                                                 streams=4, results_per_query=16)
+            row = {"rate_qps": rate, "window_us": win, "reads": nq,
+                   "p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
+                   "sustained_qps": nq / st["seconds"], "sustained_lookups_per_s": st["lookups"] / st["seconds"],
+                   "host_link_GBs": nq * read_len / st["seconds"] / 1e9, "batches": st["batches"],
+                   "hit_fraction": float(np.mean(cnt > 0)), "longest_pass_ms": st["longest_pass_s"] * 1e3}
+            log(f"serve reads {rate:.0f} q/s window {win:.0f} us: p50 {row['p50_ms']:.3f} ms p99 {row['p99_ms']:.3f} ms, "
+                f"{row['sustained_qps']:.0f} reads/s, {row['sustained_lookups_per_s'] / 1e9:.2f} G lookups/s, "
+                f"{row['host_link_GBs']:.1f} GB/s of bases, {100 * row['hit_fraction']:.1f} % reads with hits")
+            rows.append(row)
+        if args.check_queries > 0:
+            import oracle as O
+            if ref is None:
+                geoms = [O.geom(b_, s_, h_) for (b_, s_, h_) in tab["geoms"]]
+                ref = O.Hibf(geoms, hibf.download_arena(), tab["word_offsets"], tab["tb_offsets"], tab["next_ibf"],
+                             tab["user_bin"], n_g)
+            for q in rng.choice(nq, size=min(args.check_queries, nq), replace=False):
+                q = int(q)
+                read = data[int(r_off[q]):int(r_off[q + 1])].tobytes().decode()
+                hashes_q = O.kmer_hashes(read, k)
+                want = ref.membership_for(hashes_q, O.kmer_threshold(hashes_q.size, 0.7))
+                assert int(cnt[q]) == len(want) and ids[q, :min(len(want), 16)].tolist() == want[:16], \
+                    f"serving result differs at read {q}"
+            log(f"serving parity vs oracle on {min(args.check_queries, nq)} sampled reads: True")
+    best = max(rows, key=lambda r: r["sustained_qps"])
+    out = {"metric": "reads/s (HIBF serving loop, reads in)", "value": best["sustained_qps"], "unit": "reads/s",
+           "n_gpus": 1, "higher_is_better": True, "data": "synthetic", "dtype": "u8 bases / u64 hash",
+           "config": {"workload": f"serving reads of {read_len} bases (k = {k}, t = 0.7 x #k-mers) on an HIBF of "
+                                  f"{n_g} random genomes x {g_len} bases, tmax 64, Poisson arrivals, 4 streams"},
            "sweep": rows}
     print(json.dumps(out), flush=True)
 

@@ -73,8 +73,10 @@ def reads_csr(reads) -> tuple[np.ndarray, np.ndarray]:
 
 def kmer_hash(reads, k: int) -> tuple[torch.Tensor, torch.Tensor]:
     """Device k-mer hashes of DNA reads (hibf_gpu_kmer_hash): (hashes, offsets)
-    as uint64 device tensors in CSR form, one row of hashes per read."""
-    data, offsets = reads_csr(reads)
+    as uint64 device tensors in CSR form, one row of hashes per read.  `reads`
+    is a list of str/bytes or a (bases uint8, offsets uint64) CSR pair."""
+    data, offsets = reads if isinstance(reads, tuple) else reads_csr(reads)
+    offsets = np.ascontiguousarray(offsets, dtype=np.uint64)
     dev = _dev()
     d_seq = torch.from_numpy(np.ascontiguousarray(data).copy()).to(dev) if data.size else \
         torch.zeros(1, dtype=torch.uint8, device=dev)
