@@ -120,19 +120,27 @@ __global__ void __launch_bounds__(256) clear_kernel(uint64_t * __restrict__ word
     }
 }
 
-// bulk_contains for n values: out[i*bin_words + w] = AND_k row_k[w]
-// (src/interleaved_bloom_filter.cpp:204-291).  One wave per 64 values: lane j
-// hashes value j once, then the wave walks the 64 values, broadcasting each
-// value's row starts (readlane) and moving its bin_words-word rows with
-// coalesced loads and stores.
+// Batched bulk_contains (src/interleaved_bloom_filter.cpp:204-291): one wave per
+// 64 values.  Lane j hashes value j; the wave then walks the values U at a
+// time, broadcasting their row starts (readlane), issuing the h row loads of
+// all U values before it ANDs and stores them, so U x h row fetches are in
+// flight per wave.  With vec set (even row width, 16-byte aligned words and
+// output) a lane moves 16 bytes per load (a 1 KiB row per wave instruction);
+// otherwise 8.  Result rows are written with non-temporal stores when nt is set
+// (they are not read again here).
+template <int U>
 __global__ void __launch_bounds__(256) contains_kernel(IbfView g,
                                                        uint64_t const * __restrict__ values,
                                                        uint64_t n,
-                                                       uint64_t * __restrict__ out)
+                                                       uint64_t * __restrict__ out,
+                                                       uint32_t vec,
+                                                       uint32_t nt)
 {
+    typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));
     int const lane = static_cast<int>(threadIdx.x & 63u);
     uint64_t const waves = static_cast<uint64_t>(gridDim.x) * (blockDim.x >> 6);
     uint64_t const bw = g.bin_words;
+    uint32_t const hf = g.hash_funs;
     for (uint64_t base = (static_cast<uint64_t>(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 64u;
          base < n; base += waves * 64u)
     {
@@ -141,23 +149,61 @@ __global__ void __launch_bounds__(256) contains_kernel(IbfView g,
         uint64_t rows[5];
 #pragma unroll
         for (int k = 0; k < 5; ++k)
-            rows[k] = static_cast<uint32_t>(k) < g.hash_funs ? row_word(v, hash_seed(k), g.hash_shift, g.bin_size, bw)
-                                                             : 0ULL;
-        for (uint32_t i = 0; i < nv; ++i)
+            rows[k] = static_cast<uint32_t>(k) < hf ? row_word(v, hash_seed(k), g.hash_shift, g.bin_size, bw) : 0ULL;
+        for (uint32_t i = 0; i < nv; i += U)
         {
-            uint64_t r[5];
+            uint64_t r[U][5];
 #pragma unroll
-            for (int k = 0; k < 5; ++k)
-                r[k] = static_cast<uint32_t>(k) < g.hash_funs ? readlane64(rows[k], static_cast<int>(i)) : 0ULL;
-            uint64_t * const o = out + (base + i) * bw;
-            for (uint64_t w = static_cast<uint64_t>(lane); w < bw; w += 64u)
+            for (int u = 0; u < U; ++u)
+#pragma unroll
+                for (int k = 0; k < 5; ++k)
+                    r[u][k] = static_cast<uint32_t>(k) < hf ? readlane64(rows[k], static_cast<int>(min(i + u, 63u)))
+                                                            : 0ULL;
+            if (vec)
             {
-                uint64_t acc = g.words[r[0] + w];
+                for (uint64_t w = 2u * static_cast<uint64_t>(lane); w < bw; w += 128u)
+                {
+                    u64x2 acc[U];
 #pragma unroll
-                for (int k = 1; k < 5; ++k)
-                    if (static_cast<uint32_t>(k) < g.hash_funs)
-                        acc &= g.words[r[k] + w];
-                o[w] = acc;
+                    for (int u = 0; u < U; ++u)
+                        acc[u] = *reinterpret_cast<u64x2 const *>(g.words + r[u][0] + w);
+#pragma unroll
+                    for (int k = 1; k < 5; ++k)
+                        if (static_cast<uint32_t>(k) < hf)
+#pragma unroll
+                            for (int u = 0; u < U; ++u)
+                                acc[u] &= *reinterpret_cast<u64x2 const *>(g.words + r[u][k] + w);
+#pragma unroll
+                    for (int u = 0; u < U; ++u)
+                        if (i + u < nv)
+                        {
+                            u64x2 * const o = reinterpret_cast<u64x2 *>(out + (base + i + u) * bw + w);
+                            if (nt)
+                                __builtin_nontemporal_store(acc[u], o);
+                            else
+                                *o = acc[u];
+                        }
+                }
+            }
+            else
+            {
+                for (uint64_t w = static_cast<uint64_t>(lane); w < bw; w += 64u)
+                {
+                    uint64_t acc[U];
+#pragma unroll
+                    for (int u = 0; u < U; ++u)
+                        acc[u] = g.words[r[u][0] + w];
+#pragma unroll
+                    for (int k = 1; k < 5; ++k)
+                        if (static_cast<uint32_t>(k) < hf)
+#pragma unroll
+                            for (int u = 0; u < U; ++u)
+                                acc[u] &= g.words[r[u][k] + w];
+#pragma unroll
+                    for (int u = 0; u < U; ++u)
+                        if (i + u < nv)
+                            out[(base + i + u) * bw + w] = acc[u];
+                }
             }
         }
     }
@@ -335,7 +381,10 @@ hipError_t launch_contains(IbfView const & g, uint64_t const * values, uint64_t 
 {
     if (n == 0)
         return hipSuccess;
-    hipLaunchKernelGGL(contains_kernel, dim3(grid_for(n, 256)), dim3(256), 0, stream, g, values, n, out);
+    uint32_t const vec = g.bin_words % 2u == 0u && reinterpret_cast<uintptr_t>(g.words) % 16u == 0u &&
+                         reinterpret_cast<uintptr_t>(out) % 16u == 0u;
+    hipLaunchKernelGGL(contains_kernel<4>, dim3(grid_for(n, 256)), dim3(256), 0, stream, g, values, n, out, vec,
+                       g_count_tuning.nt_stores);
     return hipGetLastError();
 }
 

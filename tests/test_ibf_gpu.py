@@ -160,6 +160,29 @@ def test_counter_store_forms(nt_stores, knobs):
                               href.bulk_count_batch(hv, ho, [20] * 50, width)), width
 
 
+@pytest.mark.parametrize("nt_stores", [0, 1])
+@pytest.mark.parametrize("bins", [8192, 9000])
+def test_contains_store_forms(nt_stores, bins, knobs):
+    """bulk_contains rows are the oracle's with plain and non-temporal stores,
+    into a 16-byte aligned output and into one 8 bytes off (the 8-byte path),
+    for an even (128) and an odd (141) row width in words."""
+    import torch
+    _say(f"contains stores nt={nt_stores} bins={bins}")
+    knobs(count__nt_stores=nt_stores)
+    ref, dev, vals, _, rng = _pair(bins, 1024, 3, 20000, 11)
+    q = np.concatenate([vals[:333], _rand_u64(rng, 300)])
+    want = np.stack([ref.bulk_contains(int(v)) for v in q])
+    bw = want.shape[1]
+    qd = torch.from_numpy(q.view(np.int64)).cuda()
+    for shift in (0, 1):
+        buf = torch.full((q.size * bw + 1,), -1, dtype=torch.int64, device="cuda")
+        out = buf[shift:shift + q.size * bw]
+        dev.bulk_contains_into(qd, out)
+        got = out.cpu().numpy().view(np.uint64).reshape(q.size, bw)
+        assert np.array_equal(got, want), shift
+        assert int(buf[q.size * bw if shift == 0 else 0]) == -1, shift  # nothing past the rows
+
+
 @pytest.mark.parametrize("h", [2, 3, 4])
 @pytest.mark.parametrize("counter_bits", [8, 16])
 @pytest.mark.parametrize("unroll,waves", [(1, 1), (2, 4), (4, 2), (8, 4), (2, 1), (4, 4)])
