@@ -1,14 +1,18 @@
 // Host-side layout computation: a restatement of the reference's
 // hierarchical_binning / simple_binning DP (src/layout/hierarchical_binning.cpp,
-// src/layout/simple_binning.cpp) in its disable_estimate_union form, plus the
-// layout text format (src/layout/layout.cpp).
+// src/layout/simple_binning.cpp), with union estimates and similarity
+// rearrangement when sketches are given, plus the layout text format
+// (src/layout/layout.cpp).
 #include "layout.hpp"
 
 #include <algorithm>
 #include <barrier>
+#include <chrono>
 #include <cmath>
+#include <stdexcept>
 #include <thread>
 #include <limits>
+#include <memory>
 #include <sstream>
 #include <utility>
 
@@ -102,7 +106,13 @@ struct Store
     std::vector<double> const * fpr_correction;
     double relaxed_fpr_correction;
     std::vector<uint64_t> previous; // bin indices of the merged bins above
+    Sketching * sketching;          // null: sums instead of union estimates
 };
+
+double seconds_since(std::chrono::steady_clock::time_point t0)
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
 
 struct MaxTracker
 {
@@ -214,8 +224,32 @@ uint64_t hierarchical_binning(Store & d, Config const & c)
     auto const & counts = *d.kmer_counts;
     auto const & corr = *d.fpr_correction;
     double const rcorr = d.relaxed_fpr_correction;
-    // sketch::toolbox::sort_by_cardinalities (toolbox.cpp:23-33)
-    std::sort(d.positions.begin(), d.positions.end(), [&](uint64_t a, uint64_t b) { return counts[a] > counts[b]; });
+    // hierarchical_binning.cpp:35-56: sort by count, then rearrange by similarity.
+    seqan::hibf::sketch::toolbox::sort_by_cardinalities(counts, d.positions);
+    Sketching * const sk = d.sketching;
+    if (sk && !c.disable_rearrangement)
+    {
+        auto const t0 = std::chrono::steady_clock::now();
+        if (sk->pair_unions)
+            seqan::hibf::sketch::toolbox::rearrange_bins(*sk->sketches, counts, d.positions, c.max_rearrangement_ratio,
+                                                         c.threads, sk->pair_unions);
+        else
+            seqan::hibf::sketch::toolbox::rearrange_bins(*sk->sketches, counts, d.positions, c.max_rearrangement_ratio,
+                                                         c.threads);
+        sk->rearrangement_seconds += seconds_since(t0);
+    }
+    std::unique_ptr<HostUnionEstimates> host_unions;
+    UnionEstimates * unions = nullptr;
+    if (sk)
+    {
+        if (!sk->unions)
+            host_unions = std::make_unique<HostUnionEstimates>(*sk->sketches);
+        unions = sk->unions ? sk->unions : host_unions.get();
+        auto const t0 = std::chrono::steady_clock::now();
+        if (!unions->start(d.positions))
+            throw std::runtime_error("union estimates failed");
+        sk->union_seconds += seconds_since(t0);
+    }
 
     uint64_t const num_user_bins = d.positions.size();
     uint64_t const num_technical_bins =
@@ -235,10 +269,20 @@ uint64_t hierarchical_binning(Store & d, Config const & c)
         trace[i][0] = {0u, 0u};
     }
     uint64_t sum = counts[d.positions[0]];
+    std::vector<uint64_t> union_est; // prefix unions (row 0 of the DP)
+    uint64_t const * union_col = nullptr;
+    if (unions)
+    {
+        union_est.assign(num_user_bins, 0u);
+        auto const t0 = std::chrono::steady_clock::now();
+        if (!unions->prefix(union_est.data()))
+            throw std::runtime_error("union estimates failed");
+        sk->union_seconds += seconds_since(t0);
+    }
     for (uint64_t j = 1; j < num_user_bins; ++j)
     {
         sum += counts[d.positions[j]];
-        matrix[0][j] = static_cast<uint64_t>(sum * rcorr);
+        matrix[0][j] = static_cast<uint64_t>((unions ? union_est[j] : sum) * rcorr);
         ll[0][j] = max_merge_levels(c, j + 1) * sum;
         trace[0][j] = {0u, j - 1};
     }
@@ -286,12 +330,18 @@ uint64_t hierarchical_binning(Store & d, Config const & c)
         auto const & lprev = ll[i - 1];
         uint64_t jp = j - 1;
         uint64_t weight = current_weight;
-        auto get_weight = [&]() -> double { return rcorr * static_cast<double>(weight); };
-        while (jp != 0 && ((i - trow[jp].first) < 2) && get_weight() < static_cast<double>(minimum))
+        // :191-198: the merged bin's size is the union estimate of {jp + 1 .. j}
+        // (or the sum of the counts), relaxed-FPR corrected and truncated.
+        auto get_weight = [&]() -> uint64_t
+        {
+            uint64_t const uncorrected = !unions ? weight : jp + 1 == j ? current_weight : union_col[jp + 1];
+            return static_cast<uint64_t>(rcorr * static_cast<double>(uncorrected));
+        };
+        while (jp != 0 && ((i - trow[jp].first) < 2) && get_weight() < minimum)
         {
             weight += sorted_counts[jp];
             --jp;
-            uint64_t const score = std::max<uint64_t>(mprev[jp], static_cast<uint64_t>(get_weight()));
+            uint64_t const score = std::max<uint64_t>(mprev[jp], get_weight());
             uint64_t const ll_kmers = lprev[jp] + mml[j - jp] * weight;
             uint64_t const full_score =
                 static_cast<uint64_t>(static_cast<double>(score * (i + 1)) + c.alpha * static_cast<double>(ll_kmers));
@@ -310,10 +360,21 @@ uint64_t hierarchical_binning(Store & d, Config const & c)
     uint64_t const work = num_technical_bins * num_user_bins;
     unsigned const hw = std::max(1u, std::thread::hardware_concurrency());
     unsigned const nthreads = work < (1u << 16) ? 1u : std::min<unsigned>(16u, hw);
+    auto column_unions = [&](uint64_t j)
+    {
+        if (!unions)
+            return true;
+        auto const t0 = std::chrono::steady_clock::now();
+        union_col = unions->column(j);
+        sk->union_seconds += seconds_since(t0);
+        return union_col != nullptr;
+    };
     if (nthreads == 1)
     {
         for (uint64_t j = 1; j < num_user_bins; ++j)
         {
+            if (!column_unions(j))
+                throw std::runtime_error("union estimates failed");
             for (uint64_t i = 1; i < num_technical_bins; ++i)
                 cell(i, j);
             for (uint64_t i = 0; i < num_technical_bins; ++i)
@@ -326,10 +387,16 @@ uint64_t hierarchical_binning(Store & d, Config const & c)
     else
     {
         std::barrier sync(static_cast<std::ptrdiff_t>(nthreads));
+        bool failed = false;
         auto worker = [&](unsigned t)
         {
             for (uint64_t j = 1; j < num_user_bins; ++j)
             {
+                if (t == 0 && !column_unions(j))
+                    failed = true;
+                sync.arrive_and_wait();
+                if (failed)
+                    return;
                 for (uint64_t i = 1 + t; i < num_technical_bins; i += nthreads)
                     cell(i, j);
                 sync.arrive_and_wait();
@@ -347,26 +414,38 @@ uint64_t hierarchical_binning(Store & d, Config const & c)
         worker(0);
         for (auto & th : pool)
             th.join();
+        if (failed)
+            throw std::runtime_error("union estimates failed");
     }
 
     // backtracking (:302-331) with backtrack_merged_bin / backtrack_split_bin
     MaxTracker tracker;
+    // :258-287: with sketches the merged bin's size is the estimate of the
+    // union of its user bins' sketches.
     auto merged = [&](uint64_t tj, uint64_t next_j, uint64_t bin_id, bool is_first_row)
     {
         uint64_t kmer_count = counts[d.positions[tj]];
-        Store lower{d.layout, d.kmer_counts, {d.positions[tj]}, d.fpr_correction, d.relaxed_fpr_correction, d.previous};
+        seqan::hibf::sketch::hyperloglog merged_sketch =
+            sk ? (*sk->sketches)[d.positions[tj]] : seqan::hibf::sketch::hyperloglog{};
+        Store lower{d.layout, d.kmer_counts, {d.positions[tj]}, d.fpr_correction, d.relaxed_fpr_correction, d.previous,
+                    d.sketching};
         lower.previous.push_back(bin_id);
         tj -= !is_first_row;
         while (tj != next_j)
         {
             tj -= is_first_row;
-            kmer_count += counts[d.positions[tj]];
+            if (sk)
+                merged_sketch.merge((*sk->sketches)[d.positions[tj]]);
+            else
+                kmer_count += counts[d.positions[tj]];
             lower.positions.push_back(d.positions[tj]);
             tj -= !is_first_row;
         }
         std::vector<uint64_t> const prev = lower.previous;
         uint64_t const lower_max = add_lower_level(lower, c);
         d.layout->max_bins.push_back({prev, lower_max});
+        if (sk)
+            kmer_count = static_cast<uint64_t>(merged_sketch.estimate());
         tracker.update(bin_id, static_cast<uint64_t>(kmer_count * rcorr));
     };
     auto split = [&](uint64_t tj, uint64_t nb, uint64_t bin_id)
@@ -407,12 +486,39 @@ uint64_t hierarchical_binning(Store & d, Config const & c)
 
 } // namespace
 
-Layout compute_layout(Config const & c, std::vector<uint64_t> const & kmer_counts)
+bool HostUnionEstimates::start(std::vector<uint64_t> const & positions)
+{
+    positions_ = positions;
+    return true;
+}
+
+bool HostUnionEstimates::prefix(uint64_t * prefix)
+{
+    if (positions_.empty())
+        return true;
+    seqan::hibf::sketch::hyperloglog running = sketches_[positions_[0]];
+    for (uint64_t j = 1; j < positions_.size(); ++j)
+        prefix[j] = static_cast<uint64_t>(running.merge_and_estimate(sketches_[positions_[j]]));
+    return true;
+}
+
+uint64_t const * HostUnionEstimates::column(uint64_t j)
+{
+    column_.resize(j);
+    seqan::hibf::sketch::hyperloglog running = sketches_[positions_[j]];
+    for (uint64_t k = j; k-- > 0;)
+        column_[k] = static_cast<uint64_t>(running.merge_and_estimate(sketches_[positions_[k]]));
+    return column_.data();
+}
+
+Layout compute_layout(Config const & c, std::vector<uint64_t> const & kmer_counts, Sketching * sketching)
 {
     Layout layout;
     std::vector<double> const corr = fpr_correction(c.maximum_fpr, c.number_of_hash_functions, c.tmax);
+    if (c.disable_estimate_union || (sketching && !sketching->sketches))
+        sketching = nullptr;
     Store store{&layout, &kmer_counts, {}, &corr,
-                relaxed_fpr_correction(c.maximum_fpr, c.relaxed_fpr, c.number_of_hash_functions), {}};
+                relaxed_fpr_correction(c.maximum_fpr, c.relaxed_fpr, c.number_of_hash_functions), {}, sketching};
     store.positions.resize(c.number_of_user_bins);
     for (uint64_t i = 0; i < c.number_of_user_bins; ++i)
         store.positions[i] = i;

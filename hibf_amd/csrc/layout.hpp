@@ -9,6 +9,9 @@
 #include <string>
 #include <vector>
 
+#include <hibf/sketch/hyperloglog.hpp>
+#include <hibf/sketch/toolbox.hpp>
+
 namespace hibf_amd::host
 {
 
@@ -68,10 +71,57 @@ struct Layout
     static bool from_text(std::string const & text, Layout & out); // layout.cpp read_from
 };
 
+// Estimated sizes of the unions of contiguous runs of one DP level's ordered
+// user bins (toolbox::precompute_union_estimates_for and
+// precompute_initial_union_estimates).
+class UnionEstimates
+{
+public:
+    virtual ~UnionEstimates() = default;
+    // A DP level starts; positions are its user bins in DP order (sorted and
+    // rearranged).  Returns false on failure (the layout then fails).
+    virtual bool start(std::vector<uint64_t> const & positions) = 0;
+    // prefix[j] for 1 <= j < positions.size(): estimate of the union of positions[0 .. j].
+    virtual bool prefix(uint64_t * prefix) = 0;
+    // For j = 1, 2, ... in increasing order within one level: a pointer to j
+    // values, the k-th the estimate of the union of positions[k .. j]; valid
+    // until the next call.  Null on failure.
+    virtual uint64_t const * column(uint64_t j) = 0;
+};
+
+// The reference's sequential merges on the host, one column at a time.
+class HostUnionEstimates final : public UnionEstimates
+{
+public:
+    explicit HostUnionEstimates(std::vector<seqan::hibf::sketch::hyperloglog> const & sketches) : sketches_{sketches} {}
+    bool start(std::vector<uint64_t> const & positions) override;
+    bool prefix(uint64_t * prefix) override;
+    uint64_t const * column(uint64_t j) override;
+
+private:
+    std::vector<seqan::hibf::sketch::hyperloglog> const & sketches_;
+    std::vector<uint64_t> positions_;
+    std::vector<uint64_t> column_;
+};
+
+// What the union-estimate and rearrangement modes need: the user bins' sketches
+// (host copies), the DP's union estimates, and the merged-sketch estimates of
+// the clustering (toolbox::cluster_bins).  Null members fall back to the host.
+struct Sketching
+{
+    std::vector<seqan::hibf::sketch::hyperloglog> const * sketches{};
+    UnionEstimates * unions{};
+    seqan::hibf::sketch::toolbox::detail::union_batch pair_unions{};
+    double union_seconds{};       // time in union estimates (filled by compute_layout)
+    double rearrangement_seconds{}; // time in rearrange_bins
+};
+
 // layout/compute_layout.cpp:26-70 with hierarchical_binning / simple_binning.
-// Union cardinalities of merged bins are the sums of the user-bin
-// cardinalities (the reference's disable_estimate_union mode), and user bins
-// are not rearranged by similarity.
-Layout compute_layout(Config const & c, std::vector<uint64_t> const & kmer_counts);
+// Without sketches (or with disable_estimate_union) merged-bin cardinalities
+// are the sums of the user-bin cardinalities and user bins are not rearranged
+// (the reference's disable_estimate_union layout); with them the DP uses union
+// estimates and, unless disable_rearrangement, rearranges each level's user
+// bins by similarity first, as the reference.
+Layout compute_layout(Config const & c, std::vector<uint64_t> const & kmer_counts, Sketching * sketching = nullptr);
 
 } // namespace hibf_amd::host

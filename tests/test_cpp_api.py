@@ -16,7 +16,7 @@ def test_cpp_headers_compile(tmp_path):
     for hdr in ("hibf/interleaved_bloom_filter.hpp", "hibf/hierarchical_interleaved_bloom_filter.hpp",
                 "hibf/config.hpp", "hibf/misc/counting_vector.hpp", "hibf/misc/bit_vector.hpp",
                 "hibf/misc/print.hpp", "hibf/misc/timer.hpp", "hibf/layout/layout.hpp", "hibf/sketch/hyperloglog.hpp",
-                "hibf/sketch/compute_sketches.hpp", "hibf_gpu.h"):
+                "hibf/sketch/compute_sketches.hpp", "hibf/sketch/toolbox.hpp", "hibf_gpu.h"):
         src = tmp_path / "t.cpp"
         src.write_text(f"#include <{hdr}>\nint main() {{}}\n")
         r = subprocess.run(["g++", "-std=c++20", "-fsyntax-only", f"-I{ROOT}/include", str(src)],
@@ -195,3 +195,36 @@ def test_cpp_compute_sketches_on_device():
     r = subprocess.run([exe, "device"], input=text, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     assert r.stdout.splitlines()[-1] == f"device same {len(sizes)}"
+
+
+def test_cpp_sketch_toolbox_known_answers(tmp_path):
+    """seqan::hibf::sketch::toolbox (include/hibf/sketch/toolbox.hpp) prints
+    every expectation of the reference's toolbox_test.cpp: union estimates of
+    the four-sketch fixture (1635 = estimate of 1,500 mt19937_64(0) values),
+    the seeded shuffle, prune, rotate, trace, cluster_bins and the two
+    rearrange_bins cases (libstdc++ results)."""
+    exe = tmp_path / "toolbox"
+    r = subprocess.run(["g++", "-std=c++20", "-O2", "-Wall", f"-I{ROOT}/include",
+                        os.path.join(ROOT, "tests", "cpp", "toolbox.cpp"), "-o", str(exe)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    lines = r.stdout.splitlines()
+    assert lines[-1] == "ok" and len(lines) == 19, r.stdout
+
+
+@pytest.mark.parametrize("n,tmax,seed", [(500, 64, 1), (3000, 64, 2), (5000, 128, 3), (70, 64, 4)])
+def test_layout_union_mode_reduces_to_sum_mode(tmp_path, n, tmax, seed):
+    """With union estimates equal to the exact sums of the counts, the DP's
+    union-estimate mode places every user bin as the disable_estimate_union
+    mode (same user-bin records and merged-bin paths), over several levels."""
+    exe = tmp_path / "layout_union"
+    r = subprocess.run(["g++", "-std=c++20", "-O2", "-Wall", f"-I{ROOT}/include", f"-I{ROOT}/hibf_amd/csrc",
+                        os.path.join(ROOT, "tests", "cpp", "layout_union.cpp"),
+                        os.path.join(ROOT, "hibf_amd", "csrc", "layout.cpp"), "-o", str(exe)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    r = subprocess.run([str(exe), str(n), str(tmax), str(seed)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.startswith("ok "), r.stdout
