@@ -394,7 +394,7 @@ def hibf_sketch_leg(sizes, spec, args):
         f"({total / sketch_s / 1e9:.1f} G values/s), estimates {est_s * 1e3:.1f} ms; "
         f"median relative error {np.median(np.abs(counts / sizes - 1)):.4f}")
     if args is None:
-        return None, counts
+        return None, counts, regs
     leg = {"user_bins": int(sizes.size), "values": total, "bits": bits, "seconds": sketch_s,
            "values_per_s": total / sketch_s, "estimate_seconds": est_s,
            "median_relative_error": float(np.median(np.abs(counts / sizes - 1)))}
@@ -421,7 +421,7 @@ def hibf_sketch_leg(sizes, spec, args):
             cpu_s = time.perf_counter() - t0
             leg["cpu_baseline"] = {"value": reps * vals.size / cpu_s, "unit": "values/s", "cores": 1, "kind": "port",
                                    "sample": f"oracle hyperloglog add of user bin {u} ({vals.size} values) x {reps}"}
-    return leg, counts
+    return leg, counts, regs
 
 
 def run_hibf(args):
@@ -442,11 +442,14 @@ def run_hibf(args):
     n_ub = args.hibf_user_bins
     sizes = W.hibf_user_bin_sizes(spec["size_seed"], n_ub)
     # (f) build side: one HyperLogLog sketch per user bin on the device; the layout sees their estimates.
-    sketch_leg, counts = hibf_sketch_leg(sizes, spec, args)
+    sketch_leg, counts, regs = hibf_sketch_leg(sizes, spec, args)
     t0 = time.perf_counter()
-    layout = W.cached_layout(counts, spec["config"])
+    layout_times = {}
+    layout = W.cached_layout(counts, spec["config"], registers=regs, times=layout_times)
     layout_s = time.perf_counter() - t0
-    log(f"layout ({n_ub} user bins, {sizes.sum():.3e} values, HyperLogLog estimates): {layout_s:.1f} s")
+    del regs
+    log(f"layout ({n_ub} user bins, {sizes.sum():.3e} values, HyperLogLog estimates, union estimates and "
+        f"rearrangement): {layout_s:.1f} s" + (f" {layout_times}" if layout_times else " (stored)"))
     t0 = time.perf_counter()
     hibf = DeviceHibf.build_synthetic(sizes, spec["build_seed"], layout_text=layout, **spec["config"])
     build_s = time.perf_counter() - t0
@@ -730,6 +733,8 @@ def run_hibf(args):
         "parity_sample_ok": parity,
         "build_s": build_s,
         "layout_s": layout_s,
+        "layout": {"mode": "union estimates + rearrangement (reference default)",
+                   "computed": layout_times or None},
         "sketches": sketch_leg,
     }
     print(json.dumps(out), flush=True)
@@ -747,7 +752,9 @@ def run_serve(args):
     spec = W.HIBF_AT_SCALE
     n_ub = args.hibf_user_bins
     sizes = W.hibf_user_bin_sizes(spec["size_seed"], n_ub)
-    layout = W.cached_layout(hibf_sketch_leg(sizes, spec, None)[1], spec["config"])
+    _, counts, regs = hibf_sketch_leg(sizes, spec, None)
+    layout = W.cached_layout(counts, spec["config"], registers=regs)
+    del regs
     hibf = DeviceHibf.build_synthetic(sizes, spec["build_seed"], layout_text=layout, **spec["config"])
     tab = hibf.tables()
     rates = [float(r) for r in args.serve_rates.split(",")]

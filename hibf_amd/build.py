@@ -16,7 +16,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libhibf_gpu.so")
 
 HIP_SOURCES = ["ibf_kernels.hip", "count_wide.hip", "hibf_kernels.hip", "hibf_wave.hip", "serving.hip", "scan.hip",
-               "kmer.hip", "hll.hip", "capi.hip"]
+               "kmer.hip", "hll.hip", "hll_union.hip", "capi.hip"]
 CXX_SOURCES = ["layout.cpp", "build.cpp", "archive.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

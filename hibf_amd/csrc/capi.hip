@@ -15,6 +15,7 @@
 #include "hibf_gpu.h"
 #include "build.hpp"
 #include "kernels.hpp"
+#include "hll_union.hpp"
 #include "layout.hpp"
 
 using namespace hibf_amd;
@@ -190,6 +191,7 @@ struct hibf_gpu_hibf_s
     // Phase times of the build that made this HIBF (seconds): sketches and
     // their estimates, layout, allocation (arena and tables), device fill.
     double build_times[4]{};
+    double layout_times[2]{}; // union estimation and rearrangement (parts of build_times[1])
     uint64_t * d_arena{nullptr};
     HibfIbfDesc * d_desc{nullptr};
     uint64_t * d_tb_info{nullptr};
@@ -979,6 +981,15 @@ int hibf_gpu_hibf_build_times(hibf_gpu_hibf h, double * h_seconds)
     return HIBF_GPU_OK;
 }
 
+int hibf_gpu_hibf_layout_times(hibf_gpu_hibf h, double * h_seconds)
+{
+    if (!h || !h_seconds)
+        return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "null argument");
+    h_seconds[0] = h->layout_times[0];
+    h_seconds[1] = h->layout_times[1];
+    return HIBF_GPU_OK;
+}
+
 int hibf_gpu_hibf_ibf_view(hibf_gpu_hibf h, uint64_t i, hibf_gpu_ibf * out)
 {
     if (!h || !out)
@@ -1333,14 +1344,14 @@ int estimate_kmer_counts(host::Config const & hc,
                          uint64_t const * d_offsets,
                          uint64_t const * h_sizes,
                          uint64_t seed,
-                         std::vector<uint64_t> & counts)
+                         std::vector<uint64_t> & counts,
+                         DeviceBuffer & regs)
 {
     uint64_t const n = hc.number_of_user_bins;
     uint32_t const bits = static_cast<uint32_t>(hc.sketch_bits);
     counts.assign(n, 0u);
     if (n > (~0ULL >> bits))
         return fail(HIBF_GPU_ERR_UNSUPPORTED, "number_of_user_bins x 2^sketch_bits sketch registers exceed 2^64 bytes");
-    DeviceBuffer regs;
     HIBF_TRY_HIP(regs.alloc(static_cast<size_t>(n) << bits), "sketch registers");
     if (d_values)
     {
@@ -1427,6 +1438,8 @@ int run_plan(host::Plan const & plan,
     auto const t_end = std::chrono::steady_clock::now();
     h->build_times[0] = phase_s[0];
     h->build_times[1] = phase_s[1];
+    h->layout_times[0] = phase_s[2];
+    h->layout_times[1] = phase_s[3];
     h->build_times[2] = std::chrono::duration<double>(t_fill - t_alloc).count();
     h->build_times[3] = std::chrono::duration<double>(t_end - t_fill).count();
     *out = h;
@@ -1446,10 +1459,66 @@ int get_layout(host::Config const & hc, std::vector<uint64_t> const & counts, ch
     return HIBF_GPU_OK;
 }
 
+// The layout over HyperLogLog sketches resident on the device, with union
+// estimates and (unless disabled) similarity rearrangement, as the reference's
+// compute_layout(config, kmer_counts, sketches): the union estimates of the DP
+// and the clustering's merged-sketch estimates come from the device (sketches
+// of at most 12 bits; wider ones are estimated on the host); the clustering's
+// bookkeeping runs on host copies of the sketches.  times[0]
+// and [1] get the union-estimation and rearrangement seconds.
+int sketched_layout(host::Config const & hc,
+                    std::vector<uint64_t> const & counts,
+                    uint8_t const * d_regs,
+                    host::Layout & layout,
+                    double * times)
+{
+    uint64_t const n = hc.number_of_user_bins;
+    uint32_t const bits = static_cast<uint32_t>(hc.sketch_bits);
+    std::vector<uint8_t> regs(static_cast<size_t>(n) << bits);
+    HIBF_TRY_HIP(hipDeviceSynchronize(), "sketch registers"); // the sketches may come from any stream
+    HIBF_TRY_HIP(hipMemcpy(regs.data(), d_regs, regs.size(), hipMemcpyDeviceToHost), "sketch registers");
+    std::vector<seqan::hibf::sketch::hyperloglog> sketches;
+    sketches.reserve(n);
+    for (uint64_t u = 0; u < n; ++u)
+        sketches.push_back(seqan::hibf::sketch::hyperloglog::from_registers(static_cast<uint8_t>(bits),
+                                                                            regs.data() + (u << bits)));
+    std::unique_ptr<DeviceUnionEstimates> device_unions;
+    std::unique_ptr<DevicePairUnions> device_pairs;
+    host::Sketching sk;
+    sk.sketches = &sketches;
+    if (bits <= kUnionMaxBits)
+    {
+        if (int rc = DeviceUnionEstimates::create(d_regs, n, bits, device_unions))
+            return rc;
+        if (int rc = DevicePairUnions::create(bits, device_pairs))
+            return rc;
+        sk.unions = device_unions.get();
+        sk.pair_unions = device_pairs.get();
+    }
+    try
+    {
+        layout = host::compute_layout(hc, counts, &sk);
+    }
+    catch (std::out_of_range const &)
+    {
+        // An empty user bin: its distance to every other is 0 / 0 and the
+        // clustering finds no closest pair (the reference fails the same way).
+        return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "similarity rearrangement needs non-empty user bins");
+    }
+    catch (std::runtime_error const &)
+    {
+        return HIBF_GPU_ERR_DEVICE; // the union estimates set the message
+    }
+    times[0] = sk.union_seconds;
+    times[1] = sk.rearrangement_seconds;
+    return HIBF_GPU_OK;
+}
+
 // The layout of a build: the given text, or the DP over the user bins'
 // HyperLogLog estimates (values from the device CSR, or the synthetic stream
-// of `seed` when d_values is null).  phase_s[0] and [1] get the sketch and
-// layout times.
+// of `seed` when d_values is null), with union estimates from the sketches
+// unless disable_estimate_union.  phase_s[0] .. [3] get the sketch, layout,
+// union-estimation and rearrangement times.
 int layout_for(host::Config const & hc,
                std::vector<uint64_t> const & sizes,
                char const * layout_text,
@@ -1469,10 +1538,13 @@ int layout_for(host::Config const & hc,
     }
     auto const t0 = clock::now();
     std::vector<uint64_t> kmer_counts;
-    if (int rc = estimate_kmer_counts(hc, d_values, d_offsets, d_values ? nullptr : sizes.data(), seed, kmer_counts))
+    DeviceBuffer regs;
+    if (int rc = estimate_kmer_counts(hc, d_values, d_offsets, d_values ? nullptr : sizes.data(), seed, kmer_counts,
+                                      regs))
         return rc;
     auto const t1 = clock::now();
-    int const rc = get_layout(hc, kmer_counts, nullptr, layout);
+    int const rc = hc.disable_estimate_union ? get_layout(hc, kmer_counts, nullptr, layout)
+                                             : sketched_layout(hc, kmer_counts, regs.u8(), layout, phase_s + 2);
     phase_s[0] = std::chrono::duration<double>(t1 - t0).count();
     phase_s[1] = std::chrono::duration<double>(clock::now() - t1).count();
     return rc;
@@ -1513,6 +1585,42 @@ int hibf_gpu_layout_compute(hibf_gpu_config const * c,
     try
     {
         return copy_text(host::compute_layout(hc, counts).to_text(), text, cap, needed);
+    }
+    catch (std::exception const & ex)
+    {
+        return fail(HIBF_GPU_ERR_OUT_OF_MEMORY, ex.what());
+    }
+}
+
+int hibf_gpu_layout_compute_sketched(hibf_gpu_config const * c,
+                                     uint64_t const * h_kmer_counts,
+                                     uint8_t const * d_registers,
+                                     char * text,
+                                     uint64_t cap,
+                                     uint64_t * needed,
+                                     double * h_times)
+{
+    if (!c || !h_kmer_counts || !d_registers)
+        return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "null argument");
+    host::Config hc = to_host(*c);
+    std::string const e = host::validate_and_set_defaults(hc);
+    if (!e.empty())
+        return fail(HIBF_GPU_ERR_CONFIG, e);
+    std::vector<uint64_t> counts(h_kmer_counts, h_kmer_counts + hc.number_of_user_bins);
+    try
+    {
+        host::Layout layout;
+        double times[2]{};
+        int const rc = hc.disable_estimate_union ? get_layout(hc, counts, nullptr, layout)
+                                                 : sketched_layout(hc, counts, d_registers, layout, times);
+        if (rc)
+            return rc;
+        if (h_times)
+        {
+            h_times[0] = times[0];
+            h_times[1] = times[1];
+        }
+        return copy_text(layout.to_text(), text, cap, needed);
     }
     catch (std::exception const & ex)
     {
@@ -1571,7 +1679,7 @@ int hibf_gpu_hibf_build(hibf_gpu_config const * c,
         HIBF_TRY_HIP(hipMemcpy(d_offs.p, offs.data(), offs.size() * 8u, hipMemcpyHostToDevice), "build offsets");
         // The layout sees HyperLogLog estimates, as in the reference; IBF sizes use the exact sets
         // (the reference sizes each IBF from the k-mers it inserts).
-        double phase_s[2]{};
+        double phase_s[4]{};
         host::Layout layout;
         if (int rc = layout_for(hc, in.ub_size, layout_text, d_vals.u64(), d_offs.u64(), 0, layout, phase_s))
             return rc;
@@ -1661,7 +1769,7 @@ int hibf_gpu_hibf_build_synthetic(hibf_gpu_config const * c,
         for (uint64_t s : in.ub_size)
             if (s >= (1ULL << 36))
                 return fail(HIBF_GPU_ERR_UNSUPPORTED, "the synthetic stream supports < 2^36 values per user bin");
-        double phase_s[2]{};
+        double phase_s[4]{};
         host::Layout layout;
         if (int rc = layout_for(hc, in.ub_size, layout_text, nullptr, nullptr, seed, layout, phase_s))
             return rc;

@@ -232,7 +232,7 @@ uint64_t hierarchical_binning(Store & d, Config const & c)
         auto const t0 = std::chrono::steady_clock::now();
         if (sk->pair_unions)
             seqan::hibf::sketch::toolbox::rearrange_bins(*sk->sketches, counts, d.positions, c.max_rearrangement_ratio,
-                                                         c.threads, sk->pair_unions);
+                                                         c.threads, *sk->pair_unions);
         else
             seqan::hibf::sketch::toolbox::rearrange_bins(*sk->sketches, counts, d.positions, c.max_rearrangement_ratio,
                                                          c.threads);

@@ -111,7 +111,7 @@ struct Sketching
 {
     std::vector<seqan::hibf::sketch::hyperloglog> const * sketches{};
     UnionEstimates * unions{};
-    seqan::hibf::sketch::toolbox::detail::union_batch pair_unions{};
+    seqan::hibf::sketch::toolbox::detail::union_source * pair_unions{};
     double union_seconds{};       // time in union estimates (filled by compute_layout)
     double rearrangement_seconds{}; // time in rearrange_bins
 };

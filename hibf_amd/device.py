@@ -136,6 +136,20 @@ def hll_estimates(registers: torch.Tensor) -> np.ndarray:
     return out[:n]
 
 
+def hll_union_estimates(registers: torch.Tensor, positions) -> torch.Tensor:
+    """Union estimates of the layout DP (hibf_gpu_hll_union_estimates) for the
+    sketches `registers` (n x 2^bits uint8, device) taken in `positions` order:
+    a uint64 device tensor, column j's j values (k = 0 .. j - 1) starting at
+    j (j - 1) / 2."""
+    bits = int(registers.shape[1]).bit_length() - 1
+    pos = as_device_u64(np.asarray(positions, dtype=np.uint64))
+    npos = pos.numel()
+    out = torch.empty(max(npos * (npos - 1) // 2, 1), dtype=torch.uint64, device=registers.device)
+    regs = registers.contiguous()
+    check(lib().hibf_gpu_hll_union_estimates(_ptr(regs), regs.shape[0], bits, _ptr(pos), npos, _ptr(out), _stream()))
+    return out[: npos * (npos - 1) // 2]
+
+
 def csr(queries) -> tuple[np.ndarray, np.ndarray]:
     """(values, offsets) for a list of uint64 sequences."""
     lens = [len(q) for q in queries]

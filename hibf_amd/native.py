@@ -126,6 +126,7 @@ _SIGNATURES = {
     "hibf_gpu_hibf_device_arena": (_INT, [_V, ctypes.POINTER(_V)]),
     "hibf_gpu_hibf_ibf_view": (_INT, [_V, _U64, ctypes.POINTER(_V)]),
     "hibf_gpu_hibf_build_times": (_INT, [_V, _V]),
+    "hibf_gpu_hibf_layout_times": (_INT, [_V, _V]),
     "hibf_gpu_ibf_occupancy": (_INT, [_V, _V, ctypes.POINTER(_INT)]),
     "hibf_gpu_ibf_try_increase_bin_number_to": (_INT, [_V, _U64, ctypes.POINTER(_INT)]),
     "hibf_gpu_ibf_increase_bin_number_to": (_INT, [_V, _U64]),
@@ -136,6 +137,7 @@ _SIGNATURES = {
     "hibf_gpu_hll_sketch_synthetic": (_INT, [_U64, _V, _U64, ctypes.c_uint32, _V, _V]),
     "hibf_gpu_hll_merge": (_INT, [_V, ctypes.c_uint32, _V, _V, _U64, _V, _V]),
     "hibf_gpu_hll_estimate": (_INT, [_V, _U64, ctypes.c_uint32, _V, _V]),
+    "hibf_gpu_hll_union_estimates": (_INT, [_V, _U64, ctypes.c_uint32, _V, _U64, _V, _V]),
     "hibf_gpu_ibf_save": (_INT, [_V, ctypes.c_char_p, _INT]),
     "hibf_gpu_ibf_load": (_INT, [ctypes.c_char_p, _U64, ctypes.POINTER(_V), ctypes.POINTER(_U64)]),
     "hibf_gpu_hibf_save": (_INT, [_V, ctypes.c_char_p, _INT]),
@@ -151,6 +153,8 @@ _SIGNATURES = {
     "hibf_gpu_config_default": (_INT, [ctypes.POINTER(Config)]),
     "hibf_gpu_config_validate": (_INT, [ctypes.POINTER(Config)]),
     "hibf_gpu_layout_compute": (_INT, [ctypes.POINTER(Config), _V, ctypes.c_char_p, _U64, ctypes.POINTER(_U64)]),
+    "hibf_gpu_layout_compute_sketched": (_INT, [ctypes.POINTER(Config), _V, _V, ctypes.c_char_p, _U64,
+                                                ctypes.POINTER(_U64), _V]),
     "hibf_gpu_hibf_build": (_INT, [ctypes.POINTER(Config), _V, _V, ctypes.c_char_p, ctypes.POINTER(_V)]),
     "hibf_gpu_hibf_build_synthetic": (_INT, [ctypes.POINTER(Config), _V, _U64, ctypes.c_char_p, ctypes.POINTER(_V)]),
     "hibf_gpu_hibf_plan_stats": (_INT, [ctypes.POINTER(Config), _V, ctypes.c_char_p] + [ctypes.POINTER(_U64)] * 4),
@@ -224,6 +228,28 @@ def compute_layout(c: Config, kmer_counts) -> str:
     buf = ctypes.create_string_buffer(need.value)
     check(lib().hibf_gpu_layout_compute(ctypes.byref(c), k.ctypes.data_as(ctypes.c_void_p), buf, need.value,
                                         ctypes.byref(need)))
+    return buf.value.decode()
+
+
+def compute_layout_sketched(c: Config, kmer_counts, d_registers: int, times: list | None = None) -> str:
+    """Layout text with union estimates and rearrangement from the user bins'
+    sketches (device address of number_of_user_bins x 2^sketch_bits registers).
+    `times`, if given, receives [union estimation s, rearrangement s]."""
+    import numpy as np
+    k = np.ascontiguousarray(np.asarray(kmer_counts, dtype=np.uint64))
+    kp = k.ctypes.data_as(ctypes.c_void_p)
+    t = (ctypes.c_double * 2)()
+    need = ctypes.c_uint64(0)
+    buf = ctypes.create_string_buffer(16 << 20)  # the text of 32,768 user bins is about 1.5 MB
+    rc = lib().hibf_gpu_layout_compute_sketched(ctypes.byref(c), kp, d_registers, buf, len(buf), ctypes.byref(need),
+                                                ctypes.cast(t, ctypes.c_void_p))
+    if rc == ERR_CAPACITY:  # a second pass with room for the text
+        buf = ctypes.create_string_buffer(need.value)
+        rc = lib().hibf_gpu_layout_compute_sketched(ctypes.byref(c), kp, d_registers, buf, need.value,
+                                                    ctypes.byref(need), ctypes.cast(t, ctypes.c_void_p))
+    check(rc)
+    if times is not None:
+        times[:] = [t[0], t[1]]
     return buf.value.decode()
 
 

@@ -67,17 +67,30 @@ def layout_key(sizes: np.ndarray, config: dict) -> str:
     return h.hexdigest()[:16]
 
 
-def cached_layout(sizes: np.ndarray, config: dict, write: bool = True) -> str:
+def cached_layout(sizes: np.ndarray, config: dict, write: bool = True, registers=None, times: dict | None = None) -> str:
     """Layout text for these user-bin sizes and config.  The host DP is
     quadratic in the number of user bins (about a minute at 32,768), so a
-    computed layout is stored as layouts/<key>.layout.gz and reused."""
+    computed layout is stored as layouts/<key>.layout.gz and reused.  With
+    `registers` (the user bins' sketches on the device) and union estimates
+    enabled, the layout uses union estimates and rearrangement, as the
+    reference's default; `times` then receives the computation's seconds."""
     from . import native
-    path = os.path.join(_LAYOUT_DIR, layout_key(sizes, config) + ".layout.gz")
+    sketched = registers is not None and not config.get("disable_estimate_union", False)
+    key = layout_key(sizes, dict(config, layout="sketched") if sketched else config)
+    path = os.path.join(_LAYOUT_DIR, key + ".layout.gz")
     if os.path.exists(path):
         with gzip.open(path, "rt") as f:
             return f.read()
-    c = native.make_config(number_of_user_bins=int(sizes.size), **config)
-    text = native.compute_layout(c, sizes)
+    c = native.validate_config(native.make_config(number_of_user_bins=int(sizes.size), **config))
+    if sketched:
+        import time
+        parts = [0.0, 0.0]
+        t0 = time.perf_counter()
+        text = native.compute_layout_sketched(c, sizes, registers.data_ptr(), parts)
+        if times is not None:
+            times.update(total_s=time.perf_counter() - t0, union_estimation_s=parts[0], rearrangement_s=parts[1])
+    else:
+        text = native.compute_layout(c, sizes)
     if write:
         try:
             os.makedirs(_LAYOUT_DIR, exist_ok=True)

@@ -217,7 +217,13 @@ private:
         double phase[4]{};
         detail::check(hibf_gpu_hibf_build_times(h, phase));
         if (!layout_text)
+        {
+            double parts[2]{};
+            detail::check(hibf_gpu_hibf_layout_times(h, parts));
             layout_compute_sketches_timer.add_seconds(phase[0]);
+            layout_union_estimation_timer.add_seconds(parts[0]);
+            layout_rearrangement_timer.add_seconds(parts[1]);
+        }
         layout_dp_algorithm_timer.add_seconds(phase[1]);
         index_allocation_timer.add_seconds(phase[2]);
         fill_ibf_timer.add_seconds(phase[3]);

@@ -193,18 +193,29 @@ inline void trace(std::vector<clustering_node> const & clustering,
 namespace detail
 {
 
-// Estimates of the unions of one sketch with each of a list of others: out[k] =
-// estimate(merge(a, others[k])).  The default computes them one by one on the host.
-using union_batch = std::function<void(hyperloglog const & a, std::vector<hyperloglog const *> const & others, double * out)>;
-
-inline void host_union_batch(hyperloglog const & a, std::vector<hyperloglog const *> const & others, double * out)
+// Estimates of the unions of one sketch with each of a list of others, for
+// the clustering: unions(a, others, out) sets out[k] = estimate(merge(a,
+// *others[k])).  reset() is called when a clustering starts; within one, a
+// sketch object is not modified once passed (so an implementation may cache
+// it by address).  The default computes the estimates one by one on the host.
+struct union_source
 {
-    for (size_t k = 0; k < others.size(); ++k)
+    virtual ~union_source() = default;
+    virtual void reset() {}
+    virtual void unions(hyperloglog const & a, std::vector<hyperloglog const *> const & others, double * out) = 0;
+};
+
+struct host_union_source final : union_source
+{
+    void unions(hyperloglog const & a, std::vector<hyperloglog const *> const & others, double * out) override
     {
-        hyperloglog tmp = a;
-        out[k] = tmp.merge_and_estimate(*others[k]);
+        for (size_t k = 0; k < others.size(); ++k)
+        {
+            hyperloglog tmp = a;
+            out[k] = tmp.merge_and_estimate(*others[k]);
+        }
     }
-}
+};
 
 } // namespace detail
 
@@ -219,8 +230,9 @@ inline void cluster_bins(std::vector<hyperloglog> const & sketches,
                          size_t const first,
                          size_t const last,
                          size_t const /* num_threads */,
-                         detail::union_batch const & unions)
+                         detail::union_source & unions)
 {
+    unions.reset();
     constexpr size_t none = std::numeric_limits<size_t>::max();
     size_t const prune_steps = static_cast<size_t>(std::floor(std::sqrt(sketches.size())));
     size_t steps_without_prune = 0;
@@ -269,7 +281,7 @@ inline void cluster_bins(std::vector<hyperloglog> const & sketches,
                 others.push_back(&clustering[j].hll);
             merged.resize(others.size());
             if (!others.empty())
-                unions(clustering[i].hll, others, merged.data());
+                unions.unions(clustering[i].hll, others, merged.data());
             for (size_t j = i + 1; j < leaves; ++j)
                 dist[i].pq.push({j + first, 2 - (estimates[i] + estimates[j]) / merged[j - i - 1]});
             if (dist[i].pq.empty())
@@ -321,7 +333,7 @@ inline void cluster_bins(std::vector<hyperloglog> const & sketches,
         }
         merged.resize(others.size());
         if (!others.empty())
-            unions(fresh, others, merged.data());
+            unions.unions(fresh, others, merged.data());
         min_id = none;
         min_dist = std::numeric_limits<double>::max();
         for (size_t k = 0; k < other_index.size(); ++k)
@@ -351,7 +363,8 @@ inline void cluster_bins(std::vector<hyperloglog> const & sketches,
                          size_t const last,
                          size_t const num_threads)
 {
-    cluster_bins(sketches, positions, permutation, first, last, num_threads, detail::host_union_batch);
+    detail::host_union_source host;
+    cluster_bins(sketches, positions, permutation, first, last, num_threads, host);
 }
 
 // toolbox.cpp:72-107: intervals of the sorted order in which every count is at
@@ -362,7 +375,7 @@ inline void rearrange_bins(std::vector<hyperloglog> const & sketches,
                            std::vector<size_t> & positions,
                            double const max_ratio,
                            size_t const num_threads,
-                           detail::union_batch const & unions)
+                           detail::union_source & unions)
 {
     std::vector<size_t> permutation;
     size_t first = 0;
@@ -389,7 +402,8 @@ inline void rearrange_bins(std::vector<hyperloglog> const & sketches,
                            double const max_ratio,
                            size_t const num_threads)
 {
-    rearrange_bins(sketches, kmer_counts, positions, max_ratio, num_threads, detail::host_union_batch);
+    detail::host_union_source host;
+    rearrange_bins(sketches, kmer_counts, positions, max_ratio, num_threads, host);
 }
 
 } // namespace seqan::hibf::sketch::toolbox

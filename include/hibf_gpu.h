@@ -268,6 +268,12 @@ int hibf_gpu_hibf_device_arena(hibf_gpu_hibf hibf, uint64_t ** d_arena);
  * (hierarchical_interleaved_bloom_filter.hpp:283-290). */
 int hibf_gpu_hibf_build_times(hibf_gpu_hibf hibf, double * h_seconds);
 
+/* Parts of the layout time of that build, in seconds: [0] union estimation
+ * (on the device), [1] similarity rearrangement.  Both 0 when the layout did
+ * not use union estimates.  They feed layout_union_estimation_timer and
+ * layout_rearrangement_timer. */
+int hibf_gpu_hibf_layout_times(hibf_gpu_hibf hibf, double * h_seconds);
+
 /* ibf_vector[i] (hierarchical_interleaved_bloom_filter.hpp:209): an IBF handle
  * whose bit matrix is IBF i's words in the HIBF's arena (no copy).  Queries,
  * emplace, clear, upload and download act on the arena; the bin count cannot
@@ -496,6 +502,21 @@ int hibf_gpu_hll_merge(const uint8_t * d_registers,
 int hibf_gpu_hll_estimate(const uint8_t * d_registers, uint64_t n_sketches, uint32_t bits, double * h_estimates,
                           void * stream);
 
+/* The layout DP's union estimates (sketch::toolbox::precompute_union_estimates_for,
+ * src/sketch/toolbox.cpp:35-51, for every column at once): for all
+ * 0 <= k < j < n_positions, the estimate of the union of the sketches
+ * d_positions[k] .. d_positions[j] (merge_and_estimate, truncated to uint64),
+ * bit-identical to the reference's.  Column j's j values start at
+ * d_out[j (j - 1) / 2]; d_out holds n_positions (n_positions - 1) / 2 values.
+ * Sketches of 5 to 12 bits (else HIBF_GPU_ERR_UNSUPPORTED).  Synchronous. */
+int hibf_gpu_hll_union_estimates(const uint8_t * d_registers,
+                                 uint64_t n_sketches,
+                                 uint32_t bits,
+                                 const uint64_t * d_positions,
+                                 uint64_t n_positions,
+                                 uint64_t * d_out,
+                                 void * stream);
+
 /* ======================================================================== */
 /* Archives: cereal BinaryArchive layout                                    */
 /* ======================================================================== */
@@ -570,6 +591,25 @@ int hibf_gpu_layout_compute(const hibf_gpu_config * c,
                             char * text,
                             uint64_t cap,
                             uint64_t * needed);
+
+/* compute_layout(config, kmer_counts, sketches) (layout/compute_layout.cpp:26-70):
+ * as hibf_gpu_layout_compute, but with the user bins' HyperLogLog sketches
+ * (d_registers: number_of_user_bins x 2^sketch_bits uint8, device) the DP
+ * uses union estimates of merged bins and, unless disable_rearrangement,
+ * first rearranges each level's user bins by similarity
+ * (sketch::toolbox::rearrange_bins), as the reference.  The union and pair
+ * estimates run on the device for sketches of at most 12 bits.  With
+ * disable_estimate_union the sketches are not used.  A user bin with an empty
+ * sketch makes the rearrangement fail (HIBF_GPU_ERR_INVALID_ARGUMENT), as in
+ * the reference.  h_times (may be null) gets the seconds spent in union
+ * estimation [0] and rearrangement [1]. */
+int hibf_gpu_layout_compute_sketched(const hibf_gpu_config * config,
+                                     const uint64_t * h_kmer_counts,
+                                     const uint8_t * d_registers,
+                                     char * text,
+                                     uint64_t cap,
+                                     uint64_t * needed,
+                                     double * h_times);
 
 /* Replaces hierarchical_interleaved_bloom_filter(config&) and (config&, layout)
  * (src/hierarchical_interleaved_bloom_filter.cpp:211-253): user bin u holds

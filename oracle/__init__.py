@@ -118,6 +118,8 @@ def lib():
         L.orc_hll_merge.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32]
         L.orc_hll_estimate.restype = ctypes.c_double
         L.orc_hll_estimate.argtypes = [ctypes.c_void_p, ctypes.c_uint32]
+        L.orc_hll_union_estimates.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint64,
+                                              ctypes.c_void_p, ctypes.c_void_p]
         L.orc_mt19937_64.argtypes = [ctypes.c_uint64, ctypes.c_void_p, ctypes.c_uint64]
         _lib = L
     return _lib
@@ -352,6 +354,21 @@ def hll_merge(a: np.ndarray, b: np.ndarray) -> np.ndarray:
 def hll_estimate(regs: np.ndarray) -> float:
     r = np.ascontiguousarray(regs, dtype=np.uint8)
     return float(lib().orc_hll_estimate(_p(r), int(r.size).bit_length() - 1))
+
+
+def hll_union_estimates(registers: np.ndarray, positions) -> np.ndarray:
+    """Union estimates of the layout DP (toolbox.cpp:35-51) for every column j
+    and k < j of `positions` over sketches `registers` (n x 2^bits uint8),
+    packed by column: column j's j values start at j (j - 1) / 2."""
+    regs = np.ascontiguousarray(registers, dtype=np.uint8)
+    bits = int(regs.shape[1]).bit_length() - 1
+    pos = _u64(positions)
+    np_ = pos.size
+    out = np.zeros(max(np_ * (np_ - 1) // 2, 1), dtype=np.uint64)
+    scratch = np.zeros(regs.shape[1], dtype=np.uint8)
+    if np_ >= 2:
+        lib().orc_hll_union_estimates(_p(regs), bits, _p(pos), np_, _p(scratch), _p(out))
+    return out[:np_ * (np_ - 1) // 2]
 
 
 def mt19937_64(seed: int, n: int) -> np.ndarray:

@@ -583,6 +583,27 @@ double orc_hll_estimate(uint8_t const * regs, uint32_t bits)
     return estimate;
 }
 
+/* The layout DP's union estimates (src/sketch/toolbox.cpp:35-51,
+ * precompute_union_estimates_for): for every column j < np and k < j, the
+ * estimate of the union of the sketches at positions[k .. j], merged one by
+ * one downward from j and truncated to uint64.  Column j's j values start at
+ * out[j (j - 1) / 2].  `regs` holds the sketches back to back; `scratch` one
+ * sketch. */
+void orc_hll_union_estimates(uint8_t const * regs, uint32_t bits, uint64_t const * positions, uint64_t np,
+                             uint8_t * scratch, uint64_t * out)
+{
+    uint64_t const size = 1ULL << bits;
+    for (uint64_t j = 1; j < np; ++j)
+    {
+        memcpy(scratch, regs + positions[j] * size, size);
+        for (uint64_t k = j; k-- > 0;)
+        {
+            orc_hll_merge(scratch, regs + positions[k] * size, bits);
+            out[j * (j - 1u) / 2u + k] = (uint64_t)orc_hll_estimate(scratch, bits);
+        }
+    }
+}
+
 /* std::mt19937_64 (the C++ standard's parameters, [rand.predef]) seeded with
  * `seed`; the reference's HLL tests draw their inputs from it through
  * uniform_int_distribution<uint64_t> over the full range, which returns the

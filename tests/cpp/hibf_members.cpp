@@ -120,6 +120,7 @@ int main()
                          && hibf.layout_compute_sketches_timer.in_seconds() > 0.0
                          && hibf.layout_dp_algorithm_timer.in_seconds() > 0.0
                          && hibf.user_bin_io_timer.in_seconds() > 0.0
-                         && hibf.layout_union_estimation_timer.in_seconds() == 0.0);
+                         && hibf.layout_union_estimation_timer.in_seconds() > 0.0
+                         && hibf.layout_rearrangement_timer.in_seconds() > 0.0);
     return 0;
 }

@@ -6,9 +6,14 @@
 // from the sketches), so user bins and merged-bin paths are compared, not the
 // maximum-bin ids.  Reads "n tmax seed" from argv; prints "ok" or the first
 // difference.
+//
+// "sketched FILE n bits tmax" instead prints the layout text of the host path
+// (union estimates and rearrangement by the reference's sequential merges)
+// for the counts (n uint64) and registers (n x 2^bits bytes) in FILE.
 #include <cstdio>
 #include <cstdlib>
 #include <random>
+#include <string>
 #include <vector>
 
 #include "layout.hpp"
@@ -49,8 +54,38 @@ private:
     std::vector<uint64_t> column_;
 };
 
+int sketched(char const * path, uint64_t n, uint32_t bits, uint64_t tmax)
+{
+    FILE * f = std::fopen(path, "rb");
+    if (!f)
+        return 2;
+    std::vector<uint64_t> counts(n);
+    std::vector<uint8_t> regs(n << bits);
+    bool const ok = std::fread(counts.data(), 8, n, f) == n && std::fread(regs.data(), 1, regs.size(), f) == regs.size();
+    std::fclose(f);
+    if (!ok)
+        return 2;
+    std::vector<seqan::hibf::sketch::hyperloglog> sketches;
+    for (uint64_t u = 0; u < n; ++u)
+        sketches.push_back(seqan::hibf::sketch::hyperloglog::from_registers(static_cast<uint8_t>(bits),
+                                                                            regs.data() + (u << bits)));
+    Config c;
+    c.number_of_user_bins = n;
+    c.tmax = tmax;
+    c.sketch_bits = bits;
+    if (!validate_and_set_defaults(c).empty())
+        return 2;
+    Sketching sk;
+    sk.sketches = &sketches;
+    std::fputs(compute_layout(c, counts, &sk).to_text().c_str(), stdout);
+    return 0;
+}
+
 int main(int argc, char ** argv)
 {
+    if (argc > 5 && std::string(argv[1]) == "sketched")
+        return sketched(argv[2], std::strtoull(argv[3], nullptr, 10), static_cast<uint32_t>(std::atoi(argv[4])),
+                        std::strtoull(argv[5], nullptr, 10));
     uint64_t const n = argc > 1 ? std::strtoull(argv[1], nullptr, 10) : 500;
     uint64_t const tmax = argc > 2 ? std::strtoull(argv[2], nullptr, 10) : 64;
     uint64_t const seed = argc > 3 ? std::strtoull(argv[3], nullptr, 10) : 1;

@@ -367,10 +367,12 @@ def _same_hibf(a, b):
 def test_build_layout_sees_hyperloglog_estimates():
     """The builders hand the layout the user bins' HyperLogLog estimates, as
     the reference's build does (src/hierarchical_interleaved_bloom_filter.cpp:
-    215-220): each build equals the build from the layout of the oracle's
-    estimates (12-bit sketches), for explicit and for synthetic user bins."""
+    215-230): with disable_estimate_union each build equals the build from the
+    layout of the oracle's estimates (12-bit sketches); by default (union
+    estimates and rearrangement) it equals the build from the layout computed
+    with the user bins' sketches, for explicit and for synthetic user bins."""
     from hibf_amd import native
-    from hibf_amd.device import DeviceHibf
+    from hibf_amd.device import DeviceHibf, hll_sketches, hll_sketches_synthetic
     from hibf_amd.synth import synth_values
     _say("layout from HLL estimates")
     rng = np.random.default_rng(17)
@@ -378,13 +380,22 @@ def test_build_layout_sees_hyperloglog_estimates():
     ubs = _random_user_bins(rng, 300, 200, 20000)
     counts = [int(O.hll_estimate(O.hll_sketch(np.unique(u), 12))) for u in ubs]
     assert any(c != np.unique(u).size for c, u in zip(counts, ubs))   # the estimates are not the exact counts
-    text = native.compute_layout(native.make_config(number_of_user_bins=len(ubs), **cfg), counts)
+    sums = dict(cfg, disable_estimate_union=True)
+    text = native.compute_layout(native.make_config(number_of_user_bins=len(ubs), **sums), counts)
+    _same_hibf(DeviceHibf.build(ubs, **sums), DeviceHibf.build(ubs, layout_text=text, **sums))
+    c = native.validate_config(native.make_config(number_of_user_bins=len(ubs), **cfg))
+    text = native.compute_layout_sketched(c, counts, hll_sketches([np.unique(u) for u in ubs]).data_ptr())
+    assert text != native.compute_layout(c, counts)
     _same_hibf(DeviceHibf.build(ubs, **cfg), DeviceHibf.build(ubs, layout_text=text, **cfg))
     sizes = rng.integers(100, 30000, size=500)
     seed = 5
     counts = [int(O.hll_estimate(O.hll_sketch(synth_values(seed, u, np.arange(n, dtype=np.uint64)), 12)))
               for u, n in enumerate(sizes)]
-    text = native.compute_layout(native.make_config(number_of_user_bins=len(sizes), **cfg), counts)
+    text = native.compute_layout(native.make_config(number_of_user_bins=len(sizes), **sums), counts)
+    _same_hibf(DeviceHibf.build_synthetic(sizes, seed, **sums),
+               DeviceHibf.build_synthetic(sizes, seed, layout_text=text, **sums))
+    c = native.validate_config(native.make_config(number_of_user_bins=len(sizes), **cfg))
+    text = native.compute_layout_sketched(c, counts, hll_sketches_synthetic(sizes, seed).data_ptr())
     _same_hibf(DeviceHibf.build_synthetic(sizes, seed, **cfg),
                DeviceHibf.build_synthetic(sizes, seed, layout_text=text, **cfg))
 

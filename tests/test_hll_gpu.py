@@ -80,3 +80,76 @@ def test_hll_bit_width_is_checked():
         hll_sketches([np.arange(10, dtype=np.uint64)], bits=4)
     # 33 bits: refused before any pointer is read
     assert native.lib().hibf_gpu_hll_sketch(None, None, 1, 33, None, None) == native.ERR_INVALID_ARGUMENT
+
+
+@pytest.mark.parametrize("bits,npos", [(5, 9), (7, 65), (10, 130), (12, 257), (12, 2)])
+def test_union_estimates_match_oracle(bits, npos):
+    """All union estimates of the layout DP (every column j and k < j of a
+    shuffled order, empty and repeated sketches included) equal the oracle's
+    sequential merge_and_estimate, bit for bit after truncation; the first
+    column entries are the prefix unions."""
+    from hibf_amd.device import hll_sketches, hll_union_estimates, to_numpy
+    _say(f"union estimates bits={bits} npos={npos}")
+    rng = np.random.default_rng(bits * 1000 + npos)
+    n = npos + 3
+    sizes = rng.integers(0, 4000, size=n)
+    sizes[1] = 0
+    sets = _sets(rng, sizes)
+    sets[2] = sets[4]
+    regs = hll_sketches(sets, bits=bits)
+    pos = rng.permutation(n)[:npos]
+    got = to_numpy(hll_union_estimates(regs, pos))
+    want = O.hll_union_estimates(to_numpy(regs), pos)
+    assert got.shape == want.shape
+    assert np.array_equal(got, want)
+
+
+def test_union_estimates_refuse_wide_sketches():
+    from hibf_amd.device import hll_sketches, hll_union_estimates
+    from hibf_amd.native import HibfGpuError
+    regs = hll_sketches([np.arange(10, dtype=np.uint64)] * 3, bits=13)
+    with pytest.raises(HibfGpuError):
+        hll_union_estimates(regs, [0, 1, 2])
+
+
+def _host_sketched_layout(tmp_path, counts, regs, bits, tmax):
+    import os
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = tmp_path / "layout_union"
+    r = subprocess.run(["g++", "-std=c++20", "-O2", f"-I{root}/include", f"-I{root}/hibf_amd/csrc",
+                        os.path.join(root, "tests", "cpp", "layout_union.cpp"),
+                        os.path.join(root, "hibf_amd", "csrc", "layout.cpp"), "-o", str(exe)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    f = tmp_path / "sketches.bin"
+    f.write_bytes(np.asarray(counts, dtype=np.uint64).tobytes() + regs.tobytes())
+    r = subprocess.run([str(exe), "sketched", str(f), str(len(counts)), str(bits), str(tmax)], capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    return r.stdout
+
+
+@pytest.mark.parametrize("n,bits,tmax", [(200, 12, 64), (700, 10, 64), (1500, 12, 128)])
+def test_sketched_layout_on_device_equals_host(tmp_path, n, bits, tmax):
+    """compute_layout with sketches (union estimates + similarity
+    rearrangement): the device path (triangle of union estimates, pair
+    estimates of the clustering) gives the layout text of the host path, which
+    merges sketches one by one as the reference (its toolbox pinned by the
+    reference's known answers)."""
+    from hibf_amd import native
+    from hibf_amd.device import hll_estimates, hll_sketches, to_numpy
+    _say(f"sketched layout n={n} bits={bits} tmax={tmax}")
+    rng = np.random.default_rng(n)
+    sizes = np.clip(rng.lognormal(np.log(2000), 1.2, size=n), 50, 40000).astype(np.int64)
+    sets = _sets(rng, sizes)
+    for i in range(0, n - 3, 7):  # overlapping user bins give the clustering real choices
+        sets[i + 1] = np.concatenate([sets[i + 1], sets[i][: len(sets[i]) // 2]])
+    regs = hll_sketches(sets, bits=bits)
+    counts = hll_estimates(regs).astype(np.uint64)
+    c = native.validate_config(native.make_config(number_of_user_bins=n, tmax=tmax, sketch_bits=bits))
+    got = native.compute_layout_sketched(c, counts, regs.data_ptr())
+    want = _host_sketched_layout(tmp_path, counts, to_numpy(regs), bits, tmax)
+    assert got == want
+    plain = native.compute_layout(c, counts)
+    assert plain != got  # union estimates and rearrangement changed the layout

@@ -458,3 +458,26 @@ def test_hll_estimates_are_unbiased():
     # The small-value branch: far fewer values than registers is linear counting, close to exact.
     vals = rng.integers(0, 2**63, size=100, dtype=np.int64).astype(np.uint64)
     assert abs(O.hll_estimate(O.hll_sketch(vals, 12)) - 100) < 3
+
+
+def test_hll_union_estimates_pins():
+    """Union estimates of the layout DP: the toolbox_test.cpp:52-83 known
+    answers (four copies of the 5-bit sketch of 1,500 mt19937_64(0) values:
+    every union estimates 1635), every (k, j) equal to the estimate of the
+    register-wise max of sketches k .. j taken with numpy, and the union of a
+    set with itself equal to the set's own estimate."""
+    s = O.hll_sketch(O.mt19937_64(0, 1500), 5)
+    assert O.hll_union_estimates(np.stack([s] * 4), [0, 1, 2, 3]).tolist() == [1635] * 6
+    rng = np.random.default_rng(12)
+    for bits in (5, 7, 10):
+        sets = [rng.integers(0, 2**63, size=int(n), dtype=np.int64).astype(np.uint64)
+                for n in rng.integers(0, 3000, size=9)]
+        regs = np.stack([O.hll_sketch(v, bits) for v in sets])
+        pos = rng.permutation(9)
+        got = O.hll_union_estimates(regs, pos)
+        for j in range(1, 9):
+            for k in range(j):
+                want = int(O.hll_estimate(np.maximum.reduce(regs[pos[k:j + 1]])))
+                assert int(got[j * (j - 1) // 2 + k]) == want, (bits, k, j)
+        twice = O.hll_union_estimates(np.stack([regs[3], regs[3]]), [0, 1])
+        assert int(twice[0]) == int(O.hll_estimate(regs[3]))
