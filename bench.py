@@ -50,6 +50,11 @@ def parse():
                          "0 skips it)")
     ap.add_argument("--graph-batch", type=int, default=65536)
     ap.add_argument("--hibf-user-bins", type=int, default=32768)
+    ap.add_argument("--recompute-layout", action="store_true",
+                    help="hibf: compute the layout (union estimates + rearrangement on the device) instead of "
+                         "reading the stored one, and report its times")
+    ap.add_argument("--layout-only", action="store_true",
+                    help="hibf: stop after the sketches and the layout")
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--queries", type=int, default=1_000_000)
@@ -445,11 +450,17 @@ def run_hibf(args):
     sketch_leg, counts, regs = hibf_sketch_leg(sizes, spec, args)
     t0 = time.perf_counter()
     layout_times = {}
-    layout = W.cached_layout(counts, spec["config"], registers=regs, times=layout_times)
+    layout = W.cached_layout(counts, spec["config"], registers=regs, times=layout_times,
+                             reuse=not args.recompute_layout, write=not args.recompute_layout)
     layout_s = time.perf_counter() - t0
     del regs
     log(f"layout ({n_ub} user bins, {sizes.sum():.3e} values, HyperLogLog estimates, union estimates and "
         f"rearrangement): {layout_s:.1f} s" + (f" {layout_times}" if layout_times else " (stored)"))
+    if args.layout_only:
+        print(json.dumps({"metric": "HIBF layout (union estimates + rearrangement)", "value": layout_s, "unit": "s",
+                          "n_gpus": 1, "higher_is_better": False, "config": {"workload": f"layout of {n_ub} user bins"},
+                          "layout": layout_times or None, "sketches": sketch_leg}), flush=True)
+        return
     t0 = time.perf_counter()
     hibf = DeviceHibf.build_synthetic(sizes, spec["build_seed"], layout_text=layout, **spec["config"])
     build_s = time.perf_counter() - t0

@@ -67,18 +67,20 @@ def layout_key(sizes: np.ndarray, config: dict) -> str:
     return h.hexdigest()[:16]
 
 
-def cached_layout(sizes: np.ndarray, config: dict, write: bool = True, registers=None, times: dict | None = None) -> str:
+def cached_layout(sizes: np.ndarray, config: dict, write: bool = True, registers=None, times: dict | None = None,
+                  reuse: bool = True) -> str:
     """Layout text for these user-bin sizes and config.  The host DP is
     quadratic in the number of user bins (about a minute at 32,768), so a
     computed layout is stored as layouts/<key>.layout.gz and reused.  With
     `registers` (the user bins' sketches on the device) and union estimates
     enabled, the layout uses union estimates and rearrangement, as the
-    reference's default; `times` then receives the computation's seconds."""
+    reference's default; `times` then receives the computation's seconds.
+    reuse=False computes the layout even when it is stored."""
     from . import native
     sketched = registers is not None and not config.get("disable_estimate_union", False)
     key = layout_key(sizes, dict(config, layout="sketched") if sketched else config)
     path = os.path.join(_LAYOUT_DIR, key + ".layout.gz")
-    if os.path.exists(path):
+    if reuse and os.path.exists(path):
         with gzip.open(path, "rt") as f:
             return f.read()
     c = native.validate_config(native.make_config(number_of_user_bins=int(sizes.size), **config))
