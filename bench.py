@@ -32,6 +32,8 @@ def parse():
     ap.add_argument("--serve-rates", default="50000,100000,200000,500000",
                     help="Poisson arrival rates (queries/s) of the serving workload")
     ap.add_argument("--serve-seconds", type=float, default=1.0)
+    ap.add_argument("--serve-host-gb", type=float, default=160.0,
+                    help="serve: skip offered rates whose generated queries would need more host memory than this")
     ap.add_argument("--serve-windows", default="200,1000,2000", help="batching windows (us)")
     ap.add_argument("--serve-pageable", action="store_true",
                     help="hand the serving loop pageable query values (staged) instead of a pinned receive buffer")
@@ -775,6 +777,12 @@ def run_serve(args):
     for rate in rates:
         nq = int(rate * args.serve_seconds)
         lengths = W.query_lengths(args.seed + 11, nq)
+        # Generating the values holds about six 8-byte arrays per value at its peak, plus the pinned copy.
+        need = int(lengths.sum()) * 8 * 7
+        if need > args.serve_host_gb * 1e9:
+            log(f"serve rate {rate:.0f} q/s skipped: its {int(lengths.sum()):.3e} query values would need about "
+                f"{need / 1e9:.0f} GB of host memory (--serve-host-gb {args.serve_host_gb:.0f}; lower --serve-seconds)")
+            continue
         v_h, o_h, _ = W.hibf_queries(args.seed + 12, sizes, spec["build_seed"], lengths, 0.6)
         t_h = np.minimum((0.7 * np.diff(o_h.astype(np.int64))).astype(np.int64), 65535).astype(np.uint16)
         rng = np.random.default_rng(args.seed + 13)
@@ -785,6 +793,7 @@ def run_serve(args):
             # Requests land in a pinned receive buffer, as a server's network path would place them.
             v_in = torch.empty(v_h.size, dtype=torch.int64, pin_memory=True)
             v_in.numpy()[:] = v_h.view(np.int64)
+        lat = ids = cnt = None
         for win in windows:
             lat, ids, cnt, st = hibf.serve(v_in, o_h, t_h, arrival, window_us=win, streams=4,
                                            budget_bytes=4 << 30, results_per_query=16)
@@ -812,6 +821,7 @@ def run_serve(args):
                 got = ids[q, :min(int(cnt[q]), 16)].tolist()
                 assert int(cnt[q]) == len(want) and got == want[:16], f"serving result differs at query {q}"
             log(f"serving parity vs oracle on {min(args.check_queries, nq)} sampled queries: True")
+        del v_h, v_in, o_h, t_h, arrival, lat, ids, cnt
     best = max(rows, key=lambda r: r["sustained_lookups_per_s"])
     out = {"metric": "p99 query latency (HIBF serving loop)", "value": best["p99_ms"], "unit": "ms",
            "n_gpus": 1, "higher_is_better": False, "data": "synthetic", "dtype": "u64 hash / u16 counters",
