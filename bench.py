@@ -32,7 +32,7 @@ def parse():
     ap.add_argument("--serve-rates", default="50000,100000,200000,500000",
                     help="Poisson arrival rates (queries/s) of the serving workload")
     ap.add_argument("--serve-seconds", type=float, default=1.0)
-    ap.add_argument("--serve-host-gb", type=float, default=160.0,
+    ap.add_argument("--serve-host-gb", type=float, default=200.0,
                     help="serve: skip offered rates whose generated queries would need more host memory than this")
     ap.add_argument("--serve-windows", default="200,1000,2000", help="batching windows (us)")
     ap.add_argument("--serve-pageable", action="store_true",
@@ -777,8 +777,8 @@ def run_serve(args):
     for rate in rates:
         nq = int(rate * args.serve_seconds)
         lengths = W.query_lengths(args.seed + 11, nq)
-        # Generating the values holds about six 8-byte arrays per value at its peak, plus the pinned copy.
-        need = int(lengths.sum()) * 8 * 7
+        # The values, their pinned copy and the generator's temporaries: about 40 bytes per value at the peak.
+        need = int(lengths.sum()) * 40
         if need > args.serve_host_gb * 1e9:
             log(f"serve rate {rate:.0f} q/s skipped: its {int(lengths.sum()):.3e} query values would need about "
                 f"{need / 1e9:.0f} GB of host memory (--serve-host-gb {args.serve_host_gb:.0f}; lower --serve-seconds)")
