@@ -16,6 +16,7 @@ from __future__ import annotations
 
 import argparse
 import ctypes
+import hashlib
 import json
 import os
 import sys
@@ -87,9 +88,10 @@ def parse():
     return ap.parse_args()
 
 
-def recorded_traffic(key: str, args) -> float | None:
+def recorded_traffic(key: str, args, layout_text: str | None = None) -> float | None:
     """HBM bytes per launch measured with rocprofv3 counters on this exact
-    workload (profiles/traffic.json), or None when the workload differs."""
+    workload (profiles/traffic.json), or None when the workload (or, for an
+    HIBF, the layout the record was measured on) differs."""
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "traffic.json")
     try:
         with open(path) as f:
@@ -97,6 +99,9 @@ def recorded_traffic(key: str, args) -> float | None:
     except (OSError, KeyError, ValueError):
         return None
     if any(getattr(args, k) != v for k, v in rec["workload"].items()):
+        return None
+    if "layout_sha1" in rec and (layout_text is None or
+                                 hashlib.sha1(layout_text.encode()).hexdigest()[:16] != rec["layout_sha1"]):
         return None
     return (rec["fetch_kib"] * rec["fetch_factor"] + rec["write_kib"] * rec["write_factor"]) * 1024.0
 
@@ -735,7 +740,7 @@ def run_hibf(args):
                      "frac": bytes_b / wall_b / 1e9 / HBM_PEAK_GBS, "traffic": None},
         "fixed_batch": {"queries": nq, "length": 250, "ms": wall_a * 1e3, "lookups_per_s": lookups_a / wall_a,
                         "row_gbs": bytes_a / kern_a / 1e9, "kernel_ms": kern_a * 1e3,
-                        "traffic": recorded_traffic("hibf_fixed_batch", args)},
+                        "traffic": recorded_traffic("hibf_fixed_batch", args, layout)},
         "bulk_count": count_leg,
         "kmer_hash": kmer_leg,
         "plain_per_query": {"value": plain, "unit": "lookups/s", "queries": npl,
