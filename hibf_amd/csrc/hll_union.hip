@@ -133,7 +133,6 @@ __global__ void __launch_bounds__(256) union_triangle_kernel(uint8_t const * __r
         bool const active = k < j;
         uint32_t const * src = reinterpret_cast<uint32_t const *>(lanes + (pos[j] << bits) + L * lane_bytes);
         float acc = 0.0f;
-        uint32_t zeros = 0;
         if (active)
         {
             uint32_t S[Q];
@@ -149,24 +148,31 @@ __global__ void __launch_bounds__(256) union_triangle_kernel(uint8_t const * __r
                 acc = acc + inverse_power((x >> 8) & 0xFFu);
                 acc = acc + inverse_power((x >> 16) & 0xFFu);
                 acc = acc + inverse_power(x >> 24);
-                zeros += zero_registers(x);
             }
         }
         float sum = 0.0f;
-        uint32_t zs = 0;
 #pragma unroll
         for (int i = 0; i < 8; ++i)
-        {
             sum = sum + __shfl(acc, first + i, 64);
-            zs += static_cast<uint32_t>(__shfl(static_cast<int>(zeros), first + i, 64));
+        double e = norm / static_cast<double>(sum);
+        // The zero registers matter only below 2.5 m (small unions): counted then.
+        bool const small = active && e <= 2.5 * static_cast<double>(m);
+        if (__any(small))
+        {
+            uint32_t zeros = 0;
+            if (active)
+#pragma unroll
+                for (int w = 0; w < Q; ++w)
+                    zeros += zero_registers(M[w]);
+            uint32_t zs = 0;
+#pragma unroll
+            for (int i = 0; i < 8; ++i)
+                zs += static_cast<uint32_t>(__shfl(static_cast<int>(zeros), first + i, 64));
+            if (small && zs != 0u)
+                e = small_estimate[zs];
         }
         if (active && L == 0u)
-        {
-            double e = norm / static_cast<double>(sum);
-            if (e <= 2.5 * static_cast<double>(m) && zs != 0u)
-                e = small_estimate[zs];
             out[j * (j - 1u) / 2u + k] = static_cast<uint64_t>(e);
-        }
     }
 }
 
@@ -189,13 +195,15 @@ __global__ void __launch_bounds__(256) pair_union_kernel(uint8_t const * __restr
     uint64_t const m = 1ULL << bits;
     uint64_t const lane_bytes = m / 8u;
     float acc = 0.0f;
-    uint32_t zeros = 0;
+    uint32_t const * const sa =
+        reinterpret_cast<uint32_t const *>(slots + (static_cast<uint64_t>(a) << bits) + L * lane_bytes);
+    uint32_t const * const sb =
+        reinterpret_cast<uint32_t const *>(slots + (static_cast<uint64_t>(k < n ? idx[k] : a) << bits) + L * lane_bytes);
     if (k < n)
     {
         uint32_t A[Q], B[Q];
-        load_slice<Q>(reinterpret_cast<uint32_t const *>(slots + (static_cast<uint64_t>(a) << bits) + L * lane_bytes), A);
-        load_slice<Q>(reinterpret_cast<uint32_t const *>(slots + (static_cast<uint64_t>(idx[k]) << bits) + L * lane_bytes),
-                      B);
+        load_slice<Q>(sa, A);
+        load_slice<Q>(sb, B);
 #pragma unroll
         for (int w = 0; w < Q; ++w)
         {
@@ -204,23 +212,108 @@ __global__ void __launch_bounds__(256) pair_union_kernel(uint8_t const * __restr
             acc = acc + inverse_power((x >> 8) & 0xFFu);
             acc = acc + inverse_power((x >> 16) & 0xFFu);
             acc = acc + inverse_power(x >> 24);
-            zeros += zero_registers(x);
         }
     }
     float sum = 0.0f;
-    uint32_t zs = 0;
 #pragma unroll
     for (int i = 0; i < 8; ++i)
-    {
         sum = sum + __shfl(acc, first + i, 64);
-        zs += static_cast<uint32_t>(__shfl(static_cast<int>(zeros), first + i, 64));
+    double e = norm / static_cast<double>(sum);
+    bool const small = k < n && e <= 2.5 * static_cast<double>(m);
+    if (__any(small))
+    {
+        uint32_t zeros = 0;
+        if (k < n)
+        {
+            uint32_t A[Q], B[Q];
+            load_slice<Q>(sa, A);
+            load_slice<Q>(sb, B);
+#pragma unroll
+            for (int w = 0; w < Q; ++w)
+                zeros += zero_registers(register_max(A[w], B[w]));
+        }
+        uint32_t zs = 0;
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+            zs += static_cast<uint32_t>(__shfl(static_cast<int>(zeros), first + i, 64));
+        if (small && zs != 0u)
+            e = small_estimate[zs];
     }
     if (k < n && L == 0u)
-    {
-        double e = norm / static_cast<double>(sum);
-        if (e <= 2.5 * static_cast<double>(m) && zs != 0u)
-            e = small_estimate[zs];
         out[k] = e;
+}
+
+// Every pair i < j of the slots base .. base + L - 1: out[i L - i (i + 1) / 2 +
+// j - i - 1] = estimate(merge(i, j)).  A block per row i (grid-stride), 32
+// pairs per block pass, 8 threads per pair.
+template <int Q>
+__global__ void __launch_bounds__(256) all_pairs_kernel(uint8_t const * __restrict__ slots,
+                                                        uint32_t bits,
+                                                        uint64_t base,
+                                                        uint64_t L,
+                                                        double norm,
+                                                        double const * __restrict__ small_estimate,
+                                                        double * __restrict__ out)
+{
+    uint32_t const lane_id = static_cast<uint32_t>(threadIdx.x & 7u);
+    int const first = static_cast<int>(threadIdx.x & 63u) & ~7;
+    uint64_t const m = 1ULL << bits;
+    uint64_t const lane_bytes = m / 8u;
+    for (uint64_t i = blockIdx.x; i + 1u < L; i += gridDim.x)
+    {
+        uint32_t const * const sa = reinterpret_cast<uint32_t const *>(slots + ((base + i) << bits) + lane_id * lane_bytes);
+        double * const row = out + (i * L - i * (i + 1u) / 2u);
+        // Every thread of the block runs the same number of passes (the shuffles need the whole wave).
+        for (uint64_t j0 = i + 1u; j0 < L; j0 += blockDim.x / 8u)
+        {
+            uint64_t const j = j0 + threadIdx.x / 8u;
+            bool const valid = j < L;
+            uint32_t const * const sb =
+                reinterpret_cast<uint32_t const *>(slots + ((base + (valid ? j : i)) << bits) + lane_id * lane_bytes);
+            float acc = 0.0f;
+            if (valid)
+            {
+                uint32_t A[Q], B[Q];
+                load_slice<Q>(sa, A);
+                load_slice<Q>(sb, B);
+#pragma unroll
+                for (int w = 0; w < Q; ++w)
+                {
+                    uint32_t const x = register_max(A[w], B[w]);
+                    acc = acc + inverse_power(x & 0xFFu);
+                    acc = acc + inverse_power((x >> 8) & 0xFFu);
+                    acc = acc + inverse_power((x >> 16) & 0xFFu);
+                    acc = acc + inverse_power(x >> 24);
+                }
+            }
+            float sum = 0.0f;
+#pragma unroll
+            for (int t = 0; t < 8; ++t)
+                sum = sum + __shfl(acc, first + t, 64);
+            double e = norm / static_cast<double>(sum);
+            bool const small = valid && e <= 2.5 * static_cast<double>(m);
+            if (__any(small))
+            {
+                uint32_t zeros = 0;
+                if (valid)
+                {
+                    uint32_t A[Q], B[Q];
+                    load_slice<Q>(sa, A);
+                    load_slice<Q>(sb, B);
+#pragma unroll
+                    for (int w = 0; w < Q; ++w)
+                        zeros += zero_registers(register_max(A[w], B[w]));
+                }
+                uint32_t zs = 0;
+#pragma unroll
+                for (int t = 0; t < 8; ++t)
+                    zs += static_cast<uint32_t>(__shfl(static_cast<int>(zeros), first + t, 64));
+                if (small && zs != 0u)
+                    e = small_estimate[zs];
+            }
+            if (valid && lane_id == 0u)
+                row[j - i - 1u] = e;
+        }
     }
 }
 
@@ -266,6 +359,31 @@ hipError_t pair_unions(uint8_t const * slots, uint32_t bits, uint32_t a, uint32_
     case 10: return launch_pairs<32>(slots, bits, a, idx, n, norm, table, out, s);
     case 11: return launch_pairs<64>(slots, bits, a, idx, n, norm, table, out, s);
     default: return launch_pairs<128>(slots, bits, a, idx, n, norm, table, out, s);
+    }
+}
+
+template <int Q>
+hipError_t launch_all_pairs(uint8_t const * slots, uint32_t bits, uint64_t base, uint64_t L, double norm,
+                            double const * table, double * out, hipStream_t s)
+{
+    unsigned const blocks = static_cast<unsigned>(std::min<uint64_t>(L, 1u << 16));
+    hipLaunchKernelGGL(all_pairs_kernel<Q>, dim3(blocks), dim3(256), 0, s, slots, bits, base, L, norm, table, out);
+    return hipGetLastError();
+}
+
+hipError_t all_pairs_launch(uint8_t const * slots, uint32_t bits, uint64_t base, uint64_t L, double norm,
+                            double const * table, double * out, hipStream_t s)
+{
+    switch (bits)
+    {
+    case 5: return launch_all_pairs<1>(slots, bits, base, L, norm, table, out, s);
+    case 6: return launch_all_pairs<2>(slots, bits, base, L, norm, table, out, s);
+    case 7: return launch_all_pairs<4>(slots, bits, base, L, norm, table, out, s);
+    case 8: return launch_all_pairs<8>(slots, bits, base, L, norm, table, out, s);
+    case 9: return launch_all_pairs<16>(slots, bits, base, L, norm, table, out, s);
+    case 10: return launch_all_pairs<32>(slots, bits, base, L, norm, table, out, s);
+    case 11: return launch_all_pairs<64>(slots, bits, base, L, norm, table, out, s);
+    default: return launch_all_pairs<128>(slots, bits, base, L, norm, table, out, s);
     }
 }
 
@@ -608,21 +726,11 @@ uint32_t DevicePairUnions::slot_of(seqan::hibf::sketch::hyperloglog const * h)
     return it->second;
 }
 
-void DevicePairUnions::unions(seqan::hibf::sketch::hyperloglog const & a,
-                              std::vector<seqan::hibf::sketch::hyperloglog const *> const & others,
-                              double * out)
+// Lane-major copies of the sketches first seen in this call (lane L's
+// registers 8 q + 7 - L), queued to their slots.
+void DevicePairUnions::upload_pending()
 {
-    uint64_t const n = others.size();
-    if (n == 0)
-        return;
     uint64_t const m = 1ULL << bits_;
-    pending_.clear();
-    uint32_t const a_slot = slot_of(&a);
-    std::vector<uint32_t> idx(n);
-    for (uint64_t k = 0; k < n; ++k)
-        idx[k] = slot_of(others[k]);
-    ensure(n, used_);
-    // Lane-major copies of the new sketches: lane L's registers 8 q + 7 - L.
     for (uint64_t p = 0; p < pending_.size(); ++p)
     {
         std::vector<uint8_t> const & r = pending_[p]->registers();
@@ -639,6 +747,53 @@ void DevicePairUnions::unions(seqan::hibf::sketch::hyperloglog const & a,
     if (!pending_.empty())
         check(hipMemcpyAsync(d_slots_ + (uploaded_ << bits_), h_stage_, pending_.size() * m, hipMemcpyHostToDevice, s_),
               "pair unions upload");
+}
+
+void DevicePairUnions::all_pairs(std::vector<seqan::hibf::sketch::hyperloglog const *> const & leaves, double * out)
+{
+    uint64_t const L = leaves.size();
+    if (L < 2u)
+        return;
+    // The leaves take consecutive slots (a clustering starts with them).
+    pending_.clear();
+    uint64_t const base = used_;
+    bool consecutive = true;
+    for (uint64_t i = 0; i < L && consecutive; ++i)
+        consecutive = slot_of(leaves[i]) == base + i;
+    ensure(0, used_);
+    upload_pending();
+    uploaded_ = used_;
+    if (!consecutive) // a leaf seen before in this clustering: row by row
+        return union_source::all_pairs(leaves, out);
+    uint64_t const pairs_n = L * (L - 1u) / 2u;
+    double * d_tri = nullptr;
+    check(hipMalloc(reinterpret_cast<void **>(&d_tri), pairs_n * sizeof(double)), "pair unions triangle");
+    hipError_t e = all_pairs_launch(d_slots_, bits_, base, L, norm_, d_table_, d_tri, s_);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(out, d_tri, pairs_n * sizeof(double), hipMemcpyDeviceToHost, s_);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(s_);
+    (void)hipFree(d_tri);
+    check(e, "pair unions triangle");
+    ++calls;
+    pairs += pairs_n;
+}
+
+void DevicePairUnions::unions(seqan::hibf::sketch::hyperloglog const & a,
+                              std::vector<seqan::hibf::sketch::hyperloglog const *> const & others,
+                              double * out)
+{
+    uint64_t const n = others.size();
+    if (n == 0)
+        return;
+    uint64_t const m = 1ULL << bits_;
+    pending_.clear();
+    uint32_t const a_slot = slot_of(&a);
+    std::vector<uint32_t> idx(n);
+    for (uint64_t k = 0; k < n; ++k)
+        idx[k] = slot_of(others[k]);
+    ensure(n, used_);
+    upload_pending();
     std::copy(idx.begin(), idx.end(), h_idx_);
     check(hipMemcpyAsync(d_idx_, h_idx_, n * sizeof(uint32_t), hipMemcpyHostToDevice, s_), "pair unions upload");
     check(pair_unions(d_slots_, bits_, a_slot, d_idx_, n, norm_, d_table_, d_out_, s_), "pair unions");

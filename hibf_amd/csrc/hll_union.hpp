@@ -88,6 +88,8 @@ public:
     void unions(seqan::hibf::sketch::hyperloglog const & a,
                 std::vector<seqan::hibf::sketch::hyperloglog const *> const & others,
                 double * out) override;
+    // One launch for every pair of the leaves (the clustering's first distances).
+    void all_pairs(std::vector<seqan::hibf::sketch::hyperloglog const *> const & leaves, double * out) override;
 
     uint64_t calls{};
     uint64_t pairs{};
@@ -95,6 +97,7 @@ public:
 private:
     DevicePairUnions() = default;
     uint32_t slot_of(seqan::hibf::sketch::hyperloglog const * h);
+    void upload_pending();
     void ensure(uint64_t pairs_needed, uint64_t slots_needed);
     void check(hipError_t e, char const * what);
 

@@ -203,6 +203,17 @@ struct union_source
     virtual ~union_source() = default;
     virtual void reset() {}
     virtual void unions(hyperloglog const & a, std::vector<hyperloglog const *> const & others, double * out) = 0;
+    // Every pair i < j of `leaves`: out[i L - i (i + 1) / 2 + (j - i - 1)] = estimate(merge(i, j)), L leaves.
+    virtual void all_pairs(std::vector<hyperloglog const *> const & leaves, double * out)
+    {
+        size_t const n = leaves.size();
+        std::vector<hyperloglog const *> others;
+        for (size_t i = 0; i + 1 < n; ++i)
+        {
+            others.assign(leaves.begin() + i + 1, leaves.end());
+            unions(*leaves[i], others, out + (i * n - i * (i + 1) / 2));
+        }
+    }
 };
 
 struct host_union_source final : union_source
@@ -272,18 +283,17 @@ inline void cluster_bins(std::vector<hyperloglog> const & sketches,
     size_t min_id = none;
     double min_dist = std::numeric_limits<double>::max();
     {
-        std::vector<hyperloglog const *> others;
-        std::vector<double> merged;
+        std::vector<hyperloglog const *> leaf_sketches(leaves);
+        for (size_t i = 0; i < leaves; ++i)
+            leaf_sketches[i] = &clustering[i].hll;
+        std::vector<double> merged(leaves * (leaves - 1) / 2);
+        if (leaves > 1)
+            unions.all_pairs(leaf_sketches, merged.data());
         for (size_t i = 0; i < leaves; ++i)
         {
-            others.clear();
+            double const * const row = merged.data() + (i * leaves - i * (i + 1) / 2);
             for (size_t j = i + 1; j < leaves; ++j)
-                others.push_back(&clustering[j].hll);
-            merged.resize(others.size());
-            if (!others.empty())
-                unions.unions(clustering[i].hll, others, merged.data());
-            for (size_t j = i + 1; j < leaves; ++j)
-                dist[i].pq.push({j + first, 2 - (estimates[i] + estimates[j]) / merged[j - i - 1]});
+                dist[i].pq.push({j + first, 2 - (estimates[i] + estimates[j]) / row[j - i - 1]});
             if (dist[i].pq.empty())
                 continue;
             if (dist[i].pq.top().dist < min_dist)
