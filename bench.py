@@ -867,13 +867,19 @@ def run_serve_reads(args):
     ref = None
     for rate in rates:
         nq = int(rate * args.serve_seconds)
-        g = rng.integers(0, n_g, size=nq)
-        start = rng.integers(0, g_len - read_len, size=nq)
-        idx = (g * g_len + start)[:, None] + np.arange(read_len)[None, :]
-        bases = genome_chars[idx]
-        err = rng.random(bases.shape) < 0.01
-        bases[err] = acgt[rng.integers(0, 4, size=int(err.sum()))]
-        data = np.ascontiguousarray(bases).reshape(-1)
+        # Reads are generated a million at a time: the gather indices of all of
+        # them at once would take 2 KB of host memory per read.
+        data = np.empty(nq * read_len, dtype=np.uint8)
+        for c0 in range(0, nq, 1 << 20):
+            c1 = min(nq, c0 + (1 << 20))
+            g = rng.integers(0, n_g, size=c1 - c0)
+            start = rng.integers(0, g_len - read_len, size=c1 - c0)
+            idx = (g * g_len + start)[:, None] + np.arange(read_len)[None, :]
+            bases = genome_chars[idx]
+            err = rng.random(bases.shape) < 0.01
+            bases[err] = acgt[rng.integers(0, 4, size=int(err.sum()))]
+            data[c0 * read_len:c1 * read_len] = bases.reshape(-1)
+            del g, start, idx, bases, err
         r_off = np.arange(nq + 1, dtype=np.uint64) * np.uint64(read_len)
         arrival = np.cumsum(rng.exponential(1.0 / rate, size=nq))
         arrival -= arrival[0]

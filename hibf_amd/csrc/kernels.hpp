@@ -274,17 +274,12 @@ int hll_sketch_synthetic(uint64_t seed, uint64_t const * h_sizes, uint64_t n, ui
 int hll_estimates(uint8_t const * d_registers, uint64_t n, uint32_t bits, double * h_estimates, hipStream_t s);
 
 // k-mer hashes of reads on the device without host waits (kmer.hip).
-size_t kmer_hash_temp_bytes(uint64_t max_seqs);
-hipError_t kmer_hash_async(uint8_t const * d_seq,
-                           uint64_t const * d_seq_offsets,
-                           uint64_t n,
-                           uint32_t k,
-                           uint64_t * d_hash_offsets,
-                           uint64_t * d_hashes,
-                           void * temp,
-                           size_t temp_bytes,
-                           hipStream_t s);
-hipError_t kmer_thresholds_async(uint64_t const * d_hash_offsets, uint64_t n, double fraction, uint16_t * d_thresholds,
-                                 hipStream_t s);
+hipError_t kmer_hash_given_offsets_async(uint8_t const * d_seq,
+                                         uint64_t const * d_seq_offsets,
+                                         uint64_t n,
+                                         uint32_t k,
+                                         uint64_t const * d_hash_offsets,
+                                         uint64_t * d_hashes,
+                                         hipStream_t s);
 
 } // namespace hibf_amd

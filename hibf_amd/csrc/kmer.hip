@@ -52,25 +52,59 @@ __global__ void zero_first_kernel(uint64_t * p)
         p[0] = 0u;
 }
 
-// One block per read (grid-stride); thread t computes the windows t, t + 128, ...
-__global__ void __launch_bounds__(128) kmer_hash_kernel(uint8_t const * __restrict__ seq,
-                                                        uint64_t const * __restrict__ seq_offsets,
-                                                        uint64_t n,
-                                                        uint32_t k,
-                                                        uint64_t const * __restrict__ hash_offsets,
-                                                        uint64_t * __restrict__ hashes)
+// One block of 256 threads per read (grid-stride).  The read is taken in
+// segments of up to 8,192 bases that overlap by k - 1: the block stages a
+// segment's bytes in LDS (coalesced), packs them into 2-bit ranks, 32 bases per
+// 64-bit word with the first base most significant, and every window's hash
+// is then a funnel shift of two neighbouring words: bits 2 p .. 2 p + 2 k of the
+// packed stream.
+constexpr uint32_t kHashThreads = 256;
+constexpr uint32_t kSegment = 32u * kHashThreads;
+
+__global__ void __launch_bounds__(kHashThreads) kmer_hash_kernel(uint8_t const * __restrict__ seq,
+                                                                 uint64_t const * __restrict__ seq_offsets,
+                                                                 uint64_t n,
+                                                                 uint32_t k,
+                                                                 uint64_t const * __restrict__ hash_offsets,
+                                                                 uint64_t * __restrict__ hashes)
 {
+    __shared__ uint8_t raw[kSegment];
+    __shared__ uint64_t words[kHashThreads + 1];
+    uint32_t const tid = threadIdx.x;
     for (uint64_t i = blockIdx.x; i < n; i += gridDim.x)
     {
         uint8_t const * const s = seq + seq_offsets[i];
+        uint64_t const len = seq_offsets[i + 1] - seq_offsets[i];
         uint64_t const first = hash_offsets[i];
         uint64_t const count = hash_offsets[i + 1] - first;
-        for (uint64_t p = threadIdx.x; p < count; p += blockDim.x)
+        for (uint64_t b0 = 0; b0 < count; b0 += kSegment - (k - 1u))
         {
-            uint64_t h = 0;
-            for (uint32_t j = 0; j < k; ++j)
-                h = (h << 2) | dna4_rank(s[p + j]);
-            hashes[first + p] = h;
+            uint32_t const seg = static_cast<uint32_t>(min<uint64_t>(kSegment, len - b0));
+            uint32_t const nwords = (seg + 31u) / 32u;
+            for (uint32_t x = tid; x < nwords * 32u; x += kHashThreads)
+                raw[x] = x < seg ? s[b0 + x] : static_cast<uint8_t>('A');
+            __syncthreads();
+            if (tid < nwords)
+            {
+                uint64_t w = 0;
+#pragma unroll 8
+                for (uint32_t j = 0; j < 32u; ++j)
+                    w = (w << 2) | dna4_rank(raw[32u * tid + j]);
+                words[tid] = w;
+            }
+            if (tid == 0)
+                words[nwords] = 0;
+            __syncthreads();
+            uint64_t const end = min<uint64_t>(count, b0 + seg - (k - 1u));
+            for (uint64_t p = b0 + tid; p < end; p += kHashThreads)
+            {
+                uint32_t const q = static_cast<uint32_t>(p - b0);
+                uint32_t const off = q & 31u;
+                uint64_t const hi = words[q >> 5];
+                uint64_t const stream = off ? (hi << (2u * off)) | (words[(q >> 5) + 1u] >> (64u - 2u * off)) : hi;
+                hashes[first + p] = stream >> (64u - 2u * k);
+            }
+            __syncthreads();
         }
     }
 }
@@ -107,46 +141,20 @@ int device_error(hipError_t e, char const * what)
 namespace hibf_amd
 {
 
-size_t kmer_hash_temp_bytes(uint64_t max_seqs)
-{
-    size_t bytes = 0;
-    (void)inclusive_sum_u64(nullptr, bytes, nullptr, nullptr, std::max<uint64_t>(max_seqs, 1u), nullptr);
-    return std::max<size_t>(bytes, 1u);
-}
-
-// Queues the hashes of n reads (CSR offsets on the device) without a host wait: offsets by a device
-// scan into d_hash_offsets (n + 1 entries), then the hashes.  d_hashes must hold one hash per base
-// (a read of length L has L - k + 1 <= L k-mers); temp holds kmer_hash_temp_bytes(n) bytes.
-hipError_t kmer_hash_async(uint8_t const * d_seq,
-                           uint64_t const * d_seq_offsets,
-                           uint64_t n,
-                           uint32_t k,
-                           uint64_t * d_hash_offsets,
-                           uint64_t * d_hashes,
-                           void * temp,
-                           size_t temp_bytes,
-                           hipStream_t s)
-{
-    hipLaunchKernelGGL(zero_first_kernel, dim3(1), dim3(64), 0, s, d_hash_offsets);
-    if (n == 0)
-        return hipGetLastError();
-    hipLaunchKernelGGL(kmer_count_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, d_seq_offsets, n, k,
-                       d_hash_offsets + 1);
-    hipError_t e = inclusive_sum_u64(temp, temp_bytes, d_hash_offsets + 1, d_hash_offsets + 1, n, s);
-    if (e != hipSuccess)
-        return e;
-    hipLaunchKernelGGL(kmer_hash_kernel, dim3(blocks_for(n, 1)), dim3(128), 0, s, d_seq, d_seq_offsets, n, k,
-                       d_hash_offsets, d_hashes);
-    return hipGetLastError();
-}
-
-hipError_t kmer_thresholds_async(uint64_t const * d_hash_offsets, uint64_t n, double fraction, uint16_t * d_thresholds,
-                                 hipStream_t s)
+// The hashes alone, with hash offsets the caller computed (n + 1 values,
+// hash_offsets[i + 1] - hash_offsets[i] = max(0, length of read i - k + 1)).
+hipError_t kmer_hash_given_offsets_async(uint8_t const * d_seq,
+                                         uint64_t const * d_seq_offsets,
+                                         uint64_t n,
+                                         uint32_t k,
+                                         uint64_t const * d_hash_offsets,
+                                         uint64_t * d_hashes,
+                                         hipStream_t s)
 {
     if (n == 0)
         return hipSuccess;
-    hipLaunchKernelGGL(kmer_threshold_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, d_hash_offsets, n, fraction,
-                       d_thresholds);
+    hipLaunchKernelGGL(kmer_hash_kernel, dim3(blocks_for(n, 1)), dim3(kHashThreads), 0, s, d_seq, d_seq_offsets, n,
+                       k, d_hash_offsets, d_hashes);
     return hipGetLastError();
 }
 
@@ -195,8 +203,8 @@ int hibf_gpu_kmer_hash(uint8_t const * d_seq,
         return device_error(e, "kmer_hash offsets");
     if (*h_total > hashes_cap)
         return hibf_amd::set_error(HIBF_GPU_ERR_CAPACITY, "hash buffer too small (see *h_total)");
-    hipLaunchKernelGGL(kmer_hash_kernel, dim3(blocks_for(n_seqs, 1)), dim3(128), 0, s, d_seq, d_seq_offsets, n_seqs, k,
-                       d_hash_offsets, d_hashes);
+    hipLaunchKernelGGL(kmer_hash_kernel, dim3(blocks_for(n_seqs, 1)), dim3(kHashThreads), 0, s, d_seq, d_seq_offsets,
+                       n_seqs, k, d_hash_offsets, d_hashes);
     e = hipGetLastError();
     if (e == hipSuccess)
         e = hipStreamSynchronize(s);

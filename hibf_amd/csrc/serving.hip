@@ -34,6 +34,7 @@ struct Slot
     hibf_gpu_agent agent{};
     uint64_t * h_vals{}; // pinned staging
     uint64_t * h_offs{};
+    uint64_t * h_hash_offs{}; // reads in: k-mer offsets of the batch, computed on the host
     uint16_t * h_thr{};
     uint64_t * h_ids{};
     uint64_t * h_hs{};
@@ -44,11 +45,9 @@ struct Slot
     uint64_t * d_ids{};
     uint64_t * d_hs{};
     uint64_t * d_hc{};
-    // reads in: the batch's k-mer hashes, their offsets and the scan's scratch
+    // reads in: the batch's k-mer hashes and their offsets
     uint64_t * d_hashes{};
     uint64_t * d_hash_offs{};
-    void * d_temp{};
-    size_t temp_bytes{};
     uint64_t q0{}, q1{};
     uint64_t bytes{};
     bool busy{false};
@@ -153,12 +152,12 @@ int serve_impl(hibf_gpu_hibf hibf,
                 (void)hipStreamSynchronize(s.stream);
             for (void * d : {static_cast<void *>(s.d_vals), static_cast<void *>(s.d_offs), static_cast<void *>(s.d_thr),
                              static_cast<void *>(s.d_ids), static_cast<void *>(s.d_hs), static_cast<void *>(s.d_hc),
-                             static_cast<void *>(s.d_hashes), static_cast<void *>(s.d_hash_offs), s.d_temp})
+                             static_cast<void *>(s.d_hashes), static_cast<void *>(s.d_hash_offs)})
                 if (d)
                     (void)hipFree(d);
             for (void * hp : {static_cast<void *>(s.h_vals), static_cast<void *>(s.h_offs),
-                              static_cast<void *>(s.h_thr), static_cast<void *>(s.h_ids), static_cast<void *>(s.h_hs),
-                              static_cast<void *>(s.h_hc)})
+                              static_cast<void *>(s.h_hash_offs), static_cast<void *>(s.h_thr),
+                              static_cast<void *>(s.h_ids), static_cast<void *>(s.h_hs), static_cast<void *>(s.h_hc)})
                 if (hp)
                     (void)hipHostFree(hp);
             if (s.done)
@@ -206,9 +205,24 @@ int serve_impl(hibf_gpu_hibf hibf,
         uint64_t const * d_o = s.d_offs;
         if (reads)
         {
-            step(hibf_amd::kmer_hash_async(reinterpret_cast<uint8_t const *>(s.d_vals), s.d_offs, nb, k,
-                                           s.d_hash_offs, s.d_hashes, s.d_temp, s.temp_bytes, s.stream));
-            step(hibf_amd::kmer_thresholds_async(s.d_hash_offs, nb, fraction, s.d_thr, s.stream));
+            // A read of length L has max(0, L - k + 1) k-mers and the threshold
+            // uint16_t(fraction * k-mers) (test/tutorial/search.cpp:42): both from
+            // the lengths the host already has, so the device only hashes.
+            uint64_t total = 0;
+            s.h_hash_offs[0] = 0;
+            for (uint64_t i = 0; i < nb; ++i)
+            {
+                uint64_t const len = s.h_offs[i + 1] - s.h_offs[i];
+                uint64_t const windows = len >= k ? len - k + 1u : 0u;
+                total += windows;
+                s.h_hash_offs[i + 1] = total;
+                double const t = fraction * static_cast<double>(windows);
+                s.h_thr[i] = t >= 65535.0 ? uint16_t{65535} : t <= 0.0 ? uint16_t{0} : static_cast<uint16_t>(t);
+            }
+            step(hipMemcpyAsync(s.d_hash_offs, s.h_hash_offs, (nb + 1) * 8u, hipMemcpyHostToDevice, s.stream));
+            step(hipMemcpyAsync(s.d_thr, s.h_thr, nb * 2u, hipMemcpyHostToDevice, s.stream));
+            step(hibf_amd::kmer_hash_given_offsets_async(reinterpret_cast<uint8_t const *>(s.d_vals), s.d_offs, nb, k,
+                                                         s.d_hash_offs, s.d_hashes, s.stream));
             d_v = s.d_hashes;
             d_o = s.d_hash_offs;
         }
@@ -253,8 +267,7 @@ int serve_impl(hibf_gpu_hibf hibf,
         {
             SERVE_TRY(hipMalloc(reinterpret_cast<void **>(&s.d_hashes), V * 8u));
             SERVE_TRY(hipMalloc(reinterpret_cast<void **>(&s.d_hash_offs), (B + 1) * 8u));
-            s.temp_bytes = hibf_amd::kmer_hash_temp_bytes(B);
-            SERVE_TRY(hipMalloc(&s.d_temp, s.temp_bytes));
+            SERVE_TRY(hipHostMalloc(reinterpret_cast<void **>(&s.h_hash_offs), (B + 1) * 8u, hipHostMallocDefault));
         }
         SERVE_TRY(hipMalloc(reinterpret_cast<void **>(&s.d_offs), (B + 1) * 8u));
         SERVE_TRY(hipMalloc(reinterpret_cast<void **>(&s.d_thr), B * 2u));

@@ -26,6 +26,9 @@ def test_kmer_hash_matches_oracle(k):
     _say(f"kmer hash k={k}")
     rng = random.Random(k)
     reads = _reads(rng, 300, 0, 400, "ACGTACGTACGTacgtNU") + ["", "A" * (k - 1), "C" * k]
+    # Reads longer than the kernel's 8,192-base segments, and lengths at the segment seams.
+    reads += _reads(rng, 1, 20000, 20000) + _reads(rng, 1, 8192, 8192) + _reads(rng, 1, 8192 + k - 1, 8192 + k - 1)
+    reads += _reads(rng, 1, 8192 + k, 8192 + k) + _reads(rng, 1, 16384 - k + 2, 16384 - k + 2)
     hashes, offsets = kmer_hash(reads, k)
     h, o = to_numpy(hashes), to_numpy(offsets)
     assert o[0] == 0 and o.size == len(reads) + 1
