@@ -22,6 +22,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 
@@ -394,14 +395,26 @@ double normalization(uint32_t bits)
     return alpha * static_cast<double>(size) * static_cast<double>(size);
 }
 
-// Per bit width: m ln(m / z) for z = 0 .. m (entry 0 unused), computed once with
-// the reference's expression and kept on the device for the process.
+// Per device and bit width: m ln(m / z) for z = 0 .. m (entry 0 unused),
+// computed once with the reference's expression and kept on the device for the
+// process.
 double const * small_estimate_table(uint32_t bits, hipError_t & e)
 {
-    static double * tables[hibf_amd::kUnionMaxBits + 1] = {};
-    e = hipSuccess;
-    if (tables[bits])
-        return tables[bits];
+    constexpr int kMaxDevices = 64;
+    static double * tables[kMaxDevices][hibf_amd::kUnionMaxBits + 1] = {};
+    static std::mutex lock;
+    int dev = 0;
+    e = hipGetDevice(&dev);
+    if (e != hipSuccess)
+        return nullptr;
+    if (dev < 0 || dev >= kMaxDevices)
+    {
+        e = hipErrorInvalidDevice;
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> const guard{lock};
+    if (tables[dev][bits])
+        return tables[dev][bits];
     uint64_t const size = 1ULL << bits;
     std::vector<double> h(size + 1u, 0.0);
     for (uint64_t z = 1; z <= size; ++z)
@@ -416,7 +429,7 @@ double const * small_estimate_table(uint32_t bits, hipError_t & e)
             (void)hipFree(d);
         return nullptr;
     }
-    tables[bits] = d;
+    tables[dev][bits] = d;
     return d;
 }
 
