@@ -271,7 +271,8 @@ int hibf_gpu_hibf_build_times(hibf_gpu_hibf hibf, double * h_seconds);
 /* Parts of the layout time of that build, in seconds: [0] union estimation
  * (on the device), [1] similarity rearrangement.  Both 0 when the layout did
  * not use union estimates.  They feed layout_union_estimation_timer and
- * layout_rearrangement_timer. */
+ * layout_rearrangement_timer (hierarchical_interleaved_bloom_filter.hpp:283-290;
+ * filled by compute_layout, src/layout/compute_layout.cpp:26-55). */
 int hibf_gpu_hibf_layout_times(hibf_gpu_hibf hibf, double * h_seconds);
 
 /* ibf_vector[i] (hierarchical_interleaved_bloom_filter.hpp:209): an IBF handle
