@@ -70,7 +70,8 @@ def build_tools(verbose: bool = False) -> list[str]:
 
 
 CPP_TESTS = [os.path.join(ROOT, "tests", "cpp", "snippets.cpp"), os.path.join(ROOT, "tests", "cpp", "archive_roundtrip.cpp"),
-             os.path.join(ROOT, "tests", "cpp", "hibf_members.cpp"), os.path.join(ROOT, "tests", "cpp", "hll_device.cpp")]
+             os.path.join(ROOT, "tests", "cpp", "hibf_members.cpp"), os.path.join(ROOT, "tests", "cpp", "hll_device.cpp"),
+             os.path.join(ROOT, "tests", "cpp", "compute_layout.cpp")]
 
 
 def build_cpp_tests(verbose: bool = False) -> list[str]:

@@ -1470,7 +1470,8 @@ int sketched_layout(host::Config const & hc,
                     std::vector<uint64_t> const & counts,
                     uint8_t const * d_regs,
                     host::Layout & layout,
-                    double * times)
+                    double * times,
+                    std::vector<uint64_t> const * positions = nullptr)
 {
     uint64_t const n = hc.number_of_user_bins;
     uint32_t const bits = static_cast<uint32_t>(hc.sketch_bits);
@@ -1497,7 +1498,7 @@ int sketched_layout(host::Config const & hc,
     }
     try
     {
-        layout = host::compute_layout(hc, counts, &sk);
+        layout = host::compute_layout(hc, counts, &sk, positions);
     }
     catch (std::out_of_range const &)
     {
@@ -1613,6 +1614,67 @@ int hibf_gpu_layout_compute_sketched(hibf_gpu_config const * c,
         double times[2]{};
         int const rc = hc.disable_estimate_union ? get_layout(hc, counts, nullptr, layout)
                                                  : sketched_layout(hc, counts, d_registers, layout, times);
+        if (rc)
+            return rc;
+        if (h_times)
+        {
+            h_times[0] = times[0];
+            h_times[1] = times[1];
+        }
+        return copy_text(layout.to_text(), text, cap, needed);
+    }
+    catch (std::exception const & ex)
+    {
+        return fail(HIBF_GPU_ERR_OUT_OF_MEMORY, ex.what());
+    }
+}
+
+int hibf_gpu_layout_compute_host_sketches(hibf_gpu_config const * c,
+                                          uint64_t const * h_kmer_counts,
+                                          uint8_t const * h_registers,
+                                          uint64_t const * h_positions,
+                                          uint64_t n_positions,
+                                          char * text,
+                                          uint64_t cap,
+                                          uint64_t * needed,
+                                          double * h_times)
+{
+    if (!c || !h_kmer_counts || !h_registers || (!h_positions && n_positions))
+        return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "null argument");
+    host::Config hc = to_host(*c);
+    std::string const e = host::validate_and_set_defaults(hc);
+    if (!e.empty())
+        return fail(HIBF_GPU_ERR_CONFIG, e);
+    uint64_t const n = hc.number_of_user_bins;
+    uint32_t const bits = static_cast<uint32_t>(hc.sketch_bits);
+    if (n > (~0ULL >> bits))
+        return fail(HIBF_GPU_ERR_UNSUPPORTED, "number_of_user_bins x 2^sketch_bits sketch registers exceed 2^64 bytes");
+    std::vector<uint64_t> positions;
+    if (h_positions)
+    {
+        if (n_positions == 0)
+            return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "no user bins to lay out");
+        positions.assign(h_positions, h_positions + n_positions);
+        for (uint64_t const pos : positions)
+            if (pos >= n)
+                return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "position out of range");
+    }
+    try
+    {
+        std::vector<uint64_t> counts(h_kmer_counts, h_kmer_counts + n);
+        host::Layout layout;
+        double times[2]{};
+        int rc = HIBF_GPU_OK;
+        if (hc.disable_estimate_union)
+            layout = host::compute_layout(hc, counts, nullptr, h_positions ? &positions : nullptr);
+        else
+        {
+            DeviceBuffer d_regs;
+            HIBF_TRY_HIP(d_regs.alloc(static_cast<size_t>(n) << bits), "sketch registers");
+            HIBF_TRY_HIP(hipMemcpy(d_regs.p, h_registers, static_cast<size_t>(n) << bits, hipMemcpyHostToDevice),
+                         "sketch registers");
+            rc = sketched_layout(hc, counts, d_regs.u8(), layout, times, h_positions ? &positions : nullptr);
+        }
         if (rc)
             return rc;
         if (h_times)

@@ -511,7 +511,10 @@ uint64_t const * HostUnionEstimates::column(uint64_t j)
     return column_.data();
 }
 
-Layout compute_layout(Config const & c, std::vector<uint64_t> const & kmer_counts, Sketching * sketching)
+Layout compute_layout(Config const & c,
+                      std::vector<uint64_t> const & kmer_counts,
+                      Sketching * sketching,
+                      std::vector<uint64_t> const * positions)
 {
     Layout layout;
     std::vector<double> const corr = fpr_correction(c.maximum_fpr, c.number_of_hash_functions, c.tmax);
@@ -519,9 +522,14 @@ Layout compute_layout(Config const & c, std::vector<uint64_t> const & kmer_count
         sketching = nullptr;
     Store store{&layout, &kmer_counts, {}, &corr,
                 relaxed_fpr_correction(c.maximum_fpr, c.relaxed_fpr, c.number_of_hash_functions), {}, sketching};
-    store.positions.resize(c.number_of_user_bins);
-    for (uint64_t i = 0; i < c.number_of_user_bins; ++i)
-        store.positions[i] = i;
+    if (positions)
+        store.positions = *positions;
+    else
+    {
+        store.positions.resize(c.number_of_user_bins);
+        for (uint64_t i = 0; i < c.number_of_user_bins; ++i)
+            store.positions[i] = i;
+    }
     layout.top_level_max_bin_id = hierarchical_binning(store, c);
     // compute_layout.cpp:61-67: max bins by depth (stable here; the order of
     // equal depths does not change the built filter's answers).

@@ -122,6 +122,10 @@ struct Sketching
 // (the reference's disable_estimate_union layout); with them the DP uses union
 // estimates and, unless disable_rearrangement, rearranges each level's user
 // bins by similarity first, as the reference.
-Layout compute_layout(Config const & c, std::vector<uint64_t> const & kmer_counts, Sketching * sketching = nullptr);
+// positions: the user bins to lay out (all of them when null).
+Layout compute_layout(Config const & c,
+                      std::vector<uint64_t> const & kmer_counts,
+                      Sketching * sketching = nullptr,
+                      std::vector<uint64_t> const * positions = nullptr);
 
 } // namespace hibf_amd::host

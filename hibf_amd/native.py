@@ -155,6 +155,8 @@ _SIGNATURES = {
     "hibf_gpu_layout_compute": (_INT, [ctypes.POINTER(Config), _V, ctypes.c_char_p, _U64, ctypes.POINTER(_U64)]),
     "hibf_gpu_layout_compute_sketched": (_INT, [ctypes.POINTER(Config), _V, _V, ctypes.c_char_p, _U64,
                                                 ctypes.POINTER(_U64), _V]),
+    "hibf_gpu_layout_compute_host_sketches": (_INT, [ctypes.POINTER(Config), _V, _V, _V, _U64, ctypes.c_char_p, _U64,
+                                                     ctypes.POINTER(_U64), _V]),
     "hibf_gpu_hibf_build": (_INT, [ctypes.POINTER(Config), _V, _V, ctypes.c_char_p, ctypes.POINTER(_V)]),
     "hibf_gpu_hibf_build_synthetic": (_INT, [ctypes.POINTER(Config), _V, _U64, ctypes.c_char_p, ctypes.POINTER(_V)]),
     "hibf_gpu_hibf_plan_stats": (_INT, [ctypes.POINTER(Config), _V, ctypes.c_char_p] + [ctypes.POINTER(_U64)] * 4),

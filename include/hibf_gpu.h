@@ -612,6 +612,22 @@ int hibf_gpu_layout_compute_sketched(const hibf_gpu_config * config,
                                      uint64_t * needed,
                                      double * h_times);
 
+/* compute_layout(config, kmer_counts, sketches, positions, timers)
+ * (layout/compute_layout.cpp:26-55) with the sketches in host memory
+ * (h_registers: number_of_user_bins x 2^sketch_bits uint8; uploaded for the
+ * call) and, when h_positions is not null, only the n_positions user bins it
+ * lists laid out (each < number_of_user_bins).  Otherwise as
+ * hibf_gpu_layout_compute_sketched.  Synchronous. */
+int hibf_gpu_layout_compute_host_sketches(const hibf_gpu_config * config,
+                                          const uint64_t * h_kmer_counts,
+                                          const uint8_t * h_registers,
+                                          const uint64_t * h_positions,
+                                          uint64_t n_positions,
+                                          char * text,
+                                          uint64_t cap,
+                                          uint64_t * needed,
+                                          double * h_times);
+
 /* Replaces hierarchical_interleaved_bloom_filter(config&) and (config&, layout)
  * (src/hierarchical_interleaved_bloom_filter.cpp:211-253): user bin u holds
  * h_values[h_ub_offsets[u] .. h_ub_offsets[u+1]) (duplicates ignored).  The

@@ -16,7 +16,8 @@ def test_cpp_headers_compile(tmp_path):
     for hdr in ("hibf/interleaved_bloom_filter.hpp", "hibf/hierarchical_interleaved_bloom_filter.hpp",
                 "hibf/config.hpp", "hibf/misc/counting_vector.hpp", "hibf/misc/bit_vector.hpp",
                 "hibf/misc/print.hpp", "hibf/misc/timer.hpp", "hibf/layout/layout.hpp", "hibf/sketch/hyperloglog.hpp",
-                "hibf/sketch/compute_sketches.hpp", "hibf/sketch/toolbox.hpp", "hibf_gpu.h"):
+                "hibf/sketch/compute_sketches.hpp", "hibf/sketch/toolbox.hpp", "hibf/layout/compute_layout.hpp",
+                "hibf_gpu.h"):
         src = tmp_path / "t.cpp"
         src.write_text(f"#include <{hdr}>\nint main() {{}}\n")
         r = subprocess.run(["g++", "-std=c++20", "-fsyntax-only", f"-I{ROOT}/include", str(src)],
@@ -228,3 +229,32 @@ def test_layout_union_mode_reduces_to_sum_mode(tmp_path, n, tmax, seed):
     r = subprocess.run([str(exe), str(n), str(tmax), str(seed)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     assert r.stdout.startswith("ok "), r.stdout
+
+
+@pytest.mark.gpu
+def test_cpp_compute_layout_on_device():
+    """seqan::hibf::layout::compute_layout (union estimates and rearrangement on
+    the device): the reference's dispatch case, a subset of positions, the
+    timers, and the 300-user-bin layout equal to the library's own sketched
+    layout of the same user bins."""
+    import numpy as np
+    import oracle as O
+    from hibf_amd import native
+    from hibf_amd.device import hll_estimates, hll_sketches
+    exe = os.path.join(ROOT, "tests", "cpp", "compute_layout")
+    assert os.path.exists(exe), "build tests/cpp/compute_layout first (python -m hibf_amd.build)"
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    lines = r.stdout.splitlines()
+    assert lines[:3] == ["dispatch ok", "timers ok", "positions_subset ok"], lines[:3]
+    stream = O.mt19937_64(5, 300 * 3100).tolist()
+    sets, at = [], 0
+    for _ in range(300):
+        n = 50 + stream[at] % 3000
+        sets.append(np.asarray(stream[at + 1:at + 1 + n], dtype=np.uint64))
+        at += 1 + n
+    regs = hll_sketches(sets)
+    counts = hll_estimates(regs).astype(np.uint64)
+    c = native.validate_config(native.make_config(number_of_user_bins=300, tmax=64))
+    want = native.compute_layout_sketched(c, counts, regs.data_ptr())
+    assert "\n".join(lines[3:]) + "\n" == want
