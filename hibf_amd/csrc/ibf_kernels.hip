@@ -383,8 +383,15 @@ hipError_t launch_contains(IbfView const & g, uint64_t const * values, uint64_t 
         return hipSuccess;
     uint32_t const vec = g.bin_words % 2u == 0u && reinterpret_cast<uintptr_t>(g.words) % 16u == 0u &&
                          reinterpret_cast<uintptr_t>(out) % 16u == 0u;
-    hipLaunchKernelGGL(contains_kernel<4>, dim3(grid_for(n, 256)), dim3(256), 0, stream, g, values, n, out, vec,
-                       g_count_tuning.nt_stores);
+    uint32_t const nt = g_count_tuning.nt_stores;
+    unsigned const grid = grid_for(n, 256);
+    switch (g_count_tuning.contains_unroll)
+    {
+    case 1: hipLaunchKernelGGL(contains_kernel<1>, dim3(grid), dim3(256), 0, stream, g, values, n, out, vec, nt); break;
+    case 2: hipLaunchKernelGGL(contains_kernel<2>, dim3(grid), dim3(256), 0, stream, g, values, n, out, vec, nt); break;
+    case 8: hipLaunchKernelGGL(contains_kernel<8>, dim3(grid), dim3(256), 0, stream, g, values, n, out, vec, nt); break;
+    default: hipLaunchKernelGGL(contains_kernel<4>, dim3(grid), dim3(256), 0, stream, g, values, n, out, vec, nt);
+    }
     return hipGetLastError();
 }
 

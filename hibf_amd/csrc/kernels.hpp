@@ -46,6 +46,7 @@ struct CountTuning
 {
     uint32_t wide_kernel = 1; // use count_wide_kernel for even bin_words >= 64
     uint32_t wide_unroll = 2; // values in flight per wave (1, 2, 4, 8)
+    uint32_t contains_unroll = 4; // bulk_contains: values whose rows a wave loads at once (1, 2, 4, 8)
     uint32_t wide_waves = 4;  // waves per block (1..4)
     uint32_t hibf_wave = 0;   // HIBF descent: one wave per query (1) or one block per query (0)
     uint32_t nt_loads = 0;    // HIBF descent: non-temporal row loads

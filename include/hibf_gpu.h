@@ -54,9 +54,10 @@ const char * hibf_gpu_last_error(void);
  * (descent over rows of <= 4 words: bit-sliced per-lane counts 1, one LDS
  * atomic per set bit 0), "hibf.nt_loads" (0/1), "hibf.block_waves" (waves
  * per query block of the membership descent: 1, 2, 4 default, 8), "count.nt_stores"
- * (non-temporal stores of the IBF count kernel's counters and of the HIBF
- * count's zero fill: 1, default; plain stores 0).  Results never depend on
- * these values. */
+ * (non-temporal stores of the IBF count kernel's counters, of the HIBF
+ * count's zero fill and of bulk_contains' rows: 1, default; plain stores 0),
+ * "contains.unroll" (values whose rows one wave of bulk_contains loads at
+ * once: 1, 2, 4 default, 8).  Results never depend on these values. */
 int hibf_gpu_tune_set(const char * key, int64_t value);
 int hibf_gpu_tune_get(const char * key, int64_t * value);
 

@@ -125,7 +125,7 @@ def test_bulk_count_long_queries_wide_rows(bins):
 def knobs():
     """Sets launch-shape knobs for one test and restores the defaults after it."""
     from hibf_amd import native
-    keys = ("count.wide_unroll", "count.wide_waves", "count.counter_bits", "count.nt_stores")
+    keys = ("count.wide_unroll", "count.wide_waves", "count.counter_bits", "count.nt_stores", "contains.unroll")
     saved = {k: native.tune_get(k) for k in keys}
 
     def set_(**kv):
@@ -160,15 +160,17 @@ def test_counter_store_forms(nt_stores, knobs):
                               href.bulk_count_batch(hv, ho, [20] * 50, width)), width
 
 
+@pytest.mark.parametrize("unroll", [1, 2, 4, 8])
 @pytest.mark.parametrize("nt_stores", [0, 1])
 @pytest.mark.parametrize("bins", [8192, 9000])
-def test_contains_store_forms(nt_stores, bins, knobs):
-    """bulk_contains rows are the oracle's with plain and non-temporal stores,
-    into a 16-byte aligned output and into one 8 bytes off (the 8-byte path),
-    for an even (128) and an odd (141) row width in words."""
+def test_contains_store_forms(unroll, nt_stores, bins, knobs):
+    """bulk_contains rows are the oracle's for 1, 2, 4 and 8 values in flight
+    per wave, with plain and non-temporal stores, into a 16-byte aligned output
+    and into one 8 bytes off (the 8-byte path), for an even (128) and an odd
+    (141) row width in words; 633 values leave a ragged last group."""
     import torch
-    _say(f"contains stores nt={nt_stores} bins={bins}")
-    knobs(count__nt_stores=nt_stores)
+    _say(f"contains stores unroll={unroll} nt={nt_stores} bins={bins}")
+    knobs(count__nt_stores=nt_stores, contains__unroll=unroll)
     ref, dev, vals, _, rng = _pair(bins, 1024, 3, 20000, 11)
     q = np.concatenate([vals[:333], _rand_u64(rng, 300)])
     want = np.stack([ref.bulk_contains(int(v)) for v in q])
