@@ -486,6 +486,20 @@ uint64_t hierarchical_binning(Store & d, Config const & c)
 
 } // namespace
 
+uint64_t simple_binning_layout(std::vector<uint64_t> const & kmer_counts,
+                               std::vector<uint64_t> const & positions,
+                               std::vector<double> const & fpr_correction,
+                               uint64_t num_technical_bins,
+                               Layout & layout)
+{
+    if (positions.size() > num_technical_bins)
+        throw std::runtime_error("You cannot have less technical bins than user bins for this simple binning "
+                                 "algorithm. Please see the hierarchical_binning algorithm or increase the number "
+                                 "of technical bins.");
+    Store store{&layout, &kmer_counts, positions, &fpr_correction, 1.0, {}, nullptr};
+    return simple_binning(store, num_technical_bins);
+}
+
 bool HostUnionEstimates::start(std::vector<uint64_t> const & positions)
 {
     positions_ = positions;

@@ -122,6 +122,15 @@ struct Sketching
 // (the reference's disable_estimate_union layout); with them the DP uses union
 // estimates and, unless disable_rearrangement, rearranges each level's user
 // bins by similarity first, as the reference.
+// simple_binning.cpp:19-116 on its own: user bins `positions` (in that order)
+// over num_technical_bins bins with the given FPR corrections; appends the
+// records to `layout` and returns the id of the fullest bin.
+uint64_t simple_binning_layout(std::vector<uint64_t> const & kmer_counts,
+                               std::vector<uint64_t> const & positions,
+                               std::vector<double> const & fpr_correction,
+                               uint64_t num_technical_bins,
+                               Layout & layout);
+
 // positions: the user bins to lay out (all of them when null).
 Layout compute_layout(Config const & c,
                       std::vector<uint64_t> const & kmer_counts,

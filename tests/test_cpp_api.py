@@ -258,3 +258,19 @@ def test_cpp_compute_layout_on_device():
     c = native.validate_config(native.make_config(number_of_user_bins=300, tmax=64))
     want = native.compute_layout_sketched(c, counts, regs.data_ptr())
     assert "\n".join(lines[3:]) + "\n" == want
+
+
+def test_layout_dp_reference_expectations(tmp_path):
+    """The layout DP reproduces every expectation of the reference's
+    hierarchical_binning_test.cpp (eight trees: split and merged bins, empty-bin
+    fraction, alpha 1, two to four levels, merged technical bin 0) and
+    simple_binning_test.cpp (two layouts and the too-few-bins error)."""
+    exe = tmp_path / "binning_goldens"
+    r = subprocess.run(["g++", "-std=c++20", "-O2", "-Wall", f"-I{ROOT}/include", f"-I{ROOT}/hibf_amd/csrc",
+                        os.path.join(ROOT, "tests", "cpp", "binning_goldens.cpp"),
+                        os.path.join(ROOT, "hibf_amd", "csrc", "layout.cpp"), "-o", str(exe)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
+    lines = r.stdout.splitlines()
+    assert r.returncode == 0 and len(lines) == 11 and all(line.endswith(" ok") for line in lines), r.stdout
