@@ -157,16 +157,20 @@ uint64_t simple_binning(Store & d, uint64_t num_technical_bins)
         uint64_t const corrected = static_cast<uint64_t>(init_card * corr[i + 1]);
         matrix[i][0] = divide_and_ceil(corrected, i + 1u);
     }
+    // The size of a split into s bins depends on the column only: split[s]
+    // (the reference recomputes it for every cell).
+    std::vector<uint64_t> split(extra_bins + 1u, 0u);
     for (uint64_t j = 1; j < num_user_bins; ++j)
     {
         double const card = static_cast<double>(counts[d.positions[j]]);
+        for (uint64_t sb = 1; sb <= extra_bins; ++sb)
+            split[sb] = divide_and_ceil(static_cast<uint64_t>(card * corr[sb]), sb);
         for (uint64_t i = j; i < j + extra_bins; ++i)
         {
             uint64_t minimum = maxv;
             for (uint64_t ip = j - 1; ip < i; ++ip)
             {
-                uint64_t const corrected = static_cast<uint64_t>(card * corr[i - ip]);
-                uint64_t const score = std::max<uint64_t>(divide_and_ceil(corrected, i - ip), matrix[ip][j - 1]);
+                uint64_t const score = std::max<uint64_t>(split[i - ip], matrix[ip][j - 1]);
                 if (score < minimum)
                 {
                     trace[i][j] = ip;
@@ -303,18 +307,26 @@ uint64_t hierarchical_binning(Store & d, Config const & c)
         colM[i] = matrix[i][0];
         colL[i] = ll[i][0];
     }
+    // split_size[s]: the size of column j's user bin split into s technical bins,
+    // which the vertical scan of every cell of the column needs (computed once
+    // per column here; the reference recomputes it per cell).
+    std::vector<uint64_t> split_size(num_technical_bins, 0u);
+    auto column_split = [&](uint64_t j)
+    {
+        double const card = static_cast<double>(sorted_counts[j]);
+        for (uint64_t sb = 1; sb < num_technical_bins; ++sb)
+            split_size[sb] = divide_and_ceil(static_cast<uint64_t>(card * corr[sb]), sb);
+    };
     auto cell = [&](uint64_t i, uint64_t j)
     {
         uint64_t const current_weight = sorted_counts[j];
-        double const card = static_cast<double>(current_weight);
         uint64_t minimum = maxv;
         uint64_t full_minimum = maxv;
         std::pair<uint64_t, uint64_t> best{maxv, maxv};
         uint64_t best_ll = 0;
         for (uint64_t ip = 0; ip < i; ++ip)
         {
-            uint64_t const corrected = static_cast<uint64_t>(card * corr[i - ip]);
-            uint64_t const score = std::max<uint64_t>(divide_and_ceil(corrected, i - ip), colM[ip]);
+            uint64_t const score = std::max<uint64_t>(split_size[i - ip], colM[ip]);
             uint64_t const full_score =
                 static_cast<uint64_t>(static_cast<double>(score * (i + 1)) + c.alpha * static_cast<double>(colL[ip]));
             if (full_score < full_minimum)
@@ -375,6 +387,7 @@ uint64_t hierarchical_binning(Store & d, Config const & c)
         {
             if (!column_unions(j))
                 throw std::runtime_error("union estimates failed");
+            column_split(j);
             for (uint64_t i = 1; i < num_technical_bins; ++i)
                 cell(i, j);
             for (uint64_t i = 0; i < num_technical_bins; ++i)
@@ -392,8 +405,11 @@ uint64_t hierarchical_binning(Store & d, Config const & c)
         {
             for (uint64_t j = 1; j < num_user_bins; ++j)
             {
-                if (t == 0 && !column_unions(j))
-                    failed = true;
+                if (t == 0)
+                {
+                    failed = !column_unions(j);
+                    column_split(j);
+                }
                 sync.arrive_and_wait();
                 if (failed)
                     return;
