@@ -766,6 +766,10 @@ def run_serve(args):
     from hibf_amd import workloads as W
     from hibf_amd.device import DeviceHibf
 
+    from hibf_amd import native
+    for kv in args.tune:
+        k, v = kv.split("=")
+        native.tune_set(k, int(v))
     torch.cuda.set_device(0)
     spec = W.HIBF_AT_SCALE
     n_ub = args.hibf_user_bins
@@ -843,8 +847,12 @@ def run_serve_reads(args):
     them (1 % base errors), Poisson arrivals; bases go to the device, which
     hashes them and uses threshold uint16_t(0.7 x #k-mers)."""
     import torch
+    from hibf_amd import native
     from hibf_amd.device import DeviceHibf, kmer_hash, to_numpy
 
+    for kv in args.tune:
+        k_, v_ = kv.split("=")
+        native.tune_set(k_, int(v_))
     torch.cuda.set_device(0)
     k, read_len = 19, 250
     rng = np.random.default_rng(args.seed + 41)
