@@ -1,12 +1,15 @@
 // The layout DP against the expectations of the reference's
 // test/unit/hibf/layout/hierarchical_binning_test.cpp:38-302 and
-// simple_binning_test.cpp:17-72 (inputs and expected records copied as data).
+// simple_binning_test.cpp:17-72 (inputs and expected records copied as data),
+// and the FPR corrections of fpr_correction_test.cpp:13-43.
 // The hierarchical cases run with the tests' raw tmax (no rounding to 64) and
 // union estimation disabled; compute_layout orders the merged-bin records by
 // depth (stably), so the expected ones are ordered the same way before the
 // comparison.  Prints "<case> ok" or "<case> FAILED" per case.
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
+#include <utility>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -90,6 +93,23 @@ int main()
                  {{{0, 0}, 0, 32, 5}, {{0, 0}, 32, 32, 4}, {{0, 1}, 0, 32, 7}, {{0, 1}, 32, 32, 6}, {{1, 0}, 0, 32, 1},
                   {{1, 0}, 32, 32, 0}, {{1, 1}, 0, 32, 3}, {{1, 1}, 32, 32, 2}});
 
+    {   // fpr_correction_test.cpp:13-43
+        std::vector<double> const one = fpr_correction(0.05, 2, 8);
+        bool ok = one.size() == 9u;
+        for (uint64_t const v : {9123u, 123u, 12u, 87123u, 8123u, 4660u})
+            ok = ok && static_cast<double>(v) == static_cast<double>(v) * one[1];
+        std::vector<double> const c = fpr_correction(0.01, 5, 256);
+        std::vector<std::pair<size_t, double>> const want{{1, 1.0},       {2, 1.192316},  {4, 1.412390},
+                                                          {8, 1.664459},  {16, 1.953384}, {32, 2.284738},
+                                                          {64, 2.664909}, {128, 3.101225}, {256, 3.602093}};
+        ok = ok && c.size() == 257u;
+        for (auto const & [i, v] : want)
+            ok = ok && c.size() > i && std::abs(c[i] - v) <= 0.00001;
+        for (size_t i = 1; ok && i < 256u; ++i)
+            ok = c[i] <= c[i + 1];
+        std::printf("fpr_correction %s\n", ok ? "ok" : "FAILED");
+        failures += !ok;
+    }
     std::vector<double> const ones(65, 1.0);
     {
         Layout l;

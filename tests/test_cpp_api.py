@@ -264,7 +264,8 @@ def test_layout_dp_reference_expectations(tmp_path):
     """The layout DP reproduces every expectation of the reference's
     hierarchical_binning_test.cpp (eight trees: split and merged bins, empty-bin
     fraction, alpha 1, two to four levels, merged technical bin 0) and
-    simple_binning_test.cpp (two layouts and the too-few-bins error)."""
+    simple_binning_test.cpp (two layouts and the too-few-bins error), and the
+    FPR corrections of fpr_correction_test.cpp."""
     exe = tmp_path / "binning_goldens"
     r = subprocess.run(["g++", "-std=c++20", "-O2", "-Wall", f"-I{ROOT}/include", f"-I{ROOT}/hibf_amd/csrc",
                         os.path.join(ROOT, "tests", "cpp", "binning_goldens.cpp"),
@@ -273,4 +274,4 @@ def test_layout_dp_reference_expectations(tmp_path):
     assert r.returncode == 0, r.stderr
     r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
     lines = r.stdout.splitlines()
-    assert r.returncode == 0 and len(lines) == 11 and all(line.endswith(" ok") for line in lines), r.stdout
+    assert r.returncode == 0 and len(lines) == 12 and all(line.endswith(" ok") for line in lines), r.stdout
