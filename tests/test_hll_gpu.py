@@ -153,3 +153,29 @@ def test_sketched_layout_on_device_equals_host(tmp_path, n, bits, tmax):
     assert got == want
     plain = native.compute_layout(c, counts)
     assert plain != got  # union estimates and rearrangement changed the layout
+
+
+def test_wide_sketch_build_uses_host_union_estimates():
+    """Sketches of 13 bits are past the device union estimates: the layout's
+    union and pair estimates run on the host (the reference's sequential
+    merges), give the layout of compute_layout_sketched's host path, and the
+    built HIBF answers like the oracle on its tables."""
+    from hibf_amd import native
+    from hibf_amd.device import DeviceHibf, csr, hll_estimates, hll_sketches, to_numpy
+    from hibf_helpers import oracle_of
+    _say("13-bit sketches")
+    rng = np.random.default_rng(13)
+    ubs = _sets(rng, rng.integers(50, 800, size=90))
+    dev = DeviceHibf.build(ubs, tmax=64, sketch_bits=13)
+    regs = hll_sketches([np.unique(u) for u in ubs], bits=13)
+    c = native.validate_config(native.make_config(number_of_user_bins=len(ubs), tmax=64, sketch_bits=13))
+    text = native.compute_layout_sketched(c, hll_estimates(regs).astype(np.uint64), regs.data_ptr())
+    same = DeviceHibf.build(ubs, layout_text=text, tmax=64, sketch_bits=13)
+    assert np.array_equal(dev.download_arena(), same.download_arena())
+    ref = oracle_of(dev)
+    qs = [np.concatenate([ubs[int(i)][:30], _sets(rng, [10])[0]]) for i in rng.integers(0, len(ubs), size=40)]
+    v, o = csr(qs)
+    ids, hoff = dev.membership_for(v, o, 25)
+    ids, hoff = to_numpy(ids), to_numpy(hoff)
+    for q in range(len(qs)):
+        assert ids[int(hoff[q]):int(hoff[q + 1])].tolist() == ref.membership_for(qs[q], 25)
