@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -122,6 +123,53 @@ int ensure(void *& ptr, size_t & have, size_t bytes, char const * what)
     return HIBF_GPU_OK;
 }
 
+} // namespace
+
+// Workspace of hibf_gpu_hibf_membership_launch_noalloc, the entry point for
+// callers that cannot allocate device memory.  One copy per GPU, held from
+// when the library's code object is loaded on that GPU until the process
+// ends; every call on a GPU shares it, so calls that overlap on one GPU race.
+// Layout: the descent counters (256 bytes), the wave kernel's overflow query
+// list (8 bytes per query, kNoAllocMaxQueries queries), then the block
+// kernel's per-block hit lists.  In host code `scratch` names a host-side
+// stand-in: kernels get the device address from hipGetSymbolAddress.
+namespace
+{
+constexpr uint64_t kNoAllocScratchBytes = 64ULL << 20;
+constexpr uint64_t kNoAllocCounterBytes = 256;
+constexpr uint64_t kNoAllocMaxQueries = 65536;
+constexpr uint64_t kNoAllocHitListOffset = kNoAllocCounterBytes + kNoAllocMaxQueries * sizeof(uint64_t);
+static_assert(kNoAllocHitListOffset % 256 == 0, "hit lists start 256-byte aligned");
+constexpr int kNoAllocMaxDevices = 64;
+} // namespace
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace
+{
+std::mutex g_noalloc_mutex;
+unsigned char * g_noalloc_scratch[kNoAllocMaxDevices] = {};
+
+// Device address of this GPU's `scratch`, looked up once per device with the
+// device current (the first no-allocation call on it).
+int noalloc_scratch(unsigned char ** out)
+{
+    int dev = 0;
+    HIBF_TRY_HIP(hipGetDevice(&dev), "hipGetDevice");
+    if (dev < 0 || dev >= kNoAllocMaxDevices)
+        return fail(HIBF_GPU_ERR_UNSUPPORTED, "no-allocation workspace: device index beyond 63");
+    std::lock_guard<std::mutex> lock(g_noalloc_mutex);
+    if (!g_noalloc_scratch[dev])
+    {
+        void * p = nullptr;
+        HIBF_TRY_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+        if (!p)
+            return fail(HIBF_GPU_ERR_DEVICE, "hipGetSymbolAddress(scratch) returned null");
+        g_noalloc_scratch[dev] = static_cast<unsigned char *>(p);
+    }
+    *out = g_noalloc_scratch[dev];
+    return HIBF_GPU_OK;
+}
 } // namespace
 
 namespace hibf_amd
@@ -1041,11 +1089,16 @@ int hibf_gpu_hibf_download_arena(hibf_gpu_hibf h, uint64_t * h_arena, uint64_t n
 
 namespace
 {
-HibfLaunch launch_of(hibf_gpu_hibf h, hibf_gpu_agent a)
+HibfLaunch launch_with(hibf_gpu_hibf h, unsigned long long * counters)
 {
     return HibfLaunch{h->d_desc,      h->d_arena,   h->d_tb_info, h->max_bins, h->depth, h->n_user_bins,
                       h->max_hits,    (h->max_bins + 63u) / 64u, h->uniform_hash_funs, g_count_tuning.nt_stores,
-                      0u,             a->d_total};
+                      0u,             counters};
+}
+
+HibfLaunch launch_of(hibf_gpu_hibf h, hibf_gpu_agent a)
+{
+    return launch_with(h, a->d_total);
 }
 
 // Blocks of `waves` waves for a query-per-block launch: 32 waves per CU (8 per
@@ -1172,6 +1225,62 @@ int hibf_gpu_hibf_membership_launch(hibf_gpu_hibf h,
         return HIBF_GPU_OK;
     return enqueue_membership(h, a, d_values, d_offsets, nq, d_thresholds, d_ids, ids_cap, d_hit_start, d_hit_count,
                               static_cast<hipStream_t>(stream));
+}
+
+int hibf_gpu_hibf_membership_launch_noalloc(hibf_gpu_hibf h,
+                                            uint64_t const * d_values,
+                                            uint64_t const * d_offsets,
+                                            uint64_t nq,
+                                            uint16_t const * d_thresholds,
+                                            uint64_t * d_ids,
+                                            uint64_t ids_cap,
+                                            uint64_t * d_hit_start,
+                                            uint64_t * d_hit_count,
+                                            void * stream)
+{
+    if (!h || ((!d_offsets || !d_thresholds || !d_hit_start || !d_hit_count) && nq))
+        return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "null argument");
+    if (nq == 0)
+        return HIBF_GPU_OK;
+    if (nq > kNoAllocMaxQueries)
+        return fail(HIBF_GPU_ERR_INVALID_ARGUMENT,
+                    "hibf_gpu_hibf_membership_launch_noalloc: at most 65536 queries per call (got " +
+                        std::to_string(nq) + ")");
+    uint64_t const per_block = block_hit_list_bytes(h);
+    uint64_t const max_blocks = (kNoAllocScratchBytes - kNoAllocHitListOffset) / per_block;
+    if (max_blocks == 0)
+        return fail(HIBF_GPU_ERR_UNSUPPORTED,
+                    "hibf_gpu_hibf_membership_launch_noalloc: one query's hit list (" + std::to_string(per_block) +
+                        " bytes) exceeds the 64 MiB workspace");
+    unsigned char * ws = nullptr;
+    if (int rc = noalloc_scratch(&ws))
+        return rc;
+    auto * const counters = reinterpret_cast<unsigned long long *>(ws);
+    auto * const overflow = reinterpret_cast<uint64_t *>(ws + kNoAllocCounterBytes);
+    auto * const hit_lists = reinterpret_cast<uint32_t *>(ws + kNoAllocHitListOffset);
+    hipStream_t const s = static_cast<hipStream_t>(stream);
+    HibfLaunch const L = launch_with(h, counters);
+    // Every counter starts at zero: the previous call's cursor, totals and flags stay in the workspace.
+    HIBF_TRY_HIP(launch_reset_counters(counters, 0xFFu, s), "counter reset");
+    if (wave_kernel_fits(L))
+    {
+        unsigned const ob = static_cast<unsigned>(std::min<uint64_t>(overflow_blocks(h, nq), max_blocks));
+        unsigned const blocks =
+            static_cast<unsigned>(std::min<uint64_t>(static_cast<uint64_t>(device_cus()) * 8u, (nq + 3u) / 4u));
+        HIBF_TRY_HIP(launch_hibf_wave_membership(L, d_values, d_offsets, nq, d_thresholds, std::max(blocks, 1u), d_ids,
+                                                 d_ids ? ids_cap : 0, d_hit_start, d_hit_count, overflow, s),
+                     "membership launch");
+        HIBF_TRY_HIP(launch_hibf_membership(L, d_values, d_offsets, nq, d_thresholds, hit_lists, ob, d_ids,
+                                            d_ids ? ids_cap : 0, d_hit_start, d_hit_count, overflow, s),
+                     "membership overflow launch");
+        return HIBF_GPU_OK;
+    }
+    unsigned const blocks = static_cast<unsigned>(
+        std::min<uint64_t>(hibf_blocks(nq, per_block, g_count_tuning.hibf_block_waves), max_blocks));
+    HIBF_TRY_HIP(launch_hibf_membership(L, d_values, d_offsets, nq, d_thresholds, hit_lists, blocks, d_ids,
+                                        d_ids ? ids_cap : 0, d_hit_start, d_hit_count, nullptr, s),
+                 "membership launch");
+    return HIBF_GPU_OK;
 }
 
 int hibf_gpu_hibf_membership_for(hibf_gpu_hibf h,

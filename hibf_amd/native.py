@@ -117,6 +117,7 @@ _SIGNATURES = {
     "hibf_gpu_agent_status": (_INT, [_V, ctypes.POINTER(_U64), ctypes.POINTER(_U64)]),
     "hibf_gpu_hibf_membership_prepare": (_INT, [_V, _V, _U64]),
     "hibf_gpu_hibf_membership_launch": (_INT, [_V, _V, _V, _V, _U64, _V, _V, _U64, _V, _V, _V]),
+    "hibf_gpu_hibf_membership_launch_noalloc": (_INT, [_V, _V, _V, _U64, _V, _V, _U64, _V, _V, _V]),
     "hibf_gpu_ibf_membership_for": (_INT, [_V, _V, _V, _V, _U64, _V, _V, _U64, _V, ctypes.POINTER(_U64), _V]),
     "hibf_gpu_hibf_create": (_INT, [_U64, ctypes.POINTER(Geometry), _V, _V, _V, _V, _V, _U64, ctypes.POINTER(_V)]),
     "hibf_gpu_hibf_destroy": (_INT, [_V]),

@@ -321,6 +321,31 @@ int hibf_gpu_hibf_membership_launch(hibf_gpu_hibf hibf,
                                     uint64_t * d_hit_count,
                                     void * stream);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under hipGraph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): hibf_gpu_hibf_membership_launch without an agent, whose
+ * workspace is a 64 MiB array in the library's code object, one per GPU,
+ * held until the process ends.  Same inputs and outputs (asynchronous on
+ * `stream`, ids in DFS order at d_ids[d_hit_start[q] ..
+ * + d_hit_count[q]), ids beyond ids_cap dropped: a query with
+ * d_hit_start[q] + d_hit_count[q] > ids_cap lost ids).  At most 65536
+ * queries per call (else HIBF_GPU_ERR_INVALID_ARGUMENT); an HIBF whose
+ * per-query hit list (4 bytes per run-end technical bin reachable from IBF 0)
+ * exceeds the workspace gets HIBF_GPU_ERR_UNSUPPORTED.  Allocates nothing:
+ * no hipMalloc, hipFree or allocating library call.  Every call on one GPU
+ * shares the workspace, so calls that overlap on a GPU race: serialise them
+ * (one stream, or events between streams). */
+int hibf_gpu_hibf_membership_launch_noalloc(hibf_gpu_hibf hibf,
+                                            uint64_t const * d_values,
+                                            uint64_t const * d_offsets,
+                                            uint64_t nq,
+                                            uint16_t const * d_thresholds,
+                                            uint64_t * d_ids,
+                                            uint64_t ids_cap,
+                                            uint64_t * d_hit_start,
+                                            uint64_t * d_hit_count,
+                                            void * stream);
+
 /* Serving loop (no counterpart in the reference, whose agents answer one
  * query per call).  Requests q = 0..nq-1 arrive at h_arrival_s[q] seconds
  * (non-decreasing, relative to the call's start) with values
