@@ -1,16 +1,20 @@
 #!/usr/bin/env python3
 """Benchmark of the HIBF query path on one MI355X.
 
-Workload (flat IBF): 8,192 technical bins x 2^23 rows (8 GiB bit matrix),
-3 hash functions, built by the GPU bulk emplace of 2^17 generated uint64
-values per bin; 1,000,000 queries x 250 values (70 % drawn from one bin's
-inserted set, 30 % uniform); uint16 bulk_count.  A step is one batched
-bulk_count launch over all queries, inputs resident in HBM.
+Default workload (the batched form on the 32,768-user-bin HIBF): user bins
+of log-normal sizes (median 1e5, sigma 1.5, clipped to [1e3, 5e7]), tmax 256,
+2 hash functions, FPR 0.05 / relaxed 0.3, on the reference's default layout
+(HyperLogLog estimates, union estimates, rearrangement), built on the device;
+2,000,000 queries of log-normal length (median 300, clipped to [50, 5000]),
+60 % of their values from one user bin, threshold uint16_t(0.7 L), answered by
+membership_for in hipGraph-captured batches of 65,536 queries.  A step is one
+pass over all 2M queries, inputs resident in HBM.
 
 Prints one JSON line: lookups/s (a lookup = one query value checked against
-the IBF), the HBM roofline of the count kernel measured with HIP events on the
-launch stream, the plain one-launch-per-query form, and a CPU baseline (the
-oracle restatement on a bounded sample, on this host's cores).
+one visited IBF), the HBM roofline of the descent kernel (HIP events on the
+launch stream), the plain one-launch-per-query form, the oracle on this
+host's cores as the CPU baseline, the serving loop's p50/p99 at 50k and 500k
+queries/s, and the flat 8,192-bin IBF bulk_count as an extra leg.
 """
 from __future__ import annotations
 
@@ -25,6 +29,9 @@ import time
 import numpy as np
 
 HBM_PEAK_GBS = 8000.0
+# Random 32-128-byte row reads from a 16 GiB buffer: 4.86e10 rows/s, one 128-byte line each
+# (profiles/random_row_gather_ceiling.md).
+RANDOM_LINE_CEILING = 4.86e10
 
 
 def parse():
@@ -41,9 +48,16 @@ def parse():
     ap.add_argument("--genomes", type=int, default=4096,
                     help="serve_reads: random genomes (user bins) of --genome-length bases")
     ap.add_argument("--genome-length", type=int, default=20000)
-    ap.add_argument("--workload", choices=["flat", "hibf", "serve", "serve_reads"], default="flat",
-                    help="flat: 8,192-bin flat IBF bulk_count (default); hibf: 32,768-user-bin HIBF membership "
-                         "(fixed-length batch, log-normal batched hipGraph form, plain per-query form)")
+    ap.add_argument("--workload", choices=["hibf", "flat", "serve", "serve_reads"], default="hibf",
+                    help="hibf (default): 32,768-user-bin HIBF membership (log-normal batched hipGraph form as the "
+                         "value, fixed-length batch, plain per-query form, serving p50/p99, flat IBF leg); "
+                         "flat: 8,192-bin flat IBF bulk_count alone")
+    ap.add_argument("--no-flat-leg", action="store_true", help="hibf: skip the flat IBF leg")
+    ap.add_argument("--no-serve-leg", action="store_true", help="hibf: skip the serving leg")
+    ap.add_argument("--leg-serve-rates", default="50000,500000",
+                    help="hibf: offered rates (queries/s) of the serving leg, window --leg-serve-window")
+    ap.add_argument("--leg-serve-window", type=float, default=200.0, help="hibf: serving leg batching window (us)")
+    ap.add_argument("--leg-serve-seconds", type=float, default=1.0)
     ap.add_argument("--hibf-queries", type=int, default=500_000)
     ap.add_argument("--hibf-batched-queries", type=int, default=2_000_000)
     ap.add_argument("--reads", type=int, default=200_000,
@@ -154,6 +168,12 @@ def main():
 
 
 def run_flat(args):
+    print(json.dumps(flat_measure(args)), flush=True)
+
+
+def flat_measure(args, leg: bool = False):
+    """The flat IBF workload; returns its JSON record.  leg=True: as an extra
+    leg of the HIBF run (library default shapes, shorter CPU sample)."""
     import torch
     from hibf_amd.device import Agent, DeviceIbf, as_device_u64, to_numpy
     from hibf_amd.synth import flat_ibf_queries
@@ -184,7 +204,7 @@ def run_flat(args):
     stream = torch.cuda.current_stream()
 
     tuned = None
-    if not args.no_autotune and not args.tune:
+    if not args.no_autotune and not args.tune and not leg:
         from hibf_amd import autotune
         nt = min(args.autotune_queries, nq)
         rec = autotune.tune_flat_count(ibf, values[: int(offsets_h[nt])], offsets[: nt + 1], 2, reps=2)
@@ -295,12 +315,27 @@ def run_flat(args):
     # Parity on a bounded sample (oracle on the downloaded matrix) and the CPU baseline.
     cpu = None
     parity = None
+    emplace_ok = None
     if not args.skip_cpu or args.check_queries > 0:
         import oracle as O
+        from hibf_amd.synth import synth_values
         words = ibf.download()
+        # The GPU emplace against the oracle's own emplace of the same generated values, on 64 sampled bins
+        # (bins are independent columns; bin 0 and the last bin included): their bit columns must be equal
+        # before the downloaded matrix serves as the oracle's filter below.
+        rng = np.random.default_rng(args.seed + 7)
+        col_bins = np.unique(np.concatenate([[0, bins - 1], rng.choice(bins, size=62, replace=False)]))
+        own = O.Ibf(bins, bin_size, h)
+        for b in col_bins:
+            own.emplace(synth_values(emplace_seed, int(b), np.arange(args.per_bin, dtype=np.uint64)), int(b))
+        bw = (bins + 63) // 64
+        wd, od = words.reshape(bin_size, bw), own.words.reshape(bin_size, bw)
+        emplace_ok = all(np.array_equal((wd[:, b // 64] >> np.uint64(b % 64)) & np.uint64(1),
+                                        (od[:, b // 64] >> np.uint64(b % 64)) & np.uint64(1)) for b in col_bins)
+        del own, od
+        log(f"emplace columns of {col_bins.size} bins equal to the oracle's emplace: {emplace_ok}")
         ref = O.Ibf.from_words(bins, bin_size, h, words)
         threads = max(1, min(16, os.cpu_count() or 1))
-        rng = np.random.default_rng(args.seed + 7)
         sample = np.sort(rng.choice(nq, size=min(args.check_queries, nq), replace=False))
         step()
         torch.cuda.synchronize()
@@ -309,7 +344,7 @@ def run_flat(args):
         sv = np.concatenate([values_h[offsets_h[q]:offsets_h[q + 1]] for q in sample])
         so = np.arange(len(sample) + 1, dtype=np.uint64) * np.uint64(L)
         want = ref.bulk_count_batch(sv, so, 2, threads)
-        parity = bool(np.array_equal(got, want))
+        parity = bool(np.array_equal(got, want)) and emplace_ok
         if contains is not None:
             pick = rng.choice(nc, size=min(256, nc), replace=False)
             got_c = cout.index_select(0, torch.as_tensor(pick.astype(np.int64), device=dev)).cpu().numpy()
@@ -333,8 +368,9 @@ def run_flat(args):
         if not args.skip_cpu:
             def run_cpu(n, t):
                 ref.bulk_count_batch(values_h[: n * L], offsets_h[: n + 1], 2, t)
-            n_cpu, cpu_s = timed_sample(lambda n: run_cpu(n, threads), 64, nq, args.cpu_seconds)
-            n_one, one_s = timed_sample(lambda n: run_cpu(n, 1), 16, nq, args.cpu_seconds / 4)
+            cpu_budget = args.cpu_seconds / (4 if leg else 1)
+            n_cpu, cpu_s = timed_sample(lambda n: run_cpu(n, threads), 64, nq, cpu_budget)
+            n_one, one_s = timed_sample(lambda n: run_cpu(n, 1), 16, nq, cpu_budget / 4)
             cpu = {"value": n_cpu * L / cpu_s, "unit": "lookups/s", "cores": threads, "kind": "port",
                    "sample": f"first {n_cpu} queries x {L} values of the same workload, oracle bulk_count "
                              f"(uint16) with {threads} OpenMP threads, one agent per thread, {cpu_s:.1f} s",
@@ -379,9 +415,12 @@ def run_flat(args):
         "bulk_contains": contains,
         "membership_for": member,
         "parity_sample_ok": parity,
+        "emplace_columns_ok": emplace_ok,
         "build_emplace_s": build_s,
     }
-    print(json.dumps(out), flush=True)
+    del ibf, values, offsets, counts
+    torch.cuda.empty_cache()
+    return out
 
 
 def hibf_sketch_leg(sizes, spec, args):
@@ -437,9 +476,12 @@ def hibf_sketch_leg(sizes, spec, args):
 
 
 def run_hibf(args):
-    """32,768-user-bin HIBF: (a) 500k x 250 queries, t = 0.7 L, one persistent
-    launch; (b) 2M queries of log-normal length in hipGraph-captured batches of
-    64k; (c) the plain one-launch-per-query form on a sample of (b)."""
+    """32,768-user-bin HIBF: (b) 2M queries of log-normal length in
+    hipGraph-captured batches of 64k (the reported value); (a) 500k x 250
+    queries, t = 0.7 L, one persistent launch; (c) the plain
+    one-launch-per-query form on a sample of (b); (d) bulk_count; (e) k-mer
+    hashing of reads; (f) the build-side sketches; then the serving loop at
+    50k and 500k queries/s and the flat 8,192-bin IBF as extra legs."""
     import torch
     from hibf_amd import native
     from hibf_amd import workloads as W
@@ -497,14 +539,22 @@ def run_hibf(args):
     ta_h = thresholds_of(oa_h)
     va, oa, ta = as_device_u64(va_h), as_device_u64(oa_h), upload_u16(ta_h)
 
+    # (b)'s queries: log-normal lengths, answered in hipGraph batches below.
+    nb = args.hibf_batched_queries
+    lengths_b = W.query_lengths(args.seed + 5, nb)
+    vb_h, ob_h, _ = W.hibf_queries(args.seed + 6, sizes, spec["build_seed"], lengths_b, 0.6)
+    tb_h = thresholds_of(ob_h)
+    vb, ob, tb = as_device_u64(vb_h), as_device_u64(ob_h), upload_u16(tb_h)
+
+    # Launch shapes tuned on the first queries of (b), the workload of the reported value.
     tuned = None
     if not args.no_autotune and not args.tune:
         from hibf_amd import autotune
-        nt = min(args.autotune_queries, nq)
-        rec = autotune.tune_hibf_membership(hibf, va[: int(oa_h[nt])], oa[: nt + 1], ta[:nt], reps=2)
+        nt = min(args.autotune_queries, nb)
+        rec = autotune.tune_hibf_membership(hibf, vb[: int(ob_h[nt])], ob[: nt + 1], tb[:nt], reps=2)
         tuned = {"best": rec["best"], "best_ms": rec["best_ms"], "default_ms": rec["default_ms"],
-                 "queries": nt, "candidates": len(rec["candidates"])}
-        log(f"autotune ({len(rec['candidates'])} shapes on {nt} queries): {rec['best']} "
+                 "queries": nt, "workload": "first queries of the log-normal batch", "candidates": len(rec["candidates"])}
+        log(f"autotune ({len(rec['candidates'])} shapes on {nt} queries of (b)): {rec['best']} "
             f"{rec['best_ms']:.3f} ms (default {rec['default_ms']:.3f} ms)")
         if args.autotune_save:
             rec["command"] = "python bench.py " + " ".join(sys.argv[1:])
@@ -536,12 +586,6 @@ def run_hibf(args):
         f"{bytes_a / kern_a / 1e9:.0f} GB/s rows, {st['visits'] / args.steps / nq:.2f} IBFs/query, "
         f"{total / nq:.3f} hits/query")
 
-    # (b) log-normal lengths, hipGraph batches.
-    nb = args.hibf_batched_queries
-    lengths_b = W.query_lengths(args.seed + 5, nb)
-    vb_h, ob_h, _ = W.hibf_queries(args.seed + 6, sizes, spec["build_seed"], lengths_b, 0.6)
-    tb_h = thresholds_of(ob_h)
-    vb, ob, tb = as_device_u64(vb_h), as_device_u64(ob_h), upload_u16(tb_h)
     G = args.graph_batch
     nbatches = (nb + G - 1) // G
     cap_b = 8 * G
@@ -566,17 +610,26 @@ def run_hibf(args):
     _, dropped = agent.status()
     assert not dropped, "batch output capacity too small"
     agent.stats(reset=True)
+    # Timed region: K steps of all batches; HIP events on the launch stream bracket every graph replay (a
+    # replay is the counter-reset kernel and the descent kernel).
+    evs_b = [[(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in graphs]
+             for _ in range(args.steps)]
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        for g in graphs:
+    for step_evs in evs_b:
+        for g, (e0, e1) in zip(graphs, step_evs):
+            e0.record(stream)
             g.replay()
+            e1.record(stream)
     torch.cuda.synchronize()
     wall_b = (time.perf_counter() - t0) / args.steps
+    kern_b = sum(e0.elapsed_time(e1) for step_evs in evs_b for (e0, e1) in step_evs) / 1e3 / args.steps
     st = agent.stats(reset=True)
     lookups_b = st["lookups"] / args.steps
     bytes_b = st["row_bytes"] / args.steps
     log(f"(b) {nb} log-normal queries ({int(ob_h[-1]) / nb:.0f} values/query), {nbatches} graphs: "
-        f"{wall_b * 1e3:.1f} ms/step, {lookups_b / wall_b / 1e9:.3f} G lookups/s, {bytes_b / wall_b / 1e9:.0f} GB/s rows")
+        f"{wall_b * 1e3:.1f} ms/step (kernels {kern_b * 1e3:.1f} ms), {lookups_b / wall_b / 1e9:.3f} G lookups/s, "
+        f"{bytes_b / kern_b / 1e9:.0f} GB/s rows")
 
     # (d) counting agent: uint16 bulk_count over every user bin for a prefix of (a), t = 0.7 L as in (a).
     count_leg = None
@@ -717,8 +770,21 @@ def run_hibf(args):
                 f"{cpu['one_thread']['value'] / 1e6:.2f} M lookups/s on 1 thread")
         del arena, ref
 
+    # Serving leg (the serving loop on the same resident HIBF): p50/p99 from arrival to ids on the host.
+    serving = None
+    if not args.no_serve_leg:
+        rates = [float(r) for r in args.leg_serve_rates.split(",")]
+        rows = serve_sweep(hibf, tab, sizes, spec, args, rates, [args.leg_serve_window], args.leg_serve_seconds)
+        serving = {"window_us": args.leg_serve_window, "seconds_of_arrivals": args.leg_serve_seconds,
+                   "streams": 4, "budget_bytes": 4 << 30, "rows": rows}
+
+    kernel_name = descent_kernel_name(native)
+    per_launch_bytes = bytes_b / nbatches
+    launch_s = kern_b / nbatches
+    achieved = per_launch_bytes / launch_s / 1e9
+    traffic = recorded_traffic("hibf_batched", args, layout)
     out = {
-        "metric": "k-mer lookups/s (HIBF membership_for, batched)",
+        "metric": "k-mer lookups/s (HIBF membership_for, batched hipGraph form)",
         "value": lookups_b / wall_b,
         "unit": "lookups/s",
         "n_gpus": 1,
@@ -730,14 +796,24 @@ def run_hibf(args):
         "vs_baseline": None,
         "dtype": "u64 hash / u16 counters",
         "data": "synthetic (seeded generator values, uniform uint64 noise)",
-        "config": {"workload": f"HIBF {n_ub} user bins (log-normal sizes, median 1e5, sigma 1.5), tmax 256, h=2, "
-                               f"FPR 0.05/0.3; {nb} queries, log-normal length median 300 [50, 5000], 60% from one "
-                               f"user bin, t = 0.7 L; hipGraph batches of {G}",
+        "config": {"workload": f"batched form on the HIBF at scale: {n_ub} user bins (log-normal sizes, median 1e5, "
+                               f"sigma 1.5, clipped [1e3, 5e7]), tmax 256, h=2, FPR 0.05/0.3, reference-default "
+                               f"layout; {nb} queries, log-normal length median 300 [50, 5000], 60% from one user "
+                               f"bin, t = uint16(0.7 L); persistent work-stealing descent in hipGraph batches of {G}",
+                   "queries": nb, "values_per_step": int(ob_h[-1]), "graph_batch": G, "graphs": nbatches,
                    "ibfs": len(tab["geoms"]), "arena_gb": tab["n_words"] * 8 / 1e9, "depth": tab["depth"],
-                   "tuning": {k: native.tune_get(k) for k in ("hibf.wave_kernel", "hibf.nt_loads")},
+                   "tuning": {k: native.tune_get(k) for k in ("hibf.wave_kernel", "hibf.row_per_lane",
+                                                               "hibf.sliced_counts", "hibf.nt_loads",
+                                                               "hibf.block_waves")},
                    "autotune": tuned},
-        "roofline": {"bound": "hbm", "achieved": bytes_b / wall_b / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": bytes_b / wall_b / 1e9 / HBM_PEAK_GBS, "traffic": None},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": kernel_name,
+                     "kernel_ms_per_launch": launch_s * 1e3, "launches_per_step": nbatches,
+                     "algorithmic_bytes_per_launch": per_launch_bytes,
+                     "bytes_per_lookup": bytes_b / max(lookups_b, 1),
+                     "random_line_ceiling": {
+                         "lines_per_s": RANDOM_LINE_CEILING, "source": "profiles/random_row_gather_ceiling.md",
+                         "frac": None if traffic is None else traffic / 128.0 / launch_s / RANDOM_LINE_CEILING}},
         "fixed_batch": {"queries": nq, "length": 250, "ms": wall_a * 1e3, "lookups_per_s": lookups_a / wall_a,
                         "row_gbs": bytes_a / kern_a / 1e9, "kernel_ms": kern_a * 1e3,
                         "traffic": recorded_traffic("hibf_fixed_batch", args, layout)},
@@ -748,6 +824,7 @@ def run_hibf(args):
                             "latency_ms_p99": float(np.percentile(lat, 99))},
         "batched_vs_plain": lookups_b / wall_b / plain,
         "cpu_baseline": cpu,
+        "serving": serving,
         "parity_sample_ok": parity,
         "build_s": build_s,
         "layout_s": layout_s,
@@ -755,7 +832,97 @@ def run_hibf(args):
                    "computed": layout_times or None},
         "sketches": sketch_leg,
     }
+    if not args.no_flat_leg:
+        del graphs, ids_b, hs_b, hc_b, vb, ob, tb, va, oa, ta, ids, hs, hc
+        torch.cuda.empty_cache()
+        f = flat_measure(args, leg=True)
+        out["flat_ibf"] = {k: f[k] for k in ("value", "unit", "ms_per_step", "config", "roofline", "cpu_baseline",
+                                              "plain_per_query", "batched_vs_plain", "bulk_contains",
+                                              "membership_for", "parity_sample_ok", "emplace_columns_ok",
+                                              "build_emplace_s")}
     print(json.dumps(out), flush=True)
+
+
+def descent_kernel_name(native) -> str:
+    """The membership descent kernel the current launch-shape knobs select for
+    an HIBF whose IBFs all use two hash functions and rows of at most 4 words."""
+    if native.tune_get("hibf.wave_kernel"):
+        return "hibf_wave_membership_h2_w4_kernel (+ hibf_descent_* for queries beyond its LDS id buffer)"
+    if native.tune_get("hibf.nt_loads"):
+        return "hibf_descent_kernel<false, true, 0>"
+    if native.tune_get("hibf.row_per_lane"):
+        return "hibf_descent_membership_h2_row_per_lane_kernel"
+    return f"hibf_descent_membership_h2_w4_kernel<{'true' if native.tune_get('hibf.sliced_counts') else 'false'}>"
+
+
+def serve_sweep(hibf, tab, sizes, spec, args, rates, windows, seconds):
+    """Serving loop on a resident HIBF for each offered rate and window:
+    Poisson arrivals, log-normal lengths (median 300, at most 5,000), 60 % of
+    the values from one user bin, t = 0.7 L, 4 GiB admission budget, 4
+    streams; results of sampled requests checked against the oracle.
+    Returns the rows of the sweep."""
+    import torch
+    from hibf_amd import workloads as W
+    n_ub = int(sizes.size)
+    rows = []
+    ref = None
+    for rate in rates:
+        nq = int(rate * seconds)
+        lengths = W.query_lengths(args.seed + 11, nq)
+        # The values, their pinned copy and the generator's temporaries: about 40 bytes per value at the peak.
+        need = int(lengths.sum()) * 40
+        if need > args.serve_host_gb * 1e9:
+            log(f"serve rate {rate:.0f} q/s skipped: its {int(lengths.sum()):.3e} query values would need about "
+                f"{need / 1e9:.0f} GB of host memory (--serve-host-gb {args.serve_host_gb:.0f}; lower --serve-seconds)")
+            continue
+        v_h, o_h, _ = W.hibf_queries(args.seed + 12, sizes, spec["build_seed"], lengths, 0.6)
+        t_h = np.minimum((0.7 * np.diff(o_h.astype(np.int64))).astype(np.int64), 65535).astype(np.uint16)
+        rng = np.random.default_rng(args.seed + 13)
+        arrival = np.cumsum(rng.exponential(1.0 / rate, size=nq))
+        arrival -= arrival[0]
+        v_in = v_h
+        if not args.serve_pageable:
+            # Requests land in a pinned receive buffer, as a server's network path would place them.
+            v_in = torch.empty(v_h.size, dtype=torch.int64, pin_memory=True)
+            v_in.numpy()[:] = v_h.view(np.int64)
+        lat = ids = cnt = None
+        rate_rows = []
+        for win in windows:
+            lat, ids, cnt, st = hibf.serve(v_in, o_h, t_h, arrival, window_us=win, streams=4,
+                                           budget_bytes=4 << 30, results_per_query=16)
+            row = {"rate_qps": rate, "window_us": win, "queries": nq,
+                   "p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
+                   "sustained_qps": nq / st["seconds"], "sustained_lookups_per_s": st["lookups"] / st["seconds"],
+                   "batches": st["batches"], "mean_batch": nq / max(st["batches"], 1),
+                   "max_in_flight_mb": st["max_in_flight_bytes"] / 2**20, "overflowed": st["overflowed"],
+                   "longest_pass_ms": st["longest_pass_s"] * 1e3, "longest_launch_ms": st["longest_launch_s"] * 1e3}
+            log(f"serve rate {rate:.0f} q/s window {win:.0f} us: p50 {row['p50_ms']:.3f} ms p99 {row['p99_ms']:.3f} ms, "
+                f"{row['sustained_qps']:.0f} q/s, {row['sustained_lookups_per_s'] / 1e9:.2f} G lookups/s, "
+                f"{st['batches']} batches (mean {row['mean_batch']:.1f}); longest loop pass "
+                f"{row['longest_pass_ms']:.2f} ms, longest batch launch {row['longest_launch_ms']:.2f} ms")
+            rate_rows.append(row)
+        if args.check_queries > 0:
+            import oracle as O
+            if ref is None:
+                arena = hibf.download_arena()
+                geoms = [O.geom(b_, s_, h_) for (b_, s_, h_) in tab["geoms"]]
+                ref = O.Hibf(geoms, arena, tab["word_offsets"], tab["tb_offsets"], tab["next_ibf"],
+                             tab["user_bin"], n_ub)
+            ok = True
+            for q in rng.choice(nq, size=min(args.check_queries, nq), replace=False):
+                q = int(q)
+                want = ref.membership_for(v_h[int(o_h[q]):int(o_h[q + 1])], int(t_h[q]))
+                got = ids[q, :min(int(cnt[q]), 16)].tolist()
+                if not (int(cnt[q]) == len(want) and got == want[:16]):
+                    ok = False
+                    log(f"serving result differs at query {q}")
+                    break
+            for row in rate_rows:  # the last window's results were checked
+                row["parity_sample_ok"] = ok if row is rate_rows[-1] else None
+            log(f"serving parity vs oracle on {min(args.check_queries, nq)} sampled queries: {ok}")
+        rows.extend(rate_rows)
+        del v_h, v_in, o_h, t_h, arrival, lat, ids, cnt
+    return rows
 
 
 def run_serve(args):
@@ -781,56 +948,7 @@ def run_serve(args):
     tab = hibf.tables()
     rates = [float(r) for r in args.serve_rates.split(",")]
     windows = [float(w) for w in args.serve_windows.split(",")]
-    rows = []
-    ref = None
-    for rate in rates:
-        nq = int(rate * args.serve_seconds)
-        lengths = W.query_lengths(args.seed + 11, nq)
-        # The values, their pinned copy and the generator's temporaries: about 40 bytes per value at the peak.
-        need = int(lengths.sum()) * 40
-        if need > args.serve_host_gb * 1e9:
-            log(f"serve rate {rate:.0f} q/s skipped: its {int(lengths.sum()):.3e} query values would need about "
-                f"{need / 1e9:.0f} GB of host memory (--serve-host-gb {args.serve_host_gb:.0f}; lower --serve-seconds)")
-            continue
-        v_h, o_h, _ = W.hibf_queries(args.seed + 12, sizes, spec["build_seed"], lengths, 0.6)
-        t_h = np.minimum((0.7 * np.diff(o_h.astype(np.int64))).astype(np.int64), 65535).astype(np.uint16)
-        rng = np.random.default_rng(args.seed + 13)
-        arrival = np.cumsum(rng.exponential(1.0 / rate, size=nq))
-        arrival -= arrival[0]
-        v_in = v_h
-        if not args.serve_pageable:
-            # Requests land in a pinned receive buffer, as a server's network path would place them.
-            v_in = torch.empty(v_h.size, dtype=torch.int64, pin_memory=True)
-            v_in.numpy()[:] = v_h.view(np.int64)
-        lat = ids = cnt = None
-        for win in windows:
-            lat, ids, cnt, st = hibf.serve(v_in, o_h, t_h, arrival, window_us=win, streams=4,
-                                           budget_bytes=4 << 30, results_per_query=16)
-            row = {"rate_qps": rate, "window_us": win, "queries": nq,
-                   "p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
-                   "sustained_qps": nq / st["seconds"], "sustained_lookups_per_s": st["lookups"] / st["seconds"],
-                   "batches": st["batches"], "mean_batch": nq / max(st["batches"], 1),
-                   "max_in_flight_mb": st["max_in_flight_bytes"] / 2**20, "overflowed": st["overflowed"],
-                   "longest_pass_ms": st["longest_pass_s"] * 1e3, "longest_launch_ms": st["longest_launch_s"] * 1e3}
-            log(f"serve rate {rate:.0f} q/s window {win:.0f} us: p50 {row['p50_ms']:.3f} ms p99 {row['p99_ms']:.3f} ms, "
-                f"{row['sustained_qps']:.0f} q/s, {row['sustained_lookups_per_s'] / 1e9:.2f} G lookups/s, "
-                f"{st['batches']} batches (mean {row['mean_batch']:.1f}); longest loop pass "
-                f"{row['longest_pass_ms']:.2f} ms, longest batch launch {row['longest_launch_ms']:.2f} ms")
-            rows.append(row)
-        if args.check_queries > 0:
-            import oracle as O
-            if ref is None:
-                arena = hibf.download_arena()
-                geoms = [O.geom(b_, s_, h_) for (b_, s_, h_) in tab["geoms"]]
-                ref = O.Hibf(geoms, arena, tab["word_offsets"], tab["tb_offsets"], tab["next_ibf"],
-                             tab["user_bin"], n_ub)
-            for q in rng.choice(nq, size=min(args.check_queries, nq), replace=False):
-                q = int(q)
-                want = ref.membership_for(v_h[int(o_h[q]):int(o_h[q + 1])], int(t_h[q]))
-                got = ids[q, :min(int(cnt[q]), 16)].tolist()
-                assert int(cnt[q]) == len(want) and got == want[:16], f"serving result differs at query {q}"
-            log(f"serving parity vs oracle on {min(args.check_queries, nq)} sampled queries: True")
-        del v_h, v_in, o_h, t_h, arrival, lat, ids, cnt
+    rows = serve_sweep(hibf, tab, sizes, spec, args, rates, windows, args.serve_seconds)
     best = max(rows, key=lambda r: r["sustained_lookups_per_s"])
     out = {"metric": "p99 query latency (HIBF serving loop)", "value": best["p99_ms"], "unit": "ms",
            "n_gpus": 1, "higher_is_better": False, "data": "synthetic", "dtype": "u64 hash / u16 counters",
