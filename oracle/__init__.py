@@ -121,6 +121,14 @@ def lib():
         L.orc_hll_union_estimates.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint64,
                                               ctypes.c_void_p, ctypes.c_void_p]
         L.orc_mt19937_64.argtypes = [ctypes.c_uint64, ctypes.c_void_p, ctypes.c_uint64]
+        L.orc_synth_value.restype = ctypes.c_uint64
+        L.orc_synth_value.argtypes = [ctypes.c_uint64] * 3
+        L.orc_ibf_emplace_synthetic.argtypes = [ctypes.c_void_p, ctypes.POINTER(OrcIbf)] + [ctypes.c_uint64] * 4 + \
+            [ctypes.c_int]
+        L.orc_ibf_emplace_synthetic_items.argtypes = [ctypes.c_void_p, ctypes.POINTER(OrcIbf), ctypes.c_uint64] + \
+            [ctypes.c_void_p] * 4 + [ctypes.c_uint64, ctypes.c_int]
+        L.orc_ibf_emplace_parallel.argtypes = [ctypes.c_void_p, ctypes.POINTER(OrcIbf), ctypes.c_void_p,
+                                               ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int]
         _lib = L
     return _lib
 
@@ -176,6 +184,23 @@ class Ibf:
         v = _u64(values)
         b = _u64(np.broadcast_to(np.asarray(bins, dtype=np.uint64), v.shape))
         lib().orc_ibf_emplace(_p(self.words), ctypes.byref(self.g), _p(v), _p(b), v.size)
+
+    def emplace_parallel(self, values, bins, threads: int = 16) -> None:
+        """emplace with `threads` threads (atomic OR; same result)."""
+        v = _u64(values)
+        b = _u64(np.broadcast_to(np.asarray(bins, dtype=np.uint64), v.shape))
+        lib().orc_ibf_emplace_parallel(_p(self.words), ctypes.byref(self.g), _p(v), _p(b), v.size, threads)
+
+    def emplace_synthetic(self, seed: int, first_bin: int, n_bins: int, per_bin: int, threads: int = 16) -> None:
+        """Bin first_bin + b receives synth_value(seed, first_bin + b, j), j < per_bin."""
+        lib().orc_ibf_emplace_synthetic(_p(self.words), ctypes.byref(self.g), seed, first_bin, n_bins, per_bin,
+                                        threads)
+
+    def emplace_synthetic_items(self, seed: int, sets, begins, counts, bins, threads: int = 16) -> None:
+        """Item k: synth_value(seed, sets[k], begins[k] + j), j < counts[k], into bin bins[k]."""
+        a = [_u64(x) for x in (sets, begins, counts, bins)]
+        lib().orc_ibf_emplace_synthetic_items(_p(self.words), ctypes.byref(self.g), seed, *[_p(x) for x in a],
+                                              a[0].size, threads)
 
     def clear(self, bin: int) -> None:
         lib().orc_ibf_clear(_p(self.words), ctypes.byref(self.g), bin)
@@ -290,6 +315,11 @@ class Hibf:
         hits = np.zeros(nq, dtype=np.uint64)
         lib().orc_hibf_membership_count_batch(ctypes.byref(self.s), _p(v), _p(o), nq, _p(t), _p(hits), threads)
         return hits
+
+
+def synth_value(seed: int, bin: int, i: int) -> int:
+    """Test-input generator (the synthetic stream of the device builds)."""
+    return int(lib().orc_synth_value(seed, bin, i))
 
 
 def geom(bins: int, bin_size: int, hash_funs: int) -> OrcIbf:

@@ -27,6 +27,11 @@
  *   orc_hll_*              hyperloglog_test.cpp expectations and exact counts
  *                          (see the HyperLogLog section).
  *   orc_mt19937_64         the C++ standard's 10,000th output for seed 5489.
+ *   orc_synth_value        test-input generator, not reference code: its
+ *                          Python twin (hibf_amd/synth.py) and the device
+ *                          stream must agree (tests/test_oracle.py).
+ *   orc_ibf_emplace_*      parallel forms of orc_ibf_emplace: equal to it on
+ *                          the same pairs (tests/test_oracle.py).
  */
 #include <math.h>
 #include <stdint.h>
@@ -122,6 +127,91 @@ void orc_ibf_emplace(uint64_t * words, orc_ibf const * g, uint64_t const * value
             words[idx >> 6] |= 1ULL << (idx & 63u);
         }
     }
+}
+
+/* Test-input generator (not reference code): value i of the seeded synthetic
+ * set `bin`, the splitmix/murmur 64-bit finaliser of
+ * (bin << 36 | i) + seed * 0x9E3779B97F4A7C15.  The tests draw user bins and
+ * technical bins from this stream; the device builds use the same stream. */
+uint64_t orc_synth_value(uint64_t seed, uint64_t bin, uint64_t i)
+{
+    uint64_t k = ((bin << 36) | i) + seed * 0x9E3779B97F4A7C15ULL;
+    k ^= k >> 33;
+    k *= 0xFF51AFD7ED558CCDULL;
+    k ^= k >> 33;
+    k *= 0xC4CEB9FE1A85EC53ULL;
+    k ^= k >> 33;
+    return k;
+}
+
+/* The reference's config constructor fills an IBF with input_fn(i) for every
+ * bin in parallel (src/interleaved_bloom_filter.cpp:104-117); here bin
+ * first_bin + b receives orc_synth_value(seed, first_bin + b, j) for
+ * j < per_bin.  Threads own whole 64-bin words (the reference's rule for
+ * concurrent emplace, interleaved_bloom_filter.hpp:142-144), so the result is
+ * the sequential emplace's. */
+void orc_ibf_emplace_synthetic(uint64_t * words, orc_ibf const * g, uint64_t seed, uint64_t first_bin,
+                               uint64_t n_bins, uint64_t per_bin, int threads)
+{
+    uint64_t const w0 = first_bin / 64u;
+    uint64_t const w1 = n_bins ? (first_bin + n_bins - 1u) / 64u + 1u : w0;
+    (void)threads;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 1) num_threads(threads > 0 ? threads : 1)
+#endif
+    for (int64_t w = (int64_t)w0; w < (int64_t)w1; ++w)
+    {
+        uint64_t const lo = (uint64_t)w * 64u < first_bin ? first_bin : (uint64_t)w * 64u;
+        uint64_t const hi = ((uint64_t)w + 1u) * 64u < first_bin + n_bins ? ((uint64_t)w + 1u) * 64u : first_bin + n_bins;
+        for (uint64_t b = lo; b < hi; ++b)
+            for (uint64_t j = 0; j < per_bin; ++j)
+            {
+                uint64_t const v = orc_synth_value(seed, b, j);
+                orc_ibf_emplace(words, g, &v, &b, 1u);
+            }
+    }
+}
+
+/* Emplace of generated ranges: item k puts orc_synth_value(seed, set[k],
+ * begin[k] + j), j < count[k], into bin bins[k].  Items run on `threads`
+ * threads with atomic OR on the words, so the result is the sequential
+ * emplace's whatever the order. */
+void orc_ibf_emplace_synthetic_items(uint64_t * words, orc_ibf const * g, uint64_t seed, uint64_t const * set,
+                                     uint64_t const * begin, uint64_t const * count, uint64_t const * bins,
+                                     uint64_t n_items, int threads)
+{
+    (void)threads;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 1) num_threads(threads > 0 ? threads : 1)
+#endif
+    for (int64_t k = 0; k < (int64_t)n_items; ++k)
+        for (uint64_t j = 0; j < count[k]; ++j)
+        {
+            uint64_t const v = orc_synth_value(seed, set[k], begin[k] + j);
+            for (uint64_t i = 0; i < g->hash_funs; ++i)
+            {
+                uint64_t const idx =
+                    orc_hash_and_fit(v, ORC_HASH_SEEDS[i], g->hash_shift, g->bin_size, g->technical_bins) + bins[k];
+                __atomic_fetch_or(&words[idx >> 6], 1ULL << (idx & 63u), __ATOMIC_RELAXED);
+            }
+        }
+}
+
+/* orc_ibf_emplace over (value, bin) pairs on `threads` threads with atomic OR. */
+void orc_ibf_emplace_parallel(uint64_t * words, orc_ibf const * g, uint64_t const * values, uint64_t const * bins,
+                              uint64_t n, int threads)
+{
+    (void)threads;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static) num_threads(threads > 0 ? threads : 1)
+#endif
+    for (int64_t k = 0; k < (int64_t)n; ++k)
+        for (uint64_t i = 0; i < g->hash_funs; ++i)
+        {
+            uint64_t const idx =
+                orc_hash_and_fit(values[k], ORC_HASH_SEEDS[i], g->hash_shift, g->bin_size, g->technical_bins) + bins[k];
+            __atomic_fetch_or(&words[idx >> 6], 1ULL << (idx & 63u), __ATOMIC_RELAXED);
+        }
 }
 
 /* src/interleaved_bloom_filter.cpp:142-148: clear(bin) zeroes the bin's column. */

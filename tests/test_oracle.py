@@ -224,6 +224,37 @@ def test_emplace_sets_h_bits_in_its_column_only():
                         assert (wi % ibf.g.bin_words) * 64 + bit == b
 
 
+def test_synth_stream_and_parallel_emplace_forms():
+    """The synthetic test-input stream is the same in C and numpy, and the
+    oracle's parallel emplace forms equal its sequential emplace."""
+    from hibf_amd.synth import synth_values
+    rng = np.random.default_rng(5)
+    for _ in range(200):
+        seed, b, i = (int(x) for x in rng.integers(0, 2**20, size=3))
+        assert O.synth_value(seed, b, i) == int(synth_values(seed, b, i))
+    bins, bin_size, h = 200, 3001, 3
+    seq = O.Ibf(bins, bin_size, h)
+    for b in range(5, 190):
+        seq.emplace(synth_values(9, b, np.arange(300, dtype=np.uint64)), b)
+    gen = O.Ibf(bins, bin_size, h)
+    gen.emplace_synthetic(9, 5, 185, 300, threads=4)
+    assert np.array_equal(gen.words, seq.words)
+    items = O.Ibf(bins, bin_size, h)
+    # the same sets as ragged items in shuffled order, split into two ranges each
+    order = rng.permutation(np.arange(5, 190))
+    sets = np.repeat(order, 2)
+    begins = np.tile([0, 120], order.size)
+    counts = np.tile([120, 180], order.size)
+    items.emplace_synthetic_items(9, sets, begins, counts, sets, threads=3)
+    assert np.array_equal(items.words, seq.words)
+    vals = rng.integers(0, 2**63, size=5000, dtype=np.int64).astype(np.uint64)
+    bs = rng.integers(0, bins, size=vals.size).astype(np.uint64)
+    a, b = O.Ibf(bins, bin_size, h), O.Ibf(bins, bin_size, h)
+    a.emplace(vals, bs)
+    b.emplace_parallel(vals, bs, threads=4)
+    assert np.array_equal(a.words, b.words)
+
+
 def test_no_false_negatives():
     rng = np.random.default_rng(5)
     ibf = O.Ibf(100, 4096, 3)
