@@ -243,20 +243,28 @@ def flat_measure(args, leg: bool = False):
     log(f"batched: {ms_per_step:.2f} ms/step, kernel {np.mean(kernel_ms):.2f} ms, {value / 1e9:.3f} G lookups/s, "
         f"{achieved:.0f} GB/s of rows")
 
-    # Plain form: one launch per query (async back to back), then per-query latency (synchronised).
+    # Plain form: one launch per query (async back to back), then per-query latency (synchronised).  The
+    # launches call the C-ABI directly, as a C++ caller would, without the Python wrapper's buffer checks.
     npl = min(args.plain_queries, nq)
     outs = counts[:1]
+    c_count = native.lib().hibf_gpu_ibf_bulk_count
+    p_vals, p_out, p_s = ctypes.c_void_p(values.data_ptr()), ctypes.c_void_p(outs.data_ptr()), \
+        ctypes.c_void_p(stream.cuda_stream)
+    p_off = [ctypes.c_void_p(offsets[q:q + 2].data_ptr()) for q in range(npl)]
+
+    def plain_count(q):
+        native.check(c_count(ibf.h, p_vals, p_off[q], 1, 2, p_out, p_s))
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for q in range(npl):
-        ibf.bulk_count_into(values, offsets[q:q + 2], outs, 2)
+        plain_count(q)
     torch.cuda.synchronize()
     plain_s = time.perf_counter() - t0
     plain_value = npl * L / plain_s if npl else float("nan")
     lat = [0.0]
     for q in range(min(300, npl)):
         t1 = time.perf_counter()
-        ibf.bulk_count_into(values, offsets[q:q + 2], outs, 2)
+        plain_count(q)
         torch.cuda.synchronize()
         lat.append((time.perf_counter() - t1) * 1e3)
     log(f"plain: {plain_value / 1e9:.4f} G lookups/s; latency p50 {np.percentile(lat, 50):.3f} ms "
@@ -684,15 +692,24 @@ def run_hibf(args):
             f"{kmer_leg['hashes_per_s'] / 1e9:.2f} G hashes/s")
         del seq, soff, hoff, hashes_r
 
-    # (c) plain form: one launch per query of (b), back to back.
+    # (c) plain form: one launch per query of (b), back to back.  The launches call the C-ABI directly, as a
+    # C++ caller would, without the Python wrapper's buffer checks.
     npl = min(args.plain_queries, nb)
     hibf.membership_prepare(agent, 1)
     ids1 = torch.empty(1 << 16, dtype=torch.uint64, device=dev)
+    c_launch = native.lib().hibf_gpu_hibf_membership_launch
+    P = lambda t_: ctypes.c_void_p(t_.data_ptr())  # noqa: E731
+    p_plain = [(P(ob[q:q + 2]), P(tb[q:q + 1]), P(hs_b[q:q + 1]), P(hc_b[q:q + 1])) for q in range(npl)]
+    p_vb, p_ids1, p_stream = P(vb), P(ids1), ctypes.c_void_p(stream.cuda_stream)
+
+    def plain_launch(q):
+        o_, t_, s_, c_ = p_plain[q]
+        native.check(c_launch(hibf.h, agent.h, p_vb, o_, 1, t_, p_ids1, ids1.numel(), s_, c_, p_stream))
     torch.cuda.synchronize()
     agent.stats(reset=True)
     t0 = time.perf_counter()
     for q in range(npl):
-        hibf.membership_launch(agent, vb, ob[q:q + 2], tb[q:q + 1], ids1, hs_b[q:q + 1], hc_b[q:q + 1])
+        plain_launch(q)
     torch.cuda.synchronize()
     wall_c = time.perf_counter() - t0
     st = agent.stats(reset=True)
@@ -700,7 +717,7 @@ def run_hibf(args):
     lat = []
     for q in range(min(300, npl)):
         t1 = time.perf_counter()
-        hibf.membership_launch(agent, vb, ob[q:q + 2], tb[q:q + 1], ids1, hs_b[q:q + 1], hc_b[q:q + 1])
+        plain_launch(q)
         torch.cuda.synchronize()
         lat.append((time.perf_counter() - t1) * 1e3)
     log(f"(c) plain per-query: {plain / 1e9:.4f} G lookups/s; latency p50 {np.percentile(lat, 50):.3f} ms "

@@ -9,12 +9,18 @@
 #include <algorithm>
 #include <cmath>
 #include <optional>
+#include <string>
 
 namespace hibf_amd::host
 {
 
 namespace
 {
+
+// Largest technical-bin count of one IBF a layout may ask for (far above any
+// layout's; the device descent handles at most 8,192), so that a malformed
+// layout cannot size tables of arbitrary length.
+constexpr uint64_t kMaxLayoutBins = 1ULL << 24;
 
 struct Node
 {
@@ -58,9 +64,15 @@ std::string make_graph(Layout const & layout, Node & root)
     }
     for (auto const & rec : layout.user_bins)
     {
+        if (rec.number_of_technical_bins == 0)
+            return "layout: user bin " + std::to_string(rec.idx) + " occupies no technical bin";
+        if (rec.storage_TB_id >= kMaxLayoutBins || rec.number_of_technical_bins > kMaxLayoutBins - rec.storage_TB_id)
+            return "layout: technical bin index beyond " + std::to_string(kMaxLayoutBins);
         Node * cur = &root;
         for (uint64_t bin : rec.previous_TB_indices)
         {
+            if (bin >= kMaxLayoutBins)
+                return "layout: technical bin index beyond " + std::to_string(kMaxLayoutBins);
             cur->number_of_technical_bins = std::max(cur->number_of_technical_bins, bin + 1);
             bool found = false;
             for (Node & child : cur->children)
@@ -84,6 +96,30 @@ std::string make_graph(Layout const & layout, Node & root)
 }
 
 uint64_t divide_and_ceil(uint64_t a, uint64_t b) { return (a + (b - 1u)) / b; }
+
+// The bounds a layout is checked against before its tables are sized from it.
+bool node_bins_in_range(Node const & node, std::string & error)
+{
+    if (node.number_of_technical_bins == 0)
+    {
+        error = "layout: IBF without records";
+        return false;
+    }
+    if (node.max_bin_index >= node.number_of_technical_bins)
+    {
+        error = "layout: fullest technical bin " + std::to_string(node.max_bin_index) + " beyond the IBF's " +
+                std::to_string(node.number_of_technical_bins) + " technical bins";
+        return false;
+    }
+    for (Node const & child : node.children)
+        if (child.parent_bin_index >= node.number_of_technical_bins)
+        {
+            error = "layout: merged bin " + std::to_string(child.parent_bin_index) + " beyond the IBF's " +
+                    std::to_string(node.number_of_technical_bins) + " technical bins";
+            return false;
+        }
+    return true;
+}
 
 struct Builder
 {
@@ -134,6 +170,8 @@ struct Builder
     {
         uint32_t const pos = static_cast<uint32_t>(plan.ibfs.size());
         plan.ibfs.emplace_back();
+        if (!node_bins_in_range(node, error))
+            return pos;
         uint64_t const ntb = node.number_of_technical_bins;
         {
             PlannedIbf & p = plan.ibfs[pos];
@@ -148,6 +186,8 @@ struct Builder
         if (max_merged)
         {
             uint32_t const child = build(node.children[*node.favourite_child_idx], false, max_ubs);
+            if (!error.empty())
+                return pos;
             plan.ibfs[pos].next_ibf[node.max_bin_index] = child;
             max_elements = union_size(max_ubs);
         }
@@ -169,6 +209,12 @@ struct Builder
             fill_user_bins(plan.ibfs[pos], rec);
         }
         // construct_ibf (construct_ibf.cpp:26-64)
+        if (max_bin_tbs >= corr.size())
+        {
+            error = "layout: user bin split over " + std::to_string(max_bin_tbs) + " technical bins, more than " +
+                    "the top level's " + std::to_string(corr.size() - 1u);
+            return pos;
+        }
         uint64_t const per_bin = divide_and_ceil(max_elements, max_bin_tbs);
         double const fpr = max_merged ? c.relaxed_fpr : c.maximum_fpr;
         uint64_t const bits = bin_size_in_bits(fpr, c.number_of_hash_functions, per_bin);
@@ -194,6 +240,8 @@ struct Builder
             std::vector<uint32_t> local;
             Node const & child = node.children[ci];
             uint32_t const cpos = build(child, false, local);
+            if (!error.empty())
+                return pos;
             plan.ibfs[pos].next_ibf[child.parent_bin_index] = cpos;
             whole_segments(pos, child.parent_bin_index, local);
             subtree.insert(subtree.end(), local.begin(), local.end());
@@ -249,6 +297,9 @@ std::string build_plan(Config const & c, Layout const & layout, BuildInput const
     for (auto const & p : out.ibfs)
         if (p.bin_size == 0)
             return "The size of a bin must be > 0.";
+    for (Segment const & seg : out.segments)
+        if (seg.ibf >= out.ibfs.size() || seg.tb >= out.ibfs[seg.ibf].bins)
+            return "layout: a user bin's technical bins run past the end of its IBF";
     return {};
 }
 

@@ -118,7 +118,7 @@ int ensure(void *& ptr, size_t & have, size_t bytes, char const * what)
         have = 0;
     }
     size_t const want = std::max<size_t>(bytes, 256);
-    HIBF_TRY_HIP(hipMalloc(&ptr, want), what);
+    HIBF_TRY_HIP(hibf_amd::device_malloc(&ptr, want), what);
     have = want;
     return HIBF_GPU_OK;
 }
@@ -300,6 +300,8 @@ int hibf_gpu_tune_set(char const * key, int64_t value)
         g_count_tuning.row_per_lane = static_cast<uint32_t>(value);
     else if (k == "hibf.sliced_counts" && (value == 0 || value == 1))
         g_count_tuning.sliced_counts = static_cast<uint32_t>(value);
+    else if (k == "alloc.max_bytes" && value >= 0)
+        g_alloc_max_bytes = static_cast<uint64_t>(value);
     else
         return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "unknown tuning key or value out of range: " + k);
     return HIBF_GPU_OK;
@@ -332,6 +334,8 @@ int hibf_gpu_tune_get(char const * key, int64_t * value)
         *value = g_count_tuning.row_per_lane;
     else if (k == "hibf.sliced_counts")
         *value = g_count_tuning.sliced_counts;
+    else if (k == "alloc.max_bytes")
+        *value = static_cast<int64_t>(g_alloc_max_bytes);
     else
         return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "unknown tuning key: " + k);
     return HIBF_GPU_OK;
@@ -342,7 +346,7 @@ int hibf_gpu_malloc(void ** d_ptr, uint64_t bytes)
     if (!d_ptr)
         return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "null argument");
     *d_ptr = nullptr;
-    HIBF_TRY_HIP(hipMalloc(d_ptr, std::max<uint64_t>(bytes, 1)), "hibf_gpu_malloc");
+    HIBF_TRY_HIP(hibf_amd::device_malloc(d_ptr, std::max<uint64_t>(bytes, 1)), "hibf_gpu_malloc");
     return HIBF_GPU_OK;
 }
 
@@ -384,7 +388,7 @@ int hibf_gpu_ibf_create(uint64_t bins, uint64_t bin_size, uint32_t hash_funs, hi
     if (!h)
         return fail(HIBF_GPU_ERR_OUT_OF_MEMORY, "host allocation failed");
     h->g = g;
-    hipError_t e = hipMalloc(reinterpret_cast<void **>(&h->d_words), g.n_words * sizeof(uint64_t));
+    hipError_t e = hibf_amd::device_malloc(reinterpret_cast<void **>(&h->d_words), g.n_words * sizeof(uint64_t));
     if (e == hipSuccess)
         e = hipMemset(h->d_words, 0, g.n_words * sizeof(uint64_t));
     if (e != hipSuccess)
@@ -536,7 +540,7 @@ int hibf_gpu_ibf_clear(hibf_gpu_ibf ibf, uint64_t const * h_bins, uint64_t n, vo
         return fail(HIBF_GPU_ERR_UNSUPPORTED, "too many bins");
     hipStream_t const s = static_cast<hipStream_t>(stream);
     uint64_t * d_bins = nullptr;
-    HIBF_TRY_HIP(hipMalloc(reinterpret_cast<void **>(&d_bins), n * sizeof(uint64_t)), "clear alloc");
+    HIBF_TRY_HIP(hibf_amd::device_malloc(reinterpret_cast<void **>(&d_bins), n * sizeof(uint64_t)), "clear alloc");
     hipError_t e = hipMemcpyAsync(d_bins, h_bins, n * sizeof(uint64_t), hipMemcpyHostToDevice, s);
     if (e == hipSuccess)
         e = launch_clear(ibf->d_words, ibf->g.bin_size, static_cast<uint32_t>(ibf->g.bin_words), d_bins,
@@ -593,7 +597,7 @@ int hibf_gpu_ibf_increase_bin_number_to(hibf_gpu_ibf ibf, uint64_t new_bins)
     }
     std::copy(ibf->occupancy.begin(), ibf->occupancy.end(), occ.begin());
     uint64_t * d_new = nullptr;
-    HIBF_TRY_HIP(hipMalloc(reinterpret_cast<void **>(&d_new), g.n_words * sizeof(uint64_t)), "increase_bin_number_to");
+    HIBF_TRY_HIP(hibf_amd::device_malloc(reinterpret_cast<void **>(&d_new), g.n_words * sizeof(uint64_t)), "increase_bin_number_to");
     hipError_t e = launch_restride(ibf->d_words, ibf->g.bin_words, d_new, g.bin_words, g.bin_size, nullptr);
     if (e == hipSuccess)
         e = hipDeviceSynchronize();
@@ -647,7 +651,7 @@ int hibf_gpu_agent_create(uint64_t max_queries_per_launch, hibf_gpu_agent * out)
     if (!a)
         return fail(HIBF_GPU_ERR_OUT_OF_MEMORY, "host allocation failed");
     a->max_queries = max_queries_per_launch ? max_queries_per_launch : 65536u;
-    hipError_t e = hipMalloc(reinterpret_cast<void **>(&a->d_total), 8 * sizeof(unsigned long long));
+    hipError_t e = hibf_amd::device_malloc(reinterpret_cast<void **>(&a->d_total), 8 * sizeof(unsigned long long));
     if (e == hipSuccess)
         e = hipMemset(a->d_total, 0, 8 * sizeof(unsigned long long));
     if (e == hipSuccess)
@@ -928,16 +932,16 @@ int hibf_gpu_hibf_create(uint64_t n_ibfs,
     if (static_cast<size_t>(h->max_bins) * (1u + h->depth) * sizeof(uint32_t) > 64u * 1024u)
         return fail(HIBF_GPU_ERR_UNSUPPORTED, "largest IBF x depth exceeds the LDS budget of the descent kernel");
     h->n_words = end_words;
-    hipError_t e = hipMalloc(reinterpret_cast<void **>(&h->d_arena), std::max<uint64_t>(end_words, 1) * 8u);
+    hipError_t e = hibf_amd::device_malloc(reinterpret_cast<void **>(&h->d_arena), std::max<uint64_t>(end_words, 1) * 8u);
     if (e == hipSuccess)
         e = h_arena ? hipMemcpy(h->d_arena, h_arena, end_words * 8u, hipMemcpyHostToDevice)
                     : hipMemset(h->d_arena, 0, end_words * 8u);
     if (e == hipSuccess)
-        e = hipMalloc(reinterpret_cast<void **>(&h->d_desc), n_ibfs * sizeof(HibfIbfDesc));
+        e = hibf_amd::device_malloc(reinterpret_cast<void **>(&h->d_desc), n_ibfs * sizeof(HibfIbfDesc));
     if (e == hipSuccess)
         e = hipMemcpy(h->d_desc, desc.data(), n_ibfs * sizeof(HibfIbfDesc), hipMemcpyHostToDevice);
     if (e == hipSuccess)
-        e = hipMalloc(reinterpret_cast<void **>(&h->d_tb_info), std::max<uint64_t>(n_tbs, 1) * 8u);
+        e = hibf_amd::device_malloc(reinterpret_cast<void **>(&h->d_tb_info), std::max<uint64_t>(n_tbs, 1) * 8u);
     if (e == hipSuccess)
         e = hipMemcpy(h->d_tb_info, info.data(), n_tbs * 8u, hipMemcpyHostToDevice);
     if (e != hipSuccess)
@@ -1313,6 +1317,9 @@ int hibf_gpu_hibf_membership_for(hibf_gpu_hibf h,
     for (int attempt = 0; attempt < 2; ++attempt)
     {
         uint64_t const cap = a->ids_tmp_bytes / 8u;
+        // The dropped flag describes this call's result: a first attempt whose
+        // temporary id buffer was too small must not leave it set.
+        HIBF_TRY_HIP(launch_reset_counters(a->d_total, 1u << 5, s), "flag reset");
         if (int rc = enqueue_membership(h, a, d_values, d_offsets, nq, d_thresholds,
                                         static_cast<uint64_t *>(a->ids_tmp), cap,
                                         static_cast<uint64_t *>(a->hit_start), static_cast<uint64_t *>(a->hit_count),
@@ -1441,7 +1448,7 @@ struct DeviceBuffer
         if (p)
             (void)hipFree(p);
     }
-    hipError_t alloc(size_t bytes) { return hipMalloc(&p, std::max<size_t>(bytes, 16u)); }
+    hipError_t alloc(size_t bytes) { return hibf_amd::device_malloc(&p, std::max<size_t>(bytes, 16u)); }
     uint8_t * u8() const { return static_cast<uint8_t *>(p); }
     uint64_t * u64() const { return static_cast<uint64_t *>(p); }
 };
@@ -1529,11 +1536,11 @@ int run_plan(host::Plan const & plan,
             if (p)
                 (void)hipFree(p);
     };
-    hipError_t e = hipMalloc(&d_segs, std::max<size_t>(plan.segments.size(), 1) * sizeof(EmplaceSegment));
+    hipError_t e = hibf_amd::device_malloc(&d_segs, std::max<size_t>(plan.segments.size(), 1) * sizeof(EmplaceSegment));
     if (e == hipSuccess)
         e = hipMemcpy(d_segs, plan.segments.data(), plan.segments.size() * sizeof(EmplaceSegment), hipMemcpyHostToDevice);
     if (e == hipSuccess)
-        e = hipMalloc(&d_chunks, std::max<size_t>(chunks.size(), 1) * sizeof(EmplaceChunk));
+        e = hibf_amd::device_malloc(&d_chunks, std::max<size_t>(chunks.size(), 1) * sizeof(EmplaceChunk));
     if (e == hipSuccess)
         e = hipMemcpy(d_chunks, chunks.data(), chunks.size() * sizeof(EmplaceChunk), hipMemcpyHostToDevice);
     if (e == hipSuccess)

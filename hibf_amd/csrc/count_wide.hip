@@ -211,6 +211,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((HF > 
 } // namespace
 
 CountTuning g_count_tuning{};
+uint64_t g_alloc_max_bytes = 0;
 
 bool count_wide_applies(IbfView const & g)
 {

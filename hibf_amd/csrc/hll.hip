@@ -291,7 +291,7 @@ int run_sketches(std::vector<HllChunk> const & chunks,
         return e == hipSuccess ? HIBF_GPU_OK : device_error(e, "hll sketch");
     }
     void * d_chunks = nullptr;
-    e = hipMalloc(&d_chunks, chunks.size() * sizeof(HllChunk));
+    e = hibf_amd::device_malloc(&d_chunks, chunks.size() * sizeof(HllChunk));
     if (e == hipSuccess)
         e = hipMemcpyAsync(d_chunks, chunks.data(), chunks.size() * sizeof(HllChunk), hipMemcpyHostToDevice, s);
     if (e == hipSuccess)
@@ -355,7 +355,7 @@ int hll_estimates(uint8_t const * d_registers, uint64_t n, uint32_t bits, double
     if (n == 0)
         return HIBF_GPU_OK;
     void * d = nullptr;
-    hipError_t e = hipMalloc(&d, n * (sizeof(float) + sizeof(uint32_t)));
+    hipError_t e = hibf_amd::device_malloc(&d, n * (sizeof(float) + sizeof(uint32_t)));
     std::vector<float> sums(n);
     std::vector<uint32_t> zeros(n);
     if (e == hipSuccess)

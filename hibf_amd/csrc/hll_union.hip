@@ -19,6 +19,7 @@
 // the host with the reference's expression.  Sketches are stored lane-major
 // so that a lane's registers are contiguous.
 #include "hll_union.hpp"
+#include "kernels.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -420,7 +421,7 @@ double const * small_estimate_table(uint32_t bits, hipError_t & e)
     for (uint64_t z = 1; z <= size; ++z)
         h[z] = static_cast<double>(size) * std::log(static_cast<double>(size) / static_cast<double>(z));
     double * d = nullptr;
-    e = hipMalloc(&d, h.size() * sizeof(double));
+    e = hibf_amd::device_malloc(&d, h.size() * sizeof(double));
     if (e == hipSuccess)
         e = hipMemcpy(d, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice);
     if (e != hipSuccess)
@@ -515,7 +516,7 @@ int DeviceUnionEstimates::create(uint8_t const * d_registers,
     if (e == hipSuccess)
         e = hipEventCreate(&u->e1_);
     if (e == hipSuccess)
-        e = hipMalloc(reinterpret_cast<void **>(&u->d_lanes_), std::max<uint64_t>(n << bits, 1u));
+        e = hibf_amd::device_malloc(reinterpret_cast<void **>(&u->d_lanes_), std::max<uint64_t>(n << bits, 1u));
     if (e == hipSuccess)
         e = hipHostMalloc(reinterpret_cast<void **>(&u->h_window_), u->window_cap_ * sizeof(uint64_t));
     if (e == hipSuccess)
@@ -544,7 +545,7 @@ bool DeviceUnionEstimates::start(std::vector<uint64_t> const & positions)
         if (d_pos_)
             (void)hipFree(d_pos_);
         d_pos_ = nullptr;
-        e = hipMalloc(reinterpret_cast<void **>(&d_pos_), np_ * sizeof(uint64_t));
+        e = hibf_amd::device_malloc(reinterpret_cast<void **>(&d_pos_), np_ * sizeof(uint64_t));
         pos_cap_ = e == hipSuccess ? np_ : 0u;
     }
     if (e == hipSuccess && values > tri_cap_)
@@ -552,7 +553,7 @@ bool DeviceUnionEstimates::start(std::vector<uint64_t> const & positions)
         if (d_tri_)
             (void)hipFree(d_tri_);
         d_tri_ = nullptr;
-        e = hipMalloc(reinterpret_cast<void **>(&d_tri_), values * sizeof(uint64_t));
+        e = hibf_amd::device_malloc(reinterpret_cast<void **>(&d_tri_), values * sizeof(uint64_t));
         tri_cap_ = e == hipSuccess ? values : 0u;
     }
     if (e == hipSuccess)
@@ -582,7 +583,7 @@ bool DeviceUnionEstimates::prefix(uint64_t * prefix)
     if (np_ < 2u)
         return true;
     uint64_t * d_heads = nullptr;
-    hipError_t e = hipMalloc(reinterpret_cast<void **>(&d_heads), np_ * sizeof(uint64_t));
+    hipError_t e = hibf_amd::device_malloc(reinterpret_cast<void **>(&d_heads), np_ * sizeof(uint64_t));
     if (e == hipSuccess)
     {
         hipLaunchKernelGGL(column_heads_kernel, dim3(static_cast<unsigned>((np_ + 255u) / 256u)), dim3(256), 0, s_,
@@ -693,8 +694,8 @@ void DevicePairUnions::ensure(uint64_t pairs_needed, uint64_t slots_needed)
         h_idx_ = nullptr;
         h_out_ = nullptr;
         pair_cap_ = 0;
-        check(hipMalloc(reinterpret_cast<void **>(&d_idx_), cap * sizeof(uint32_t)), "pair unions buffers");
-        check(hipMalloc(reinterpret_cast<void **>(&d_out_), cap * sizeof(double)), "pair unions buffers");
+        check(hibf_amd::device_malloc(reinterpret_cast<void **>(&d_idx_), cap * sizeof(uint32_t)), "pair unions buffers");
+        check(hibf_amd::device_malloc(reinterpret_cast<void **>(&d_out_), cap * sizeof(double)), "pair unions buffers");
         check(hipHostMalloc(reinterpret_cast<void **>(&h_idx_), cap * sizeof(uint32_t)), "pair unions buffers");
         check(hipHostMalloc(reinterpret_cast<void **>(&h_out_), cap * sizeof(double)), "pair unions buffers");
         pair_cap_ = cap;
@@ -703,7 +704,7 @@ void DevicePairUnions::ensure(uint64_t pairs_needed, uint64_t slots_needed)
     {
         uint64_t const cap = std::max<uint64_t>(slots_needed, 2u * slot_cap_);
         uint8_t * fresh = nullptr;
-        check(hipMalloc(reinterpret_cast<void **>(&fresh), cap << bits_), "pair unions slots");
+        check(hibf_amd::device_malloc(reinterpret_cast<void **>(&fresh), cap << bits_), "pair unions slots");
         if (d_slots_)
         {
             hipError_t const e = hipMemcpyAsync(fresh, d_slots_, uploaded_ << bits_, hipMemcpyDeviceToDevice, s_);
@@ -780,7 +781,7 @@ void DevicePairUnions::all_pairs(std::vector<seqan::hibf::sketch::hyperloglog co
         return union_source::all_pairs(leaves, out);
     uint64_t const pairs_n = L * (L - 1u) / 2u;
     double * d_tri = nullptr;
-    check(hipMalloc(reinterpret_cast<void **>(&d_tri), pairs_n * sizeof(double)), "pair unions triangle");
+    check(hibf_amd::device_malloc(reinterpret_cast<void **>(&d_tri), pairs_n * sizeof(double)), "pair unions triangle");
     hipError_t e = all_pairs_launch(d_slots_, bits_, base, L, norm_, d_table_, d_tri, s_);
     if (e == hipSuccess)
         e = hipMemcpyAsync(out, d_tri, pairs_n * sizeof(double), hipMemcpyDeviceToHost, s_);
@@ -847,7 +848,7 @@ int hibf_gpu_hll_union_estimates(uint8_t const * d_registers,
         if (p >= n_sketches)
             return hibf_amd::set_error(HIBF_GPU_ERR_INVALID_ARGUMENT, "position out of range");
     uint8_t * lanes = nullptr;
-    e = hipMalloc(reinterpret_cast<void **>(&lanes), n_sketches << bits);
+    e = hibf_amd::device_malloc(reinterpret_cast<void **>(&lanes), n_sketches << bits);
     if (e == hipSuccess)
         e = hibf_amd::hll_lane_major(d_registers, n_sketches, bits, lanes, s);
     if (e == hipSuccess)

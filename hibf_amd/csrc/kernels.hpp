@@ -58,6 +58,31 @@ struct CountTuning
 };
 extern CountTuning g_count_tuning;
 
+// Largest single device allocation the library makes, in bytes (0: no
+// limit).  Set through hibf_gpu_tune_set("alloc.max_bytes"); a larger request
+// fails as hipMalloc fails when memory runs out, so the out-of-memory paths of
+// every entry point can be exercised.
+extern uint64_t g_alloc_max_bytes;
+
+// Every device allocation of the library goes through here.
+inline hipError_t device_malloc(void ** p, size_t bytes)
+{
+    if (g_alloc_max_bytes != 0 && bytes > g_alloc_max_bytes)
+    {
+        *p = nullptr;
+        return hipErrorOutOfMemory;
+    }
+    return hipMalloc(p, bytes);
+}
+template <typename T>
+hipError_t device_malloc(T ** p, size_t bytes)
+{
+    void * v = nullptr;
+    hipError_t const e = device_malloc(&v, bytes);
+    *p = static_cast<T *>(v);
+    return e;
+}
+
 struct IbfView;
 bool count_wide_applies(IbfView const & g);
 hipError_t launch_count_wide(IbfView const & g,

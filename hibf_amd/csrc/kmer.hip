@@ -190,7 +190,7 @@ int hibf_gpu_kmer_hash(uint8_t const * d_seq,
     hipError_t e = hibf_amd::inclusive_sum_u64(nullptr, temp_bytes, d_hash_offsets + 1, d_hash_offsets + 1, n_seqs, s);
     void * temp = nullptr;
     if (e == hipSuccess)
-        e = hipMalloc(&temp, std::max<size_t>(temp_bytes, 1));
+        e = hibf_amd::device_malloc(&temp, std::max<size_t>(temp_bytes, 1));
     if (e == hipSuccess)
         e = hibf_amd::inclusive_sum_u64(temp, temp_bytes, d_hash_offsets + 1, d_hash_offsets + 1, n_seqs, s);
     if (e == hipSuccess)

@@ -24,8 +24,26 @@
 #include "hibf_gpu.h"
 #include "kernels.hpp"
 
+namespace hibf_amd
+{
+int set_error(int status, std::string msg);
+}
+
 namespace
 {
+
+// Status of a failed HIP call of the loop, with its detail for hibf_gpu_last_error.
+int serve_fail(hipError_t e, char const * what)
+{
+    int const status = (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) ? HIBF_GPU_ERR_OUT_OF_MEMORY
+                                                                                    : HIBF_GPU_ERR_DEVICE;
+    return hibf_amd::set_error(status, std::string("serving loop: ") + what + ": " + hipGetErrorString(e));
+}
+
+int serve_reject(int status, std::string msg)
+{
+    return hibf_amd::set_error(status, "serving loop: " + std::move(msg));
+}
 
 struct Slot
 {
@@ -120,24 +138,35 @@ int serve_impl(hibf_gpu_hibf hibf,
     bool const reads = eb == 1u;
     if (!hibf || !params || !h_in || !h_offsets || (!reads && !h_thresholds) || !h_arrival_s || !h_latency_s || !h_ids
         || !h_counts)
-        return HIBF_GPU_ERR_INVALID_ARGUMENT;
+        return serve_reject(HIBF_GPU_ERR_INVALID_ARGUMENT, "null argument");
     if (reads && (k == 0 || k > 32 || !(fraction >= 0.0)))
-        return HIBF_GPU_ERR_INVALID_ARGUMENT;
+        return serve_reject(HIBF_GPU_ERR_INVALID_ARGUMENT, "k must be in [1, 32] and fraction >= 0");
     hibf_gpu_serve_params const p = *params;
     if (p.streams == 0 || p.max_batch == 0 || p.results_per_query == 0 || p.max_batch_values == 0)
-        return HIBF_GPU_ERR_INVALID_ARGUMENT;
+        return serve_reject(HIBF_GPU_ERR_INVALID_ARGUMENT,
+                            "streams, max_batch, max_batch_values and results_per_query must be > 0");
+    if (p.streams > 64)
+        return serve_reject(HIBF_GPU_ERR_INVALID_ARGUMENT, "at most 64 streams");
+    if (!(p.window_us >= 0.0) || p.window_us > 60e6)
+        return serve_reject(HIBF_GPU_ERR_INVALID_ARGUMENT, "window_us must be in [0, 6e7]");
     uint32_t const R = p.results_per_query;
     // device bytes of a request: its elements (+ its hashes for a read), offset, threshold, ids
     auto bytes_of = [&](uint64_t q)
     { return (h_offsets[q + 1] - h_offsets[q]) * (eb + (reads ? 8u : 0u)) + 34u + 8u * R; };
     for (uint64_t q = 0; q < nq; ++q)
     {
+        if (h_offsets[q + 1] < h_offsets[q])
+            return serve_reject(HIBF_GPU_ERR_INVALID_ARGUMENT, "offsets decrease at request " + std::to_string(q));
         if (h_offsets[q + 1] - h_offsets[q] > p.max_batch_values)
-            return HIBF_GPU_ERR_INVALID_ARGUMENT; // a single request larger than a batch
+            return serve_reject(HIBF_GPU_ERR_INVALID_ARGUMENT,
+                                "request " + std::to_string(q) + " is larger than max_batch_values");
         if (bytes_of(q) > p.budget_bytes)
-            return HIBF_GPU_ERR_BUDGET; // could never be admitted
-        if (q && h_arrival_s[q] < h_arrival_s[q - 1])
-            return HIBF_GPU_ERR_INVALID_ARGUMENT;
+            return serve_reject(HIBF_GPU_ERR_BUDGET, "request " + std::to_string(q) + " needs " +
+                                                         std::to_string(bytes_of(q)) +
+                                                         " bytes of device memory, more than the budget");
+        if (!(h_arrival_s[q] >= 0.0) || (q && h_arrival_s[q] < h_arrival_s[q - 1]))
+            return serve_reject(HIBF_GPU_ERR_INVALID_ARGUMENT,
+                                "arrival times must be >= 0 and non-decreasing (request " + std::to_string(q) + ")");
     }
 
     bool const pinned = host_pinned(h_in, h_offsets[nq] * eb);
@@ -171,11 +200,12 @@ int serve_impl(hibf_gpu_hibf hibf,
 #define SERVE_TRY(expr)                                                                                                \
     do                                                                                                                 \
     {                                                                                                                  \
-        if ((expr) != hipSuccess)                                                                                      \
+        hipError_t const e_ = (expr);                                                                                  \
+        if (e_ != hipSuccess)                                                                                          \
         {                                                                                                              \
             (void)hipGetLastError();                                                                                   \
             cleanup();                                                                                                 \
-            return HIBF_GPU_ERR_DEVICE;                                                                                \
+            return serve_fail(e_, #expr);                                                                              \
         }                                                                                                              \
     } while (0)
     uint64_t const B = p.max_batch;
@@ -234,7 +264,7 @@ int serve_impl(hibf_gpu_hibf hibf,
         if (e != hipSuccess)
         {
             (void)hipGetLastError();
-            return HIBF_GPU_ERR_DEVICE;
+            return serve_fail(e, "batch input copies");
         }
         if (int const r = hibf_gpu_hibf_membership_launch(hibf, s.agent, d_v, d_o, nb, s.d_thr, s.d_ids, nb * R,
                                                           s.d_hs, s.d_hc, s.stream))
@@ -246,7 +276,7 @@ int serve_impl(hibf_gpu_hibf hibf,
         if (e != hipSuccess)
         {
             (void)hipGetLastError();
-            return HIBF_GPU_ERR_DEVICE;
+            return serve_fail(e, "batch result copies");
         }
         return HIBF_GPU_OK;
     };
@@ -262,18 +292,18 @@ int serve_impl(hibf_gpu_hibf hibf,
         SERVE_TRY(hipHostMalloc(reinterpret_cast<void **>(&s.h_ids), B * R * 8u, hipHostMallocDefault));
         SERVE_TRY(hipHostMalloc(reinterpret_cast<void **>(&s.h_hs), B * 8u, hipHostMallocDefault));
         SERVE_TRY(hipHostMalloc(reinterpret_cast<void **>(&s.h_hc), B * 8u, hipHostMallocDefault));
-        SERVE_TRY(hipMalloc(reinterpret_cast<void **>(&s.d_vals), std::max<uint64_t>(V * eb, 8u)));
+        SERVE_TRY(hibf_amd::device_malloc(reinterpret_cast<void **>(&s.d_vals), std::max<uint64_t>(V * eb, 8u)));
         if (reads)
         {
-            SERVE_TRY(hipMalloc(reinterpret_cast<void **>(&s.d_hashes), V * 8u));
-            SERVE_TRY(hipMalloc(reinterpret_cast<void **>(&s.d_hash_offs), (B + 1) * 8u));
+            SERVE_TRY(hibf_amd::device_malloc(reinterpret_cast<void **>(&s.d_hashes), V * 8u));
+            SERVE_TRY(hibf_amd::device_malloc(reinterpret_cast<void **>(&s.d_hash_offs), (B + 1) * 8u));
             SERVE_TRY(hipHostMalloc(reinterpret_cast<void **>(&s.h_hash_offs), (B + 1) * 8u, hipHostMallocDefault));
         }
-        SERVE_TRY(hipMalloc(reinterpret_cast<void **>(&s.d_offs), (B + 1) * 8u));
-        SERVE_TRY(hipMalloc(reinterpret_cast<void **>(&s.d_thr), B * 2u));
-        SERVE_TRY(hipMalloc(reinterpret_cast<void **>(&s.d_ids), B * R * 8u));
-        SERVE_TRY(hipMalloc(reinterpret_cast<void **>(&s.d_hs), B * 8u));
-        SERVE_TRY(hipMalloc(reinterpret_cast<void **>(&s.d_hc), B * 8u));
+        SERVE_TRY(hibf_amd::device_malloc(reinterpret_cast<void **>(&s.d_offs), (B + 1) * 8u));
+        SERVE_TRY(hibf_amd::device_malloc(reinterpret_cast<void **>(&s.d_thr), B * 2u));
+        SERVE_TRY(hibf_amd::device_malloc(reinterpret_cast<void **>(&s.d_ids), B * R * 8u));
+        SERVE_TRY(hibf_amd::device_malloc(reinterpret_cast<void **>(&s.d_hs), B * 8u));
+        SERVE_TRY(hibf_amd::device_malloc(reinterpret_cast<void **>(&s.d_hc), B * 8u));
         if ((rc = hibf_gpu_agent_create(B, &s.agent)) != HIBF_GPU_OK ||
             (rc = hibf_gpu_hibf_membership_prepare(hibf, s.agent, B)) != HIBF_GPU_OK)
         {
@@ -382,7 +412,7 @@ int serve_impl(hibf_gpu_hibf hibf,
             {
                 (void)hipGetLastError();
                 cleanup();
-                return HIBF_GPU_ERR_DEVICE;
+                return serve_fail(e, "batch completion");
             }
             double const t = now_s(t0);
             for (uint64_t q = s.q0; q < s.q1; ++q)

@@ -58,6 +58,19 @@ def as_device_u64(x) -> torch.Tensor:
     return torch.from_numpy(a.view(np.int64)).to(_dev()).view(torch.uint64)
 
 
+def _check_buffer(t, name: str, min_numel: int, element_size: int | None = 8, exact: bool = False) -> None:
+    """Raises ValueError unless t is a contiguous device tensor of at least
+    (exactly, with exact=True) min_numel elements of element_size bytes: the
+    C-ABI reads and writes these buffers by count, without their sizes."""
+    if not isinstance(t, torch.Tensor) or not t.is_cuda or not t.is_contiguous():
+        raise ValueError(f"{name} must be a contiguous device tensor")
+    if element_size is not None and t.element_size() != element_size:
+        raise ValueError(f"{name} must have {element_size}-byte elements (got {t.element_size()})")
+    if (t.numel() != min_numel) if exact else (t.numel() < min_numel):
+        raise ValueError(f"{name} must have {'exactly' if exact else 'at least'} {min_numel} elements "
+                         f"(got {t.numel()})")
+
+
 def to_numpy_u64(t: torch.Tensor) -> np.ndarray:
     return t.view(torch.int64).cpu().numpy().view(np.uint64)
 
@@ -278,8 +291,14 @@ class DeviceIbf:
 
     def bulk_count_into(self, values: torch.Tensor, offsets: torch.Tensor, out: torch.Tensor,
                         counter_bytes: int = 2, stream=None) -> None:
-        """bulk_count into a preallocated (nq, bin_count) tensor, asynchronously."""
+        """bulk_count into a preallocated (nq, bin_count) tensor, asynchronously.
+        values must hold the offsets[nq] values the offsets address."""
         nq = offsets.numel() - 1
+        _check_buffer(offsets, "offsets", 1)
+        _check_buffer(values, "values", 1 if nq else 0)
+        if counter_bytes not in _DTYPES:
+            raise ValueError("counter_bytes must be 1, 2, 4 or 8")
+        _check_buffer(out, "out", nq * self.bin_count(), counter_bytes)
         s = _stream() if stream is None else ctypes.c_void_p(stream.cuda_stream)
         check(lib().hibf_gpu_ibf_bulk_count(self.h, _ptr(values), _ptr(offsets), nq, counter_bytes, _ptr(out), s))
 
@@ -516,6 +535,12 @@ class DeviceHibf:
         """Asynchronous membership over the CSR batch (offsets has nq+1 entries);
         query q's ids are ids[hit_start[q] : hit_start[q] + hit_count[q]]."""
         nq = offsets.numel() - 1
+        _check_buffer(offsets, "offsets", 1)
+        _check_buffer(values, "values", 1 if nq else 0)
+        _check_buffer(thresholds, "thresholds", nq, 2, exact=True)
+        _check_buffer(ids, "ids", 0)
+        _check_buffer(hit_start, "hit_start", nq)
+        _check_buffer(hit_count, "hit_count", nq)
         s = _stream() if stream is None else ctypes.c_void_p(stream.cuda_stream)
         check(lib().hibf_gpu_hibf_membership_launch(self.h, agent.h, _ptr(values), _ptr(offsets), nq,
                                                     _ptr(thresholds), _ptr(ids), ids.numel(), _ptr(hit_start),
@@ -538,6 +563,10 @@ class DeviceHibf:
         t = np.ascontiguousarray(thresholds, dtype=np.uint16)
         a = np.ascontiguousarray(arrival_s, dtype=np.float64)
         nq = o.size - 1
+        if nq < 0 or t.size != nq or a.size != nq:
+            raise ValueError("offsets need nq + 1 entries and thresholds and arrival_s one per request")
+        if (v.numel() if isinstance(v, torch.Tensor) else v.size) < (int(o[-1]) if nq >= 0 else 0):
+            raise ValueError("values hold fewer elements than offsets[-1] addresses")
         lat = np.zeros(nq, dtype=np.float64)
         ids = np.zeros((nq, results_per_query), dtype=np.uint64)
         cnt = np.zeros(nq, dtype=np.uint32)
@@ -568,6 +597,8 @@ class DeviceHibf:
             b = np.ascontiguousarray(data if data.size else np.zeros(1, dtype=np.uint8), dtype=np.uint8)
         a = np.ascontiguousarray(arrival_s, dtype=np.float64)
         nq = o.size - 1
+        if nq < 0 or a.size != nq or (nq > 0 and int(o[-1]) > data.size):
+            raise ValueError("offsets need nq + 1 entries within the bases and arrival_s one per read")
         lat = np.zeros(nq, dtype=np.float64)
         ids = np.zeros((nq, results_per_query), dtype=np.uint64)
         cnt = np.zeros(nq, dtype=np.uint32)
@@ -585,10 +616,12 @@ class DeviceHibf:
         """HIBF bulk_count into a preallocated (nq, number_of_user_bins) device
         tensor; thresholds is a uint64 device tensor of one entry per query."""
         nq = offsets.numel() - 1
-        if out.numel() < nq * self.number_of_user_bins or out.element_size() != counter_bytes:
-            raise ValueError("out must hold nq x number_of_user_bins counters of counter_bytes bytes")
-        if thresholds.numel() < nq or thresholds.element_size() != 8:
-            raise ValueError("thresholds must be a 64-bit tensor of one entry per query")
+        _check_buffer(offsets, "offsets", 1)
+        _check_buffer(values, "values", 1 if nq else 0)
+        if counter_bytes not in _DTYPES:
+            raise ValueError("counter_bytes must be 1, 2, 4 or 8")
+        _check_buffer(out, "out", nq * self.number_of_user_bins, counter_bytes)
+        _check_buffer(thresholds, "thresholds", nq, 8, exact=True)
         check(lib().hibf_gpu_hibf_bulk_count(self.h, agent.h, _ptr(values), _ptr(offsets), nq, _ptr(thresholds),
                                              counter_bytes, _ptr(out), _stream()))
 

@@ -18,6 +18,7 @@
 #include <memory>
 #include <ranges>
 #include <sstream>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -67,6 +68,26 @@ struct agent_handle
 private:
     hibf_gpu_agent a_{nullptr};
 };
+
+// The batched agents take one threshold per query; anything else would send
+// the device kernels past the end of the thresholds buffer.
+template <typename queries_t, typename thresholds_t>
+void check_batch_thresholds(queries_t const & queries, thresholds_t const & thresholds)
+{
+    auto const nq = static_cast<size_t>(std::ranges::distance(queries));
+    if (thresholds.size() != nq)
+        throw std::invalid_argument{"one threshold per query is required (" + std::to_string(thresholds.size()) +
+                                    " thresholds for " + std::to_string(nq) + " queries)"};
+}
+
+// Uploads n elements of T (one zero element when n == 0, so that the device
+// buffer exists and no empty vector's data() is read).
+template <typename T>
+void upload_or_zero(device_buffer & d, std::vector<T> const & h)
+{
+    T const zero{};
+    d.upload(h.empty() ? &zero : h.data(), std::max<size_t>(h.size(), 1) * sizeof(T));
+}
 
 // Flattens a batch of queries into CSR form on the device.
 struct query_batch
@@ -294,6 +315,7 @@ public:
     std::vector<std::vector<uint64_t>> membership_for_batch(queries_t const & queries,
                                                             std::vector<uint16_t> const & thresholds)
     {
+        detail::check_batch_thresholds(queries, thresholds);
         run(queries, thresholds);
         std::vector<std::vector<uint64_t>> out(batch_.nq);
         for (size_t q = 0; q < batch_.nq; ++q)
@@ -308,7 +330,7 @@ private:
         try
         {
             batch_.upload(queries);
-            d_thr_.upload(thresholds.data(), std::max<size_t>(thresholds.size(), 1) * 2u);
+            detail::upload_or_zero(d_thr_, thresholds);
             d_hoff_.reserve((batch_.nq + 1) * 8u);
             uint64_t total = 0;
             uint64_t cap = std::max<uint64_t>(cap_, 1024u);
@@ -374,13 +396,15 @@ public:
                                                               size_t const threshold) && noexcept = delete;
 
     // Batched form: row q of the result (number_of_user_bins counters) is query q's counts.
+    // std::invalid_argument unless there is one threshold per query.
     template <typename queries_t>
-    std::vector<value_t> bulk_count_batch(queries_t const & queries, std::vector<uint64_t> const & thresholds) noexcept
+    std::vector<value_t> bulk_count_batch(queries_t const & queries, std::vector<uint64_t> const & thresholds)
     {
+        detail::check_batch_thresholds(queries, thresholds);
         try
         {
             batch_.upload(queries);
-            d_thr_.upload(thresholds.data(), std::max<size_t>(thresholds.size(), 1) * 8u);
+            detail::upload_or_zero(d_thr_, thresholds);
             size_t const n = batch_.nq * hibf_ptr->number_of_user_bins;
             d_counts_.reserve(std::max<size_t>(n, 1) * sizeof(value_t));
             detail::check(hibf_gpu_hibf_bulk_count(hibf_ptr->handle(), agent_.get(), batch_.values.as<uint64_t>(),
