@@ -11,6 +11,7 @@
 #include <mutex>
 #include <new>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "hibf_gpu.h"
@@ -240,6 +241,7 @@ struct hibf_gpu_hibf_s
     // their estimates, layout, allocation (arena and tables), device fill.
     double build_times[4]{};
     double layout_times[2]{}; // union estimation and rearrangement (parts of build_times[1])
+    std::string layout_text;  // the layout the build used (reference text format); empty if not built here
     uint64_t * d_arena{nullptr};
     HibfIbfDesc * d_desc{nullptr};
     uint64_t * d_tb_info{nullptr};
@@ -302,6 +304,8 @@ int hibf_gpu_tune_set(char const * key, int64_t value)
         g_count_tuning.sliced_counts = static_cast<uint32_t>(value);
     else if (k == "alloc.max_bytes" && value >= 0)
         g_alloc_max_bytes = static_cast<uint64_t>(value);
+    else if (k == "build.split_order" && (value == 0 || value == 1))
+        g_count_tuning.split_order = static_cast<uint32_t>(value);
     else
         return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "unknown tuning key or value out of range: " + k);
     return HIBF_GPU_OK;
@@ -336,6 +340,8 @@ int hibf_gpu_tune_get(char const * key, int64_t * value)
         *value = g_count_tuning.sliced_counts;
     else if (k == "alloc.max_bytes")
         *value = static_cast<int64_t>(g_alloc_max_bytes);
+    else if (k == "build.split_order")
+        *value = g_count_tuning.split_order;
     else
         return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "unknown tuning key: " + k);
     return HIBF_GPU_OK;
@@ -1492,6 +1498,7 @@ int estimate_kmer_counts(host::Config const & hc,
 // runs the segment emplace over explicit values (device CSR) or the synthetic
 // stream.
 int run_plan(host::Plan const & plan,
+             host::Layout const & layout,
              uint64_t n_user_bins,
              uint64_t const * d_ub_offsets,
              uint64_t const * d_values,
@@ -1530,9 +1537,10 @@ int run_plan(host::Plan const & plan,
             chunks.push_back({static_cast<uint32_t>(s), static_cast<uint32_t>(std::min<uint64_t>(4096u, plan.segments[s].count - b)), b});
     void * d_segs = nullptr;
     void * d_chunks = nullptr;
+    void * d_chunk_ids = nullptr;
     auto cleanup = [&]()
     {
-        for (void * p : {d_segs, d_chunks})
+        for (void * p : {d_segs, d_chunks, d_chunk_ids})
             if (p)
                 (void)hipFree(p);
     };
@@ -1543,10 +1551,16 @@ int run_plan(host::Plan const & plan,
         e = hibf_amd::device_malloc(&d_chunks, std::max<size_t>(chunks.size(), 1) * sizeof(EmplaceChunk));
     if (e == hipSuccess)
         e = hipMemcpy(d_chunks, chunks.data(), chunks.size() * sizeof(EmplaceChunk), hipMemcpyHostToDevice);
+    if (e == hipSuccess && !plan.chunk_ids.empty())
+    {
+        e = hibf_amd::device_malloc(&d_chunk_ids, plan.chunk_ids.size());
+        if (e == hipSuccess)
+            e = hipMemcpy(d_chunk_ids, plan.chunk_ids.data(), plan.chunk_ids.size(), hipMemcpyHostToDevice);
+    }
     if (e == hipSuccess)
         e = launch_segment_emplace(h->d_desc, h->d_arena, static_cast<EmplaceSegment *>(d_segs),
                                    static_cast<EmplaceChunk *>(d_chunks), chunks.size(), d_values, d_ub_offsets, seed,
-                                   nullptr);
+                                   static_cast<uint8_t const *>(d_chunk_ids), plan.chunk_id_bytes, nullptr);
     if (e == hipSuccess)
         e = hipDeviceSynchronize();
     cleanup();
@@ -1562,6 +1576,14 @@ int run_plan(host::Plan const & plan,
     h->layout_times[1] = phase_s[3];
     h->build_times[2] = std::chrono::duration<double>(t_fill - t_alloc).count();
     h->build_times[3] = std::chrono::duration<double>(t_end - t_fill).count();
+    try
+    {
+        h->layout_text = layout.to_text();
+    }
+    catch (std::exception const &)
+    {
+        h->layout_text.clear();
+    }
     *out = h;
     return HIBF_GPU_OK;
 }
@@ -1672,6 +1694,13 @@ int layout_for(host::Config const & hc,
 }
 
 } // namespace
+
+int hibf_gpu_hibf_layout(hibf_gpu_hibf h, char * text, uint64_t cap, uint64_t * needed)
+{
+    if (!h || (!text && cap))
+        return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "null argument");
+    return copy_text(h->layout_text, text, cap, needed);
+}
 
 uint64_t hibf_gpu_bin_size_in_bits(double fpr, uint64_t hash_count, uint64_t elements)
 {
@@ -1810,6 +1839,75 @@ int hibf_gpu_layout_compute_host_sketches(hibf_gpu_config const * c,
     }
 }
 
+namespace
+{
+// Host threads of the split-bin order replay (at most 16, the share of one GPU's host cores).
+unsigned build_threads()
+{
+    unsigned const hw = std::thread::hardware_concurrency();
+    return std::max(1u, std::min(16u, hw ? hw : 1u));
+}
+
+// The caller's user bins in their given order; tags index the sorted distinct copy on the device.
+class ArraySource final : public host::ValueSource
+{
+public:
+    ArraySource(uint64_t const * values, uint64_t const * offsets, std::vector<uint64_t> const & sorted,
+                std::vector<uint64_t> const & sorted_offsets, uint64_t n_user_bins) :
+        values_{values},
+        offsets_{offsets},
+        tags_(offsets[n_user_bins])
+    {
+        for (uint64_t u = 0; u < n_user_bins; ++u)
+        {
+            auto const b = sorted.begin() + static_cast<int64_t>(sorted_offsets[u]);
+            auto const e = sorted.begin() + static_cast<int64_t>(sorted_offsets[u + 1]);
+            for (uint64_t j = offsets[u]; j < offsets[u + 1]; ++j)
+                tags_[j] = static_cast<uint32_t>(std::lower_bound(b, e, values[j]) - b);
+        }
+    }
+    uint64_t read(uint32_t u, uint64_t pos, uint64_t cap, uint64_t * vals, uint32_t * tags) const override
+    {
+        uint64_t const n = offsets_[u + 1] - offsets_[u];
+        if (pos >= n)
+            return 0;
+        uint64_t const k = std::min(cap, n - pos);
+        std::memcpy(vals, values_ + offsets_[u] + pos, k * sizeof(uint64_t));
+        std::memcpy(tags, tags_.data() + offsets_[u] + pos, k * sizeof(uint32_t));
+        return k;
+    }
+
+private:
+    uint64_t const * values_;
+    uint64_t const * offsets_;
+    std::vector<uint32_t> tags_;
+};
+
+// The synthetic user bins: value i of user bin u is hibf_gpu_synth_value(seed, u, i).
+class SynthSource final : public host::ValueSource
+{
+public:
+    SynthSource(uint64_t seed, std::vector<uint64_t> const & sizes) : seed_{seed}, sizes_{sizes} {}
+    uint64_t read(uint32_t u, uint64_t pos, uint64_t cap, uint64_t * vals, uint32_t * tags) const override
+    {
+        uint64_t const n = sizes_[u];
+        if (pos >= n)
+            return 0;
+        uint64_t const k = std::min(cap, n - pos);
+        for (uint64_t j = 0; j < k; ++j)
+        {
+            vals[j] = hibf_gpu_synth_value(seed_, u, pos + j);
+            tags[j] = static_cast<uint32_t>(pos + j);
+        }
+        return k;
+    }
+
+private:
+    uint64_t seed_;
+    std::vector<uint64_t> const & sizes_;
+};
+} // namespace
+
 int hibf_gpu_hibf_build(hibf_gpu_config const * c,
                         uint64_t const * h_ub_offsets,
                         uint64_t const * h_values,
@@ -1865,11 +1963,21 @@ int hibf_gpu_hibf_build(hibf_gpu_config const * c,
         host::Layout layout;
         if (int rc = layout_for(hc, in.ub_size, layout_text, d_vals.u64(), d_offs.u64(), 0, layout, phase_s))
             return rc;
+        std::unique_ptr<ArraySource> source;
+        if (g_count_tuning.split_order)
+        {
+            for (uint64_t u = 0; u < nub; ++u)
+                if (in.ub_size[u] >= (1ULL << 32))
+                    return fail(HIBF_GPU_ERR_UNSUPPORTED, "a user bin of 2^32 distinct values or more");
+            source = std::make_unique<ArraySource>(h_values, h_ub_offsets, vals, offs, nub);
+            in.values = source.get();
+            in.threads = build_threads();
+        }
         host::Plan plan;
         std::string const pe = host::build_plan(hc, layout, in, plan);
         if (!pe.empty())
             return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, pe);
-        return run_plan(plan, nub, d_offs.u64(), d_vals.u64(), 0, phase_s, out);
+        return run_plan(plan, layout, nub, d_offs.u64(), d_vals.u64(), 0, phase_s, out);
     }
     catch (std::exception const & ex)
     {
@@ -1955,11 +2063,20 @@ int hibf_gpu_hibf_build_synthetic(hibf_gpu_config const * c,
         host::Layout layout;
         if (int rc = layout_for(hc, in.ub_size, layout_text, nullptr, nullptr, seed, layout, phase_s))
             return rc;
+        SynthSource const source{seed, in.ub_size};
+        if (g_count_tuning.split_order)
+        {
+            for (uint64_t u = 0; u < nub; ++u)
+                if (in.ub_size[u] >= (1ULL << 32))
+                    return fail(HIBF_GPU_ERR_UNSUPPORTED, "a user bin of 2^32 values or more");
+            in.values = &source;
+            in.threads = build_threads();
+        }
         host::Plan plan;
         std::string const pe = host::build_plan(hc, layout, in, plan);
         if (!pe.empty())
             return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, pe);
-        return run_plan(plan, nub, nullptr, nullptr, seed, phase_s, out);
+        return run_plan(plan, layout, nub, nullptr, nullptr, seed, phase_s, out);
     }
     catch (std::exception const & ex)
     {

@@ -75,7 +75,9 @@ __global__ void __launch_bounds__(256) segment_emplace_kernel(HibfIbfDesc const 
                                                               uint64_t n_chunks,
                                                               uint64_t const * __restrict__ values,
                                                               uint64_t const * __restrict__ ub_offsets,
-                                                              uint64_t seed)
+                                                              uint64_t seed,
+                                                              uint8_t const * __restrict__ chunk_ids,
+                                                              uint32_t chunk_id_bytes)
 {
     for (uint64_t c = blockIdx.x; c < n_chunks; c += gridDim.x)
     {
@@ -83,12 +85,18 @@ __global__ void __launch_bounds__(256) segment_emplace_kernel(HibfIbfDesc const 
         EmplaceSegment const sg = segs[ch.seg];
         HibfIbfDesc const d = desc[sg.ibf];
         uint64_t * const words = arena + d.word_offset;
-        uint64_t const word_in_row = sg.tb >> 6;
-        unsigned long long const bit = 1ULL << (sg.tb & 63u);
+        bool const chunked = (sg.flags & 1u) != 0u;
         uint64_t const base = values ? ub_offsets[sg.ub] : 0ULL;
         for (uint32_t t = threadIdx.x; t < ch.len; t += blockDim.x)
         {
-            uint64_t const i = sg.begin + ch.start + t;
+            // chunked: value i of the user bin, its technical bin from its chunk number
+            uint64_t const i = chunked ? ch.start + t : sg.begin + ch.start + t;
+            uint32_t tb = sg.tb;
+            if (chunked)
+                tb += chunk_id_bytes == 1u ? chunk_ids[sg.begin + i]
+                                           : reinterpret_cast<uint16_t const *>(chunk_ids)[sg.begin + i];
+            uint64_t const word_in_row = tb >> 6;
+            unsigned long long const bit = 1ULL << (tb & 63u);
             uint64_t const v = values ? values[base + i] : synth_value(seed, sg.ub, i);
 #pragma unroll
             for (int k = 0; k < 5; ++k)
@@ -338,13 +346,15 @@ hipError_t launch_segment_emplace(HibfIbfDesc const * desc,
                                   uint64_t const * values,
                                   uint64_t const * ub_offsets,
                                   uint64_t seed,
+                                  uint8_t const * chunk_ids,
+                                  uint32_t chunk_id_bytes,
                                   hipStream_t stream)
 {
     if (n_chunks == 0)
         return hipSuccess;
     unsigned const grid = static_cast<unsigned>(n_chunks < (1u << 20) ? n_chunks : (1u << 20));
     hipLaunchKernelGGL(segment_emplace_kernel, dim3(grid), dim3(256), 0, stream, desc, arena, segs, chunks, n_chunks,
-                       values, ub_offsets, seed);
+                       values, ub_offsets, seed, chunk_ids, chunk_id_bytes);
     return hipGetLastError();
 }
 

@@ -55,6 +55,7 @@ struct CountTuning
     uint32_t sliced_counts = 1; // HIBF descent over rows <= 4 words: bit-sliced per-lane counts
     uint32_t nt_stores = 1;     // count_wide_kernel and the HIBF counting zero fill: non-temporal (streaming) stores
     uint32_t hibf_block_waves = 4; // HIBF membership, query per block: waves per block (1, 2, 4)
+    uint32_t split_order = 1;      // HIBF build: split user bins chunked in the reference's set order (1) or by value ranges (0)
 };
 extern CountTuning g_count_tuning;
 
@@ -238,14 +239,16 @@ hipError_t launch_hibf_count(HibfLaunch const & h,
                              unsigned blocks,
                              hipStream_t stream);
 
-// Device form of a build-plan segment: values [begin, begin+count) of user
-// bin `ub` go to technical bin `tb` of IBF `ibf` (same layout as host::Segment).
+// Device form of a build-plan segment (same layout as host::Segment): values
+// [begin, begin+count) of user bin `ub` go to technical bin `tb` of IBF `ibf`;
+// with bit 0 of flags set (a split user bin chunked in the reference's set
+// order), value i of the user bin goes to technical bin tb + chunk_ids[begin + i].
 struct EmplaceSegment
 {
     uint32_t ibf;
     uint32_t tb;
     uint32_t ub;
-    uint32_t pad;
+    uint32_t flags;
     uint64_t begin;
     uint64_t count;
 };
@@ -260,7 +263,8 @@ struct EmplaceChunk
 
 // Executes a build plan: sets the bits of every (value, technical bin) pair of
 // every segment.  Values come from values[ub_offsets[ub] + i] or, when
-// values == nullptr, from synth_value(seed, ub, i).
+// values == nullptr, from synth_value(seed, ub, i).  chunk_ids (entries of
+// chunk_id_bytes, 1 or 2) serve the chunked segments.
 hipError_t launch_segment_emplace(HibfIbfDesc const * desc,
                                   uint64_t * arena,
                                   EmplaceSegment const * segs,
@@ -269,6 +273,8 @@ hipError_t launch_segment_emplace(HibfIbfDesc const * desc,
                                   uint64_t const * values,
                                   uint64_t const * ub_offsets,
                                   uint64_t seed,
+                                  uint8_t const * chunk_ids,
+                                  uint32_t chunk_id_bytes,
                                   hipStream_t stream);
 
 // out[offsets[q] + i] = src[start[q] + i] for i < count[q].

@@ -468,6 +468,17 @@ class DeviceHibf:
                 "next_ibf": nx, "user_bin": ub, "n_words": n_words.value, "n_user_bins": n_ub.value,
                 "depth": depth.value}
 
+    def layout_text(self) -> str:
+        """The layout this HIBF was built from (reference text format; empty
+        when it was created from tables or loaded)."""
+        need = ctypes.c_uint64()
+        rc = lib().hibf_gpu_hibf_layout(self.h, None, 0, ctypes.byref(need))
+        if rc not in (native.OK, native.ERR_CAPACITY):
+            check(rc)
+        buf = ctypes.create_string_buffer(int(need.value))
+        check(lib().hibf_gpu_hibf_layout(self.h, buf, need.value, ctypes.byref(need)))
+        return buf.value.decode()
+
     def __del__(self):
         _release(self, "hibf_gpu_hibf_destroy")
 

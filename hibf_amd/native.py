@@ -127,6 +127,7 @@ _SIGNATURES = {
     "hibf_gpu_hibf_device_arena": (_INT, [_V, ctypes.POINTER(_V)]),
     "hibf_gpu_hibf_ibf_view": (_INT, [_V, _U64, ctypes.POINTER(_V)]),
     "hibf_gpu_hibf_build_times": (_INT, [_V, _V]),
+    "hibf_gpu_hibf_layout": (_INT, [_V, ctypes.c_char_p, _U64, ctypes.POINTER(_U64)]),
     "hibf_gpu_hibf_layout_times": (_INT, [_V, _V]),
     "hibf_gpu_ibf_occupancy": (_INT, [_V, _V, ctypes.POINTER(_INT)]),
     "hibf_gpu_ibf_try_increase_bin_number_to": (_INT, [_V, _U64, ctypes.POINTER(_INT)]),

@@ -57,7 +57,12 @@ const char * hibf_gpu_last_error(void);
  * (non-temporal stores of the IBF count kernel's counters, of the HIBF
  * count's zero fill and of bulk_contains' rows: 1, default; plain stores 0),
  * "contains.unroll" (values whose rows one wave of bulk_contains loads at
- * once: 1, 2, 4 default, 8).  Results never depend on these values. */
+ * once: 1, 2, 4 default, 8).  Results never depend on these values.
+ * Two keys change what the library does: "build.split_order" (1 default:
+ * HIBF builds chunk split user bins in the reference's set order; 0: by value
+ * ranges) and "alloc.max_bytes" (largest single device allocation the
+ * library makes, 0 default = no limit; larger requests fail with
+ * HIBF_GPU_ERR_OUT_OF_MEMORY, for exercising those paths). */
 int hibf_gpu_tune_set(const char * key, int64_t value);
 int hibf_gpu_tune_get(const char * key, int64_t * value);
 
@@ -261,6 +266,14 @@ int hibf_gpu_hibf_download_arena(hibf_gpu_hibf hibf, uint64_t * h_arena, uint64_
 int hibf_gpu_hibf_arena_words(hibf_gpu_hibf hibf, uint64_t * n_words);
 /* Device address of the arena (IBF i's words start at word_offsets[i]). */
 int hibf_gpu_hibf_device_arena(hibf_gpu_hibf hibf, uint64_t ** d_arena);
+
+/* The layout this HIBF was built from, in the reference's layout text format
+ * (src/layout/layout.cpp:125-136): the given one or the one the build
+ * computed; empty for an HIBF created from tables or loaded.  No counterpart
+ * in the reference's HIBF, which does not keep its layout.  Writes the text
+ * and its terminating NUL; HIBF_GPU_ERR_CAPACITY + *needed when cap is too
+ * small. */
+int hibf_gpu_hibf_layout(hibf_gpu_hibf hibf, char * text, uint64_t cap, uint64_t * needed);
 
 /* Phase times of the build that made this HIBF, in seconds (all 0 for an HIBF
  * created from tables or loaded): [0] user-bin sketches and their estimates,
@@ -656,10 +669,16 @@ int hibf_gpu_layout_compute_host_sketches(const hibf_gpu_config * config,
 
 /* Replaces hierarchical_interleaved_bloom_filter(config&) and (config&, layout)
  * (src/hierarchical_interleaved_bloom_filter.cpp:211-253): user bin u holds
- * h_values[h_ub_offsets[u] .. h_ub_offsets[u+1]) (duplicates ignored).  The
- * layout is computed unless layout_text (reference layout format) is given.
- * IBF geometries and tables follow the reference's hierarchical build; the
- * bit matrices are filled by the device bulk emplace. */
+ * h_values[h_ub_offsets[u] .. h_ub_offsets[u+1]) in the order the reference's
+ * input_fn would insert them (duplicates ignored).  The layout is computed
+ * unless layout_text (reference layout format) is given.  IBF geometries and
+ * tables follow the reference's hierarchical build; the bit matrices are
+ * filled by the device bulk emplace.  A user bin split over several technical
+ * bins is chunked in the iteration order the reference's k-mer set
+ * (robin_hood::unordered_flat_set<uint64_t>) has after the same insertions,
+ * clears and merges (src/build/insert_into_ibf.cpp:28-47), replayed on the
+ * host; the tuning key "build.split_order" = 0 chunks by sorted value ranges
+ * instead (faster, matrices differ from the reference's). */
 int hibf_gpu_hibf_build(const hibf_gpu_config * c,
                         uint64_t const * h_ub_offsets,
                         uint64_t const * h_values,
@@ -667,7 +686,8 @@ int hibf_gpu_hibf_build(const hibf_gpu_config * c,
                         hibf_gpu_hibf * out);
 
 /* As hibf_gpu_hibf_build for synthetic user bins: user bin u holds
- * hibf_gpu_synth_value(seed, u, i) for i < h_ub_sizes[u] (pairwise distinct). */
+ * hibf_gpu_synth_value(seed, u, i) for i < h_ub_sizes[u] (pairwise distinct),
+ * inserted in order of i. */
 int hibf_gpu_hibf_build_synthetic(const hibf_gpu_config * c,
                                   uint64_t const * h_ub_sizes,
                                   uint64_t seed,

@@ -63,8 +63,12 @@ class OrcHibf(ctypes.Structure):
 
 
 def build() -> str:
-    """Compile liboracle.so with the committed Makefile."""
+    """Compile liboracle.so with the committed Makefile, and, where the
+    reference tree is present, oracle/_ref/robin_hood_ref (the reference's own
+    robin_hood set driven by ref_robin_hood.cpp)."""
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    if os.path.exists("/root/reference/include/hibf/contrib/robin_hood.hpp"):
+        subprocess.run(["make", "-s", "-C", _HERE, "ref"], check=True)
     return _LIB_PATH
 
 
@@ -121,6 +125,18 @@ def lib():
         L.orc_hll_union_estimates.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint64,
                                               ctypes.c_void_p, ctypes.c_void_p]
         L.orc_mt19937_64.argtypes = [ctypes.c_uint64, ctypes.c_void_p, ctypes.c_uint64]
+        L.orc_fs_new.restype = ctypes.c_void_p
+        L.orc_fs_new.argtypes = []
+        L.orc_fs_delete.argtypes = [ctypes.c_void_p]
+        L.orc_fs_size.restype = ctypes.c_uint64
+        L.orc_fs_size.argtypes = [ctypes.c_void_p]
+        L.orc_fs_clear.argtypes = [ctypes.c_void_p]
+        L.orc_fs_order.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        L.orc_fs_emplace_synth.argtypes = [ctypes.c_void_p] + [ctypes.c_uint64] * 4
+        L.orc_fs_emplace_values.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64]
+        L.orc_fs_insert_set.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        L.orc_fs_script.restype = ctypes.c_uint64
+        L.orc_fs_script.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_uint64]
         L.orc_synth_value.restype = ctypes.c_uint64
         L.orc_synth_value.argtypes = [ctypes.c_uint64] * 3
         L.orc_ibf_emplace_synthetic.argtypes = [ctypes.c_void_p, ctypes.POINTER(OrcIbf)] + [ctypes.c_uint64] * 4 + \
@@ -315,6 +331,53 @@ class Hibf:
         hits = np.zeros(nq, dtype=np.uint64)
         lib().orc_hibf_membership_count_batch(ctypes.byref(self.s), _p(v), _p(o), nq, _p(t), _p(hits), threads)
         return hits
+
+
+class FlatSet:
+    """The oracle's restatement of robin_hood::unordered_flat_set<uint64_t>, the
+    set whose iteration order the reference chunks split user bins in."""
+
+    def __init__(self):
+        self.p = ctypes.c_void_p(lib().orc_fs_new())
+
+    def __del__(self):
+        try:
+            lib().orc_fs_delete(self.p)
+        except Exception:
+            pass
+
+    def __len__(self) -> int:
+        return int(lib().orc_fs_size(self.p))
+
+    def insert_synth(self, seed: int, set_: int, begin: int, count: int) -> None:
+        lib().orc_fs_emplace_synth(self.p, seed, set_, begin, count)
+
+    def insert_values(self, values) -> None:
+        v = _u64(values)
+        lib().orc_fs_emplace_values(self.p, _p(v), v.size)
+
+    def insert_set(self, other: "FlatSet") -> None:
+        lib().orc_fs_insert_set(self.p, other.p)
+
+    def clear(self) -> None:
+        lib().orc_fs_clear(self.p)
+
+    def order(self) -> np.ndarray:
+        out = np.zeros(max(len(self), 1), dtype=np.uint64)
+        lib().orc_fs_order(self.p, _p(out))
+        return out[: len(self)]
+
+
+def flat_set_script(script: str) -> str:
+    """Runs a golden-order script (the protocol of oracle/ref_robin_hood.cpp)
+    on the oracle's restatement of robin_hood::unordered_flat_set<uint64_t>."""
+    cap = 1 << 16
+    while True:
+        buf = ctypes.create_string_buffer(cap)
+        need = lib().orc_fs_script(script.encode(), buf, cap)
+        if need <= cap:
+            return buf.value.decode()
+        cap = int(need)
 
 
 def synth_value(seed: int, bin: int, i: int) -> int:

@@ -32,8 +32,13 @@
  *                          stream must agree (tests/test_oracle.py).
  *   orc_ibf_emplace_*      parallel forms of orc_ibf_emplace: equal to it on
  *                          the same pairs (tests/test_oracle.py).
+ *   orc_fs_*               golden iteration orders written by the reference's
+ *                          own robin_hood set (tests/golden/robin_hood_orders.json)
+ *                          and, where the reference tree is present, that set
+ *                          run on random scripts (tests/test_split_order.py).
  */
 #include <math.h>
+#include <stdio.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -726,4 +731,399 @@ void orc_mt19937_64(uint64_t seed, uint64_t * out, uint64_t n)
         y ^= y >> 43;
         out[k] = y;
     }
+}
+
+/* ------------------------------------------------------------------------
+ * Split-bin chunk order.  The reference chunks a split user bin in the
+ * iteration order of a robin_hood::unordered_flat_set<uint64_t>
+ * (src/build/insert_into_ibf.cpp:28-47; robin_hood 3.11.5 vendored as
+ * include/hibf/contrib/robin_hood.hpp, Table<true, 80, uint64_t, void, ...>).
+ * orc_fs restates that table's slot order: keyToIdx (hash_int, then
+ * h *= multiplier, h ^= h >> 33; info = inc + ((h & 31) >> shift), home =
+ * (h >> 5) & mask), insertKeyPrepareEmptySpot, shiftUp, insert_move,
+ * rehashPowerOfTwo, initData, try_increase_info, increase_size and clear.
+ * Iteration is slot order.  Pinned by the golden orders the reference's own
+ * set writes (oracle/ref_robin_hood.cpp -> tests/golden/robin_hood_orders.json)
+ * and, where the reference tree is present, by running that program.
+ * ------------------------------------------------------------------------ */
+typedef struct orc_fs
+{
+    uint64_t * keys;
+    uint8_t * info; /* nb slots, sentinel, 7 padding bytes */
+    uint64_t mask, n, max_allowed, multiplier;
+    uint32_t inc, shift;
+} orc_fs;
+
+static uint64_t orc_fs_max_allowed(uint64_t slots)
+{
+    return slots <= UINT64_MAX / 100u ? slots * 80u / 100u : (slots / 100u) * 80u;
+}
+
+static uint64_t orc_fs_with_buffer(uint64_t slots)
+{
+    uint64_t const m = orc_fs_max_allowed(slots);
+    return slots + (m < 255u ? m : 255u);
+}
+
+void orc_fs_init(orc_fs * s)
+{
+    memset(s, 0, sizeof(*s));
+    s->multiplier = 0xC4CEB9FE1A85EC53ULL;
+    s->inc = 32u;
+}
+
+void orc_fs_free(orc_fs * s)
+{
+    free(s->keys);
+    free(s->info);
+    orc_fs_init(s);
+}
+
+static void orc_fs_key_to_idx(orc_fs const * s, uint64_t key, uint64_t * idx, uint32_t * info)
+{
+    uint64_t h = key;
+    h ^= h >> 33;
+    h *= 0xFF51AFD7ED558CCDULL; /* robin_hood::hash_int */
+    h ^= h >> 33;
+    h *= s->multiplier;
+    h ^= h >> 33;
+    *info = s->inc + (uint32_t)((h & 31u) >> s->shift);
+    *idx = (h >> 5) & s->mask;
+}
+
+static void orc_fs_init_data(orc_fs * s, uint64_t slots)
+{
+    uint64_t const nb = orc_fs_with_buffer(slots);
+    s->n = 0;
+    s->mask = slots - 1u;
+    s->max_allowed = orc_fs_max_allowed(slots);
+    s->keys = (uint64_t *)malloc(nb * sizeof(uint64_t));
+    s->info = (uint8_t *)calloc(nb + 8u, 1);
+    s->info[nb] = 1;
+    s->inc = 32u;
+    s->shift = 0;
+}
+
+static int orc_fs_try_increase_info(orc_fs * s)
+{
+    uint64_t nb;
+    if (s->inc <= 2u)
+        return 0;
+    s->inc >>= 1;
+    ++s->shift;
+    nb = orc_fs_with_buffer(s->mask + 1u);
+    for (uint64_t i = 0; i < nb; i += 8)
+    {
+        uint64_t v;
+        memcpy(&v, s->info + i, 8);
+        v = (v >> 1) & 0x7F7F7F7F7F7F7F7FULL;
+        memcpy(s->info + i, &v, 8);
+    }
+    s->info[nb] = 1;
+    s->max_allowed = orc_fs_max_allowed(s->mask + 1u);
+    return 1;
+}
+
+static void orc_fs_shift_up(orc_fs * s, uint64_t start, uint64_t insertion_idx)
+{
+    uint64_t idx = start;
+    s->keys[idx] = s->keys[idx - 1];
+    while (--idx != insertion_idx)
+        s->keys[idx] = s->keys[idx - 1];
+    idx = start;
+    while (idx != insertion_idx)
+    {
+        s->info[idx] = (uint8_t)(s->info[idx - 1] + s->inc);
+        if (s->info[idx] + s->inc > 0xFFu)
+            s->max_allowed = 0;
+        --idx;
+    }
+}
+
+static void orc_fs_insert_move(orc_fs * s, uint64_t key)
+{
+    uint64_t idx, insertion_idx;
+    uint32_t info;
+    uint8_t insertion_info;
+    if (s->max_allowed == 0 && !orc_fs_try_increase_info(s))
+        abort(); /* the set throws an overflow error here */
+    orc_fs_key_to_idx(s, key, &idx, &info);
+    while (info <= s->info[idx])
+    {
+        ++idx;
+        info += s->inc;
+    }
+    insertion_idx = idx;
+    insertion_info = (uint8_t)info;
+    if (insertion_info + s->inc > 0xFFu)
+        s->max_allowed = 0;
+    while (s->info[idx] != 0)
+    {
+        ++idx;
+        info += s->inc;
+    }
+    if (idx != insertion_idx)
+        orc_fs_shift_up(s, idx, insertion_idx);
+    s->keys[insertion_idx] = key;
+    s->info[insertion_idx] = insertion_info;
+    ++s->n;
+}
+
+static void orc_fs_rehash(orc_fs * s, uint64_t slots)
+{
+    uint64_t * const old_keys = s->keys;
+    uint8_t * const old_info = s->info;
+    uint64_t const old_nb = orc_fs_with_buffer(s->mask + 1u);
+    orc_fs_init_data(s, slots);
+    if (old_info && old_nb > 1u)
+        for (uint64_t i = 0; i < old_nb; ++i)
+            if (old_info[i] != 0)
+                orc_fs_insert_move(s, old_keys[i]);
+    free(old_keys);
+    free(old_info);
+}
+
+static void orc_fs_increase_size(orc_fs * s)
+{
+    if (s->mask == 0)
+    {
+        orc_fs_init_data(s, 8u);
+        return;
+    }
+    if (s->n < orc_fs_max_allowed(s->mask + 1u) && orc_fs_try_increase_info(s))
+        return;
+    if (s->n * 2u < orc_fs_max_allowed(s->mask + 1u))
+    {
+        s->multiplier += 0xC4CEB9FE1A85EC54ULL;
+        orc_fs_rehash(s, s->mask + 1u);
+    }
+    else
+    {
+        orc_fs_rehash(s, (s->mask + 1u) * 2u);
+    }
+}
+
+/* emplace(key): 1 when inserted, 0 when present. */
+int orc_fs_emplace(orc_fs * s, uint64_t key)
+{
+    for (int attempt = 0; attempt < 256; ++attempt)
+    {
+        uint64_t idx, insertion_idx;
+        uint32_t info, insertion_info;
+        orc_fs_key_to_idx(s, key, &idx, &info);
+        if (s->mask != 0)
+        {
+            while (info < s->info[idx])
+            {
+                ++idx;
+                info += s->inc;
+            }
+            while (info == s->info[idx])
+            {
+                if (s->keys[idx] == key)
+                    return 0;
+                ++idx;
+                info += s->inc;
+            }
+        }
+        if (s->n >= s->max_allowed)
+        {
+            orc_fs_increase_size(s);
+            continue;
+        }
+        insertion_idx = idx;
+        insertion_info = info;
+        if (insertion_info + s->inc > 0xFFu)
+            s->max_allowed = 0;
+        while (s->info[idx] != 0)
+        {
+            ++idx;
+            info += s->inc;
+        }
+        if (idx != insertion_idx)
+            orc_fs_shift_up(s, idx, insertion_idx);
+        s->keys[insertion_idx] = key;
+        s->info[insertion_idx] = (uint8_t)insertion_info;
+        ++s->n;
+        return 1;
+    }
+    abort(); /* the set throws an overflow error here */
+}
+
+void orc_fs_clear(orc_fs * s)
+{
+    uint64_t nb;
+    if (s->n == 0)
+        return;
+    s->n = 0;
+    nb = orc_fs_with_buffer(s->mask + 1u);
+    memset(s->info, 0, nb + 8u);
+    s->info[nb] = 1;
+    s->inc = 32u;
+    s->shift = 0;
+}
+
+/* Writes the keys in iteration order to out (s->n entries). */
+void orc_fs_order(orc_fs const * s, uint64_t * out)
+{
+    uint64_t k = 0;
+    if (s->mask == 0)
+        return;
+    for (uint64_t i = 0, nb = orc_fs_with_buffer(s->mask + 1u); i < nb; ++i)
+        if (s->info[i] != 0)
+            out[k++] = s->keys[i];
+}
+
+/* The golden-order script protocol of oracle/ref_robin_hood.cpp, run on
+ * orc_fs: sets 0..63.  Appends each dump line to out (NUL-terminated);
+ * returns the bytes needed (more than cap: out is cut). */
+uint64_t orc_fs_script(char const * script, char * out, uint64_t cap)
+{
+    orc_fs sets[64];
+    uint64_t used = 0;
+    char const * p = script;
+    for (int i = 0; i < 64; ++i)
+        orc_fs_init(&sets[i]);
+    if (cap)
+        out[0] = 0;
+    while (*p)
+    {
+        char op[16] = {0};
+        int a = 0, consumed = 0;
+        char const * eol = strchr(p, '\n');
+        uint64_t const len = eol ? (uint64_t)(eol - p) : strlen(p);
+        char * const line = (char *)malloc(len + 1u);
+        memcpy(line, p, len);
+        line[len] = 0;
+        p += len + (eol ? 1u : 0u);
+        if (sscanf(line, "%15s %d%n", op, &a, &consumed) < 2 || a < 0 || a >= 64)
+        {
+            free(line);
+            continue;
+        }
+        {
+            orc_fs * const s = &sets[a];
+            char const * rest = line + consumed;
+            if (!strcmp(op, "gen"))
+            {
+                unsigned long long seed, set, begin, count;
+                if (sscanf(rest, "%llu %llu %llu %llu", &seed, &set, &begin, &count) == 4)
+                    for (uint64_t j = 0; j < count; ++j)
+                        orc_fs_emplace(s, orc_synth_value(seed, set, begin + j));
+            }
+            else if (!strcmp(op, "genmod"))
+            {
+                unsigned long long seed, set, count, mod;
+                if (sscanf(rest, "%llu %llu %llu %llu", &seed, &set, &count, &mod) == 4 && mod)
+                    for (uint64_t j = 0; j < count; ++j)
+                        orc_fs_emplace(s, orc_synth_value(seed, set, j % mod));
+            }
+            else if (!strcmp(op, "vals"))
+            {
+                unsigned long long v;
+                int used_chars = 0;
+                while (sscanf(rest, "%llu%n", &v, &used_chars) == 1)
+                {
+                    orc_fs_emplace(s, v);
+                    rest += used_chars;
+                }
+            }
+            else if (!strcmp(op, "merge"))
+            {
+                int b = 0;
+                if (sscanf(rest, "%d", &b) == 1 && b >= 0 && b < 64 && b != a)
+                {
+                    uint64_t * const order = (uint64_t *)malloc((sets[b].n + 1u) * sizeof(uint64_t));
+                    uint64_t const n = sets[b].n;
+                    orc_fs_order(&sets[b], order);
+                    for (uint64_t j = 0; j < n; ++j)
+                        orc_fs_emplace(s, order[j]);
+                    free(order);
+                }
+            }
+            else if (!strcmp(op, "clear"))
+            {
+                orc_fs_clear(s);
+            }
+            else if (!strcmp(op, "dump"))
+            {
+                int full = 0;
+                uint64_t * const order = (uint64_t *)malloc((s->n + 1u) * sizeof(uint64_t));
+                uint64_t h = 1469598103934665603ULL;
+                char buf[64];
+                int w;
+                sscanf(rest, "%d", &full);
+                orc_fs_order(s, order);
+                for (uint64_t j = 0; j < s->n; ++j)
+                {
+                    h ^= order[j];
+                    h *= 1099511628211ULL;
+                }
+                w = snprintf(buf, sizeof buf, "dump %d %llu %llu %llu", a, (unsigned long long)s->n,
+                             (unsigned long long)s->mask, (unsigned long long)h);
+                if (used + (uint64_t)w < cap)
+                    memcpy(out + used, buf, (size_t)w + 1u);
+                used += (uint64_t)w;
+                if (full)
+                    for (uint64_t j = 0; j < s->n; ++j)
+                    {
+                        w = snprintf(buf, sizeof buf, " %llu", (unsigned long long)order[j]);
+                        if (used + (uint64_t)w < cap)
+                            memcpy(out + used, buf, (size_t)w + 1u);
+                        used += (uint64_t)w;
+                    }
+                if (used + 1u < cap)
+                    memcpy(out + used, "\n", 2);
+                used += 1u;
+                free(order);
+            }
+        }
+        free(line);
+    }
+    for (int i = 0; i < 64; ++i)
+        orc_fs_free(&sets[i]);
+    return used + 1u;
+}
+
+/* Heap-held sets for callers that keep several (the tests' restatement of the
+ * reference's build replays one per IBF). */
+orc_fs * orc_fs_new(void)
+{
+    orc_fs * s = (orc_fs *)malloc(sizeof(orc_fs));
+    orc_fs_init(s);
+    return s;
+}
+
+void orc_fs_delete(orc_fs * s)
+{
+    if (!s)
+        return;
+    orc_fs_free(s);
+    free(s);
+}
+
+uint64_t orc_fs_size(orc_fs const * s) { return s->n; }
+
+/* compute_kmers over generated values: synth(seed, set, begin + j), j < count. */
+void orc_fs_emplace_synth(orc_fs * s, uint64_t seed, uint64_t set, uint64_t begin, uint64_t count)
+{
+    for (uint64_t j = 0; j < count; ++j)
+        orc_fs_emplace(s, orc_synth_value(seed, set, begin + j));
+}
+
+void orc_fs_emplace_values(orc_fs * s, uint64_t const * values, uint64_t n)
+{
+    for (uint64_t j = 0; j < n; ++j)
+        orc_fs_emplace(s, values[j]);
+}
+
+/* update_parent_kmers: dst.insert(src.begin(), src.end()). */
+void orc_fs_insert_set(orc_fs * dst, orc_fs const * src)
+{
+    uint64_t * const order = (uint64_t *)malloc((src->n + 1u) * sizeof(uint64_t));
+    uint64_t const n = src->n;
+    orc_fs_order(src, order);
+    for (uint64_t j = 0; j < n; ++j)
+        orc_fs_emplace(dst, order[j]);
+    free(order);
 }

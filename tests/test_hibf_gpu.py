@@ -5,7 +5,7 @@ import numpy as np
 import pytest
 
 import oracle as O
-from hibf_helpers import expected_arena, oracle_of
+from hibf_helpers import expected_arena, explicit_split_chunks, oracle_of, synthetic_split_chunks
 
 pytestmark = pytest.mark.gpu
 
@@ -191,7 +191,8 @@ def test_hibf_parity_random(n_ub, sizes, config, descent_kernel):
     rng = np.random.default_rng(n_ub)
     ubs = _random_user_bins(rng, n_ub, *sizes)
     dev = DeviceHibf.build(ubs, **config)
-    assert np.array_equal(dev.download_arena(), expected_arena(dev, ubs)), "bit matrices differ from the plan"
+    assert np.array_equal(dev.download_arena(), expected_arena(dev, ubs, split_chunks=explicit_split_chunks(dev, ubs))), \
+        "bit matrices differ from the reference build's"
     ref = oracle_of(dev)
     qs = _mixed_queries(rng, ubs, 60, 120) + [np.zeros(0, dtype=np.uint64), ubs[0][:1]]
     thresholds = [int(t) for t in rng.integers(0, 60, size=len(qs))]
@@ -409,8 +410,10 @@ def test_hibf_synthetic_build_matches_oracle():
     seed = 77
     dev = DeviceHibf.build_synthetic(sizes, seed, tmax=64)
     ubs = [synth_values(seed, u, np.arange(s, dtype=np.uint64)) for u, s in enumerate(sizes)]
-    # split bins are chunked in generation order (the stream is distinct)
-    assert np.array_equal(dev.download_arena(), expected_arena(dev, ubs, keep_order=True))
+    # split bins are chunked in the reference's set order of the generated values
+    chunks = synthetic_split_chunks(dev, seed, sizes)
+    assert len(chunks) > 0
+    assert np.array_equal(dev.download_arena(), expected_arena(dev, ubs, keep_order=True, split_chunks=chunks))
     ref = oracle_of(dev)
     qs = _mixed_queries(rng, ubs, 80, 200)
     ts = [int(0.7 * len(q)) // 2 for q in qs]
@@ -529,3 +532,36 @@ def test_hibf_long_queries_wrap(descent_kernel):
         for th in (1, 3000):
             got = to_numpy(dev.bulk_count(values, offsets, [th] * len(qs), width))
             assert np.array_equal(got, ref.bulk_count_batch(values, offsets, [th] * len(qs), width)), (width, th)
+
+
+def test_split_bins_in_reference_set_order_with_duplicates_and_given_order():
+    """User bins handed over unsorted and with repeated values: the reference's
+    k-mer set sees them in that order (duplicates ignored), and every split
+    user bin's technical bins hold the chunks of that set's iteration order.
+    With build.split_order = 0 the same user bins are chunked by sorted value
+    ranges instead."""
+    from hibf_amd import native
+    from hibf_amd.device import DeviceHibf
+    _say("split bins: reference set order, duplicates, given order")
+    rng = np.random.default_rng(88)
+    ubs = []
+    for n in rng.integers(30, 3000, size=300):
+        v = rng.integers(0, 2**63, size=int(n), dtype=np.int64).astype(np.uint64)
+        ubs.append(np.concatenate([v, v[rng.integers(0, v.size, size=int(n) // 3)]]))  # repeats, unsorted
+    ubs[0] = np.concatenate([ubs[0], rng.integers(0, 2**63, size=40000, dtype=np.int64).astype(np.uint64)])
+    dev = DeviceHibf.build(ubs, tmax=64)
+    chunks = explicit_split_chunks(dev, ubs)
+    assert len(chunks) > 0
+    arena = dev.download_arena()
+    assert np.array_equal(arena, expected_arena(dev, ubs, split_chunks=chunks))
+    try:
+        native.tune_set("build.split_order", 0)
+        dev0 = DeviceHibf.build(ubs, layout_text=dev.layout_text(), tmax=64)
+    finally:
+        native.tune_set("build.split_order", 1)
+    arena0 = dev0.download_arena()
+    assert np.array_equal(arena0, expected_arena(dev0, ubs))
+    assert not np.array_equal(arena0, arena)
+    ref = oracle_of(dev)
+    qs = _mixed_queries(rng, ubs, 50, 150)
+    assert _membership_lists(dev, qs, [60] * len(qs)) == [ref.membership_for(q, 60) for q in qs]

@@ -19,7 +19,7 @@ import numpy as np
 import pytest
 
 import oracle as O
-from hibf_helpers import subtree_user_bins
+from hibf_helpers import layout_graph, node_split_chunks, subtree_user_bins
 
 pytestmark = pytest.mark.gpu
 
@@ -98,13 +98,60 @@ def hibf_at_scale():
     geoms = [O.geom(b, s, h) for (b, s, h) in t["geoms"]]
     ref = O.Hibf(geoms, arena, t["word_offsets"], t["tb_offsets"], t["next_ibf"], t["user_bin"], t["n_user_bins"])
     torch.cuda.synchronize()
-    return dict(dev=dev, t=t, sizes=sizes, seed=spec["build_seed"], arena=arena, ref=ref)
+    assert dev.layout_text() == layout
+    return dict(dev=dev, t=t, sizes=sizes, seed=spec["build_seed"], arena=arena, ref=ref, layout=layout)
 
 
-def _compare_ibf_columns(s, i, rng, max_merged=None):
-    """Columns of IBF i whose contents do not depend on the order in which a
-    split user bin's values are chunked (merged bins, single-bin user bins and
-    deleted bins) against the oracle's emplace of the same values."""
+def _ibf_paths(t):
+    """Technical-bin path from the root of every IBF of the exported tables."""
+    paths, stack = {0: ()}, [0]
+    while stack:
+        i = stack.pop()
+        lo = int(t["tb_offsets"][i])
+        for b in range(int(t["geoms"][i][0])):
+            if int(t["user_bin"][lo + b]) == O.BIN_MERGED:
+                j = int(t["next_ibf"][lo + b])
+                paths[j] = paths[i] + (b,)
+                stack.append(j)
+    return paths
+
+
+def _node_of(graph, path):
+    node = graph
+    for b in path:
+        node = next(c for c in node.children if c.parent_bin_index == b)
+    return node
+
+
+def _replay_values(graph, path, sizes):
+    """Values the reference's k-mer set of the IBF at `path` takes in up to its last split user bin."""
+    node = _node_of(graph, path)
+
+    def subtree(n):
+        return sum(int(sizes[u]) for u, _, _ in n.records) + sum(subtree(c) for c in n.children)
+    last = max([i for i, r in enumerate(node.records) if r[2] > 1], default=-1)
+    if last < 0 and not (node.fav is None and node.records and node.records[0][2] > 1):
+        return 0
+    v = 3 * subtree(node.children[node.fav]) if node.fav is not None else int(sizes[node.records[0][0]])
+    return v + sum(int(sizes[u]) for u, _, _ in node.records[:last + 1])
+
+
+def _synth_index(seed, u, n, keys):
+    from hibf_amd.synth import synth_values
+    vals = synth_values(seed, u, np.arange(int(n), dtype=np.uint64))
+    # keys are the set's iteration order of exactly these values: match the two sorted orders
+    assert keys.size == vals.size
+    out = np.empty(keys.size, dtype=np.int64)
+    out[np.argsort(keys)] = np.argsort(vals)
+    return out
+
+
+def _compare_ibf_columns(s, i, rng, max_merged=None, split_chunks=None):
+    """Columns of IBF i against the oracle's emplace of the same values:
+    merged bins, single-bin user bins and deleted bins, and, with split_chunks
+    (chunk numbers of each split user bin's values, as the reference's build
+    assigns them), the split user bins' technical bins."""
+    from hibf_amd.synth import synth_values
     t, sizes = s["t"], s["sizes"]
     bins, bin_size, h = (int(x) for x in t["geoms"][i])
     lo = int(t["tb_offsets"][i])
@@ -113,7 +160,7 @@ def _compare_ibf_columns(s, i, rng, max_merged=None):
     merged = [b for b in range(bins) if ub[b] == O.BIN_MERGED]
     if max_merged is not None and len(merged) > max_merged:
         merged = sorted(rng.choice(merged, size=max_merged, replace=False).tolist())
-    items, compared = [], []
+    items, compared, split_runs = [], [], []
     for b in merged:
         for u in subtree_user_bins(t, nx[b]):
             items.append((u, 0, int(sizes[u]), b))
@@ -134,32 +181,63 @@ def _compare_ibf_columns(s, i, rng, max_merged=None):
         if e - b == 1:
             items.append((u, 0, int(sizes[u]), b))
             compared.append(b)
+        elif split_chunks is not None and u in split_chunks:
+            split_runs.append((u, b, e))
+            compared.extend(range(b, e))
         b = e
     own = O.Ibf(bins, bin_size, h)
     if items:
         a = np.asarray(items, dtype=np.uint64)
         own.emplace_synthetic_items(s["seed"], a[:, 0], a[:, 1], a[:, 2], a[:, 3], threads=THREADS)
+    for u, b, e in split_runs:
+        c = split_chunks[u]
+        assert c.size == int(sizes[u]) and int(c.max()) == e - b - 1
+        own.emplace_parallel(synth_values(s["seed"], u, np.arange(int(sizes[u]), dtype=np.uint64)),
+                             np.uint64(b) + c.astype(np.uint64), threads=THREADS)
     w0 = int(t["word_offsets"][i])
     got = s["arena"][w0:w0 + own.words.size]
     bw = (bins + 63) // 64
-    return _columns_equal(got, own.words, bin_size, bw, _column_mask(compared, bw)), len(compared)
+    return _columns_equal(got, own.words, bin_size, bw, _column_mask(compared, bw)), len(compared), len(split_runs)
+
+
+def _split_chunks_of(s, path, max_splits=None):
+    sizes, seed = s["sizes"], s["seed"]
+    return node_split_chunks(s["layout"], path, lambda st, u: st.insert_synth(seed, u, 0, int(sizes[u])),
+                             lambda u, keys: _synth_index(seed, u, sizes[u], keys), max_splits)
 
 
 def test_hibf_at_scale_top_level_columns(hibf_at_scale):
+    """12 merged bins, every single-bin user bin and the first three split user
+    bins (in the reference's build order) of the top-level IBF.  Its split
+    bins come after its fullest bin, a merged bin: their chunks follow the
+    reference's k-mer set after it held that bin's whole subtree (5.5e7
+    values) and the split bins before them (1.9e7-2.3e7 values each)."""
     rng = np.random.default_rng(3)
-    _say("HIBF at scale: top-level columns (12 merged bins and every single-bin user bin)")
-    ok, n = _compare_ibf_columns(hibf_at_scale, 0, rng, max_merged=12)
-    assert n >= 12 and ok
+    _say("HIBF at scale: replay of the top level's k-mer set (favourite subtree, first split user bins)")
+    chunks = _split_chunks_of(hibf_at_scale, (), max_splits=3)
+    _say(f"HIBF at scale: top-level columns (12 merged bins, single-bin and {len(chunks)} split user bins)")
+    ok, n, n_split = _compare_ibf_columns(hibf_at_scale, 0, rng, max_merged=12, split_chunks=chunks)
+    assert n >= 12 and n_split == len(chunks) == 3 and ok
 
 
 def test_hibf_at_scale_sampled_lower_ibfs(hibf_at_scale):
+    """24 sampled lower IBFs whose split user bins the oracle replays in
+    seconds (at most 2e7 values through their k-mer set), all of their
+    columns, split bins included."""
     rng = np.random.default_rng(4)
-    n_ibfs = len(hibf_at_scale["t"]["geoms"])
-    pick = sorted(rng.choice(np.arange(1, n_ibfs), size=24, replace=False).tolist())
+    s = hibf_at_scale
+    paths = _ibf_paths(s["t"])
+    graph = layout_graph(s["layout"])
+    cheap = [i for i in range(1, len(s["t"]["geoms"])) if _replay_values(graph, paths[i], s["sizes"]) <= 20_000_000]
+    pick = sorted(rng.choice(cheap, size=24, replace=False).tolist())
     _say(f"HIBF at scale: columns of IBFs {pick}")
+    n_split = 0
     for i in pick:
-        ok, n = _compare_ibf_columns(hibf_at_scale, i, rng, max_merged=16)
+        chunks = _split_chunks_of(s, paths[i])
+        ok, _, k = _compare_ibf_columns(s, i, rng, max_merged=16, split_chunks=chunks)
+        n_split += k
         assert ok, i
+    assert n_split >= 10
 
 
 def test_hibf_at_scale_membership_and_count(hibf_at_scale):
