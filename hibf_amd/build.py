@@ -50,25 +50,6 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
-TOOLS = ["gather_ceiling.hip"]
-
-
-def build_tools(verbose: bool = False) -> list[str]:
-    """Builds the stand-alone measurement programs next to the library."""
-    out = []
-    for src in TOOLS:
-        exe = os.path.join(HERE, os.path.splitext(src)[0])
-        cmd = [HIPCC, *COMMON, *DEVICE, os.path.join(CSRC, src), "-o", exe]
-        if verbose:
-            print(" ".join(cmd), flush=True)
-        r = subprocess.run(cmd, capture_output=True, text=True)
-        if r.returncode != 0:
-            sys.stderr.write(r.stdout + r.stderr)
-            raise RuntimeError(f"hipcc failed on {src}")
-        out.append(exe)
-    return out
-
-
 CPP_TESTS = [os.path.join(ROOT, "tests", "cpp", "snippets.cpp"), os.path.join(ROOT, "tests", "cpp", "archive_roundtrip.cpp"),
              os.path.join(ROOT, "tests", "cpp", "hibf_members.cpp"), os.path.join(ROOT, "tests", "cpp", "hll_device.cpp"),
              os.path.join(ROOT, "tests", "cpp", "compute_layout.cpp")]
@@ -95,4 +76,3 @@ def build_cpp_tests(verbose: bool = False) -> list[str]:
 if __name__ == "__main__":
     print(build(verbose=True))
     print(build_cpp_tests(verbose=True))
-    print(build_tools(verbose=True))
