@@ -15,7 +15,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libhibf_gpu.so")
 
-HIP_SOURCES = ["ibf_kernels.hip", "count_wide.hip", "hibf_kernels.hip", "hibf_wave.hip", "serving.hip", "scan.hip",
+HIP_SOURCES = ["ibf_kernels.hip", "count_wide.hip", "hibf_kernels.hip", "hibf_wave.hip", "serving.hip", "server.hip", "scan.hip",
                "kmer.hip", "hll.hip", "hll_union.hip", "capi.hip"]
 CXX_SOURCES = ["layout.cpp", "build.cpp", "archive.cpp"]
 

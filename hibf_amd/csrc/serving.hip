@@ -23,97 +23,9 @@
 
 #include "hibf_gpu.h"
 #include "kernels.hpp"
+#include "serving_common.hpp"
 
-namespace hibf_amd
-{
-int set_error(int status, std::string msg);
-}
-
-namespace
-{
-
-// Status of a failed HIP call of the loop, with its detail for hibf_gpu_last_error.
-int serve_fail(hipError_t e, char const * what)
-{
-    int const status = (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) ? HIBF_GPU_ERR_OUT_OF_MEMORY
-                                                                                    : HIBF_GPU_ERR_DEVICE;
-    return hibf_amd::set_error(status, std::string("serving loop: ") + what + ": " + hipGetErrorString(e));
-}
-
-int serve_reject(int status, std::string msg)
-{
-    return hibf_amd::set_error(status, "serving loop: " + std::move(msg));
-}
-
-struct Slot
-{
-    hipStream_t stream{};
-    hipEvent_t done{};
-    hibf_gpu_agent agent{};
-    uint64_t * h_vals{}; // pinned staging
-    uint64_t * h_offs{};
-    uint64_t * h_hash_offs{}; // reads in: k-mer offsets of the batch, computed on the host
-    uint16_t * h_thr{};
-    uint64_t * h_ids{};
-    uint64_t * h_hs{};
-    uint64_t * h_hc{};
-    uint64_t * d_vals{};
-    uint64_t * d_offs{};
-    uint16_t * d_thr{};
-    uint64_t * d_ids{};
-    uint64_t * d_hs{};
-    uint64_t * d_hc{};
-    // reads in: the batch's k-mer hashes and their offsets
-    uint64_t * d_hashes{};
-    uint64_t * d_hash_offs{};
-    uint64_t q0{}, q1{};
-    uint64_t bytes{};
-    bool busy{false};
-};
-
-double now_s(std::chrono::steady_clock::time_point t0)
-{
-    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-}
-
-// True when both ends of [p, p + bytes) are pinned host memory known to HIP.
-bool host_pinned(void const * p, uint64_t bytes)
-{
-    if (!p || !bytes)
-        return false;
-    for (char const * q : {static_cast<char const *>(p), static_cast<char const *>(p) + bytes - 1})
-    {
-        hipPointerAttribute_t a{};
-        if (hipPointerGetAttributes(&a, q) != hipSuccess)
-        {
-            (void)hipGetLastError();
-            return false;
-        }
-        if (a.type != hipMemoryTypeHost)
-            return false;
-    }
-    return true;
-}
-
-// memcpy split over up to `threads` threads for copies of 8 MiB and more.
-void copy_host(void * dst, void const * src, uint64_t bytes, unsigned threads)
-{
-    if (bytes < (8ull << 20) || threads <= 1)
-    {
-        std::memcpy(dst, src, bytes);
-        return;
-    }
-    uint64_t const chunk = ((bytes + threads - 1) / threads + 4095u) / 4096u * 4096u;
-    std::vector<std::thread> pool;
-    for (uint64_t off = chunk; off < bytes; off += chunk)
-        pool.emplace_back([=] { std::memcpy(static_cast<char *>(dst) + off, static_cast<char const *>(src) + off,
-                                            std::min(chunk, bytes - off)); });
-    std::memcpy(dst, src, std::min(chunk, bytes));
-    for (std::thread & t : pool)
-        t.join();
-}
-
-} // namespace
+using namespace hibf_amd::serving;
 
 namespace
 {
@@ -176,26 +88,7 @@ int serve_impl(hibf_gpu_hibf hibf,
     auto cleanup = [&]()
     {
         for (Slot & s : slots)
-        {
-            if (s.stream)
-                (void)hipStreamSynchronize(s.stream);
-            for (void * d : {static_cast<void *>(s.d_vals), static_cast<void *>(s.d_offs), static_cast<void *>(s.d_thr),
-                             static_cast<void *>(s.d_ids), static_cast<void *>(s.d_hs), static_cast<void *>(s.d_hc),
-                             static_cast<void *>(s.d_hashes), static_cast<void *>(s.d_hash_offs)})
-                if (d)
-                    (void)hipFree(d);
-            for (void * hp : {static_cast<void *>(s.h_vals), static_cast<void *>(s.h_offs),
-                              static_cast<void *>(s.h_hash_offs), static_cast<void *>(s.h_thr),
-                              static_cast<void *>(s.h_ids), static_cast<void *>(s.h_hs), static_cast<void *>(s.h_hc)})
-                if (hp)
-                    (void)hipHostFree(hp);
-            if (s.done)
-                (void)hipEventDestroy(s.done);
-            if (s.stream)
-                (void)hipStreamDestroy(s.stream);
-            if (s.agent)
-                hibf_gpu_agent_destroy(s.agent);
-        }
+            slot_free(s);
     };
 #define SERVE_TRY(expr)                                                                                                \
     do                                                                                                                 \
@@ -210,109 +103,29 @@ int serve_impl(hibf_gpu_hibf hibf,
     } while (0)
     uint64_t const B = p.max_batch;
     uint64_t const V = p.max_batch_values;
-    // Queues one batch (requests qb .. qb + nb - 1, nv elements) on a slot: input copy, reads hashed
-    // on the device, membership launch, results back, completion event.
+    // Queues one batch (requests qb .. qb + nb - 1, nv elements) on a slot: offsets and thresholds staged,
+    // the elements copied from the pinned input directly or through the slot's pinned staging buffer.
     auto enqueue = [&](Slot & s, uint64_t const qb, uint64_t const nb, uint64_t const nv) -> int
     {
         uint64_t const base = h_offsets[qb];
         for (uint64_t i = 0; i <= nb; ++i)
             s.h_offs[i] = h_offsets[qb + i] - base;
-        hipError_t e = hipSuccess;
-        auto step = [&](hipError_t r)
-        {
-            if (e == hipSuccess)
-                e = r;
-        };
-        if (nv && pinned)
-            step(hipMemcpyAsync(s.d_vals, h_in + base * eb, nv * eb, hipMemcpyHostToDevice, s.stream));
-        else if (nv)
-        {
-            copy_host(s.h_vals, h_in + base * eb, nv * eb, copy_threads);
-            step(hipMemcpyAsync(s.d_vals, s.h_vals, nv * eb, hipMemcpyHostToDevice, s.stream));
-        }
-        step(hipMemcpyAsync(s.d_offs, s.h_offs, (nb + 1) * 8u, hipMemcpyHostToDevice, s.stream));
-        uint64_t const * d_v = s.d_vals;
-        uint64_t const * d_o = s.d_offs;
-        if (reads)
-        {
-            // A read of length L has max(0, L - k + 1) k-mers and the threshold
-            // uint16_t(fraction * k-mers) (test/tutorial/search.cpp:42): both from
-            // the lengths the host already has, so the device only hashes.
-            uint64_t total = 0;
-            s.h_hash_offs[0] = 0;
-            for (uint64_t i = 0; i < nb; ++i)
-            {
-                uint64_t const len = s.h_offs[i + 1] - s.h_offs[i];
-                uint64_t const windows = len >= k ? len - k + 1u : 0u;
-                total += windows;
-                s.h_hash_offs[i + 1] = total;
-                double const t = fraction * static_cast<double>(windows);
-                s.h_thr[i] = t >= 65535.0 ? uint16_t{65535} : t <= 0.0 ? uint16_t{0} : static_cast<uint16_t>(t);
-            }
-            step(hipMemcpyAsync(s.d_hash_offs, s.h_hash_offs, (nb + 1) * 8u, hipMemcpyHostToDevice, s.stream));
-            step(hipMemcpyAsync(s.d_thr, s.h_thr, nb * 2u, hipMemcpyHostToDevice, s.stream));
-            step(hibf_amd::kmer_hash_given_offsets_async(reinterpret_cast<uint8_t const *>(s.d_vals), s.d_offs, nb, k,
-                                                         s.d_hash_offs, s.d_hashes, s.stream));
-            d_v = s.d_hashes;
-            d_o = s.d_hash_offs;
-        }
-        else
-        {
+        if (!reads)
             std::memcpy(s.h_thr, h_thresholds + qb, nb * 2u);
-            step(hipMemcpyAsync(s.d_thr, s.h_thr, nb * 2u, hipMemcpyHostToDevice, s.stream));
-        }
-        if (e != hipSuccess)
+        char const * src = h_in + base * eb;
+        if (nv && !pinned)
         {
-            (void)hipGetLastError();
-            return serve_fail(e, "batch input copies");
+            copy_host(s.h_vals, src, nv * eb, copy_threads);
+            src = reinterpret_cast<char const *>(s.h_vals);
         }
-        if (int const r = hibf_gpu_hibf_membership_launch(hibf, s.agent, d_v, d_o, nb, s.d_thr, s.d_ids, nb * R,
-                                                          s.d_hs, s.d_hc, s.stream))
-            return r;
-        step(hipMemcpyAsync(s.h_hs, s.d_hs, nb * 8u, hipMemcpyDeviceToHost, s.stream));
-        step(hipMemcpyAsync(s.h_hc, s.d_hc, nb * 8u, hipMemcpyDeviceToHost, s.stream));
-        step(hipMemcpyAsync(s.h_ids, s.d_ids, nb * R * 8u, hipMemcpyDeviceToHost, s.stream));
-        step(hipEventRecord(s.done, s.stream));
-        if (e != hipSuccess)
-        {
-            (void)hipGetLastError();
-            return serve_fail(e, "batch result copies");
-        }
-        return HIBF_GPU_OK;
+        return slot_launch(s, hibf, nb, nv, src, eb, R, reads, k, fraction);
     };
     for (Slot & s : slots)
-    {
-        SERVE_TRY(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
-        SERVE_TRY(hipEventCreateWithFlags(&s.done, hipEventDisableTiming));
-        if (!pinned)
-            SERVE_TRY(hipHostMalloc(reinterpret_cast<void **>(&s.h_vals), std::max<uint64_t>(V * eb, 8u),
-                                    hipHostMallocDefault));
-        SERVE_TRY(hipHostMalloc(reinterpret_cast<void **>(&s.h_offs), (B + 1) * 8u, hipHostMallocDefault));
-        SERVE_TRY(hipHostMalloc(reinterpret_cast<void **>(&s.h_thr), B * 2u, hipHostMallocDefault));
-        SERVE_TRY(hipHostMalloc(reinterpret_cast<void **>(&s.h_ids), B * R * 8u, hipHostMallocDefault));
-        SERVE_TRY(hipHostMalloc(reinterpret_cast<void **>(&s.h_hs), B * 8u, hipHostMallocDefault));
-        SERVE_TRY(hipHostMalloc(reinterpret_cast<void **>(&s.h_hc), B * 8u, hipHostMallocDefault));
-        SERVE_TRY(hibf_amd::device_malloc(reinterpret_cast<void **>(&s.d_vals), std::max<uint64_t>(V * eb, 8u)));
-        if (reads)
-        {
-            SERVE_TRY(hibf_amd::device_malloc(reinterpret_cast<void **>(&s.d_hashes), V * 8u));
-            SERVE_TRY(hibf_amd::device_malloc(reinterpret_cast<void **>(&s.d_hash_offs), (B + 1) * 8u));
-            SERVE_TRY(hipHostMalloc(reinterpret_cast<void **>(&s.h_hash_offs), (B + 1) * 8u, hipHostMallocDefault));
-        }
-        SERVE_TRY(hibf_amd::device_malloc(reinterpret_cast<void **>(&s.d_offs), (B + 1) * 8u));
-        SERVE_TRY(hibf_amd::device_malloc(reinterpret_cast<void **>(&s.d_thr), B * 2u));
-        SERVE_TRY(hibf_amd::device_malloc(reinterpret_cast<void **>(&s.d_ids), B * R * 8u));
-        SERVE_TRY(hibf_amd::device_malloc(reinterpret_cast<void **>(&s.d_hs), B * 8u));
-        SERVE_TRY(hibf_amd::device_malloc(reinterpret_cast<void **>(&s.d_hc), B * 8u));
-        if ((rc = hibf_gpu_agent_create(B, &s.agent)) != HIBF_GPU_OK ||
-            (rc = hibf_gpu_hibf_membership_prepare(hibf, s.agent, B)) != HIBF_GPU_OK)
+        if ((rc = slot_alloc(s, hibf, B, V, R, eb, reads, !pinned)) != HIBF_GPU_OK)
         {
             cleanup();
             return rc;
         }
-        uint64_t v = 0, lk = 0, rb = 0;
-        (void)hibf_gpu_agent_stats(s.agent, &v, &lk, &rb, 1);
-    }
 
     // Warm the copy paths on every stream before the clock starts: the HIP runtime sets up a path the
     // first time it copies a size class, and that took about 10 ms inside the loop (a batch's latency).
@@ -419,10 +232,7 @@ int serve_impl(hibf_gpu_hibf hibf,
             {
                 uint64_t const i = q - s.q0;
                 uint64_t const n = s.h_hc[i];
-                uint64_t const cap = (s.q1 - s.q0) * R;
-                uint64_t keep = std::min<uint64_t>(n, R);
-                if (s.h_hs[i] > cap || keep > cap - s.h_hs[i])
-                    keep = 0; // the batch's id buffer was full: counted as overflowed
+                uint64_t const keep = slot_kept_ids(s, i, s.q1 - s.q0, R);
                 overflowed += (n > R || keep < std::min<uint64_t>(n, R));
                 h_counts[q] = static_cast<uint32_t>(n);
                 if (keep)

@@ -657,6 +657,62 @@ _NP_VIEW = {torch.uint8: (torch.uint8, np.uint8), torch.uint16: (torch.int16, np
             torch.uint32: (torch.int32, np.uint32), torch.uint64: (torch.int64, np.uint64)}
 
 
+class Server:
+    """The online serving loop (hibf_gpu_server_*): requests are submitted
+    while it runs, from any number of threads, and results are collected as
+    their batches finish.  The HIBF must outlive the server."""
+
+    def __init__(self, hibf: "DeviceHibf", window_us: float = 200.0, max_batch: int = 65536,
+                 max_batch_values: int = 1 << 24, budget_bytes: int = 4 << 30, streams: int = 4,
+                 results_per_query: int = 16):
+        _dev()
+        self._hibf = hibf
+        self.results_per_query = int(results_per_query)
+        p = native.ServeParams(window_us, max_batch, max_batch_values, budget_bytes, streams, results_per_query)
+        h = ctypes.c_void_p()
+        check(lib().hibf_gpu_server_create(hibf.h, ctypes.byref(p), ctypes.byref(h)))
+        self.h = h
+
+    def submit(self, values, threshold: int) -> int:
+        """Queues one request; returns its ticket."""
+        v = np.ascontiguousarray(np.asarray(values, dtype=np.uint64))
+        ticket = ctypes.c_uint64()
+        check(lib().hibf_gpu_server_submit(self.h, v.ctypes.data_as(ctypes.c_void_p), v.size, int(threshold),
+                                           ctypes.byref(ticket)))
+        return int(ticket.value)
+
+    def poll(self, max_results: int = 4096, wait_s: float = 0.0) -> list:
+        """Finished requests as (ticket, ids, count, latency_s); ids holds up to
+        results_per_query ids in the reference's depth-first order."""
+        R = self.results_per_query
+        res = (native.ServerResult * max(max_results, 1))()
+        ids = np.zeros((max(max_results, 1), R), dtype=np.uint64)
+        n = ctypes.c_uint64()
+        check(lib().hibf_gpu_server_poll(self.h, res, ids.ctypes.data_as(ctypes.c_void_p), max_results,
+                                         float(wait_s), ctypes.byref(n)))
+        return [(int(res[i].ticket), ids[i, :res[i].kept].tolist(), int(res[i].count), float(res[i].latency_s))
+                for i in range(n.value)]
+
+    def drain(self, wait_s: float = 60.0) -> None:
+        check(lib().hibf_gpu_server_drain(self.h, float(wait_s)))
+
+    def stats(self) -> dict:
+        st = native.ServeStats()
+        check(lib().hibf_gpu_server_stats(self.h, ctypes.byref(st)))
+        return {k: getattr(st, k) for k, _ in native.ServeStats._fields_}
+
+    def close(self) -> None:
+        h, self.h = getattr(self, "h", None), None
+        if h:
+            check(lib().hibf_gpu_server_destroy(h))
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def to_numpy(t: torch.Tensor) -> np.ndarray:
     """Host copy of an unsigned device tensor as the matching numpy dtype."""
     sview, ndt = _NP_VIEW[t.dtype]

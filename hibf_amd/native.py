@@ -65,6 +65,11 @@ class ServeParams(ctypes.Structure):
                 ("budget_bytes", ctypes.c_uint64), ("streams", ctypes.c_uint32), ("results_per_query", ctypes.c_uint32)]
 
 
+class ServerResult(ctypes.Structure):
+    _fields_ = [("ticket", ctypes.c_uint64), ("count", ctypes.c_uint32), ("kept", ctypes.c_uint32),
+                ("latency_s", ctypes.c_double)]
+
+
 class ServeStats(ctypes.Structure):
     _fields_ = [("seconds", ctypes.c_double), ("batches", ctypes.c_uint64), ("lookups", ctypes.c_uint64),
                 ("row_bytes", ctypes.c_uint64), ("max_in_flight_bytes", ctypes.c_uint64),
@@ -150,6 +155,12 @@ _SIGNATURES = {
                                    ctypes.POINTER(ServeStats)]),
     "hibf_gpu_hibf_serve_reads": (_INT, [_V, ctypes.POINTER(ServeParams), _V, _V, ctypes.c_uint32, ctypes.c_double,
                                          _V, _U64, _V, _V, _V, ctypes.POINTER(ServeStats)]),
+    "hibf_gpu_server_create": (_INT, [_V, ctypes.POINTER(ServeParams), ctypes.POINTER(_V)]),
+    "hibf_gpu_server_submit": (_INT, [_V, _V, _U64, ctypes.c_uint16, ctypes.POINTER(_U64)]),
+    "hibf_gpu_server_poll": (_INT, [_V, _V, _V, _U64, ctypes.c_double, ctypes.POINTER(_U64)]),
+    "hibf_gpu_server_drain": (_INT, [_V, ctypes.c_double]),
+    "hibf_gpu_server_stats": (_INT, [_V, ctypes.POINTER(ServeStats)]),
+    "hibf_gpu_server_destroy": (_INT, [_V]),
     "hibf_gpu_hibf_info": (_INT, [_V] + [ctypes.POINTER(_U64)] * 5),
     "hibf_gpu_hibf_export_tables": (_INT, [_V, ctypes.POINTER(Geometry), _V, _V, _V, _V]),
     "hibf_gpu_config_default": (_INT, [ctypes.POINTER(Config)]),

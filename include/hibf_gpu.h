@@ -409,6 +409,54 @@ int hibf_gpu_hibf_serve(hibf_gpu_hibf hibf,
                         uint32_t * h_counts,
                         hibf_gpu_serve_stats * stats);
 
+/* Online serving (no counterpart in the reference, whose agents answer one
+ * query per call): the serving loop above on a host thread of its own, fed by
+ * submissions while it runs instead of a trace.  create starts it with the
+ * same parameters (batching window, batch limits, admission budget, streams,
+ * ids kept per request) and allocates its streams and buffers; submit copies
+ * one request (n values and its membership threshold) and returns its ticket,
+ * from any number of host threads; the loop admits requests in submission
+ * order while their device memory fits the budget, batches them until the
+ * oldest has waited the window (or a batch is full) and answers each batch on
+ * a free stream.  poll returns finished requests in completion order: up to
+ * max_results records and, for record i, its first `kept` ids (the
+ * reference's depth-first order) at h_ids[i * results_per_query ..]; it waits
+ * up to wait_s seconds when none is ready.  drain waits up to wait_s seconds
+ * until every submitted request is answered (HIBF_GPU_ERR_CAPACITY if some
+ * are still outstanding then).  destroy answers what was submitted, stops
+ * the loop and frees everything; unpolled results are dropped.  Errors:
+ * submit returns HIBF_GPU_ERR_BUDGET for a request whose device memory
+ * (8 bytes per value, 34 bytes, 8 bytes per kept id) exceeds the budget and
+ * HIBF_GPU_ERR_INVALID_ARGUMENT for one longer than max_batch_values; after
+ * a device error in the loop, submit, poll and drain return it. */
+typedef struct hibf_gpu_server_s * hibf_gpu_server;
+
+typedef struct hibf_gpu_server_result
+{
+    uint64_t ticket;  /* as returned by hibf_gpu_server_submit                        */
+    uint32_t count;   /* ids of the request                                          */
+    uint32_t kept;    /* ids in h_ids: min(count, results_per_query), 0 if the batch's
+                         id buffer was full before them                              */
+    double latency_s; /* submission to result ready                                  */
+} hibf_gpu_server_result;
+
+int hibf_gpu_server_create(hibf_gpu_hibf hibf, const hibf_gpu_serve_params * params, hibf_gpu_server * out);
+int hibf_gpu_server_submit(hibf_gpu_server server,
+                           uint64_t const * h_values,
+                           uint64_t n,
+                           uint16_t threshold,
+                           uint64_t * ticket);
+int hibf_gpu_server_poll(hibf_gpu_server server,
+                         hibf_gpu_server_result * results,
+                         uint64_t * h_ids,
+                         uint64_t max_results,
+                         double wait_s,
+                         uint64_t * n_results);
+int hibf_gpu_server_drain(hibf_gpu_server server, double wait_s);
+/* Statistics since create (seconds from the first submission to the last answer). */
+int hibf_gpu_server_stats(hibf_gpu_server server, hibf_gpu_serve_stats * stats);
+int hibf_gpu_server_destroy(hibf_gpu_server server);
+
 /* The serving loop for reads (the tutorial search, test/tutorial/search.cpp:
  * 37-45, served online): request q is the read h_bases[h_read_offsets[q] ..
  * h_read_offsets[q+1]) (characters; dna4 reads anything but C/G/T/U as A);
