@@ -16,6 +16,7 @@
 #include <cstddef>
 #include <istream>
 #include <ostream>
+#include <stdexcept>
 #include <string>
 #include <string_view>
 #include <vector>
@@ -97,6 +98,8 @@ struct layout
         {
             // #LOWER_LEVEL_IBF_<i;j;...> fullest_technical_bin_idx:<id>
             size_t const first = prefix::layout_header.size() + prefix::layout_lower_level.size() + 1u;
+            if (line.size() < first)
+                malformed(line);
             size_t const space = line.find(' ', first);
             max_bins.push_back(max_bin{numbers(std::string_view{line}.substr(first, space - first)),
                                        number_after(line, prefix::layout_fullest_technical_bin_idx)});
@@ -105,18 +108,32 @@ struct layout
         {
             // <idx>\t<b;...;storage>\t<1;...;count>
             size_t const t1 = line.find('\t');
-            size_t const t2 = line.find('\t', t1 + 1);
+            size_t const t2 = t1 == std::string::npos ? t1 : line.find('\t', t1 + 1);
+            if (t2 == std::string::npos)
+                malformed(line);
+            std::vector<size_t> const idx = numbers(std::string_view{line}.substr(0, t1));
+            std::vector<size_t> bins = numbers(std::string_view{line}.substr(t1 + 1, t2 - t1 - 1));
+            std::vector<size_t> const counts = numbers(std::string_view{line}.substr(t2 + 1));
+            if (idx.empty() || bins.empty() || counts.empty())
+                malformed(line);
             user_bin u{};
-            u.idx = numbers(std::string_view{line}.substr(0, t1)).front();
-            u.previous_TB_indices = numbers(std::string_view{line}.substr(t1 + 1, t2 - t1 - 1));
-            u.storage_TB_id = u.previous_TB_indices.back();
-            u.previous_TB_indices.pop_back();
-            u.number_of_technical_bins = numbers(std::string_view{line}.substr(t2 + 1)).back();
+            u.idx = idx.front();
+            u.storage_TB_id = bins.back();
+            bins.pop_back();
+            u.previous_TB_indices = std::move(bins);
+            u.number_of_technical_bins = counts.back();
             user_bins.push_back(std::move(u));
         }
     }
 
 private:
+    // The reference asserts the format (src/layout/layout.cpp:31-47) and reads out of bounds without the
+    // assertions; here a line that does not have the format is an error.
+    [[noreturn]] static void malformed(std::string const & line)
+    {
+        throw std::invalid_argument{"[HIBF LAYOUT ERROR] Malformed layout line: " + line.substr(0, 80)};
+    }
+
     // The ';'-separated numbers of a field.
     static std::vector<size_t> numbers(std::string_view s)
     {

@@ -9,10 +9,6 @@
 //     first technical bin, values, a digest of its chunk numbers (the sum over
 //     values i of (chunk + 1) * splitmix64(i), mod 2^64; with `full` also the
 //     chunk numbers), then "seconds S".
-//   split_plan --layouts FILE...
-//     parses each file as a layout of 4 user bins of 100 values (tmax 64)
-//     and prints the plan's error or "ok"; used to run malformed layouts
-//     under the sanitizers.
 #include <chrono>
 #include <cinttypes>
 #include <cstdio>
@@ -67,39 +63,10 @@ std::string slurp(char const * path)
     return ss.str();
 }
 
-int layouts_mode(int argc, char ** argv)
-{
-    for (int i = 2; i < argc; ++i)
-    {
-        hibf_amd::host::Config c{};
-        c.number_of_user_bins = 4;
-        c.tmax = 64;
-        std::string const e = hibf_amd::host::validate_and_set_defaults(c);
-        hibf_amd::host::Layout layout;
-        if (!e.empty() || !hibf_amd::host::Layout::from_text(slurp(argv[i]), layout))
-        {
-            std::printf("%s: unparsed\n", argv[i]);
-            continue;
-        }
-        std::vector<uint64_t> sizes(4, 100);
-        Synth src{3, sizes};
-        hibf_amd::host::BuildInput in;
-        in.ub_size = sizes;
-        in.values = &src;
-        in.threads = 2;
-        hibf_amd::host::Plan plan;
-        std::string const pe = hibf_amd::host::build_plan(c, layout, in, plan);
-        std::printf("%s: %s\n", argv[i], pe.empty() ? "ok" : pe.c_str());
-    }
-    return 0;
-}
-
 } // namespace
 
 int main(int argc, char ** argv)
 {
-    if (argc >= 2 && std::string(argv[1]) == "--layouts")
-        return layouts_mode(argc, argv);
     if (argc < 6)
     {
         std::fprintf(stderr, "usage: split_plan LAYOUT SIZES SEED THREADS TMAX [full]\n");
