@@ -94,9 +94,15 @@ def parse():
                     help="launch-shape knob passed to hibf_gpu_tune_set (repeatable)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the library's default launch shapes instead of autotuning them on a sub-batch")
+    ap.add_argument("--hibf-tune-queries", type=int, default=0,
+                    help="hibf: queries of (b) the launch shapes are tuned on (default: one graph batch)")
     ap.add_argument("--autotune-queries", type=int, default=100_000,
-                    help="queries of the workload the autotuner times each candidate shape on")
+                    help="flat: queries of the workload the autotuner times each candidate shape on")
     ap.add_argument("--autotune-save", default=None, metavar="PATH", help="write the autotuner record as JSON")
+    ap.add_argument("--traffic-sweep", action="store_true",
+                    help="hibf: after the timed region, launch (a) and every batch of (b) once under each launch shape "
+                         "of the tuner's space (for rocprofv3 counter runs; a 1-element fill kernel marks each shape's (a) and (b))")
+    ap.add_argument("--sweep-rounds", type=int, default=1, help="hibf: rounds of --traffic-sweep (best time kept)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a 1 GiB elementwise copy kernel (known bytes) for counter calibration")
     return ap.parse_args()
@@ -105,7 +111,8 @@ def parse():
 def recorded_traffic(key: str, args, layout_text: str | None = None) -> float | None:
     """HBM bytes per launch measured with rocprofv3 counters on this exact
     workload (profiles/traffic.json), or None when the workload (or, for an
-    HIBF, the layout the record was measured on) differs."""
+    HIBF, the layout or the launch shape the record was measured with)
+    differs."""
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "traffic.json")
     try:
         with open(path) as f:
@@ -117,6 +124,14 @@ def recorded_traffic(key: str, args, layout_text: str | None = None) -> float | 
     if "layout_sha1" in rec and (layout_text is None or
                                  hashlib.sha1(layout_text.encode()).hexdigest()[:16] != rec["layout_sha1"]):
         return None
+    if "shapes" in rec:  # one row per launch shape (a shape can change the line count): the current one's
+        from hibf_amd import native
+        nk = len(rec["shape_keys"])
+        now = [native.tune_get(k) for k in rec["shape_keys"]]
+        row = next((r for r in rec["shapes"] if r[:nk] == now), None)
+        if row is None:
+            return None
+        rec = {**rec, **dict(zip(rec["shape_columns"], row[nk:]))}
     return (rec["fetch_kib"] * rec["fetch_factor"] + rec["write_kib"] * rec["write_factor"]) * 1024.0
 
 
@@ -558,11 +573,17 @@ def run_hibf(args):
     tuned = None
     if not args.no_autotune and not args.tune:
         from hibf_amd import autotune
-        nt = min(args.autotune_queries, nb)
-        rec = autotune.tune_hibf_membership(hibf, vb[: int(ob_h[nt])], ob[: nt + 1], tb[:nt], reps=2)
-        tuned = {"best": rec["best"], "best_ms": rec["best_ms"], "default_ms": rec["default_ms"],
-                 "queries": nt, "workload": "first queries of the log-normal batch", "candidates": len(rec["candidates"])}
-        log(f"autotune ({len(rec['candidates'])} shapes on {nt} queries of (b)): {rec['best']} "
+        # The launch form of the timed batches, on (b)'s first graph batch (or --hibf-tune-queries queries).
+        nt = min(args.hibf_tune_queries or args.graph_batch, nb)
+        ids_t = torch.empty(8 * nt, dtype=torch.uint64, device=dev)
+        hs_t = torch.empty(nt, dtype=torch.uint64, device=dev)
+        hc_t = torch.empty(nt, dtype=torch.uint64, device=dev)
+        rec = autotune.tune_hibf_membership_launch(hibf, agent, vb, ob[: nt + 1], tb[:nt], ids_t, hs_t, hc_t,
+                                                   reps=5)
+        del ids_t, hs_t, hc_t
+        tuned = {"best": rec["best"], "best_ms": rec["best_ms"], "default_ms": rec["default_ms"], "queries": nt,
+                 "workload": "the first graph batch of (b), launch form", "candidates": len(rec["candidates"])}
+        log(f"autotune ({len(rec['candidates'])} shapes on {nt} queries of (b), launch form): {rec['best']} "
             f"{rec['best_ms']:.3f} ms (default {rec['default_ms']:.3f} ms)")
         if args.autotune_save:
             rec["command"] = "python bench.py " + " ".join(sys.argv[1:])
@@ -638,6 +659,43 @@ def run_hibf(args):
     log(f"(b) {nb} log-normal queries ({int(ob_h[-1]) / nb:.0f} values/query), {nbatches} graphs: "
         f"{wall_b * 1e3:.1f} ms/step (kernels {kern_b * 1e3:.1f} ms), {lookups_b / wall_b / 1e9:.3f} G lookups/s, "
         f"{bytes_b / kern_b / 1e9:.0f} GB/s rows")
+
+    # Traffic sweep (for counter runs): (a)'s launch, then one direct launch of every batch of (b), under each
+    # launch shape of the tuner's space in turn; a one-element fill kernel marks the start of each shape's (a)
+    # launch and of its (b) batches.
+    # Each shape's (a) launch and (b) batches are timed with HIP events on the launch stream (best of
+    # --sweep-rounds rounds; the markers are outside the timed spans).
+    shape_sweep = None
+    if args.traffic_sweep:
+        from hibf_amd import autotune
+        saved = {k: native.tune_get(k) for k in autotune.HIBF_SPACE}
+        marker = torch.empty(1, dtype=torch.int64, device=dev)
+        shape_sweep = []
+        for si, knobs in enumerate(autotune._candidates(autotune.HIBF_SPACE)):
+            autotune._apply(knobs)
+            a_ms, b_ms = [], []
+            for _ in range(max(1, args.sweep_rounds)):
+                ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+                marker.fill_(si)
+                hibf.membership_prepare(agent, nq)
+                ev[0].record(stream)
+                hibf.membership_launch(agent, va, oa, ta, ids, hs, hc)
+                ev[1].record(stream)
+                hibf.membership_prepare(agent, G)
+                marker.fill_(1000 + si)  # and the start of its (b) batches
+                ev[2].record(stream)
+                for b in range(nbatches):
+                    q0, q1 = b * G, min(nb, (b + 1) * G)
+                    hibf.membership_launch(agent, vb, ob[q0:q1 + 1], tb[q0:q1], ids_b[b * cap_b:(b + 1) * cap_b],
+                                           hs_b[q0:q1], hc_b[q0:q1])
+                ev[3].record(stream)
+                torch.cuda.synchronize()
+                a_ms.append(ev[0].elapsed_time(ev[1]))
+                b_ms.append(ev[2].elapsed_time(ev[3]))
+            shape_sweep.append({"tuning": knobs, "a_ms": min(a_ms), "b_ms": min(b_ms)})
+            log(f"traffic sweep shape {si}: {knobs} (a) {min(a_ms):.3f} ms (b) {min(b_ms):.2f} ms")
+        autotune._apply(saved)
+        agent.stats(reset=True)
 
     # (d) counting agent: uint16 bulk_count over every user bin for a prefix of (a), t = 0.7 L as in (a).
     count_leg = None
@@ -849,6 +907,8 @@ def run_hibf(args):
                    "computed": layout_times or None},
         "sketches": sketch_leg,
     }
+    if shape_sweep is not None:
+        out["shape_sweep"] = shape_sweep
     if not args.no_flat_leg:
         del graphs, ids_b, hs_b, hc_b, vb, ob, tb, va, oa, ta, ids, hs, hc
         torch.cuda.empty_cache()

@@ -173,6 +173,43 @@ def tune_hibf_membership(hibf, values: torch.Tensor, offsets: torch.Tensor, thre
     return _tune(space or HIBF_SPACE, run, output, same, reps, workload)
 
 
+def tune_hibf_membership_launch(hibf, agent, values: torch.Tensor, offsets: torch.Tensor, thresholds: torch.Tensor,
+                                ids: torch.Tensor, hit_start: torch.Tensor, hit_count: torch.Tensor, reps: int = 5,
+                                space: dict | None = None) -> dict:
+    """Tunes the HIBF membership descent in its asynchronous launch form
+    (hibf_gpu_hibf_membership_launch, the form a hipGraph batch replays) on one
+    resident batch: the same kernels, grid and batch size as the caller's
+    timed launches, without the synchronous form's result download.  Outputs
+    are compared as per-query id lists: where a query's list lands in `ids`
+    follows the work-stealing schedule, its content and order do not."""
+    nq = offsets.numel() - 1
+    keys = list(space or HIBF_SPACE)
+    state = {"shape": None}
+
+    def run():
+        shape = tuple(native.tune_get(k) for k in keys)
+        if shape != state["shape"]:  # the per-block workspace depends on the shape
+            hibf.membership_prepare(agent, nq)
+            state["shape"] = shape
+        hibf.membership_launch(agent, values, offsets, thresholds, ids, hit_start, hit_count)
+
+    def output():
+        start = hit_start.view(torch.int64)
+        count = hit_count.view(torch.int64)
+        total = int(count.sum())
+        first = torch.cumsum(count, 0) - count  # each query's first position in the packed lists
+        pos = torch.repeat_interleave(start - first, count) + torch.arange(total, device=count.device)
+        return count.clone(), ids.view(torch.int64)[pos].clone()
+
+    def same(a, b):
+        return torch.equal(a[0], b[0]) and torch.equal(a[1], b[1])
+
+    tab = hibf.tables()
+    workload = {"kind": "hibf membership (launch form)", "ibfs": len(tab["geoms"]),
+                "user_bins": int(tab["n_user_bins"]), "queries": nq, "values": int(values.numel())}
+    return _tune(space or HIBF_SPACE, run, output, same, reps, workload)
+
+
 def save_record(record: dict, path: str) -> None:
     with open(path, "w") as f:
         json.dump(record, f, indent=1)

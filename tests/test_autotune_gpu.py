@@ -76,3 +76,49 @@ def test_tune_hibf_membership(restore_knobs):
     ids, hoff = to_numpy(ids), to_numpy(hoff)
     for q in range(len(qs)):
         assert ids[hoff[q]:hoff[q + 1]].tolist() == ref.membership_for(qs[q], 30)
+
+
+def test_tune_hibf_membership_launch_form(restore_knobs):
+    """The launch-form tuner (the form a hipGraph batch replays): every
+    candidate's per-query lists equal the default shape's (the tuner raises
+    otherwise), and the lists of the chosen shape equal the oracle's."""
+    import torch
+    from hibf_amd import autotune, native
+    from hibf_amd.device import Agent, DeviceHibf, as_device_u64, csr, to_numpy
+    from hibf_helpers import oracle_of
+    _say("autotune hibf membership, launch form, 700 user bins")
+    rng = np.random.default_rng(19)
+    ubs = [rng.integers(0, 2**63, size=int(n), dtype=np.int64).astype(np.uint64)
+           for n in rng.integers(20, 400, size=700)]
+    dev = DeviceHibf.build(ubs, tmax=64)
+    ref = oracle_of(dev)
+    qs = []
+    for _ in range(3000):
+        L = int(rng.integers(0, 300))
+        src = ubs[int(rng.integers(0, 700))]
+        k = int(0.8 * L)
+        qs.append(np.concatenate([src[rng.integers(0, src.size, k)],
+                                  rng.integers(0, 2**63, size=L - k, dtype=np.int64).astype(np.uint64)]))
+    v_h, o_h = csr(qs)
+    thr = np.array([max(1, int(0.7 * q.size)) for q in qs], dtype=np.uint16)
+    thr[::50] = 0  # some queries report every user bin
+    nq = len(qs)
+    t = torch.from_numpy(thr.view(np.int16)).cuda().view(torch.uint16)
+    ids = torch.empty(8 * nq + 64 * 700, dtype=torch.uint64, device="cuda")
+    hs = torch.empty(nq, dtype=torch.uint64, device="cuda")
+    hc = torch.empty(nq, dtype=torch.uint64, device="cuda")
+    agent = Agent()
+    rec = autotune.tune_hibf_membership_launch(dev, agent, as_device_u64(v_h), as_device_u64(o_h), t, ids, hs, hc,
+                                               reps=1)
+    assert len(rec["candidates"]) == len(autotune._candidates(autotune.HIBF_SPACE))
+    assert {k: native.tune_get(k) for k in autotune.HIBF_SPACE} == rec["best"]
+    dev.membership_prepare(agent, nq)
+    dev.membership_launch(agent, as_device_u64(v_h), as_device_u64(o_h), t, ids, hs, hc)
+    torch.cuda.synchronize()
+    total, dropped = agent.status()
+    assert not dropped
+    ids_n, hs_n, hc_n = to_numpy(ids), to_numpy(hs), to_numpy(hc)
+    assert int(hc_n.sum()) == total
+    for q in range(0, nq, 7):
+        got = ids_n[int(hs_n[q]):int(hs_n[q]) + int(hc_n[q])].tolist()
+        assert got == ref.membership_for(qs[q], int(thr[q])), q
