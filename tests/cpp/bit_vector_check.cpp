@@ -20,11 +20,13 @@
 #include <iterator>
 #include <random>
 #include <ranges>
+#include <sstream>
 #include <stdexcept>
 #include <utility>
 #include <vector>
 
 #include <hibf/misc/bit_vector.hpp>
+#include <hibf/misc/print.hpp>
 
 using seqan::hibf::bit_vector;
 
@@ -341,6 +343,17 @@ void size_empty_output()
     std::ranges::copy(std::vector<bool>(100, false), v.begin());
     EXPECT(std::ranges::none_of(v, [](bool b) { return b; }));
     EXPECT(v.any()); // bits 100..127 of the last word were set by the constructor and stay set
+
+    // print formats a bit_vector as 0/1 (src/misc/print.cpp prints bool as uint16_t), also after a
+    // caller set boolalpha on the stream, as the reference's snippets do before printing agent results.
+    bit_vector p(5);
+    p[1] = true;
+    p[4] = true;
+    std::ostringstream os;
+    os << std::boolalpha;
+    seqan::hibf::print(p, os);
+    seqan::hibf::print(bit_vector{}, os);
+    EXPECT(os.str() == "[0,1,0,0,1]\n[]\n");
     done("size_empty_output");
 }
 
