@@ -506,7 +506,7 @@ def run_hibf(args):
     hashing of reads; (f) the build-side sketches; then the serving loop at
     50k and 500k queries/s and the flat 8,192-bin IBF as extra legs."""
     import torch
-    from hibf_amd import native
+    from hibf_amd import autotune, native
     from hibf_amd import workloads as W
     from hibf_amd.device import Agent, DeviceHibf, as_device_u64
 
@@ -877,9 +877,7 @@ def run_hibf(args):
                                f"bin, t = uint16(0.7 L); persistent work-stealing descent in hipGraph batches of {G}",
                    "queries": nb, "values_per_step": int(ob_h[-1]), "graph_batch": G, "graphs": nbatches,
                    "ibfs": len(tab["geoms"]), "arena_gb": tab["n_words"] * 8 / 1e9, "depth": tab["depth"],
-                   "tuning": {k: native.tune_get(k) for k in ("hibf.wave_kernel", "hibf.row_per_lane",
-                                                               "hibf.sliced_counts", "hibf.nt_loads",
-                                                               "hibf.block_waves")},
+                   "tuning": {k: native.tune_get(k) for k in autotune.HIBF_SPACE},
                    "autotune": tuned},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": kernel_name,
@@ -929,7 +927,9 @@ def descent_kernel_name(native) -> str:
         return "hibf_descent_kernel<false, true, 0>"
     if native.tune_get("hibf.row_per_lane"):
         return "hibf_descent_membership_h2_row_per_lane_kernel"
-    return f"hibf_descent_membership_h2_w4_kernel<{'true' if native.tune_get('hibf.sliced_counts') else 'false'}>"
+    sliced = native.tune_get("hibf.sliced_counts")
+    unroll = native.tune_get("hibf.unroll") if sliced else 2
+    return f"hibf_descent_membership_h2_w4_kernel<{'true' if sliced else 'false'}, {unroll}>"
 
 
 def serve_sweep(hibf, tab, sizes, spec, args, rates, windows, seconds):

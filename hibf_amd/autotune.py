@@ -24,6 +24,8 @@ Search spaces:
                         one LDS atomic per set bit (0)
     hibf.nt_loads       non-temporal row loads (0, 1)
     hibf.block_waves    query per block: waves per block (4, 2, 8)
+    hibf.unroll         rows of <= 4 words, sliced counts: value groups whose
+                        rows a wave loads at once (2, 4)
 """
 from __future__ import annotations
 
@@ -52,6 +54,7 @@ HIBF_SPACE = {
     "hibf.sliced_counts": (1, 0),
     "hibf.nt_loads": (0, 1),
     "hibf.block_waves": (4, 2, 8),
+    "hibf.unroll": (4, 2),
 }
 
 

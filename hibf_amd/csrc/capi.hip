@@ -302,6 +302,8 @@ int hibf_gpu_tune_set(char const * key, int64_t value)
         g_count_tuning.row_per_lane = static_cast<uint32_t>(value);
     else if (k == "hibf.sliced_counts" && (value == 0 || value == 1))
         g_count_tuning.sliced_counts = static_cast<uint32_t>(value);
+    else if (k == "hibf.unroll" && (value == 2 || value == 4))
+        g_count_tuning.hibf_unroll = static_cast<uint32_t>(value);
     else if (k == "alloc.max_bytes" && value >= 0)
         g_alloc_max_bytes = static_cast<uint64_t>(value);
     else if (k == "build.split_order" && (value == 0 || value == 1))
@@ -338,6 +340,8 @@ int hibf_gpu_tune_get(char const * key, int64_t * value)
         *value = g_count_tuning.row_per_lane;
     else if (k == "hibf.sliced_counts")
         *value = g_count_tuning.sliced_counts;
+    else if (k == "hibf.unroll")
+        *value = g_count_tuning.hibf_unroll;
     else if (k == "alloc.max_bytes")
         *value = static_cast<int64_t>(g_alloc_max_bytes);
     else if (k == "build.split_order")
@@ -820,8 +824,8 @@ int hibf_gpu_hibf_create(uint64_t n_ibfs,
     if (!out || !geoms || !h_word_offsets || !h_tb_offsets || !h_next_ibf || !h_user_bin || n_ibfs == 0)
         return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "null argument or no IBF");
     *out = nullptr;
-    if (n_ibfs >= (1ULL << 32) || number_of_user_bins >= 0xFFFFFFFEULL)
-        return fail(HIBF_GPU_ERR_UNSUPPORTED, "too many IBFs or user bins for 32-bit device ids");
+    if (n_ibfs >= (1ULL << 31) || number_of_user_bins >= (1ULL << 31))
+        return fail(HIBF_GPU_ERR_UNSUPPORTED, "too many IBFs or user bins for 31-bit device ids");
     auto * h = new (std::nothrow) hibf_gpu_hibf_s{};
     if (!h)
         return fail(HIBF_GPU_ERR_OUT_OF_MEMORY, "host allocation failed");
@@ -1172,9 +1176,11 @@ int enqueue_membership(hibf_gpu_hibf h,
                        uint64_t ids_cap,
                        uint64_t * d_hit_start,
                        uint64_t * d_hit_count,
-                       hipStream_t s)
+                       hipStream_t s,
+                       uint32_t per_query_cap = 0u)
 {
-    HibfLaunch const L = launch_of(h, a);
+    HibfLaunch L = launch_of(h, a);
+    L.per_query_cap = per_query_cap;
     uint64_t const per_block = block_hit_list_bytes(h);
     HIBF_TRY_HIP(launch_reset_counters(a->d_total, kMembershipCounters, s), "counter reset");
     if (wave_kernel_fits(L))
@@ -1236,6 +1242,34 @@ int hibf_gpu_hibf_membership_launch(hibf_gpu_hibf h,
     return enqueue_membership(h, a, d_values, d_offsets, nq, d_thresholds, d_ids, ids_cap, d_hit_start, d_hit_count,
                               static_cast<hipStream_t>(stream));
 }
+
+// Library-internal (serving loops), so C++ linkage inside this extern "C" block.
+extern "C++" {
+namespace hibf_amd
+{
+int membership_launch_capped(hibf_gpu_hibf h,
+                             hibf_gpu_agent a,
+                             uint64_t const * d_values,
+                             uint64_t const * d_offsets,
+                             uint64_t nq,
+                             uint16_t const * d_thresholds,
+                             uint64_t * d_ids,
+                             uint32_t per_query_cap,
+                             uint64_t * d_hit_start,
+                             uint64_t * d_hit_count,
+                             hipStream_t stream)
+{
+    if (!h || !a || ((!d_offsets || !d_thresholds || !d_hit_start || !d_hit_count || !d_ids) && nq))
+        return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "null argument");
+    if (per_query_cap == 0u)
+        return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "per-query id cap must be > 0");
+    if (nq == 0)
+        return HIBF_GPU_OK;
+    return enqueue_membership(h, a, d_values, d_offsets, nq, d_thresholds, d_ids, nq * per_query_cap, d_hit_start,
+                              d_hit_count, stream, per_query_cap);
+}
+} // namespace hibf_amd
+} // extern "C++"
 
 int hibf_gpu_hibf_membership_launch_noalloc(hibf_gpu_hibf h,
                                             uint64_t const * d_values,

@@ -58,6 +58,11 @@ __device__ __forceinline__ void zero_counter_row(void * row, uint64_t n, uint32_
         store_counter(row, i, cb, 0ULL);
 }
 
+// Event entries of scan_ibf: the payload of the technical bin (user-bin id,
+// or child IBF of a merged bin) with kEvMerged set for merged bins, so that
+// processing an event reads nothing but LDS.  Device ids are below 2^31.
+constexpr uint32_t kEvMerged = 0x80000000u;
+
 // Scan the technical bins of the IBF just counted into `cnt` (one whole wave),
 // exactly as the loop of hierarchical_interleaved_bloom_filter.hpp:329-354:
 // the running sum adds every bin's count (deleted bins included), resets at a
@@ -65,6 +70,8 @@ __device__ __forceinline__ void zero_counter_row(void * row, uint64_t n, uint32_
 // iff the sum (modulo 2^(8*counter_bytes)) reaches the threshold.  Events are
 // appended to ev in technical-bin order.  In counting mode user-bin events are
 // written straight to the output row and only merged bins become events.
+// The table entries of up to four 64-bin chunks are loaded before the first
+// of them is scanned, so a 256-bin IBF costs one memory latency.
 // Returns the number of events.
 template <bool COUNTING>
 __device__ __forceinline__ uint32_t scan_ibf(HibfIbfDesc const & d,
@@ -79,53 +86,67 @@ __device__ __forceinline__ uint32_t scan_ibf(HibfIbfDesc const & d,
     uint64_t const mask = width_mask(counter_bytes);
     uint64_t carry = 0;
     uint32_t nev = 0;
-    for (uint32_t c0 = 0; c0 < d.bins; c0 += 64u)
+    for (uint32_t g0 = 0; g0 < d.bins; g0 += 256u)
     {
-        uint32_t const tb = c0 + static_cast<uint32_t>(lane);
-        bool const valid = tb < d.bins;
-        uint64_t const info = valid ? tb_info[d.tb_offset + tb] : 0ULL;
-        uint32_t const kind = static_cast<uint32_t>(info >> 32);
-        uint32_t const payload = static_cast<uint32_t>(info);
-        bool const reset = valid && (kind == kTbRunEnd || kind == kTbMerged);
-        uint64_t s = valid ? (static_cast<uint64_t>(cnt[tb]) & mask) : 0ULL;
-        // Segment heads: the bin after a reset point.
-        bool const prev_reset = __shfl_up(static_cast<int>(reset), 1, 64) != 0;
-        bool f = lane > 0 && prev_reset;
+        uint64_t infos[4];
 #pragma unroll
-        for (int dlt = 1; dlt < 64; dlt <<= 1)
+        for (int j = 0; j < 4; ++j)
         {
-            uint64_t const y = shfl64(s, lane >= dlt ? lane - dlt : lane);
-            bool const fy = __shfl(static_cast<int>(f), lane >= dlt ? lane - dlt : lane, 64) != 0;
-            if (lane >= dlt)
+            uint32_t const tb = g0 + 64u * static_cast<uint32_t>(j) + static_cast<uint32_t>(lane);
+            infos[j] = tb < d.bins ? tb_info[d.tb_offset + tb] : 0ULL;
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+        {
+            uint32_t const c0 = g0 + 64u * static_cast<uint32_t>(j);
+            if (c0 >= d.bins)
+                break;
+            uint32_t const tb = c0 + static_cast<uint32_t>(lane);
+            bool const valid = tb < d.bins;
+            uint64_t const info = infos[j];
+            uint32_t const kind = static_cast<uint32_t>(info >> 32);
+            uint32_t const payload = static_cast<uint32_t>(info);
+            bool const reset = valid && (kind == kTbRunEnd || kind == kTbMerged);
+            uint64_t s = valid ? (static_cast<uint64_t>(cnt[tb]) & mask) : 0ULL;
+            // Segment heads: the bin after a reset point.
+            bool const prev_reset = __shfl_up(static_cast<int>(reset), 1, 64) != 0;
+            bool f = lane > 0 && prev_reset;
+#pragma unroll
+            for (int dlt = 1; dlt < 64; dlt <<= 1)
             {
-                if (!f)
-                    s += y;
-                f = f || fy;
+                uint64_t const y = shfl64(s, lane >= dlt ? lane - dlt : lane);
+                bool const fy = __shfl(static_cast<int>(f), lane >= dlt ? lane - dlt : lane, 64) != 0;
+                if (lane >= dlt)
+                {
+                    if (!f)
+                        s += y;
+                    f = f || fy;
+                }
             }
+            if (!f)
+                s += carry;
+            uint64_t const sum = s & mask;
+            bool const hit = reset && sum >= threshold;
+            bool is_event = hit;
+            if constexpr (COUNTING)
+            {
+                // A user bin whose run is cut (by a deleted bin) keeps the sum of its last run that reached the
+                // threshold (hierarchical_interleaved_bloom_filter.hpp:488-491): of two such run ends in this
+                // chunk, the higher bin stores.
+                if (highest_lane_per_id(hit && kind == kTbRunEnd, payload, lane))
+                    store_counter(count_row, payload, counter_bytes, sum);
+                is_event = hit && kind == kTbMerged;
+            }
+            uint64_t const ballot = __ballot(is_event);
+            uint32_t const before = static_cast<uint32_t>(__popcll(ballot & ((1ULL << lane) - 1ULL)));
+            if (is_event)
+                ev[nev + before] = payload | (kind == kTbMerged ? kEvMerged : 0u);
+            nev += static_cast<uint32_t>(__popcll(ballot));
+            // Pending sum after this chunk: lane 63's segment sum unless it is a reset point.
+            uint64_t const last_s = shfl64(s, 63);
+            bool const last_reset = __shfl(static_cast<int>(reset), 63, 64) != 0;
+            carry = last_reset ? 0ULL : last_s;
         }
-        if (!f)
-            s += carry;
-        uint64_t const sum = s & mask;
-        bool const hit = reset && sum >= threshold;
-        bool is_event = hit;
-        if constexpr (COUNTING)
-        {
-            // A user bin whose run is cut (by a deleted bin) keeps the sum of its last run that reached the
-            // threshold (hierarchical_interleaved_bloom_filter.hpp:488-491): of two such run ends in this chunk,
-            // the higher bin stores.
-            if (highest_lane_per_id(hit && kind == kTbRunEnd, payload, lane))
-                store_counter(count_row, payload, counter_bytes, sum);
-            is_event = hit && kind == kTbMerged;
-        }
-        uint64_t const ballot = __ballot(is_event);
-        uint32_t const before = static_cast<uint32_t>(__popcll(ballot & ((1ULL << lane) - 1ULL)));
-        if (is_event)
-            ev[nev + before] = tb;
-        nev += static_cast<uint32_t>(__popcll(ballot));
-        // Pending sum after this chunk: lane 63's segment sum unless it is a reset point.
-        uint64_t const last_s = shfl64(s, 63);
-        bool const last_reset = __shfl(static_cast<int>(reset), 63, 64) != 0;
-        carry = last_reset ? 0ULL : last_s;
     }
     return nev;
 }

@@ -14,9 +14,8 @@ namespace
 struct DescentState
 {
     uint32_t nev[kHibfMaxDepth];    // events recorded at each level
-    uint32_t pos[kHibfMaxDepth];    // next event to process at each level
+    uint32_t pos[kHibfMaxDepth];    // next event of each level above the current one
     uint32_t ibf_at[kHibfMaxDepth]; // IBF visited at each level
-    uint32_t hits;                  // ids found for the current query
     uint64_t start;                 // output slot of the current query
 };
 
@@ -90,11 +89,14 @@ __device__ __forceinline__ void descent_body(HibfLaunch const & h,
         {
             zero_counter_row(row, h.n_user_bins, cb, h.nt_stores != 0u, tid, nthreads);
         }
-        if (tid == 0)
-            st.hits = 0;
-
+        // Every thread walks the same events, so the walk's state (level, IBF, next event, ids found) is
+        // kept in registers; only a level's resume point goes through LDS, written by thread 0 when the
+        // walk descends and read by all threads when it returns.
         uint32_t depth = 0;
         uint32_t ibf = 0;
+        uint32_t p = 0;
+        uint32_t nev = 0;
+        uint32_t hits = 0;
         bool descend = true;
         while (true)
         {
@@ -120,71 +122,73 @@ __device__ __forceinline__ void descent_body(HibfLaunch const & h,
                     uint32_t const n =
                         scan_ibf<COUNTING>(d, h.tb_info, cnt, threshold, cb, evs + depth * h.max_bins, row, lane);
                     if (lane == 0)
-                    {
                         st.nev[depth] = n;
-                        st.pos[depth] = 0;
-                        st.ibf_at[depth] = ibf;
-                    }
                 }
                 __syncthreads();
+                nev = st.nev[depth];
+                p = 0;
                 descend = false;
             }
-            // Next event of the current level (uniform across the block).
-            uint32_t const p = st.pos[depth];
-            if (p < st.nev[depth])
+            if (p < nev)
             {
-                uint32_t const tb = evs[depth * h.max_bins + p];
-                HibfIbfDesc const & d = h.ibfs[ibf];
-                uint64_t const info = h.tb_info[d.tb_offset + tb];
-                uint32_t const kind = static_cast<uint32_t>(info >> 32);
-                uint32_t const payload = static_cast<uint32_t>(info);
-                __syncthreads();
-                if (tid == 0)
+                // Next event of the current level, in technical-bin order.
+                uint32_t const e = evs[depth * h.max_bins + p];
+                ++p;
+                if (e & kEvMerged)
                 {
-                    st.pos[depth] = p + 1u;
-                    if (!COUNTING && kind != kTbMerged)
+                    if (tid == 0)
                     {
-                        // max_hits bounds the run ends of one descent; the check only guards the list.
-                        if (st.hits < h.max_hits)
-                        {
-                            my_scratch[st.hits] = payload;
-                            st.hits = st.hits + 1u;
-                        }
+                        st.pos[depth] = p;
+                        st.nev[depth] = nev;
+                        st.ibf_at[depth] = ibf;
+                    }
+                    ++depth;
+                    ibf = e & ~kEvMerged;
+                    descend = true;
+                }
+                else if constexpr (!COUNTING)
+                {
+                    // max_hits bounds the run ends of one descent; the check only guards the list.
+                    if (tid == 0)
+                    {
+                        if (hits < h.max_hits)
+                            my_scratch[hits] = e;
                         else
                             h.counters[5] = 1ULL;
                     }
-                }
-                __syncthreads();
-                if (kind == kTbMerged)
-                {
-                    ++depth;
-                    ibf = payload;
-                    descend = true;
+                    ++hits;
                 }
                 continue;
             }
-            // Level exhausted: return to the parent.
+            // Level exhausted: return to the parent.  The barrier keeps thread 0 from overwriting the parent's
+            // resume point (at its next merged bin) before every thread has read it.
             if (depth == 0)
                 break;
             --depth;
             ibf = st.ibf_at[depth];
+            p = st.pos[depth];
+            nev = st.nev[depth];
+            __syncthreads();
         }
 
         if constexpr (!COUNTING)
         {
+            hits = min(hits, static_cast<uint32_t>(h.max_hits));
             __syncthreads();
+            uint32_t const cap = h.per_query_cap;
             if (tid == 0)
             {
-                uint32_t const n = st.hits;
-                uint64_t const s = atomicAdd(total, static_cast<unsigned long long>(n));
+                uint64_t const found = atomicAdd(total, static_cast<unsigned long long>(hits));
+                uint64_t const s = cap ? q * cap : found;
+                uint32_t const kept = cap ? min(hits, cap) : hits;
                 st.start = s;
                 hit_start[q] = s;
-                hit_count[q] = n;
-                if (s + n > ids_cap)
+                hit_count[q] = hits;
+                if (s + kept > ids_cap)
                     h.counters[5] = 1ULL;
             }
             __syncthreads();
-            uint32_t const n = st.hits;
+            uint32_t const n = cap ? min(hits, cap) : hits;
             uint64_t const s = st.start;
             for (uint32_t i = static_cast<uint32_t>(tid); i < n; i += static_cast<uint32_t>(nthreads))
                 if (s + i < ids_cap)
@@ -232,11 +236,13 @@ hibf_descent_membership_h2_row_per_lane_kernel(HIBF_DESCENT_PARAMS)
 // Membership with two hash functions over rows of at most 4 words (the
 // tmax = 256 layout): only the 1-, 2- and 4-word counting shapes are compiled.
 // SLICE: bit-sliced per-lane counts with one LDS atomic per bin and chunk.
-template <bool SLICE>
+// U: value groups whose rows a wave loads at once (2: 32 values of 4-word rows,
+// 4: 64 values, a whole chunk per memory round trip).
+template <bool SLICE, int U = 2>
 __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8, 8)))
 hibf_descent_membership_h2_w4_kernel(HIBF_DESCENT_PARAMS)
 {
-    descent_body<false, false, 2, 2, 1, SLICE>(HIBF_DESCENT_ARGS);
+    descent_body<false, false, 2, U, 1, SLICE>(HIBF_DESCENT_ARGS);
 }
 
 #undef HIBF_DESCENT_PARAMS
@@ -415,8 +421,10 @@ hipError_t launch_hibf_membership(HibfLaunch const & h,
     bool const w4 = h2 && h.max_bin_words <= 4u;
     auto const kernel = g_count_tuning.nt_loads                ? hibf_descent_kernel<false, true, 0>
                         : w4 && g_count_tuning.row_per_lane     ? hibf_descent_membership_h2_row_per_lane_kernel
-                        : w4 && g_count_tuning.sliced_counts   ? hibf_descent_membership_h2_w4_kernel<true>
-                        : w4                                   ? hibf_descent_membership_h2_w4_kernel<false>
+                        : w4 && g_count_tuning.sliced_counts && g_count_tuning.hibf_unroll == 4u
+                                                               ? hibf_descent_membership_h2_w4_kernel<true, 4>
+                        : w4 && g_count_tuning.sliced_counts   ? hibf_descent_membership_h2_w4_kernel<true, 2>
+                        : w4                                   ? hibf_descent_membership_h2_w4_kernel<false, 2>
                         : h2                                   ? hibf_descent_membership_h2_kernel
                                                                : hibf_descent_kernel<false, false, 0>;
     unsigned const threads = 64u * g_count_tuning.hibf_block_waves;

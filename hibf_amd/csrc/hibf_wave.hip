@@ -201,15 +201,13 @@ __device__ __forceinline__ void wave_body(HibfLaunch const & h,
             uint32_t const p = pos[depth];
             if (p < nev[depth])
             {
-                uint32_t const tb = evs[depth * h.max_bins + p];
-                HibfIbfDesc const & d = h.ibfs[ibf];
-                uint64_t const info = h.tb_info[d.tb_offset + tb];
-                uint32_t const kind = static_cast<uint32_t>(info >> 32);
-                uint32_t const payload = static_cast<uint32_t>(info);
+                uint32_t const e = evs[depth * h.max_bins + p];
+                bool const merged = (e & kEvMerged) != 0u;
+                uint32_t const payload = e & ~kEvMerged;
                 if (lane == 0)
                 {
                     pos[depth] = p + 1u;
-                    if (!COUNTING && kind != kTbMerged)
+                    if (!COUNTING && !merged)
                     {
                         if (hits < kWaveResults)
                             res[hits] = payload;
@@ -217,10 +215,10 @@ __device__ __forceinline__ void wave_body(HibfLaunch const & h,
                             spilled = true;
                     }
                 }
-                if (!COUNTING && kind != kTbMerged)
+                if (!COUNTING && !merged)
                     ++hits;
                 wave_sync();
-                if (kind == kTbMerged)
+                if (merged)
                 {
                     ++depth;
                     ibf = payload;
@@ -235,7 +233,10 @@ __device__ __forceinline__ void wave_body(HibfLaunch const & h,
         }
         if constexpr (!COUNTING)
         {
-            spilled = hits > kWaveResults;
+            // With a per-query cap that the result buffer holds, the first cap ids are all that is kept.
+            uint32_t const cap = h.per_query_cap;
+            spilled = hits > kWaveResults && !(cap != 0u && cap <= kWaveResults);
+            uint32_t const kept = cap ? min(hits, cap) : hits;
             uint64_t start = 0;
             if (lane == 0)
             {
@@ -249,16 +250,17 @@ __device__ __forceinline__ void wave_body(HibfLaunch const & h,
                 }
                 else
                 {
-                    start = atomicAdd(&h.counters[0], static_cast<unsigned long long>(hits));
+                    uint64_t const found = atomicAdd(&h.counters[0], static_cast<unsigned long long>(hits));
+                    start = cap ? q * cap : found;
                     hit_start[q] = start;
                     hit_count[q] = hits;
-                    if (start + hits > ids_cap)
+                    if (start + kept > ids_cap)
                         h.counters[5] = 1ULL;
                 }
             }
             start = readlane64(start, 0);
             if (!spilled)
-                for (uint32_t i = static_cast<uint32_t>(lane); i < hits; i += 64u)
+                for (uint32_t i = static_cast<uint32_t>(lane); i < kept; i += 64u)
                     if (start + i < ids_cap)
                         ids[start + i] = res[i];
             wave_sync();

@@ -55,6 +55,7 @@ struct CountTuning
     uint32_t sliced_counts = 1; // HIBF descent over rows <= 4 words: bit-sliced per-lane counts
     uint32_t nt_stores = 1;     // count_wide_kernel and the HIBF counting zero fill: non-temporal (streaming) stores
     uint32_t hibf_block_waves = 4; // HIBF membership, query per block: waves per block (1, 2, 4)
+    uint32_t hibf_unroll = 4;      // HIBF membership over rows <= 4 words, sliced counts: value groups in flight (2, 4)
     uint32_t split_order = 1;      // HIBF build: split user bins chunked in the reference's set order (1) or by value ranges (0)
 };
 extern CountTuning g_count_tuning;
@@ -174,7 +175,11 @@ struct HibfLaunch
     uint32_t max_bin_words;   // widest row of this HIBF (words)
     uint32_t uniform_hash_funs; // hash functions of every IBF, 0 if they differ
     uint32_t nt_stores;       // counting: non-temporal stores of the zero fill
-    uint32_t pad;
+    // Membership output form.  0: query q's ids go to ids[hit_start[q] ..) with
+    // hit_start taken in atomic order, so one query may use many slots.  C > 0:
+    // query q writes its first min(n, C) ids to ids[q * C ..) (hit_start[q] =
+    // q * C) and hit_count[q] = n, whatever the other queries found.
+    uint32_t per_query_cap;
     // Agent counters (device): [0] ids found, [1] next query (work stealing),
     // [2] IBF visits, [3] row bytes fetched (h * bin_words * 8 per value and
     // visit), [4] lookups (query values checked against one IBF), [5] set when

@@ -19,6 +19,20 @@
 namespace hibf_amd
 {
 int set_error(int status, std::string msg);
+// hibf_gpu_hibf_membership_launch whose query q writes its first min(n,
+// per_query_cap) ids to d_ids[q * per_query_cap ..) and its full count n to
+// d_hit_count[q] (capi.hip).
+int membership_launch_capped(hibf_gpu_hibf h,
+                             hibf_gpu_agent a,
+                             uint64_t const * d_values,
+                             uint64_t const * d_offsets,
+                             uint64_t nq,
+                             uint16_t const * d_thresholds,
+                             uint64_t * d_ids,
+                             uint32_t per_query_cap,
+                             uint64_t * d_hit_start,
+                             uint64_t * d_hit_count,
+                             hipStream_t stream);
 }
 
 namespace hibf_amd::serving
@@ -221,8 +235,9 @@ inline int slot_launch(Slot & s, hibf_gpu_hibf hibf, uint64_t nb, uint64_t nv, c
         (void)hipGetLastError();
         return serve_fail(e, "batch input copies");
     }
-    if (int const r = hibf_gpu_hibf_membership_launch(hibf, s.agent, d_v, d_o, nb, s.d_thr, s.d_ids, nb * R, s.d_hs,
-                                                      s.d_hc, s.stream))
+    // Each request keeps its own first R ids, whatever the others in the batch found.
+    if (int const r = membership_launch_capped(hibf, s.agent, d_v, d_o, nb, s.d_thr, s.d_ids, R, s.d_hs, s.d_hc,
+                                               s.stream))
         return r;
     step(hipMemcpyAsync(s.h_hs, s.d_hs, nb * 8u, hipMemcpyDeviceToHost, s.stream));
     step(hipMemcpyAsync(s.h_hc, s.d_hc, nb * 8u, hipMemcpyDeviceToHost, s.stream));
@@ -236,9 +251,8 @@ inline int slot_launch(Slot & s, hibf_gpu_hibf hibf, uint64_t nb, uint64_t nv, c
     return HIBF_GPU_OK;
 }
 
-// Ids of request i of a finished batch: how many it has (count) and how many
-// of them, up to R, are in s.h_ids from s.h_hs[i] (0 when the batch's id buffer
-// was full before them).
+// Ids of request i of a finished batch: how many of its ids, up to R, are in
+// s.h_ids from s.h_hs[i] (= i * R).
 inline uint64_t slot_kept_ids(Slot const & s, uint64_t i, uint64_t nb, uint32_t R)
 {
     uint64_t const n = s.h_hc[i];

@@ -52,7 +52,9 @@ const char * hibf_gpu_last_error(void);
  * block 0), "hibf.row_per_lane" (descent over rows of <= 4 words: one value
  * and its whole row per lane 1, lanes per value 0), "hibf.sliced_counts"
  * (descent over rows of <= 4 words: bit-sliced per-lane counts 1, one LDS
- * atomic per set bit 0), "hibf.nt_loads" (0/1), "hibf.block_waves" (waves
+ * atomic per set bit 0), "hibf.unroll" (descent over rows of <= 4 words with
+ * sliced counts: value groups whose rows a wave loads at once, 2 or 4
+ * default), "hibf.nt_loads" (0/1), "hibf.block_waves" (waves
  * per query block of the membership descent: 1, 2, 4 default, 8), "count.nt_stores"
  * (non-temporal stores of the IBF count kernel's counters, of the HIBF
  * count's zero fill and of bulk_contains' rows: 1, default; plain stores 0),
@@ -435,8 +437,8 @@ typedef struct hibf_gpu_server_result
 {
     uint64_t ticket;  /* as returned by hibf_gpu_server_submit                        */
     uint32_t count;   /* ids of the request                                          */
-    uint32_t kept;    /* ids in h_ids: min(count, results_per_query), 0 if the batch's
-                         id buffer was full before them                              */
+    uint32_t kept;    /* ids in h_ids: min(count, results_per_query), whatever the other
+                         requests of its batch found                                 */
     double latency_s; /* submission to result ready                                  */
 } hibf_gpu_server_result;
 

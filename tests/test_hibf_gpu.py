@@ -158,13 +158,14 @@ CASES = [  # (n_ub, size range, config)
 ]
 
 
-_DESCENT_VARIANTS = {  # (hibf.wave_kernel, hibf.row_per_lane, hibf.sliced_counts, hibf.block_waves)
-    "wave": (1, 0, 1, 4),
-    "block-row-per-lane": (0, 1, 1, 4),
-    "block-lanes-per-value-sliced": (0, 0, 1, 4),
-    "block-lanes-per-value-atomics": (0, 0, 0, 4),
-    "block-2-waves": (0, 0, 1, 2),
-    "block-8-waves": (0, 0, 1, 8),
+_DESCENT_VARIANTS = {  # (hibf.wave_kernel, hibf.row_per_lane, hibf.sliced_counts, hibf.block_waves, hibf.unroll)
+    "wave": (1, 0, 1, 4, 4),
+    "block-row-per-lane": (0, 1, 1, 4, 4),
+    "block-lanes-per-value-sliced": (0, 0, 1, 4, 4),
+    "block-lanes-per-value-sliced-unroll-2": (0, 0, 1, 4, 2),
+    "block-lanes-per-value-atomics": (0, 0, 0, 4, 4),
+    "block-2-waves": (0, 0, 1, 2, 4),
+    "block-8-waves": (0, 0, 1, 8, 4),
 }
 
 
@@ -173,9 +174,10 @@ def descent_kernel(request):
     """Runs a test with each descent kernel: query per wave; query per block
     with one value and its row per lane; query per block with lanes sharing a
     value's row, counting through bit-sliced registers or with one LDS atomic
-    per set bit; and blocks of 2 and 8 waves.  Yields the variant's name."""
+    per set bit, with two or four value groups of rows in flight per wave;
+    and blocks of 2 and 8 waves.  Yields the variant's name."""
     from hibf_amd import native
-    keys = ("hibf.wave_kernel", "hibf.row_per_lane", "hibf.sliced_counts", "hibf.block_waves")
+    keys = ("hibf.wave_kernel", "hibf.row_per_lane", "hibf.sliced_counts", "hibf.block_waves", "hibf.unroll")
     saved = {k: native.tune_get(k) for k in keys}
     for k, v in zip(keys, _DESCENT_VARIANTS[request.param]):
         native.tune_set(k, v)

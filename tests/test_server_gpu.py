@@ -92,6 +92,48 @@ def test_submissions_from_two_threads_while_batches_run(served):
         assert batched[q] == ref.membership_for(qs[q], ts[q])
 
 
+@pytest.mark.parametrize("wave", [0, 1])
+def test_each_request_keeps_its_own_ids_when_one_batch_overflows(served, wave):
+    """A batch whose requests find far more ids than batch x R in total (every
+    97th request has threshold 1 and hundreds of ids) still gives every
+    request its own first R ids: a trace replayed as one batch, and the same
+    requests submitted to the online server within one window; with the
+    query-per-block descent and with the query-per-wave descent."""
+    native.tune_set("hibf.wave_kernel", wave)
+    try:
+        _overflow_case(served)
+    finally:
+        native.tune_set("hibf.wave_kernel", 0)
+
+
+def _overflow_case(served):
+    from hibf_amd.device import Server, csr
+    dev, qs, ts, batched, _ = served
+    R = 4
+    sel = list(range(0, 97 * 8, 97)) + list(range(1, 200))  # 8 many-id requests first, then 199 others
+    assert sum(len(batched[q]) for q in sel) > 2 * len(sel) * R
+    _say("serve: one batch whose ids overflow batch x R")
+    values, offsets = csr([qs[q] for q in sel])
+    lat, ids, cnt, st = dev.serve(values, offsets, np.array([ts[q] for q in sel], dtype=np.uint16),
+                                  np.zeros(len(sel)), window_us=1e5, streams=1, results_per_query=R)
+    assert st["batches"] == 1
+    for i, q in enumerate(sel):
+        want = batched[q]
+        assert int(cnt[i]) == len(want), q
+        assert ids[i, :min(len(want), R)].tolist() == want[:R], q
+    _say("server: the same requests within one window")
+    server = Server(dev, window_us=2e5, streams=1, results_per_query=R)
+    tickets = {server.submit(qs[q], ts[q]): q for q in sel}
+    got = _collect(server, len(sel))
+    st = server.stats()
+    server.close()
+    assert len(got) == len(sel)
+    assert st["batches"] <= 2
+    for ticket, (got_ids, count, _) in got.items():
+        q = tickets[ticket]
+        assert count == len(batched[q]) and got_ids == batched[q][:R], q
+
+
 def test_requests_alone_equal_requests_in_the_mix(served):
     from hibf_amd.device import Server
     dev, qs, ts, batched, _ = served
