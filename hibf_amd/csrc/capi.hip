@@ -493,6 +493,68 @@ int hibf_gpu_ibf_download(hibf_gpu_ibf ibf, uint64_t * h_words, uint64_t n_words
     return HIBF_GPU_OK;
 }
 
+namespace
+{
+// Emplace with track_occupancy: occupancy[bin] grows by one for each pair (in
+// the given order) that sets a bit unset before it, as n calls of the
+// reference's emplace do (src/interleaved_bloom_filter.cpp:119-140).  The
+// pairs' positions are sorted stably; a pair counts iff it is the first to
+// touch one of its positions and that bit was unset before the call.
+// Synchronous: the counts are added to the host copy of the occupancy.
+int emplace_tracked(hibf_gpu_ibf ibf, uint64_t const * d_values, uint64_t const * d_bins, uint64_t bin, uint64_t n,
+                    hipStream_t s)
+{
+    hibf_gpu_ibf_geometry const & g = ibf->g;
+    uint64_t const m = n * g.hash_funs;
+    if (n > (1ULL << 30) || m > static_cast<uint64_t>(INT32_MAX))
+        return fail(HIBF_GPU_ERR_UNSUPPORTED, "emplace with track_occupancy: at most 2^31 - 1 positions per call");
+    uint64_t const invalid_key = g.technical_bins * g.bin_size; // one past the last bit
+    int end_bit = 1;
+    while (end_bit < 64 && (invalid_key >> end_bit) != 0u)
+        ++end_bit;
+    size_t sort_bytes = 0;
+    HIBF_TRY_HIP(radix_sort_pairs_u64(nullptr, sort_bytes, nullptr, nullptr, nullptr, nullptr, m, end_bit, s),
+                 "tracked emplace sort size");
+    uint64_t const occ_bytes = g.technical_bins * sizeof(unsigned long long);
+    uint64_t const pair_bytes = m * 8u;
+    // keys, sorted keys, owners, sorted owners, marks, per-bin counts, sort temp
+    uint64_t const off_keys2 = pair_bytes, off_own = 2u * pair_bytes, off_own2 = 3u * pair_bytes;
+    uint64_t const off_marks = 4u * pair_bytes;
+    uint64_t const off_occ = (off_marks + n + 255u) / 256u * 256u;
+    uint64_t const off_tmp = (off_occ + occ_bytes + 255u) / 256u * 256u;
+    char * ws = nullptr;
+    HIBF_TRY_HIP(hibf_amd::device_malloc(&ws, off_tmp + sort_bytes), "tracked emplace workspace");
+    std::unique_ptr<char, void (*)(char *)> guard{ws, [](char * p) { (void)hipFree(p); }};
+    auto * keys = reinterpret_cast<uint64_t *>(ws);
+    auto * keys2 = reinterpret_cast<uint64_t *>(ws + off_keys2);
+    auto * owners = reinterpret_cast<uint64_t *>(ws + off_own);
+    auto * owners2 = reinterpret_cast<uint64_t *>(ws + off_own2);
+    auto * marks = reinterpret_cast<uint8_t *>(ws + off_marks);
+    auto * occ = reinterpret_cast<unsigned long long *>(ws + off_occ);
+    HIBF_TRY_HIP(hipMemsetAsync(marks, 0, n, s), "tracked emplace marks");
+    HIBF_TRY_HIP(hipMemsetAsync(occ, 0, occ_bytes, s), "tracked emplace counts");
+    HIBF_TRY_HIP(launch_emplace_positions(g.bin_size, static_cast<uint32_t>(g.bin_words),
+                                          static_cast<uint32_t>(g.hash_shift), static_cast<uint32_t>(g.hash_funs),
+                                          d_values, d_bins, bin, n, g.bins, invalid_key, keys, owners, s),
+                 "tracked emplace positions");
+    HIBF_TRY_HIP(radix_sort_pairs_u64(ws + off_tmp, sort_bytes, keys, keys2, owners, owners2, m, end_bit, s),
+                 "tracked emplace sort");
+    HIBF_TRY_HIP(launch_mark_first_touches(ibf->d_words, keys2, owners2, m, invalid_key, marks, s),
+                 "tracked emplace first touches");
+    HIBF_TRY_HIP(launch_emplace(ibf->d_words, g.bin_size, static_cast<uint32_t>(g.bin_words),
+                                static_cast<uint32_t>(g.hash_shift), static_cast<uint32_t>(g.hash_funs), d_values,
+                                d_bins, bin, n, g.bins, s),
+                 "emplace launch");
+    HIBF_TRY_HIP(launch_count_marked(marks, d_bins, bin, n, occ, s), "tracked emplace counts");
+    std::vector<uint64_t> add(g.technical_bins);
+    HIBF_TRY_HIP(hipMemcpyAsync(add.data(), occ, occ_bytes, hipMemcpyDeviceToHost, s), "tracked emplace copy");
+    HIBF_TRY_HIP(hipStreamSynchronize(s), "tracked emplace sync");
+    for (uint64_t b = 0; b < g.technical_bins && b < ibf->occupancy.size(); ++b)
+        ibf->occupancy[b] += add[b];
+    return HIBF_GPU_OK;
+}
+} // namespace
+
 int hibf_gpu_ibf_emplace(hibf_gpu_ibf ibf,
                          uint64_t const * d_values,
                          uint64_t const * d_bins,
@@ -504,6 +566,8 @@ int hibf_gpu_ibf_emplace(hibf_gpu_ibf ibf,
         return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "null argument");
     if (!d_bins && bin >= ibf->g.bins)
         return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "bin index >= bin_count");
+    if (ibf->track_occupancy && n)
+        return emplace_tracked(ibf, d_values, d_bins, bin, n, static_cast<hipStream_t>(stream));
     HIBF_TRY_HIP(launch_emplace(ibf->d_words, ibf->g.bin_size, static_cast<uint32_t>(ibf->g.bin_words),
                                 static_cast<uint32_t>(ibf->g.hash_shift), static_cast<uint32_t>(ibf->g.hash_funs),
                                 d_values, d_bins, bin, n, ibf->g.bins, static_cast<hipStream_t>(stream)),

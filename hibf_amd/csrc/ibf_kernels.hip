@@ -38,6 +38,77 @@ __global__ void __launch_bounds__(256) emplace_kernel(uint64_t * __restrict__ wo
     }
 }
 
+// Emplace with track_occupancy (src/interleaved_bloom_filter.cpp:119-140),
+// pairs i = 0, 1, ... in order: the reference reads each hash's bit before
+// setting it and counts the value iff some read found an unset bit.  Every
+// bit belongs to one bin, so that is the case iff the value is the first pair
+// of the call to touch one of its positions and that position was unset
+// before the call.  Step 1: key[i * h + k] = position of hash k of pair i
+// (row * technical_bins + bin, the bit index in the matrix), owner = i.
+__global__ void __launch_bounds__(256) emplace_positions_kernel(uint64_t bin_size,
+                                                                uint32_t bin_words,
+                                                                uint32_t hash_shift,
+                                                                uint32_t hash_funs,
+                                                                uint64_t const * __restrict__ values,
+                                                                uint64_t const * __restrict__ bins,
+                                                                uint64_t bin,
+                                                                uint64_t n,
+                                                                uint64_t bin_count,
+                                                                uint64_t invalid_key,
+                                                                uint64_t * __restrict__ keys,
+                                                                uint64_t * __restrict__ owners)
+{
+    uint64_t const stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
+    {
+        uint64_t const v = values[i];
+        uint64_t const b = bins ? bins[i] : bin;
+        for (uint32_t k = 0; k < hash_funs; ++k)
+        {
+            keys[i * hash_funs + k] = b < bin_count
+                                          ? row_word(v, hash_seed(static_cast<int>(k)), hash_shift, bin_size, bin_words) *
+                                                    64u +
+                                                b
+                                          : invalid_key;
+            owners[i * hash_funs + k] = i;
+        }
+    }
+}
+
+// Step 2, over the pairs sorted stably by key (equal keys keep pair order, so
+// a key's first entry is its first touch): mark the owner of a first touch of
+// a position whose bit is unset in words (read before the emplace).
+__global__ void __launch_bounds__(256) mark_first_touches_kernel(uint64_t const * __restrict__ words,
+                                                                 uint64_t const * __restrict__ keys,
+                                                                 uint64_t const * __restrict__ owners,
+                                                                 uint64_t m,
+                                                                 uint64_t invalid_key,
+                                                                 uint8_t * __restrict__ marked)
+{
+    uint64_t const stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    for (uint64_t e = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < m; e += stride)
+    {
+        uint64_t const key = keys[e];
+        if (key == invalid_key || (e > 0 && keys[e - 1] == key))
+            continue;
+        if (((words[key >> 6] >> (key & 63u)) & 1ULL) == 0ULL)
+            marked[owners[e]] = 1u;
+    }
+}
+
+// Step 3 (after the emplace): occupancy[bin of pair i] += 1 per marked pair.
+__global__ void __launch_bounds__(256) count_marked_kernel(uint8_t const * __restrict__ marked,
+                                                           uint64_t const * __restrict__ bins,
+                                                           uint64_t bin,
+                                                           uint64_t n,
+                                                           unsigned long long * __restrict__ occupancy)
+{
+    uint64_t const stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
+        if (marked[i])
+            atomicAdd(&occupancy[bins ? bins[i] : bin], 1ULL);
+}
+
 // Generator-driven bulk emplace: values synth_value(seed, first_bin + b, j)
 // for j < per_bin go to technical bin first_bin + b, b < n_bins.
 __global__ void __launch_bounds__(256) emplace_synth_kernel(uint64_t * __restrict__ words,
@@ -316,6 +387,38 @@ hipError_t launch_emplace(uint64_t * words,
         return hipSuccess;
     hipLaunchKernelGGL(emplace_kernel, dim3(grid_for(n, 256)), dim3(256), 0, stream, words, bin_size, bin_words,
                        hash_shift, hash_funs, values, bins, bin, n, bin_count);
+    return hipGetLastError();
+}
+
+hipError_t launch_emplace_positions(uint64_t bin_size, uint32_t bin_words, uint32_t hash_shift, uint32_t hash_funs,
+                                    uint64_t const * values, uint64_t const * bins, uint64_t bin, uint64_t n,
+                                    uint64_t bin_count, uint64_t invalid_key, uint64_t * keys, uint64_t * owners,
+                                    hipStream_t stream)
+{
+    if (n == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(emplace_positions_kernel, dim3(grid_for(n, 256)), dim3(256), 0, stream, bin_size, bin_words,
+                       hash_shift, hash_funs, values, bins, bin, n, bin_count, invalid_key, keys, owners);
+    return hipGetLastError();
+}
+
+hipError_t launch_mark_first_touches(uint64_t const * words, uint64_t const * keys, uint64_t const * owners,
+                                     uint64_t m, uint64_t invalid_key, uint8_t * marked, hipStream_t stream)
+{
+    if (m == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(mark_first_touches_kernel, dim3(grid_for(m, 256)), dim3(256), 0, stream, words, keys, owners,
+                       m, invalid_key, marked);
+    return hipGetLastError();
+}
+
+hipError_t launch_count_marked(uint8_t const * marked, uint64_t const * bins, uint64_t bin, uint64_t n,
+                               unsigned long long * occupancy, hipStream_t stream)
+{
+    if (n == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(count_marked_kernel, dim3(grid_for(n, 256)), dim3(256), 0, stream, marked, bins, bin, n,
+                       occupancy);
     return hipGetLastError();
 }
 

@@ -303,6 +303,23 @@ namespace hibf_amd
 // hipCUB inclusive sum; call with temp == nullptr to size temp_bytes.
 hipError_t inclusive_sum_u64(void * temp, size_t & temp_bytes, uint64_t const * in, uint64_t * out, uint64_t n,
                              hipStream_t stream);
+// hipCUB stable radix sort of (key, value) pairs on key bits [0, end_bit).
+hipError_t radix_sort_pairs_u64(void * temp, size_t & temp_bytes, uint64_t const * keys_in, uint64_t * keys_out,
+                                uint64_t const * vals_in, uint64_t * vals_out, uint64_t n, int end_bit,
+                                hipStream_t stream);
+// Steps of the emplace with track_occupancy (ibf_kernels.hip): the bit
+// positions of the pairs (key = row * technical_bins + bin, `invalid_key`
+// for bins >= bin_count; value = pair index), and, after a stable sort by
+// key, a mark on each pair that is the first to touch a position unset in
+// words, then occupancy[bin] += 1 per marked pair.
+hipError_t launch_emplace_positions(uint64_t bin_size, uint32_t bin_words, uint32_t hash_shift, uint32_t hash_funs,
+                                    uint64_t const * values, uint64_t const * bins, uint64_t bin, uint64_t n,
+                                    uint64_t bin_count, uint64_t invalid_key, uint64_t * keys, uint64_t * owners,
+                                    hipStream_t stream);
+hipError_t launch_mark_first_touches(uint64_t const * words, uint64_t const * keys, uint64_t const * owners,
+                                     uint64_t m, uint64_t invalid_key, uint8_t * marked, hipStream_t stream);
+hipError_t launch_count_marked(uint8_t const * marked, uint64_t const * bins, uint64_t bin, uint64_t n,
+                               unsigned long long * occupancy, hipStream_t stream);
 // HyperLogLog sketches (hll.hip): registers of the synthetic user bins
 // (synth_value(seed, u, i), i < h_sizes[u]) and the estimates of n sketches on
 // the device (synchronous).

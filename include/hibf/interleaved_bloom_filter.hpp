@@ -169,13 +169,28 @@ public:
             return;
         detail::device_buffer d;
         d.upload(v.data(), v.size() * 8u);
-        int rc = hibf_gpu_ibf_emplace(handle_->h, d.as<uint64_t>(), nullptr, bin.value, v.size(), nullptr);
+        int rc = HIBF_GPU_OK;
+        if (track_occupancy)
+        {
+            // The handle counts the values that set a new bit, in range order, as that many emplace calls
+            // of the reference do; its counters start from this object's public occupancy.
+            std::vector<uint64_t> occ(occupancy.begin(), occupancy.end());
+            rc = hibf_gpu_ibf_set_occupancy(handle_->h, occ.data(), 1);
+            if (rc == HIBF_GPU_OK)
+                rc = hibf_gpu_ibf_emplace(handle_->h, d.as<uint64_t>(), nullptr, bin.value, v.size(), nullptr);
+            if (rc == HIBF_GPU_OK)
+                rc = hibf_gpu_ibf_occupancy(handle_->h, occ.data(), nullptr);
+            if (rc == HIBF_GPU_OK)
+                occupancy.assign(occ.begin(), occ.end());
+        }
+        else
+        {
+            rc = hibf_gpu_ibf_emplace(handle_->h, d.as<uint64_t>(), nullptr, bin.value, v.size(), nullptr);
+        }
         if (rc == HIBF_GPU_OK)
             rc = hibf_gpu_synchronize();
         if (rc != HIBF_GPU_OK)
             std::terminate();
-        if (track_occupancy)
-            occupancy[bin.value] += v.size(); // approximate: the device path does not track new bits
     }
 
     // interleaved_bloom_filter.hpp:270, 283-297

@@ -130,19 +130,26 @@ int hibf_gpu_ibf_try_increase_bin_number_to(hibf_gpu_ibf ibf, uint64_t new_bins,
 int hibf_gpu_ibf_increase_bin_number_to(hibf_gpu_ibf ibf, uint64_t new_bins);
 
 /* Occupancy kept with the handle (interleaved_bloom_filter.hpp:451-458):
- * technical_bins counters and the track_occupancy flag.  The device emplace
- * does not update them (track_occupancy is order-dependent); clear() resets
- * the cleared bins' counters as src/interleaved_bloom_filter.cpp:145 does.
- * h_occupancy may be NULL (get: skipped; set: all zero). */
+ * technical_bins counters and the track_occupancy flag.  With the flag set,
+ * hibf_gpu_ibf_emplace updates them exactly as the reference's emplace calls
+ * in pair order would; clear() resets the cleared bins' counters as
+ * src/interleaved_bloom_filter.cpp:145 does.  h_occupancy may be NULL (get:
+ * skipped; set: all zero). */
 int hibf_gpu_ibf_occupancy(hibf_gpu_ibf ibf, uint64_t * h_occupancy, int * track_occupancy);
 int hibf_gpu_ibf_set_occupancy(hibf_gpu_ibf ibf, uint64_t const * h_occupancy, int track_occupancy);
 
 /* Bulk emplace; replaces emplace(value, bin_index) (interleaved_bloom_filter.hpp:259;
  * src/interleaved_bloom_filter.cpp:119-140) called for n (value, bin) pairs.
  * d_bins may be NULL, in which case every value goes to `bin`.  Bits are set
- * with atomicOr, so the result does not depend on order or concurrency.
+ * with atomicOr, so the bits do not depend on order or concurrency.
  * Bins >= bin_count are rejected only when d_bins == NULL (checked on the
- * host); per-pair bins are the caller's precondition as in the reference. */
+ * host); per-pair bins are the caller's precondition as in the reference
+ * (such pairs are skipped).  With track_occupancy set on the handle, each
+ * bin's occupancy grows by the number of its pairs that set a bit unset just
+ * before them, taking the pairs in order (d_values[0] first), as n calls of
+ * the reference's emplace do; that form sorts the n x h bit positions on the
+ * device, is synchronous and takes at most 2^31 - 1 positions per call
+ * (HIBF_GPU_ERR_UNSUPPORTED beyond). */
 int hibf_gpu_ibf_emplace(hibf_gpu_ibf ibf,
                          uint64_t const * d_values,
                          uint64_t const * d_bins,

@@ -87,6 +87,8 @@ def lib():
         L.orc_ibf_init.argtypes = [ctypes.POINTER(OrcIbf), ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64]
         L.orc_ibf_emplace.argtypes = [ctypes.c_void_p, ctypes.POINTER(OrcIbf), ctypes.c_void_p, ctypes.c_void_p,
                                       ctypes.c_uint64]
+        L.orc_ibf_emplace_tracked.argtypes = [ctypes.c_void_p, ctypes.POINTER(OrcIbf), ctypes.c_void_p,
+                                              ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p]
         L.orc_ibf_clear.argtypes = [ctypes.c_void_p, ctypes.POINTER(OrcIbf), ctypes.c_uint64]
         L.orc_ibf_increase_bin_number_to.argtypes = [ctypes.c_void_p, ctypes.POINTER(OrcIbf), ctypes.c_uint64]
         L.orc_ibf_increase_bin_number_to.restype = ctypes.c_int
@@ -200,6 +202,14 @@ class Ibf:
         v = _u64(values)
         b = _u64(np.broadcast_to(np.asarray(bins, dtype=np.uint64), v.shape))
         lib().orc_ibf_emplace(_p(self.words), ctypes.byref(self.g), _p(v), _p(b), v.size)
+
+    def emplace_tracked(self, values, bins, occupancy: np.ndarray) -> None:
+        """emplace of the pairs in order with track_occupancy: occupancy (one
+        uint64 per technical bin) grows by one for each value that set a new bit."""
+        v = _u64(values)
+        b = _u64(np.broadcast_to(np.asarray(bins, dtype=np.uint64), v.shape))
+        assert occupancy.dtype == np.uint64 and occupancy.size >= self.g.technical_bins and occupancy.flags.c_contiguous
+        lib().orc_ibf_emplace_tracked(_p(self.words), ctypes.byref(self.g), _p(v), _p(b), v.size, _p(occupancy))
 
     def emplace_parallel(self, values, bins, threads: int = 16) -> None:
         """emplace with `threads` threads (atomic OR; same result)."""

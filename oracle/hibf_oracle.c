@@ -134,6 +134,33 @@ void orc_ibf_emplace(uint64_t * words, orc_ibf const * g, uint64_t const * value
     }
 }
 
+/* src/interleaved_bloom_filter.cpp:119-140 with track_occupancy == true, for
+ * the pairs k = 0, 1, ... in order: exists starts true and is and-ed with each
+ * hash's bit read just before that bit is set (so a value whose two hashes hit
+ * one unset bit still finds it unset at the first read); occupancy[bin] grows
+ * by one iff !exists.  Pairs with bins[k] >= g->bins are skipped (the
+ * reference asserts bin < bins). */
+void orc_ibf_emplace_tracked(uint64_t * words, orc_ibf const * g, uint64_t const * values, uint64_t const * bins,
+                             uint64_t n, uint64_t * occupancy)
+{
+    for (uint64_t k = 0; k < n; ++k)
+    {
+        if (bins[k] >= g->bins)
+            continue;
+        int exists = 1;
+        for (uint64_t i = 0; i < g->hash_funs; ++i)
+        {
+            uint64_t idx = orc_hash_and_fit(values[k], ORC_HASH_SEEDS[i], g->hash_shift, g->bin_size, g->technical_bins)
+                         + bins[k];
+            uint64_t const bit = 1ULL << (idx & 63u);
+            exists &= (words[idx >> 6] & bit) != 0;
+            words[idx >> 6] |= bit;
+        }
+        if (!exists)
+            ++occupancy[bins[k]];
+    }
+}
+
 /* Test-input generator (not reference code): value i of the seeded synthetic
  * set `bin`, the splitmix/murmur 64-bit finaliser of
  * (bin << 36 | i) + seed * 0x9E3779B97F4A7C15.  The tests draw user bins and

@@ -5,6 +5,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <iostream>
+#include <ranges>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -85,6 +86,29 @@ int main()
 
     report("bin_count_fixed", !hibf.ibf_vector[0].try_increase_bin_number_to(sh::bin_count{hibf.ibf_vector[0].bin_count() + 1u}));
     report("version", hibf_t::version == 1u);
+
+    // interleaved_bloom_filter_test.cpp:209-239 (emplace_with_occupancy): 64 hashes into each of the
+    // first 64 of 128 bins of an IBF that tracks occupancy; emplacing them again adds nothing.
+    {
+        sh::interleaved_bloom_filter ibf{sh::bin_count{128u}, sh::bin_size{512}, sh::hash_function_count{2u}, true};
+        for (size_t bin_idx = 0; bin_idx < 64u; ++bin_idx)
+            for (size_t hash = 0; hash < 64u; ++hash)
+                ibf.emplace(hash, sh::bin_index{bin_idx});
+        auto agent = ibf.containment_agent();
+        bool occ_ok = ibf.occupancy.size() == 128u;
+        for (size_t hash = 0; hash < 64u; ++hash)
+        {
+            auto const & res = agent.bulk_contains(hash);
+            for (size_t b = 0; b < 128u; ++b)
+                occ_ok = occ_ok && res[b] == (b < 64u);
+        }
+        for (size_t b = 0; b < 128u; ++b)
+            occ_ok = occ_ok && ((b < 64u) == (ibf.occupancy[b] != 0u));
+        std::vector<size_t> const first = ibf.occupancy;
+        for (size_t bin_idx = 0; bin_idx < 64u; ++bin_idx)
+            ibf.emplace_range(std::views::iota(size_t{0}, size_t{64}), sh::bin_index{bin_idx});
+        report("emplace_with_occupancy", occ_ok && ibf.occupancy == first);
+    }
 
     // hierarchical_interleaved_bloom_filter_test.cpp:59-106: config and layout read from one stream.
     {

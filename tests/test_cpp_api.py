@@ -49,14 +49,15 @@ def test_cpp_store_and_load(tmp_path):
 @pytest.mark.gpu
 def test_cpp_hibf_public_members():
     """ibf_vector (views of the arena), prev_ibf_id, version, the build
-    timers and copy semantics of the C++ HIBF class, and the reference's
-    build_from_layout test (config and layout read from one stream)."""
+    timers and copy semantics of the C++ HIBF class, the reference's
+    build_from_layout test (config and layout read from one stream) and its
+    emplace_with_occupancy test on the C++ IBF."""
     exe = os.path.join(ROOT, "tests", "cpp", "hibf_members")
     assert os.path.exists(exe), "build tests/cpp/hibf_members first (python -m hibf_amd.build)"
     r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     lines = r.stdout.splitlines()
-    assert len(lines) == 8 and all(line.endswith(" ok") for line in lines), r.stdout
+    assert len(lines) == 9 and all(line.endswith(" ok") for line in lines), r.stdout
 
 
 def _config_io(tmp_path):

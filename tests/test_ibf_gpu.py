@@ -257,6 +257,39 @@ def test_clear_matches_oracle():
     assert np.array_equal(to_numpy(dev.bulk_count(values, offsets)), ref.bulk_count_batch(values, offsets))
 
 
+@pytest.mark.parametrize("bins,bin_size,h", [(64, 97, 2), (130, 61, 3), (5, 4096, 5), (300, 1, 2), (70, 2, 1)])
+def test_emplace_tracked_occupancy_matches_oracle(bins, bin_size, h):
+    """emplace with track_occupancy on the device: the occupancy equals the
+    oracle's sequential emplace (src/interleaved_bloom_filter.cpp:119-140) over
+    pairs with repeated values, colliding rows (small bin sizes), out-of-range
+    bins, a whole-range call into one bin and a second call on the same filter;
+    the bits equal the untracked emplace's."""
+    from hibf_amd.device import DeviceIbf
+    _say(f"tracked emplace bins={bins} bin_size={bin_size} h={h}")
+    rng = np.random.default_rng(bins * 7 + h)
+    dev = DeviceIbf(bins, bin_size, h)
+    ref = O.Ibf(bins, bin_size, h)
+    tb = int(dev.geometry.technical_bins)
+    dev.set_occupancy(np.arange(tb, dtype=np.uint64) % 3, True)  # counters continue from what the caller set
+    occ = np.arange(tb, dtype=np.uint64) % 3
+    for call in range(3):
+        vals = rng.integers(0, 2**63, size=4000, dtype=np.int64).astype(np.uint64)
+        vals[1000:1500] = vals[:500]
+        if call == 2:
+            dev.emplace(vals[:700], bins - 1)
+            ref.emplace_tracked(vals[:700], bins - 1, occ)
+        else:
+            bs = rng.integers(0, bins + 3, size=vals.size).astype(np.uint64)  # some bins out of range: skipped
+            dev.emplace(vals, bs)
+            ref.emplace_tracked(vals, bs, occ)
+        got, track = dev.occupancy()
+        assert track and np.array_equal(got, occ), call
+        assert np.array_equal(dev.download(), ref.words), call
+    dev.set_occupancy(np.zeros(tb, dtype=np.uint64), False)
+    dev.emplace(vals, 0)  # untracked: counters stay
+    assert not dev.occupancy()[0].any()
+
+
 def test_snippet_goldens_on_gpu(goldens):
     from hibf_amd.device import DeviceIbf, csr, to_numpy
     _say("snippet goldens")

@@ -224,6 +224,78 @@ def test_emplace_sets_h_bits_in_its_column_only():
                         assert (wi % ibf.g.bin_words) * 64 + bit == b
 
 
+def _first_toucher_occupancy(g, words0, values, bins, n_bins):
+    """Occupancy increments of emplacing the pairs in order, derived another
+    way than the oracle's sequential loop: a value counts iff it is the first
+    pair (in order) to touch one of its bit positions and that position was
+    unset before the call.  Positions from the oracle's hash (pinned above),
+    first touches from numpy's stable sort."""
+    h = g.hash_funs
+    pos = np.array([[O.hash_and_fit(int(v), O.HASH_SEEDS[k], g.hash_shift, g.bin_size, g.technical_bins) + int(b)
+                     for k in range(h)] for v, b in zip(values, bins)], dtype=np.uint64).reshape(-1)
+    owner = np.repeat(np.arange(len(values)), h)
+    valid = np.repeat(np.asarray(bins) < n_bins, h)
+    pos, owner = pos[valid], owner[valid]
+    order = np.argsort(pos, kind="stable")
+    first = np.ones(pos.size, dtype=bool)
+    first[1:] = pos[order][1:] != pos[order][:-1]
+    fp, fo = pos[order][first], owner[order][first]
+    unset = ((words0[(fp >> np.uint64(6)).astype(np.int64)] >> (fp & np.uint64(63))) & np.uint64(1)) == 0
+    counted = np.zeros(len(values), dtype=bool)
+    counted[fo[unset]] = True
+    occ = np.zeros(g.technical_bins, dtype=np.uint64)
+    np.add.at(occ, np.asarray(bins, dtype=np.int64)[counted], 1)
+    return occ
+
+
+def test_emplace_tracked_occupancy():
+    """emplace with track_occupancy (src/interleaved_bloom_filter.cpp:119-140)."""
+    # interleaved_bloom_filter_test.cpp:209-239: 64 hashes into each of the first 64 of 128 bins.
+    ibf = O.Ibf(128, 512, 2)
+    occ = np.zeros(128, dtype=np.uint64)
+    for b in range(64):
+        ibf.emplace_tracked(np.arange(64, dtype=np.uint64), b, occ)
+    assert all(occ[:64] != 0) and all(occ[64:] == 0)
+    for v in range(64):
+        assert ibf.bulk_contains(v)[0] & ((1 << 64) - 1) == (1 << 64) - 1
+    # bin_size 1: every hash of every value is the one bit of its column, so only the first value counts.
+    one = O.Ibf(70, 1, 3)
+    occ = np.zeros(one.g.technical_bins, dtype=np.uint64)
+    one.emplace_tracked(np.arange(1, 40, dtype=np.uint64), [b % 7 for b in range(39)], occ)
+    assert occ.tolist()[:70] == [1] * 7 + [0] * 63
+    # bin_size 2, one hash: a bin's occupancy is the number of its two rows hit.
+    two = O.Ibf(64, 2, 1)
+    occ = np.zeros(64, dtype=np.uint64)
+    vals = np.arange(100, 160, dtype=np.uint64)
+    two.emplace_tracked(vals, np.arange(60) % 5, occ)
+    for b in range(5):
+        rows = {O.hash_and_fit(int(v), O.HASH_SEEDS[0], two.g.hash_shift, 2, 64) // 64 for v in vals[b::5]}
+        assert occ[b] == len(rows)
+    # Emplacing the same pairs again finds every bit set; out-of-range bins are skipped.
+    rng = np.random.default_rng(17)
+    for bins, bs, h in [(64, 97, 2), (130, 61, 3), (5, 4096, 5)]:
+        ibf = O.Ibf(bins, bs, h)
+        ibf.emplace(rng.integers(0, 2**63, size=50, dtype=np.int64).astype(np.uint64), 0)  # bits set before the call
+        vals = rng.integers(0, 2**63, size=600, dtype=np.int64).astype(np.uint64)
+        vals[300:400] = vals[:100]  # repeated values
+        bs_ = rng.integers(0, bins + 2, size=vals.size).astype(np.uint64)
+        occ = np.zeros(ibf.g.technical_bins, dtype=np.uint64)
+        words0 = ibf.words.copy()
+        ibf.emplace_tracked(vals, bs_, occ)
+        assert np.array_equal(occ, _first_toucher_occupancy(ibf.g, words0, vals, bs_, bins))
+        plain = O.Ibf.from_words(bins, bs, h, words0.copy())
+        keep = bs_ < bins
+        plain.emplace(vals[keep], bs_[keep])
+        assert np.array_equal(plain.words, ibf.words)
+        again = occ.copy()
+        ibf.emplace_tracked(vals, bs_, again)
+        assert np.array_equal(again, occ)
+        for b in range(bins):  # each counted value set at most h of its column's bits
+            col = sum(bin(int(w)).count("1") for w in ibf.words[b // 64::ibf.g.bin_words] & np.uint64(1 << (b % 64)))
+            assert occ[b] <= np.count_nonzero(bs_ == b) and col <= h * occ[b] + sum(
+                bin(int(w)).count("1") for w in words0[b // 64::ibf.g.bin_words] & np.uint64(1 << (b % 64)))
+
+
 def test_synth_stream_and_parallel_emplace_forms():
     """The synthetic test-input stream is the same in C and numpy, and the
     oracle's parallel emplace forms equal its sequential emplace."""
