@@ -237,6 +237,10 @@ struct hibf_gpu_hibf_s
     // tables; bulk_count refuses them (its device scan stores an IBF's counts
     // before descending into its merged bins, the reference in bin order).
     bool user_bin_in_two_ibfs{false};
+    // Words of device memory per descent block for counters and event lists
+    // ((1 + depth) x max_bins) when they do not fit the LDS budget; 0 when
+    // they live in LDS.
+    uint64_t gstate_words{0};
     // Phase times of the build that made this HIBF (seconds): sketches and
     // their estimates, layout, allocation (arena and tables), device fill.
     double build_times[4]{};
@@ -912,8 +916,6 @@ int hibf_gpu_hibf_create(uint64_t n_ibfs,
         h->geoms[i] = g;
         if (h_tb_offsets[i + 1] - h_tb_offsets[i] != g.bins)
             return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "table size of an IBF differs from its bin_count");
-        if (g.bins > kHibfMaxBins)
-            return fail(HIBF_GPU_ERR_UNSUPPORTED, "an IBF of the HIBF has more than 8192 technical bins");
         if (h_word_offsets[i] % 4u)
             return fail(HIBF_GPU_ERR_INVALID_ARGUMENT, "IBF word offsets must be multiples of 4 (32-byte aligned)");
         end_words = std::max(end_words, h_word_offsets[i] + g.n_words);
@@ -1003,8 +1005,8 @@ int hibf_gpu_hibf_create(uint64_t n_ibfs,
             if (owner[k].first == owner[k - 1].first && owner[k].second != owner[k - 1].second)
                 h->user_bin_in_two_ibfs = true;
     }
-    if (static_cast<size_t>(h->max_bins) * (1u + h->depth) * sizeof(uint32_t) > 64u * 1024u)
-        return fail(HIBF_GPU_ERR_UNSUPPORTED, "largest IBF x depth exceeds the LDS budget of the descent kernel");
+    if (static_cast<uint64_t>(h->max_bins) * (1u + h->depth) * sizeof(uint32_t) > kHibfLdsStateBytes)
+        h->gstate_words = static_cast<uint64_t>(h->max_bins) * (1u + h->depth);
     h->n_words = end_words;
     hipError_t e = hibf_amd::device_malloc(reinterpret_cast<void **>(&h->d_arena), std::max<uint64_t>(end_words, 1) * 8u);
     if (e == hipSuccess)
@@ -1171,7 +1173,7 @@ HibfLaunch launch_with(hibf_gpu_hibf h, unsigned long long * counters)
 {
     return HibfLaunch{h->d_desc,      h->d_arena,   h->d_tb_info, h->max_bins, h->depth, h->n_user_bins,
                       h->max_hits,    (h->max_bins + 63u) / 64u, h->uniform_hash_funs, g_count_tuning.nt_stores,
-                      0u,             counters};
+                      0u,             h->gstate_words,           counters};
 }
 
 HibfLaunch launch_of(hibf_gpu_hibf h, hibf_gpu_agent a)
@@ -1202,10 +1204,12 @@ bool wave_kernel_fits(HibfLaunch const & L)
     return g_count_tuning.hibf_wave != 0u && hibf_wave_lds_bytes(L, 4) <= 64u * 1024u;
 }
 
-// Per-block hit list of the block-per-query kernel: room for every id one query can produce.
+// Per-block scratch of the block-per-query kernel: a hit list with room for
+// every id one query can produce, then the counters and event lists of HIBFs
+// whose state does not fit LDS.
 uint64_t block_hit_list_bytes(hibf_gpu_hibf h)
 {
-    return std::max<uint64_t>(h->max_hits, 1u) * sizeof(uint32_t);
+    return (std::max<uint64_t>(h->max_hits, 1u) + h->gstate_words) * sizeof(uint32_t);
 }
 
 unsigned overflow_blocks(hibf_gpu_hibf h, uint64_t nq)
@@ -1483,8 +1487,17 @@ int hibf_gpu_hibf_bulk_count(hibf_gpu_hibf h,
                      "hibf count launch");
         return HIBF_GPU_OK;
     }
-    HIBF_TRY_HIP(launch_hibf_count(L, d_values, d_offsets, nq, d_thresholds, counter_bytes, d_counts,
-                                   hibf_blocks(nq, 0), s),
+    uint64_t const state_bytes = h->gstate_words * sizeof(uint32_t);
+    unsigned const blocks = hibf_blocks(nq, state_bytes);
+    uint32_t * scratch = nullptr;
+    if (state_bytes)
+    {
+        if (int rc = ensure(a->scratch, a->scratch_bytes, static_cast<size_t>(blocks) * state_bytes, "count scratch"))
+            return rc;
+        scratch = static_cast<uint32_t *>(a->scratch);
+    }
+    HIBF_TRY_HIP(launch_hibf_count(L, d_values, d_offsets, nq, d_thresholds, counter_bytes, d_counts, scratch, blocks,
+                                   s),
                  "hibf count launch");
     return HIBF_GPU_OK;
 }

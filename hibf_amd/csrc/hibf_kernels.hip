@@ -23,7 +23,7 @@ struct DescentState
 // explicit stack of per-level event lists in LDS, which reproduces the
 // reference's recursion order (hierarchical_interleaved_bloom_filter.hpp:321-355
 // and :462-496) without recursion.  HF > 0: every IBF has HF hash functions.
-template <bool COUNTING, bool NT, int HF, int U = 2, int NARROW = 0, bool SLICE = true>
+template <bool COUNTING, bool NT, int HF, int U = 2, int NARROW = 0, bool SLICE = true, bool GSTATE = false>
 __device__ __forceinline__ void descent_body(HibfLaunch const & h,
                                              uint64_t const * __restrict__ values,
                                              uint64_t const * __restrict__ offsets,
@@ -43,15 +43,19 @@ __device__ __forceinline__ void descent_body(HibfLaunch const & h,
     __shared__ DescentState st;
     __shared__ uint64_t next_q;
     unsigned long long * const total = h.counters;
-    uint32_t * const cnt = smem;              // max_bins counters
-    uint32_t * const evs = smem + h.max_bins; // depth x max_bins event lists
 
     int const tid = static_cast<int>(threadIdx.x);
     int const nthreads = static_cast<int>(blockDim.x);
     int const lane = tid & 63;
     int const wave = tid >> 6;
     int const nwaves = nthreads >> 6;
-    uint32_t * const my_scratch = COUNTING ? nullptr : scratch + static_cast<uint64_t>(blockIdx.x) * h.max_hits;
+    // The block's part of scratch: its hit list (membership), then, with GSTATE, its counters and event lists.
+    uint64_t const hit_words = COUNTING ? 0u : h.max_hits;
+    uint32_t * const region =
+        scratch ? scratch + static_cast<uint64_t>(blockIdx.x) * (hit_words + (GSTATE ? h.gstate_words : 0u)) : nullptr;
+    uint32_t * const my_scratch = COUNTING ? nullptr : region;
+    uint32_t * const cnt = GSTATE ? region + hit_words : smem; // max_bins counters
+    uint32_t * const evs = cnt + h.max_bins;                   // depth x max_bins event lists
 
     // Work stealing: each block takes the next query from the agent's cursor.
     uint64_t visits = 0;
@@ -216,6 +220,15 @@ template <bool COUNTING, bool NT, int HF>
 __global__ void __launch_bounds__(512) hibf_descent_kernel(HIBF_DESCENT_PARAMS)
 {
     descent_body<COUNTING, NT, HF>(HIBF_DESCENT_ARGS);
+}
+
+// HIBFs whose counters and event lists do not fit the LDS budget (IBFs of
+// more than a few thousand technical bins, deep trees): the same descent with
+// that state in the block's region of device memory.
+template <bool COUNTING>
+__global__ void __launch_bounds__(512) hibf_descent_gstate_kernel(HIBF_DESCENT_PARAMS)
+{
+    descent_body<COUNTING, false, 0, 2, 0, true, true>(HIBF_DESCENT_ARGS);
 }
 
 // Membership with two hash functions (the HIBF default) fits 64 VGPRs without
@@ -416,10 +429,11 @@ hipError_t launch_hibf_membership(HibfLaunch const & h,
 {
     if (nq == 0)
         return hipSuccess;
-    size_t const lds = static_cast<size_t>(h.max_bins) * (1u + h.depth) * sizeof(uint32_t);
+    size_t const lds = h.gstate_words ? 0u : static_cast<size_t>(h.max_bins) * (1u + h.depth) * sizeof(uint32_t);
     bool const h2 = h.uniform_hash_funs == 2u;
     bool const w4 = h2 && h.max_bin_words <= 4u;
-    auto const kernel = g_count_tuning.nt_loads                ? hibf_descent_kernel<false, true, 0>
+    auto const kernel = h.gstate_words                         ? hibf_descent_gstate_kernel<false>
+                        : g_count_tuning.nt_loads              ? hibf_descent_kernel<false, true, 0>
                         : w4 && g_count_tuning.row_per_lane     ? hibf_descent_membership_h2_row_per_lane_kernel
                         : w4 && g_count_tuning.sliced_counts && g_count_tuning.hibf_unroll == 4u
                                                                ? hibf_descent_membership_h2_w4_kernel<true, 4>
@@ -440,16 +454,18 @@ hipError_t launch_hibf_count(HibfLaunch const & h,
                              uint64_t const * thresholds,
                              uint32_t counter_bytes,
                              void * counts,
+                             uint32_t * scratch,
                              unsigned blocks,
                              hipStream_t stream)
 {
     if (nq == 0)
         return hipSuccess;
-    size_t const lds = static_cast<size_t>(h.max_bins) * (1u + h.depth) * sizeof(uint32_t);
-    auto const kernel = h.uniform_hash_funs == 2u ? hibf_descent_kernel<true, false, 2>
-                                                  : hibf_descent_kernel<true, false, 0>;
+    size_t const lds = h.gstate_words ? 0u : static_cast<size_t>(h.max_bins) * (1u + h.depth) * sizeof(uint32_t);
+    auto const kernel = h.gstate_words               ? hibf_descent_gstate_kernel<true>
+                        : h.uniform_hash_funs == 2u ? hibf_descent_kernel<true, false, 2>
+                                                    : hibf_descent_kernel<true, false, 0>;
     hipLaunchKernelGGL(kernel, dim3(blocks), dim3(256), lds, stream, h, values, offsets, nq, nullptr, thresholds,
-                       counter_bytes, counts, nullptr, nullptr, 0ULL, nullptr, nullptr, nullptr);
+                       counter_bytes, counts, scratch, nullptr, 0ULL, nullptr, nullptr, nullptr);
     return hipGetLastError();
 }
 

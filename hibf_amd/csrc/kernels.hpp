@@ -9,9 +9,11 @@
 namespace hibf_amd
 {
 
-// Largest IBF (technical bins) the HIBF descent kernel handles: per-IBF
-// counters live in LDS.
-constexpr uint32_t kHibfMaxBins = 8192;
+// LDS the descent kernels may give a block's per-IBF counters and per-level
+// event lists ((1 + depth) x largest IBF x 4 bytes).  HIBFs that need more
+// (large IBFs, deep trees) keep that state in a per-block region of device
+// memory instead (HibfLaunch::gstate_words).
+constexpr uint64_t kHibfLdsStateBytes = 64u * 1024u;
 // Deepest HIBF (levels) the descent kernel handles.
 constexpr uint32_t kHibfMaxDepth = 16;
 
@@ -180,6 +182,10 @@ struct HibfLaunch
     // query q writes its first min(n, C) ids to ids[q * C ..) (hit_start[q] =
     // q * C) and hit_count[q] = n, whatever the other queries found.
     uint32_t per_query_cap;
+    // 0: a block's counters and event lists are in LDS.  Otherwise they are
+    // (1 + depth) x max_bins words of device memory per block, after the
+    // block's hit list in the scratch the launch is given.
+    uint64_t gstate_words;
     // Agent counters (device): [0] ids found, [1] next query (work stealing),
     // [2] IBF visits, [3] row bytes fetched (h * bin_words * 8 per value and
     // visit), [4] lookups (query values checked against one IBF), [5] set when
@@ -195,6 +201,8 @@ hipError_t launch_reset_counters(unsigned long long * counters, uint32_t mask, h
 // Membership descent, one block per query.  Writes, per query,
 // hit_start/hit_count into ids_tmp (atomic order of queries).  With a query
 // list it answers only the queries listed there (counters[6] entries).
+// scratch holds, per block, max_hits words of hit list and then, when
+// h.gstate_words != 0, that many words of counters and event lists.
 hipError_t launch_hibf_membership(HibfLaunch const & h,
                                   uint64_t const * values,
                                   uint64_t const * offsets,
@@ -234,6 +242,8 @@ hipError_t launch_hibf_wave_count(HibfLaunch const & h,
                                   unsigned blocks,
                                   hipStream_t stream);
 
+// Counting descent, one block per query; scratch holds gstate_words words per
+// block when h.gstate_words != 0 (unused otherwise).
 hipError_t launch_hibf_count(HibfLaunch const & h,
                              uint64_t const * values,
                              uint64_t const * offsets,
@@ -241,6 +251,7 @@ hipError_t launch_hibf_count(HibfLaunch const & h,
                              uint64_t const * thresholds,
                              uint32_t counter_bytes,
                              void * counts,
+                             uint32_t * scratch,
                              unsigned blocks,
                              hipStream_t stream);
 

@@ -249,7 +249,13 @@ typedef struct hibf_gpu_hibf_s * hibf_gpu_hibf;
  *                       (NULL: allocate zeroed and fill later with hibf_gpu_hibf_emplace)
  *   h_tb_offsets        n_ibfs+1 CSR offsets into the per-technical-bin tables
  *   h_next_ibf          next_ibf_id[i][b]
- *   h_user_bin          ibf_bin_to_user_bin_id[i][b] (HIBF_GPU_BIN_MERGED/DELETED sentinels) */
+ *   h_user_bin          ibf_bin_to_user_bin_id[i][b] (HIBF_GPU_BIN_MERGED/DELETED sentinels)
+ * IBFs of any number of technical bins are accepted.  When (1 + depth) x the
+ * largest IBF's technical bins x 4 bytes exceeds 64 KiB, the descent keeps a
+ * query's per-IBF counters and per-level event lists in device memory (that
+ * many bytes per block in the agent's workspace) instead of LDS.  Limits:
+ * depth <= 16, fewer than 2^31 IBFs and user bins, word offsets multiples of
+ * 4 (HIBF_GPU_ERR_UNSUPPORTED / _INVALID_ARGUMENT otherwise). */
 int hibf_gpu_hibf_create(uint64_t n_ibfs,
                          hibf_gpu_ibf_geometry const * geoms,
                          uint64_t const * h_arena,

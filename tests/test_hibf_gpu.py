@@ -235,6 +235,93 @@ def _device_hibf(t):
                       t["number_of_user_bins"], arena=t["arena"])
 
 
+def _large_ibf_tree(root_bins: int, seed: int):
+    """A hand-built HIBF past the LDS budget of the descent: a root IBF of
+    root_bins technical bins (user bins of one TB, a few split over two, three
+    merged bins at the end), children of 700, 100 and 64 bins, and a
+    grandchild of 300 bins below the first child.  Each user bin holds 12
+    random values; a merged bin holds every value below it."""
+    rng = np.random.default_rng(seed)
+    M = O.BIN_MERGED
+    root_ub = []
+    u = 0
+    while len(root_ub) < root_bins - 3:
+        split = rng.random() < 0.05 and len(root_ub) < root_bins - 4
+        root_ub += [u, u] if split else [u]
+        u += 1
+    layout = [root_ub + [M, M, M]]
+    children = {}
+    for size in (700, 100, 64, 300):
+        n = size - 1 if size == 700 else size  # child 1's last bin is merged: the grandchild
+        layout.append(list(range(u, u + n)) + ([M] if n < size else []))
+        u += n
+    children = {(0, root_bins - 3): 1, (0, root_bins - 2): 2, (0, root_bins - 1): 3, (1, 699): 4}
+    n_ub = u
+    vals = [rng.integers(0, 2**63, size=12, dtype=np.int64).astype(np.uint64) for _ in range(n_ub)]
+
+    def below(i):
+        out = []
+        for b, x in enumerate(layout[i]):
+            out += [vals[x]] if x != M else [below(children[(i, b)])]
+        return np.concatenate(out)
+
+    geoms, words, offs, tb_offs, nxt, ubs = [], [], [], [0], [], []
+    at = 0
+    for i, row in enumerate(layout):
+        g = O.Ibf(len(row), 4096, 2)
+        for b, x in enumerate(row):
+            g.emplace(vals[x] if x != M else below(children[(i, b)]), b)
+        geoms.append((len(row), 4096, 2))
+        offs.append(at)
+        w = np.concatenate([g.words, np.zeros((-g.words.size) % 4, dtype=np.uint64)])
+        words.append(w)
+        at += w.size
+        tb_offs.append(tb_offs[-1] + len(row))
+        nxt += [children.get((i, b), i) for b in range(len(row))]
+        ubs += row
+    t = dict(geoms=geoms, word_offsets=offs, tb_offsets=tb_offs, next_ibf=nxt, user_bin=ubs,
+             number_of_user_bins=n_ub, arena=np.concatenate(words))
+    return t, vals
+
+
+@pytest.mark.parametrize("root_bins,children", [(9000, True), (12345, False)])
+def test_large_ibfs_descend_with_state_in_device_memory(root_bins, children, descent_kernel):
+    """IBFs of more than 8,192 technical bins (rows of 141 and 193 words,
+    counted 128 words at a time) and trees whose counters and event lists
+    exceed the LDS budget: membership lists in depth-first order and counts
+    at every width against the oracle, t = 0 included (every user bin)."""
+    from hibf_amd.device import to_numpy
+    _say(f"large IBFs root_bins={root_bins} children={children} kernel={descent_kernel}")
+    t, vals = _large_ibf_tree(root_bins, root_bins)
+    if not children:  # the root alone: its merged bins become deleted bins
+        n = t["geoms"][0][0]
+        t = dict(geoms=t["geoms"][:1], word_offsets=[0], tb_offsets=[0, n],
+                 next_ibf=[0] * n, user_bin=[x if x != O.BIN_MERGED else O.BIN_DELETED for x in t["user_bin"][:n]],
+                 number_of_user_bins=max(x for x in t["user_bin"][:n] if x < O.BIN_DELETED) + 1,
+                 arena=t["arena"][:t["word_offsets"][1] if len(t["word_offsets"]) > 1 else None])
+    dev = _device_hibf(t)
+    ref = oracle_of(dev)
+    rng = np.random.default_rng(5)
+    n_ub = t["number_of_user_bins"]
+    qs, ts = [], []
+    for k in range(40):
+        u = int(rng.integers(0, n_ub)) if k % 4 else n_ub - 1 - int(rng.integers(0, 400))  # deep user bins too
+        u = min(u, len(vals) - 1)
+        qs.append(np.concatenate([vals[u], rng.integers(0, 2**63, size=4, dtype=np.int64).astype(np.uint64)]))
+        ts.append(10)
+    qs += [vals[3], np.zeros(0, dtype=np.uint64)]
+    ts += [0, 0]
+    got = _membership_lists(dev, qs, ts)
+    for q, th, g in zip(qs, ts, got):
+        assert g == ref.membership_for(q, th)
+    assert sorted(set(got[-2])) == list(range(n_ub))  # t = 0: every user bin
+    values, offsets = _csr(qs)
+    cthr = [max(1, th) for th in ts]
+    for width in (1, 2, 4, 8):
+        c = to_numpy(dev.bulk_count(values, offsets, cthr, counter_bytes=width))
+        assert np.array_equal(c, ref.bulk_count_batch(values, offsets, cthr, width)), width
+
+
 def test_deleted_bin_hits_carry_into_the_next_run(descent_kernel):
     """A deleted TB adds its hits to the running sum without ending a run
     (hierarchical_interleaved_bloom_filter.hpp:331-340, 475-484): on the tiny
