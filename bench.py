@@ -135,8 +135,9 @@ def recorded_traffic(key: str, args, layout_text: str | None = None) -> float | 
     return (rec["fetch_kib"] * rec["fetch_factor"] + rec["write_kib"] * rec["write_factor"]) * 1024.0
 
 
-def host_cpu() -> dict:
-    """CPU model and core count of the host the CPU baseline runs on."""
+def host_cpu(simd: bool) -> dict:
+    """CPU model and core count of the host the CPU baseline runs on, and the
+    counting path the oracle used."""
     model = None
     try:
         with open("/proc/cpuinfo") as f:
@@ -146,8 +147,10 @@ def host_cpu() -> dict:
                     break
     except OSError:
         pass
-    return {"cpu_model": model, "nproc": os.cpu_count(),
-            "counting_path": "scalar (countr_zero over the AND of the rows; no AVX-512 path in the oracle)"}
+    path = ("AVX-512BW (counting_vector.hpp:252-278 restated: masked add of 32 uint16 counters per register, the "
+            "path of a -march=native reference build on this host)" if simd else
+            "scalar (countr_zero over the AND of the rows, counting_vector.hpp:279-311; this host has no AVX-512BW)")
+    return {"cpu_model": model, "nproc": os.cpu_count(), "counting_path": path}
 
 
 def timed_sample(run, n_probe: int, n_max: int, seconds: float) -> tuple[int, float]:
@@ -392,12 +395,14 @@ def flat_measure(args, leg: bool = False):
             def run_cpu(n, t):
                 ref.bulk_count_batch(values_h[: n * L], offsets_h[: n + 1], 2, t)
             cpu_budget = args.cpu_seconds / (4 if leg else 1)
+            simd = O.set_simd_counting(True)
             n_cpu, cpu_s = timed_sample(lambda n: run_cpu(n, threads), 64, nq, cpu_budget)
             n_one, one_s = timed_sample(lambda n: run_cpu(n, 1), 16, nq, cpu_budget / 4)
+            O.set_simd_counting(False)
             cpu = {"value": n_cpu * L / cpu_s, "unit": "lookups/s", "cores": threads, "kind": "port",
                    "sample": f"first {n_cpu} queries x {L} values of the same workload, oracle bulk_count "
                              f"(uint16) with {threads} OpenMP threads, one agent per thread, {cpu_s:.1f} s",
-                   "one_thread": {"value": n_one * L / one_s, "queries": n_one, "seconds": one_s}, **host_cpu()}
+                   "one_thread": {"value": n_one * L / one_s, "queries": n_one, "seconds": one_s}, **host_cpu(simd)}
             log(f"cpu baseline: {cpu['value'] / 1e6:.2f} M lookups/s on {threads} threads, "
                 f"{cpu['one_thread']['value'] / 1e6:.2f} M lookups/s on 1 thread")
         del words, ref
@@ -824,8 +829,10 @@ def run_hibf(args):
 
             def run_cpu(n, t):
                 ref.membership_hits_batch(va_h[: n * 250], oa_h[: n + 1], ta_h[:n], t)
+            simd = O.set_simd_counting(True)
             n_cpu, cpu_s = timed_sample(lambda n: run_cpu(n, threads), 200, nq, args.cpu_seconds)
             n_one, one_s = timed_sample(lambda n: run_cpu(n, 1), 50, nq, args.cpu_seconds / 4)
+            O.set_simd_counting(False)
             # Lookups (a value checked against one visited IBF) of the same prefix, counted by the device.
             hibf.membership_prepare(agent, n_cpu)
             agent.stats(reset=True)
@@ -840,7 +847,7 @@ def run_hibf(args):
                    "sample": f"first {n_cpu} queries of (a) ({lk_cpu} lookups), oracle membership_for with "
                              f"{threads} OpenMP threads, one agent per thread, {cpu_s:.1f} s",
                    "query_values_per_s": n_cpu * 250 / cpu_s,
-                   "one_thread": {"value": lk_one / one_s, "queries": n_one, "seconds": one_s}, **host_cpu()}
+                   "one_thread": {"value": lk_one / one_s, "queries": n_one, "seconds": one_s}, **host_cpu(simd)}
             log(f"cpu baseline: {cpu['value'] / 1e6:.2f} M lookups/s on {threads} threads, "
                 f"{cpu['one_thread']['value'] / 1e6:.2f} M lookups/s on 1 thread")
         del arena, ref

@@ -89,6 +89,8 @@ def lib():
                                       ctypes.c_uint64]
         L.orc_ibf_emplace_tracked.argtypes = [ctypes.c_void_p, ctypes.POINTER(OrcIbf), ctypes.c_void_p,
                                               ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p]
+        L.orc_set_simd_counting.restype = ctypes.c_int
+        L.orc_set_simd_counting.argtypes = [ctypes.c_int]
         L.orc_ibf_clear.argtypes = [ctypes.c_void_p, ctypes.POINTER(OrcIbf), ctypes.c_uint64]
         L.orc_ibf_increase_bin_number_to.argtypes = [ctypes.c_void_p, ctypes.POINTER(OrcIbf), ctypes.c_uint64]
         L.orc_ibf_increase_bin_number_to.restype = ctypes.c_int
@@ -160,6 +162,13 @@ def _p(a: np.ndarray):
 
 def _u64(a) -> np.ndarray:
     return np.ascontiguousarray(np.asarray(a, dtype=np.uint64))
+
+
+def set_simd_counting(on: bool) -> bool:
+    """Selects the AVX-512 restatement of counting_vector += for uint16_t
+    counters (counting_vector.hpp:252-278) when the CPU has AVX-512BW;
+    returns whether it is now in use."""
+    return bool(lib().orc_set_simd_counting(1 if on else 0))
 
 
 def hash_and_fit(h: int, seed: int, hash_shift: int, bin_size: int, technical_bins: int) -> int:

@@ -320,10 +320,63 @@ static void orc_store(void * p, uint64_t i, int width, uint64_t v)
     }
 }
 
+#if defined(__x86_64__)
+#include <immintrin.h>
+/* counting_vector.hpp:252-278, the path a reference build takes with
+ * HIBF_HAS_AVX512 (-march=native on an AVX-512 host), for uint16_t counters:
+ * 32 counters per 512-bit register; the 32 bits of the bit vector that cover
+ * them pick the lanes that get 1 (maskz_mov of set1(1)), added with
+ * wrap-around.  The reference loads and stores whole registers of its padded
+ * counter buffer; here the last chunk loads and stores only the counters below
+ * nbits, which gives the same counts. */
+__attribute__((target("avx512f,avx512bw"))) static void orc_counting_add_u16_avx512(uint16_t * counts,
+                                                                                   uint64_t const * bits,
+                                                                                   uint64_t nbits)
+{
+    uint32_t const * mask = (uint32_t const *)bits;
+    __m512i const one = _mm512_set1_epi16(1);
+    uint64_t const chunks = (nbits + 31u) / 32u;
+    for (uint64_t c = 0; c < chunks; ++c)
+    {
+        __mmask32 const live = (c + 1u) * 32u <= nbits ? (__mmask32)0xFFFFFFFFu
+                                                      : (__mmask32)((1ULL << (nbits - c * 32u)) - 1u);
+        __m512i v = _mm512_maskz_loadu_epi16(live, counts + c * 32u);
+        v = _mm512_add_epi16(v, _mm512_maskz_mov_epi16((__mmask32)mask[c], one));
+        _mm512_mask_storeu_epi16(counts + c * 32u, live, v);
+    }
+}
+#endif
+
+/* 1: uint16_t counting uses the AVX-512 restatement above (set by
+ * orc_set_simd_counting when the CPU has AVX-512BW). */
+static int orc_simd_counting = 0;
+
+/* Selects the counting path of uint16_t counters: on != 0 and a CPU with
+ * AVX-512BW gives the AVX-512 restatement, otherwise the scalar one.  Returns
+ * the path now in use (1 = AVX-512). */
+int orc_set_simd_counting(int on)
+{
+#if defined(__x86_64__)
+    __builtin_cpu_init();
+    orc_simd_counting = on && __builtin_cpu_supports("avx512bw") && __builtin_cpu_supports("avx512f");
+#else
+    (void)on;
+    orc_simd_counting = 0;
+#endif
+    return orc_simd_counting;
+}
+
 /* counting_vector.hpp:171-175 and the scalar impl :279-311: for each set bit
  * b of the bit vector (b < nbits), counts[b] += 1 modulo 2^(8*width). */
 void orc_counting_vector_add(void * counts, int width, uint64_t const * bits, uint64_t nbits)
 {
+#if defined(__x86_64__)
+    if (orc_simd_counting && width == 2)
+    {
+        orc_counting_add_u16_avx512((uint16_t *)counts, bits, nbits);
+        return;
+    }
+#endif
     uint64_t words = orc_divide_and_ceil(nbits, 64u);
     for (uint64_t w = 0; w < words; ++w)
     {

@@ -296,6 +296,42 @@ def test_emplace_tracked_occupancy():
                 bin(int(w)).count("1") for w in words0[b // 64::ibf.g.bin_words] & np.uint64(1 << (b % 64)))
 
 
+def test_simd_counting_equals_scalar():
+    """The AVX-512 restatement of counting_vector += for uint16_t counters
+    (counting_vector.hpp:252-278) counts exactly as the scalar path
+    (:279-311): bin counts that are and are not multiples of 32, wrap-around
+    past 65,535, and HIBF membership and counts built on it."""
+    if not O.set_simd_counting(True):
+        O.set_simd_counting(False)
+        pytest.skip("this CPU has no AVX-512BW")
+    try:
+        rng = np.random.default_rng(23)
+        cases = []
+        for bins, bs, h in [(1, 64, 2), (31, 300, 2), (32, 300, 3), (33, 300, 2), (100, 500, 2), (256, 997, 2),
+                            (1000, 211, 3)]:
+            ibf = O.Ibf(bins, bs, h)
+            vals = rng.integers(0, 2**63, size=3000, dtype=np.int64).astype(np.uint64)
+            ibf.emplace(vals, rng.integers(0, bins, size=vals.size).astype(np.uint64))
+            qs = [np.concatenate([vals[rng.integers(0, vals.size, 80)], rng.integers(0, 2**63, 40, dtype=np.int64)
+                                  .astype(np.uint64)]) for _ in range(20)]
+            qs.append(np.tile(vals[:7], 10000))  # 70,000 values: uint16 counts wrap
+            cases.append((ibf, qs))
+        simd = [[ibf.bulk_count(q, 2) for q in qs] for ibf, qs in cases]
+        O.set_simd_counting(False)
+        scalar = [[ibf.bulk_count(q, 2) for q in qs] for ibf, qs in cases]
+        for a, b in zip(simd, scalar):
+            for x, y in zip(a, b):
+                assert np.array_equal(x, y)
+        assert int(simd[0][-1][0]) == 70000 % 65536  # one bin holding every value of the long query
+        ibf, qs = cases[4]
+        O.set_simd_counting(True)
+        m_simd = [ibf.membership_for(q, 60) for q in qs]
+        O.set_simd_counting(False)
+        assert m_simd == [ibf.membership_for(q, 60) for q in qs]
+    finally:
+        O.set_simd_counting(False)
+
+
 def test_synth_stream_and_parallel_emplace_forms():
     """The synthetic test-input stream is the same in C and numpy, and the
     oracle's parallel emplace forms equal its sequential emplace."""
