@@ -276,9 +276,10 @@ __device__ __forceinline__ void count_into_lds(IbfView const & g,
 
 // Runtime dispatch of count_into_lds on log2(lanes per value): 0..6 give
 // LPV = 1..64 with one word per lane (bin_words <= 64); 7 gives 64 lanes with
-// two words each, over tiles of 128 words.  Used by kernels that visit IBFs of
-// different widths (the HIBF descent).  `lds` may also be device memory.
-template <int U, bool NT = false, int HF = 0, bool SLICE = true>
+// two words each (bin_words <= 128, or any width over tiles of 128 words with
+// TILED).  Used by kernels that visit IBFs of different widths (the HIBF
+// descent).  `lds` may also be device memory.
+template <int U, bool NT = false, int HF = 0, bool SLICE = true, bool TILED = false>
 __device__ __forceinline__ void count_into_lds_dispatch(uint32_t lpv_log,
                                                         IbfView const & g,
                                                         uint64_t const * __restrict__ vals,
@@ -298,10 +299,18 @@ __device__ __forceinline__ void count_into_lds_dispatch(uint32_t lpv_log,
     case 5: count_into_lds<32, 1, U, NT, HF, SLICE>(g, vals, L, 0, g.bin_words, lds, wave, nwaves, lane); break;
     case 6: count_into_lds<64, 1, U, NT, HF, SLICE>(g, vals, L, 0, g.bin_words, lds, wave, nwaves, lane); break;
     default:
-        // 64 lanes with two words each cover 128 words; wider rows are counted tile by tile.
-        for (uint32_t w0 = 0; w0 < g.bin_words; w0 += 128u)
-            count_into_lds<64, 2, U, NT, HF, SLICE>(g, vals, L, w0, min(128u, g.bin_words - w0), lds + w0 * 64u, wave,
-                                                    nwaves, lane);
+        // 64 lanes with two words each cover 128 words; with TILED, wider rows are counted tile by tile (only
+        // kernels whose state is in device memory meet IBFs that wide; the loop costs the others registers).
+        if constexpr (TILED)
+        {
+            for (uint32_t w0 = 0; w0 < g.bin_words; w0 += 128u)
+                count_into_lds<64, 2, U, NT, HF, SLICE>(g, vals, L, w0, min(128u, g.bin_words - w0), lds + w0 * 64u,
+                                                        wave, nwaves, lane);
+        }
+        else
+        {
+            count_into_lds<64, 2, U, NT, HF, SLICE>(g, vals, L, 0, g.bin_words, lds, wave, nwaves, lane);
+        }
         break;
     }
 }

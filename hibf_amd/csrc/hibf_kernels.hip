@@ -116,7 +116,7 @@ __device__ __forceinline__ void descent_body(HibfLaunch const & h,
                 else if constexpr (NARROW == 1)
                     count_into_lds_dispatch_narrow<U, NT, HF>(d.lpv_log, g, vals, L, cnt, wave, nwaves, lane);
                 else
-                    count_into_lds_dispatch<U, NT, HF, SLICE>(d.lpv_log, g, vals, L, cnt, wave, nwaves, lane);
+                    count_into_lds_dispatch<U, NT, HF, SLICE, GSTATE>(d.lpv_log, g, vals, L, cnt, wave, nwaves, lane);
                 ++visits;
                 lookups += L;
                 row_bytes += static_cast<uint64_t>(L) * d.hash_funs * d.bin_words * 8u;
