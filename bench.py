@@ -930,13 +930,12 @@ def descent_kernel_name(native) -> str:
     an HIBF whose IBFs all use two hash functions and rows of at most 4 words."""
     if native.tune_get("hibf.wave_kernel"):
         return "hibf_wave_membership_h2_w4_kernel (+ hibf_descent_* for queries beyond its LDS id buffer)"
-    if native.tune_get("hibf.nt_loads"):
-        return "hibf_descent_kernel<false, true, 0>"
+    nt = "true" if native.tune_get("hibf.nt_loads") else "false"
     if native.tune_get("hibf.row_per_lane"):
-        return "hibf_descent_membership_h2_row_per_lane_kernel"
+        return f"hibf_descent_membership_h2_row_per_lane_kernel<{nt}>"
     sliced = native.tune_get("hibf.sliced_counts")
     unroll = native.tune_get("hibf.unroll") if sliced else 2
-    return f"hibf_descent_membership_h2_w4_kernel<{'true' if sliced else 'false'}, {unroll}>"
+    return f"hibf_descent_membership_h2_w4_kernel<{'true' if sliced else 'false'}, {unroll}, {nt}>"
 
 
 def serve_sweep(hibf, tab, sizes, spec, args, rates, windows, seconds):

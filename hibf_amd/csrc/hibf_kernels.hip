@@ -233,29 +233,57 @@ __global__ void __launch_bounds__(512) hibf_descent_gstate_kernel(HIBF_DESCENT_P
 
 // Membership with two hash functions (the HIBF default) fits 64 VGPRs without
 // spilling, so it runs at 8 waves per SIMD: more row fetches in flight.
+// NT: non-temporal row loads.
+template <bool NT = false>
 __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8, 8)))
 hibf_descent_membership_h2_kernel(HIBF_DESCENT_PARAMS)
 {
-    descent_body<false, false, 2, 2, 0, false>(HIBF_DESCENT_ARGS);
+    descent_body<false, NT, 2, 2, 0, false>(HIBF_DESCENT_ARGS);
 }
 
 // Same, for HIBFs whose rows are at most 4 words: one value and its whole row per lane.
+template <bool NT = false>
 __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8, 8)))
 hibf_descent_membership_h2_row_per_lane_kernel(HIBF_DESCENT_PARAMS)
 {
-    descent_body<false, false, 2, 1, 2>(HIBF_DESCENT_ARGS);
+    descent_body<false, NT, 2, 1, 2>(HIBF_DESCENT_ARGS);
 }
 
 // Membership with two hash functions over rows of at most 4 words (the
 // tmax = 256 layout): only the 1-, 2- and 4-word counting shapes are compiled.
 // SLICE: bit-sliced per-lane counts with one LDS atomic per bin and chunk.
 // U: value groups whose rows a wave loads at once (2: 32 values of 4-word rows,
-// 4: 64 values, a whole chunk per memory round trip).
-template <bool SLICE, int U = 2>
+// 4: 64 values, a whole chunk per memory round trip).  NT: non-temporal row loads.
+template <bool SLICE, int U = 2, bool NT = false>
 __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(8, 8)))
 hibf_descent_membership_h2_w4_kernel(HIBF_DESCENT_PARAMS)
 {
-    descent_body<false, false, 2, U, 1, SLICE>(HIBF_DESCENT_ARGS);
+    descent_body<false, NT, 2, U, 1, SLICE>(HIBF_DESCENT_ARGS);
+}
+
+using DescentKernel = void (*)(HibfLaunch, uint64_t const *, uint64_t const *, uint64_t, uint16_t const *,
+                               uint64_t const *, uint32_t, void *, uint32_t *, uint64_t *, uint64_t, uint64_t *,
+                               uint64_t *, uint64_t const *);
+
+// The membership kernel of an HIBF under the current launch-shape knobs.
+template <bool NT>
+DescentKernel membership_kernel(HibfLaunch const & h)
+{
+    bool const h2 = h.uniform_hash_funs == 2u;
+    bool const w4 = h2 && h.max_bin_words <= 4u;
+    CountTuning const & t = g_count_tuning;
+    if (h.gstate_words)
+        return hibf_descent_gstate_kernel<false>;
+    if (w4 && t.row_per_lane)
+        return hibf_descent_membership_h2_row_per_lane_kernel<NT>;
+    if (w4 && t.sliced_counts)
+        return t.hibf_unroll == 4u ? hibf_descent_membership_h2_w4_kernel<true, 4, NT>
+                                   : hibf_descent_membership_h2_w4_kernel<true, 2, NT>;
+    if (w4)
+        return hibf_descent_membership_h2_w4_kernel<false, 2, NT>;
+    if (h2)
+        return hibf_descent_membership_h2_kernel<NT>;
+    return hibf_descent_kernel<false, NT, 0>;
 }
 
 #undef HIBF_DESCENT_PARAMS
@@ -430,17 +458,7 @@ hipError_t launch_hibf_membership(HibfLaunch const & h,
     if (nq == 0)
         return hipSuccess;
     size_t const lds = h.gstate_words ? 0u : static_cast<size_t>(h.max_bins) * (1u + h.depth) * sizeof(uint32_t);
-    bool const h2 = h.uniform_hash_funs == 2u;
-    bool const w4 = h2 && h.max_bin_words <= 4u;
-    auto const kernel = h.gstate_words                         ? hibf_descent_gstate_kernel<false>
-                        : g_count_tuning.nt_loads              ? hibf_descent_kernel<false, true, 0>
-                        : w4 && g_count_tuning.row_per_lane     ? hibf_descent_membership_h2_row_per_lane_kernel
-                        : w4 && g_count_tuning.sliced_counts && g_count_tuning.hibf_unroll == 4u
-                                                               ? hibf_descent_membership_h2_w4_kernel<true, 4>
-                        : w4 && g_count_tuning.sliced_counts   ? hibf_descent_membership_h2_w4_kernel<true, 2>
-                        : w4                                   ? hibf_descent_membership_h2_w4_kernel<false, 2>
-                        : h2                                   ? hibf_descent_membership_h2_kernel
-                                                               : hibf_descent_kernel<false, false, 0>;
+    DescentKernel const kernel = g_count_tuning.nt_loads ? membership_kernel<true>(h) : membership_kernel<false>(h);
     unsigned const threads = 64u * g_count_tuning.hibf_block_waves;
     hipLaunchKernelGGL(kernel, dim3(blocks), dim3(threads), lds, stream, h, values, offsets, nq, thresholds, nullptr,
                        2u, nullptr, scratch, ids_tmp, ids_cap, hit_start, hit_count, qlist);
