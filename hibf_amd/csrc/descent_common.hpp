@@ -107,25 +107,55 @@ __device__ __forceinline__ uint32_t scan_ibf(HibfIbfDesc const & d,
             uint32_t const kind = static_cast<uint32_t>(info >> 32);
             uint32_t const payload = static_cast<uint32_t>(info);
             bool const reset = valid && (kind == kTbRunEnd || kind == kTbMerged);
-            uint64_t s = valid ? (static_cast<uint64_t>(cnt[tb]) & mask) : 0ULL;
-            // Segment heads: the bin after a reset point.
-            bool const prev_reset = __shfl_up(static_cast<int>(reset), 1, 64) != 0;
-            bool f = lane > 0 && prev_reset;
-#pragma unroll
-            for (int dlt = 1; dlt < 64; dlt <<= 1)
+            uint64_t sum;
+            if (counter_bytes <= 4u)
             {
-                uint64_t const y = shfl64(s, lane >= dlt ? lane - dlt : lane);
-                bool const fy = __shfl(static_cast<int>(f), lane >= dlt ? lane - dlt : lane, 64) != 0;
-                if (lane >= dlt)
+                // Sums modulo 2^32 suffice for counters of up to 4 bytes: a run's sum is the difference of two
+                // plain prefix sums (the one at its reset point and the one at the reset point before it), or
+                // the carried sum plus the prefix when no reset precedes it in this chunk.
+                uint32_t const p = wave_inclusive_add_u32(valid ? cnt[tb] : 0u);
+                uint64_t const resets = __ballot(reset);
+                uint64_t const before = resets & ((1ULL << lane) - 1ULL);
+                int const prev = before ? 63 - __builtin_clzll(before) : lane;
+                uint32_t const at_prev = static_cast<uint32_t>(__shfl(static_cast<int>(p), prev, 64));
+                sum = static_cast<uint64_t>(before ? p - at_prev : static_cast<uint32_t>(carry) + p) & mask;
+                uint32_t const total = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(p), 63));
+                if (resets)
                 {
-                    if (!f)
-                        s += y;
-                    f = f || fy;
+                    int const last = 63 - __builtin_clzll(resets);
+                    carry = total - static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(p), last));
+                }
+                else
+                {
+                    carry = static_cast<uint32_t>(carry) + total;
                 }
             }
-            if (!f)
-                s += carry;
-            uint64_t const sum = s & mask;
+            else
+            {
+                uint64_t s = valid ? (static_cast<uint64_t>(cnt[tb]) & mask) : 0ULL;
+                // Segment heads: the bin after a reset point.
+                bool const prev_reset = __shfl_up(static_cast<int>(reset), 1, 64) != 0;
+                bool f = lane > 0 && prev_reset;
+#pragma unroll
+                for (int dlt = 1; dlt < 64; dlt <<= 1)
+                {
+                    uint64_t const y = shfl64(s, lane >= dlt ? lane - dlt : lane);
+                    bool const fy = __shfl(static_cast<int>(f), lane >= dlt ? lane - dlt : lane, 64) != 0;
+                    if (lane >= dlt)
+                    {
+                        if (!f)
+                            s += y;
+                        f = f || fy;
+                    }
+                }
+                if (!f)
+                    s += carry;
+                sum = s & mask;
+                // Pending sum after this chunk: lane 63's segment sum unless it is a reset point.
+                uint64_t const last_s = shfl64(s, 63);
+                bool const last_reset = __shfl(static_cast<int>(reset), 63, 64) != 0;
+                carry = last_reset ? 0ULL : last_s;
+            }
             bool const hit = reset && sum >= threshold;
             bool is_event = hit;
             if constexpr (COUNTING)
@@ -142,10 +172,6 @@ __device__ __forceinline__ uint32_t scan_ibf(HibfIbfDesc const & d,
             if (is_event)
                 ev[nev + before] = payload | (kind == kTbMerged ? kEvMerged : 0u);
             nev += static_cast<uint32_t>(__popcll(ballot));
-            // Pending sum after this chunk: lane 63's segment sum unless it is a reset point.
-            uint64_t const last_s = shfl64(s, 63);
-            bool const last_reset = __shfl(static_cast<int>(reset), 63, 64) != 0;
-            carry = last_reset ? 0ULL : last_s;
         }
     }
     return nev;

@@ -111,6 +111,23 @@ __device__ __forceinline__ bool highest_lane_per_id(bool store, uint32_t id, int
     return store;
 }
 
+// Inclusive prefix sum of a uint32 over the 64 lanes of a wave: lane i gets
+// x[0] + ... + x[i] modulo 2^32.  DPP row shifts by 1, 2, 4 and 8 lanes build
+// the prefix inside each row of 16 lanes (lanes whose source lies before the
+// row start add 0); broadcasting lane 15 of a row into the next (rows 1, 3)
+// and lane 31 into rows 2 and 3 adds the totals of the rows before.
+__device__ __forceinline__ uint32_t wave_inclusive_add_u32(uint32_t x)
+{
+    int v = static_cast<int>(x);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false); // row_shr:1
+    v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false); // row_shr:2
+    v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false); // row_shr:4
+    v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false); // row_shr:8
+    v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false); // row_bcast:15, rows 1 and 3
+    v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false); // row_bcast:31, rows 2 and 3
+    return static_cast<uint32_t>(v);
+}
+
 __device__ __forceinline__ uint64_t readlane64(uint64_t x, int src)
 {
     uint32_t lo = __builtin_amdgcn_readlane(static_cast<int>(static_cast<uint32_t>(x)), src);
