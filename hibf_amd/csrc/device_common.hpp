@@ -197,42 +197,104 @@ __device__ __forceinline__ void count_into_lds(IbfView const & g,
         for (uint32_t s = 0; s < nv; s += VPS * U)
         {
             uint64_t x[U][CW];
-#pragma unroll
-            for (int u = 0; u < U; ++u)
+            if constexpr (LPV <= 4)
             {
-                uint32_t const idx = s + static_cast<uint32_t>(u * VPS + sub);
-                bool const act = idx < nv;
-                uint64_t r[KMAX];
+                // All U x CW x h row loads of this step are issued before any is used: every lane loads without a
+                // branch (a branch around the loads makes the compiler wait for them at its end), lanes without a
+                // value or a word read word 0 of the matrix (always in bounds) and drop it, and a scheduling fence
+                // keeps the ANDs after the last load.
+                uint64_t ld[U][CW][KMAX];
 #pragma unroll
-                for (int k = 0; k < KMAX; ++k)
+                for (int u = 0; u < U; ++u)
                 {
-                    r[k] = 0ULL;
-                    if (HF > 0 || static_cast<uint32_t>(k) < hf)
+                    uint32_t const idx = s + static_cast<uint32_t>(u * VPS + sub);
+                    bool const act = idx < nv;
+                    uint64_t r[KMAX];
+#pragma unroll
+                    for (int k = 0; k < KMAX; ++k)
                     {
-                        if constexpr (LPV == 1 && U == 1)
-                            r[k] = rows[k]; // each lane's own value (idx == lane)
-                        else if constexpr (LPV == 64)
-                            r[k] = readlane64(rows[k], static_cast<int>(min(idx, 63u)));
-                        else
-                            r[k] = shfl64(rows[k], static_cast<int>(min(idx, 63u)));
+                        r[k] = 0ULL;
+                        if (HF > 0 || static_cast<uint32_t>(k) < hf)
+                        {
+                            if constexpr (LPV == 1 && U == 1)
+                                r[k] = rows[k]; // each lane's own value (idx == lane)
+                            else if constexpr (LPV == 64)
+                                r[k] = readlane64(rows[k], static_cast<int>(min(idx, 63u)));
+                            else
+                                r[k] = shfl64(rows[k], static_cast<int>(min(idx, 63u)));
+                        }
+                    }
+#pragma unroll
+                    for (int c = 0; c < CW; ++c)
+                    {
+                        uint32_t const wt = wl + static_cast<uint32_t>(c * LPV); // word within tile
+                        bool const live = act && wt < tw;
+                        uint64_t const w = live ? w0 + wt : 0ULL;
+#pragma unroll
+                        for (int k = 0; k < KMAX; ++k)
+                            ld[u][c][k] = (HF > 0 || static_cast<uint32_t>(k) < hf)
+                                              ? load_row_word<NT>(words + (live ? r[k] : 0ULL) + w)
+                                              : ~0ULL;
                     }
                 }
+                __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-                for (int c = 0; c < CW; ++c)
+                for (int u = 0; u < U; ++u)
                 {
-                    uint32_t const wt = wl + static_cast<uint32_t>(c * LPV); // word within tile
-                    uint64_t acc = 0ULL;
-                    if (act && wt < tw)
+                    bool const act = s + static_cast<uint32_t>(u * VPS + sub) < nv;
+#pragma unroll
+                    for (int c = 0; c < CW; ++c)
                     {
-                        uint64_t const w = w0 + wt;
-                        acc = load_row_word<NT>(words + r[0] + w);
+                        uint32_t const wt = wl + static_cast<uint32_t>(c * LPV);
+                        bool const live = act && wt < tw;
+                        uint64_t acc = ld[u][c][0];
 #pragma unroll
                         for (int k = 1; k < KMAX; ++k)
-                            if (HF > 0 || static_cast<uint32_t>(k) < hf)
-                                acc &= load_row_word<NT>(words + r[k] + w);
-                        acc &= tail_mask(g, static_cast<uint32_t>(w));
+                            acc &= ld[u][c][k];
+                        acc &= tail_mask(g, w0 + wt);
+                        x[u][c] = live ? acc : 0ULL;
                     }
-                    x[u][c] = acc;
+                }
+            }
+            else
+            {
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                {
+                    uint32_t const idx = s + static_cast<uint32_t>(u * VPS + sub);
+                    bool const act = idx < nv;
+                    uint64_t r[KMAX];
+#pragma unroll
+                    for (int k = 0; k < KMAX; ++k)
+                    {
+                        r[k] = 0ULL;
+                        if (HF > 0 || static_cast<uint32_t>(k) < hf)
+                        {
+                            if constexpr (LPV == 1 && U == 1)
+                                r[k] = rows[k]; // each lane's own value (idx == lane)
+                            else if constexpr (LPV == 64)
+                                r[k] = readlane64(rows[k], static_cast<int>(min(idx, 63u)));
+                            else
+                                r[k] = shfl64(rows[k], static_cast<int>(min(idx, 63u)));
+                        }
+                    }
+#pragma unroll
+                    for (int c = 0; c < CW; ++c)
+                    {
+                        uint32_t const wt = wl + static_cast<uint32_t>(c * LPV); // word within tile
+                        uint64_t acc = 0ULL;
+                        if (act && wt < tw)
+                        {
+                            uint64_t const w = w0 + wt;
+                            acc = load_row_word<NT>(words + r[0] + w);
+#pragma unroll
+                            for (int k = 1; k < KMAX; ++k)
+                                if (HF > 0 || static_cast<uint32_t>(k) < hf)
+                                    acc &= load_row_word<NT>(words + r[k] + w);
+                            acc &= tail_mask(g, static_cast<uint32_t>(w));
+                        }
+                        x[u][c] = acc;
+                    }
                 }
             }
 #pragma unroll
