@@ -128,6 +128,30 @@ __device__ __forceinline__ uint32_t wave_inclusive_add_u32(uint32_t x)
     return static_cast<uint32_t>(v);
 }
 
+// Values every lane of the wave holds alike, moved to scalar registers, so
+// that addresses built from them are uniform.
+__device__ __forceinline__ uint32_t uniform32(uint32_t x)
+{
+    return static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(x)));
+}
+__device__ __forceinline__ uint64_t uniform64(uint64_t x)
+{
+    uint32_t const lo = uniform32(static_cast<uint32_t>(x));
+    uint32_t const hi = uniform32(static_cast<uint32_t>(x >> 32));
+    return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+
+// Load of a 32- or 64-bit word of memory that no thread writes while the
+// kernel runs: through the constant address space, so that a uniform address
+// becomes a scalar load served by the scalar data cache.
+template <typename T>
+__device__ __forceinline__ T const_load(T const * p)
+{
+    static_assert(sizeof(T) == 4 || sizeof(T) == 8, "scalar loads are of dwords");
+    using const_ptr = T const __attribute__((address_space(4))) *;
+    return *(const_ptr)p; // address-space cast: flat and constant addresses of global memory are the same
+}
+
 __device__ __forceinline__ uint64_t readlane64(uint64_t x, int src)
 {
     uint32_t lo = __builtin_amdgcn_readlane(static_cast<int>(static_cast<uint32_t>(x)), src);

@@ -78,14 +78,15 @@ __device__ __forceinline__ void descent_body(HibfLaunch const & h,
             }
         }
         __syncthreads();
-        uint64_t const q = next_q;
+        uint64_t const q = uniform64(next_q);
         __syncthreads();
         if (q >= nq)
             break;
-        uint64_t const beg = offsets[q];
-        uint32_t const L = static_cast<uint32_t>(offsets[q + 1] - beg);
+        // The query's offsets and threshold (read-only inputs, uniform addresses) come through scalar loads.
+        uint64_t const beg = const_load(offsets + q);
+        uint32_t const L = static_cast<uint32_t>(const_load(offsets + q + 1) - beg);
         uint64_t const * const vals = values + beg;
-        uint64_t const threshold = COUNTING ? thr64[q] : static_cast<uint64_t>(thr16[q]);
+        uint64_t const threshold = COUNTING ? const_load(thr64 + q) : static_cast<uint64_t>(thr16[q]);
         uint32_t const cb = COUNTING ? counter_bytes : 2u;
         void * const row = COUNTING ? static_cast<void *>(static_cast<char *>(counts) + q * h.n_user_bins * cb)
                                     : nullptr;
@@ -106,7 +107,8 @@ __device__ __forceinline__ void descent_body(HibfLaunch const & h,
         {
             if (descend)
             {
-                HibfIbfDesc const d = h.ibfs[ibf];
+                // Scalar loads of the descriptor where the registers allow them (the narrow-row kernels).
+                HibfIbfDesc const d = NARROW == 1 ? load_desc(h.ibfs + ibf) : h.ibfs[ibf];
                 for (uint32_t i = static_cast<uint32_t>(tid); i < d.bins; i += static_cast<uint32_t>(nthreads))
                     cnt[i] = 0u;
                 __syncthreads();
@@ -129,14 +131,14 @@ __device__ __forceinline__ void descent_body(HibfLaunch const & h,
                         st.nev[depth] = n;
                 }
                 __syncthreads();
-                nev = st.nev[depth];
+                nev = uniform32(st.nev[depth]);
                 p = 0;
                 descend = false;
             }
             if (p < nev)
             {
                 // Next event of the current level, in technical-bin order.
-                uint32_t const e = evs[depth * h.max_bins + p];
+                uint32_t const e = uniform32(evs[depth * h.max_bins + p]);
                 ++p;
                 if (e & kEvMerged)
                 {
@@ -169,9 +171,9 @@ __device__ __forceinline__ void descent_body(HibfLaunch const & h,
             if (depth == 0)
                 break;
             --depth;
-            ibf = st.ibf_at[depth];
-            p = st.pos[depth];
-            nev = st.nev[depth];
+            ibf = uniform32(st.ibf_at[depth]);
+            p = uniform32(st.pos[depth]);
+            nev = uniform32(st.nev[depth]);
             __syncthreads();
         }
 

@@ -42,6 +42,21 @@ struct HibfIbfDesc
 
 void count_shape(uint32_t bin_words, uint32_t & lpv_log, uint32_t & cw, uint32_t & tile_words);
 
+// An IBF's descriptor at a uniform address, with scalar loads (the tables do
+// not change while a kernel runs).
+__device__ __forceinline__ HibfIbfDesc load_desc(HibfIbfDesc const * p)
+{
+    static_assert(sizeof(HibfIbfDesc) == 48, "six 64-bit words");
+    uint64_t const * const w = reinterpret_cast<uint64_t const *>(p);
+    uint64_t raw[6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i)
+        raw[i] = const_load(w + i);
+    HibfIbfDesc d;
+    __builtin_memcpy(&d, raw, sizeof d);
+    return d;
+}
+
 // Launch-shape knobs of the count kernels, chosen by the autotuner
 // (hibf_gpu_tune_set); defaults are the tuned values for MI355X.
 struct CountTuning
