@@ -104,6 +104,35 @@ def kmer_hash(reads, k: int) -> tuple[torch.Tensor, torch.Tensor]:
     return hashes[: total.value], out_off
 
 
+# seqan3::views::minimiser_hash's default seed.
+MINIMISER_SEED = 0x8F3F73B5CF1C9ADE
+
+
+def minimiser_hash(reads, shape: int, window_size: int, seed: int = MINIMISER_SEED) -> tuple[torch.Tensor, torch.Tensor]:
+    """Device minimiser hashes of DNA reads (hibf_gpu_minimiser_hash), as
+    seqan3::views::minimiser_hash(shape, window_size{window_size}, seed{seed})
+    computes them: (hashes, offsets) as uint64 device tensors in CSR form.
+    `shape` is seqan3's binary literal (most significant set bit = first
+    position; ungapped{k} = 2^k - 1).  `reads` is a list of str/bytes or a
+    (bases uint8, offsets uint64) CSR pair."""
+    data, offsets = reads if isinstance(reads, tuple) else reads_csr(reads)
+    offsets = np.ascontiguousarray(offsets, dtype=np.uint64)
+    dev = _dev()
+    d_seq = torch.from_numpy(np.ascontiguousarray(data).copy()).to(dev) if data.size else \
+        torch.zeros(1, dtype=torch.uint8, device=dev)
+    d_off = as_device_u64(offsets)
+    n = offsets.size - 1
+    out_off = torch.empty(n + 1, dtype=torch.uint64, device=dev)
+    total = ctypes.c_uint64()
+    # At most one minimiser per k-mer position: the base count bounds them.
+    cap = max(int(offsets[-1]), 1)
+    hashes = torch.empty(cap, dtype=torch.uint64, device=dev)
+    check(lib().hibf_gpu_minimiser_hash(_ptr(d_seq), _ptr(d_off), n, ctypes.c_uint64(shape), window_size,
+                                        ctypes.c_uint64(seed), _ptr(out_off), _ptr(hashes), cap,
+                                        ctypes.byref(total), _stream()))
+    return hashes[: total.value], out_off
+
+
 def hll_sketches(sets, bits: int = 12) -> torch.Tensor:
     """HyperLogLog sketches of value sets on the device (hibf_gpu_hll_sketch):
     a (len(sets), 2^bits) uint8 device tensor, one row of registers per set

@@ -139,6 +139,7 @@ _SIGNATURES = {
     "hibf_gpu_ibf_increase_bin_number_to": (_INT, [_V, _U64]),
     "hibf_gpu_ibf_set_occupancy": (_INT, [_V, _V, _INT]),
     "hibf_gpu_kmer_hash": (_INT, [_V, _V, _U64, _U32, _V, _V, _U64, ctypes.POINTER(_U64), _V]),
+    "hibf_gpu_minimiser_hash": (_INT, [_V, _V, _U64, _U64, _U32, _U64, _V, _V, _U64, ctypes.POINTER(_U64), _V]),
     "hibf_gpu_kmer_thresholds": (_INT, [_V, _U64, ctypes.c_double, _V, _V]),
     "hibf_gpu_hll_sketch": (_INT, [_V, _V, _U64, ctypes.c_uint32, _V, _V]),
     "hibf_gpu_hll_sketch_synthetic": (_INT, [_U64, _V, _U64, ctypes.c_uint32, _V, _V]),

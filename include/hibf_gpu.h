@@ -551,6 +551,40 @@ int hibf_gpu_kmer_hash(const uint8_t * d_seq,
                        uint64_t * h_total,
                        void * stream);
 
+/* Minimiser hashes of DNA reads: seqan3::views::minimiser_hash(shape,
+ * window_size, seed) over dna4 (seqan3 3.4.2, the version the reference pins
+ * for its tutorial, cmake/package-lock.cmake:50-56; the library itself takes
+ * any uint64 values, include/hibf/hierarchical_interleaved_bloom_filter.hpp:
+ * 88-93 names minimisers among them).  `shape` is seqan3's binary literal:
+ * the most significant set bit is the window's first position, so
+ * ungapped{k} is 2^k - 1 and 0b1101 uses positions 0, 1 and 3 of 4; its bit
+ * length (the shape size) is at most 32.  Per window position the value is
+ * min(forward ^ seed, reverse ^ seed), forward the base-4 number of the dna4
+ * ranks at the used positions (first most significant), reverse the same
+ * shape over the window's reverse complement.  Of each window of
+ * window_size - size + 1 values (all values, when a read has fewer) the
+ * minimum is emitted when it changes as seqan3's minimiser view emits it:
+ * the first window's rightmost minimum; then the entering value if strictly
+ * smaller than the tracked one, or the window's rightmost minimum again when
+ * the tracked one leaves.  A read shorter than the shape has none.  Output as
+ * hibf_gpu_kmer_hash (CSR offsets, *h_total, HIBF_GPU_ERR_CAPACITY).
+ * HIBF_GPU_ERR_INVALID_ARGUMENT for shape 0 or window_size < shape size
+ * (seqan3 throws std::invalid_argument there); HIBF_GPU_ERR_UNSUPPORTED for
+ * shapes over 32 positions or windows over HIBF_GPU_MAX_MINIMISER_WINDOW
+ * values.  No counterpart in the reference library.  Synchronous. */
+#define HIBF_GPU_MAX_MINIMISER_WINDOW 4096u
+int hibf_gpu_minimiser_hash(const uint8_t * d_seq,
+                            const uint64_t * d_seq_offsets,
+                            uint64_t n_seqs,
+                            uint64_t shape,
+                            uint32_t window_size,
+                            uint64_t seed,
+                            uint64_t * d_hash_offsets,
+                            uint64_t * d_hashes,
+                            uint64_t hashes_cap,
+                            uint64_t * h_total,
+                            void * stream);
+
 /* Per-read thresholds uint16_t(fraction * number of k-mers)
  * (test/tutorial/search.cpp:42), truncated and capped at 65535.  Asynchronous. */
 int hibf_gpu_kmer_thresholds(const uint64_t * d_hash_offsets,

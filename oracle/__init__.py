@@ -149,6 +149,9 @@ def lib():
             [ctypes.c_void_p] * 4 + [ctypes.c_uint64, ctypes.c_int]
         L.orc_ibf_emplace_parallel.argtypes = [ctypes.c_void_p, ctypes.POINTER(OrcIbf), ctypes.c_void_p,
                                                ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int]
+        L.orc_minimiser_hash.restype = ctypes.c_uint64
+        L.orc_minimiser_hash.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint32,
+                                         ctypes.c_uint64, ctypes.c_void_p, ctypes.c_uint64]
         _lib = L
     return _lib
 
@@ -433,6 +436,28 @@ def kmer_hashes(seq, k: int) -> np.ndarray:
     for j in range(k):
         h = (h << np.uint64(2)) | r[j:j + n]
     return h
+
+
+# seqan3::views::minimiser_hash's default seed (seqan3 3.4.2, search/views/minimiser_hash.hpp).
+MINIMISER_SEED = 0x8F3F73B5CF1C9ADE
+
+
+def minimiser_hashes(seq, shape: int, window_size: int, seed: int = MINIMISER_SEED) -> np.ndarray:
+    """Minimiser hashes of one DNA read as seqan3::views::minimiser_hash(shape,
+    window_size{window_size}, seed{seed}) over dna4 computes them (restated in
+    hibf_oracle.c, orc_minimiser_hash).  `shape` is seqan3's binary literal:
+    the most significant set bit is the window's first position
+    (ungapped{k} = 2^k - 1)."""
+    if shape <= 0 or shape.bit_length() > 32:
+        raise ValueError("shape must be non-zero with at most 32 positions")
+    if window_size < shape.bit_length():
+        raise ValueError("The size of the shape cannot be greater than the window size.")
+    b = np.frombuffer(seq.encode() if isinstance(seq, str) else bytes(seq), dtype=np.uint8)
+    n = max(0, b.size - shape.bit_length() + 1)
+    out = np.zeros(max(n, 1), dtype=np.uint64)
+    got = lib().orc_minimiser_hash(_p(b) if b.size else None, b.size, shape, window_size, seed, _p(out), out.size)
+    assert got <= out.size
+    return out[:got].copy()
 
 
 def kmer_threshold(n_kmers: int, fraction: float) -> int:

@@ -36,6 +36,11 @@
  *                          own robin_hood set (tests/golden/robin_hood_orders.json)
  *                          and, where the reference tree is present, that set
  *                          run on random scripts (tests/test_split_order.py).
+ *   orc_minimiser_hash     seqan3's minimiser_hash (a dependency of the
+ *                          reference's tutorial, absent here): an independent
+ *                          chain form of the rule, closed forms and invariants
+ *                          (tests/test_oracle.py); parity with seqan3's own
+ *                          output is unpinned.
  */
 #include <math.h>
 #include <stdio.h>
@@ -1206,4 +1211,119 @@ void orc_fs_insert_set(orc_fs * dst, orc_fs const * src)
     for (uint64_t j = 0; j < n; ++j)
         orc_fs_emplace(dst, order[j]);
     free(order);
+}
+
+/* ------------------------------------------------------------------------
+ * Minimiser hashes of a DNA read: seqan3::views::minimiser_hash(shape,
+ * window_size, seed) over dna4, seqan3 3.4.2 (the version the reference pins
+ * for its tutorial, cmake/package-lock.cmake:50-56; seqan3 is not part of the
+ * reference tree, and the reference's own tutorial hashes with kmer_hash,
+ * test/tutorial/search.cpp:40).  Restated from seqan3's published definition
+ * (search/views/minimiser_hash.hpp, search/views/minimiser.hpp):
+ *   - shape: bits as written in seqan3's binary literal, the most significant
+ *     set bit is the window's first position (0b1101: use, use, skip, use);
+ *     size = bit length, the k-mer hash is the base-4 number of the dna4 ranks
+ *     at the used positions, first position most significant;
+ *   - forward[i]  = hash of the window at i, XOR seed;
+ *   - reverse[i]  = the hash at i of complement(text) reversed, taken from the
+ *     far end, XOR seed (the shape is applied to the reverse complement
+ *     window, not mirrored);
+ *   - value[i] = min(forward[i], reverse[i]); n = len - size + 1 values;
+ *   - a window of W = window_size - size + 1 values (W = n when n < W); the
+ *     first window emits its minimum, the rightmost one on ties; after that,
+ *     when a value enters, the tracked minimum is recomputed (rightmost on
+ *     ties) and emitted if its position just left the window, else the
+ *     entering value is emitted and tracked if strictly smaller, else nothing.
+ * Pinned by: the chain form of the same rule computed independently in
+ * tests/test_oracle.py (next position = the first strictly smaller value
+ * within W, else the rightmost minimum of the following W values), W = 1
+ * (every canonical value is emitted), the forward hashes against the k-mer
+ * oracle, and window-minimum invariants.  Parity with seqan3's own output is
+ * unpinned (no seqan3 output is held anywhere the tests can read). */
+static inline uint64_t orc_dna4_rank(uint8_t c)
+{
+    switch (c | 0x20u)
+    {
+    case 'c': return 1u;
+    case 'g': return 2u;
+    case 't':
+    case 'u': return 3u;
+    default: return 0u;
+    }
+}
+
+static uint64_t orc_shape_hash(uint8_t const * seq, uint64_t i, uint64_t shape, uint32_t size, int reverse)
+{
+    uint64_t h = 0;
+    for (uint32_t t = 0; t < size; ++t)
+    {
+        if (!((shape >> (size - 1u - t)) & 1u))
+            continue;
+        uint64_t const r = reverse ? 3u - orc_dna4_rank(seq[i + size - 1u - t]) : orc_dna4_rank(seq[i + t]);
+        h = (h << 2) | r;
+    }
+    return h;
+}
+
+/* Writes up to cap hashes to out; returns the number of minimisers (also when
+ * it exceeds cap).  shape != 0 with bit length <= 32 and window_size >= that
+ * length are the caller's to check. */
+uint64_t orc_minimiser_hash(uint8_t const * seq, uint64_t len, uint64_t shape, uint32_t window_size, uint64_t seed,
+                            uint64_t * out, uint64_t cap)
+{
+    uint32_t size = 0;
+    while (size < 64u && (shape >> size))
+        ++size;
+    if (len < size)
+        return 0;
+    uint64_t const n = len - size + 1u;
+    uint64_t w = (uint64_t)window_size - size + 1u;
+    if (w > n)
+        w = n;
+    uint64_t * const ring = (uint64_t *)malloc(w * sizeof(uint64_t)); /* window values, oldest at head */
+    uint64_t head = 0, count = 0;
+    uint64_t min_value = 0, min_offset = 0; /* offset from the window's oldest value */
+#define ORC_VALUE(i) ({ uint64_t f_ = orc_shape_hash(seq, (i), shape, size, 0) ^ seed; \
+                        uint64_t r_ = orc_shape_hash(seq, (i), shape, size, 1) ^ seed; f_ < r_ ? f_ : r_; })
+#define ORC_EMIT(v) do { if (count < cap) out[count] = (v); ++count; } while (0)
+    for (uint64_t i = 0; i < w; ++i)
+        ring[i] = ORC_VALUE(i);
+    for (uint64_t i = 0; i < w; ++i)
+        if (i == 0 || ring[i] <= min_value)
+        {
+            min_value = ring[i];
+            min_offset = i;
+        }
+    ORC_EMIT(min_value);
+    for (uint64_t i = w; i < n; ++i)
+    {
+        uint64_t const v = ORC_VALUE(i);
+        ring[head] = v; /* the oldest value leaves, v enters at the back */
+        head = (head + 1u) % w;
+        if (min_offset == 0)
+        {
+            for (uint64_t j = 0; j < w; ++j)
+            {
+                uint64_t const x = ring[(head + j) % w];
+                if (j == 0 || x <= min_value)
+                {
+                    min_value = x;
+                    min_offset = j;
+                }
+            }
+            ORC_EMIT(min_value);
+        }
+        else if (v < min_value)
+        {
+            min_value = v;
+            min_offset = w - 1u;
+            ORC_EMIT(min_value);
+        }
+        else
+            --min_offset;
+    }
+#undef ORC_VALUE
+#undef ORC_EMIT
+    free(ring);
+    return count;
 }

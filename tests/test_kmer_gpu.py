@@ -44,6 +44,47 @@ def test_kmer_hash_rejects_bad_k():
             kmer_hash(["ACGT"], k)
 
 
+_MINIMISER_CASES = [
+    # (shape, window_size, seed)
+    (0b1111, 4, 0),                          # window of one k-mer: every canonical value
+    ((1 << 19) - 1, 23, O.MINIMISER_SEED),   # ungapped k = 19, 5 k-mers per window
+    ((1 << 32) - 1, 40, O.MINIMISER_SEED),   # k = 32 fills 64 bits
+    ((1 << 12) - 1, 70, 0),                  # 59 k-mers per window
+    (0b1101, 8, 0),                          # gapped
+    (0b11011001110111, 30, O.MINIMISER_SEED),  # gapped, size 14, count 10
+    ((1 << 16) - 1, 16 + 4095, 12345),       # the largest window: 4,096 k-mers
+]
+
+
+@pytest.mark.parametrize("shape,window,seed", _MINIMISER_CASES)
+def test_minimiser_hash_matches_oracle(shape, window, seed):
+    from hibf_amd.device import minimiser_hash, to_numpy
+    _say(f"minimiser hash shape={shape:#x} window={window} seed={seed:#x}")
+    rng = random.Random(shape ^ window)
+    size = shape.bit_length()
+    w = window - size + 1
+    reads = _reads(rng, 200, 0, 300, "ACGTACGTACGTacgtNU") + ["", "A" * (size - 1), "C" * size, "A" * 500]
+    # Low-complexity reads (long runs of equal values: ties everywhere) and
+    # reads past the kernel's 512-position segments, at and around the seams.
+    reads += ["ACG" * 300, "AAAAAAAAAAAAAAAAAAAACAAAAAAAAAAAAAAAAAAAA" * 30]
+    for nk in (512, 513, 511 + w, 512 + w, 1024 + w - 1, 3000):
+        reads += _reads(rng, 1, nk + size - 1, nk + size - 1)
+    reads += _reads(rng, 1, 9000, 9000)
+    hashes, offsets = minimiser_hash(reads, shape, window, seed)
+    h, o = to_numpy(hashes), to_numpy(offsets)
+    assert o[0] == 0 and o.size == len(reads) + 1
+    for i, r in enumerate(reads):
+        assert h[int(o[i]):int(o[i + 1])].tolist() == O.minimiser_hashes(r, shape, window, seed).tolist(), i
+
+
+def test_minimiser_hash_rejects_bad_arguments():
+    from hibf_amd.device import minimiser_hash
+    from hibf_amd.native import HibfGpuError
+    for shape, window in ((0, 8), (0b1111, 3), ((1 << 33) - 1, 40), (0b1111, 4 + 4096)):
+        with pytest.raises(HibfGpuError):
+            minimiser_hash(["ACGTACGTAC"], shape, window)
+
+
 def test_search_reads_like_the_tutorial():
     """User bins are k-mer sets of random genomes; reads are sampled from them
     (some with errors, some random); ids equal the oracle's membership_for on

@@ -521,6 +521,142 @@ def test_kmer_hashes_brute_force(k):
         assert O.kmer_hashes(seq, k).tolist() == _kmer_brute(seq, k)
 
 
+# ---------------------------------------------- minimisers (reads in) --
+# seqan3::views::minimiser_hash (seqan3 3.4.2, the reference tutorial's
+# dependency, absent here) restated in orc_minimiser_hash; pinned below by a
+# second, structurally different statement of the rule, by closed forms and
+# by invariants.  Parity with seqan3's own output stays unpinned.
+
+_COMP = {"A": "T", "C": "G", "G": "C", "T": "A"}
+
+
+def _dna4(seq: str) -> str:
+    """dna4 reading of a string: U as T, every other non-ACGT letter as A."""
+    return "".join({"A": "A", "C": "C", "G": "G", "T": "T", "U": "T"}.get(ch.upper(), "A") for ch in seq)
+
+
+def _canonical_values(seq: str, shape: int, seed: int) -> list[int]:
+    """min(forward ^ seed, reverse ^ seed) per window, from string operations:
+    the used letters of each window as a k-mer (forward), and the same shape
+    laid over the reverse complement of the window (reverse)."""
+    d = _dna4(seq)
+    size = shape.bit_length()
+    used = [t for t in range(size) if (shape >> (size - 1 - t)) & 1]
+    out = []
+    for i in range(len(d) - size + 1):
+        win = d[i:i + size]
+        rc = "".join(_COMP[ch] for ch in reversed(win))
+        f = int(O.kmer_hashes("".join(win[t] for t in used), len(used))[0])
+        r = int(O.kmer_hashes("".join(rc[t] for t in used), len(used))[0])
+        out.append(min(f ^ seed, r ^ seed))
+    return out
+
+
+def _minimiser_chain(values: list[int], window: int) -> list[tuple[int, int]]:
+    """The emitted (window index, position) pairs as a chain over positions:
+    from the rightmost minimum of the first window, the next position is the
+    first strictly smaller value within `window` positions (taken when it
+    enters), else the rightmost minimum of the `window` values after the
+    current one (taken when the current one leaves)."""
+    n = len(values)
+    if n == 0:
+        return []
+    w = min(window, n)
+
+    def rightmost_min(lo, hi):
+        best = lo
+        for t in range(lo, hi):
+            if values[t] <= values[best]:
+                best = t
+        return best
+
+    p = rightmost_min(0, w)
+    chain = [(0, p)]
+    while True:
+        smaller = next((t for t in range(p + 1, min(p + w, n - 1) + 1) if values[t] < values[p]), None)
+        if smaller is not None:
+            p = smaller
+            chain.append((p - w + 1, p))
+        elif p + w <= n - 1:
+            left = p + 1  # the window in which the current position has just left
+            p = rightmost_min(p + 1, p + w + 1)
+            chain.append((left, p))
+        else:
+            return chain
+
+
+def test_minimiser_hashes_equal_the_chain_form():
+    rng = random.Random(11)
+    for _ in range(1500):
+        size = rng.randint(1, 9)
+        shape = rng.randint(1 << (size - 1), (1 << size) - 1)
+        window = rng.randint(size, size + 14)
+        seed = rng.choice([0, O.MINIMISER_SEED, rng.getrandbits(64)])
+        seq = "".join(rng.choice("ACGTACGTAAACCnU") for _ in range(rng.randint(0, 90)))
+        values = _canonical_values(seq, shape, seed)
+        expect = [values[p] for _, p in _minimiser_chain(values, window - size + 1)]
+        assert O.minimiser_hashes(seq, shape, window, seed).tolist() == expect, (seq, shape, window, seed)
+
+
+def test_minimiser_hashes_window_of_one_value_emits_every_value():
+    rng = random.Random(12)
+    for k in (1, 4, 19, 31, 32):
+        for seed in (0, O.MINIMISER_SEED):
+            seq = "".join(rng.choice("ACGT") for _ in range(200))
+            fwd = O.kmer_hashes(seq, k).astype(object)
+            rev = O.kmer_hashes("".join(_COMP[ch] for ch in reversed(seq)), k)[::-1].astype(object)
+            expect = [min(int(f) ^ seed, int(r) ^ seed) for f, r in zip(fwd, rev)]
+            assert O.minimiser_hashes(seq, (1 << k) - 1, k, seed).tolist() == expect
+
+
+def test_minimiser_hashes_hand_examples():
+    full = (1 << 64) - 1
+    # one window of AC with k = 2, seed 0: forward AC = 1, reverse complement GT = 11
+    assert O.minimiser_hashes("AC", 0b11, 2, 0).tolist() == [1]
+    # palindromic k-mer ACGT: forward = reverse = 0b00011011
+    assert O.minimiser_hashes("ACGT", 0b1111, 4, 0).tolist() == [27]
+    # TTTT: forward 255, reverse AAAA = 0
+    assert O.minimiser_hashes("TTTT", 0b1111, 4, 0).tolist() == [0]
+    # seed all ones: min(~f, ~r) = ~max(f, r)
+    assert O.minimiser_hashes("TTTT", 0b1111, 4, full).tolist() == [full ^ 255]
+    # gapped 101 over ACG: forward A_G = 0b0010 = 2, reverse complement CGT -> C_T = 0b0111 = 7
+    assert O.minimiser_hashes("ACG", 0b101, 3, 0).tolist() == [2]
+    # shorter than the shape: nothing; shorter than the window: one minimum
+    assert O.minimiser_hashes("ACG", 0b1111, 8, 0).tolist() == []
+    values = _canonical_values("GATTACA", 0b111, 0)
+    assert O.minimiser_hashes("GATTACA", 0b111, 40, 0).tolist() == [min(values)]
+    with pytest.raises(ValueError):
+        O.minimiser_hashes("ACGT", 0b1111, 3, 0)
+
+
+def test_minimiser_hashes_invariants():
+    """Every emitted value is the minimum of the window it is emitted at, every
+    window's minimum is the last value emitted so far, and the values of a read
+    and of its reverse complement share their window minima."""
+    rng = random.Random(13)
+    for _ in range(300):
+        k = rng.randint(1, 12)
+        window = rng.randint(k, k + 20)
+        seed = rng.choice([0, O.MINIMISER_SEED])
+        seq = "".join(rng.choice("ACGT") for _ in range(rng.randint(k, 150)))
+        values = _canonical_values(seq, (1 << k) - 1, seed)
+        w = min(window - k + 1, len(values))
+        chain = _minimiser_chain(values, window - k + 1)
+        got = O.minimiser_hashes(seq, (1 << k) - 1, window, seed).tolist()
+        assert got == [values[p] for _, p in chain]
+        minima = [min(values[j:j + w]) for j in range(len(values) - w + 1)]
+        last, c = None, 0
+        for j, m in enumerate(minima):
+            while c < len(chain) and chain[c][0] <= j:
+                last = values[chain[c][1]]
+                c += 1
+            assert last == m, (seq, k, window, j)
+        rc = "".join(_COMP[ch] for ch in reversed(seq))
+        rc_values = _canonical_values(rc, (1 << k) - 1, seed)
+        assert rc_values == values[::-1]
+        assert set(O.minimiser_hashes(rc, (1 << k) - 1, window, seed).tolist()) == set(minima)
+
+
 # ------------------------------------------------ HyperLogLog (build side) --
 # Known answers from the reference's test/unit/hibf/sketch/hyperloglog_test.cpp.
 
