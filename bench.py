@@ -730,7 +730,9 @@ def run_hibf(args):
     # (e) reads in: k-mer hashes (k = 19) of random 250-base reads, as the tutorial search feeds membership_for.
     kmer_leg = None
     if args.reads > 0:
+        import oracle as O
         from hibf_amd import native as N
+        from hibf_amd.device import to_numpy
         rng_r = np.random.default_rng(args.seed + 21)
         nr, rl, k = args.reads, 250, 19
         seq = torch.from_numpy(np.frombuffer(b"ACGT", dtype=np.uint8)[rng_r.integers(0, 4, size=nr * rl)].copy()).to(dev)
@@ -753,6 +755,44 @@ def run_hibf(args):
                     "hashes_per_s": total.value / r_s, "GBs": (nr * rl + total.value * 8) / r_s / 1e9}
         log(f"(e) k-mer hashes of {nr} x {rl} bases (k = {k}): {r_s * 1e3:.2f} ms per call incl. offsets and sync, "
             f"{kmer_leg['hashes_per_s'] / 1e9:.2f} G hashes/s")
+
+        # The same reads as minimisers: ungapped{19}, window_size{23}, seqan3's default seed.
+        shape, window, mseed = (1 << k) - 1, 23, 0x8F3F73B5CF1C9ADE
+
+        def minimise_reads():
+            N.check(N.lib().hibf_gpu_minimiser_hash(ctypes.c_void_p(seq.data_ptr()), ctypes.c_void_p(soff.data_ptr()),
+                                                    nr, ctypes.c_uint64(shape), window, ctypes.c_uint64(mseed),
+                                                    ctypes.c_void_p(hoff.data_ptr()),
+                                                    ctypes.c_void_p(hashes_r.data_ptr()), hashes_r.numel(),
+                                                    ctypes.byref(total), ctypes.c_void_p(stream.cuda_stream)))
+        minimise_reads()
+        t1 = time.perf_counter()
+        for _ in range(3):
+            minimise_reads()
+        m_s = (time.perf_counter() - t1) / 3
+        mini = {"shape": shape, "window_size": window, "seed": mseed, "minimisers": int(total.value),
+                "seconds": m_s, "kmers_per_s": nr * (rl - k + 1) / m_s, "reads_per_s": nr / m_s}
+        nchk = min(nr, 256)
+        h_m, o_m = to_numpy(hashes_r), to_numpy(hoff)
+        bases = seq[: nchk * rl].cpu().numpy().tobytes()
+        mini["parity_sample_ok"] = all(
+            h_m[int(o_m[i]):int(o_m[i + 1])].tolist() == O.minimiser_hashes(bases[i * rl:(i + 1) * rl], shape, window,
+                                                                             mseed).tolist()
+            for i in range(nchk))
+        if not args.skip_cpu:
+            # Oracle restatement on one core (a per-k-mer hash loop, as the definition reads), ~1 s of reads.
+            ncpu = max(1, min(nr, int(2000 * max(args.cpu_seconds, 0.1) / 10)))
+            cb = seq[: ncpu * rl].cpu().numpy().tobytes()
+            t1 = time.perf_counter()
+            for i in range(ncpu):
+                O.minimiser_hashes(cb[i * rl:(i + 1) * rl], shape, window, mseed)
+            c_s = time.perf_counter() - t1
+            mini["cpu_baseline"] = {"value": ncpu * (rl - k + 1) / c_s, "unit": "k-mers/s", "cores": 1,
+                                    "kind": "port", "sample": f"oracle minimiser_hash of the first {ncpu} reads"}
+        kmer_leg["minimisers"] = mini
+        log(f"(e) minimisers of the same reads (k = {k}, window {window}): {m_s * 1e3:.2f} ms per call incl. offsets "
+            f"and sync, {mini['kmers_per_s'] / 1e9:.2f} G k-mers/s, {total.value / nr:.1f} minimisers per read, "
+            f"parity on {nchk} reads {mini['parity_sample_ok']}")
         del seq, soff, hoff, hashes_r
 
     # (c) plain form: one launch per query of (b), back to back.  The launches call the C-ABI directly, as a

@@ -140,23 +140,21 @@ __global__ void __launch_bounds__(256) kmer_threshold_kernel(uint64_t const * __
 // The same rule as a chain over positions: from the first window's rightmost
 // minimum p, the next emitted position is the first t in (p, p + W] with a
 // strictly smaller value (it enters before p leaves), else - when p + W is a
-// position - the rightmost minimum of (p, p + W] (taken as p leaves).  Every
-// lane computes the next position of its own positions from LDS; lane 0 walks
-// the chain.  One wave per read, the read in segments of kMiniSeg chain
-// positions whose values are held with a look-ahead of W.
+// position - the rightmost minimum of (p, p + W] (taken as p leaves).  One
+// wave per read, the read in segments of kMiniSeg chain positions whose values
+// are held with a look-ahead of W.
 
-constexpr uint32_t kMiniThreads = 64;
+constexpr uint32_t kMiniThreads = 64; // one wave per read
 constexpr uint32_t kMiniSeg = 512;
-constexpr uint32_t kMiniEnd = 0xFFFFFFFFu;
 
 struct MiniParams
 {
     uint64_t shape; // seqan3's binary literal: most significant set bit = first position
     uint64_t seed;
     uint32_t size;  // bit length of shape, <= 32
-    uint32_t window; // W, values per window, <= kMaxMinimiserWindow
+    uint32_t window; // W, values per window, <= HIBF_GPU_MAX_MINIMISER_WINDOW
     uint32_t gapped;
-    uint32_t words_cap; // packed 64-bit words of bases per segment (+ one zero word)
+    uint32_t words_cap; // packed 64-bit words of bases per segment, whole 256-base chunks (+ one zero word)
 };
 
 // Reverse complement of a window of `size` bases packed 2 bits each (first base
@@ -180,13 +178,64 @@ __device__ __forceinline__ uint64_t shape_gather(uint64_t x, uint64_t shape, uin
     return h;
 }
 
+// LDS of one block: the segment's packed bases (words_cap words), its values
+// (kMiniSeg + W), next positions (kMiniSeg, 16-bit, from a 16-byte boundary)
+// and chain flags (kMiniSeg bytes).
+__host__ __device__ inline size_t minimiser_next_offset(MiniParams const & mp) // in 64-bit words
+{
+    return (static_cast<size_t>(mp.words_cap) + kMiniSeg + mp.window + 1u) & ~static_cast<size_t>(1);
+}
+
 size_t minimiser_lds_bytes(MiniParams const & mp)
 {
-    return (static_cast<size_t>(mp.words_cap) + kMiniSeg + mp.window) * 8u + kMiniSeg * 4u;
+    return minimiser_next_offset(mp) * 8u + kMiniSeg * 3u;
+}
+
+constexpr uint32_t kMiniSink = 0xFFFFFFFFu; // no next position: the chain ends
+constexpr uint32_t kMiniOwn = kMiniSeg / kMiniThreads; // consecutive positions per lane (8)
+
+// Next chain position after segment position q (absolute pa = b0 + q): the
+// first of the following values (up to we of them, within the read) that is
+// strictly smaller, else - when the window after pa is whole - the rightmost
+// minimum of those values; kMiniSink when neither exists.
+__device__ __forceinline__ uint32_t minimiser_next(uint64_t const * vals, uint32_t q, uint64_t pa, uint64_t we,
+                                                   uint64_t nk)
+{
+    uint64_t const v = vals[q];
+    uint32_t const cnt = static_cast<uint32_t>(min<uint64_t>(pa + we, nk - 1u) - pa); // values after pa to scan
+    uint32_t nx = kMiniSink;
+    uint32_t best_at = kMiniSink;
+    uint64_t best = 0;
+    for (uint32_t t = 1; t <= cnt; ++t)
+    {
+        uint64_t const x = vals[q + t];
+        if (x < v)
+        {
+            nx = q + t;
+            break;
+        }
+        if (best_at == kMiniSink || x <= best)
+        {
+            best = x;
+            best_at = q + t;
+        }
+    }
+    if (nx == kMiniSink && pa + we <= nk - 1u)
+        nx = best_at;
+    return nx;
 }
 
 // COUNT: counts[i] = number of minimisers of read i.  WRITE: read i's
 // minimisers to hashes[hash_offsets[i] ...].
+//
+// Per segment: lanes pack the bases (4 per lane, 8 lanes joined by shuffles
+// into a 64-bit word), hash every position into LDS and take every position's
+// next chain position (neighbouring lanes on neighbouring positions).  Each
+// lane then takes 8 consecutive positions and works out, for every entry into
+// them, where the chain leaves them and which it visits (a bit mask); the
+// wave follows the chain from lane to lane through those exits with readlane
+// (scalar), each lane flags the positions visited from its entry, and the
+// flagged values are written in position order (ballot prefix sums).
 template <bool WRITE>
 __global__ void __launch_bounds__(kMiniThreads) minimiser_kernel(uint8_t const * __restrict__ seq,
                                                                  uint64_t const * __restrict__ seq_offsets,
@@ -197,11 +246,13 @@ __global__ void __launch_bounds__(kMiniThreads) minimiser_kernel(uint8_t const *
                                                                  uint64_t * __restrict__ hashes)
 {
     extern __shared__ uint64_t mini_lds[];
-    uint64_t * const words = mini_lds;                         // words_cap
-    uint64_t * const vals = mini_lds + mp.words_cap;           // kMiniSeg + W
-    uint32_t * const nxt = reinterpret_cast<uint32_t *>(vals + kMiniSeg + mp.window); // kMiniSeg
+    uint64_t * const words = mini_lds;               // words_cap
+    uint64_t * const vals = mini_lds + mp.words_cap; // kMiniSeg + W
+    uint16_t * const next16 = reinterpret_cast<uint16_t *>(mini_lds + minimiser_next_offset(mp)); // kMiniSeg
+    uint8_t * const flags = reinterpret_cast<uint8_t *>(next16 + kMiniSeg);               // kMiniSeg
     uint32_t const lane = threadIdx.x;
     uint32_t const size = mp.size;
+    uint32_t const own = lane * kMiniOwn; // this lane's first position in a segment
     for (uint64_t i = blockIdx.x; i < n; i += gridDim.x)
     {
         uint8_t const * const s = seq + seq_offsets[i];
@@ -210,24 +261,29 @@ __global__ void __launch_bounds__(kMiniThreads) minimiser_kernel(uint8_t const *
         uint64_t emitted = 0;
         uint64_t const out = WRITE ? hash_offsets[i] : 0u;
         uint64_t const we = min<uint64_t>(mp.window, nk);
-        uint64_t p = 0;
+        uint64_t p = 0; // the chain's current position (absolute), emitted in the segment that holds it
         bool live = nk != 0;
         for (uint64_t b0 = 0; live && b0 < nk; b0 += kMiniSeg)
         {
             uint64_t const e = min<uint64_t>(nk, b0 + kMiniSeg + we); // values held: [b0, e)
             uint32_t const m = static_cast<uint32_t>(e - b0);
-            uint32_t const nw = (m + size - 1u + 31u) / 32u;           // words of bases [b0, e + size - 1)
-            for (uint32_t x = lane; x <= nw; x += kMiniThreads)
+            uint32_t const chunks = (m + size - 1u + 255u) / 256u;      // 256 bases [b0 + 256 c, ...) each
+            for (uint32_t c = 0; c < chunks; ++c)
             {
-                uint64_t w = 0;
-                if (x < nw)
-                {
-                    uint64_t const first = b0 + 32u * x;
-                    for (uint32_t j = 0; j < 32u; ++j)
-                        w = (w << 2) | (first + j < len ? dna4_rank(s[first + j]) : 0u);
-                }
-                words[x] = w;
+                uint64_t const at = b0 + 256u * c + 4u * lane;
+                uint32_t v = 0;
+#pragma unroll
+                for (uint32_t j = 0; j < 4u; ++j)
+                    v = (v << 2) | (at + j < len ? dna4_rank(s[at + j]) : 0u);
+                unsigned long long x = v;
+                x = (x << 8) | __shfl_down(x, 1);
+                x = (x << 16) | __shfl_down(x, 2);
+                x = (x << 32) | __shfl_down(x, 4);
+                if ((lane & 7u) == 0u)
+                    words[8u * c + (lane >> 3)] = x;
             }
+            if (lane == 0)
+                words[8u * chunks] = 0;
             __syncthreads();
             for (uint32_t q = lane; q < m; q += kMiniThreads)
             {
@@ -241,33 +297,6 @@ __global__ void __launch_bounds__(kMiniThreads) minimiser_kernel(uint8_t const *
                 vals[q] = min(fwd ^ mp.seed, rev ^ mp.seed);
             }
             __syncthreads();
-            uint32_t const pe = static_cast<uint32_t>(min<uint64_t>(nk, b0 + kMiniSeg) - b0);
-            for (uint32_t q = lane; q < pe; q += kMiniThreads)
-            {
-                uint64_t const v = vals[q];
-                uint64_t const pa = b0 + q;
-                uint64_t const last = min<uint64_t>(pa + we, nk - 1u);
-                uint32_t nx = kMiniEnd;
-                uint64_t best = 0;
-                uint32_t best_at = kMiniEnd;
-                for (uint64_t t = pa + 1u; t <= last; ++t)
-                {
-                    uint64_t const x = vals[t - b0];
-                    if (x < v)
-                    {
-                        nx = static_cast<uint32_t>(t - b0);
-                        break;
-                    }
-                    if (best_at == kMiniEnd || x <= best)
-                    {
-                        best = x;
-                        best_at = static_cast<uint32_t>(t - b0);
-                    }
-                }
-                if (nx == kMiniEnd && pa + we <= nk - 1u)
-                    nx = best_at;
-                nxt[q] = nx;
-            }
             if (b0 == 0)
             {
                 // The first window's rightmost minimum: per lane, then across
@@ -291,32 +320,100 @@ __global__ void __launch_bounds__(kMiniThreads) minimiser_kernel(uint8_t const *
                     }
                 }
                 p = best_at;
-                if (lane == 0)
-                {
-                    if (WRITE)
-                        hashes[out] = best;
-                    emitted = 1;
-                }
+            }
+            uint32_t const pe = static_cast<uint32_t>(min<uint64_t>(nk, b0 + kMiniSeg) - b0);
+            if (p >= b0 + pe)
+            {
+                __syncthreads();
+                continue; // the chain jumps past this segment's positions (large windows)
+            }
+            // Next positions (16-bit; beyond the segment's values: the sink),
+            // then per lane 8 consecutive ones and per entry offset r the
+            // positions visited (mask) and the exit (the first position after
+            // them, in a later lane or segment, or the sink).
+            for (uint32_t q = lane; q < pe; q += kMiniThreads)
+            {
+                uint32_t const nx = minimiser_next(vals, q, b0 + q, we, nk);
+                next16[q] = static_cast<uint16_t>(nx == kMiniSink ? 0xFFFFu : nx);
             }
             __syncthreads();
-            if (lane == 0)
+            uint32_t nxt[kMiniOwn];
             {
-                while (p < b0 + pe)
+                uint4 const packed = reinterpret_cast<uint4 const *>(next16)[lane]; // positions own .. own + 7
+                uint32_t const w4[4] = {packed.x, packed.y, packed.z, packed.w};
+#pragma unroll
+                for (uint32_t r = 0; r < kMiniOwn; ++r)
                 {
-                    uint32_t const nx = nxt[p - b0];
-                    if (nx == kMiniEnd)
-                    {
-                        live = false;
-                        break;
-                    }
-                    if (WRITE)
-                        hashes[out + emitted] = vals[nx];
-                    ++emitted;
-                    p = b0 + nx;
+                    uint32_t const h = (w4[r / 2] >> (16u * (r % 2))) & 0xFFFFu;
+                    nxt[r] = h == 0xFFFFu ? kMiniSink : h;
                 }
             }
-            p = hibf_amd::uniform64(p);
-            live = hibf_amd::uniform32(live ? 1u : 0u) != 0u;
+            uint32_t const stay = min(own + kMiniOwn, pe); // positions below this stay in the lane
+            uint32_t mask[kMiniOwn];
+            uint32_t exit[kMiniOwn];
+#pragma unroll
+            for (int r = static_cast<int>(kMiniOwn) - 1; r >= 0; --r)
+            {
+                uint32_t const nx = nxt[r];
+                mask[r] = own + r < pe ? 1u << r : 0u;
+                exit[r] = own + r < pe ? nx : kMiniSink;
+                if (own + r < pe && nx != kMiniSink && nx < stay)
+#pragma unroll
+                    for (uint32_t j = r + 1; j < kMiniOwn; ++j)
+                        if (nx == own + j)
+                        {
+                            mask[r] |= mask[j];
+                            exit[r] = exit[j];
+                        }
+            }
+            // Follow the chain across lanes (uniform).
+            uint32_t entry = 0xFFu;
+            uint32_t at = static_cast<uint32_t>(p - b0);
+            uint32_t x = kMiniSink;
+            for (;;)
+            {
+                uint32_t const l = at / kMiniOwn;
+                uint32_t const o = at % kMiniOwn;
+                entry = lane == l ? o : entry;
+                x = kMiniSink;
+#pragma unroll
+                for (uint32_t r = 0; r < kMiniOwn; ++r)
+                    if (o == r)
+                        x = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(exit[r]), static_cast<int>(l)));
+                if (x == kMiniSink || x >= pe)
+                    break;
+                at = x;
+            }
+            uint32_t mine = 0;
+#pragma unroll
+            for (uint32_t r = 0; r < kMiniOwn; ++r)
+                if (entry == r)
+                    mine = mask[r];
+            if (WRITE)
+            {
+                uint64_t bytes = 0;
+#pragma unroll
+                for (uint32_t r = 0; r < kMiniOwn; ++r)
+                    bytes |= static_cast<uint64_t>((mine >> r) & 1u) << (8u * r);
+                reinterpret_cast<uint64_t *>(flags)[lane] = bytes;
+                __syncthreads();
+                for (uint32_t c0 = 0; c0 < pe; c0 += kMiniThreads)
+                {
+                    uint32_t const q = c0 + lane;
+                    bool const f = q < pe && flags[q];
+                    uint64_t const ballot = __ballot(f);
+                    if (f)
+                        hashes[out + emitted + __builtin_popcountll(ballot & ((1ull << lane) - 1ull))] = vals[q];
+                    emitted += static_cast<uint64_t>(__builtin_popcountll(ballot));
+                }
+            }
+            else
+            {
+                uint32_t const incl = hibf_amd::wave_inclusive_add_u32(static_cast<uint32_t>(__builtin_popcount(mine)));
+                emitted += static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(incl), 63));
+            }
+            live = x != kMiniSink;
+            p = b0 + (live ? x : 0u);
             __syncthreads(); // the next segment overwrites the LDS arrays
         }
         if (!WRITE && lane == 0)
@@ -448,7 +545,7 @@ int hibf_gpu_minimiser_hash(uint8_t const * d_seq,
     mp.size = size;
     mp.window = static_cast<uint32_t>(w);
     mp.gapped = shape != ((size == 64u ? 0ull : (1ull << size)) - 1ull) ? 1u : 0u;
-    mp.words_cap = (kMiniSeg + mp.window + size - 1u + 31u) / 32u + 1u;
+    mp.words_cap = (kMiniSeg + mp.window + size - 1u + 255u) / 256u * 8u + 1u;
     size_t const lds = minimiser_lds_bytes(mp);
     hipStream_t const s = static_cast<hipStream_t>(stream);
     *h_total = 0;
