@@ -206,19 +206,25 @@ __device__ __forceinline__ uint32_t minimiser_next(uint64_t const * vals, uint32
     uint32_t nx = kMiniSink;
     uint32_t best_at = kMiniSink;
     uint64_t best = 0;
-    for (uint32_t t = 1; t <= cnt; ++t)
+    // Four values per round trip: their loads are issued before any is compared.
+    for (uint32_t t0 = 1; t0 <= cnt && nx == kMiniSink; t0 += 4u)
     {
-        uint64_t const x = vals[q + t];
-        if (x < v)
-        {
-            nx = q + t;
-            break;
-        }
-        if (best_at == kMiniSink || x <= best)
-        {
-            best = x;
-            best_at = q + t;
-        }
+        uint64_t x[4];
+#pragma unroll
+        for (uint32_t u = 0; u < 4u; ++u)
+            x[u] = vals[q + (t0 + u <= cnt ? t0 + u : 0u)];
+#pragma unroll
+        for (uint32_t u = 0; u < 4u; ++u)
+            if (t0 + u <= cnt && nx == kMiniSink)
+            {
+                if (x[u] < v)
+                    nx = q + t0 + u;
+                else if (best_at == kMiniSink || x[u] <= best)
+                {
+                    best = x[u];
+                    best_at = q + t0 + u;
+                }
+            }
     }
     if (nx == kMiniSink && pa + we <= nk - 1u)
         nx = best_at;
@@ -366,20 +372,26 @@ __global__ void __launch_bounds__(kMiniThreads) minimiser_kernel(uint8_t const *
                             exit[r] = exit[j];
                         }
             }
-            // Follow the chain across lanes (uniform).
+            // Follow the chain across lanes (uniform): each lane's exits in
+            // LDS (over its own next positions, which it alone read), then one
+            // broadcast read per lane the chain passes through.
+            {
+                uint32_t h[kMiniOwn];
+#pragma unroll
+                for (uint32_t r = 0; r < kMiniOwn; ++r)
+                    h[r] = exit[r] == kMiniSink ? 0xFFFFu : exit[r];
+                reinterpret_cast<uint4 *>(next16)[lane] =
+                    make_uint4(h[0] | (h[1] << 16), h[2] | (h[3] << 16), h[4] | (h[5] << 16), h[6] | (h[7] << 16));
+            }
+            __syncthreads();
             uint32_t entry = 0xFFu;
             uint32_t at = static_cast<uint32_t>(p - b0);
             uint32_t x = kMiniSink;
             for (;;)
             {
-                uint32_t const l = at / kMiniOwn;
-                uint32_t const o = at % kMiniOwn;
-                entry = lane == l ? o : entry;
-                x = kMiniSink;
-#pragma unroll
-                for (uint32_t r = 0; r < kMiniOwn; ++r)
-                    if (o == r)
-                        x = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(exit[r]), static_cast<int>(l)));
+                entry = lane == at / kMiniOwn ? at % kMiniOwn : entry;
+                uint32_t const h = hibf_amd::uniform32(next16[at]);
+                x = h == 0xFFFFu ? kMiniSink : h;
                 if (x == kMiniSink || x >= pe)
                     break;
                 at = x;
