@@ -551,12 +551,16 @@ class DeviceHibf:
             self.h, agent.h, _ptr(v), _ptr(o), nq, _ptr(t), _ptr(ids), cap, _ptr(hoff), ctypes.byref(total),
             _stream()), nq, o.device)
 
-    def membership_for_reads(self, reads, k: int, fraction: float = 0.7, agent: Agent | None = None):
+    def membership_for_reads(self, reads, k: int, fraction: float = 0.7, agent: Agent | None = None,
+                             minimiser: tuple | None = None):
         """The tutorial search (test/tutorial/search.cpp:37-45) on the device:
         each read's k-mer hashes are queried with threshold
-        uint16_t(fraction * number of k-mers).  Returns one id list per read,
-        in the reference's depth-first order."""
-        hashes, offsets = kmer_hash(reads, k)
+        uint16_t(fraction * number of k-mers).  With minimiser = (shape,
+        window_size) or (shape, window_size, seed), the read's minimiser hashes
+        (minimiser_hash) are queried instead, with threshold
+        uint16_t(fraction * number of minimisers), and k is not used.  Returns
+        one id list per read, in the reference's depth-first order."""
+        hashes, offsets = kmer_hash(reads, k) if minimiser is None else minimiser_hash(reads, *minimiser)
         n = offsets.numel() - 1
         thr = torch.empty(max(n, 1), dtype=torch.uint16, device=offsets.device)
         check(lib().hibf_gpu_kmer_thresholds(_ptr(offsets), n, float(fraction), _ptr(thr), _stream()))

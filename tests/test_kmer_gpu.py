@@ -147,3 +147,31 @@ def test_serving_reads_equals_the_tutorial_search(pinned):
             want = ref.membership_for(hashes, O.kmer_threshold(hashes.size, 0.7))
             assert int(cnt[i]) == len(want), i
             assert ids[i, :min(len(want), 16)].tolist() == want[:16], i
+
+
+def test_search_reads_by_minimisers():
+    """User bins are the minimiser sets of random genomes; reads sampled from
+    them (some with errors) are queried by their minimisers with threshold
+    uint16_t(0.7 * #minimisers); ids equal the oracle's membership_for on the
+    oracle's minimisers."""
+    from hibf_amd.device import DeviceHibf
+    _say("search by minimisers")
+    shape, window = (1 << 15) - 1, 25
+    rng = random.Random(9)
+    genomes = ["".join(rng.choice("ACGT") for _ in range(rng.randint(2000, 6000))) for _ in range(80)]
+    ubs = [np.unique(O.minimiser_hashes(g, shape, window)) for g in genomes]
+    dev = DeviceHibf.build(ubs, tmax=64)
+    ref = oracle_of(dev)
+    reads = []
+    for _ in range(300):
+        g = rng.choice(genomes)
+        at = rng.randint(0, len(g) - 150)
+        r = list(g[at:at + 150])
+        for _ in range(rng.randint(0, 3)):
+            r[rng.randrange(150)] = rng.choice("ACGT")
+        reads.append("".join(r))
+    reads += ["".join(rng.choice("ACGT") for _ in range(150)) for _ in range(20)] + ["ACG", ""]
+    got = dev.membership_for_reads(reads, 0, 0.7, minimiser=(shape, window))
+    for i, r in enumerate(reads):
+        hashes = O.minimiser_hashes(r, shape, window)
+        assert got[i] == ref.membership_for(hashes, O.kmer_threshold(hashes.size, 0.7)), i
