@@ -66,6 +66,9 @@ def parse():
                     help="queries of (a) for the HIBF bulk_count leg (uint16 counts of every user bin, t = 0.7 L; "
                          "0 skips it)")
     ap.add_argument("--graph-batch", type=int, default=65536)
+    ap.add_argument("--streams", type=int, default=3,
+                    help="hibf: streams the graph batches of (b) alternate over (one agent each), so that a batch "
+                         "starts while the previous ones drain")
     ap.add_argument("--hibf-user-bins", type=int, default=32768)
     ap.add_argument("--recompute-layout", action="store_true",
                     help="hibf: compute the layout (union estimates + rearrangement on the device) instead of "
@@ -626,44 +629,65 @@ def run_hibf(args):
     ids_b = torch.empty(nbatches * cap_b, dtype=torch.uint64, device=dev)
     hs_b = torch.empty(nb, dtype=torch.uint64, device=dev)
     hc_b = torch.empty(nb, dtype=torch.uint64, device=dev)
-    hibf.membership_prepare(agent, G)
+    # Batch b's graph runs on stream b % NS with its own agent (workspace, counters): consecutive batches are
+    # independent, so a batch's blocks start on the CUs the previous batches' last queries leave free.
+    NS = max(1, args.streams)
+    agents_b = [agent] + [Agent() for _ in range(NS - 1)]
+    streams_b = [stream] + [torch.cuda.Stream() for _ in range(NS - 1)]
+    for ag in agents_b:
+        hibf.membership_prepare(ag, G)
     graphs = []
     for b in range(nbatches):
         q0, q1 = b * G, min(nb, (b + 1) * G)
-        args_b = (agent, vb, ob[q0:q1 + 1], tb[q0:q1], ids_b[b * cap_b:(b + 1) * cap_b], hs_b[q0:q1], hc_b[q0:q1])
+        args_b = (agents_b[b % NS], vb, ob[q0:q1 + 1], tb[q0:q1], ids_b[b * cap_b:(b + 1) * cap_b], hs_b[q0:q1],
+                  hc_b[q0:q1])
         hibf.membership_launch(*args_b)  # warm the capture inputs once outside the graph
         g = torch.cuda.CUDAGraph()
         with torch.cuda.graph(g):
             hibf.membership_launch(*args_b)
         graphs.append(g)
     torch.cuda.synchronize()
-    agent.stats(reset=True)
     for g in graphs:  # warm replay
         g.replay()
     torch.cuda.synchronize()
-    _, dropped = agent.status()
-    assert not dropped, "batch output capacity too small"
-    agent.stats(reset=True)
-    # Timed region: K steps of all batches; HIP events on the launch stream bracket every graph replay (a
-    # replay is the counter-reset kernel and the descent kernel).
+    for ag in agents_b:
+        _, dropped = ag.status()
+        assert not dropped, "batch output capacity too small"
+        ag.stats(reset=True)
+    # Timed region: K steps of all batches.  HIP events: one pair brackets every graph replay on its stream
+    # (a replay is the counter-reset kernel and the descent kernel; with several streams these spans overlap),
+    # and one pair on the launch stream brackets all K steps, the other streams waiting for the first event and
+    # the launch stream for their last work: the device time of the K steps.
     evs_b = [[(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in graphs]
              for _ in range(args.steps)]
+    span0, span1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
+    span0.record(stream)
+    for sx in streams_b[1:]:
+        sx.wait_event(span0)
     for step_evs in evs_b:
-        for g, (e0, e1) in zip(graphs, step_evs):
-            e0.record(stream)
-            g.replay()
-            e1.record(stream)
+        for bi, (g, (e0, e1)) in enumerate(zip(graphs, step_evs)):
+            sx = streams_b[bi % NS]
+            with torch.cuda.stream(sx):
+                e0.record(sx)
+                g.replay()
+                e1.record(sx)
+    for sx in streams_b[1:]:
+        stream.wait_stream(sx)
+    span1.record(stream)
     torch.cuda.synchronize()
     wall_b = (time.perf_counter() - t0) / args.steps
+    device_b = span0.elapsed_time(span1) / 1e3 / args.steps
     kern_b = sum(e0.elapsed_time(e1) for step_evs in evs_b for (e0, e1) in step_evs) / 1e3 / args.steps
-    st = agent.stats(reset=True)
-    lookups_b = st["lookups"] / args.steps
-    bytes_b = st["row_bytes"] / args.steps
-    log(f"(b) {nb} log-normal queries ({int(ob_h[-1]) / nb:.0f} values/query), {nbatches} graphs: "
-        f"{wall_b * 1e3:.1f} ms/step (kernels {kern_b * 1e3:.1f} ms), {lookups_b / wall_b / 1e9:.3f} G lookups/s, "
-        f"{bytes_b / kern_b / 1e9:.0f} GB/s rows")
+    lookups_b = bytes_b = 0
+    for ag in agents_b:
+        st = ag.stats(reset=True)
+        lookups_b += st["lookups"] / args.steps
+        bytes_b += st["row_bytes"] / args.steps
+    log(f"(b) {nb} log-normal queries ({int(ob_h[-1]) / nb:.0f} values/query), {nbatches} graphs on {NS} streams: "
+        f"{wall_b * 1e3:.1f} ms/step (device {device_b * 1e3:.1f} ms; replays {kern_b * 1e3:.1f} ms summed over "
+        f"overlapping spans), {lookups_b / wall_b / 1e9:.3f} G lookups/s, {bytes_b / device_b / 1e9:.0f} GB/s rows")
 
     # Traffic sweep (for counter runs): (a)'s launch, then one direct launch of every batch of (b), under each
     # launch shape of the tuner's space in turn; a one-element fill kernel marks the start of each shape's (a)
@@ -902,7 +926,9 @@ def run_hibf(args):
 
     kernel_name = descent_kernel_name(native)
     per_launch_bytes = bytes_b / nbatches
-    launch_s = kern_b / nbatches
+    # Launches overlap on NS streams: a launch's share of the device time is the device time of a step over its
+    # launches (each replay's own span, start to end, is reported beside it).
+    launch_s = device_b / nbatches
     achieved = per_launch_bytes / launch_s / 1e9
     traffic = recorded_traffic("hibf_batched", args, layout)
     out = {
@@ -929,6 +955,7 @@ def run_hibf(args):
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": kernel_name,
                      "kernel_ms_per_launch": launch_s * 1e3, "launches_per_step": nbatches,
+                     "streams": NS, "replay_span_ms_mean": kern_b / nbatches * 1e3,
                      "algorithmic_bytes_per_launch": per_launch_bytes,
                      "bytes_per_lookup": bytes_b / max(lookups_b, 1),
                      "random_line_ceiling": {
