@@ -29,9 +29,10 @@ import time
 import numpy as np
 
 HBM_PEAK_GBS = 8000.0
-# Random 32-128-byte row reads from a 16 GiB buffer: 4.86e10 rows/s, one 128-byte line each
-# (profiles/random_row_gather_ceiling.md).
-RANDOM_LINE_CEILING = 4.86e10
+# Random 32-byte row reads from a 16 GiB buffer, one 128-byte line each: the fastest of the load forms and
+# launch shapes measured, 5.36e10 rows/s with non-temporal loads, two lanes per row
+# (profiles/random_row_gather_ceiling.md; cached loads reach 4.89e10).
+RANDOM_LINE_CEILING = 5.36e10
 
 
 def parse():
@@ -98,7 +99,7 @@ def parse():
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the library's default launch shapes instead of autotuning them on a sub-batch")
     ap.add_argument("--hibf-tune-queries", type=int, default=0,
-                    help="hibf: queries of (b) the launch shapes are tuned on (default: one graph batch)")
+                    help="hibf: queries of (b) the launch shapes are tuned on (default: two graph batches per stream)")
     ap.add_argument("--autotune-queries", type=int, default=100_000,
                     help="flat: queries of the workload the autotuner times each candidate shape on")
     ap.add_argument("--autotune-save", default=None, metavar="PATH", help="write the autotuner record as JSON")
@@ -577,22 +578,32 @@ def run_hibf(args):
     tb_h = thresholds_of(ob_h)
     vb, ob, tb = as_device_u64(vb_h), as_device_u64(ob_h), upload_u16(tb_h)
 
-    # Launch shapes tuned on the first queries of (b), the workload of the reported value.
+    # (b)'s graph batches alternate over NS streams, batch b on stream b % NS with its own agent (workspace,
+    # counters): consecutive batches are independent, so a batch's blocks start on the CUs the previous
+    # batches' last queries leave free.
+    G = args.graph_batch
+    NS = max(1, args.streams)
+    agents_b = [agent] + [Agent() for _ in range(NS - 1)]
+    streams_b = [stream] + [torch.cuda.Stream() for _ in range(NS - 1)]
+
+    # Launch shapes tuned on the first queries of (b), the workload of the reported value, in the form the
+    # timed batches run in: batches of G launched over the NS streams, timed together.
     tuned = None
     if not args.no_autotune and not args.tune:
         from hibf_amd import autotune
-        # The launch form of the timed batches, on (b)'s first graph batch (or --hibf-tune-queries queries).
-        nt = min(args.hibf_tune_queries or args.graph_batch, nb)
+        nt = min(args.hibf_tune_queries or 2 * NS * G, nb)
         ids_t = torch.empty(8 * nt, dtype=torch.uint64, device=dev)
         hs_t = torch.empty(nt, dtype=torch.uint64, device=dev)
         hc_t = torch.empty(nt, dtype=torch.uint64, device=dev)
-        rec = autotune.tune_hibf_membership_launch(hibf, agent, vb, ob[: nt + 1], tb[:nt], ids_t, hs_t, hc_t,
-                                                   reps=5)
+        rec = autotune.tune_hibf_membership_batches(hibf, agents_b, streams_b, vb, ob[: nt + 1], tb[:nt], ids_t,
+                                                    hs_t, hc_t, G, reps=3)
         del ids_t, hs_t, hc_t
+        nbt = (nt + G - 1) // G
         tuned = {"best": rec["best"], "best_ms": rec["best_ms"], "default_ms": rec["default_ms"], "queries": nt,
-                 "workload": "the first graph batch of (b), launch form", "candidates": len(rec["candidates"])}
-        log(f"autotune ({len(rec['candidates'])} shapes on {nt} queries of (b), launch form): {rec['best']} "
-            f"{rec['best_ms']:.3f} ms (default {rec['default_ms']:.3f} ms)")
+                 "workload": f"the first {nbt} graph batches of (b) over {NS} streams, launch form",
+                 "candidates": len(rec["candidates"])}
+        log(f"autotune ({len(rec['candidates'])} shapes on the first {nbt} batches of (b) over {NS} streams): "
+            f"{rec['best']} {rec['best_ms']:.3f} ms (default {rec['default_ms']:.3f} ms)")
         if args.autotune_save:
             rec["command"] = "python bench.py " + " ".join(sys.argv[1:])
             autotune.save_record(rec, args.autotune_save)
@@ -623,17 +634,11 @@ def run_hibf(args):
         f"{bytes_a / kern_a / 1e9:.0f} GB/s rows, {st['visits'] / args.steps / nq:.2f} IBFs/query, "
         f"{total / nq:.3f} hits/query")
 
-    G = args.graph_batch
     nbatches = (nb + G - 1) // G
     cap_b = 8 * G
     ids_b = torch.empty(nbatches * cap_b, dtype=torch.uint64, device=dev)
     hs_b = torch.empty(nb, dtype=torch.uint64, device=dev)
     hc_b = torch.empty(nb, dtype=torch.uint64, device=dev)
-    # Batch b's graph runs on stream b % NS with its own agent (workspace, counters): consecutive batches are
-    # independent, so a batch's blocks start on the CUs the previous batches' last queries leave free.
-    NS = max(1, args.streams)
-    agents_b = [agent] + [Agent() for _ in range(NS - 1)]
-    streams_b = [stream] + [torch.cuda.Stream() for _ in range(NS - 1)]
     for ag in agents_b:
         hibf.membership_prepare(ag, G)
     graphs = []

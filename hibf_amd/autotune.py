@@ -16,7 +16,8 @@ Search spaces:
     count.counter_bits  LDS counter width, 16-bit pairs or 8-bit quads
     count.wide_unroll   values in flight per wave (1, 2, 4, 8)
     count.wide_waves    waves per query block (1, 2, 4)
-  HIBF membership descent:
+  HIBF membership descent (one launch at a time, tune_hibf_membership_launch, or
+  batches overlapped over several streams, tune_hibf_membership_batches):
     hibf.wave_kernel    one query per wave (1) or per block (0)
     hibf.row_per_lane   rows of <= 4 words: one value and its whole row per
                         lane (1), or lanes sharing a value's row (0)
@@ -210,6 +211,63 @@ def tune_hibf_membership_launch(hibf, agent, values: torch.Tensor, offsets: torc
     tab = hibf.tables()
     workload = {"kind": "hibf membership (launch form)", "ibfs": len(tab["geoms"]),
                 "user_bins": int(tab["n_user_bins"]), "queries": nq, "values": int(values.numel())}
+    return _tune(space or HIBF_SPACE, run, output, same, reps, workload)
+
+
+def tune_hibf_membership_batches(hibf, agents, streams, values: torch.Tensor, offsets: torch.Tensor,
+                                 thresholds: torch.Tensor, ids: torch.Tensor, hit_start: torch.Tensor,
+                                 hit_count: torch.Tensor, batch: int, reps: int = 3, space: dict | None = None) -> dict:
+    """Tunes the HIBF membership descent in the form a caller overlaps its
+    batches in: the queries of `offsets` cut into batches of `batch`, batch b
+    launched (hibf_gpu_hibf_membership_launch) on streams[b % len(streams)]
+    with agents[b % len(agents)], all batches timed as one span (the other
+    streams wait for its start and the current stream for their ends).
+    streams[0] must be the current stream.  `ids` is split evenly over the
+    batches (a batch's hit_start values count from its slice).  Outputs are
+    compared as per-query id lists."""
+    nq = offsets.numel() - 1
+    nb = (nq + batch - 1) // batch
+    cap = ids.numel() // max(nb, 1)
+    ns = len(streams)
+    keys = list(space or HIBF_SPACE)
+    state = {"shape": None}
+
+    def run():
+        shape = tuple(native.tune_get(k) for k in keys)
+        if shape != state["shape"]:  # the per-block workspace depends on the shape
+            for ag in agents:
+                hibf.membership_prepare(ag, batch)
+            state["shape"] = shape
+        main = torch.cuda.current_stream()
+        start = torch.cuda.Event()
+        start.record(main)
+        for sx in streams[1:]:
+            sx.wait_event(start)
+        for b in range(nb):
+            q0, q1 = b * batch, min(nq, (b + 1) * batch)
+            hibf.membership_launch(agents[b % len(agents)], values, offsets[q0:q1 + 1], thresholds[q0:q1],
+                                   ids[b * cap:(b + 1) * cap], hit_start[q0:q1], hit_count[q0:q1],
+                                   stream=streams[b % ns])
+        for sx in streams[1:]:
+            main.wait_stream(sx)
+
+    def output():
+        # A batch's hit_start values count from its own slice of ids.
+        slice0 = torch.arange(nq, device=hit_start.device, dtype=torch.int64) // batch * cap
+        start = hit_start.view(torch.int64) + slice0
+        count = hit_count.view(torch.int64)
+        total = int(count.sum())
+        first = torch.cumsum(count, 0) - count  # each query's first position in the packed lists
+        pos = torch.repeat_interleave(start - first, count) + torch.arange(total, device=count.device)
+        return count.clone(), ids.view(torch.int64)[pos].clone()
+
+    def same(a, b):
+        return torch.equal(a[0], b[0]) and torch.equal(a[1], b[1])
+
+    tab = hibf.tables()
+    workload = {"kind": f"hibf membership (launch form, {nb} batches of {batch} over {ns} streams)",
+                "ibfs": len(tab["geoms"]), "user_bins": int(tab["n_user_bins"]), "queries": nq,
+                "values": int(values.numel())}
     return _tune(space or HIBF_SPACE, run, output, same, reps, workload)
 
 

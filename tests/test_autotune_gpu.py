@@ -122,3 +122,48 @@ def test_tune_hibf_membership_launch_form(restore_knobs):
     for q in range(0, nq, 7):
         got = ids_n[int(hs_n[q]):int(hs_n[q]) + int(hc_n[q])].tolist()
         assert got == ref.membership_for(qs[q], int(thr[q])), q
+
+
+def test_tune_hibf_membership_batches_over_streams(restore_knobs):
+    """The overlapped-batches tuner: 3,000 queries in batches of 500 over three
+    streams with one agent each; every candidate's per-query lists equal the
+    default shape's, and the chosen shape's lists equal the oracle's."""
+    import torch
+    from hibf_amd import autotune, native
+    from hibf_amd.device import Agent, DeviceHibf, as_device_u64, csr, to_numpy
+    from hibf_helpers import oracle_of
+    _say("autotune hibf membership, batches over three streams, 700 user bins")
+    rng = np.random.default_rng(23)
+    ubs = [rng.integers(0, 2**63, size=int(n), dtype=np.int64).astype(np.uint64)
+           for n in rng.integers(20, 400, size=700)]
+    dev = DeviceHibf.build(ubs, tmax=64)
+    ref = oracle_of(dev)
+    qs = []
+    for _ in range(3000):
+        L = int(rng.integers(0, 300))
+        src = ubs[int(rng.integers(0, 700))]
+        k = int(0.8 * L)
+        qs.append(np.concatenate([src[rng.integers(0, src.size, k)],
+                                  rng.integers(0, 2**63, size=L - k, dtype=np.int64).astype(np.uint64)]))
+    v_h, o_h = csr(qs)
+    thr = np.array([max(1, int(0.7 * q.size)) for q in qs], dtype=np.uint16)
+    nq, batch = len(qs), 500
+    t = torch.from_numpy(thr.view(np.int16)).cuda().view(torch.uint16)
+    ids = torch.empty(6 * (8 * batch + 64 * 700), dtype=torch.uint64, device="cuda")
+    hs = torch.empty(nq, dtype=torch.uint64, device="cuda")
+    hc = torch.empty(nq, dtype=torch.uint64, device="cuda")
+    agents = [Agent() for _ in range(3)]
+    streams = [torch.cuda.current_stream()] + [torch.cuda.Stream() for _ in range(2)]
+    rec = autotune.tune_hibf_membership_batches(dev, agents, streams, as_device_u64(v_h), as_device_u64(o_h), t,
+                                                ids, hs, hc, batch, reps=1)
+    assert len(rec["candidates"]) == len(autotune._candidates(autotune.HIBF_SPACE))
+    assert {k: native.tune_get(k) for k in autotune.HIBF_SPACE} == rec["best"]
+    torch.cuda.synchronize()
+    for ag in agents:
+        assert not ag.status()[1]
+    ids_n, hs_n, hc_n = to_numpy(ids), to_numpy(hs), to_numpy(hc)
+    cap = ids.numel() // 6  # batch b's hit_start values count from ids[b * cap]
+    for q in range(0, nq, 7):
+        at = q // batch * cap + int(hs_n[q])
+        got = ids_n[at:at + int(hc_n[q])].tolist()
+        assert got == ref.membership_for(qs[q], int(thr[q])), q
