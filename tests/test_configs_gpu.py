@@ -129,3 +129,83 @@ def test_graph_batches_equal_plain_per_query_launches():
         # eager work between replays
         dev.membership_for(v_h, o_h[:5], thr_h[:4])
         torch.cuda.synchronize()
+
+
+def test_graph_batches_over_three_streams_equal_one_stream():
+    """The batched form as bench.py runs it: graph batches alternating over
+    three streams, one agent per stream, so consecutive batches overlap on the
+    device.  Three rounds of replays (with eager work between them) give the
+    per-query lists of the same batches replayed on one stream, and the
+    oracle's on a sample."""
+    import torch
+    from hibf_amd import workloads as W
+    from hibf_amd.device import Agent, DeviceHibf, as_device_u64, to_numpy
+    _say("graph batches over three streams vs one stream, 2,000 user bins")
+    rng = np.random.default_rng(2001)
+    sizes = np.clip(np.round(rng.lognormal(np.log(2000), 1.0, size=2000)), 100, 50_000).astype(np.uint64)
+    dev = DeviceHibf.build_synthetic(sizes, 5, tmax=64)
+    ref = oracle_of(dev)
+    nq = 6000
+    lengths = W.query_lengths(13, nq)
+    v_h, o_h, _ = W.hibf_queries(14, sizes, 5, lengths, 0.6)
+    thr_h = np.minimum((0.7 * np.diff(o_h.astype(np.int64))).astype(np.int64), 65535).astype(np.uint16)
+    dv, do = as_device_u64(v_h), as_device_u64(o_h)
+    dt = torch.from_numpy(thr_h.view(np.int16)).cuda().view(torch.uint16)
+    G, NS = 500, 3
+    nb = (nq + G - 1) // G
+    cap = 64 * G
+
+    def capture(agents):
+        ids_b = torch.empty(nb * cap, dtype=torch.uint64, device="cuda")
+        hs = torch.empty(nq, dtype=torch.uint64, device="cuda")
+        hc = torch.empty(nq, dtype=torch.uint64, device="cuda")
+        for ag in agents:
+            dev.membership_prepare(ag, G)
+        graphs = []
+        for b in range(nb):
+            q0, q1 = b * G, min(nq, (b + 1) * G)
+            a = (agents[b % len(agents)], dv, do[q0:q1 + 1], dt[q0:q1], ids_b[b * cap:(b + 1) * cap], hs[q0:q1],
+                 hc[q0:q1])
+            dev.membership_launch(*a)
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g):
+                dev.membership_launch(*a)
+            graphs.append(g)
+        torch.cuda.synchronize()
+        return graphs, ids_b, hs, hc
+
+    def lists(ids_b, hs, hc):
+        ids_n, hs_n, hc_n = to_numpy(ids_b), to_numpy(hs), to_numpy(hc)
+        return [ids_n[(q // G) * cap + int(hs_n[q]):(q // G) * cap + int(hs_n[q]) + int(hc_n[q])].tolist()
+                for q in range(nq)]
+
+    one_agent = Agent()
+    g1, ids1, hs1, hc1 = capture([one_agent])
+    for g in g1:
+        g.replay()
+    torch.cuda.synchronize()
+    serial = lists(ids1, hs1, hc1)
+    for q in range(0, nq, 11):
+        assert serial[q] == ref.membership_for(v_h[int(o_h[q]):int(o_h[q + 1])], int(thr_h[q])), q
+
+    agents = [Agent() for _ in range(NS)]
+    streams = [torch.cuda.current_stream()] + [torch.cuda.Stream() for _ in range(NS - 1)]
+    g3, ids3, hs3, hc3 = capture(agents)
+    for rnd in range(3):
+        ids3.fill_(0)
+        hc3.fill_(0)
+        start = torch.cuda.Event()
+        start.record(streams[0])
+        for sx in streams[1:]:
+            sx.wait_event(start)
+        for b, g in enumerate(g3):
+            with torch.cuda.stream(streams[b % NS]):
+                g.replay()
+        for sx in streams[1:]:
+            streams[0].wait_stream(sx)
+        torch.cuda.synchronize()
+        for ag in agents:
+            assert not ag.status()[1]
+        assert lists(ids3, hs3, hc3) == serial, rnd
+        dev.membership_for(v_h, o_h[:5], thr_h[:4])  # eager work between rounds
+        torch.cuda.synchronize()
