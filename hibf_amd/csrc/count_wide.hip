@@ -92,9 +92,12 @@ __device__ void store_tile(uint32_t const * lds,
 
 // HF > 0 fixes the number of hash functions (== g.hash_funs) at compile time,
 // so only HF row starts per value occupy registers; HF == 0 reads it at run time.
-// CB is the LDS counter width in bits (8 or 16).
+// CB is the LDS counter width in bits (8 or 16).  Instances that fit 64 VGPRs
+// are held to them (8 waves per SIMD); 8-bit counters at U = 4 need more (at
+// 64 they spilled 20 bytes per lane), so they get the registers instead.
 template <int U, int HF, int CB>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((HF > 0 && U <= 4) ? 8 : 1, 8))) count_wide_kernel(IbfView g,
+__global__ void __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu((HF > 0 && (U < 4 || (U == 4 && CB == 16))) ? 8 : 1, 8))) count_wide_kernel(IbfView g,
                                                          uint64_t const * __restrict__ values,
                                                          uint64_t const * __restrict__ offsets,
                                                          uint64_t nq,
