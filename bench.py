@@ -431,7 +431,8 @@ def flat_measure(args, leg: bool = False):
             "bins": bins, "bin_size": bin_size, "hash_functions": h, "queries": nq, "query_length": L,
             "counter": "uint16",
             "tuning": {k: native.tune_get(k) for k in ("count.wide_kernel", "count.wide_unroll",
-                                                        "count.wide_waves", "count.counter_bits")},
+                                                        "count.wide_waves", "count.counter_bits",
+                                                        "count.wide_nt_loads")},
             "autotune": tuned,
         },
         "roofline": {

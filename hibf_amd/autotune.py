@@ -16,6 +16,7 @@ Search spaces:
     count.counter_bits  LDS counter width, 16-bit pairs or 8-bit quads
     count.wide_unroll   values in flight per wave (1, 2, 4, 8)
     count.wide_waves    waves per query block (1, 2, 4)
+    count.wide_nt_loads non-temporal row loads (0, 1)
   HIBF membership descent (one launch at a time, tune_hibf_membership_launch, or
   batches overlapped over several streams, tune_hibf_membership_batches):
     hibf.wave_kernel    one query per wave (1) or per block (0)
@@ -43,6 +44,7 @@ FLAT_SPACE = {
     "count.counter_bits": (16, 8),
     "count.wide_unroll": (1, 2, 4, 8),
     "count.wide_waves": (1, 2, 4),
+    "count.wide_nt_loads": (0, 1),
 }
 # A candidate replaces the default shape only when it is faster by more than
 # this fraction: smaller differences are within the run-to-run noise of a

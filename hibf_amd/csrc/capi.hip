@@ -296,6 +296,8 @@ int hibf_gpu_tune_set(char const * key, int64_t value)
         g_count_tuning.nt_loads = static_cast<uint32_t>(value);
     else if (k == "count.nt_stores" && (value == 0 || value == 1))
         g_count_tuning.nt_stores = static_cast<uint32_t>(value);
+    else if (k == "count.wide_nt_loads" && (value == 0 || value == 1))
+        g_count_tuning.wide_nt_loads = static_cast<uint32_t>(value);
     else if (k == "contains.unroll" && (value == 1 || value == 2 || value == 4 || value == 8))
         g_count_tuning.contains_unroll = static_cast<uint32_t>(value);
     else if (k == "hibf.block_waves" && (value == 1 || value == 2 || value == 4 || value == 8))
@@ -334,6 +336,8 @@ int hibf_gpu_tune_get(char const * key, int64_t * value)
         *value = g_count_tuning.nt_loads;
     else if (k == "count.nt_stores")
         *value = g_count_tuning.nt_stores;
+    else if (k == "count.wide_nt_loads")
+        *value = g_count_tuning.wide_nt_loads;
     else if (k == "contains.unroll")
         *value = g_count_tuning.contains_unroll;
     else if (k == "hibf.block_waves")

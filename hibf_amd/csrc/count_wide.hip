@@ -42,6 +42,20 @@ __device__ __forceinline__ void put_counter(void * out, uint64_t i, uint32_t cb,
 
 // Writes nb counters of a tile (CB-bit fields packed in lds) to out[first ..).
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));
+
+// A lane's two words of a row (16 bytes); NT: non-temporal (streaming) load.
+template <bool NT>
+__device__ __forceinline__ ulonglong2 load_pair(uint64_t const * p)
+{
+    if constexpr (NT)
+    {
+        u64x2 const v = __builtin_nontemporal_load(reinterpret_cast<u64x2 const *>(p));
+        return make_ulonglong2(v.x, v.y);
+    }
+    else
+        return *reinterpret_cast<ulonglong2 const *>(p);
+}
 
 template <int CB>
 __device__ void store_tile(uint32_t const * lds,
@@ -95,7 +109,8 @@ __device__ void store_tile(uint32_t const * lds,
 // CB is the LDS counter width in bits (8 or 16).  Instances that fit 64 VGPRs
 // are held to them (8 waves per SIMD); 8-bit counters at U = 4 need more (at
 // 64 they spilled 20 bytes per lane), so they get the registers instead.
-template <int U, int HF, int CB>
+// NT: non-temporal row loads.
+template <int U, int HF, int CB, bool NT = false>
 __global__ void __launch_bounds__(256)
 __attribute__((amdgpu_waves_per_eu((HF > 0 && (U < 4 || (U == 4 && CB == 16))) ? 8 : 1, 8))) count_wide_kernel(IbfView g,
                                                          uint64_t const * __restrict__ values,
@@ -164,13 +179,13 @@ __attribute__((amdgpu_waves_per_eu((HF > 0 && (U < 4 || (U == 4 && CB == 16))) ?
                                 uint64_t const r0 = readlane64(rows[0], static_cast<int>(idx)) + wl;
                                 if (mine)
                                 {
-                                    ulonglong2 a = *reinterpret_cast<ulonglong2 const *>(words + r0);
+                                    ulonglong2 a = load_pair<NT>(words + r0);
 #pragma unroll
                                     for (int k = 1; k < KMAX; ++k)
                                         if (HF > 0 || static_cast<uint32_t>(k) < hf)
                                         {
                                             uint64_t const rk = readlane64(rows[k], static_cast<int>(idx)) + wl;
-                                            ulonglong2 const b = *reinterpret_cast<ulonglong2 const *>(words + rk);
+                                            ulonglong2 const b = load_pair<NT>(words + rk);
                                             a.x &= b.x;
                                             a.y &= b.y;
                                         }
@@ -237,6 +252,17 @@ hipError_t launch_count_wide(IbfView const & g,
     unsigned const grid = static_cast<unsigned>(nq < max_blocks ? nq : max_blocks);
     dim3 const blk(64u * g_count_tuning.wide_waves);
     using kernel_t = void (*)(IbfView, uint64_t const *, uint64_t const *, uint64_t, uint32_t, uint32_t, void *);
+    // Non-temporal row loads, for 2 or 3 hash functions: [counter bits 16, 8][unroll 1, 2, 4, 8][2, 3]
+    static kernel_t const table_nt[2][4][2] = {
+        {{count_wide_kernel<1, 2, 16, true>, count_wide_kernel<1, 3, 16, true>},
+         {count_wide_kernel<2, 2, 16, true>, count_wide_kernel<2, 3, 16, true>},
+         {count_wide_kernel<4, 2, 16, true>, count_wide_kernel<4, 3, 16, true>},
+         {count_wide_kernel<8, 2, 16, true>, count_wide_kernel<8, 3, 16, true>}},
+        {{count_wide_kernel<1, 2, 8, true>, count_wide_kernel<1, 3, 8, true>},
+         {count_wide_kernel<2, 2, 8, true>, count_wide_kernel<2, 3, 8, true>},
+         {count_wide_kernel<4, 2, 8, true>, count_wide_kernel<4, 3, 8, true>},
+         {count_wide_kernel<8, 2, 8, true>, count_wide_kernel<8, 3, 8, true>}},
+    };
     // [counter bits 16, 8][unroll 1, 2, 4, 8][hash functions: run time, 2, 3]
     static kernel_t const table[2][4][3] = {
         {{count_wide_kernel<1, 0, 16>, count_wide_kernel<1, 2, 16>, count_wide_kernel<1, 3, 16>},
@@ -251,8 +277,10 @@ hipError_t launch_count_wide(IbfView const & g,
     uint32_t const u = g_count_tuning.wide_unroll;
     int const ui = u == 1u ? 0 : u == 2u ? 1 : u == 8u ? 3 : 2;
     int const hi = g.hash_funs == 2u ? 1 : g.hash_funs == 3u ? 2 : 0;
-    hipLaunchKernelGGL(table[narrow ? 1 : 0][ui][hi], dim3(grid), blk, lds, stream, g, values, offsets, nq,
-                       counter_bytes, g_count_tuning.nt_stores, counts);
+    kernel_t const k = g_count_tuning.wide_nt_loads && hi != 0 ? table_nt[narrow ? 1 : 0][ui][hi - 1]
+                                                              : table[narrow ? 1 : 0][ui][hi];
+    hipLaunchKernelGGL(k, dim3(grid), blk, lds, stream, g, values, offsets, nq, counter_bytes,
+                       g_count_tuning.nt_stores, counts);
     return hipGetLastError();
 }
 

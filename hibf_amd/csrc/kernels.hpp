@@ -71,6 +71,7 @@ struct CountTuning
     uint32_t row_per_lane = 0;  // HIBF descent: one value and its whole row per lane when every row is <= 4 words
     uint32_t sliced_counts = 1; // HIBF descent over rows <= 4 words: bit-sliced per-lane counts
     uint32_t nt_stores = 1;     // count_wide_kernel and the HIBF counting zero fill: non-temporal (streaming) stores
+    uint32_t wide_nt_loads = 1; // count_wide_kernel with 2 or 3 hash functions: non-temporal row loads
     uint32_t hibf_block_waves = 4; // HIBF membership, query per block: waves per block (1, 2, 4)
     uint32_t hibf_unroll = 4;      // HIBF membership over rows <= 4 words, sliced counts: value groups in flight (2, 4)
     uint32_t split_order = 1;      // HIBF build: split user bins chunked in the reference's set order (1) or by value ranges (0)

@@ -58,7 +58,8 @@ const char * hibf_gpu_last_error(void);
  * per query block of the membership descent: 1, 2, 4 default, 8), "count.nt_stores"
  * (non-temporal stores of the IBF count kernel's counters, of the HIBF
  * count's zero fill and of bulk_contains' rows: 1, default; plain stores 0),
- * "contains.unroll" (values whose rows one wave of bulk_contains loads at
+ * "count.wide_nt_loads" (non-temporal row loads of the wide count kernel for
+ * IBFs with 2 or 3 hash functions: 1, default; plain loads 0), "contains.unroll" (values whose rows one wave of bulk_contains loads at
  * once: 1, 2, 4 default, 8).  Results never depend on these values.
  * Two keys change what the library does: "build.split_order" (1 default:
  * HIBF builds chunk split user bins in the reference's set order; 0: by value
