@@ -355,6 +355,24 @@ int hll_sketch_synthetic(uint64_t seed, uint64_t const * h_sizes, uint64_t n, ui
 int hll_estimates(uint8_t const * d_registers, uint64_t n, uint32_t bits, double * h_estimates, hipStream_t s);
 
 // k-mer hashes of reads on the device without host waits (kmer.hip).
+// Minimiser hashes (kmer.hip): shape/window validation as hibf_gpu_minimiser_hash
+// (a status), the offsets scan's temporary bytes for n reads, and the
+// asynchronous count / scan / thresholds / write pipeline.
+int minimiser_check(uint64_t shape, uint32_t window_size);
+hipError_t minimiser_scan_bytes(uint64_t n, size_t & bytes);
+hipError_t minimiser_hash_async(uint8_t const * d_seq,
+                                uint64_t const * d_seq_offsets,
+                                uint64_t n,
+                                uint64_t shape,
+                                uint32_t window_size,
+                                uint64_t seed,
+                                uint64_t * d_hash_offsets,
+                                uint64_t * d_hashes,
+                                void * scan_tmp,
+                                size_t scan_bytes,
+                                double fraction,
+                                uint16_t * d_thresholds,
+                                hipStream_t s);
 hipError_t kmer_hash_given_offsets_async(uint8_t const * d_seq,
                                          uint64_t const * d_seq_offsets,
                                          uint64_t n,

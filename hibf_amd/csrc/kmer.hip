@@ -445,6 +445,32 @@ int device_error(hipError_t e, char const * what)
     return hibf_amd::set_error(HIBF_GPU_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
 }
 
+// Checks a shape and window as hibf_gpu_minimiser_hash documents them and
+// fills the kernel parameters; returns a status (HIBF_GPU_OK when valid).
+int minimiser_params(uint64_t shape, uint32_t window_size, uint64_t seed, MiniParams & mp)
+{
+    if (shape == 0)
+        return hibf_amd::set_error(HIBF_GPU_ERR_INVALID_ARGUMENT, "the shape must have at least one position");
+    uint32_t const size = 64u - static_cast<uint32_t>(__builtin_clzll(shape));
+    if (size > 32u)
+        return hibf_amd::set_error(HIBF_GPU_ERR_UNSUPPORTED, "shapes of more than 32 positions are not supported");
+    if (window_size < size)
+        return hibf_amd::set_error(HIBF_GPU_ERR_INVALID_ARGUMENT,
+                                   "The size of the shape cannot be greater than the window size.");
+    uint64_t const w = static_cast<uint64_t>(window_size) - size + 1u;
+    if (w > HIBF_GPU_MAX_MINIMISER_WINDOW)
+        return hibf_amd::set_error(HIBF_GPU_ERR_UNSUPPORTED,
+                                   "windows of more than 4096 k-mers (window_size - shape size + 1) are not supported");
+    mp = MiniParams{};
+    mp.shape = shape;
+    mp.seed = seed;
+    mp.size = size;
+    mp.window = static_cast<uint32_t>(w);
+    mp.gapped = shape != (1ull << size) - 1ull ? 1u : 0u;
+    mp.words_cap = (kMiniSeg + mp.window + size - 1u + 255u) / 256u * 8u + 1u;
+    return HIBF_GPU_OK;
+}
+
 unsigned minimiser_blocks(uint64_t n)
 {
     return static_cast<unsigned>(std::max<uint64_t>(1u, std::min<uint64_t>(n, 1u << 16)));
@@ -454,6 +480,64 @@ unsigned minimiser_blocks(uint64_t n)
 
 namespace hibf_amd
 {
+
+int minimiser_check(uint64_t shape, uint32_t window_size)
+{
+    MiniParams mp;
+    return minimiser_params(shape, window_size, 0u, mp);
+}
+
+hipError_t minimiser_scan_bytes(uint64_t n, size_t & bytes)
+{
+    bytes = 0;
+    return n ? inclusive_sum_u64(nullptr, bytes, nullptr, nullptr, n, nullptr) : hipSuccess;
+}
+
+// Minimisers of n reads, asynchronous on s: the count pass, the offsets scan in
+// scan_tmp (minimiser_scan_bytes), the thresholds uint16_t(fraction * count)
+// when d_thresholds is given, and the write pass.  d_hashes must hold every
+// minimiser (at most one per base).  The shape and window must have passed
+// minimiser_check.
+hipError_t minimiser_hash_async(uint8_t const * d_seq,
+                                uint64_t const * d_seq_offsets,
+                                uint64_t n,
+                                uint64_t shape,
+                                uint32_t window_size,
+                                uint64_t seed,
+                                uint64_t * d_hash_offsets,
+                                uint64_t * d_hashes,
+                                void * scan_tmp,
+                                size_t scan_bytes,
+                                double fraction,
+                                uint16_t * d_thresholds,
+                                hipStream_t s)
+{
+    MiniParams mp;
+    if (minimiser_params(shape, window_size, seed, mp) != HIBF_GPU_OK)
+        return hipErrorInvalidValue;
+    size_t const lds = minimiser_lds_bytes(mp);
+    hipLaunchKernelGGL(zero_first_kernel, dim3(1), dim3(64), 0, s, d_hash_offsets);
+    if (n == 0)
+        return hipGetLastError();
+    hipLaunchKernelGGL(minimiser_kernel<false>, dim3(minimiser_blocks(n)), dim3(kMiniThreads), lds, s, d_seq,
+                       d_seq_offsets, n, mp, d_hash_offsets + 1, nullptr, nullptr);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess)
+        e = inclusive_sum_u64(scan_tmp, scan_bytes, d_hash_offsets + 1, d_hash_offsets + 1, n, s);
+    if (e == hipSuccess && d_thresholds)
+    {
+        hipLaunchKernelGGL(kmer_threshold_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, d_hash_offsets, n,
+                           fraction, d_thresholds);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess)
+    {
+        hipLaunchKernelGGL(minimiser_kernel<true>, dim3(minimiser_blocks(n)), dim3(kMiniThreads), lds, s, d_seq,
+                           d_seq_offsets, n, mp, nullptr, d_hash_offsets, d_hashes);
+        e = hipGetLastError();
+    }
+    return e;
+}
 
 // The hashes alone, with hash offsets the caller computed (n + 1 values,
 // hash_offsets[i + 1] - hash_offsets[i] = max(0, length of read i - k + 1)).
@@ -539,25 +623,9 @@ int hibf_gpu_minimiser_hash(uint8_t const * d_seq,
 {
     if (!d_seq_offsets || !d_hash_offsets || !h_total || (!d_seq && n_seqs) || (!d_hashes && hashes_cap))
         return hibf_amd::set_error(HIBF_GPU_ERR_INVALID_ARGUMENT, "null argument");
-    if (shape == 0)
-        return hibf_amd::set_error(HIBF_GPU_ERR_INVALID_ARGUMENT, "the shape must have at least one position");
-    uint32_t const size = 64u - static_cast<uint32_t>(__builtin_clzll(shape));
-    if (size > 32u)
-        return hibf_amd::set_error(HIBF_GPU_ERR_UNSUPPORTED, "shapes of more than 32 positions are not supported");
-    if (window_size < size)
-        return hibf_amd::set_error(HIBF_GPU_ERR_INVALID_ARGUMENT,
-                                   "The size of the shape cannot be greater than the window size.");
-    uint64_t const w = static_cast<uint64_t>(window_size) - size + 1u;
-    if (w > HIBF_GPU_MAX_MINIMISER_WINDOW)
-        return hibf_amd::set_error(HIBF_GPU_ERR_UNSUPPORTED,
-                                   "windows of more than 4096 k-mers (window_size - shape size + 1) are not supported");
-    MiniParams mp{};
-    mp.shape = shape;
-    mp.seed = seed;
-    mp.size = size;
-    mp.window = static_cast<uint32_t>(w);
-    mp.gapped = shape != ((size == 64u ? 0ull : (1ull << size)) - 1ull) ? 1u : 0u;
-    mp.words_cap = (kMiniSeg + mp.window + size - 1u + 255u) / 256u * 8u + 1u;
+    MiniParams mp;
+    if (int const rc = minimiser_params(shape, window_size, seed, mp))
+        return rc;
     size_t const lds = minimiser_lds_bytes(mp);
     hipStream_t const s = static_cast<hipStream_t>(stream);
     *h_total = 0;

@@ -202,8 +202,7 @@ void hibf_gpu_server_s::run()
                     reqs.push_back({r.ticket, r.submitted_s, r.bytes});
                 }
                 admitted.erase(admitted.begin(), admitted.begin() + static_cast<int64_t>(nb));
-                if (int const rc = slot_launch(s, hibf, nb, nv, reinterpret_cast<char const *>(s.h_vals), 8u, R, false,
-                                               0u, 0.0))
+                if (int const rc = slot_launch(s, hibf, nb, nv, reinterpret_cast<char const *>(s.h_vals), 8u, R, false))
                 {
                     fail(rc);
                     return;

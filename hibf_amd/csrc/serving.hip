@@ -31,15 +31,14 @@ namespace
 {
 
 // The loop.  Requests are elements of `eb` bytes (8: k-mer hashes with host thresholds; 1: read
-// bases, hashed with k on the device, thresholds from `fraction`).
+// bases, hashed on the device as `rh` says (k-mers or minimisers), thresholds from its fraction).
 int serve_impl(hibf_gpu_hibf hibf,
                hibf_gpu_serve_params const * params,
                char const * h_in,
                uint64_t const eb,
                uint64_t const * h_offsets,
                uint16_t const * h_thresholds,
-               uint32_t const k,
-               double const fraction,
+               ReadHashing const & rh,
                double const * h_arrival_s,
                uint64_t nq,
                double * h_latency_s,
@@ -51,8 +50,13 @@ int serve_impl(hibf_gpu_hibf hibf,
     if (!hibf || !params || !h_in || !h_offsets || (!reads && !h_thresholds) || !h_arrival_s || !h_latency_s || !h_ids
         || !h_counts)
         return serve_reject(HIBF_GPU_ERR_INVALID_ARGUMENT, "null argument");
-    if (reads && (k == 0 || k > 32 || !(fraction >= 0.0)))
-        return serve_reject(HIBF_GPU_ERR_INVALID_ARGUMENT, "k must be in [1, 32] and fraction >= 0");
+    if (reads && !(rh.fraction >= 0.0))
+        return serve_reject(HIBF_GPU_ERR_INVALID_ARGUMENT, "fraction must be >= 0");
+    if (reads && !rh.mini && (rh.k == 0 || rh.k > 32))
+        return serve_reject(HIBF_GPU_ERR_INVALID_ARGUMENT, "k must be in [1, 32]");
+    if (reads && rh.mini)
+        if (int const rc = hibf_amd::minimiser_check(rh.shape, rh.window_size))
+            return rc;
     hibf_gpu_serve_params const p = *params;
     if (p.streams == 0 || p.max_batch == 0 || p.results_per_query == 0 || p.max_batch_values == 0)
         return serve_reject(HIBF_GPU_ERR_INVALID_ARGUMENT,
@@ -118,10 +122,10 @@ int serve_impl(hibf_gpu_hibf hibf,
             copy_host(s.h_vals, src, nv * eb, copy_threads);
             src = reinterpret_cast<char const *>(s.h_vals);
         }
-        return slot_launch(s, hibf, nb, nv, src, eb, R, reads, k, fraction);
+        return slot_launch(s, hibf, nb, nv, src, eb, R, reads, rh);
     };
     for (Slot & s : slots)
-        if ((rc = slot_alloc(s, hibf, B, V, R, eb, reads, !pinned)) != HIBF_GPU_OK)
+        if ((rc = slot_alloc(s, hibf, B, V, R, eb, reads, !pinned, reads && rh.mini)) != HIBF_GPU_OK)
         {
             cleanup();
             return rc;
@@ -339,8 +343,8 @@ extern "C" int hibf_gpu_hibf_serve(hibf_gpu_hibf hibf,
                                    uint32_t * h_counts,
                                    hibf_gpu_serve_stats * stats)
 {
-    return serve_impl(hibf, params, reinterpret_cast<char const *>(h_values), 8u, h_offsets, h_thresholds, 0u, 0.0,
-                      h_arrival_s, nq, h_latency_s, h_ids, h_counts, stats);
+    return serve_impl(hibf, params, reinterpret_cast<char const *>(h_values), 8u, h_offsets, h_thresholds,
+                      ReadHashing{}, h_arrival_s, nq, h_latency_s, h_ids, h_counts, stats);
 }
 
 extern "C" int hibf_gpu_hibf_serve_reads(hibf_gpu_hibf hibf,
@@ -356,6 +360,34 @@ extern "C" int hibf_gpu_hibf_serve_reads(hibf_gpu_hibf hibf,
                                          uint32_t * h_counts,
                                          hibf_gpu_serve_stats * stats)
 {
-    return serve_impl(hibf, params, reinterpret_cast<char const *>(h_bases), 1u, h_read_offsets, nullptr, k, fraction,
+    ReadHashing rh;
+    rh.k = k;
+    rh.fraction = fraction;
+    return serve_impl(hibf, params, reinterpret_cast<char const *>(h_bases), 1u, h_read_offsets, nullptr, rh,
+                      h_arrival_s, nq, h_latency_s, h_ids, h_counts, stats);
+}
+
+extern "C" int hibf_gpu_hibf_serve_reads_minimisers(hibf_gpu_hibf hibf,
+                                                    hibf_gpu_serve_params const * params,
+                                                    uint8_t const * h_bases,
+                                                    uint64_t const * h_read_offsets,
+                                                    uint64_t shape,
+                                                    uint32_t window_size,
+                                                    uint64_t seed,
+                                                    double fraction,
+                                                    double const * h_arrival_s,
+                                                    uint64_t nq,
+                                                    double * h_latency_s,
+                                                    uint64_t * h_ids,
+                                                    uint32_t * h_counts,
+                                                    hibf_gpu_serve_stats * stats)
+{
+    ReadHashing rh;
+    rh.mini = true;
+    rh.shape = shape;
+    rh.window_size = window_size;
+    rh.seed = seed;
+    rh.fraction = fraction;
+    return serve_impl(hibf, params, reinterpret_cast<char const *>(h_bases), 1u, h_read_offsets, nullptr, rh,
                       h_arrival_s, nq, h_latency_s, h_ids, h_counts, stats);
 }

@@ -69,9 +69,11 @@ struct Slot
     uint64_t * d_ids{};
     uint64_t * d_hs{};
     uint64_t * d_hc{};
-    // reads in: the batch's k-mer hashes and their offsets
+    // reads in: the batch's k-mer or minimiser hashes and their offsets
     uint64_t * d_hashes{};
     uint64_t * d_hash_offs{};
+    void * d_scan_tmp{}; // minimisers: the offsets scan's temporary storage
+    size_t scan_bytes{};
     uint64_t q0{}, q1{};
     uint64_t bytes{};
     bool busy{false};
@@ -127,7 +129,7 @@ inline void slot_free(Slot & s)
         (void)hipStreamSynchronize(s.stream);
     for (void * d : {static_cast<void *>(s.d_vals), static_cast<void *>(s.d_offs), static_cast<void *>(s.d_thr),
                      static_cast<void *>(s.d_ids), static_cast<void *>(s.d_hs), static_cast<void *>(s.d_hc),
-                     static_cast<void *>(s.d_hashes), static_cast<void *>(s.d_hash_offs)})
+                     static_cast<void *>(s.d_hashes), static_cast<void *>(s.d_hash_offs), s.d_scan_tmp})
         if (d)
             (void)hipFree(d);
     for (void * hp : {static_cast<void *>(s.h_vals), static_cast<void *>(s.h_offs), static_cast<void *>(s.h_hash_offs),
@@ -144,10 +146,23 @@ inline void slot_free(Slot & s)
     s = Slot{};
 }
 
+// How reads are hashed: k-mers of length k (mini == false), or minimisers of
+// (shape, window_size, seed) as hibf_gpu_minimiser_hash computes them; the
+// threshold is uint16_t(fraction * number of hashes).
+struct ReadHashing
+{
+    uint32_t k{};
+    bool mini{};
+    uint64_t shape{};
+    uint32_t window_size{};
+    uint64_t seed{};
+    double fraction{};
+};
+
 // Allocates a slot for batches of at most B requests and V elements of eb bytes
 // with R result ids each; staging: pinned buffers for the elements (pageable input).
 inline int slot_alloc(Slot & s, hibf_gpu_hibf hibf, uint64_t B, uint64_t V, uint32_t R, uint64_t eb, bool reads,
-                      bool staging)
+                      bool staging, bool mini = false)
 {
     auto hip = [&](hipError_t e, char const * what) { return e == hipSuccess ? HIBF_GPU_OK : serve_fail(e, what); };
     int rc = HIBF_GPU_OK;
@@ -169,6 +184,10 @@ inline int slot_alloc(Slot & s, hibf_gpu_hibf hibf, uint64_t B, uint64_t V, uint
         if (!rc)
             rc = hip(hipHostMalloc(reinterpret_cast<void **>(&s.h_hash_offs), (B + 1) * 8u, hipHostMallocDefault),
                      "hash offsets");
+        if (!rc && mini)
+            rc = hip(minimiser_scan_bytes(B, s.scan_bytes), "minimiser scan size");
+        if (!rc && mini)
+            rc = hip(device_malloc(&s.d_scan_tmp, std::max<size_t>(s.scan_bytes, 8u)), "minimiser scan storage");
     }
     if (!rc) rc = hip(device_malloc(&s.d_offs, (B + 1) * 8u), "device offsets");
     if (!rc) rc = hip(device_malloc(&s.d_thr, B * 2u), "device thresholds");
@@ -190,7 +209,7 @@ inline int slot_alloc(Slot & s, hibf_gpu_hibf hibf, uint64_t B, uint64_t V, uint
 // memory) to the device, reads hashed there (thresholds from the lengths),
 // the membership launch, the results back and the completion event.
 inline int slot_launch(Slot & s, hibf_gpu_hibf hibf, uint64_t nb, uint64_t nv, char const * h_src, uint64_t eb,
-                       uint32_t R, bool reads, uint32_t k, double fraction)
+                       uint32_t R, bool reads, ReadHashing const & rh = ReadHashing{})
 {
     hipError_t e = hipSuccess;
     auto step = [&](hipError_t r)
@@ -203,8 +222,20 @@ inline int slot_launch(Slot & s, hibf_gpu_hibf hibf, uint64_t nb, uint64_t nv, c
     step(hipMemcpyAsync(s.d_offs, s.h_offs, (nb + 1) * 8u, hipMemcpyHostToDevice, s.stream));
     uint64_t const * d_v = s.d_vals;
     uint64_t const * d_o = s.d_offs;
-    if (reads)
+    if (reads && rh.mini)
     {
+        // Minimiser counts are known only once the reads are hashed: the
+        // count pass, the scan and the thresholds run on the device.
+        step(minimiser_hash_async(reinterpret_cast<uint8_t const *>(s.d_vals), s.d_offs, nb, rh.shape,
+                                  rh.window_size, rh.seed, s.d_hash_offs, s.d_hashes, s.d_scan_tmp,
+                                  std::max<size_t>(s.scan_bytes, 8u), rh.fraction, s.d_thr, s.stream));
+        d_v = s.d_hashes;
+        d_o = s.d_hash_offs;
+    }
+    else if (reads)
+    {
+        uint32_t const k = rh.k;
+        double const fraction = rh.fraction;
         // A read of length L has max(0, L - k + 1) k-mers and the threshold
         // uint16_t(fraction * k-mers) (test/tutorial/search.cpp:42): both from
         // the lengths the host already has, so the device only hashes.

@@ -625,10 +625,14 @@ class DeviceHibf:
 
     def serve_reads(self, reads, arrival_s: np.ndarray, k: int = 19, fraction: float = 0.7, window_us: float = 500.0,
                     max_batch: int = 65536, max_batch_bases: int = 1 << 26, budget_bytes: int = 4 << 30,
-                    streams: int = 4, results_per_query: int = 16, pinned: bool = True):
+                    streams: int = 4, results_per_query: int = 16, pinned: bool = True,
+                    minimiser: tuple | None = None):
         """The serving loop for reads (hibf_gpu_hibf_serve_reads): reads (str or
         bytes, or a (bases, offsets) pair) arrive at arrival_s; each is hashed
         on the device and answered with threshold uint16_t(fraction * #k-mers).
+        With minimiser = (shape, window_size) or (shape, window_size, seed) the
+        reads are hashed to minimisers instead
+        (hibf_gpu_hibf_serve_reads_minimisers) and k is not used.
         With pinned, the bases are placed in pinned host memory first, as a
         server's receive buffer would hold them.  Returns (latency_s,
         ids[nq, R], counts, stats dict)."""
@@ -650,8 +654,15 @@ class DeviceHibf:
         st = native.ServeStats()
         P = lambda x: ctypes.c_void_p(x.data_ptr()) if isinstance(x, torch.Tensor) \
             else x.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
-        check(lib().hibf_gpu_hibf_serve_reads(self.h, ctypes.byref(p), P(b), P(o), k, float(fraction), P(a), nq,
-                                              P(lat), P(ids), P(cnt), ctypes.byref(st)))
+        if minimiser is None:
+            check(lib().hibf_gpu_hibf_serve_reads(self.h, ctypes.byref(p), P(b), P(o), k, float(fraction), P(a), nq,
+                                                  P(lat), P(ids), P(cnt), ctypes.byref(st)))
+        else:
+            shape, window = minimiser[0], minimiser[1]
+            seed = minimiser[2] if len(minimiser) > 2 else MINIMISER_SEED
+            check(lib().hibf_gpu_hibf_serve_reads_minimisers(self.h, ctypes.byref(p), P(b), P(o), shape, window, seed,
+                                                             float(fraction), P(a), nq, P(lat), P(ids), P(cnt),
+                                                             ctypes.byref(st)))
         stats = {k_: getattr(st, k_) for k_, _ in native.ServeStats._fields_}
         return lat, ids, cnt, stats
 

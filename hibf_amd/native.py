@@ -156,6 +156,9 @@ _SIGNATURES = {
                                    ctypes.POINTER(ServeStats)]),
     "hibf_gpu_hibf_serve_reads": (_INT, [_V, ctypes.POINTER(ServeParams), _V, _V, ctypes.c_uint32, ctypes.c_double,
                                          _V, _U64, _V, _V, _V, ctypes.POINTER(ServeStats)]),
+    "hibf_gpu_hibf_serve_reads_minimisers": (_INT, [_V, ctypes.POINTER(ServeParams), _V, _V, _U64, _U32, _U64,
+                                                    ctypes.c_double, _V, _U64, _V, _V, _V,
+                                                    ctypes.POINTER(ServeStats)]),
     "hibf_gpu_server_create": (_INT, [_V, ctypes.POINTER(ServeParams), ctypes.POINTER(_V)]),
     "hibf_gpu_server_submit": (_INT, [_V, _V, _U64, ctypes.c_uint16, ctypes.POINTER(_U64)]),
     "hibf_gpu_server_poll": (_INT, [_V, _V, _V, _U64, ctypes.c_double, ctypes.POINTER(_U64)]),

@@ -494,6 +494,28 @@ int hibf_gpu_hibf_serve_reads(hibf_gpu_hibf hibf,
                               uint32_t * h_counts,
                               hibf_gpu_serve_stats * stats);
 
+/* The serving loop for reads hashed to minimisers: as hibf_gpu_hibf_serve_reads,
+ * with each read's minimiser hashes (shape, window_size, seed as
+ * hibf_gpu_minimiser_hash takes them) computed on the device per batch and the
+ * threshold uint16_t(fraction * number of minimisers) taken there; a read's
+ * budget counts its bases, hashes (at most one per base) and outputs.
+ * Shape and window errors as hibf_gpu_minimiser_hash.  No counterpart in the
+ * reference library. */
+int hibf_gpu_hibf_serve_reads_minimisers(hibf_gpu_hibf hibf,
+                                         const hibf_gpu_serve_params * params,
+                                         const uint8_t * h_bases,
+                                         const uint64_t * h_read_offsets,
+                                         uint64_t shape,
+                                         uint32_t window_size,
+                                         uint64_t seed,
+                                         double fraction,
+                                         const double * h_arrival_s,
+                                         uint64_t nq,
+                                         double * h_latency_s,
+                                         uint64_t * h_ids,
+                                         uint32_t * h_counts,
+                                         hibf_gpu_serve_stats * stats);
+
 /* Batched HIBF counting_agent_type<T>::bulk_count
  * (hierarchical_interleaved_bloom_filter.hpp:462-496, 549-566): d_counts gets
  * nq * number_of_user_bins counters of counter_bytes (1, 2, 4, 8); a user bin's

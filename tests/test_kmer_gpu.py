@@ -175,3 +175,41 @@ def test_search_reads_by_minimisers():
     for i, r in enumerate(reads):
         hashes = O.minimiser_hashes(r, shape, window)
         assert got[i] == ref.membership_for(hashes, O.kmer_threshold(hashes.size, 0.7)), i
+
+
+def test_serving_reads_by_minimisers():
+    """hibf_gpu_hibf_serve_reads_minimisers (bases copied, minimiser counts,
+    offsets, thresholds and hashes computed on the device inside the serving
+    loop) answers every read as the oracle's membership_for on its minimisers
+    with threshold uint16_t(0.7 * #minimisers), for Poisson arrivals and two
+    batching windows; empty, short and N-only reads included; a window shorter
+    than the shape is rejected."""
+    from hibf_amd.device import DeviceHibf
+    from hibf_amd.native import HibfGpuError
+    _say("serving reads by minimisers")
+    shape, window = 0b11011011101111111, 24   # gapped, size 17
+    rng = random.Random(21)
+    genomes = ["".join(rng.choice("ACGT") for _ in range(rng.randint(2000, 5000))) for _ in range(60)]
+    ubs = [np.unique(O.minimiser_hashes(g, shape, window)) for g in genomes]
+    dev = DeviceHibf.build(ubs, tmax=64)
+    ref = oracle_of(dev)
+    reads = []
+    for _ in range(2000):
+        g = genomes[rng.randrange(len(genomes))]
+        start = rng.randrange(0, len(g) - 300)
+        reads.append(g[start:start + rng.randint(40, 300)])
+    reads[3], reads[9], reads[50] = "", "ACGTACGT", "N" * 40
+    nrng = np.random.default_rng(5)
+    arrival = np.cumsum(nrng.exponential(1.0 / 200000.0, size=len(reads)))
+    for win_us in (50.0, 500.0):
+        lat, ids, cnt, st = dev.serve_reads(reads, arrival, fraction=0.7, window_us=win_us, max_batch=512,
+                                            max_batch_bases=1 << 17, results_per_query=16,
+                                            minimiser=(shape, window))
+        assert st["batches"] >= 2 and (lat > 0).all()
+        for i, r in enumerate(reads):
+            hashes = O.minimiser_hashes(r, shape, window)
+            want = ref.membership_for(hashes, O.kmer_threshold(hashes.size, 0.7))
+            assert int(cnt[i]) == len(want), i
+            assert ids[i, :min(len(want), 16)].tolist() == want[:16], i
+    with pytest.raises(HibfGpuError):
+        dev.serve_reads(reads[:10], arrival[:10], minimiser=(shape, 10))
