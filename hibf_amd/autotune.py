@@ -28,6 +28,8 @@ Search spaces:
     hibf.block_waves    query per block: waves per block (4, 2, 8)
     hibf.unroll         rows of <= 4 words, sliced counts: value groups whose
                         rows a wave loads at once (2, 4)
+    hibf.stream_kib     rows of <= 4 words: IBFs of at least this many KiB take
+                        non-temporal row loads (64 MiB; 0: none)
 """
 from __future__ import annotations
 
@@ -58,6 +60,7 @@ HIBF_SPACE = {
     "hibf.nt_loads": (0, 1),
     "hibf.block_waves": (4, 2, 8),
     "hibf.unroll": (4, 2),
+    "hibf.stream_kib": (1 << 16, 0),
 }
 
 

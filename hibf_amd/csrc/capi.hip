@@ -298,6 +298,8 @@ int hibf_gpu_tune_set(char const * key, int64_t value)
         g_count_tuning.nt_stores = static_cast<uint32_t>(value);
     else if (k == "count.wide_nt_loads" && (value == 0 || value == 1))
         g_count_tuning.wide_nt_loads = static_cast<uint32_t>(value);
+    else if (k == "hibf.stream_kib" && value >= 0 && value <= 0x7FFFFFFF)
+        g_count_tuning.hibf_stream_kib = static_cast<uint32_t>(value);
     else if (k == "contains.unroll" && (value == 1 || value == 2 || value == 4 || value == 8))
         g_count_tuning.contains_unroll = static_cast<uint32_t>(value);
     else if (k == "hibf.block_waves" && (value == 1 || value == 2 || value == 4 || value == 8))
@@ -338,6 +340,8 @@ int hibf_gpu_tune_get(char const * key, int64_t * value)
         *value = g_count_tuning.nt_stores;
     else if (k == "count.wide_nt_loads")
         *value = g_count_tuning.wide_nt_loads;
+    else if (k == "hibf.stream_kib")
+        *value = g_count_tuning.hibf_stream_kib;
     else if (k == "contains.unroll")
         *value = g_count_tuning.contains_unroll;
     else if (k == "hibf.block_waves")
@@ -1175,9 +1179,11 @@ namespace
 {
 HibfLaunch launch_with(hibf_gpu_hibf h, unsigned long long * counters)
 {
+    uint64_t const stream_bytes =
+        g_count_tuning.hibf_stream_kib ? static_cast<uint64_t>(g_count_tuning.hibf_stream_kib) << 10 : ~0ull;
     return HibfLaunch{h->d_desc,      h->d_arena,   h->d_tb_info, h->max_bins, h->depth, h->n_user_bins,
                       h->max_hits,    (h->max_bins + 63u) / 64u, h->uniform_hash_funs, g_count_tuning.nt_stores,
-                      0u,             h->gstate_words,           counters};
+                      0u,             h->gstate_words,           stream_bytes, counters};
 }
 
 HibfLaunch launch_of(hibf_gpu_hibf h, hibf_gpu_agent a)

@@ -116,7 +116,13 @@ __device__ __forceinline__ void descent_body(HibfLaunch const & h,
                 if constexpr (NARROW == 2)
                     count_into_lds_row_per_lane<NT, HF>(g, vals, L, cnt, wave, nwaves, lane);
                 else if constexpr (NARROW == 1)
-                    count_into_lds_dispatch_narrow<U, NT, HF>(d.lpv_log, g, vals, L, cnt, wave, nwaves, lane);
+                {
+                    // IBFs too large for the caches to keep rows between queries stream their rows.
+                    if (!NT && static_cast<uint64_t>(d.bin_size) * d.bin_words * 8u >= h.stream_bytes)
+                        count_into_lds_dispatch_narrow<U, true, HF>(d.lpv_log, g, vals, L, cnt, wave, nwaves, lane);
+                    else
+                        count_into_lds_dispatch_narrow<U, NT, HF>(d.lpv_log, g, vals, L, cnt, wave, nwaves, lane);
+                }
                 else
                     count_into_lds_dispatch<U, NT, HF, SLICE, GSTATE>(d.lpv_log, g, vals, L, cnt, wave, nwaves, lane);
                 ++visits;

@@ -74,6 +74,8 @@ struct CountTuning
     uint32_t wide_nt_loads = 1; // count_wide_kernel with 2 or 3 hash functions: non-temporal row loads
     uint32_t hibf_block_waves = 4; // HIBF membership, query per block: waves per block (1, 2, 4)
     uint32_t hibf_unroll = 4;      // HIBF membership over rows <= 4 words, sliced counts: value groups in flight (2, 4)
+    uint32_t hibf_stream_kib = 1u << 16; // HIBF descent over rows <= 4 words: IBFs of at least this many KiB
+                                         // (default 64 MiB) take non-temporal row loads (0: none)
     uint32_t split_order = 1;      // HIBF build: split user bins chunked in the reference's set order (1) or by value ranges (0)
 };
 extern CountTuning g_count_tuning;
@@ -202,6 +204,10 @@ struct HibfLaunch
     // (1 + depth) x max_bins words of device memory per block, after the
     // block's hit list in the scratch the launch is given.
     uint64_t gstate_words;
+    // IBFs of at least this many bytes are read with non-temporal (streaming)
+    // row loads by the narrow-row kernels: too large for the caches to keep
+    // their rows between queries.  ~0: never.
+    uint64_t stream_bytes;
     // Agent counters (device): [0] ids found, [1] next query (work stealing),
     // [2] IBF visits, [3] row bytes fetched (h * bin_words * 8 per value and
     // visit), [4] lookups (query values checked against one IBF), [5] set when

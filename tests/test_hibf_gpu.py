@@ -158,14 +158,16 @@ CASES = [  # (n_ub, size range, config)
 ]
 
 
-_DESCENT_VARIANTS = {  # (hibf.wave_kernel, hibf.row_per_lane, hibf.sliced_counts, hibf.block_waves, hibf.unroll)
-    "wave": (1, 0, 1, 4, 4),
-    "block-row-per-lane": (0, 1, 1, 4, 4),
-    "block-lanes-per-value-sliced": (0, 0, 1, 4, 4),
-    "block-lanes-per-value-sliced-unroll-2": (0, 0, 1, 4, 2),
-    "block-lanes-per-value-atomics": (0, 0, 0, 4, 4),
-    "block-2-waves": (0, 0, 1, 2, 4),
-    "block-8-waves": (0, 0, 1, 8, 4),
+_DESCENT_VARIANTS = {  # (hibf.wave_kernel, hibf.row_per_lane, hibf.sliced_counts, hibf.block_waves, hibf.unroll,
+    #  hibf.stream_kib)
+    "wave": (1, 0, 1, 4, 4, 1 << 16),
+    "block-row-per-lane": (0, 1, 1, 4, 4, 1 << 16),
+    "block-lanes-per-value-sliced": (0, 0, 1, 4, 4, 1 << 16),
+    "block-lanes-per-value-sliced-unroll-2": (0, 0, 1, 4, 2, 1 << 16),
+    "block-lanes-per-value-atomics": (0, 0, 0, 4, 4, 1 << 16),
+    "block-2-waves": (0, 0, 1, 2, 4, 1 << 16),
+    "block-8-waves": (0, 0, 1, 8, 4, 1 << 16),
+    "block-streamed-ibfs": (0, 0, 1, 4, 4, 1),  # every IBF of 1 KiB and more with non-temporal row loads
 }
 
 
@@ -175,9 +177,11 @@ def descent_kernel(request):
     with one value and its row per lane; query per block with lanes sharing a
     value's row, counting through bit-sliced registers or with one LDS atomic
     per set bit, with two or four value groups of rows in flight per wave;
-    and blocks of 2 and 8 waves.  Yields the variant's name."""
+    blocks of 2 and 8 waves; and every IBF read with non-temporal row loads
+    (hibf.stream_kib = 1).  Yields the variant's name."""
     from hibf_amd import native
-    keys = ("hibf.wave_kernel", "hibf.row_per_lane", "hibf.sliced_counts", "hibf.block_waves", "hibf.unroll")
+    keys = ("hibf.wave_kernel", "hibf.row_per_lane", "hibf.sliced_counts", "hibf.block_waves", "hibf.unroll",
+            "hibf.stream_kib")
     saved = {k: native.tune_get(k) for k in keys}
     for k, v in zip(keys, _DESCENT_VARIANTS[request.param]):
         native.tune_set(k, v)
