@@ -123,6 +123,10 @@ __device__ __forceinline__ void descent_body(HibfLaunch const & h,
                     else
                         count_into_lds_dispatch_narrow<U, NT, HF>(d.lpv_log, g, vals, L, cnt, wave, nwaves, lane);
                 }
+                // (Not in the 8-wave two-hash membership kernel: a second row-load form does not fit its 64 VGPRs.)
+                else if (!NT && !GSTATE && (COUNTING || HF != 2)
+                         && static_cast<uint64_t>(d.bin_size) * d.bin_words * 8u >= h.stream_bytes)
+                    count_into_lds_dispatch<U, true, HF, SLICE, GSTATE>(d.lpv_log, g, vals, L, cnt, wave, nwaves, lane);
                 else
                     count_into_lds_dispatch<U, NT, HF, SLICE, GSTATE>(d.lpv_log, g, vals, L, cnt, wave, nwaves, lane);
                 ++visits;

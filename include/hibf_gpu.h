@@ -54,9 +54,10 @@ const char * hibf_gpu_last_error(void);
  * (descent over rows of <= 4 words: bit-sliced per-lane counts 1, one LDS
  * atomic per set bit 0), "hibf.unroll" (descent over rows of <= 4 words with
  * sliced counts: value groups whose rows a wave loads at once, 2 or 4
- * default), "hibf.nt_loads" (0/1), "hibf.stream_kib" (descent over rows of
- * <= 4 words: IBFs of at least this many KiB take non-temporal row loads;
- * 65536 default, 0 none), "hibf.block_waves" (waves
+ * default), "hibf.nt_loads" (0/1), "hibf.stream_kib" (IBFs of at least this
+ * many KiB take non-temporal row loads in the count and in the membership
+ * descent, except two-hash membership over rows of more than 4 words and
+ * state in device memory; 65536 default, 0 none), "hibf.block_waves" (waves
  * per query block of the membership descent: 1, 2, 4 default, 8), "count.nt_stores"
  * (non-temporal stores of the IBF count kernel's counters, of the HIBF
  * count's zero fill and of bulk_contains' rows: 1, default; plain stores 0),
