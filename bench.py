@@ -49,6 +49,9 @@ def parse():
     ap.add_argument("--genomes", type=int, default=4096,
                     help="serve_reads: random genomes (user bins) of --genome-length bases")
     ap.add_argument("--genome-length", type=int, default=20000)
+    ap.add_argument("--serve-minimisers", action="store_true",
+                    help="serve_reads: hash genomes and reads to minimisers (ungapped 19, window 23, seqan3's seed) "
+                         "instead of 19-mers")
     ap.add_argument("--workload", choices=["hibf", "flat", "serve", "serve_reads"], default="hibf",
                     help="hibf (default): 32,768-user-bin HIBF membership (log-normal batched hipGraph form as the "
                          "value, fixed-length batch, plain per-query form, serving p50/p99, flat IBF leg); "
@@ -1119,10 +1122,12 @@ def run_serve_reads(args):
     """Serving loop for reads (the tutorial search online): user bins are the
     19-mer sets of random genomes; requests are 250-base reads sampled from
     them (1 % base errors), Poisson arrivals; bases go to the device, which
-    hashes them and uses threshold uint16_t(0.7 x #k-mers)."""
+    hashes them and uses threshold uint16_t(0.7 x #k-mers).  With
+    --serve-minimisers both the genomes and the reads are hashed to minimisers
+    (ungapped{19}, window_size{23}, seqan3's default seed) instead."""
     import torch
     from hibf_amd import native
-    from hibf_amd.device import DeviceHibf, kmer_hash, to_numpy
+    from hibf_amd.device import MINIMISER_SEED, DeviceHibf, kmer_hash, minimiser_hash, to_numpy
 
     for kv in args.tune:
         k_, v_ = kv.split("=")
@@ -1136,12 +1141,14 @@ def run_serve_reads(args):
     genome_chars = acgt[genome]
     t0 = time.perf_counter()
     offs = np.arange(n_g + 1, dtype=np.uint64) * np.uint64(g_len)
-    hashes, hoff = kmer_hash((genome_chars, offs), k)
+    mini = (2**k - 1, 23, MINIMISER_SEED) if args.serve_minimisers else None
+    hashes, hoff = minimiser_hash((genome_chars, offs), *mini) if mini else kmer_hash((genome_chars, offs), k)
     h, ho = to_numpy(hashes), to_numpy(hoff)
     ubs = [h[int(ho[i]):int(ho[i + 1])] for i in range(n_g)]
     hibf = DeviceHibf.build(ubs, tmax=64)
     tab = hibf.tables()
-    log(f"serve_reads: {n_g} genomes x {g_len} bases, {int(ho[-1])} k-mers, {len(tab['geoms'])} IBFs, "
+    log(f"serve_reads: {n_g} genomes x {g_len} bases, {int(ho[-1])} {'minimisers' if mini else 'k-mers'}, "
+        f"{len(tab['geoms'])} IBFs, "
         f"built in {time.perf_counter() - t0:.1f} s")
     rates = [float(r) for r in args.serve_rates.split(",")]
     windows = [float(w) for w in args.serve_windows.split(",")]
@@ -1168,7 +1175,7 @@ def run_serve_reads(args):
         for win in windows:
             lat, ids, cnt, st = hibf.serve_reads((data, r_off), arrival, k=k, fraction=0.7, window_us=win,
                                                  max_batch=65536, max_batch_bases=1 << 26, budget_bytes=4 << 30,
-                                                 streams=4, results_per_query=16)
+                                                 streams=4, results_per_query=16, minimiser=mini)
             row = {"rate_qps": rate, "window_us": win, "reads": nq,
                    "p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
                    "sustained_qps": nq / st["seconds"], "sustained_lookups_per_s": st["lookups"] / st["seconds"],
@@ -1187,7 +1194,7 @@ def run_serve_reads(args):
             for q in rng.choice(nq, size=min(args.check_queries, nq), replace=False):
                 q = int(q)
                 read = data[int(r_off[q]):int(r_off[q + 1])].tobytes().decode()
-                hashes_q = O.kmer_hashes(read, k)
+                hashes_q = O.minimiser_hashes(read, *mini) if mini else O.kmer_hashes(read, k)
                 want = ref.membership_for(hashes_q, O.kmer_threshold(hashes_q.size, 0.7))
                 assert int(cnt[q]) == len(want) and ids[q, :min(len(want), 16)].tolist() == want[:16], \
                     f"serving result differs at read {q}"
@@ -1195,7 +1202,9 @@ def run_serve_reads(args):
     best = max(rows, key=lambda r: r["sustained_qps"])
     out = {"metric": "reads/s (HIBF serving loop, reads in)", "value": best["sustained_qps"], "unit": "reads/s",
            "n_gpus": 1, "higher_is_better": True, "data": "synthetic", "dtype": "u8 bases / u64 hash",
-           "config": {"workload": f"serving reads of {read_len} bases (k = {k}, t = 0.7 x #k-mers) on an HIBF of "
+           "config": {"workload": f"serving reads of {read_len} bases "
+                                  + ("(minimisers of ungapped 19 in windows of 23, t = 0.7 x #minimisers)" if mini
+                                     else f"(k = {k}, t = 0.7 x #k-mers)") + " on an HIBF of "
                                   f"{n_g} random genomes x {g_len} bases, tmax 64, Poisson arrivals, 4 streams"},
            "sweep": rows}
     print(json.dumps(out), flush=True)
